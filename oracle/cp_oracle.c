@@ -1,0 +1,931 @@
+/*
+ * cp_oracle.c -- CPU FP64 restatement of the reference plasticity hot path.
+ * TEST INFRASTRUCTURE ONLY (see cp_oracle.h): called by tests/, by
+ * __graft_entry__.smoke() as the checker and by bench.py's cpu_baseline leg.
+ * The product library never links or loads it.
+ *
+ * Citations are reference file:line, relative to the reference's src/.
+ */
+#include "cp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+static const double OR_PI = 3.141592653589793;
+
+/* ====================================================================== */
+/* Material law -- Material.cu:3-23 (linearDiagram / powerDiagram); the    */
+/* Swift law is new: sigma = sigmaT (1 + eps_p/eps0)^n, eps0 = sigmaT/E,    */
+/* eps_p = eps - sigma/E, solved for sigma by a safeguarded Newton step.    */
+/* ====================================================================== */
+static double swift_sigma(double E, double sigmaT, double n, double eps)
+{
+    /* s = sigma/sigmaT solves g(s) = s - (1 + t - s)^n = 0, t = E eps/sigmaT >= 1.
+     * g is increasing and convex on [1, t]: Newton from s = 1 overshoots once
+     * and then decreases monotonically to the root. */
+    double t = E * eps / sigmaT;
+    double s = 1.0;
+    for (int it = 0; it < 60; ++it) {
+        double w = 1.0 + t - s;
+        double pw = pow(w, n);
+        double g = s - pw;
+        double dg = 1.0 + n * pw / w;
+        double sn = s - g / dg;
+        if (sn < 1.0) sn = 1.0;
+        if (sn > t) sn = t;
+        double d = fabs(sn - s);
+        s = sn;
+        if (d <= 1e-15 * s) break;
+    }
+    return sigmaT * s;
+}
+
+double or_material_f(const or_material* m, double eps)
+{
+    double epsT = m->sigmaT / m->E;
+    if (eps < epsT) return eps * m->E;
+    switch (m->law) {
+    case OR_LAW_LINEAR:
+        return m->sigmaT + (eps - epsT) * m->K_T;
+    case OR_LAW_POWER:
+        return m->sigmaT + m->A * pow(eps - epsT + pow(m->E / (m->A * m->m), 1. / (m->m - 1.)), m->m)
+               - m->A * pow(m->E / (m->A * m->m), m->m / (m->m - 1.));
+    default:
+        return swift_sigma(m->E, m->sigmaT, m->n, eps);
+    }
+}
+
+/* ====================================================================== */
+/* Mesh generators                                                         */
+/* ====================================================================== */
+
+/* genRectangle, order 1 -- Mesh.cu:17-109 */
+void or_rect_sizes(int N1, int N2, int* nn, int* ne, int* nb, int* blen)
+{
+    *nn = (N1 + 1) * (N2 + 1);
+    *ne = N1 * N2;
+    *nb = 4;
+    if (blen) { blen[0] = blen[2] = N2 + 1; blen[1] = blen[3] = N1 + 1; }
+}
+
+void or_gen_rect(double x1, double x2, double y1, double y2, int N1, int N2,
+                 double* x, double* y, int* conn, int* bord, int* blen)
+{
+    double h1 = (x2 - x1) / N1, h2 = (y2 - y1) / N2;
+    for (int i = 0; i <= N1; ++i)
+        for (int j = 0; j <= N2; ++j) {
+            int k = i * (N2 + 1) + j;
+            x[k] = x1 + h1 * i;
+            y[k] = y1 + h2 * j;
+        }
+    for (int i = 0; i < N1; ++i)
+        for (int j = 0; j < N2; ++j) {
+            int b = 4 * (i * N2 + j);
+            conn[b + 0] = (i + 1) * (N2 + 1) + j + 1;
+            conn[b + 1] = i * (N2 + 1) + j + 1;
+            conn[b + 2] = i * (N2 + 1) + j;
+            conn[b + 3] = (i + 1) * (N2 + 1) + j;
+        }
+    blen[0] = blen[2] = N2 + 1;
+    blen[1] = blen[3] = N1 + 1;
+    int* b0 = bord;
+    int* b1 = b0 + blen[0];
+    int* b2 = b1 + blen[1];
+    int* b3 = b2 + blen[2];
+    for (int i = 0; i <= N2; ++i) {
+        b0[N2 - i] = (N1 + 1) * (N2 + 1) - i - 1;
+        b2[N2 - i] = i;
+    }
+    for (int i = 0; i <= N1; ++i) {
+        b1[N1 - i] = (i + 1) * (N2 + 1) - 1;
+        b3[N1 - i] = (N1 - i) * (N2 + 1);
+    }
+}
+
+/* genRectWithHole radial segment count -- Mesh.cu:114-116 */
+int or_hole_radN(double x1, double x2, double y1, double y2, double r, int N1, int N2)
+{
+    int radN = (N1 + N2) / 4;
+    radN = (int)round((double)radN * (x2 - x1 + y2 - y1 - 4. * r) * 2. / (x2 - x1 + y2 - y1));
+    return radN;
+}
+
+void or_hole_sizes(double x1, double x2, double y1, double y2, double r, int N1, int N2,
+                   int* nn, int* ne, int* nb, int* blen)
+{
+    int borderN = (N1 + N2) * 2;
+    int radN = or_hole_radN(x1, x2, y1, y2, r, N1, N2);
+    *nn = borderN * (radN + 1);
+    *ne = borderN * radN;
+    *nb = 5;
+    if (blen) {
+        blen[0] = blen[2] = N1 + 1;
+        blen[1] = blen[3] = N2 + 1;
+        blen[4] = borderN + 1;
+    }
+}
+
+/* renumerateRing -- Mesh.cu:332-384 (odd-width middle column fixed). */
+void or_renumerate_ring(int nn, int ne, double* x, double* y, int* conn,
+                        int nb, int* bord, const int* blen, int borderN)
+{
+    int* nw = (int*)malloc(sizeof(int) * nn);
+    int radN = nn / borderN;
+    int newN = 0, jBack = borderN - 1, jFront = 0;
+    while (jBack > jFront) {
+        for (int i = 0; i < radN; ++i) nw[i * borderN + jFront] = newN++;
+        for (int i = 0; i < radN; ++i) nw[i * borderN + jBack] = newN++;
+        ++jFront;
+        --jBack;
+    }
+    if (jBack == jFront)
+        for (int i = 0; i < radN; ++i) nw[i * borderN + jFront] = newN++;
+    for (int k = 0; k < 4 * ne; ++k) conn[k] = nw[conn[k]];
+    int off = 0;
+    for (int b = 0; b < nb; ++b) {
+        for (int j = 0; j < blen[b]; ++j) bord[off + j] = nw[bord[off + j]];
+        off += blen[b];
+    }
+    double* tx = (double*)malloc(sizeof(double) * nn);
+    double* ty = (double*)malloc(sizeof(double) * nn);
+    for (int i = 0; i < nn; ++i) { tx[nw[i]] = x[i]; ty[nw[i]] = y[i]; }
+    memcpy(x, tx, sizeof(double) * nn);
+    memcpy(y, ty, sizeof(double) * nn);
+    free(tx); free(ty); free(nw);
+}
+
+/* genRectWithHole, order 1 -- Mesh.cu:112-245 */
+void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double holeRad, int N1, int N2,
+                           double* x, double* y, int* conn, int* bord, int* blen)
+{
+    int borderN = (N1 + N2) * 2;
+    int radN = or_hole_radN(x1, x2, y1, y2, holeRad, N1, N2);
+    int nodeCount = borderN * (radN + 1);
+    blen[0] = blen[2] = N1 + 1;
+    blen[1] = blen[3] = N2 + 1;
+    blen[4] = borderN + 1;
+    int* b0 = bord;
+    int* b1 = b0 + blen[0];
+    int* b2 = b1 + blen[1];
+    int* b3 = b2 + blen[2];
+    int* b4 = b3 + blen[3];
+
+    int nodeId = 0;
+    double hx = (x2 - x1) / N1, hy = (y2 - y1) / N2;
+    for (int i = 0; i < N1; ++i, ++nodeId) { x[nodeId] = x1 + i * hx; y[nodeId] = y1; b0[i] = nodeId; }
+    b0[blen[0] - 1] = nodeId;
+    for (int j = 0; j < N2; ++j, ++nodeId) { x[nodeId] = x2; y[nodeId] = y1 + j * hy; b1[j] = nodeId; }
+    b1[blen[1] - 1] = nodeId;
+    for (int i = 0; i < N1; ++i, ++nodeId) { x[nodeId] = x2 - i * hx; y[nodeId] = y2; b2[i] = nodeId; }
+    b2[blen[2] - 1] = nodeId;
+    for (int j = 0; j < N2; ++j, ++nodeId) { x[nodeId] = x1; y[nodeId] = y2 - j * hy; b3[j] = nodeId; }
+    b3[blen[3] - 1] = 0;
+
+    double phi = 2. * OR_PI / borderN;
+    nodeId = radN * borderN;
+    double cx = 0.5 * (x1 + x2), cy = 0.5 * (y1 + y2);
+    double start = phi * (N1 + N2 * 1.5);
+    for (int i = 0; i < borderN; ++i, ++nodeId) {
+        double a = start + (double)i * phi;
+        x[nodeId] = cx + cos(a) * holeRad;
+        y[nodeId] = cy + sin(a) * holeRad;
+        b4[borderN - i - 1] = nodeId;
+    }
+    b4[blen[4] - 1] = nodeCount - 1;
+
+    /* rays: node(i, j) = node(0, j) + ((node(radN, j) - node(0, j)) / radN) * i  (Mesh.cu:180-190) */
+    for (int j = 0; j < borderN; ++j) {
+        double rhx = (x[radN * borderN + j] - x[j]) / radN;
+        double rhy = (y[radN * borderN + j] - y[j]) / radN;
+        for (int i = 1; i < radN; ++i) {
+            x[i * borderN + j] = x[j] + rhx * (double)i;
+            y[i * borderN + j] = y[j] + rhy * (double)i;
+        }
+    }
+
+    for (int i = 0; i < radN; ++i)
+        for (int j = 0; j < borderN - 1; ++j) {
+            int b = 4 * (j * radN + i);
+            conn[b + 1] = (i + 1) * borderN + j;
+            conn[b + 2] = i * borderN + j;
+            conn[b + 3] = i * borderN + j + 1;
+            conn[b + 0] = (i + 1) * borderN + j + 1;
+        }
+    for (int i = 0; i < radN; ++i) {
+        int b = 4 * ((borderN - 1) * radN + i);
+        conn[b + 1] = (i + 1) * borderN + borderN - 1;
+        conn[b + 2] = i * borderN + borderN - 1;
+        conn[b + 3] = i * borderN;
+        conn[b + 0] = (i + 1) * borderN;
+    }
+    or_renumerate_ring(nodeCount, borderN * radN, x, y, conn, 5, bord, blen, borderN);
+}
+
+/* genRing, order 1 -- Mesh.cu:249-329 */
+void or_ring_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    *nn = (Nr + 1) * Nphi;
+    *ne = Nr * Nphi;
+    *nb = 2;
+    if (blen) { blen[0] = blen[1] = Nphi + 1; }
+}
+
+void or_gen_ring(double a, double b, int Nphi, int Nr,
+                 double* x, double* y, int* conn, int* bord, int* blen)
+{
+    blen[0] = blen[1] = Nphi + 1;
+    int* b0 = bord;
+    int* b1 = bord + blen[0];
+    double h_r = (b - a) / Nr, h_phi = 2. * OR_PI / Nphi;
+    for (int j = 0; j < Nphi; ++j) {
+        for (int i = 0; i <= Nr; ++i) {
+            double r = b - h_r * i;
+            double phi = h_phi * j;
+            x[i * Nphi + j] = r * cos(phi);
+            y[i * Nphi + j] = r * sin(phi);
+        }
+        b1[Nphi - j] = j;
+        b0[j] = Nr * Nphi + j;
+    }
+    b1[0] = 0;
+    b0[blen[1] - 1] = Nr * Nphi;
+    for (int i = 0; i < Nr; ++i)
+        for (int j = 0; j < Nphi - 1; ++j) {
+            int k = 4 * (j * Nr + i);
+            conn[k + 0] = (i + 1) * Nphi + j + 1;
+            conn[k + 3] = i * Nphi + j + 1;
+            conn[k + 2] = i * Nphi + j;
+            conn[k + 1] = (i + 1) * Nphi + j;
+        }
+    for (int i = 0; i < Nr; ++i) {
+        int k = 4 * ((Nphi - 1) * Nr + i);
+        conn[k + 0] = (i + 1) * Nphi;
+        conn[k + 3] = i * Nphi;
+        conn[k + 2] = i * Nphi + Nphi - 1;
+        conn[k + 1] = (i + 1) * Nphi + Nphi - 1;
+    }
+    or_renumerate_ring((Nr + 1) * Nphi, Nr * Nphi, x, y, conn, 2, bord, blen, Nphi);
+}
+
+/* Quarter plate with a quarter hole (new generator; ray construction of
+ * Mesh.cu:169-190 restricted to the first quadrant).  Node (i, j): j is the
+ * ray index (0 at y=0 .. Nt at x=0), i the radial index (0 on the outer
+ * edges .. Nr on the hole); id = j*(Nr+1) + i. */
+void or_quarter_plate_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    *nn = (Nt + 1) * (Nr + 1);
+    *ne = Nt * Nr;
+    *nb = 5;
+    if (blen) {
+        blen[0] = Nr + 1; blen[1] = Nt / 2 + 1; blen[2] = Nt - Nt / 2 + 1;
+        blen[3] = Nr + 1; blen[4] = Nt + 1;
+    }
+}
+
+void or_gen_quarter_plate(double L, double r, int Nt, int Nr,
+                          double* x, double* y, int* conn, int* bord, int* blen)
+{
+    int half = Nt / 2;
+    for (int j = 0; j <= Nt; ++j) {
+        double ox, oy;
+        if (j <= half) { ox = L; oy = L * (double)j / (double)half; }
+        else { ox = L - L * (double)(j - half) / (double)(Nt - half); oy = L; }
+        double hx, hy;
+        if (j == 0) { hx = r; hy = 0.; }
+        else if (j == Nt) { hx = 0.; hy = r; }
+        else {
+            double t = 0.5 * OR_PI * (double)j / (double)Nt;
+            hx = r * cos(t); hy = r * sin(t);
+        }
+        double rhx = (hx - ox) / Nr, rhy = (hy - oy) / Nr;
+        int base = j * (Nr + 1);
+        x[base] = ox; y[base] = oy;
+        for (int i = 1; i < Nr; ++i) {
+            x[base + i] = ox + rhx * (double)i;
+            y[base + i] = oy + rhy * (double)i;
+        }
+        x[base + Nr] = hx; y[base + Nr] = hy;
+    }
+    for (int j = 0; j < Nt; ++j)
+        for (int i = 0; i < Nr; ++i) {
+            int b = 4 * (j * Nr + i);
+            conn[b + 0] = (j + 1) * (Nr + 1) + i + 1;
+            conn[b + 1] = j * (Nr + 1) + i + 1;
+            conn[b + 2] = j * (Nr + 1) + i;
+            conn[b + 3] = (j + 1) * (Nr + 1) + i;
+        }
+    int dnn, dne, dnb;
+    or_quarter_plate_sizes(Nt, Nr, &dnn, &dne, &dnb, blen);
+    int* b0 = bord;
+    int* b1 = b0 + blen[0];
+    int* b2 = b1 + blen[1];
+    int* b3 = b2 + blen[2];
+    int* b4 = b3 + blen[3];
+    for (int k = 0; k <= Nr; ++k) b0[k] = Nr - k;                        /* (r,0) -> (L,0) */
+    for (int k = 0; k <= half; ++k) b1[k] = k * (Nr + 1);                /* (L,0) -> (L,L) */
+    for (int k = 0; k <= Nt - half; ++k) b2[k] = (half + k) * (Nr + 1);  /* (L,L) -> (0,L) */
+    for (int k = 0; k <= Nr; ++k) b3[k] = Nt * (Nr + 1) + k;             /* (0,L) -> (0,r) */
+    for (int k = 0; k <= Nt; ++k) b4[k] = (Nt - k) * (Nr + 1) + Nr;      /* (0,r) -> (r,0) */
+}
+
+/* Quarter ring, radially graded (new generator).  Node (i, j): i radial
+ * from the inner radius (0) to the outer (Nr), j angular from theta=0 to
+ * theta=pi/2; id = j*(Nr+1) + i.  Element sizes h_k = h0 * ratio^k. */
+void or_quarter_ring_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    *nn = (Nt + 1) * (Nr + 1);
+    *ne = Nt * Nr;
+    *nb = 4;
+    if (blen) { blen[0] = Nt + 1; blen[1] = Nr + 1; blen[2] = Nt + 1; blen[3] = Nr + 1; }
+}
+
+void or_gen_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
+                         double* x, double* y, int* conn, int* bord, int* blen)
+{
+    double* rho = (double*)malloc(sizeof(double) * (Nr + 1));
+    double h0 = (ratio == 1.0) ? (b - a) / Nr : (b - a) * (ratio - 1.0) / (pow(ratio, Nr) - 1.0);
+    rho[0] = a;
+    double h = h0;
+    for (int i = 1; i < Nr; ++i) { rho[i] = rho[i - 1] + h; h *= ratio; }
+    rho[Nr] = b;
+    for (int j = 0; j <= Nt; ++j) {
+        double c, s;
+        if (j == 0) { c = 1.; s = 0.; }
+        else if (j == Nt) { c = 0.; s = 1.; }
+        else { double t = 0.5 * OR_PI * (double)j / (double)Nt; c = cos(t); s = sin(t); }
+        for (int i = 0; i <= Nr; ++i) {
+            x[j * (Nr + 1) + i] = rho[i] * c;
+            y[j * (Nr + 1) + i] = rho[i] * s;
+        }
+    }
+    free(rho);
+    for (int j = 0; j < Nt; ++j)
+        for (int i = 0; i < Nr; ++i) {
+            int k = 4 * (j * Nr + i);
+            conn[k + 0] = (j + 1) * (Nr + 1) + i + 1;
+            conn[k + 1] = (j + 1) * (Nr + 1) + i;
+            conn[k + 2] = j * (Nr + 1) + i;
+            conn[k + 3] = j * (Nr + 1) + i + 1;
+        }
+    blen[0] = Nt + 1; blen[1] = Nr + 1; blen[2] = Nt + 1; blen[3] = Nr + 1;
+    int* b0 = bord;
+    int* b1 = b0 + blen[0];
+    int* b2 = b1 + blen[1];
+    int* b3 = b2 + blen[2];
+    for (int k = 0; k <= Nt; ++k) b0[k] = (Nt - k) * (Nr + 1);            /* inner arc, pi/2 -> 0 */
+    for (int k = 0; k <= Nr; ++k) b3[k] = k;                              /* theta=0, a -> b */
+    for (int k = 0; k <= Nt; ++k) b2[k] = k * (Nr + 1) + Nr;              /* outer arc, 0 -> pi/2 */
+    for (int k = 0; k <= Nr; ++k) b1[k] = Nt * (Nr + 1) + Nr - k;         /* theta=pi/2, b -> a */
+}
+
+/* findMaxIndexDiff -- Mesh.cu:417-441 (4-node part) */
+int or_max_index_diff(int ne, const int* conn)
+{
+    int mx = 0;
+    for (int e = 0; e < ne; ++e)
+        for (int i = 4 * e; i < 4 * (e + 1); ++i)
+            for (int j = i + 1; j < 4 * (e + 1); ++j) {
+                int d = abs(conn[i] - conn[j]);
+                if (d > mx) mx = d;
+            }
+    return mx;
+}
+
+/* elemSpace4 -- Mesh.cuh:258-267 */
+double or_elem_space4(const double* x, const double* y, const int* conn, int e)
+{
+    double s1 = 0., s2 = 0.;
+    for (int i = 4 * e + 1; i < 4 * (e + 1); ++i) {
+        s1 += x[conn[i - 1]] * y[conn[i]];
+        s2 += y[conn[i - 1]] * x[conn[i]];
+    }
+    s1 += x[conn[4 * (e + 1) - 1]] * y[conn[4 * e]];
+    s2 += y[conn[4 * (e + 1) - 1]] * x[conn[4 * e]];
+    return fabs(s1 - s2) * 0.5;
+}
+
+/* ====================================================================== */
+/* CSR pattern -- CudaSparseSLAE.cuh:303-458 (dim = 2, 4-node elements).   */
+/* adj(n) = other nodes of n's elements in first-appearance order scanning */
+/* elements in storage order and local nodes in local order.               */
+/* ====================================================================== */
+static int* build_adjacency(int nn, int ne, const int* conn, int** adjPos_out)
+{
+    int* cnt = (int*)calloc((size_t)nn + 1, sizeof(int));
+    for (int k = 0; k < 4 * ne; ++k) cnt[conn[k] + 1] += 3;
+    for (int n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
+    int* all = (int*)malloc(sizeof(int) * ((size_t)cnt[nn] + 1));
+    int* fill = (int*)malloc(sizeof(int) * ((size_t)nn + 1));
+    memcpy(fill, cnt, sizeof(int) * (size_t)nn);
+    for (int e = 0; e < ne; ++e)
+        for (int i = 0; i < 4; ++i) {
+            int node = conn[4 * e + i];
+            for (int j = 0; j < 4; ++j)
+                if (i != j) all[fill[node]++] = conn[4 * e + j];
+        }
+    /* unique, first appearance */
+    int* ucnt = (int*)calloc((size_t)nn + 1, sizeof(int));
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nn; ++n) {
+        int b = cnt[n], len = cnt[n + 1] - cnt[n], u = 0;
+        for (int k = 0; k < len; ++k) {
+            int v = all[b + k], seen = 0;
+            for (int q = 0; q < u; ++q)
+                if (all[b + q] == v) { seen = 1; break; }
+            if (!seen) all[b + u++] = v;
+        }
+        ucnt[n + 1] = u;
+    }
+    for (int n = 0; n < nn; ++n) ucnt[n + 1] += ucnt[n];
+    int* adj = (int*)malloc(sizeof(int) * ((size_t)ucnt[nn] + 1));
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nn; ++n)
+        memcpy(adj + ucnt[n], all + cnt[n], sizeof(int) * (size_t)(ucnt[n + 1] - ucnt[n]));
+    free(all); free(fill); free(cnt);
+    *adjPos_out = ucnt;
+    return adj;
+}
+
+long long or_csr_nnz(int nn, int ne, const int* conn)
+{
+    int* adjPos;
+    int* adj = build_adjacency(nn, ne, conn, &adjPos);
+    long long nnz = 4LL * ((long long)adjPos[nn] + nn);
+    free(adj); free(adjPos);
+    return nnz;
+}
+
+void or_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+{
+    int* adjPos;
+    int* adj = build_adjacency(nn, ne, conn, &adjPos);
+    /* rows[2n+i+1] = 4*begin + 2(i+1) size + 2(2n+i+1)   (CudaSparseSLAE.cuh:414-422) */
+    rows[0] = 0;
+    for (int n = 0; n < nn; ++n) {
+        int b = adjPos[n], s = adjPos[n + 1] - adjPos[n];
+        for (int i = 0; i < 2; ++i) rows[2 * n + i + 1] = b * 4 + (i + 1) * s * 2 + (2 * n + i + 1) * 2;
+    }
+    /* cols (CudaSparseSLAE.cuh:426-440) */
+    for (int n = 0; n < nn; ++n) {
+        int b = adjPos[n], s = adjPos[n + 1] - adjPos[n];
+        for (int i = 0; i < 2; ++i) {
+            int r0 = rows[2 * n + i];
+            cols[r0] = 2 * n;
+            cols[r0 + 1] = 2 * n + 1;
+            for (int l = 0; l < s; ++l) {
+                cols[r0 + (l + 1) * 2] = adj[b + l] * 2;
+                cols[r0 + (l + 1) * 2 + 1] = adj[b + l] * 2 + 1;
+            }
+        }
+    }
+    free(adj); free(adjPos);
+}
+
+/* ====================================================================== */
+/* Element math                                                            */
+/* ====================================================================== */
+
+/* calcB4 -- PlasticitySolver.cu:25-64.  Gauss points (r, s) = (p_i, p_j),
+ * k = 2i + j, p = -+0.577350269189626; B row 0 = dN/dx, row 1 = dN/dy. */
+void or_calc_b4(const double* xx, const double* yy, const int* conn, int e,
+                double B[4][2][4], double detJ[4])
+{
+    const double GS[2] = {-0.577350269189626, 0.577350269189626};
+    int b = 4 * e;
+    double x1 = xx[conn[b]], y1 = yy[conn[b]], x2 = xx[conn[b + 1]], y2 = yy[conn[b + 1]],
+           x3 = xx[conn[b + 2]], y3 = yy[conn[b + 2]], x4 = xx[conn[b + 3]], y4 = yy[conn[b + 3]];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) {
+            int k = i * 2 + j;
+            double r = GS[i], s = GS[j];
+            double J0 = 0.25 * ((1. + s) * x1 - (1. + s) * x2 - (1. - s) * x3 + (1. - s) * x4);
+            double J2 = 0.25 * ((1. + r) * x1 + (1. - r) * x2 - (1. - r) * x3 - (1. + r) * x4);
+            double J1 = 0.25 * ((1. + s) * y1 - (1. + s) * y2 - (1. - s) * y3 + (1. - s) * y4);
+            double J3 = 0.25 * ((1. + r) * y1 + (1. - r) * y2 - (1. - r) * y3 - (1. + r) * y4);
+            detJ[k] = J0 * J3 - J1 * J2;
+            double inv = 1. / detJ[k];
+            double i0 = J3 * inv, i1 = -J1 * inv, i2 = -J2 * inv, i3 = J0 * inv;
+            B[k][0][0] = 0.25 * ((1. + s) * i0 + (1. + r) * i1);
+            B[k][1][0] = 0.25 * ((1. + s) * i2 + (1. + r) * i3);
+            B[k][0][1] = 0.25 * ((-1. - s) * i0 + (1. - r) * i1);
+            B[k][1][1] = 0.25 * ((-1. - s) * i2 + (1. - r) * i3);
+            B[k][0][2] = 0.25 * ((-1. + s) * i0 - (1. - r) * i1);
+            B[k][1][2] = 0.25 * ((-1. + s) * i2 - (1. - r) * i3);
+            B[k][0][3] = 0.25 * ((1. - s) * i0 - (1. + r) * i1);
+            B[k][1][3] = 0.25 * ((1. - s) * i2 - (1. + r) * i3);
+        }
+}
+
+/* formC_plainStress / formC_plainStrain -- PlasticitySolver.cu:226-242; C row-major 3x3 */
+void or_form_c(int plane, double E, double nu, double C[9])
+{
+    for (int i = 0; i < 9; ++i) C[i] = 0.;
+    double coef;
+    if (plane == OR_PLANE_STRESS) {
+        C[0] = C[4] = 1.; C[1] = C[3] = nu; C[8] = (1. - nu) * 0.5;
+        coef = E / (1. - nu * nu);
+    } else {
+        C[0] = C[4] = 1.; C[1] = C[3] = nu / (1. - nu); C[8] = (1. - 2. * nu) * 0.5 / (1. - nu);
+        coef = E * (1. - nu) / ((1. + nu) * (1 - 2. * nu));
+    }
+    for (int i = 0; i < 9; ++i) C[i] *= coef;
+}
+
+#define KE(a, b) Ke[(a) * ((a) + 1) / 2 + (b)]
+
+/* formKe4 -- PlasticitySolver.cu:156-186; Ke packed lower triangle (SymMatrix.cuh:26-28) */
+void or_form_ke4(const double B[4][2][4], const double detJ[4], const double C[9], double h, double Ke[36])
+{
+    for (int q = 0; q < 36; ++q) Ke[q] = 0.;
+    for (int k = 0; k < 4; ++k) {
+        double dJ = detJ[k];
+        for (int i = 0; i < 4; ++i) {
+            int i2 = i * 2;
+            double BC0[3], BC1[3];
+            for (int j = 0; j < 3; ++j) {
+                BC0[j] = (B[k][0][i] * C[0 * 3 + j] + B[k][1][i] * C[2 * 3 + j]) * dJ;
+                BC1[j] = (B[k][1][i] * C[1 * 3 + j] + B[k][0][i] * C[2 * 3 + j]) * dJ;
+            }
+            KE(i2, i2) += BC0[0] * B[k][0][i] + BC0[2] * B[k][1][i];
+            KE(i2 + 1, i2) += BC1[0] * B[k][0][i] + BC1[2] * B[k][1][i];
+            KE(i2 + 1, i2 + 1) += BC1[1] * B[k][1][i] + BC1[2] * B[k][0][i];
+            for (int j = 0; j < i; ++j) {
+                int j2 = j * 2;
+                KE(i2, j2) += BC0[0] * B[k][0][j] + BC0[2] * B[k][1][j];
+                KE(i2, j2 + 1) += BC0[1] * B[k][1][j] + BC0[2] * B[k][0][j];
+                KE(i2 + 1, j2) += BC1[0] * B[k][0][j] + BC1[2] * B[k][1][j];
+                KE(i2 + 1, j2 + 1) += BC1[1] * B[k][1][j] + BC1[2] * B[k][0][j];
+            }
+        }
+    }
+    for (int q = 0; q < 36; ++q) Ke[q] *= h;
+}
+
+static int csr_find(const int* rows, const int* cols, int r, int c)
+{
+    for (int p = rows[r]; p < rows[r + 1]; ++p)
+        if (cols[p] == c) return p;
+    return -1;
+}
+
+/* distribKe4 -- PlasticitySolver.cu:711-746 (sequential, element order) */
+static void distrib_ke4(const int* rows, const int* cols, const int* conn, int e, const double* Ke, double* vals)
+{
+#define ADD(r, c, v) vals[csr_find(rows, cols, (r), (c))] += (v)
+    for (int i = 0; i < 4; ++i) {
+        int gi = 2 * conn[4 * e + i];
+        ADD(gi, gi, KE(2 * i, 2 * i));
+        ADD(gi + 1, gi, KE(2 * i + 1, 2 * i));
+        ADD(gi, gi + 1, KE(2 * i + 1, 2 * i));
+        ADD(gi + 1, gi + 1, KE(2 * i + 1, 2 * i + 1));
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < i; ++j) {
+            int gi = 2 * conn[4 * e + i], gj = 2 * conn[4 * e + j];
+            ADD(gi, gj, KE(2 * i, 2 * j));
+            ADD(gi + 1, gj, KE(2 * i + 1, 2 * j));
+            ADD(gi, gj + 1, KE(2 * i, 2 * j + 1));
+            ADD(gi + 1, gj + 1, KE(2 * i + 1, 2 * j + 1));
+            ADD(gj, gi, KE(2 * i, 2 * j));
+            ADD(gj, gi + 1, KE(2 * i + 1, 2 * j));
+            ADD(gj + 1, gi, KE(2 * i, 2 * j + 1));
+            ADD(gj + 1, gi + 1, KE(2 * i + 1, 2 * j + 1));
+        }
+#undef ADD
+}
+
+/* fillGlobalStiffness (CPU) -- PlasticitySolver.cu:786-810: C from (E_c, nu_c), Ke, scatter. */
+void or_assemble(int nn, int ne, const double* x, const double* y, const int* conn,
+                 const int* rows, const int* cols, const double* Ec, const double* nuc,
+                 int plane, double h, double* vals)
+{
+    long long nnz = rows[2 * nn];
+    memset(vals, 0, sizeof(double) * (size_t)nnz);
+    double* Kes = (double*)malloc(sizeof(double) * 36 * (size_t)ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        double B[4][2][4], dJ[4], C[9];
+        or_calc_b4(x, y, conn, e, B, dJ);
+        or_form_c(plane, Ec[e], nuc[e], C);
+        or_form_ke4(B, dJ, C, h, Kes + 36 * (size_t)e);
+    }
+    for (int e = 0; e < ne; ++e) distrib_ke4(rows, cols, conn, e, Kes + 36 * (size_t)e, vals);
+    free(Kes);
+}
+
+/* ====================================================================== */
+/* Loads and boundary conditions -- PlasticitySolver.cu:812-1005           */
+/* ====================================================================== */
+void or_rhs_border_force(const double* x, const double* y, const int* border, int len,
+                         const double* fval, int normOrient, double h, double* rp)
+{
+    for (int i = 1; i < len; ++i) {
+        int n1 = border[i - 1], n2 = border[i];
+        double hd1 = 0.5 * (x[n1] - x[n2]), hd2 = 0.5 * (y[n1] - y[n2]);
+        double dl = sqrt(hd1 * hd1 + hd2 * hd2);
+        double f1x = fval[2 * (i - 1)], f1y = fval[2 * (i - 1) + 1];
+        double f2x = fval[2 * i], f2y = fval[2 * i + 1];
+        if (normOrient) {
+            double tx = x[n2] - x[n1], ty = y[n2] - y[n1];
+            double tn = 1. / sqrt(tx * tx + ty * ty);
+            tx *= tn; ty *= tn;
+            double nx = ty, ny = -tx;
+            double nn_ = 1. / sqrt(nx * nx + ny * ny);
+            nx *= nn_; ny *= nn_;
+            double a1x = nx * f1x + tx * f1y, a1y = ny * f1x + ty * f1y;
+            double a2x = nx * f2x + tx * f2y, a2y = ny * f2x + ty * f2y;
+            f1x = a1x; f1y = a1y; f2x = a2x; f2y = a2y;
+        }
+        double c = h * dl;
+        rp[2 * n1] += f1x * c;
+        rp[2 * n1 + 1] += f1y * c;
+        rp[2 * n2] += f2x * c;
+        rp[2 * n2 + 1] += f2y * c;
+    }
+}
+
+/* 4-node body force with a constant field (PlasticitySolver.cu:899-910) */
+void or_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
+                       double gx, double gy, double rho, double h, double* rp)
+{
+    for (int e = 0; e < ne; ++e) {
+        double space = or_elem_space4(x, y, conn, e);
+        double coef = 0.25 * rho * h * space;
+        for (int i = 4 * e; i < 4 * (e + 1); ++i) {
+            int gi = 2 * conn[i];
+            rp[gi] += coef * gx;
+            rp[gi + 1] += coef * gy;
+        }
+    }
+}
+
+void or_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask)
+{
+    for (int i = 0; i < nn; ++i) {
+        if (vertical) { if (fabs(x[i] - coord) < 1e-14) mask[2 * i] = 0; }
+        else { if (fabs(y[i] - coord) < 1e-14) mask[2 * i + 1] = 0; }
+    }
+}
+
+void or_displacement_border(const int* border, int len, double ux, double uy,
+                            int fixMidX, int fixMidY, unsigned char* mask, double* u)
+{
+    int end = len - (border[0] == border[len - 1] ? 1 : 0);
+    if (fixMidX) { int gi = 2 * border[len / 2]; mask[gi] = 0; u[gi] = ux; }
+    else for (int i = 0; i < end; ++i) { int gi = 2 * border[i]; mask[gi] = 0; u[gi] = ux; }
+    if (fixMidY) { int gi = 2 * border[len / 2] + 1; mask[gi] = 0; u[gi] = uy; }
+    else for (int i = 0; i < end; ++i) { int gi = 2 * border[i] + 1; mask[gi] = 0; u[gi] = uy; }
+}
+
+/* ====================================================================== */
+/* Masked CG -- ConjGradSolver.h:52-101 (CPU) / ConjGradCuda.cu:185-231    */
+/* ====================================================================== */
+void or_spmv(int N, const int* rows, const int* cols, const double* vals, const double* x, double* y)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double s = 0.;
+        for (int j = rows[i]; j < rows[i + 1]; ++j) s += vals[j] * x[cols[j]];
+        y[i] = s;
+    }
+}
+
+int or_cg(int N, const int* rows, const int* cols, const double* vals, const double* b,
+          const unsigned char* mask, double* x, double tol, int maxit, int quirk, int* iters)
+{
+    double normB = 0.;
+    for (int i = 0; i < N; ++i) normB += b[i] * b[i];
+    *iters = 0;
+    double eps2 = tol * tol;
+    double* r = (double*)malloc(sizeof(double) * N);
+    double* z = (double*)malloc(sizeof(double) * N);
+    double* Az = (double*)malloc(sizeof(double) * N);
+    double* xp = quirk ? (double*)malloc(sizeof(double) * N) : NULL;
+    double rho = 0.;
+#pragma omp parallel for reduction(+ : rho) schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double s = 0.;
+        for (int j = rows[i]; j < rows[i + 1]; ++j) s += vals[j] * x[cols[j]];
+        double rk = mask[i] * (b[i] - s);
+        z[i] = r[i] = rk;
+        rho += rk * rk;
+    }
+    /* Displacement-driven problems have b = 0: the reference's rho/||b||^2
+     * test would then never be met; measure against ||r0||^2 instead. */
+    if (normB == 0.) normB = rho;
+    double invNormB = normB > 0. ? 1. / normB : 0.;
+    int k = 0, status = OR_CG_OK;
+    if (rho != 0.) {
+        do {
+            if (quirk) memcpy(xp, x, sizeof(double) * N);
+            double Azz = 0.;
+#pragma omp parallel for reduction(+ : Azz) schedule(static)
+            for (int i = 0; i < N; ++i) {
+                double s = 0.;
+                for (int j = rows[i]; j < rows[i + 1]; ++j) s += vals[j] * z[cols[j]];
+                Az[i] = s;
+                Azz += s * z[i];
+            }
+            double alpha = rho / Azz, rn = 0.;
+#pragma omp parallel for reduction(+ : rn) schedule(static)
+            for (int i = 0; i < N; ++i) {
+                x[i] += alpha * z[i];
+                double rk = r[i] - mask[i] * alpha * Az[i];
+                r[i] = rk;
+                rn += rk * rk;
+            }
+            double beta = rn / rho;
+#pragma omp parallel for schedule(static)
+            for (int i = 0; i < N; ++i) z[i] = r[i] + beta * z[i];
+            rho = rn;
+            ++k;
+            if (rho != rho) { status = OR_CG_NAN; break; }
+            if (k >= maxit) { if (rho * invNormB > eps2) status = OR_CG_MAXIT; break; }
+        } while (rho * invNormB > eps2);
+        if (quirk && (k % 2 == 1)) memcpy(x, xp, sizeof(double) * N);
+    }
+    *iters = k;
+    free(r); free(z); free(Az); free(xp);
+    return status;
+}
+
+/* ====================================================================== */
+/* Secant (variable elasticity parameters) update -- PlasticitySolver.cu   */
+/* :1320-1335 (CPU) and plastStress :1590-1610 (GPU)                        */
+/* out: sxx, syy, tau, sigI, epsI, sigTab, psi(G), nuc_new                  */
+/* ====================================================================== */
+void or_secant_update(const or_material* m, double Ec, double nuc,
+                      double exx, double eyy, double gxy, double out[8])
+{
+    double C[9];
+    or_form_c(m->plane, Ec, nuc, C);
+    double sxx = C[0] * exx + C[1] * eyy;
+    double syy = C[1] * exx + C[4] * eyy;
+    double tau = C[8] * gxy;
+    double sigI = sqrt(0.5 * (sxx * sxx + syy * syy + (sxx - syy) * (sxx - syy) + 6. * tau * tau));
+    double epsI = sigI / Ec;
+    double sigTab = or_material_f(m, epsI);
+    double psi = (sigTab != 0.) ? m->E * epsI / sigTab : 1.;
+    out[0] = sxx; out[1] = syy; out[2] = tau; out[3] = sigI; out[4] = epsI; out[5] = sigTab;
+    out[6] = psi;
+    out[7] = 0.5 * (1. - (1. - 2. * m->nu) / psi);
+}
+
+/* Aitken (Irons-Tuck) relaxation of the secant iteration, in the variable
+ * x = log(psi): r_k = log G(psi_k) - log psi_k, omega_k = -omega_{k-1}
+ * (r_{k-1}.(r_k - r_{k-1})) / |r_k - r_{k-1}|^2, x_{k+1} = x_k + omega_k r_k.
+ * Safeguard: whenever |r_k| > |r_{k-1}| (the transient while the plastic zone
+ * forms) a plain step is taken and omega restarts at 1.  The final update of
+ * a load level is always a plain one, as in the reference loop. */
+#define OR_OMEGA_MIN 0.05
+#define OR_OMEGA_MAX 100.0
+
+int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
+             const double* rp, const unsigned char* mask, double* u,
+             const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
+{
+    int N = 2 * nn;
+    long long nnz = or_csr_nnz(nn, ne, conn);
+    int* rows = (int*)malloc(sizeof(int) * (N + 1));
+    int* cols = (int*)malloc(sizeof(int) * (size_t)nnz);
+    double* vals = (double*)malloc(sizeof(double) * (size_t)nnz);
+    or_csr_pattern(nn, ne, conn, rows, cols);
+
+    double (*B)[2][4] = (double (*)[2][4])malloc(sizeof(double) * 32 * (size_t)ne);
+    double* dJ = (double*)malloc(sizeof(double) * 4 * (size_t)ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) or_calc_b4(x, y, conn, e, (double (*)[2][4])(B + 4 * (size_t)e), dJ + 4 * (size_t)e);
+
+    double* F[OR_NFIELDS];
+    for (int f = 0; f < OR_NFIELDS; ++f) F[f] = st_ + (size_t)f * ne;
+    double* G = (double*)malloc(sizeof(double) * ne);
+    double* rcur = (double*)malloc(sizeof(double) * ne);
+    double* rprev = (double*)malloc(sizeof(double) * ne);
+    if (!opt->warm) {
+        for (int e = 0; e < ne; ++e) { F[OR_F_PSI][e] = 1.; F[OR_F_EC][e] = mat->E; F[OR_F_NUC][e] = mat->nu; }
+    }
+
+    stats->outer_iters = 0;
+    stats->cg_iters = 0;
+    stats->rel_err = 0.;
+    stats->status = 1;
+    double omega_prev = 1., rr_prev = 0.;
+    for (int e = 0; e < ne; ++e) rprev[e] = 0.;
+    int maxOuter = opt->fixed_outer > 0 ? opt->fixed_outer : (opt->elastic ? 1 : opt->max_outer);
+
+    for (int k = 0; k < maxOuter; ++k) {
+        or_assemble(nn, ne, x, y, conn, rows, cols, F[OR_F_EC], F[OR_F_NUC], mat->plane, mat->h, vals);
+        int it = 0;
+        int cgs = or_cg(N, rows, cols, vals, rp, mask, u, opt->cg_tol, opt->cg_maxit, opt->cg_quirk, &it);
+        stats->cg_iters += it;
+        stats->outer_iters = k + 1;
+        if (cgs == OR_CG_NAN) { stats->status = 2; break; }
+
+        /* strain4 (:1517-1549) + plastStress (:1590-1610) */
+        double num = 0., den = 0., rr = 0., dr = 0., dd = 0.;
+#pragma omp parallel for reduction(+ : num, den, rr, dr, dd) schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            const int* c = conn + 4 * e;
+            double u1 = u[2 * c[0]], v1 = u[2 * c[0] + 1], u2 = u[2 * c[1]], v2 = u[2 * c[1] + 1],
+                   u3 = u[2 * c[2]], v3 = u[2 * c[2] + 1], u4 = u[2 * c[3]], v4 = u[2 * c[3] + 1];
+            double exx = 0., eyy = 0., gxy = 0.;
+            for (int g = 0; g < 4; ++g) {
+                double (*b)[4] = B[4 * (size_t)e + g];
+                exx += u1 * b[0][0] + u2 * b[0][1] + u3 * b[0][2] + u4 * b[0][3];
+                eyy += v1 * b[1][0] + v2 * b[1][1] + v3 * b[1][2] + v4 * b[1][3];
+                gxy += u1 * b[1][0] + u2 * b[1][1] + u3 * b[1][2] + u4 * b[1][3] +
+                       v1 * b[0][0] + v2 * b[0][1] + v3 * b[0][2] + v4 * b[0][3];
+            }
+            exx *= 0.25; eyy *= 0.25; gxy *= 0.25;
+            double o[8];
+            or_secant_update(mat, F[OR_F_EC][e], F[OR_F_NUC][e], exx, eyy, gxy, o);
+            F[OR_F_EXX][e] = exx; F[OR_F_EYY][e] = eyy; F[OR_F_GXY][e] = gxy;
+            F[OR_F_SXX][e] = o[0]; F[OR_F_SYY][e] = o[1]; F[OR_F_TAU][e] = o[2];
+            F[OR_F_SIGI][e] = o[3]; F[OR_F_EPSI][e] = o[4]; F[OR_F_SIGTAB][e] = o[5];
+            G[e] = o[6];
+            double d = o[3] - o[5];
+            num += d * d;
+            den += o[5] * o[5];
+            if (opt->aitken) {
+                double rc = log(o[6]) - log(F[OR_F_PSI][e]);
+                double q = rc - rprev[e];
+                rcur[e] = rc;
+                rr += rc * rc;
+                dr += rprev[e] * q;
+                dd += q * q;
+            }
+        }
+        double relErr = sqrt(num / den);
+        stats->rel_err = relErr;
+        if (relErr != relErr) { stats->status = 2; break; }
+        int last = opt->elastic || (opt->fixed_outer > 0 ? (k + 1 == opt->fixed_outer)
+                                                         : (relErr <= opt->tol || k + 1 == maxOuter));
+        int plain = last || !opt->aitken || k == 0 || rr > rr_prev;
+        double omega = 1.;
+        if (!plain) {
+            omega = dd > 0. ? -omega_prev * dr / dd : omega_prev;
+            if (omega < OR_OMEGA_MIN) omega = OR_OMEGA_MIN;
+            if (omega > OR_OMEGA_MAX) omega = OR_OMEGA_MAX;
+        }
+#pragma omp parallel for schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            double p = G[e];
+            if (!plain) {
+                p = exp(log(F[OR_F_PSI][e]) + omega * rcur[e]);
+                if (p < 1.) p = 1.;
+            }
+            F[OR_F_PSI][e] = p;
+            F[OR_F_EC][e] = mat->E / p;
+            F[OR_F_NUC][e] = 0.5 * (1. - (1. - 2. * mat->nu) / p);
+            rprev[e] = rcur[e];
+        }
+        omega_prev = omega;
+        rr_prev = rr;
+        if (last) {
+            if (opt->elastic || opt->fixed_outer > 0 || relErr <= opt->tol) stats->status = 0;
+            break;
+        }
+    }
+
+    if (!opt->elastic) {
+        /* plastDeform -- PlasticitySolver.cu:1641-1653 / calcPlastDeform :1356-1368 */
+        double Gm = mat->E * 0.5 / (1. + mat->nu);
+        for (int e = 0; e < ne; ++e) {
+            double Ec = F[OR_F_EC][e], nuc = F[OR_F_NUC][e];
+            double ex = (1. / Ec - 1. / mat->E) * F[OR_F_SXX][e];
+            double ey = (1. / Ec - 1. / mat->E) * F[OR_F_SYY][e];
+            double Gc = Ec * 0.5 / (1. + nuc);
+            double gp = (1. / Gc - 1. / Gm) * F[OR_F_TAU][e];
+            F[OR_F_EXXP][e] = ex; F[OR_F_EYYP][e] = ey; F[OR_F_GXYP][e] = gp;
+            F[OR_F_EPSIP][e] = sqrt(2.) / 3. * sqrt((ex - ey) * (ex - ey) + 6. * gp * gp);
+            F[OR_F_PSI][e] = mat->E / Ec;
+        }
+    }
+    free(rows); free(cols); free(vals); free(B); free(dJ); free(G); free(rcur); free(rprev);
+    return stats->status;
+}
+
+/* deformEnergy -- PlasticitySolver.cuh:415-445 (4-node part) */
+double or_deform_energy(int ne, const double* x, const double* y, const int* conn,
+                        const or_material* m, const double* st)
+{
+    const double* sigI = st + (size_t)OR_F_SIGI * ne;
+    const double* epsI = st + (size_t)OR_F_EPSI * ne;
+    double energy = 0.;
+    for (int e = 0; e < ne; ++e) {
+        double vol = m->h * or_elem_space4(x, y, conn, e);
+        if (sigI[e] > m->sigmaT) {
+            double ET = m->sigmaT / m->E;
+            energy += m->sigmaT * ET * 0.5 * vol;
+            double he = (epsI[e] - ET) * 0.01;
+            for (int j = 0; j < 100; ++j)
+                energy += he * 0.5 * (or_material_f(m, ET + he * j) + or_material_f(m, ET + he * (j + 1))) * vol;
+        } else
+            energy += epsI[e] * sigI[e] * 0.5 * vol;
+    }
+    return energy;
+}

@@ -1,0 +1,68 @@
+"""Extract the reference's committed result files into small numpy fixtures.
+
+Run in the development container (the reference checkout is not present on
+the GPU machine):  python tests/golden/make_fixtures.py [REFERENCE_ROOT]
+
+Inputs (reference data/, ASCII outputs of the reference's own experiments):
+  solverTest.vtk         Main.cpp:14-38   elastic 4x4 rectangle, CPU path
+  holePlatePlastPow.vtk  Main.cpp:252-296 48x32 hole plate, power law, S=170
+  diagramLin.txt / diagramPow.txt  Main.cpp:929-994  uniaxial sigma(eps)
+Outputs: tests/golden/*.npz (data only: node coordinates, connectivity,
+per-element fields and nodal displacements exactly as printed).
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def read_vtk(path):
+    with open(path) as f:
+        tok = f.read().split("\n")
+    out = {"fields": {}}
+    i = 0
+    while i < len(tok):
+        line = tok[i].strip()
+        if line.startswith("POINTS"):
+            n = int(line.split()[1])
+            pts = np.array([list(map(float, tok[i + 1 + k].split()))[:2] for k in range(n)])
+            out["points"] = pts
+            i += n + 1
+            continue
+        if line.startswith("POLYGONS"):
+            n = int(line.split()[1])
+            polys = np.array([list(map(int, tok[i + 1 + k].split()))[1:] for k in range(n)], dtype=np.int32)
+            out["polys"] = polys
+            i += n + 1
+            continue
+        parts = line.split()
+        if len(parts) == 4 and parts[3] == "float" and parts[0] not in ("POINTS",):
+            name, ncomp, n = parts[0], int(parts[1]), int(parts[2])
+            vals = np.array(list(map(float, tok[i + 1].split())))
+            assert vals.size == ncomp * n, (name, vals.size)
+            out["fields"][name] = vals.reshape(n, ncomp) if ncomp > 1 else vals
+            i += 2
+            continue
+        i += 1
+    return out
+
+
+def main(ref_root):
+    data = os.path.join(ref_root, "data")
+    for name in ("solverTest", "holePlatePlastPow"):
+        v = read_vtk(os.path.join(data, name + ".vtk"))
+        arrays = {"points": v["points"], "polys": v["polys"]}
+        for k, a in v["fields"].items():
+            arrays["f_" + k] = a
+        np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
+        print(name, {k: a.shape for k, a in arrays.items()})
+    lin = np.loadtxt(os.path.join(data, "diagramLin.txt"))
+    pw = np.loadtxt(os.path.join(data, "diagramPow.txt"))
+    np.savez_compressed(os.path.join(HERE, "diagram.npz"), lin=lin, pow=pw)
+    print("diagram", lin.shape, pw.shape)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

@@ -1,0 +1,284 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only tests/, __graft_entry__.smoke() (as the checker)
+and bench.py's cpu_baseline leg use it.  The oracle restates the reference
+(SATOH-SD/CudaPlasticity) in plain C; see oracle/cp_oracle.h.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U8 = C.POINTER(C.c_ubyte)
+
+LAW_LINEAR, LAW_POWER, LAW_SWIFT = 0, 1, 2
+PLANE_STRESS, PLANE_STRAIN = 0, 1
+FIELDS = ["exx", "eyy", "gxy", "epsI", "sxx", "syy", "tau", "sigI", "sigTab", "psi", "Ec", "nuc",
+          "exx_p", "eyy_p", "gxy_p", "epsI_p"]
+
+
+class Material(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("h", C.c_double), ("sigmaT", C.c_double),
+                ("law", C.c_int), ("plane", C.c_int), ("K_T", C.c_double), ("A", C.c_double),
+                ("m", C.c_double), ("n", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("tol", C.c_double),
+                ("max_outer", C.c_int), ("fixed_outer", C.c_int), ("aitken", C.c_int),
+                ("cg_quirk", C.c_int), ("elastic", C.c_int), ("warm", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("cg_iters", C.c_longlong), ("rel_err", C.c_double),
+                ("status", C.c_int)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    lib = C.CDLL(LIB_PATH)
+    lib.or_material_f.restype = C.c_double
+    lib.or_material_f.argtypes = [C.POINTER(Material), C.c_double]
+    lib.or_hole_radN.restype = C.c_int
+    lib.or_csr_nnz.restype = C.c_longlong
+    lib.or_max_index_diff.restype = C.c_int
+    lib.or_elem_space4.restype = C.c_double
+    lib.or_deform_energy.restype = C.c_double
+    lib.or_solve.restype = C.c_int
+    lib.or_cg.restype = C.c_int
+    return lib
+
+
+lib = _load()
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def u8p(a):
+    return a.ctypes.data_as(_U8)
+
+
+class Mesh:
+    """Plain arrays: x, y (nn), conn (ne, 4) int32, borders (list of int32 arrays)."""
+
+    def __init__(self, x, y, conn, borders):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 4)
+        self.borders = [np.ascontiguousarray(b, dtype=np.int32) for b in borders]
+
+    @property
+    def nn(self):
+        return self.x.size
+
+    @property
+    def ne(self):
+        return self.conn.shape[0]
+
+
+def _gen(sizes_fn, sizes_args, gen_fn, gen_args):
+    nn, ne, nb = C.c_int(), C.c_int(), C.c_int()
+    blen = np.zeros(8, dtype=np.int32)
+    sizes_fn(*sizes_args, C.byref(nn), C.byref(ne), C.byref(nb), ip(blen))
+    x = np.zeros(nn.value)
+    y = np.zeros(nn.value)
+    conn = np.zeros(4 * ne.value, dtype=np.int32)
+    bord = np.zeros(int(blen[: nb.value].sum()), dtype=np.int32)
+    gen_fn(*gen_args, dp(x), dp(y), ip(conn), ip(bord), ip(blen))
+    borders, off = [], 0
+    for b in range(nb.value):
+        borders.append(bord[off: off + blen[b]].copy())
+        off += blen[b]
+    return Mesh(x, y, conn, borders)
+
+
+def gen_rect(x1, x2, y1, y2, N1, N2):
+    """genRectangle (Mesh.cu:17-109)."""
+    f = C.c_double
+    return _gen(lib.or_rect_sizes, (N1, N2), lib.or_gen_rect, (f(x1), f(x2), f(y1), f(y2), N1, N2))
+
+
+def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2):
+    """genRectWithHole + renumerateRing (Mesh.cu:112-245, 332-384)."""
+    f = C.c_double
+    return _gen(lib.or_hole_sizes, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2),
+                lib.or_gen_rect_with_hole, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2))
+
+
+def gen_ring(a, b, Nphi, Nr):
+    """genRing + renumerateRing (Mesh.cu:249-329)."""
+    f = C.c_double
+    return _gen(lib.or_ring_sizes, (Nphi, Nr), lib.or_gen_ring, (f(a), f(b), Nphi, Nr))
+
+
+def gen_quarter_plate(L, r, Nt, Nr):
+    f = C.c_double
+    return _gen(lib.or_quarter_plate_sizes, (Nt, Nr), lib.or_gen_quarter_plate, (f(L), f(r), Nt, Nr))
+
+
+def gen_quarter_ring(a, b, Nt, Nr, ratio):
+    f = C.c_double
+    return _gen(lib.or_quarter_ring_sizes, (Nt, Nr), lib.or_gen_quarter_ring,
+                (f(a), f(b), Nt, Nr, f(ratio)))
+
+
+def material(E, nu, sigmaT=1e300, law=LAW_LINEAR, K_T=None, A=0.0, m=0.0, n=0.0, h=1.0, plane=PLANE_STRESS):
+    return Material(E, nu, h, sigmaT, law, plane, E if K_T is None else K_T, A, m, n)
+
+
+def material_f(mat, eps):
+    return lib.or_material_f(C.byref(mat), C.c_double(eps))
+
+
+def csr_pattern(mesh):
+    nnz = lib.or_csr_nnz(mesh.nn, mesh.ne, ip(mesh.conn))
+    rows = np.zeros(2 * mesh.nn + 1, dtype=np.int32)
+    cols = np.zeros(nnz, dtype=np.int32)
+    lib.or_csr_pattern(mesh.nn, mesh.ne, ip(mesh.conn), ip(rows), ip(cols))
+    return rows, cols
+
+
+def assemble(mesh, rows, cols, Ec, nuc, plane=PLANE_STRESS, h=1.0):
+    vals = np.zeros(cols.size)
+    lib.or_assemble(mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn), ip(rows), ip(cols),
+                    dp(np.ascontiguousarray(Ec, dtype=np.float64)),
+                    dp(np.ascontiguousarray(nuc, dtype=np.float64)), plane, C.c_double(h), dp(vals))
+    return vals
+
+
+def calc_b4(mesh, e):
+    B = np.zeros((4, 2, 4))
+    dJ = np.zeros(4)
+    lib.or_calc_b4(dp(mesh.x), dp(mesh.y), ip(mesh.conn), e, dp(B), dp(dJ))
+    return B, dJ
+
+
+def form_c(plane, E, nu):
+    Cm = np.zeros(9)
+    lib.or_form_c(plane, C.c_double(E), C.c_double(nu), dp(Cm))
+    return Cm.reshape(3, 3)
+
+
+def form_ke4(B, dJ, Cm, h=1.0):
+    Ke = np.zeros(36)
+    lib.or_form_ke4(dp(np.ascontiguousarray(B)), dp(np.ascontiguousarray(dJ)),
+                    dp(np.ascontiguousarray(Cm, dtype=np.float64).reshape(9)), C.c_double(h), dp(Ke))
+    full = np.zeros((8, 8))
+    for a in range(8):
+        for b in range(a + 1):
+            full[a, b] = full[b, a] = Ke[a * (a + 1) // 2 + b]
+    return full
+
+
+class Problem:
+    """Load vector, Dirichlet mask and prescribed displacements for a mesh,
+    built with the reference's initConditions rules (PlasticitySolver.cu:812-1005)."""
+
+    def __init__(self, mesh, h=1.0):
+        self.mesh = mesh
+        self.h = h
+        self.rp = np.zeros(2 * mesh.nn)
+        self.mask = np.ones(2 * mesh.nn, dtype=np.uint8)
+        self.u = np.zeros(2 * mesh.nn)
+
+    def force(self, border_id, fx, fy, norm_orient=False):
+        b = self.mesh.borders[border_id]
+        fv = np.tile(np.array([fx, fy], dtype=np.float64), b.size)
+        lib.or_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
+                                int(norm_orient), C.c_double(self.h), dp(self.rp))
+        return self
+
+    def force_values(self, border_id, fv, norm_orient=False):
+        b = self.mesh.borders[border_id]
+        fv = np.ascontiguousarray(fv, dtype=np.float64).reshape(-1)
+        lib.or_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
+                                int(norm_orient), C.c_double(self.h), dp(self.rp))
+        return self
+
+    def gravity(self, g=9.81, rho=1.0):
+        lib.or_rhs_body_force(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
+                              C.c_double(0.0 * g), C.c_double(-g), C.c_double(rho), C.c_double(self.h),
+                              dp(self.rp))
+        return self
+
+    def displacement(self, border_id, ux, uy, fix_mid_x=False, fix_mid_y=False):
+        b = self.mesh.borders[border_id]
+        lib.or_displacement_border(ip(b), b.size, C.c_double(ux), C.c_double(uy), int(fix_mid_x),
+                                   int(fix_mid_y), u8p(self.mask), dp(self.u))
+        return self
+
+    def fix_vert_axis(self, x):
+        lib.or_fix_axis(self.mesh.nn, dp(self.mesh.x), dp(self.mesh.y), 1, C.c_double(x), u8p(self.mask))
+        return self
+
+    def fix_hor_axis(self, y):
+        lib.or_fix_axis(self.mesh.nn, dp(self.mesh.x), dp(self.mesh.y), 0, C.c_double(y), u8p(self.mask))
+        return self
+
+
+def options(cg_tol=1e-7, cg_maxit=1000000, tol=1e-5, max_outer=50, fixed_outer=0, aitken=False,
+            cg_quirk=False, elastic=False, warm=False):
+    return Options(cg_tol, cg_maxit, tol, max_outer, fixed_outer, int(aitken), int(cg_quirk),
+                   int(elastic), int(warm))
+
+
+def solve(prob, mat, opt, u=None, state=None, rp=None):
+    """One load level of the secant loop.  Returns (u, state dict, stats)."""
+    m = prob.mesh
+    u = prob.u.copy() if u is None else np.ascontiguousarray(u, dtype=np.float64).copy()
+    st = np.zeros((len(FIELDS), m.ne)) if state is None else np.ascontiguousarray(state).copy()
+    rp = prob.rp if rp is None else np.ascontiguousarray(rp, dtype=np.float64)
+    stats = Stats()
+    lib.or_solve(m.nn, m.ne, dp(m.x), dp(m.y), ip(m.conn), dp(rp), u8p(prob.mask), dp(u),
+                 C.byref(mat), C.byref(opt), dp(st), C.byref(stats))
+    return u, st, stats
+
+
+def fields(st):
+    return {name: st[i] for i, name in enumerate(FIELDS)}
+
+
+def cg(rows, cols, vals, b, mask, x0, tol, maxit=1000000, quirk=False):
+    x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+    it = C.c_int()
+    status = lib.or_cg(b.size, ip(rows), ip(cols), dp(vals), dp(b), u8p(mask), dp(x), C.c_double(tol),
+                       maxit, int(quirk), C.byref(it))
+    return x, it.value, status
+
+
+def deform_energy(mesh, mat, st):
+    return lib.or_deform_energy(mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn), C.byref(mat),
+                                dp(np.ascontiguousarray(st)))
+
+
+def elem_space4(mesh, e):
+    return lib.or_elem_space4(dp(mesh.x), dp(mesh.y), ip(mesh.conn), e)
+
+
+def solve_steps(prob, mat, nsteps, opt, final_factor=1.0):
+    """Incremental loading: level n applies n/nsteps of the load, warm-started
+    from level n-1's displacements and secant moduli."""
+    u = prob.u * (1.0 / nsteps * final_factor)
+    st = None
+    per = []
+    for n in range(1, nsteps + 1):
+        lam = final_factor * n / nsteps
+        u = np.where(prob.mask == 0, prob.u * lam, u)
+        o = Options.from_buffer_copy(opt)
+        o.warm = 1 if st is not None else 0
+        u, st, s = solve(prob, mat, o, u=u, state=st, rp=prob.rp * lam)
+        per.append((s.outer_iters, s.cg_iters, s.rel_err, s.status))
+    return u, st, per

@@ -1,0 +1,348 @@
+"""Pins of the CPU oracle against the reference's own committed results.
+
+Fixtures (tests/golden/*.npz, extracted by tests/golden/make_fixtures.py):
+  solverTest.npz         -- Main.cpp:14-38, 4x4 elastic rectangle, CPU path
+  holePlatePlastPow.npz  -- Main.cpp:252-296, 48x32 hole plate, power law
+  diagram.npz            -- Main.cpp:929-994, uniaxial sigma(eps), 40 levels
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+E_304L, S_T_304L, NU_304L = 193e9, 170e6, 0.29
+
+
+def load(name):
+    return np.load(os.path.join(GOLD, name + ".npz"))
+
+
+def relnorm(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(b)
+
+
+# --------------------------------------------------------------- meshes
+
+def test_gen_rect_matches_solverTest_mesh():
+    fx = load("solverTest")
+    m = O.gen_rect(-1, 1, -1, 1, 4, 4)
+    assert np.array_equal(m.conn, fx["polys"])
+    assert np.array_equal(np.c_[m.x, m.y], fx["points"])
+
+
+def test_gen_rect_with_hole_matches_fixture_mesh():
+    """genRectWithHole + renumerateRing: connectivity is bit-exact, points
+    agree to the 6 digits the fixture prints (Mesh.cu:112-245,332-384)."""
+    fx = load("holePlatePlastPow")
+    m = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)
+    assert (m.nn, m.ne) == (5280, 5120)
+    assert np.array_equal(m.conn, fx["polys"])
+    assert np.abs(np.c_[m.x, m.y] - fx["points"]).max() < 1e-5 * 3
+    assert m.conn[0].tolist() == [67, 1, 0, 66]
+
+
+def test_csr_pattern_fixture_mesh():
+    """CudaSparseSLAE.cuh:303-458 on the fixture mesh: nnz and the row-0
+    column order (node 0 adjacency [67, 1, 66, 34, 33] in first-appearance order)."""
+    m = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)
+    rows, cols = O.csr_pattern(m)
+    assert rows[-1] == cols.size == 186240
+    assert cols[rows[0]:rows[1]].tolist() == [0, 1, 134, 135, 2, 3, 132, 133, 68, 69, 66, 67]
+
+
+def test_csr_pattern_brute_force():
+    """Independent statement of the pattern rule on a small mesh: for node n,
+    the row lists n itself then every other node met while scanning elements
+    in storage order (local order within an element), first appearance kept."""
+    m = O.gen_rect_with_hole(-1, 1, -1, 1, 0.3, 4, 4)
+    rows, cols = O.csr_pattern(m)
+    adj = {n: [] for n in range(m.nn)}
+    for e in range(m.ne):
+        for i in range(4):
+            for j in range(4):
+                if i != j and m.conn[e, j] not in adj[m.conn[e, i]]:
+                    adj[m.conn[e, i]].append(m.conn[e, j])
+    ptr = 0
+    for n in range(m.nn):
+        nodes = [n] + adj[n]
+        expect = [2 * a + c for a in nodes for c in (0, 1)]
+        for i in (0, 1):
+            assert rows[2 * n + i] == ptr
+            assert cols[ptr:ptr + len(expect)].tolist() == expect
+            ptr += len(expect)
+    assert rows[-1] == ptr
+
+
+# --------------------------------------------------------------- element math
+
+def test_ke_unit_square_closed_form():
+    """Q4 plane-stress stiffness of a unit square (the closed form used by
+    the 88-line topology optimisation code, nodes BL, BR, TR, TL)."""
+    E, nu = 1.0, 0.3
+    k = [0.5 - nu / 6, 0.125 + nu / 8, -0.25 - nu / 12, -0.125 + 3 * nu / 8,
+         -0.25 + nu / 12, -0.125 - nu / 8, nu / 6, 0.125 - 3 * nu / 8]
+    idx = [[0, 1, 2, 3, 4, 5, 6, 7], [1, 0, 7, 6, 5, 4, 3, 2], [2, 7, 0, 5, 6, 3, 4, 1],
+           [3, 6, 5, 0, 7, 2, 1, 4], [4, 5, 6, 7, 0, 1, 2, 3], [5, 4, 3, 2, 1, 0, 7, 6],
+           [6, 3, 4, 1, 2, 7, 0, 5], [7, 2, 1, 4, 3, 6, 5, 0]]
+    KE = E / (1 - nu * nu) * np.array([[k[i] for i in row] for row in idx])
+    m = O.gen_rect(0, 1, 0, 1, 1, 1)          # element 0 = [TR, TL, BL, BR]
+    B, dJ = O.calc_b4(m, 0)
+    Ke = O.form_ke4(B, dJ, O.form_c(O.PLANE_STRESS, E, nu))
+    perm = [2, 3, 0, 1]                        # reference local node -> BL,BR,TR,TL index
+    P = np.zeros((8, 8))
+    for a, t in enumerate(perm):
+        P[2 * a, 2 * t] = P[2 * a + 1, 2 * t + 1] = 1
+    np.testing.assert_allclose(Ke, P @ KE @ P.T, atol=1e-13)
+    assert np.allclose(dJ, 0.25)
+
+
+def test_b4_patch_and_rigid_modes():
+    """Partition of unity (rows of B sum to zero), exact constant strain for a
+    linear field, and three rigid-body null modes of Ke on a distorted quad."""
+    m = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8)
+    e = 17
+    B, dJ = O.calc_b4(m, e)
+    assert np.abs(B.sum(axis=2)).max() < 1e-12
+    a, b, c, d = 1e-3, -2e-3, 5e-4, 7e-4
+    xs, ys = m.x[m.conn[e]], m.y[m.conn[e]]
+    u, v = a * xs + b * ys, c * xs + d * ys
+    for g in range(4):
+        assert np.isclose(B[g, 0] @ u, a, atol=1e-15)
+        assert np.isclose(B[g, 1] @ v, d, atol=1e-15)
+        assert np.isclose(B[g, 1] @ u + B[g, 0] @ v, b + c, atol=1e-15)
+    Ke = O.form_ke4(B, dJ, O.form_c(O.PLANE_STRESS, 2.1e5, 0.3))
+    modes = [np.tile([1, 0], 4), np.tile([0, 1], 4), np.ravel(np.c_[-ys, xs])]
+    for mode in modes:
+        assert np.abs(Ke @ mode).max() < 1e-9 * np.abs(Ke).max()
+    w = np.linalg.eigvalsh(Ke)
+    assert (np.abs(w) < 1e-9 * w.max()).sum() == 3
+
+
+def test_assembly_equals_dense_sum():
+    m = O.gen_quarter_plate(1.0, 0.3, 4, 3)
+    rows, cols = O.csr_pattern(m)
+    Ec = np.linspace(1.0e5, 2.1e5, m.ne)
+    nuc = np.linspace(0.3, 0.45, m.ne)
+    vals = O.assemble(m, rows, cols, Ec, nuc)
+    N = 2 * m.nn
+    K = np.zeros((N, N))
+    for i in range(N):
+        K[i, cols[rows[i]:rows[i + 1]]] = vals[rows[i]:rows[i + 1]]
+    D = np.zeros((N, N))
+    for e in range(m.ne):
+        B, dJ = O.calc_b4(m, e)
+        Ke = O.form_ke4(B, dJ, O.form_c(O.PLANE_STRESS, Ec[e], nuc[e]))
+        dofs = np.ravel([[2 * n, 2 * n + 1] for n in m.conn[e]])
+        D[np.ix_(dofs, dofs)] += Ke
+    np.testing.assert_allclose(K, D, atol=1e-9 * np.abs(D).max())
+    assert np.abs(K - K.T).max() < 1e-9 * np.abs(K).max()
+
+
+def test_cg_equals_dense_reduced_solve():
+    """Masked CG (ConjGradSolver.h:52-101) solves K_ff u_f = b_f - K_fc u_c."""
+    m = O.gen_quarter_plate(1.0, 0.2, 8, 5)
+    p = O.Problem(m).force(1, 100.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
+    p.u[2 * m.borders[2] + 1] = 1e-5            # a prescribed field on fixed DOFs
+    p.mask[2 * m.borders[2] + 1] = 0
+    rows, cols = O.csr_pattern(m)
+    vals = O.assemble(m, rows, cols, np.full(m.ne, 2.1e5), np.full(m.ne, 0.3))
+    x, it, st = O.cg(rows, cols, vals, p.rp, p.mask, p.u, 1e-14)
+    N = 2 * m.nn
+    K = np.zeros((N, N))
+    for i in range(N):
+        K[i, cols[rows[i]:rows[i + 1]]] = vals[rows[i]:rows[i + 1]]
+    f = p.mask.astype(bool)
+    uf = np.linalg.solve(K[np.ix_(f, f)], p.rp[f] - K[np.ix_(f, ~f)] @ p.u[~f])
+    assert st == 0
+    np.testing.assert_allclose(x[f], uf, rtol=1e-9, atol=1e-12 * np.abs(uf).max())
+    np.testing.assert_array_equal(x[~f], p.u[~f])
+
+
+# --------------------------------------------------------------- reference results
+
+def test_solverTest_elastic_fixture():
+    """solverTest (Main.cpp:14-38): uniaxial traction 1 on a 4x4 block."""
+    fx = load("solverTest")
+    m = O.gen_rect(-1, 1, -1, 1, 4, 4)
+    p = O.Problem(m).displacement(2, 0.0, 0.0, False, True).force(0, 1.0, 0.0)
+    mat = O.material(E_304L * 1e-6, NU_304L * 1e-6, S_T_304L, O.LAW_LINEAR, K_T=E_304L * 1e-6)
+    u, st, s = O.solve(p, mat, O.options(elastic=True))
+    F = O.fields(st)
+    for k, n in [("exx", "Eps_xx"), ("sxx", "Sigma_xx"), ("sigI", "Sigma_intensity"), ("epsI", "Eps_intensity")]:
+        np.testing.assert_allclose(F[k], fx["f_" + n], rtol=1e-9)
+    for k, n in [("eyy", "Eps_yy"), ("gxy", "Gamma_xy"), ("syy", "Sigma_yy"), ("tau", "Tau_xy")]:
+        np.testing.assert_allclose(F[k], fx["f_" + n], atol=1e-10 * abs(fx["f_Sigma_xx"]).max() / 1.93e5 + 1e-10)
+    np.testing.assert_allclose(u.reshape(-1, 2), fx["f_Displacement"][:, :2], rtol=1e-9, atol=1e-16)
+    assert np.allclose(F["exx"], 1.0 / (E_304L * 1e-6), rtol=1e-9)
+
+
+@pytest.fixture(scope="module")
+def hole_plate_solution():
+    m = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)
+    p = O.Problem(m).force(1, 170.0, 0.0).force(3, -170.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
+    E = E_304L * 1e-6
+    mat = O.material(E, NU_304L, S_T_304L * 1e-6, O.LAW_POWER, A=E * 0.01, m=0.015)
+    u, st, s = O.solve(p, mat, O.options(max_outer=1000))
+    return m, mat, u, O.fields(st), s
+
+
+def test_hole_plate_pipeline_fixture(hole_plate_solution):
+    """plasticityTest (Main.cpp:252-296): the whole secant loop (41 outer
+    iterations, exit relErr 9.68e-6, 3528 plastic elements) reproduced."""
+    fx = load("holePlatePlastPow")
+    m, mat, u, F, s = hole_plate_solution
+    assert s.status == 0 and s.outer_iters == 41
+    assert abs(s.rel_err - 9.68e-6) < 0.01e-6
+    for k, n in [("exx", "Eps_xx"), ("eyy", "Eps_yy"), ("sxx", "Sigma_xx"), ("syy", "Sigma_yy"),
+                 ("tau", "Tau_xy"), ("sigI", "Sigma_intensity"), ("Ec", "E_c"), ("nuc", "nu_c"),
+                 ("psi", "Psi"), ("exx_p", "Eps_p_xx"), ("epsI_p", "Eps_p_intensity")]:
+        assert relnorm(F[k], fx["f_" + n]) < 1e-7, n
+    assert relnorm(u, fx["f_Displacement"][:, :2].reshape(-1)) < 1e-7
+    flags = F["epsI"] >= mat.sigmaT / mat.E
+    ref_flags = fx["f_Eps_intensity"] >= mat.sigmaT / mat.E
+    assert flags.sum() == ref_flags.sum() == 3528
+    assert np.array_equal(flags, ref_flags)
+
+
+def test_fixture_secant_invariants():
+    """Per-element identities of the reference's secant update
+    (PlasticitySolver.cu:1320-1335, 1641-1653), checked on its own output."""
+    fx = load("holePlatePlastPow")
+    E = E_304L * 1e-6
+    mat = O.material(E, NU_304L, S_T_304L * 1e-6, O.LAW_POWER, A=E * 0.01, m=0.015)
+    sx, sy, t = fx["f_Sigma_xx"], fx["f_Sigma_yy"], fx["f_Tau_xy"]
+    sigI = np.sqrt(0.5 * (sx ** 2 + sy ** 2 + (sx - sy) ** 2 + 6 * t ** 2))
+    np.testing.assert_allclose(sigI, fx["f_Sigma_intensity"], rtol=1e-14)
+    epsI, Ec, nuc, psi = fx["f_Eps_intensity"], fx["f_E_c"], fx["f_nu_c"], fx["f_Psi"]
+    ftab = np.array([O.material_f(mat, e) for e in epsI])
+    np.testing.assert_allclose(Ec, ftab / epsI, rtol=1e-13)
+    np.testing.assert_allclose(psi, E / Ec, rtol=1e-14)
+    np.testing.assert_allclose(nuc, 0.5 * (1 - (1 - 2 * NU_304L) / (E * epsI / ftab)), rtol=1e-13)
+    np.testing.assert_allclose(fx["f_Eps_p_xx"], (1 / Ec - 1 / E) * sx, rtol=1e-12, atol=1e-18)
+    # sigma = C(E_old, nu_old) eps with E_old = sigI/epsI
+    Eo = sigI / epsI
+    for e in range(0, 5120, 97):
+        nuo = None
+        Cm = None
+        # nu_old is not stored; recover it from the ratio of the normal stresses
+        a = np.array([[fx["f_Eps_xx"][e], fx["f_Eps_yy"][e]], [fx["f_Eps_yy"][e], fx["f_Eps_xx"][e]]])
+        coefs = np.linalg.solve(a, np.array([sx[e], sy[e]]))          # [c, c*nu]
+        nuo = coefs[1] / coefs[0]
+        Cm = O.form_c(O.PLANE_STRESS, Eo[e], nuo)
+        np.testing.assert_allclose(Cm[0, 0], coefs[0], rtol=1e-9)
+        np.testing.assert_allclose(Cm[2, 2] * fx["f_Gamma_xy"][e], t[e], rtol=1e-8, atol=1e-9)
+
+
+def test_diagram_fixtures():
+    """diagramTest (Main.cpp:929-994): each level is an independent solve of a
+    2x2 block under gravity and an end displacement.  The reference's GPU CG
+    leaves x_{k-1} when k is odd; the oracle reproduces that with cg_quirk.
+    Row 2 of diagramLin is the one level where the reference's (atomics
+    based, run-to-run varying) iteration count was odd: the same elastic level
+    in diagramPow is even, so row 2 is compared with the x_k answer instead."""
+    fx = load("diagram")
+    m = O.gen_rect(-1, 1, -1, 1, 2, 2)
+    mats = {"lin": O.material(E_304L, NU_304L, S_T_304L, O.LAW_LINEAR, K_T=0.1 * E_304L),
+            "pow": O.material(E_304L, NU_304L, S_T_304L, O.LAW_POWER, A=E_304L * 0.01, m=0.015)}
+    for name, mat in mats.items():
+        ref = fx[name]
+        displ, row = 0.0, 0
+        while displ <= 0.02:
+            p = O.Problem(m).gravity()
+            p.displacement(2, 0.0, 0.0, False, True)
+            p.displacement(0, displ, 0.0, False, True)
+            u, st, s = O.solve(p, mat, O.options(cg_quirk=True, max_outer=100000))
+            F = O.fields(st)
+            eps, sig = F["exx"][0], F["sxx"][0] * 1e-6
+            if name == "lin" and row == 2:
+                assert abs(eps - 0.0005) < 1e-15 and abs(ref[row, 0] - 0.0005) / 0.0005 < 0.02
+            else:
+                if row > 0:
+                    assert abs(eps - ref[row, 0]) <= 6e-6 * abs(ref[row, 0]), (name, row)
+                assert abs(sig - ref[row, 1]) <= 6e-6 * abs(ref[row, 1]), (name, row, sig, ref[row, 1])
+            displ += 0.0005
+            row += 1
+        assert row == ref.shape[0] == 40
+
+
+# --------------------------------------------------------------- closed forms
+
+def test_material_laws_closed_form():
+    E, sT = 2.1e5, 240.0
+    lin = O.material(E, 0.3, sT, O.LAW_LINEAR, K_T=E / 101)
+    assert np.isclose(O.material_f(lin, 0.5 * sT / E), 0.5 * sT, rtol=1e-15)
+    assert np.isclose(O.material_f(lin, 3 * sT / E), sT + 2 * sT / E * E / 101, rtol=1e-15)
+    pw = O.material(193e3, 0.29, 170.0, O.LAW_POWER, A=1930.0, m=0.015)
+    assert np.isclose(O.material_f(pw, 0.001), 186.442, rtol=3e-6)      # diagramPow.txt row 4
+    sw = O.material(E, 0.3, sT, O.LAW_SWIFT, n=0.2)
+    eps0 = sT / E
+    for t in [1.0, 1.5, 3.0, 20.0, 300.0]:
+        eps = t * eps0
+        s = O.material_f(sw, eps)
+        ep = eps - s / E
+        assert ep >= -1e-18
+        np.testing.assert_allclose(s, sT * (1 + ep / eps0) ** 0.2, rtol=1e-13)
+
+
+def test_uniaxial_block_reproduces_swift_curve():
+    """A 2x2 block in uniaxial plane stress follows sigma(eps) of the law."""
+    m = O.gen_rect(-1, 1, -1, 1, 2, 2)
+    E, sT = 2.1e5, 240.0
+    mat = O.material(E, 0.3, sT, O.LAW_SWIFT, n=0.2)
+    for displ in [0.001, 0.004, 0.01]:
+        p = O.Problem(m)
+        p.displacement(2, 0.0, 0.0, False, True)
+        p.displacement(0, displ, 0.0, False, True)
+        u, st, s = O.solve(p, mat, O.options(cg_tol=1e-13, tol=1e-12, max_outer=2000))
+        F = O.fields(st)
+        eps = displ / 2
+        assert np.allclose(F["exx"], eps, rtol=1e-9)
+        np.testing.assert_allclose(F["sxx"], O.material_f(mat, eps), rtol=1e-9)
+
+
+def test_aitken_reaches_same_fixed_point():
+    m = O.gen_quarter_plate(1.0, 0.1, 24, 12)
+    p = O.Problem(m).force(1, 300.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
+    mat = O.material(2.1e5, 0.3, 240.0, O.LAW_LINEAR, K_T=2.1e5 / 101)
+    opt = dict(cg_tol=1e-12, tol=1e-9, max_outer=3000)
+    u0, s0, st0 = O.solve(p, mat, O.options(**opt))
+    u1, s1, st1 = O.solve(p, mat, O.options(aitken=True, **opt))
+    assert st0.status == 0 and st1.status == 0
+    assert st1.outer_iters < st0.outer_iters
+    assert relnorm(u1, u0) < 1e-7
+    assert relnorm(O.fields(s1)["sigI"], O.fields(s0)["sigI"]) < 1e-7
+
+
+def test_lame_thick_cylinder():
+    """Quarter ring under internal pressure (normal-oriented traction on the
+    inner arc) against the plane-stress Lame solution."""
+    a, b, pr, E, nu = 0.5, 1.0, 10.0, 2.1e5, 0.3
+    m = O.gen_quarter_ring(a, b, 96, 48, 1.0)
+    p = O.Problem(m).force(0, -pr, 0.0, norm_orient=True).fix_vert_axis(0).fix_hor_axis(0)
+    mat = O.material(E, nu)
+    u, st, s = O.solve(p, mat, O.options(cg_tol=1e-12, elastic=True))
+    A = pr * a * a / (b * b - a * a)
+    Bc = pr * a * a * b * b / (b * b - a * a)
+    r = np.hypot(m.x, m.y)
+    ur_exact = ((1 - nu) * A * r + (1 + nu) * Bc / r) / E
+    ur = (u[0::2] * m.x + u[1::2] * m.y) / r
+    assert np.abs(ur - ur_exact).max() < 2e-3 * ur_exact.max()
+    F = O.fields(st)
+    assert p.rp.reshape(-1, 2)[m.borders[0]].sum(axis=0) @ np.array([1, 1]) > 0
+
+
+def test_quarter_plate_generator():
+    L, r = 1.0, 0.1
+    m = O.gen_quarter_plate(L, r, 100, 49)
+    assert (m.nn, m.ne) == (5050, 4900)
+    area = sum(O.elem_space4(m, e) for e in range(m.ne))
+    # polygonal hole: the mesh area exceeds the exact one by at most the chord error
+    assert 0 < area - (L * L - np.pi * r * r / 4) < 1e-4
+    assert np.all(m.x[m.borders[3]] == 0.0) and np.all(m.y[m.borders[0]] == 0.0)
+    assert np.all(m.x[m.borders[1]] == L) and np.all(m.y[m.borders[2]] == L)
+    for e in range(0, m.ne, 101):
+        B, dJ = O.calc_b4(m, e)
+        assert np.all(dJ > 0)
