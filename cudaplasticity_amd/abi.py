@@ -1,0 +1,364 @@
+"""ctypes binding of the C-ABI in include/cp_abi.h (libcp_hip.so).
+
+The library is the product: every call here goes straight to the gfx950
+kernels or to the library's own host-side setup code.  Loading fails loudly
+when the library has not been built; there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "libcp_hip.so")
+HEADER = os.path.join(ROOT, "include", "cp_abi.h")
+
+CP_OK, CP_ERR_INVALID, CP_ERR_NOMEM, CP_ERR_DEVICE, CP_ERR_NOT_CONVERGED, CP_ERR_NAN, CP_ERR_NO_DEVICE, \
+    CP_ERR_BUDGET, CP_ERR_STATE = range(9)
+LAW_LINEAR, LAW_POWER, LAW_SWIFT = 0, 1, 2
+PLANE_STRESS, PLANE_STRAIN = 0, 1
+FP64, MIXED = 0, 1
+FIELDS = ["exx", "eyy", "gxy", "epsI", "sxx", "syy", "tau", "sigI", "sigTab", "psi", "Ec", "nuc",
+          "exx_p", "eyy_p", "gxy_p", "epsI_p"]
+F_U, F_RHS = 100, 101
+
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+_U8 = C.POINTER(C.c_ubyte)
+
+
+class Material(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("h", C.c_double), ("sigmaT", C.c_double),
+                ("law", C.c_int), ("plane", C.c_int), ("K_T", C.c_double), ("A", C.c_double),
+                ("m", C.c_double), ("n", C.c_double)]
+
+
+class Options(C.Structure):
+    _fields_ = [("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("tol", C.c_double), ("max_outer", C.c_int),
+                ("fixed_outer", C.c_int), ("aitken", C.c_int), ("precision", C.c_int), ("warm", C.c_int),
+                ("inner_tol", C.c_double), ("reserved", C.c_int * 4)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("status", C.c_int), ("cg_iters", C.c_longlong),
+                ("refinements", C.c_longlong), ("rel_err", C.c_double), ("seconds", C.c_double),
+                ("cg_seconds", C.c_double), ("asm_seconds", C.c_double), ("rm_seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("batch_cg", C.c_int),
+                ("reserved", C.c_int * 5)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("status", C.c_int), ("steps_done", C.c_int), ("cg_iters", C.c_longlong),
+                ("outer_iters", C.c_int), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double)]
+
+
+class CPError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() (no CPU fallback exists)")
+    lib = C.CDLL(path)
+    lib.cp_last_error.restype = C.c_char_p
+    lib.cp_abi_version.restype = C.c_int
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = load_library()
+    return _lib
+
+
+def check(code):
+    if code != CP_OK:
+        raise CPError(code, lib().cp_last_error().decode())
+    return code
+
+
+def dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def u8p(a):
+    return a.ctypes.data_as(_U8)
+
+
+def material(E, nu, sigmaT=1e300, law=LAW_LINEAR, K_T=None, A=0.0, m=0.0, n=0.0, h=1.0, plane=PLANE_STRESS):
+    return Material(E, nu, h, sigmaT, law, plane, E if K_T is None else K_T, A, m, n)
+
+
+def options(cg_tol=1e-7, cg_maxit=0, tol=1e-5, max_outer=1000, fixed_outer=0, aitken=False, precision=FP64,
+            warm=False, inner_tol=1e-3):
+    o = Options()
+    o.cg_tol, o.cg_maxit, o.tol, o.max_outer, o.fixed_outer = cg_tol, cg_maxit, tol, max_outer, fixed_outer
+    o.aitken, o.precision, o.warm, o.inner_tol = int(aitken), precision, int(warm), inner_tol
+    return o
+
+
+# ------------------------------------------------------------------ host-side setup (library code)
+
+class Mesh:
+    """Node coordinates, connectivity (ne x 4, reference local order) and borders."""
+
+    def __init__(self, x, y, conn, borders):
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.y = np.ascontiguousarray(y, dtype=np.float64)
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 4)
+        self.borders = [np.ascontiguousarray(b, dtype=np.int32) for b in borders]
+
+    @property
+    def nn(self):
+        return self.x.size
+
+    @property
+    def ne(self):
+        return self.conn.shape[0]
+
+
+def _gen(sizes_name, sizes_args, gen_name, gen_args):
+    L = lib()
+    nn, ne, nb = C.c_int(), C.c_int(), C.c_int()
+    blen = np.zeros(8, dtype=np.int32)
+    check(getattr(L, sizes_name)(*sizes_args, C.byref(nn), C.byref(ne), C.byref(nb), ip(blen)))
+    x, y = np.zeros(nn.value), np.zeros(nn.value)
+    conn = np.zeros(4 * ne.value, dtype=np.int32)
+    bord = np.zeros(int(blen[:nb.value].sum()), dtype=np.int32)
+    check(getattr(L, gen_name)(*gen_args, dp(x), dp(y), ip(conn), ip(bord), ip(blen)))
+    borders, off = [], 0
+    for b in range(nb.value):
+        borders.append(bord[off:off + blen[b]].copy())
+        off += blen[b]
+    return Mesh(x, y, conn, borders)
+
+
+_f = C.c_double
+
+
+def gen_rect(x1, x2, y1, y2, N1, N2):
+    """Mesh::genRectangle (Mesh.cu:17-109)."""
+    return _gen("cp_mesh_rect_sizes", (N1, N2), "cp_mesh_rect", (_f(x1), _f(x2), _f(y1), _f(y2), N1, N2))
+
+
+def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2):
+    """Mesh::genRectWithHole (Mesh.cu:112-245)."""
+    a = (_f(x1), _f(x2), _f(y1), _f(y2), _f(r), N1, N2)
+    return _gen("cp_mesh_hole_sizes", a, "cp_mesh_hole", a)
+
+
+def gen_ring(a, b, Nphi, Nr):
+    """Mesh::genRing (Mesh.cu:249-329)."""
+    return _gen("cp_mesh_ring_sizes", (Nphi, Nr), "cp_mesh_ring", (_f(a), _f(b), Nphi, Nr))
+
+
+def gen_quarter_plate(L, r, Nt, Nr):
+    return _gen("cp_mesh_quarter_plate_sizes", (Nt, Nr), "cp_mesh_quarter_plate", (_f(L), _f(r), Nt, Nr))
+
+
+def gen_quarter_ring(a, b, Nt, Nr, ratio):
+    return _gen("cp_mesh_quarter_ring_sizes", (Nt, Nr), "cp_mesh_quarter_ring", (_f(a), _f(b), Nt, Nr, _f(ratio)))
+
+
+def csr_pattern(mesh):
+    nnz = C.c_longlong()
+    check(lib().cp_csr_nnz(mesh.nn, mesh.ne, ip(mesh.conn), C.byref(nnz)))
+    rows = np.zeros(2 * mesh.nn + 1, dtype=np.int32)
+    cols = np.zeros(nnz.value, dtype=np.int32)
+    check(lib().cp_csr_pattern(mesh.nn, mesh.ne, ip(mesh.conn), ip(rows), ip(cols)))
+    return rows, cols
+
+
+class Loads:
+    """Load vector (at load factor 1), Dirichlet mask and prescribed values,
+    built by the library's initConditions rules (PlasticitySolver.cu:1007-1216)."""
+
+    def __init__(self, mesh, h=1.0):
+        self.mesh, self.h = mesh, h
+        self.rp = np.zeros(2 * mesh.nn)
+        self.mask = np.ones(2 * mesh.nn, dtype=np.uint8)
+        self.u = np.zeros(2 * mesh.nn)
+
+    def force(self, border_id, fx, fy, norm_orient=False):
+        b = self.mesh.borders[border_id]
+        fv = np.tile(np.array([fx, fy], dtype=np.float64), b.size)
+        return self.force_values(border_id, fv, norm_orient)
+
+    def force_values(self, border_id, fv, norm_orient=False):
+        b = self.mesh.borders[border_id]
+        fv = np.ascontiguousarray(fv, dtype=np.float64).reshape(-1)
+        check(lib().cp_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
+                                        int(norm_orient), _f(self.h), dp(self.rp)))
+        return self
+
+    def gravity(self, g=9.81, rho=1.0):
+        check(lib().cp_rhs_body_force(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
+                                      _f(0.0), _f(-g), _f(rho), _f(self.h), dp(self.rp)))
+        return self
+
+    def displacement(self, border_id, ux, uy, fix_mid_x=False, fix_mid_y=False):
+        b = self.mesh.borders[border_id]
+        check(lib().cp_displacement_border(ip(b), b.size, _f(ux), _f(uy), int(fix_mid_x), int(fix_mid_y),
+                                           u8p(self.mask), dp(self.u)))
+        return self
+
+    def fix_vert_axis(self, x):
+        check(lib().cp_fix_axis(self.mesh.nn, dp(self.mesh.x), dp(self.mesh.y), 1, _f(x), u8p(self.mask)))
+        return self
+
+    def fix_hor_axis(self, y):
+        check(lib().cp_fix_axis(self.mesh.nn, dp(self.mesh.x), dp(self.mesh.y), 0, _f(y), u8p(self.mask)))
+        return self
+
+
+# ------------------------------------------------------------------ device objects
+
+class Context:
+    def __init__(self, device=0):
+        self.ptr = C.c_void_p()
+        check(lib().cp_ctx_create(device, C.byref(self.ptr)))
+
+    def info(self):
+        name = C.create_string_buffer(256)
+        mem, cus = C.c_size_t(), C.c_int()
+        check(lib().cp_device_info(self.ptr, name, 256, C.byref(mem), C.byref(cus)))
+        return {"name": name.value.decode(), "total_mem": mem.value, "cu_count": cus.value}
+
+    def close(self):
+        if self.ptr:
+            lib().cp_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+class Problem:
+    """One device-resident problem (PlasticitySolver construction)."""
+
+    def __init__(self, ctx, mesh, loads, mat):
+        self.mesh, self.mat = mesh, mat
+        self.ptr = C.c_void_p()
+        check(lib().cp_problem_create(ctx.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
+                                      dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), C.byref(self.ptr)))
+
+    def close(self):
+        if self.ptr:
+            lib().cp_problem_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def csr(self):
+        nnz = C.c_longlong()
+        check(lib().cp_problem_csr_nnz(self.ptr, C.byref(nnz)))
+        rows = np.zeros(2 * self.mesh.nn + 1, dtype=np.int32)
+        cols = np.zeros(nnz.value, dtype=np.int32)
+        check(lib().cp_problem_get_csr(self.ptr, ip(rows), ip(cols)))
+        return rows, cols
+
+    def values(self):
+        nnz = C.c_longlong()
+        check(lib().cp_problem_csr_nnz(self.ptr, C.byref(nnz)))
+        v = np.zeros(nnz.value)
+        check(lib().cp_problem_get_values(self.ptr, dp(v)))
+        return v
+
+    def layout(self):
+        a, b, w = C.c_longlong(), C.c_longlong(), C.c_int()
+        check(lib().cp_problem_layout(self.ptr, C.byref(a), C.byref(b), C.byref(w)))
+        return {"nblk_stored": a.value, "nblk": b.value, "max_width": w.value}
+
+    def set_load(self, factor):
+        check(lib().cp_set_load(self.ptr, _f(factor)))
+
+    def reset(self):
+        check(lib().cp_reset_state(self.ptr))
+
+    def assemble(self, precision=FP64):
+        check(lib().cp_assemble(self.ptr, precision))
+
+    def cg(self, tol=1e-7, maxit=0, precision=FP64, inner_tol=1e-3):
+        it, rr = C.c_longlong(), C.c_double()
+        check(lib().cp_cg(self.ptr, _f(tol), maxit, precision, _f(inner_tol), C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
+    def return_map(self):
+        r = C.c_double()
+        check(lib().cp_return_map(self.ptr, C.byref(r)))
+        return r.value
+
+    def solve(self, opt=None, elastic=False, allow=(CP_OK,)):
+        st = Stats()
+        fn = lib().cp_solve_elastic if elastic else lib().cp_solve
+        code = fn(self.ptr, C.byref(opt) if opt is not None else None, C.byref(st))
+        if code not in allow:
+            check(code)
+        return st
+
+    def solve_steps(self, nsteps, final_factor=1.0, opt=None):
+        per = (Stats * nsteps)()
+        check(lib().cp_solve_steps(self.ptr, nsteps, _f(final_factor), C.byref(opt) if opt is not None else None,
+                                   per))
+        return [s for s in per]
+
+    def field(self, name):
+        if name == "u":
+            out = np.zeros(2 * self.mesh.nn)
+            check(lib().cp_download(self.ptr, F_U, dp(out)))
+            return out
+        if name == "rhs":
+            out = np.zeros(2 * self.mesh.nn)
+            check(lib().cp_download(self.ptr, F_RHS, dp(out)))
+            return out
+        out = np.zeros(self.mesh.ne)
+        check(lib().cp_download(self.ptr, FIELDS.index(name), dp(out)))
+        return out
+
+    def fields(self):
+        return {n: self.field(n) for n in FIELDS}
+
+    def upload_state(self, u=None, psi=None):
+        check(lib().cp_upload_state(self.ptr, dp(np.ascontiguousarray(u, dtype=np.float64)) if u is not None else None,
+                                    dp(np.ascontiguousarray(psi, dtype=np.float64)) if psi is not None else None))
+
+    def internal_force(self):
+        f = np.zeros(2 * self.mesh.nn)
+        check(lib().cp_internal_force(self.ptr, dp(f)))
+        return f
+
+    def last_cg_ms_per_iter(self):
+        ms = C.c_double()
+        check(lib().cp_last_cg_kernel_ms(self.ptr, C.byref(ms)))
+        return ms.value
+
+    def bench_spmv(self, precision=FP64, reps=50):
+        ms, bs, ba = C.c_double(), C.c_double(), C.c_double()
+        check(lib().cp_bench_spmv(self.ptr, precision, reps, C.byref(ms), C.byref(bs), C.byref(ba)))
+        return ms.value, bs.value, ba.value
+
+
+def declared_functions(header=HEADER):
+    """Names of every function declared in include/cp_abi.h."""
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(cp_\w+)\s*\(", txt, re.M)))

@@ -1,0 +1,804 @@
+// cp_kernels.h -- gfx950 device code of the plasticity hot path.
+//
+// Layout in HBM (one problem):
+//   xy[nn]      double2  node coordinates
+//   conn[ne]    int4     element nodes, reference local order
+//   ecnu[ne]    double2  secant modulus E_c and Poisson ratio nu_c per element
+//   matrix      2x2-block sliced ELL: slices of 64 consecutive nodes (one
+//               wave each); slice s holds width_s blocks per node stored
+//               block-major (offset_s + k*64 + lane), so a wave's loads of
+//               block k are contiguous.  col[] = neighbour node, values as
+//               two double2 planes (a00,a01 | a10,a11) in FP64 or one float4
+//               plane in FP32.  Row order inside a node = reference CSR
+//               column order (self first, then first appearance).
+//   inc[]       int2     node -> (element, packed slot bytes k0..k3): where
+//               block K_e(i, j) lands in the node's row; assembly gathers
+//               per row in element order, so sums are deterministic.
+//   vectors     one double2 (or float2) per node: DOFs (2n, 2n+1).
+//
+// Reductions are deterministic: wave64 butterflies, a fixed-order LDS step
+// and a "last block" that sums the per-block partials in block order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+namespace cpk {
+
+constexpr int kBlock = 256;  // 4 waves of 64
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- helpers
+
+// Workgroups are dispatched round-robin over the 8 XCDs; map them so that
+// each XCD (and its L2) works on one contiguous range of nodes.
+__device__ __forceinline__ int xcd_block(int nblocks)
+{
+    const int b = blockIdx.x;
+    const int q = nblocks >> 3, rem = nblocks & 7;
+    const int x = b & 7, k = b >> 3;
+    return x * q + (x < rem ? x : rem) + k;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Block-wide sum of K values; thread 0 returns the totals.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double* sh)
+{
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = wave_sum(v[k]);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) sh[wave * K + k] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = ((sh[k] + sh[K + k]) + sh[2 * K + k]) + sh[3 * K + k];
+    __syncthreads();
+}
+
+// Grid-wide deterministic sum: every block publishes its partial, the last
+// block to arrive sums all partials in block order.  Returns true in the
+// last block; thread 0 of it holds the totals in v.
+template <int K>
+__device__ __forceinline__ bool grid_sum(double (&v)[K], double* partials, unsigned* counter, int nblocks, int lb)
+{
+    __shared__ double sh[4 * K];
+    __shared__ int s_last;
+    block_sum<K>(v, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) partials[(size_t)lb * K + k] = v[k];
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        s_last = (t == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] = 0.;
+    for (int j = threadIdx.x; j < nblocks; j += kBlock)
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)j * K + k];
+    block_sum<K>(acc, sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) v[k] = acc[k];
+        *counter = 0u;
+    }
+    return true;
+}
+
+template <typename T> struct V2;
+template <> struct V2<double> { using type = double2; };
+template <> struct V2<float> { using type = float2; };
+
+// ---------------------------------------------------------------- material
+
+struct MatParams {
+    double E, nu, h, sigmaT;
+    int law, plane;
+    double K_T, A, m, n;
+};
+
+// Material.cu:3-23 (linear / power); Swift law solved by Newton for sigma.
+__device__ __forceinline__ double mat_f(const MatParams& M, double eps)
+{
+    const double epsT = M.sigmaT / M.E;
+    if (eps < epsT) return eps * M.E;
+    if (M.law == 0) return M.sigmaT + (eps - epsT) * M.K_T;
+    if (M.law == 1)
+        return M.sigmaT + M.A * pow(eps - epsT + pow(M.E / (M.A * M.m), 1. / (M.m - 1.)), M.m) -
+               M.A * pow(M.E / (M.A * M.m), M.m / (M.m - 1.));
+    const double t = M.E * eps / M.sigmaT;
+    double s = 1.0;
+    for (int it = 0; it < 60; ++it) {
+        const double w = 1.0 + t - s;
+        const double pw = pow(w, M.n);
+        const double g = s - pw;
+        const double dg = 1.0 + M.n * pw / w;
+        double sn = s - g / dg;
+        sn = sn < 1.0 ? 1.0 : (sn > t ? t : sn);
+        const double d = fabs(sn - s);
+        s = sn;
+        if (d <= 1e-15 * s) break;
+    }
+    return M.sigmaT * s;
+}
+
+// formC (PlasticitySolver.cu:226-242): returns C00 (= C11), C01, C22.
+__device__ __forceinline__ void elastic_c(int plane, double E, double nu, double& c00, double& c01, double& c22)
+{
+    if (plane == 0) {
+        const double coef = E / (1. - nu * nu);
+        c00 = coef;
+        c01 = nu * coef;
+        c22 = (1. - nu) * 0.5 * coef;
+    } else {
+        const double coef = E * (1. - nu) / ((1. + nu) * (1 - 2. * nu));
+        c00 = coef;
+        c01 = nu / (1. - nu) * coef;
+        c22 = (1. - 2. * nu) * 0.5 / (1. - nu) * coef;
+    }
+}
+
+// calcB4 (PlasticitySolver.cu:25-64) at Gauss point g = 2i + j,
+// (r, s) = (p_i, p_j), p = -+1/sqrt(3): B0 = dN/dx, B1 = dN/dy, detJ.
+__device__ __forceinline__ void shape_grad(const double (&x)[4], const double (&y)[4], int g, double (&B0)[4],
+                                           double (&B1)[4], double& detJ)
+{
+    const double P = 0.577350269189626;
+    const double r = (g & 2) ? P : -P;
+    const double s = (g & 1) ? P : -P;
+    const double J0 = 0.25 * ((1. + s) * x[0] - (1. + s) * x[1] - (1. - s) * x[2] + (1. - s) * x[3]);
+    const double J2 = 0.25 * ((1. + r) * x[0] + (1. - r) * x[1] - (1. - r) * x[2] - (1. + r) * x[3]);
+    const double J1 = 0.25 * ((1. + s) * y[0] - (1. + s) * y[1] - (1. - s) * y[2] + (1. - s) * y[3]);
+    const double J3 = 0.25 * ((1. + r) * y[0] + (1. - r) * y[1] - (1. - r) * y[2] - (1. + r) * y[3]);
+    detJ = J0 * J3 - J1 * J2;
+    const double inv = 1. / detJ;
+    const double i0 = J3 * inv, i1 = -J1 * inv, i2 = -J2 * inv, i3 = J0 * inv;
+    B0[0] = 0.25 * ((1. + s) * i0 + (1. + r) * i1);
+    B1[0] = 0.25 * ((1. + s) * i2 + (1. + r) * i3);
+    B0[1] = 0.25 * ((-1. - s) * i0 + (1. - r) * i1);
+    B1[1] = 0.25 * ((-1. - s) * i2 + (1. - r) * i3);
+    B0[2] = 0.25 * ((-1. + s) * i0 - (1. - r) * i1);
+    B1[2] = 0.25 * ((-1. + s) * i2 - (1. - r) * i3);
+    B0[3] = 0.25 * ((1. - s) * i0 - (1. + r) * i1);
+    B1[3] = 0.25 * ((1. - s) * i2 - (1. + r) * i3);
+}
+
+__device__ __forceinline__ double pick4(const double (&v)[4], int i)
+{
+    return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
+// ---------------------------------------------------------------- assembly
+
+struct AsmArgs {
+    const int* incPtr;
+    const int2* inc;
+    const int4* conn;
+    const double2* xy;
+    const double2* ecnu;
+    const int2* slice;  // {offset in blocks, width}
+    double2* va;        // FP64 plane (a00, a01)
+    double2* vb;        // FP64 plane (a10, a11)
+    float4* vf;         // FP32 values
+    int nn, nblocks, plane;
+    double h;
+};
+
+// Element stiffness (formKe4, PlasticitySolver.cu:156-186) gathered per
+// node: thread n computes, for every element e containing n (local index i),
+// the 2x2 blocks K_e(i, j), j = 0..3, and adds them to the slots of its
+// row; slot order and element order are fixed, so the sums are
+// deterministic.  W = compile-time bound on the row width.
+template <int W, bool F64>
+__global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
+{
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    double acc[W][4];
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.;
+
+    for (int q = a.incPtr[n]; q < a.incPtr[n + 1]; ++q) {
+        const int2 ic = a.inc[q];
+        const int4 c = a.conn[ic.x];
+        const double2 en = a.ecnu[ic.x];
+        const unsigned slots = (unsigned)ic.y;
+        const int sl[4] = {(int)(slots & 255u), (int)((slots >> 8) & 255u), (int)((slots >> 16) & 255u),
+                           (int)(slots >> 24)};
+        const int i = sl[0] == 0 ? 0 : (sl[1] == 0 ? 1 : (sl[2] == 0 ? 2 : 3));
+        double x[4], y[4];
+        {
+            const double2 p0 = a.xy[c.x], p1 = a.xy[c.y], p2 = a.xy[c.z], p3 = a.xy[c.w];
+            x[0] = p0.x; y[0] = p0.y; x[1] = p1.x; y[1] = p1.y;
+            x[2] = p2.x; y[2] = p2.y; x[3] = p3.x; y[3] = p3.y;
+        }
+        double c00, c01, c22;
+        elastic_c(a.plane, en.x, en.y, c00, c01, c22);
+        double blk[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) blk[j][0] = blk[j][1] = blk[j][2] = blk[j][3] = 0.;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            double B0[4], B1[4], dJ;
+            shape_grad(x, y, g, B0, B1, dJ);
+            const double bi0 = pick4(B0, i), bi1 = pick4(B1, i);
+            // BC = B_i^T C detJ (rows 0 and 1 of B^T C for node i)
+            const double t00 = bi0 * c00 * dJ, t01 = bi0 * c01 * dJ, t02 = bi1 * c22 * dJ;
+            const double t10 = bi1 * c01 * dJ, t11 = bi1 * c00 * dJ, t12 = bi0 * c22 * dJ;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                blk[j][0] += t00 * B0[j] + t02 * B1[j];
+                blk[j][1] += t01 * B1[j] + t02 * B0[j];
+                blk[j][2] += t10 * B0[j] + t12 * B1[j];
+                blk[j][3] += t11 * B1[j] + t12 * B0[j];
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+#pragma unroll
+            for (int k = 0; k < W; ++k) {
+                if (k == sl[j]) {
+                    acc[k][0] += blk[j][0] * a.h;
+                    acc[k][1] += blk[j][1] * a.h;
+                    acc[k][2] += blk[j][2] * a.h;
+                    acc[k][3] += blk[j][3] * a.h;
+                }
+            }
+        }
+    }
+    const int2 sl = a.slice[n >> 6];
+    const size_t base = (size_t)sl.x + (n & 63);
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        if (k < sl.y) {
+            const size_t q = base + (size_t)k * kWave;
+            if (F64) {
+                a.va[q] = make_double2(acc[k][0], acc[k][1]);
+                a.vb[q] = make_double2(acc[k][2], acc[k][3]);
+            } else {
+                a.vf[q] = make_float4((float)acc[k][0], (float)acc[k][1], (float)acc[k][2], (float)acc[k][3]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- CG
+
+struct CgScal {
+    double alpha, beta, gamma, delta;
+    double bnorm2, tol2;
+    double gamma0;
+    int iters, done, status, maxit;
+    unsigned counter, pad;
+};
+
+template <typename T>
+struct CgArgs {
+    using V = typename V2<T>::type;
+    const int* col;
+    const int2* slice;
+    const void* va;  // FP64: double2 (a00,a01); FP32: float4
+    const void* vb;  // FP64: double2 (a10,a11)
+    V* x;
+    V* p;
+    V* r[2];
+    V* w[2];
+    V* s[2];
+    const V* b;
+    const uchar2* mask;
+    CgScal* sc;
+    double* partials;
+    int nn, nblocks;
+};
+
+// Matrix values are streamed once per iteration: non-temporal loads keep
+// them from evicting the gathered vectors from L2.
+typedef double nd2 __attribute__((ext_vector_type(2)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void load_block(const CgArgs<double>& a, size_t q, double& m00, double& m01, double& m10,
+                                           double& m11)
+{
+    const nd2 u = __builtin_nontemporal_load(reinterpret_cast<const nd2*>(a.va) + q);
+    const nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2*>(a.vb) + q);
+    m00 = u.x; m01 = u.y; m10 = v.x; m11 = v.y;
+}
+
+__device__ __forceinline__ void load_block(const CgArgs<float>& a, size_t q, float& m00, float& m01, float& m10,
+                                           float& m11)
+{
+    const nf4 u = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(a.va) + q);
+    m00 = u.x; m01 = u.y; m10 = u.z; m11 = u.w;
+}
+
+// r_0 = mask (b - A x), p_{-1} = 0, s_{-1} = 0  (cgInitSparseMask, ConjGradCuda.cu:15-37).
+// zero_x: x = 0 and the SpMV is skipped (inner solve of the refinement loop).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init1(CgArgs<T> a, int zero_x)
+{
+    using V = typename V2<T>::type;
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    T yx = 0, yy = 0;
+    if (zero_x) {
+        a.x[n] = V{0, 0};
+    } else {
+        const int2 sl = a.slice[n >> 6];
+        size_t q = (size_t)sl.x + (n & 63);
+        for (int k = 0; k < sl.y; ++k, q += kWave) {
+            const V xm = a.x[a.col[q]];
+            T m00, m01, m10, m11;
+            load_block(a, q, m00, m01, m10, m11);
+            yx += m00 * xm.x + m01 * xm.y;
+            yy += m10 * xm.x + m11 * xm.y;
+        }
+    }
+    const V bn = a.b[n];
+    const uchar2 mk = a.mask[n];
+    a.r[0][n] = V{mk.x ? (T)(bn.x - yx) : (T)0, mk.y ? (T)(bn.y - yy) : (T)0};
+    a.p[n] = V{0, 0};
+    a.s[1][n] = V{0, 0};
+}
+
+// w_0 = mask (A r_0); gamma_0, delta_0 -> alpha_0.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_init2(CgArgs<T> a)
+{
+    using V = typename V2<T>::type;
+    const int lb = xcd_block(a.nblocks);
+    const int n = lb * kBlock + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (n < a.nn) {
+        const int2 sl = a.slice[n >> 6];
+        size_t q = (size_t)sl.x + (n & 63);
+        T yx = 0, yy = 0;
+        for (int k = 0; k < sl.y; ++k, q += kWave) {
+            const V rm = a.r[0][a.col[q]];
+            T m00, m01, m10, m11;
+            load_block(a, q, m00, m01, m10, m11);
+            yx += m00 * rm.x + m01 * rm.y;
+            yy += m10 * rm.x + m11 * rm.y;
+        }
+        const uchar2 mk = a.mask[n];
+        const V wn = V{mk.x ? yx : (T)0, mk.y ? yy : (T)0};
+        a.w[0][n] = wn;
+        const V rn = a.r[0][n];
+        v[0] = (double)rn.x * rn.x + (double)rn.y * rn.y;
+        v[1] = (double)wn.x * rn.x + (double)wn.y * rn.y;
+    }
+    if (grid_sum<2>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
+        CgScal* sc = a.sc;
+        const double g = v[0], d = v[1];
+        if (sc->bnorm2 == 0.) sc->bnorm2 = g;
+        sc->gamma0 = g;
+        sc->gamma = g;
+        sc->delta = d;
+        sc->beta = 0.;
+        sc->alpha = g / d;
+        sc->iters = 0;
+        sc->status = (g != g || d != d) ? 2 : 0;
+        sc->done = (g == 0. || sc->status != 0) ? 1 : 0;
+    }
+}
+
+// One Chronopoulos-Gear CG iteration in a single launch:
+//   p_i = r_i + beta_i p_{i-1};  s_i = w_i + beta_i s_{i-1}   (own node)
+//   x_{i+1} = x_i + alpha_i p_i; r_{i+1} = r_i - alpha_i s_i  (own node)
+//   w_{i+1} = mask (A r_{i+1}), with r_{i+1} at the neighbours formed on the
+//   fly from r_i, w_i, s_{i-1} (ping-pong buffers, parity = iteration count)
+//   gamma_{i+1} = (r, r), delta_{i+1} = (w, r) -> the last block forms
+//   beta_{i+1} = gamma_{i+1}/gamma_i, alpha_{i+1} = gamma_{i+1} /
+//   (delta_{i+1} - beta_{i+1} gamma_{i+1}/alpha_i) and the stop test
+//   gamma/||b||^2 <= tol^2 (ConjGradCuda.cu:203-222).
+// w and s carry the Dirichlet mask, so r stays zero on fixed DOFs.
+template <typename T>
+__device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
+{
+    using V = typename V2<T>::type;
+    CgScal* sc = a.sc;
+    const int it = sc->iters;
+    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
+    const int par = it & 1;
+    const V* rin = a.r[par];
+    const V* win = a.w[par];
+    const V* sprev = a.s[par ^ 1];
+    V* rout = a.r[par ^ 1];
+    V* wout = a.w[par ^ 1];
+    V* sout = a.s[par];
+    const int n = lb * kBlock + threadIdx.x;
+    double v[2] = {0., 0.};
+    if (n < a.nn) {
+        const V ri = rin[n], wi = win[n], sp = sprev[n], pp = a.p[n], xi = a.x[n];
+        const V si = V{wi.x + beta * sp.x, wi.y + beta * sp.y};
+        const V pi = V{ri.x + beta * pp.x, ri.y + beta * pp.y};
+        const V rn = V{ri.x - alpha * si.x, ri.y - alpha * si.y};
+        a.p[n] = pi;
+        sout[n] = si;
+        a.x[n] = V{xi.x + alpha * pi.x, xi.y + alpha * pi.y};
+        rout[n] = rn;
+
+        const int2 sl = a.slice[n >> 6];
+        size_t q = (size_t)sl.x + (n & 63);
+        T yx = 0, yy = 0;
+        for (int k = 0; k < sl.y; ++k, q += kWave) {
+            const int m = a.col[q];
+            const V rm = rin[m], wm = win[m], sm = sprev[m];
+            const T zx = rm.x - alpha * (wm.x + beta * sm.x);
+            const T zy = rm.y - alpha * (wm.y + beta * sm.y);
+            T m00, m01, m10, m11;
+            load_block(a, q, m00, m01, m10, m11);
+            yx += m00 * zx + m01 * zy;
+            yy += m10 * zx + m11 * zy;
+        }
+        const uchar2 mk = a.mask[n];
+        const V wn = V{mk.x ? yx : (T)0, mk.y ? yy : (T)0};
+        wout[n] = wn;
+        v[0] = (double)rn.x * rn.x + (double)rn.y * rn.y;
+        v[1] = (double)wn.x * rn.x + (double)wn.y * rn.y;
+    }
+    if (grid_sum<2>(v, a.partials, &sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
+        const double g = v[0], d = v[1];
+        const int iters = it + 1;
+        sc->iters = iters;
+        if (g != g || d != d) {
+            sc->status = 2;
+            sc->done = 1;
+            return;
+        }
+        const double bp = g / sc->gamma;
+        const double ap = g / (d - bp * g / sc->alpha);
+        sc->gamma = g;
+        sc->delta = d;
+        sc->beta = bp;
+        sc->alpha = ap;
+        if (!(g * (1. / sc->bnorm2) > sc->tol2)) {
+            sc->done = 1;
+        } else if (iters >= sc->maxit) {
+            sc->done = 1;
+            sc->status = 1;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_iter(CgArgs<T> a)
+{
+    if (a.sc->done) return;
+    cg_iter_body<T>(a, xcd_block(a.nblocks));
+}
+
+// Batched form for the serving loop: one launch advances one CG iteration of
+// every problem in the table; tiles[t] = {problem, local block}.  Each
+// problem keeps its own scalars, partials and counter, so its result is the
+// same alone or in a batch.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_iter_batched(const CgArgs<T>* probs, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const CgArgs<T> a = probs[t.x];
+    if (a.sc->done) return;
+    cg_iter_body<T>(a, t.y);
+}
+
+static __global__ void k_cg_setup(CgScal* sc, double bnorm2, double tol2, int maxit)
+{
+    sc->bnorm2 = bnorm2;
+    sc->tol2 = tol2;
+    sc->maxit = maxit;
+    sc->iters = 0;
+    sc->done = 0;
+    sc->status = 0;
+    sc->counter = 0u;
+}
+
+// Standalone SpMV y = A x (bandwidth measurement of the stored format).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_spmv(CgArgs<T> a, const typename V2<T>::type* xin,
+                                                 typename V2<T>::type* yout)
+{
+    using V = typename V2<T>::type;
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    const int2 sl = a.slice[n >> 6];
+    size_t q = (size_t)sl.x + (n & 63);
+    T yx = 0, yy = 0;
+    for (int k = 0; k < sl.y; ++k, q += kWave) {
+        const V xm = xin[a.col[q]];
+        T m00, m01, m10, m11;
+        load_block(a, q, m00, m01, m10, m11);
+        yx += m00 * xm.x + m01 * xm.y;
+        yy += m10 * xm.x + m11 * xm.y;
+    }
+    yout[n] = V{yx, yy};
+}
+
+// ---------------------------------------------------------------- FP64 residual (matrix free)
+
+struct RefScal {
+    double rnorm2;
+    unsigned counter, pad;
+};
+
+struct ResArgs {
+    const int* incPtr;
+    const int2* inc;
+    const int4* conn;
+    const double2* xy;
+    const double2* ecnu;
+    const double2* u;
+    const double2* b;      // NULL: output the internal force itself
+    const uchar2* mask;
+    double2* r64;          // out
+    float2* r32;           // out (optional): FP32 copy for the inner solve
+    RefScal* sc;
+    double* partials;
+    int nn, nblocks, plane;
+    double h;
+};
+
+// Element internal force f_i = h sum_gp detJ B_i^T C (B u_e) gathered per
+// node (the element kernel applied to u), r = mask (b - f) in FP64.
+static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
+{
+    const int lb = xcd_block(a.nblocks);
+    const int n = lb * kBlock + threadIdx.x;
+    double v[1] = {0.};
+    if (n < a.nn) {
+        double fx = 0., fy = 0.;
+        for (int q = a.incPtr[n]; q < a.incPtr[n + 1]; ++q) {
+            const int2 ic = a.inc[q];
+            const int4 c = a.conn[ic.x];
+            const double2 en = a.ecnu[ic.x];
+            const unsigned slots = (unsigned)ic.y;
+            const int i = (slots & 255u) == 0 ? 0 : (((slots >> 8) & 255u) == 0 ? 1 : (((slots >> 16) & 255u) == 0 ? 2 : 3));
+            const int cc[4] = {c.x, c.y, c.z, c.w};
+            double x[4], y[4], ux[4], uy[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double2 p = a.xy[cc[j]];
+                const double2 w = a.u[cc[j]];
+                x[j] = p.x; y[j] = p.y; ux[j] = w.x; uy[j] = w.y;
+            }
+            double c00, c01, c22;
+            elastic_c(a.plane, en.x, en.y, c00, c01, c22);
+            double ex = 0., ey = 0.;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                double B0[4], B1[4], dJ;
+                shape_grad(x, y, g, B0, B1, dJ);
+                const double exx = B0[0] * ux[0] + B0[1] * ux[1] + B0[2] * ux[2] + B0[3] * ux[3];
+                const double eyy = B1[0] * uy[0] + B1[1] * uy[1] + B1[2] * uy[2] + B1[3] * uy[3];
+                const double gxy = B1[0] * ux[0] + B1[1] * ux[1] + B1[2] * ux[2] + B1[3] * ux[3] +
+                                   B0[0] * uy[0] + B0[1] * uy[1] + B0[2] * uy[2] + B0[3] * uy[3];
+                const double sx = c00 * exx + c01 * eyy, sy = c01 * exx + c00 * eyy, tx = c22 * gxy;
+                const double bi0 = pick4(B0, i), bi1 = pick4(B1, i);
+                ex += dJ * (bi0 * sx + bi1 * tx);
+                ey += dJ * (bi1 * sy + bi0 * tx);
+            }
+            fx += ex * a.h;
+            fy += ey * a.h;
+        }
+        double rx = fx, ry = fy;
+        if (a.b) {
+            const double2 bn = a.b[n];
+            const uchar2 mk = a.mask[n];
+            rx = mk.x ? bn.x - fx : 0.;
+            ry = mk.y ? bn.y - fy : 0.;
+            v[0] = rx * rx + ry * ry;
+        }
+        a.r64[n] = make_double2(rx, ry);
+        if (a.r32) a.r32[n] = make_float2((float)rx, (float)ry);
+    }
+    if (a.sc && grid_sum<1>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) a.sc->rnorm2 = v[0];
+}
+
+// u += (double) d on every node (the refinement update).
+static __global__ __launch_bounds__(kBlock) void k_add_correction(double2* u, const float2* d, int nn, int nblocks)
+{
+    const int n = xcd_block(nblocks) * kBlock + threadIdx.x;
+    if (n >= nn) return;
+    const double2 a = u[n];
+    const float2 c = d[n];
+    u[n] = make_double2(a.x + (double)c.x, a.y + (double)c.y);
+}
+
+// ---------------------------------------------------------------- return map
+
+struct RmScal {
+    double rel_err, omega, omega_prev, rr_prev;
+    int plain, last, converged, status;
+    unsigned counter, pad;
+};
+
+struct RmArgs {
+    const int4* conn;
+    const double2* xy;
+    const double2* u;
+    const double2* ecnu;
+    const double* psi;
+    double* F;            // per-element fields, field f at F[f*ne + e]
+    double* G;            // plain secant update of psi
+    double* rcur;         // log G - log psi
+    const double* rprev;
+    RmScal* sc;
+    double* partials;
+    MatParams M;
+    int ne, nblocks;
+    int k, aitken, fixed_outer, max_outer, elastic;
+    double tol;
+};
+
+// strain4 (PlasticitySolver.cu:1517-1549) + plastStress (:1590-1610) +
+// exitNorm2 (:1664-1691) for one element per thread, always in FP64,
+// followed by the decision of the safeguarded Irons-Tuck step.
+static __global__ __launch_bounds__(kBlock) void k_return_map(RmArgs a)
+{
+    const int lb = xcd_block(a.nblocks);
+    const int e = lb * kBlock + threadIdx.x;
+    double v[5] = {0., 0., 0., 0., 0.};
+    if (e < a.ne) {
+        const int4 c = a.conn[e];
+        const int cc[4] = {c.x, c.y, c.z, c.w};
+        double x[4], y[4], ux[4], uy[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const double2 p = a.xy[cc[j]];
+            const double2 w = a.u[cc[j]];
+            x[j] = p.x; y[j] = p.y; ux[j] = w.x; uy[j] = w.y;
+        }
+        double exx = 0., eyy = 0., gxy = 0.;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            double B0[4], B1[4], dJ;
+            shape_grad(x, y, g, B0, B1, dJ);
+            exx += ux[0] * B0[0] + ux[1] * B0[1] + ux[2] * B0[2] + ux[3] * B0[3];
+            eyy += uy[0] * B1[0] + uy[1] * B1[1] + uy[2] * B1[2] + uy[3] * B1[3];
+            gxy += ux[0] * B1[0] + ux[1] * B1[1] + ux[2] * B1[2] + ux[3] * B1[3] + uy[0] * B0[0] + uy[1] * B0[1] +
+                   uy[2] * B0[2] + uy[3] * B0[3];
+        }
+        exx *= 0.25;
+        eyy *= 0.25;
+        gxy *= 0.25;
+        const double2 en = a.ecnu[e];
+        double c00, c01, c22;
+        elastic_c(a.M.plane, en.x, en.y, c00, c01, c22);
+        const double sxx = c00 * exx + c01 * eyy;
+        const double syy = c01 * exx + c00 * eyy;
+        const double tau = c22 * gxy;
+        const double sigI = sqrt(0.5 * (sxx * sxx + syy * syy + (sxx - syy) * (sxx - syy) + 6. * tau * tau));
+        const double epsI = sigI / en.x;
+        const double sigTab = mat_f(a.M, epsI);
+        const double G = (sigTab != 0.) ? a.M.E * epsI / sigTab : 1.;
+        const size_t ne = (size_t)a.ne;
+        a.F[0 * ne + e] = exx;
+        a.F[1 * ne + e] = eyy;
+        a.F[2 * ne + e] = gxy;
+        a.F[3 * ne + e] = epsI;
+        a.F[4 * ne + e] = sxx;
+        a.F[5 * ne + e] = syy;
+        a.F[6 * ne + e] = tau;
+        a.F[7 * ne + e] = sigI;
+        a.F[8 * ne + e] = sigTab;
+        a.G[e] = G;
+        const double d = sigI - sigTab;
+        v[0] = d * d;
+        v[1] = sigTab * sigTab;
+        if (a.aitken) {
+            const double rc = log(G) - log(a.psi[e]);
+            const double rp = a.rprev[e];
+            const double qd = rc - rp;
+            a.rcur[e] = rc;
+            v[2] = rc * rc;
+            v[3] = rp * qd;
+            v[4] = qd * qd;
+        }
+    }
+    if (grid_sum<5>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
+        RmScal* sc = a.sc;
+        const double rel = sqrt(v[0] / v[1]);
+        sc->rel_err = rel;
+        const int nan = rel != rel;
+        const int last = a.elastic || nan ||
+                         (a.fixed_outer > 0 ? (a.k + 1 == a.fixed_outer) : (rel <= a.tol || a.k + 1 == a.max_outer));
+        const int plain = last || !a.aitken || a.k == 0 || v[2] > sc->rr_prev;
+        double om = 1.;
+        if (!plain) {
+            om = v[4] > 0. ? -sc->omega_prev * v[3] / v[4] : sc->omega_prev;
+            om = om < 0.05 ? 0.05 : (om > 100.0 ? 100.0 : om);
+        }
+        sc->omega = om;
+        sc->omega_prev = om;
+        sc->rr_prev = v[2];
+        sc->plain = plain;
+        sc->last = last;
+        sc->converged = last && !nan && (a.elastic || a.fixed_outer > 0 || rel <= a.tol);
+        sc->status = nan ? 2 : 0;
+    }
+}
+
+// psi_{k+1} = G (plain) or exp(log psi + omega r) (Aitken), then
+// E_c = E/psi, nu_c = (1 - (1 - 2 nu)/psi)/2 (PlasticitySolver.cu:1607-1609).
+static __global__ __launch_bounds__(kBlock) void k_apply_update(double* psi, double2* ecnu, const double* G, const double* rcur,
+                                                         double* rprev, const RmScal* sc, double E, double nu, int ne,
+                                                         int nblocks, int aitken)
+{
+    const int e = xcd_block(nblocks) * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    double p = G[e];
+    if (!sc->plain) {
+        p = exp(log(psi[e]) + sc->omega * rcur[e]);
+        if (p < 1.) p = 1.;
+    }
+    psi[e] = p;
+    ecnu[e] = make_double2(E / p, 0.5 * (1. - (1. - 2. * nu) / p));
+    if (aitken) rprev[e] = rcur[e];
+}
+
+// plastDeform (PlasticitySolver.cu:1641-1653).
+static __global__ __launch_bounds__(kBlock) void k_plastic_strains(double* F, double* psi, const double2* ecnu, double E, double nu,
+                                                            int ne, int nblocks)
+{
+    const int e = xcd_block(nblocks) * kBlock + threadIdx.x;
+    if (e >= ne) return;
+    const size_t N = (size_t)ne;
+    const double Gm = E * 0.5 / (1. + nu);
+    const double Ec = ecnu[e].x, nuc = ecnu[e].y;
+    const double ex = (1. / Ec - 1. / E) * F[4 * N + e];
+    const double ey = (1. / Ec - 1. / E) * F[5 * N + e];
+    const double Gc = Ec * 0.5 / (1. + nuc);
+    const double gp = (1. / Gc - 1. / Gm) * F[6 * N + e];
+    F[12 * N + e] = ex;
+    F[13 * N + e] = ey;
+    F[14 * N + e] = gp;
+    F[15 * N + e] = sqrt(2.) / 3. * sqrt((ex - ey) * (ex - ey) + 6. * gp * gp);
+    psi[e] = E / Ec;
+}
+
+// ---------------------------------------------------------------- small element-wise kernels
+
+static __global__ void k_init_state(double* psi, double2* ecnu, double* rprev, double E, double nu, int ne)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    psi[e] = 1.;
+    ecnu[e] = make_double2(E, nu);
+    rprev[e] = 0.;
+}
+
+// b = factor * rp0; u = factor * u0 on fixed DOFs (free DOFs keep u, or 0 when reset).
+static __global__ void k_set_load(double2* b, double2* u, const double2* rp0, const double2* u0, const uchar2* mask,
+                           double factor, int reset, int nn)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const double2 r = rp0[n], p = u0[n];
+    const uchar2 mk = mask[n];
+    b[n] = make_double2(factor * r.x, factor * r.y);
+    double2 w = reset ? make_double2(0., 0.) : u[n];
+    if (!mk.x) w.x = factor * p.x;
+    if (!mk.y) w.y = factor * p.y;
+    u[n] = w;
+}
+
+static __global__ void k_rm_reset(RmScal* sc)
+{
+    sc->omega_prev = 1.;
+    sc->rr_prev = 0.;
+    sc->counter = 0u;
+}
+
+}  // namespace cpk

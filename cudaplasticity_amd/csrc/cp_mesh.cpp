@@ -1,0 +1,565 @@
+// cp_mesh.cpp -- host-side setup of the plasticity hot path: mesh generators,
+// load / boundary-condition rules and the reference CSR sparsity pattern.
+// sin/cos pairs go through glibc sincos so that every build (gcc or clang)
+// produces the same coordinates.  These run once per problem on the host;
+// the device path starts at
+// cp_problem_create.  Reference citations are relative to its src/.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "cp_abi.h"
+#include "cp_internal.h"
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;
+
+struct BorderWriter {
+    int* bord;
+    const int* blen;
+    int* at(int b) const {
+        int off = 0;
+        for (int i = 0; i < b; ++i) off += blen[i];
+        return bord + off;
+    }
+};
+
+// Mesh::renumerateRing (Mesh.cu:332-384): columns of the ring are numbered
+// alternately from the front and the back so the band stays narrow.  An odd
+// column count numbers the middle column last (the reference writes that
+// map reversed, Mesh.cu:347-349).
+void renumber_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
+                   int columns)
+{
+    const int rowsPerCol = nn / columns;
+    std::vector<int> perm(nn);
+    int next = 0;
+    int lo = 0, hi = columns - 1;
+    for (; lo < hi; ++lo, --hi) {
+        for (int i = 0; i < rowsPerCol; ++i) perm[i * columns + lo] = next++;
+        for (int i = 0; i < rowsPerCol; ++i) perm[i * columns + hi] = next++;
+    }
+    if (lo == hi)
+        for (int i = 0; i < rowsPerCol; ++i) perm[i * columns + lo] = next++;
+    for (int k = 0; k < 4 * ne; ++k) conn[k] = perm[conn[k]];
+    int total = 0;
+    for (int b = 0; b < nb; ++b) total += blen[b];
+    for (int k = 0; k < total; ++k) bord[k] = perm[bord[k]];
+    std::vector<double> nx(nn), ny(nn);
+    for (int i = 0; i < nn; ++i) {
+        nx[perm[i]] = x[i];
+        ny[perm[i]] = y[i];
+    }
+    std::memcpy(x, nx.data(), sizeof(double) * nn);
+    std::memcpy(y, ny.data(), sizeof(double) * nn);
+}
+
+int radial_segments(double x1, double x2, double y1, double y2, double r, int N1, int N2)
+{
+    // Mesh.cu:115-116
+    int radN = (N1 + N2) / 4;
+    return (int)std::round(double(radN) * (x2 - x1 + y2 - y1 - 4. * r) * 2. / (x2 - x1 + y2 - y1));
+}
+
+}  // namespace
+
+extern "C" {
+
+int cp_mesh_rect_sizes(int N1, int N2, int* nn, int* ne, int* nb, int* blen)
+{
+    if (N1 < 1 || N2 < 1) return cp::fail(CP_ERR_INVALID, "cp_mesh_rect_sizes: N1, N2 must be >= 1");
+    *nn = (N1 + 1) * (N2 + 1);
+    *ne = N1 * N2;
+    *nb = 4;
+    if (blen) {
+        blen[0] = blen[2] = N2 + 1;
+        blen[1] = blen[3] = N1 + 1;
+    }
+    return CP_OK;
+}
+
+// Mesh::genRectangle (Mesh.cu:42-101), order 1.
+int cp_mesh_rect(double x1, double x2, double y1, double y2, int N1, int N2, double* x, double* y, int* conn,
+                 int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_rect_sizes(N1, N2, &nn, &ne, &nb, blen)) return s;
+    const double hx = (x2 - x1) / N1, hy = (y2 - y1) / N2;
+    const int stride = N2 + 1;
+    for (int i = 0; i <= N1; ++i)
+        for (int j = 0; j <= N2; ++j) {
+            x[i * stride + j] = x1 + hx * i;
+            y[i * stride + j] = y1 + hy * j;
+        }
+    for (int i = 0; i < N1; ++i)
+        for (int j = 0; j < N2; ++j) {
+            int* c = conn + 4 * (i * N2 + j);
+            c[0] = (i + 1) * stride + j + 1;
+            c[1] = i * stride + j + 1;
+            c[2] = i * stride + j;
+            c[3] = (i + 1) * stride + j;
+        }
+    BorderWriter bw{bord, blen};
+    int *right = bw.at(0), *top = bw.at(1), *left = bw.at(2), *bottom = bw.at(3);
+    for (int j = 0; j <= N2; ++j) {
+        right[j] = N1 * stride + j;      // (x2, y1) -> (x2, y2)
+        left[j] = N2 - j;                // (x1, y2) -> (x1, y1)
+    }
+    for (int i = 0; i <= N1; ++i) {
+        top[i] = (N1 - i) * stride + N2; // (x2, y2) -> (x1, y2)
+        bottom[i] = i * stride;          // (x1, y1) -> (x2, y1)
+    }
+    return CP_OK;
+}
+
+int cp_mesh_hole_sizes(double x1, double x2, double y1, double y2, double r, int N1, int N2, int* nn, int* ne,
+                       int* nb, int* blen)
+{
+    if (N1 < 1 || N2 < 1 || !(r > 0)) return cp::fail(CP_ERR_INVALID, "cp_mesh_hole_sizes: bad arguments");
+    const int borderN = 2 * (N1 + N2);
+    const int radN = radial_segments(x1, x2, y1, y2, r, N1, N2);
+    if (radN < 1) return cp::fail(CP_ERR_INVALID, "cp_mesh_hole_sizes: hole too large for the plate");
+    *nn = borderN * (radN + 1);
+    *ne = borderN * radN;
+    *nb = 5;
+    if (blen) {
+        blen[0] = blen[2] = N1 + 1;
+        blen[1] = blen[3] = N2 + 1;
+        blen[4] = borderN + 1;
+    }
+    return CP_OK;
+}
+
+// Mesh::genRectWithHole (Mesh.cu:112-245), order 1: outer rectangle nodes
+// counter-clockwise from (x1, y1), hole nodes on a circle, straight rays
+// between outer node j and hole node j, then renumerateRing.
+int cp_mesh_hole(double x1, double x2, double y1, double y2, double r, int N1, int N2, double* x, double* y,
+                 int* conn, int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_hole_sizes(x1, x2, y1, y2, r, N1, N2, &nn, &ne, &nb, blen)) return s;
+    const int borderN = 2 * (N1 + N2);
+    const int radN = radial_segments(x1, x2, y1, y2, r, N1, N2);
+    BorderWriter bw{bord, blen};
+    int *bBottom = bw.at(0), *bRight = bw.at(1), *bTop = bw.at(2), *bLeft = bw.at(3), *bHole = bw.at(4);
+
+    const double hx = (x2 - x1) / N1, hy = (y2 - y1) / N2;
+    int id = 0;
+    for (int i = 0; i < N1; ++i, ++id) { x[id] = x1 + i * hx; y[id] = y1; bBottom[i] = id; }
+    bBottom[N1] = id;
+    for (int j = 0; j < N2; ++j, ++id) { x[id] = x2; y[id] = y1 + j * hy; bRight[j] = id; }
+    bRight[N2] = id;
+    for (int i = 0; i < N1; ++i, ++id) { x[id] = x2 - i * hx; y[id] = y2; bTop[i] = id; }
+    bTop[N1] = id;
+    for (int j = 0; j < N2; ++j, ++id) { x[id] = x1; y[id] = y2 - j * hy; bLeft[j] = id; }
+    bLeft[N2] = 0;
+
+    const double dphi = 2. * kPi / borderN;
+    const double cx = 0.5 * (x1 + x2), cy = 0.5 * (y1 + y2);
+    const double phi0 = dphi * (N1 + N2 * 1.5);
+    const int holeBase = radN * borderN;
+    for (int k = 0; k < borderN; ++k) {
+        const double a = phi0 + double(k) * dphi;
+        double sa, ca;
+        ::sincos(a, &sa, &ca);
+        x[holeBase + k] = cx + ca * r;
+        y[holeBase + k] = cy + sa * r;
+        bHole[borderN - 1 - k] = holeBase + k;
+    }
+    bHole[borderN] = nn - 1;
+    for (int j = 0; j < borderN; ++j) {
+        const double sx = (x[holeBase + j] - x[j]) / radN, sy = (y[holeBase + j] - y[j]) / radN;
+        for (int i = 1; i < radN; ++i) {
+            x[i * borderN + j] = x[j] + sx * double(i);
+            y[i * borderN + j] = y[j] + sy * double(i);
+        }
+    }
+    for (int j = 0; j < borderN; ++j) {
+        const int jn = (j + 1 == borderN) ? 0 : j + 1;
+        for (int i = 0; i < radN; ++i) {
+            int* c = conn + 4 * (j * radN + i);
+            c[0] = (i + 1) * borderN + jn;
+            c[1] = (i + 1) * borderN + j;
+            c[2] = i * borderN + j;
+            c[3] = i * borderN + jn;
+        }
+    }
+    renumber_ring(nn, ne, x, y, conn, nb, bord, blen, borderN);
+    return CP_OK;
+}
+
+int cp_mesh_ring_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    if (Nphi < 3 || Nr < 1) return cp::fail(CP_ERR_INVALID, "cp_mesh_ring_sizes: bad arguments");
+    *nn = (Nr + 1) * Nphi;
+    *ne = Nr * Nphi;
+    *nb = 2;
+    if (blen) blen[0] = blen[1] = Nphi + 1;
+    return CP_OK;
+}
+
+// Mesh::genRing (Mesh.cu:249-329), order 1.
+int cp_mesh_ring(double a, double b, int Nphi, int Nr, double* x, double* y, int* conn, int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_ring_sizes(Nphi, Nr, &nn, &ne, &nb, blen)) return s;
+    BorderWriter bw{bord, blen};
+    int *inner = bw.at(0), *outer = bw.at(1);
+    const double hr = (b - a) / Nr, hphi = 2. * kPi / Nphi;
+    for (int j = 0; j < Nphi; ++j) {
+        for (int i = 0; i <= Nr; ++i) {
+            const double rr = b - hr * i, phi = hphi * j;
+            double sp, cp;
+            ::sincos(phi, &sp, &cp);
+            x[i * Nphi + j] = rr * cp;
+            y[i * Nphi + j] = rr * sp;
+        }
+        outer[Nphi - j] = j;
+        inner[j] = Nr * Nphi + j;
+    }
+    outer[0] = 0;
+    inner[Nphi] = Nr * Nphi;
+    for (int j = 0; j < Nphi; ++j) {
+        const int jn = (j + 1 == Nphi) ? 0 : j + 1;
+        for (int i = 0; i < Nr; ++i) {
+            int* c = conn + 4 * (j * Nr + i);
+            c[0] = (i + 1) * Nphi + jn;
+            c[1] = (i + 1) * Nphi + j;
+            c[2] = i * Nphi + j;
+            c[3] = i * Nphi + jn;
+        }
+    }
+    renumber_ring(nn, ne, x, y, conn, nb, bord, blen, Nphi);
+    return CP_OK;
+}
+
+int cp_mesh_quarter_plate_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    if (Nt < 2 || Nr < 1) return cp::fail(CP_ERR_INVALID, "cp_mesh_quarter_plate_sizes: bad arguments");
+    *nn = (Nt + 1) * (Nr + 1);
+    *ne = Nt * Nr;
+    *nb = 5;
+    if (blen) {
+        blen[0] = Nr + 1;
+        blen[1] = Nt / 2 + 1;
+        blen[2] = Nt - Nt / 2 + 1;
+        blen[3] = Nr + 1;
+        blen[4] = Nt + 1;
+    }
+    return CP_OK;
+}
+
+// Quarter plate with a quarter hole (new).  Ray j joins outer-edge point j
+// (x = L for the first Nt/2 segments, then y = L) to hole point j at angle
+// (pi/2) j/Nt; radial index i runs from the outer edge (0) to the hole (Nr).
+int cp_mesh_quarter_plate(double L, double r, int Nt, int Nr, double* x, double* y, int* conn, int* bord,
+                          int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_quarter_plate_sizes(Nt, Nr, &nn, &ne, &nb, blen)) return s;
+    if (!(r > 0) || !(r < L)) return cp::fail(CP_ERR_INVALID, "cp_mesh_quarter_plate: need 0 < r < L");
+    const int half = Nt / 2, col = Nr + 1;
+    for (int j = 0; j <= Nt; ++j) {
+        double ox, oy, hx, hy;
+        if (j <= half) {
+            ox = L;
+            oy = L * double(j) / double(half);
+        } else {
+            ox = L - L * double(j - half) / double(Nt - half);
+            oy = L;
+        }
+        if (j == 0) { hx = r; hy = 0.; }
+        else if (j == Nt) { hx = 0.; hy = r; }
+        else {
+            const double t = 0.5 * kPi * double(j) / double(Nt);
+            double st, ct;
+            ::sincos(t, &st, &ct);
+            hx = r * ct;
+            hy = r * st;
+        }
+        const double sx = (hx - ox) / Nr, sy = (hy - oy) / Nr;
+        x[j * col] = ox;
+        y[j * col] = oy;
+        for (int i = 1; i < Nr; ++i) {
+            x[j * col + i] = ox + sx * double(i);
+            y[j * col + i] = oy + sy * double(i);
+        }
+        x[j * col + Nr] = hx;
+        y[j * col + Nr] = hy;
+    }
+    for (int j = 0; j < Nt; ++j)
+        for (int i = 0; i < Nr; ++i) {
+            int* c = conn + 4 * (j * Nr + i);
+            c[0] = (j + 1) * col + i + 1;
+            c[1] = j * col + i + 1;
+            c[2] = j * col + i;
+            c[3] = (j + 1) * col + i;
+        }
+    BorderWriter bw{bord, blen};
+    int *b0 = bw.at(0), *b1 = bw.at(1), *b2 = bw.at(2), *b3 = bw.at(3), *b4 = bw.at(4);
+    for (int k = 0; k <= Nr; ++k) { b0[k] = Nr - k; b3[k] = Nt * col + k; }
+    for (int k = 0; k <= half; ++k) b1[k] = k * col;
+    for (int k = 0; k <= Nt - half; ++k) b2[k] = (half + k) * col;
+    for (int k = 0; k <= Nt; ++k) b4[k] = (Nt - k) * col + Nr;
+    return CP_OK;
+}
+
+int cp_mesh_quarter_ring_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    if (Nt < 1 || Nr < 1) return cp::fail(CP_ERR_INVALID, "cp_mesh_quarter_ring_sizes: bad arguments");
+    *nn = (Nt + 1) * (Nr + 1);
+    *ne = Nt * Nr;
+    *nb = 4;
+    if (blen) {
+        blen[0] = blen[2] = Nt + 1;
+        blen[1] = blen[3] = Nr + 1;
+    }
+    return CP_OK;
+}
+
+// Graded quarter ring (new): radial sizes h0 * ratio^k from the inner radius.
+int cp_mesh_quarter_ring(double a, double b, int Nt, int Nr, double ratio, double* x, double* y, int* conn,
+                         int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_quarter_ring_sizes(Nt, Nr, &nn, &ne, &nb, blen)) return s;
+    if (!(a > 0) || !(b > a) || !(ratio > 0)) return cp::fail(CP_ERR_INVALID, "cp_mesh_quarter_ring: bad radii/ratio");
+    std::vector<double> rad(Nr + 1);
+    const double h0 = (ratio == 1.0) ? (b - a) / Nr : (b - a) * (ratio - 1.0) / (std::pow(ratio, Nr) - 1.0);
+    rad[0] = a;
+    double h = h0;
+    for (int i = 1; i < Nr; ++i) {
+        rad[i] = rad[i - 1] + h;
+        h *= ratio;
+    }
+    rad[Nr] = b;
+    const int col = Nr + 1;
+    for (int j = 0; j <= Nt; ++j) {
+        double c, s;
+        if (j == 0) { c = 1.; s = 0.; }
+        else if (j == Nt) { c = 0.; s = 1.; }
+        else {
+            const double t = 0.5 * kPi * double(j) / double(Nt);
+            ::sincos(t, &s, &c);
+        }
+        for (int i = 0; i <= Nr; ++i) {
+            x[j * col + i] = rad[i] * c;
+            y[j * col + i] = rad[i] * s;
+        }
+    }
+    for (int j = 0; j < Nt; ++j)
+        for (int i = 0; i < Nr; ++i) {
+            int* q = conn + 4 * (j * Nr + i);
+            q[0] = (j + 1) * col + i + 1;
+            q[1] = (j + 1) * col + i;
+            q[2] = j * col + i;
+            q[3] = j * col + i + 1;
+        }
+    BorderWriter bw{bord, blen};
+    int *inner = bw.at(0), *left = bw.at(1), *outer = bw.at(2), *bottom = bw.at(3);
+    for (int k = 0; k <= Nt; ++k) {
+        inner[k] = (Nt - k) * col;
+        outer[k] = k * col + Nr;
+    }
+    for (int k = 0; k <= Nr; ++k) {
+        bottom[k] = k;
+        left[k] = Nt * col + Nr - k;
+    }
+    return CP_OK;
+}
+
+int cp_mesh_max_index_diff(int ne, const int* conn, int* out)
+{
+    int mx = 0;
+    for (int e = 0; e < ne; ++e)
+        for (int i = 0; i < 4; ++i)
+            for (int j = i + 1; j < 4; ++j) mx = std::max(mx, std::abs(conn[4 * e + i] - conn[4 * e + j]));
+    *out = mx;
+    return CP_OK;
+}
+
+// Border traction (PlasticitySolver.cu:1070-1103): each segment [n1, n2]
+// adds f(n) * h * |n2 - n1| / 2 to both end nodes; with norm_orient the
+// value is (normal, tangential) with normal = (t.y, -t.x).
+int cp_rhs_border_force(const double* x, const double* y, const int* border, int len, const double* fval,
+                        int norm_orient, double h, double* rp)
+{
+    for (int k = 1; k < len; ++k) {
+        const int a = border[k - 1], b = border[k];
+        const double hdx = 0.5 * (x[a] - x[b]), hdy = 0.5 * (y[a] - y[b]);
+        const double dl = std::sqrt(hdx * hdx + hdy * hdy);
+        double fa[2] = {fval[2 * (k - 1)], fval[2 * (k - 1) + 1]};
+        double fb[2] = {fval[2 * k], fval[2 * k + 1]};
+        if (norm_orient) {
+            double tx = x[b] - x[a], ty = y[b] - y[a];
+            const double it = 1. / std::sqrt(tx * tx + ty * ty);
+            tx *= it;
+            ty *= it;
+            double nx = ty, ny = -tx;
+            const double in = 1. / std::sqrt(nx * nx + ny * ny);
+            nx *= in;
+            ny *= in;
+            const double ax = nx * fa[0] + tx * fa[1], ay = ny * fa[0] + ty * fa[1];
+            const double bx = nx * fb[0] + tx * fb[1], by = ny * fb[0] + ty * fb[1];
+            fa[0] = ax; fa[1] = ay; fb[0] = bx; fb[1] = by;
+        }
+        const double w = h * dl;
+        rp[2 * a] += fa[0] * w;
+        rp[2 * a + 1] += fa[1] * w;
+        rp[2 * b] += fb[0] * w;
+        rp[2 * b + 1] += fb[1] * w;
+    }
+    return CP_OK;
+}
+
+int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn, double gx, double gy,
+                      double rho, double h, double* rp)
+{
+    for (int e = 0; e < ne; ++e) {
+        const int* c = conn + 4 * e;
+        // elemSpace4 (Mesh.cuh:258-267)
+        double s1 = 0., s2 = 0.;
+        for (int i = 1; i < 4; ++i) {
+            s1 += x[c[i - 1]] * y[c[i]];
+            s2 += y[c[i - 1]] * x[c[i]];
+        }
+        s1 += x[c[3]] * y[c[0]];
+        s2 += y[c[3]] * x[c[0]];
+        const double w = 0.25 * rho * h * (std::fabs(s1 - s2) * 0.5);
+        for (int i = 0; i < 4; ++i) {
+            rp[2 * c[i]] += w * gx;
+            rp[2 * c[i] + 1] += w * gy;
+        }
+    }
+    return CP_OK;
+}
+
+int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask)
+{
+    const double* v = vertical ? x : y;
+    const int comp = vertical ? 0 : 1;
+    for (int i = 0; i < nn; ++i)
+        if (std::fabs(v[i] - coord) < 1e-14) mask[2 * i + comp] = 0;
+    return CP_OK;
+}
+
+int cp_displacement_border(const int* border, int len, double ux, double uy, int fix_mid_x, int fix_mid_y,
+                           unsigned char* mask, double* u)
+{
+    if (len < 1) return cp::fail(CP_ERR_INVALID, "cp_displacement_border: empty border");
+    const int end = len - (border[0] == border[len - 1] ? 1 : 0);
+    const double val[2] = {ux, uy};
+    const int mid[2] = {fix_mid_x, fix_mid_y};
+    for (int comp = 0; comp < 2; ++comp) {
+        if (mid[comp]) {
+            const int d = 2 * border[len / 2] + comp;
+            mask[d] = 0;
+            u[d] = val[comp];
+        } else {
+            for (int i = 0; i < end; ++i) {
+                const int d = 2 * border[i] + comp;
+                mask[d] = 0;
+                u[d] = val[comp];
+            }
+        }
+    }
+    return CP_OK;
+}
+
+int cp_csr_nnz(int nn, int ne, const int* conn, long long* nnz)
+{
+    cp::Adjacency adj;
+    if (int s = cp::build_adjacency(nn, ne, conn, adj)) return s;
+    *nnz = 4LL * (long long)adj.ptr[nn];
+    return CP_OK;
+}
+
+int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+{
+    cp::Adjacency adj;
+    if (int s = cp::build_adjacency(nn, ne, conn, adj)) return s;
+    cp::export_csr(adj, nn, rows, cols);
+    return CP_OK;
+}
+
+}  // extern "C"
+
+namespace cp {
+
+// Node adjacency in the reference's order (CudaSparseSLAE.cuh:339-391):
+// the node itself, then the other nodes of its elements in the order they
+// are met scanning elements in storage order (local order inside an
+// element), duplicates dropped.  Built per node from an element-ordered
+// incidence list, so the per-node work is independent (OpenMP).
+int build_adjacency(int nn, int ne, const int* conn, Adjacency& adj)
+{
+    for (long long k = 0; k < 4LL * ne; ++k)
+        if (conn[k] < 0 || conn[k] >= nn) return fail(CP_ERR_INVALID, "connectivity index out of range");
+    for (int e = 0; e < ne; ++e) {
+        const int* c = conn + 4 * e;
+        if (c[0] == c[1] || c[0] == c[2] || c[0] == c[3] || c[1] == c[2] || c[1] == c[3] || c[2] == c[3])
+            return fail(CP_ERR_INVALID, "degenerate element (repeated node)");
+    }
+    std::vector<int> incPtr(nn + 1, 0);
+    for (long long k = 0; k < 4LL * ne; ++k) ++incPtr[conn[k] + 1];
+    for (int n = 0; n < nn; ++n) incPtr[n + 1] += incPtr[n];
+    std::vector<int> inc(incPtr[nn]);  // (element << 2) | local index, element order per node
+    {
+        std::vector<int> fill(incPtr.begin(), incPtr.end() - 1);
+        for (int e = 0; e < ne; ++e)
+            for (int i = 0; i < 4; ++i) inc[fill[conn[4 * e + i]]++] = (e << 2) | i;
+    }
+    std::vector<int> cnt(nn + 1, 0);
+    std::vector<int> scratch(incPtr[nn] * 3 + nn + 1);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nn; ++n) {
+        int* list = scratch.data() + 3 * incPtr[n] + n;
+        int u = 0;
+        list[u++] = n;
+        for (int q = incPtr[n]; q < incPtr[n + 1]; ++q) {
+            const int e = inc[q] >> 2, i = inc[q] & 3;
+            for (int j = 0; j < 4; ++j) {
+                if (j == i) continue;
+                const int v = conn[4 * e + j];
+                bool seen = false;
+                for (int t = 1; t < u; ++t)
+                    if (list[t] == v) { seen = true; break; }
+                if (!seen) list[u++] = v;
+            }
+        }
+        cnt[n + 1] = u;
+    }
+    adj.ptr.assign(nn + 1, 0);
+    for (int n = 0; n < nn; ++n) adj.ptr[n + 1] = adj.ptr[n] + cnt[n + 1];
+    adj.node.resize(adj.ptr[nn]);
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < nn; ++n)
+        std::memcpy(adj.node.data() + adj.ptr[n], scratch.data() + 3 * incPtr[n] + n,
+                    sizeof(int) * (adj.ptr[n + 1] - adj.ptr[n]));
+    adj.incPtr.swap(incPtr);
+    adj.inc.swap(inc);
+    return CP_OK;
+}
+
+// Reference CSR layout (CudaSparseSLAE.cuh:414-440): rows 2n and 2n+1 both
+// list the columns 2m, 2m+1 for every m in adj(n) (self first).
+void export_csr(const Adjacency& adj, int nn, int* rows, int* cols)
+{
+    rows[0] = 0;
+    for (int n = 0; n < nn; ++n) {
+        const int len = 2 * (adj.ptr[n + 1] - adj.ptr[n]);
+        rows[2 * n + 1] = rows[2 * n] + len;
+        rows[2 * n + 2] = rows[2 * n + 1] + len;
+        for (int i = 0; i < 2; ++i) {
+            int* c = cols + rows[2 * n + i];
+            for (int k = adj.ptr[n]; k < adj.ptr[n + 1]; ++k) {
+                *c++ = 2 * adj.node[k];
+                *c++ = 2 * adj.node[k] + 1;
+            }
+        }
+    }
+}
+
+}  // namespace cp

@@ -1,0 +1,97 @@
+// cp_problem.h -- host-side view of one device-resident plasticity problem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "cp_abi.h"
+#include "cp_internal.h"
+#include "cp_kernels.h"
+
+#define CP_HIP(expr)                                                                                         \
+    do {                                                                                                     \
+        hipError_t _e = (expr);                                                                              \
+        if (_e != hipSuccess) return cp::failf(CP_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+struct cp_ctx {
+    int device;
+    hipDeviceProp_t prop;
+};
+
+namespace cp {
+
+// Bump allocator over one hipMalloc (256-byte aligned pieces).
+struct Arena {
+    char* base = nullptr;
+    size_t size = 0, used = 0;
+    template <typename T>
+    T* take(size_t count)
+    {
+        size_t bytes = (count * sizeof(T) + 255) & ~size_t(255);
+        T* p = reinterpret_cast<T*>(base + used);
+        used += bytes;
+        return p;
+    }
+};
+
+// Device memory groups of a problem: common data, the FP64 solve and the
+// mixed-precision solve (allocated on first use).
+struct Layout {
+    size_t common, fp64, mixed;
+};
+
+Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax);
+
+}  // namespace cp
+
+struct cp_problem {
+    cp_ctx* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = true;
+    int nn = 0, ne = 0, nslices = 0, max_w = 0;
+    int nbn = 0, nbe = 0;  // blocks over nodes / elements
+    long long nblk = 0, nblk_stored = 0;
+    cpk::MatParams M{};
+    double load = 1.0;
+    std::vector<double> rp0;   // host copy for ||b||^2
+    std::vector<int> adj_ptr, adj_node;
+    long long last_cg_iters = 0;
+    double last_cg_ms_per_iter = 0.;
+    bool have_fp64_values = false;
+
+    cp::Arena common, a64, amx;
+    // common
+    double2 *xy = nullptr, *ecnu = nullptr, *rp0d = nullptr, *u0d = nullptr, *b = nullptr, *u = nullptr;
+    int4* conn = nullptr;
+    uchar2* mask = nullptr;
+    double *psi = nullptr, *F = nullptr, *G = nullptr, *rcur = nullptr, *rprev = nullptr, *partials = nullptr;
+    int *inc_ptr = nullptr, *col = nullptr;
+    int2 *inc = nullptr, *slice = nullptr;
+    cpk::CgScal *sc64 = nullptr, *sc32 = nullptr;
+    cpk::RmScal* rms = nullptr;
+    cpk::RefScal* refs = nullptr;
+    // FP64 solve
+    double2 *va = nullptr, *vb = nullptr, *p64 = nullptr, *r64[2] = {}, *w64[2] = {}, *s64[2] = {};
+    // mixed solve
+    float4* vf = nullptr;
+    float2 *x32 = nullptr, *p32 = nullptr, *r32[2] = {}, *w32[2] = {}, *s32[2] = {}, *b32 = nullptr;
+    double2* res64 = nullptr;
+    // pinned host mirrors of the device scalars
+    cpk::CgScal* h_cg = nullptr;
+    cpk::RmScal* h_rm = nullptr;
+    cpk::RefScal* h_ref = nullptr;
+    hipEvent_t ev[8] = {};
+
+    cpk::CgArgs<double> cg64_args() const;
+    cpk::CgArgs<float> cg32_args() const;
+};
+
+namespace cp {
+int problem_alloc_solver(cp_problem* P, int precision);
+int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
+                   const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                   hipStream_t stream, cp_problem** out);
+int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st);
+}  // namespace cp

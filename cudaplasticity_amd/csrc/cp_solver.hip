@@ -1,0 +1,1015 @@
+// cp_solver.hip -- device problems and the secant (variable elasticity
+// parameters) loop of PlasticitySolver::solveCUDA (PlasticitySolver.cu:469-512)
+// on gfx950: node-centric assembly into 2x2-block sliced ELL, fused
+// single-launch CG (FP64, or FP32 with FP64 iterative refinement), element
+// return map with Aitken acceleration.  No host round trip inside a CG
+// iteration; the host polls a device flag between chunks of launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "cp_problem.h"
+
+namespace {
+thread_local std::string g_err;
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int blocks_for(long long n) { return (int)((n + cpk::kBlock - 1) / cpk::kBlock); }
+}  // namespace
+
+namespace cp {
+
+int fail(int code, const char* msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int failf(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax)
+{
+    auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+    Layout L{};
+    L.common = al(sizeof(double2) * nn) * 5 +      // xy, rp0, u0, b, u
+               al(sizeof(uchar2) * nn) +
+               al(sizeof(int4) * ne) + al(sizeof(double2) * ne) +
+               al(sizeof(double) * ne) * (4 + CP_NFIELDS) +   // psi, G, rcur, rprev, fields
+               al(sizeof(int) * (nn + 1)) + al(sizeof(int2) * ninc) +
+               al(sizeof(int) * nblk_stored) + al(sizeof(int2) * nslices) +
+               al(sizeof(double) * 5 * (size_t)nbmax) +
+               4 * 256;                                        // scalars
+    L.fp64 = al(sizeof(double2) * nblk_stored) * 2 + al(sizeof(double2) * nn) * 7;
+    L.mixed = al(sizeof(float4) * nblk_stored) + al(sizeof(float2) * nn) * 9 + al(sizeof(double2) * nn);
+    return L;
+}
+
+}  // namespace cp
+
+cpk::CgArgs<double> cp_problem::cg64_args() const
+{
+    cpk::CgArgs<double> a{};
+    a.col = col;
+    a.slice = slice;
+    a.va = va;
+    a.vb = vb;
+    a.x = u;
+    a.p = p64;
+    for (int i = 0; i < 2; ++i) {
+        a.r[i] = r64[i];
+        a.w[i] = w64[i];
+        a.s[i] = s64[i];
+    }
+    a.b = b;
+    a.mask = mask;
+    a.sc = sc64;
+    a.partials = partials;
+    a.nn = nn;
+    a.nblocks = nbn;
+    return a;
+}
+
+cpk::CgArgs<float> cp_problem::cg32_args() const
+{
+    cpk::CgArgs<float> a{};
+    a.col = col;
+    a.slice = slice;
+    a.va = vf;
+    a.vb = nullptr;
+    a.x = x32;
+    a.p = p32;
+    for (int i = 0; i < 2; ++i) {
+        a.r[i] = r32[i];
+        a.w[i] = w32[i];
+        a.s[i] = s32[i];
+    }
+    a.b = b32;
+    a.mask = mask;
+    a.sc = sc32;
+    a.partials = partials;
+    a.nn = nn;
+    a.nblocks = nbn;
+    return a;
+}
+
+namespace cp {
+
+int problem_alloc_solver(cp_problem* P, int precision)
+{
+    if (precision == CP_FP64) {
+        if (P->a64.base) return CP_OK;
+        Layout L = problem_layout(P->nn, P->ne, P->nblk_stored, 0, P->nslices, 0);
+        void* mem = nullptr;
+        if (hipMalloc(&mem, L.fp64) != hipSuccess) {
+            (void)hipGetLastError();
+            return failf(CP_ERR_NOMEM, "hipMalloc of %zu bytes (FP64 solver) failed", L.fp64);
+        }
+        P->a64.base = (char*)mem;
+        P->a64.size = L.fp64;
+        P->va = P->a64.take<double2>(P->nblk_stored);
+        P->vb = P->a64.take<double2>(P->nblk_stored);
+        P->p64 = P->a64.take<double2>(P->nn);
+        for (int i = 0; i < 2; ++i) {
+            P->r64[i] = P->a64.take<double2>(P->nn);
+            P->w64[i] = P->a64.take<double2>(P->nn);
+            P->s64[i] = P->a64.take<double2>(P->nn);
+        }
+        return CP_OK;
+    }
+    if (precision == CP_MIXED) {
+        if (P->amx.base) return CP_OK;
+        Layout L = problem_layout(P->nn, P->ne, P->nblk_stored, 0, P->nslices, 0);
+        void* mem = nullptr;
+        if (hipMalloc(&mem, L.mixed) != hipSuccess) {
+            (void)hipGetLastError();
+            return failf(CP_ERR_NOMEM, "hipMalloc of %zu bytes (mixed solver) failed", L.mixed);
+        }
+        P->amx.base = (char*)mem;
+        P->amx.size = L.mixed;
+        P->vf = P->amx.take<float4>(P->nblk_stored);
+        P->x32 = P->amx.take<float2>(P->nn);
+        P->p32 = P->amx.take<float2>(P->nn);
+        for (int i = 0; i < 2; ++i) {
+            P->r32[i] = P->amx.take<float2>(P->nn);
+            P->w32[i] = P->amx.take<float2>(P->nn);
+            P->s32[i] = P->amx.take<float2>(P->nn);
+        }
+        P->b32 = P->amx.take<float2>(P->nn);
+        P->res64 = P->amx.take<double2>(P->nn);
+        return CP_OK;
+    }
+    return failf(CP_ERR_INVALID, "unknown precision %d", precision);
+}
+
+static int max_width_instantiated(int w)
+{
+    if (w <= 9) return 9;
+    if (w <= 12) return 12;
+    if (w <= 16) return 16;
+    if (w <= 24) return 24;
+    if (w <= 32) return 32;
+    return 0;
+}
+
+int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
+                   const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                   hipStream_t stream, cp_problem** out)
+{
+    *out = nullptr;
+    if (!ctx || nn < 1 || ne < 1 || !x || !y || !conn || !rp || !mask || !mat)
+        return fail(CP_ERR_INVALID, "cp_problem_create: null argument or empty mesh");
+    if (!(mat->E > 0) || !(mat->sigmaT > 0) || mat->law < 0 || mat->law > 2 || mat->plane < 0 || mat->plane > 1)
+        return fail(CP_ERR_INVALID, "cp_problem_create: invalid material");
+    if (nn > (1 << 30) / 2 || ne > (1 << 29)) return fail(CP_ERR_INVALID, "cp_problem_create: mesh too large");
+    Adjacency adj;
+    if (int s = build_adjacency(nn, ne, conn, adj)) return s;
+
+    std::unique_ptr<cp_problem> P(new cp_problem());
+    P->ctx = ctx;
+    P->nn = nn;
+    P->ne = ne;
+    P->M = cpk::MatParams{mat->E, mat->nu, mat->h, mat->sigmaT, mat->law, mat->plane, mat->K_T, mat->A, mat->m, mat->n};
+    P->nslices = (nn + cpk::kWave - 1) / cpk::kWave;
+    P->nbn = blocks_for(nn);
+    P->nbe = blocks_for(ne);
+
+    // sliced ELL layout
+    std::vector<int2> slices(P->nslices);
+    long long off = 0;
+    int maxw = 0;
+    for (int s = 0; s < P->nslices; ++s) {
+        int w = 0;
+        for (int n = s * cpk::kWave; n < std::min(nn, (s + 1) * cpk::kWave); ++n)
+            w = std::max(w, adj.ptr[n + 1] - adj.ptr[n]);
+        if (off > (long long)INT32_MAX - (long long)w * cpk::kWave)
+            return fail(CP_ERR_INVALID, "cp_problem_create: matrix too large for 32-bit slice offsets");
+        slices[s] = make_int2((int)off, w);
+        off += (long long)w * cpk::kWave;
+        maxw = std::max(maxw, w);
+    }
+    P->nblk_stored = off;
+    P->nblk = adj.ptr[nn];
+    P->max_w = maxw;
+    if (!max_width_instantiated(maxw))
+        return failf(CP_ERR_INVALID, "cp_problem_create: node with %d neighbours (max 31 supported)", maxw - 1);
+    std::vector<int> hcol(off);
+    for (int s = 0; s < P->nslices; ++s)
+        for (int lane = 0; lane < cpk::kWave; ++lane) {
+            const int n = s * cpk::kWave + lane;
+            for (int k = 0; k < slices[s].y; ++k) {
+                int c = n < nn ? n : 0;
+                if (n < nn && k < adj.ptr[n + 1] - adj.ptr[n]) c = adj.node[adj.ptr[n] + k];
+                hcol[(size_t)slices[s].x + (size_t)k * cpk::kWave + lane] = c;
+            }
+        }
+    // node -> (element, slots of K_e(i, j) in the node's row)
+    const long long ninc = adj.incPtr[nn];
+    std::vector<int2> hinc(ninc);
+    for (int n = 0; n < nn; ++n) {
+        const int* row = adj.node.data() + adj.ptr[n];
+        const int deg = adj.ptr[n + 1] - adj.ptr[n];
+        for (int q = adj.incPtr[n]; q < adj.incPtr[n + 1]; ++q) {
+            const int e = adj.inc[q] >> 2;
+            unsigned packed = 0;
+            for (int j = 0; j < 4; ++j) {
+                const int v = conn[4 * e + j];
+                int k = 0;
+                while (k < deg && row[k] != v) ++k;
+                packed |= (unsigned)k << (8 * j);
+            }
+            hinc[q] = make_int2(e, (int)packed);
+        }
+    }
+
+    Layout L = problem_layout(nn, ne, P->nblk_stored, ninc, P->nslices, std::max(P->nbn, P->nbe));
+    void* mem = nullptr;
+    if (hipMalloc(&mem, L.common) != hipSuccess) {
+        (void)hipGetLastError();
+        return failf(CP_ERR_NOMEM, "hipMalloc of %zu bytes failed", L.common);
+    }
+    P->common.base = (char*)mem;
+    P->common.size = L.common;
+    Arena& A = P->common;
+    P->xy = A.take<double2>(nn);
+    P->rp0d = A.take<double2>(nn);
+    P->u0d = A.take<double2>(nn);
+    P->b = A.take<double2>(nn);
+    P->u = A.take<double2>(nn);
+    P->mask = A.take<uchar2>(nn);
+    P->conn = A.take<int4>(ne);
+    P->ecnu = A.take<double2>(ne);
+    P->psi = A.take<double>(ne);
+    P->G = A.take<double>(ne);
+    P->rcur = A.take<double>(ne);
+    P->rprev = A.take<double>(ne);
+    P->F = A.take<double>((size_t)ne * CP_NFIELDS);
+    P->inc_ptr = A.take<int>(nn + 1);
+    P->inc = A.take<int2>(ninc);
+    P->col = A.take<int>(P->nblk_stored);
+    P->slice = A.take<int2>(P->nslices);
+    P->partials = A.take<double>(5 * (size_t)std::max(P->nbn, P->nbe));
+    P->sc64 = A.take<cpk::CgScal>(1);
+    P->sc32 = A.take<cpk::CgScal>(1);
+    P->rms = A.take<cpk::RmScal>(1);
+    P->refs = A.take<cpk::RefScal>(1);
+
+    if (stream) {
+        P->stream = stream;
+        P->own_stream = false;
+    } else {
+        CP_HIP(hipStreamCreateWithFlags(&P->stream, hipStreamNonBlocking));
+    }
+    for (auto& e : P->ev) CP_HIP(hipEventCreate(&e));
+    void* pinned = nullptr;
+    CP_HIP(hipHostMalloc(&pinned, 1024, hipHostMallocDefault));
+    P->h_cg = (cpk::CgScal*)pinned;
+    P->h_rm = (cpk::RmScal*)((char*)pinned + 256);
+    P->h_ref = (cpk::RefScal*)((char*)pinned + 512);
+
+    std::vector<double2> hxy(nn), hrp(nn), hu0(nn);
+    std::vector<uchar2> hmask(nn);
+    for (int n = 0; n < nn; ++n) {
+        hxy[n] = make_double2(x[n], y[n]);
+        hrp[n] = make_double2(rp[2 * n], rp[2 * n + 1]);
+        hu0[n] = u0 ? make_double2(u0[2 * n], u0[2 * n + 1]) : make_double2(0., 0.);
+        hmask[n] = make_uchar2(mask[2 * n] ? 1 : 0, mask[2 * n + 1] ? 1 : 0);
+    }
+    hipStream_t st = P->stream;
+    CP_HIP(hipMemcpyAsync(P->xy, hxy.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->rp0d, hrp.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->u0d, hu0.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->mask, hmask.data(), sizeof(uchar2) * nn, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->conn, conn, sizeof(int4) * ne, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->inc_ptr, adj.incPtr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->inc, hinc.data(), sizeof(int2) * ninc, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->col, hcol.data(), sizeof(int) * off, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->slice, slices.data(), sizeof(int2) * P->nslices, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemsetAsync(P->F, 0, sizeof(double) * (size_t)ne * CP_NFIELDS, st));
+    CP_HIP(hipMemsetAsync(P->sc64, 0, sizeof(cpk::CgScal), st));
+    CP_HIP(hipMemsetAsync(P->sc32, 0, sizeof(cpk::CgScal), st));
+    CP_HIP(hipMemsetAsync(P->rms, 0, sizeof(cpk::RmScal), st));
+    CP_HIP(hipMemsetAsync(P->refs, 0, sizeof(cpk::RefScal), st));
+    const int nbe = P->nbe, nbn = P->nbn;
+    hipLaunchKernelGGL(cpk::k_init_state, dim3(nbe), dim3(cpk::kBlock), 0, st, P->psi, P->ecnu, P->rprev, P->M.E,
+                       P->M.nu, ne);
+    hipLaunchKernelGGL(cpk::k_set_load, dim3(nbn), dim3(cpk::kBlock), 0, st, P->b, P->u, P->rp0d, P->u0d, P->mask,
+                       1.0, 1, nn);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipStreamSynchronize(st));
+    P->rp0.assign(rp, rp + 2 * (size_t)nn);
+    P->adj_ptr.swap(adj.ptr);
+    P->adj_node.swap(adj.node);
+    *out = P.release();
+    return CP_OK;
+}
+
+// ---------------------------------------------------------------- kernels drivers
+
+static int launch_assemble(cp_problem* P, int precision)
+{
+    if (int s = problem_alloc_solver(P, precision)) return s;
+    cpk::AsmArgs a{};
+    a.incPtr = P->inc_ptr;
+    a.inc = P->inc;
+    a.conn = P->conn;
+    a.xy = P->xy;
+    a.ecnu = P->ecnu;
+    a.slice = P->slice;
+    a.va = P->va;
+    a.vb = P->vb;
+    a.vf = P->vf;
+    a.nn = P->nn;
+    a.nblocks = P->nbn;
+    a.plane = P->M.plane;
+    a.h = P->M.h;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    const bool f64 = precision == CP_FP64;
+    switch (max_width_instantiated(P->max_w)) {
+#define CP_ASM_CASE(W)                                                                                  \
+    case W:                                                                                             \
+        if (f64) hipLaunchKernelGGL((cpk::k_assemble<W, true>), g, b, 0, P->stream, a);                 \
+        else hipLaunchKernelGGL((cpk::k_assemble<W, false>), g, b, 0, P->stream, a);                   \
+        break;
+        CP_ASM_CASE(9)
+        CP_ASM_CASE(12)
+        CP_ASM_CASE(16)
+        CP_ASM_CASE(24)
+        CP_ASM_CASE(32)
+#undef CP_ASM_CASE
+    default:
+        return fail(CP_ERR_INVALID, "row width too large");
+    }
+    CP_HIP(hipGetLastError());
+    if (f64) P->have_fp64_values = true;
+    return CP_OK;
+}
+
+static double host_bnorm2(const cp_problem* P)
+{
+    double s = 0.;
+    for (double v : P->rp0) s += (P->load * v) * (P->load * v);
+    return s;
+}
+
+// Runs a CG solve whose setup kernels have been launched: chunks of
+// iteration launches, polling the done flag between chunks.
+template <typename T>
+static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
+                             long long* iters, int* status, double* ms)
+{
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    CP_HIP(hipEventRecord(P->ev[6], st));
+    long long launched = 0;
+    int chunk = (int)std::min<long long>(4096, std::max<long long>(16, P->last_cg_iters / 2));
+    double g_prev = -1.;
+    long long it_prev = 0;
+    for (;;) {
+        for (int j = 0; j < chunk; ++j) hipLaunchKernelGGL(cpk::k_cg_iter<T>, g, b, 0, st, a);
+        CP_HIP(hipGetLastError());
+        launched += chunk;
+        CP_HIP(hipMemcpyAsync(P->h_cg, dsc, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        const cpk::CgScal h = *P->h_cg;
+        if (h.done) break;
+        // predict the remaining iterations from the observed decay of gamma
+        const double target = tol2 * (bnorm2 > 0. ? bnorm2 : h.bnorm2);
+        int next = std::min(4096, chunk * 2);
+        if (g_prev > 0. && h.gamma > 0. && h.gamma < g_prev && target > 0.) {
+            const double rate = std::log(g_prev / h.gamma) / double(h.iters - it_prev);
+            const double rem = std::log(h.gamma / target) / rate;
+            if (rem > 0 && rem < 1e7) next = (int)std::min(4096., std::max(8., rem * 1.05 + 4.));
+        }
+        g_prev = h.gamma;
+        it_prev = h.iters;
+        chunk = next;
+        if (launched > (long long)h.maxit + 8192) break;  // the kernel stops itself at maxit
+    }
+    CP_HIP(hipEventRecord(P->ev[7], st));
+    CP_HIP(hipEventSynchronize(P->ev[7]));
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[6], P->ev[7]));
+    *ms = t;
+    *iters = P->h_cg->iters;
+    *status = P->h_cg->status;
+    return CP_OK;
+}
+
+// One masked CG solve of K u = b from the current u (FP64).
+static int cg_fp64(cp_problem* P, double tol, int maxit, long long* iters, double* relres, double* ms)
+{
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    const double bn = host_bnorm2(P);
+    auto a = P->cg64_args();
+    hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, bn, tol * tol, maxit);
+    hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
+    hipLaunchKernelGGL(cpk::k_cg_init2<double>, g, b, 0, st, a);
+    CP_HIP(hipGetLastError());
+    int status = 0;
+    if (int s = run_cg_iterations<double>(P, a, P->sc64, tol * tol, bn, iters, &status, ms)) return s;
+    const cpk::CgScal h = *P->h_cg;
+    if (relres) *relres = std::sqrt(h.gamma / (h.bnorm2 > 0 ? h.bnorm2 : 1.));
+    P->last_cg_iters = *iters;
+    if (*iters > 0) P->last_cg_ms_per_iter = *ms / double(*iters);
+    if (status == 2) return fail(CP_ERR_NAN, "CG: NaN in the residual");
+    if (status == 1) return failf(CP_ERR_NOT_CONVERGED, "CG: %d iterations without convergence", maxit);
+    return CP_OK;
+}
+
+// FP32 CG on K32 d = r64 with FP64 iterative refinement of u.
+static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long long* iters, long long* refs,
+                    double* relres, double* ms)
+{
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    const double bn = host_bnorm2(P);
+    cpk::ResArgs ra{};
+    ra.incPtr = P->inc_ptr;
+    ra.inc = P->inc;
+    ra.conn = P->conn;
+    ra.xy = P->xy;
+    ra.ecnu = P->ecnu;
+    ra.u = P->u;
+    ra.b = P->b;
+    ra.mask = P->mask;
+    ra.r64 = P->res64;
+    ra.r32 = P->b32;
+    ra.sc = P->refs;
+    ra.partials = P->partials;
+    ra.nn = P->nn;
+    ra.nblocks = P->nbn;
+    ra.plane = P->M.plane;
+    ra.h = P->M.h;
+    auto a = P->cg32_args();
+    *iters = 0;
+    *refs = 0;
+    *ms = 0.;
+    double ref_norm = bn;
+    int stalled = 0;
+    double prev = -1.;
+    for (int k = 0; k < 200; ++k) {
+        hipLaunchKernelGGL(cpk::k_residual64, g, b, 0, st, ra);
+        CP_HIP(hipGetLastError());
+        CP_HIP(hipMemcpyAsync(P->h_ref, P->refs, sizeof(cpk::RefScal), hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        const double rn = P->h_ref->rnorm2;
+        ++*refs;
+        if (rn != rn) return fail(CP_ERR_NAN, "refinement: NaN residual");
+        if (ref_norm == 0.) ref_norm = rn;
+        if (relres) *relres = ref_norm > 0 ? std::sqrt(rn / ref_norm) : 0.;
+        if (rn == 0. || rn <= tol * tol * ref_norm) return CP_OK;
+        if (prev > 0. && rn > 0.25 * prev) {
+            if (++stalled >= 3) return failf(CP_ERR_NOT_CONVERGED, "refinement stalled at relative residual %.3e",
+                                             std::sqrt(rn / ref_norm));
+        }
+        prev = rn;
+        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, rn, inner_tol * inner_tol, maxit);
+        hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
+        hipLaunchKernelGGL(cpk::k_cg_init2<float>, g, b, 0, st, a);
+        CP_HIP(hipGetLastError());
+        long long it = 0;
+        int status = 0;
+        double t = 0.;
+        if (int s = run_cg_iterations<float>(P, a, P->sc32, inner_tol * inner_tol, rn, &it, &status, &t)) return s;
+        *iters += it;
+        *ms += t;
+        P->last_cg_iters = it;
+        if (it > 0) P->last_cg_ms_per_iter = t / double(it);
+        if (status == 2) return fail(CP_ERR_NAN, "inner CG: NaN");
+        hipLaunchKernelGGL(cpk::k_add_correction, g, b, 0, st, P->u, (const float2*)P->x32, P->nn, P->nbn);
+        CP_HIP(hipGetLastError());
+    }
+    return fail(CP_ERR_NOT_CONVERGED, "refinement: 200 passes without convergence");
+}
+
+static int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max_outer, int elastic,
+                             double tol)
+{
+    cpk::RmArgs a{};
+    a.conn = P->conn;
+    a.xy = P->xy;
+    a.u = P->u;
+    a.ecnu = P->ecnu;
+    a.psi = P->psi;
+    a.F = P->F;
+    a.G = P->G;
+    a.rcur = P->rcur;
+    a.rprev = P->rprev;
+    a.sc = P->rms;
+    a.partials = P->partials;
+    a.M = P->M;
+    a.ne = P->ne;
+    a.nblocks = P->nbe;
+    a.k = k;
+    a.aitken = aitken;
+    a.fixed_outer = fixed_outer;
+    a.max_outer = max_outer;
+    a.elastic = elastic;
+    a.tol = tol;
+    const dim3 g(P->nbe), b(cpk::kBlock);
+    hipLaunchKernelGGL(cpk::k_return_map, g, b, 0, P->stream, a);
+    hipLaunchKernelGGL(cpk::k_apply_update, g, b, 0, P->stream, P->psi, P->ecnu, (const double*)P->G,
+                       (const double*)P->rcur, P->rprev, (const cpk::RmScal*)P->rms, P->M.E, P->M.nu, P->ne, P->nbe,
+                       aitken);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipMemcpyAsync(P->h_rm, P->rms, sizeof(cpk::RmScal), hipMemcpyDeviceToHost, P->stream));
+    return CP_OK;
+}
+
+static int set_load(cp_problem* P, double factor, int reset)
+{
+    P->load = factor;
+    hipLaunchKernelGGL(cpk::k_set_load, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, P->b, P->u, P->rp0d, P->u0d,
+                       P->mask, factor, reset, P->nn);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+static int reset_state(cp_problem* P)
+{
+    hipLaunchKernelGGL(cpk::k_init_state, dim3(P->nbe), dim3(cpk::kBlock), 0, P->stream, P->psi, P->ecnu, P->rprev,
+                       P->M.E, P->M.nu, P->ne);
+    CP_HIP(hipGetLastError());
+    return set_load(P, P->load, 1);
+}
+
+// One load level: the secant loop of solveCUDA (PlasticitySolver.cu:487-503)
+// with caps, optional Aitken steps and the mixed-precision linear solve.
+int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
+{
+    cp_stats S{};
+    const double t0 = now_s();
+    const int prec = opt->precision;
+    if (prec != CP_FP64 && prec != CP_MIXED) return fail(CP_ERR_INVALID, "unknown precision");
+    if (int s = problem_alloc_solver(P, prec)) return s;
+    hipStream_t stream = P->stream;
+    if (!opt->warm)
+        if (int s = reset_state(P)) return s;
+    hipLaunchKernelGGL(cpk::k_rm_reset, dim3(1), dim3(1), 0, stream, P->rms);
+    const int max_outer = opt->fixed_outer > 0 ? opt->fixed_outer : (elastic ? 1 : std::max(1, opt->max_outer));
+    const int maxit = opt->cg_maxit > 0 ? opt->cg_maxit : 1000000000;
+    const double inner = opt->inner_tol > 0 ? opt->inner_tol : 1e-3;
+    int rc = CP_OK;
+    for (int k = 0; k < max_outer; ++k) {
+        CP_HIP(hipEventRecord(P->ev[0], stream));
+        if (int s = launch_assemble(P, prec)) return s;
+        CP_HIP(hipEventRecord(P->ev[1], stream));
+        long long it = 0, refs = 0;
+        double relres = 0., cgms = 0.;
+        int s = prec == CP_FP64 ? cg_fp64(P, opt->cg_tol, maxit, &it, &relres, &cgms)
+                                : cg_mixed(P, opt->cg_tol, maxit, inner, &it, &refs, &relres, &cgms);
+        S.cg_iters += it;
+        S.refinements += refs;
+        S.cg_seconds += cgms * 1e-3;
+        if (s == CP_ERR_NAN) {
+            rc = s;
+            S.status = CP_ERR_NAN;
+            S.outer_iters = k + 1;
+            break;
+        }
+        if (s && s != CP_ERR_NOT_CONVERGED) return s;
+        CP_HIP(hipEventRecord(P->ev[2], stream));
+        if (int q = launch_return_map(P, k, opt->aitken, opt->fixed_outer, max_outer, elastic, opt->tol)) return q;
+        CP_HIP(hipEventRecord(P->ev[3], stream));
+        CP_HIP(hipStreamSynchronize(stream));
+        float ta = 0.f, tr = 0.f;
+        CP_HIP(hipEventElapsedTime(&ta, P->ev[0], P->ev[1]));
+        CP_HIP(hipEventElapsedTime(&tr, P->ev[2], P->ev[3]));
+        S.asm_seconds += ta * 1e-3;
+        S.rm_seconds += tr * 1e-3;
+        const cpk::RmScal h = *P->h_rm;
+        S.outer_iters = k + 1;
+        S.rel_err = h.rel_err;
+        if (h.status == 2) {
+            rc = fail(CP_ERR_NAN, "NaN in the secant update");
+            S.status = CP_ERR_NAN;
+            break;
+        }
+        if (h.last) {
+            S.status = h.converged ? CP_OK : CP_ERR_NOT_CONVERGED;
+            break;
+        }
+    }
+    if (!elastic && S.status != CP_ERR_NAN) {
+        hipLaunchKernelGGL(cpk::k_plastic_strains, dim3(P->nbe), dim3(cpk::kBlock), 0, stream, P->F, P->psi,
+                           (const double2*)P->ecnu, P->M.E, P->M.nu, P->ne, P->nbe);
+        CP_HIP(hipGetLastError());
+    }
+    CP_HIP(hipStreamSynchronize(stream));
+    S.seconds = now_s() - t0;
+    if (st) *st = S;
+    if (rc) return rc;
+    if (S.status == CP_ERR_NOT_CONVERGED) return fail(CP_ERR_NOT_CONVERGED, "secant loop hit its iteration cap");
+    return CP_OK;
+}
+
+}  // namespace cp
+
+// ====================================================================== C-ABI
+
+extern "C" {
+
+const char* cp_last_error(void) { return g_err.c_str(); }
+
+int cp_abi_version(void) { return CP_ABI_VERSION; }
+
+int cp_ctx_create(int device, cp_ctx** out)
+{
+    if (!out) return cp::fail(CP_ERR_INVALID, "cp_ctx_create: null out");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return cp::fail(CP_ERR_NO_DEVICE, "no HIP device visible");
+    }
+    if (device < 0 || device >= n) return cp::failf(CP_ERR_INVALID, "device %d out of range (%d devices)", device, n);
+    CP_HIP(hipSetDevice(device));
+    cp_ctx* c = new cp_ctx();
+    c->device = device;
+    if (hipGetDeviceProperties(&c->prop, device) != hipSuccess) {
+        delete c;
+        return cp::fail(CP_ERR_DEVICE, "hipGetDeviceProperties failed");
+    }
+    *out = c;
+    return CP_OK;
+}
+
+int cp_ctx_destroy(cp_ctx* ctx)
+{
+    delete ctx;
+    return CP_OK;
+}
+
+int cp_device_info(cp_ctx* ctx, char* name, int name_len, size_t* total_mem, int* cu_count)
+{
+    if (!ctx) return cp::fail(CP_ERR_INVALID, "null ctx");
+    if (name && name_len > 0) {
+        std::snprintf(name, name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+    }
+    if (total_mem) *total_mem = ctx->prop.totalGlobalMem;
+    if (cu_count) *cu_count = ctx->prop.multiProcessorCount;
+    return CP_OK;
+}
+
+int cp_problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
+                      const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                      cp_problem** out)
+{
+    if (!out) return cp::fail(CP_ERR_INVALID, "null out");
+    if (ctx) CP_HIP(hipSetDevice(ctx->device));
+    return cp::problem_create(ctx, nn, ne, x, y, conn, rp, u0, mask, mat, nullptr, out);
+}
+
+int cp_problem_destroy(cp_problem* P)
+{
+    if (!P) return CP_OK;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    for (auto& e : P->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (P->h_cg) (void)hipHostFree(P->h_cg);
+    if (P->common.base) (void)hipFree(P->common.base);
+    if (P->a64.base) (void)hipFree(P->a64.base);
+    if (P->amx.base) (void)hipFree(P->amx.base);
+    if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
+    delete P;
+    return CP_OK;
+}
+
+int cp_problem_bytes(int nn, int ne, long long nnz, int precision, size_t* bytes)
+{
+    if (!bytes || nn < 1 || ne < 1) return cp::fail(CP_ERR_INVALID, "cp_problem_bytes: bad arguments");
+    // stored blocks ~ true blocks (nnz/4) plus slice padding; 4 incidences per element
+    const long long nblk = nnz / 4 + (long long)nn / 8 + 64 * 16;
+    const int nb = blocks_for(std::max(nn, ne));
+    cp::Layout L = cp::problem_layout(nn, ne, nblk, 4LL * ne, (nn + 63) / 64, nb);
+    *bytes = L.common + (precision == CP_FP64 ? L.fp64 : L.mixed);
+    return CP_OK;
+}
+
+int cp_problem_csr_nnz(cp_problem* P, long long* nnz)
+{
+    if (!P || !nnz) return cp::fail(CP_ERR_INVALID, "null argument");
+    *nnz = 4LL * P->nblk;
+    return CP_OK;
+}
+
+int cp_problem_get_csr(cp_problem* P, int* rows, int* cols)
+{
+    if (!P || !rows || !cols) return cp::fail(CP_ERR_INVALID, "null argument");
+    cp::Adjacency adj;
+    adj.ptr = P->adj_ptr;
+    adj.node = P->adj_node;
+    cp::export_csr(adj, P->nn, rows, cols);
+    return CP_OK;
+}
+
+int cp_problem_get_values(cp_problem* P, double* vals)
+{
+    if (!P || !vals) return cp::fail(CP_ERR_INVALID, "null argument");
+    if (!P->have_fp64_values) return cp::fail(CP_ERR_STATE, "no FP64 assembly has run yet");
+    std::vector<double2> a(P->nblk_stored), b(P->nblk_stored);
+    std::vector<int2> sl(P->nslices);
+    CP_HIP(hipMemcpyAsync(a.data(), P->va, sizeof(double2) * P->nblk_stored, hipMemcpyDeviceToHost, P->stream));
+    CP_HIP(hipMemcpyAsync(b.data(), P->vb, sizeof(double2) * P->nblk_stored, hipMemcpyDeviceToHost, P->stream));
+    CP_HIP(hipMemcpyAsync(sl.data(), P->slice, sizeof(int2) * P->nslices, hipMemcpyDeviceToHost, P->stream));
+    CP_HIP(hipStreamSynchronize(P->stream));
+    long long row0 = 0;
+    for (int n = 0; n < P->nn; ++n) {
+        const int deg = P->adj_ptr[n + 1] - P->adj_ptr[n];
+        const long long row1 = row0 + 2 * deg;
+        const size_t base = (size_t)sl[n >> 6].x + (n & 63);
+        for (int k = 0; k < deg; ++k) {
+            const size_t q = base + (size_t)k * cpk::kWave;
+            vals[row0 + 2 * k] = a[q].x;
+            vals[row0 + 2 * k + 1] = a[q].y;
+            vals[row1 + 2 * k] = b[q].x;
+            vals[row1 + 2 * k + 1] = b[q].y;
+        }
+        row0 = row1 + 2 * deg;
+    }
+    return CP_OK;
+}
+
+int cp_set_load(cp_problem* P, double factor)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (int s = cp::set_load(P, factor, 0)) return s;
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
+int cp_reset_state(cp_problem* P)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (int s = cp::reset_state(P)) return s;
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
+int cp_assemble(cp_problem* P, int precision)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (int s = cp::launch_assemble(P, precision)) return s;
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
+int cp_cg(cp_problem* P, double tol, int maxit, int precision, double inner_tol, long long* iters, double* relres)
+{
+    if (!P || !iters) return cp::fail(CP_ERR_INVALID, "null argument");
+    if (int s = cp::problem_alloc_solver(P, precision)) return s;
+    double ms = 0.;
+    if (maxit <= 0) maxit = 1000000000;
+    if (precision == CP_FP64) return cp::cg_fp64(P, tol, maxit, iters, relres, &ms);
+    long long refs = 0;
+    return cp::cg_mixed(P, tol, maxit, inner_tol > 0 ? inner_tol : 1e-3, iters, &refs, relres, &ms);
+}
+
+int cp_return_map(cp_problem* P, double* rel_err)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    hipLaunchKernelGGL(cpk::k_rm_reset, dim3(1), dim3(1), 0, P->stream, P->rms);
+    if (int s = cp::launch_return_map(P, 0, 0, 0, 1, 1, 0.)) return s;
+    CP_HIP(hipStreamSynchronize(P->stream));
+    if (rel_err) *rel_err = P->h_rm->rel_err;
+    return P->h_rm->status == 2 ? cp::fail(CP_ERR_NAN, "NaN in the secant update") : CP_OK;
+}
+
+int cp_plastic_strains(cp_problem* P)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    hipLaunchKernelGGL(cpk::k_plastic_strains, dim3(P->nbe), dim3(cpk::kBlock), 0, P->stream, P->F, P->psi,
+                       (const double2*)P->ecnu, P->M.E, P->M.nu, P->ne, P->nbe);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
+int cp_internal_force(cp_problem* P, double* f_host)
+{
+    if (!P || !f_host) return cp::fail(CP_ERR_INVALID, "null argument");
+    if (int s = cp::problem_alloc_solver(P, CP_MIXED)) return s;
+    cpk::ResArgs ra{};
+    ra.incPtr = P->inc_ptr;
+    ra.inc = P->inc;
+    ra.conn = P->conn;
+    ra.xy = P->xy;
+    ra.ecnu = P->ecnu;
+    ra.u = P->u;
+    ra.r64 = P->res64;
+    ra.nn = P->nn;
+    ra.nblocks = P->nbn;
+    ra.plane = P->M.plane;
+    ra.h = P->M.h;
+    hipLaunchKernelGGL(cpk::k_residual64, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, ra);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipMemcpyAsync(f_host, P->res64, sizeof(double2) * P->nn, hipMemcpyDeviceToHost, P->stream));
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
+static cp_options defaults_if_null(const cp_options* o)
+{
+    cp_options d{};
+    d.cg_tol = 1e-7;
+    d.cg_maxit = 0;
+    d.tol = 1e-5;
+    d.max_outer = 1000;
+    d.precision = CP_FP64;
+    d.inner_tol = 1e-3;
+    return o ? *o : d;
+}
+
+int cp_solve(cp_problem* P, const cp_options* opt, cp_stats* stats)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    cp_options o = defaults_if_null(opt);
+    return cp::solve_level(P, &o, 0, stats);
+}
+
+int cp_solve_elastic(cp_problem* P, const cp_options* opt, cp_stats* stats)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    cp_options o = defaults_if_null(opt);
+    return cp::solve_level(P, &o, 1, stats);
+}
+
+int cp_solve_steps(cp_problem* P, int nsteps, double final_factor, const cp_options* opt, cp_stats* per_step)
+{
+    if (!P || nsteps < 1) return cp::fail(CP_ERR_INVALID, "cp_solve_steps: bad arguments");
+    cp_options o = defaults_if_null(opt);
+    const bool warm0 = o.warm != 0;
+    for (int n = 1; n <= nsteps; ++n) {
+        const double lam = final_factor * double(n) / double(nsteps);
+        if (int s = cp::set_load(P, lam, 0)) return s;
+        o.warm = (n > 1 || warm0) ? 1 : 0;
+        if (!o.warm) P->load = lam;
+        cp_stats st{};
+        int s = cp::solve_level(P, &o, 0, &st);
+        if (per_step) per_step[n - 1] = st;
+        if (s) return s;
+    }
+    return CP_OK;
+}
+
+int cp_download(cp_problem* P, int field, double* host)
+{
+    if (!P || !host) return cp::fail(CP_ERR_INVALID, "null argument");
+    hipStream_t st = P->stream;
+    if (field == CP_F_U || field == CP_F_RHS) {
+        CP_HIP(hipMemcpyAsync(host, field == CP_F_U ? P->u : P->b, sizeof(double2) * P->nn, hipMemcpyDeviceToHost, st));
+    } else if (field == CP_F_EC || field == CP_F_NUC) {
+        std::vector<double2> t(P->ne);
+        CP_HIP(hipMemcpyAsync(t.data(), P->ecnu, sizeof(double2) * P->ne, hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        for (int e = 0; e < P->ne; ++e) host[e] = field == CP_F_EC ? t[e].x : t[e].y;
+        return CP_OK;
+    } else if (field == CP_F_PSI) {
+        CP_HIP(hipMemcpyAsync(host, P->psi, sizeof(double) * P->ne, hipMemcpyDeviceToHost, st));
+    } else if (field >= 0 && field < CP_NFIELDS) {
+        CP_HIP(hipMemcpyAsync(host, P->F + (size_t)field * P->ne, sizeof(double) * P->ne, hipMemcpyDeviceToHost, st));
+    } else {
+        return cp::failf(CP_ERR_INVALID, "unknown field %d", field);
+    }
+    CP_HIP(hipStreamSynchronize(st));
+    return CP_OK;
+}
+
+int cp_upload_state(cp_problem* P, const double* u, const double* psi)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    hipStream_t st = P->stream;
+    if (u) CP_HIP(hipMemcpyAsync(P->u, u, sizeof(double2) * P->nn, hipMemcpyHostToDevice, st));
+    std::vector<double2> en;
+    if (psi) {
+        en.resize(P->ne);
+        for (int e = 0; e < P->ne; ++e) {
+            if (!(psi[e] > 0)) return cp::fail(CP_ERR_INVALID, "psi must be positive");
+            en[e] = make_double2(P->M.E / psi[e], 0.5 * (1. - (1. - 2. * P->M.nu) / psi[e]));
+        }
+        CP_HIP(hipMemcpyAsync(P->psi, psi, sizeof(double) * P->ne, hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(P->ecnu, en.data(), sizeof(double2) * P->ne, hipMemcpyHostToDevice, st));
+    }
+    CP_HIP(hipStreamSynchronize(st));
+    return CP_OK;
+}
+
+int cp_last_cg_kernel_ms(cp_problem* P, double* ms)
+{
+    if (!P || !ms) return cp::fail(CP_ERR_INVALID, "null argument");
+    *ms = P->last_cg_ms_per_iter;
+    return CP_OK;
+}
+
+int cp_problem_layout(cp_problem* P, long long* nblk_stored, long long* nblk, int* max_width)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (nblk_stored) *nblk_stored = P->nblk_stored;
+    if (nblk) *nblk = P->nblk;
+    if (max_width) *max_width = P->max_w;
+    return CP_OK;
+}
+
+int cp_bench_spmv(cp_problem* P, int precision, int reps, double* ms, double* bytes_stored, double* bytes_algo)
+{
+    if (!P || reps < 1) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (int s = cp::launch_assemble(P, precision)) return s;
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    const double s = precision == CP_FP64 ? 8. : 4.;
+    CP_HIP(hipEventRecord(P->ev[4], st));
+    if (precision == CP_FP64) {
+        auto a = P->cg64_args();
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(cpk::k_spmv<double>, g, b, 0, st, a, (const double2*)P->u, P->w64[i & 1]);
+    } else {
+        auto a = P->cg32_args();
+        CP_HIP(hipMemsetAsync(P->x32, 0, sizeof(float2) * P->nn, st));
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(cpk::k_spmv<float>, g, b, 0, st, a, (const float2*)P->x32, P->w32[i & 1]);
+    }
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipEventRecord(P->ev[5], st));
+    CP_HIP(hipEventSynchronize(P->ev[5]));
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
+    if (ms) *ms = t / reps;
+    const double vec = 2. * s * P->nn;   // x read once, y written once
+    if (bytes_stored) *bytes_stored = P->nblk_stored * (4. * s + 4.) + 8. * P->nslices + 2. * vec;
+    if (bytes_algo) *bytes_algo = P->nblk * (4. * s + 4.) + 8. * P->nslices + 2. * vec;
+    return CP_OK;
+}
+
+int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* bytes_algo)
+{
+    if (!P || reps < 1) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (int s = cp::launch_assemble(P, precision)) return s;
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    const double s = precision == CP_FP64 ? 8. : 4.;
+    if (precision == CP_FP64) {
+        auto a = P->cg64_args();
+        // tol 0 and a huge cap keep every launch live
+        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, 1.0, 0.0, 2000000000);
+        hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
+        hipLaunchKernelGGL(cpk::k_cg_init2<double>, g, b, 0, st, a);
+        CP_HIP(hipEventRecord(P->ev[4], st));
+        for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(cpk::k_cg_iter<double>, g, b, 0, st, a);
+    } else {
+        auto a = P->cg32_args();
+        // right-hand side of the inner solve: the FP64 residual of the current u
+        cpk::ResArgs ra{};
+        ra.incPtr = P->inc_ptr;
+        ra.inc = P->inc;
+        ra.conn = P->conn;
+        ra.xy = P->xy;
+        ra.ecnu = P->ecnu;
+        ra.u = P->u;
+        ra.b = P->b;
+        ra.mask = P->mask;
+        ra.r64 = P->res64;
+        ra.r32 = P->b32;
+        ra.nn = P->nn;
+        ra.nblocks = P->nbn;
+        ra.plane = P->M.plane;
+        ra.h = P->M.h;
+        hipLaunchKernelGGL(cpk::k_residual64, g, b, 0, st, ra);
+        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, 1.0, 0.0, 2000000000);
+        hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
+        hipLaunchKernelGGL(cpk::k_cg_init2<float>, g, b, 0, st, a);
+        CP_HIP(hipEventRecord(P->ev[4], st));
+        for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(cpk::k_cg_iter<float>, g, b, 0, st, a);
+    }
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipEventRecord(P->ev[5], st));
+    CP_HIP(hipEventSynchronize(P->ev[5]));
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
+    if (ms) *ms = t / reps;
+    // matrix once + 5 vector reads (r, w, s_prev, p, x) + 5 writes + mask
+    if (bytes_algo) *bytes_algo = P->nblk * (4. * s + 4.) + 8. * P->nslices + 10. * 2. * s * P->nn + 2. * P->nn;
+    return CP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,274 @@
+/*
+ * cp_abi.h -- C-ABI of the MI355X (gfx950) plasticity hot path.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (CP_OK == 0); cp_last_error() gives the message of the calling thread's
+ * last failure.  Nothing here throws across the boundary.
+ *
+ * The reference (SATOH-SD/CudaPlasticity) has no FFI: its boundary is the
+ * C++ classes Mesh / Material / LoadConditions / PlasticitySolver.  The C++
+ * drop-in API in include/cudaplasticity/ implements those classes on top of
+ * the entry points below; each entry point names the reference interface
+ * it replaces (paths relative to the reference's src/).
+ */
+#ifndef CP_ABI_H
+#define CP_ABI_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CP_ABI_VERSION 1
+
+/* status codes */
+#define CP_OK                 0
+#define CP_ERR_INVALID        1   /* bad argument / unsupported mesh */
+#define CP_ERR_NOMEM          2   /* device or host allocation failed */
+#define CP_ERR_DEVICE         3   /* HIP runtime error */
+#define CP_ERR_NOT_CONVERGED  4   /* iteration cap reached */
+#define CP_ERR_NAN            5   /* NaN/Inf residual or stress */
+#define CP_ERR_NO_DEVICE      6   /* no HIP device / library built without one */
+#define CP_ERR_BUDGET         7   /* request larger than the serving HBM budget */
+#define CP_ERR_STATE          8   /* call out of order */
+
+/* material laws (Material.cu:3-44; CP_LAW_SWIFT is new) */
+#define CP_LAW_LINEAR  0   /* sigmaT + (eps - epsT) K_T above epsT = sigmaT/E */
+#define CP_LAW_POWER   1   /* reference power law with A, m */
+#define CP_LAW_SWIFT   2   /* sigma = sigmaT (1 + eps_p/eps0)^n, eps0 = sigmaT/E */
+
+#define CP_PLANE_STRESS 0  /* PlasticitySolver.cuh:15 plainCondition::stress */
+#define CP_PLANE_STRAIN 1
+
+/* precision of the linear solve */
+#define CP_FP64   0        /* FP64 values, vectors and arithmetic */
+#define CP_MIXED  1        /* FP32 CG (values, SpMV, vector updates) + FP64 iterative refinement */
+
+/* per-element fields for cp_download (PlasticitySolver.cuh:49-52) */
+#define CP_F_EXX    0
+#define CP_F_EYY    1
+#define CP_F_GXY    2
+#define CP_F_EPSI   3
+#define CP_F_SXX    4
+#define CP_F_SYY    5
+#define CP_F_TAU    6
+#define CP_F_SIGI   7
+#define CP_F_SIGTAB 8
+#define CP_F_PSI    9
+#define CP_F_EC     10
+#define CP_F_NUC    11
+#define CP_F_EXXP   12
+#define CP_F_EYYP   13
+#define CP_F_GXYP   14
+#define CP_F_EPSIP  15
+#define CP_NFIELDS  16
+#define CP_F_U      100    /* nodal displacements, 2*nn doubles (u0,v0,u1,v1,...) */
+#define CP_F_RHS    101    /* current load vector, 2*nn doubles */
+
+typedef struct cp_ctx cp_ctx;
+typedef struct cp_problem cp_problem;
+typedef struct cp_server cp_server;
+
+/* Material.cuh:9-41 (E, nu, h, sigmaT, K_T, A, m) plus the plane condition
+ * (PlasticitySolver::setPlainCondition) and the Swift exponent n. */
+typedef struct {
+    double E, nu, h, sigmaT;
+    int law, plane;
+    double K_T, A, m, n;
+} cp_material;
+
+/* Tolerances default to the reference's hard-coded values
+ * (PlasticitySolver.cu:454,495,503): cg_tol 1e-7, tol 1e-5. */
+typedef struct {
+    double cg_tol;       /* ||r||/||b|| stop of the linear solve */
+    int    cg_maxit;     /* per linear solve */
+    double tol;          /* plastic exit: ||sigma_i - f(eps_i)|| / ||f(eps_i)|| */
+    int    max_outer;    /* cap on secant iterations per load level */
+    int    fixed_outer;  /* > 0: run exactly this many secant iterations */
+    int    aitken;       /* 1: safeguarded Irons-Tuck acceleration of psi */
+    int    precision;    /* CP_FP64 or CP_MIXED */
+    int    warm;         /* 1: keep u and the secant moduli from the previous call */
+    double inner_tol;    /* CP_MIXED: relative reduction of each FP32 inner solve */
+    int    reserved[4];
+} cp_options;
+
+typedef struct {
+    int       outer_iters;
+    int       status;        /* CP_OK, CP_ERR_NOT_CONVERGED, CP_ERR_NAN */
+    long long cg_iters;      /* inner CG iterations (all refinements) */
+    long long refinements;   /* CP_MIXED: FP64 residual evaluations */
+    double    rel_err;       /* last plastic exit measure */
+    double    seconds;       /* wall time of the call */
+    double    cg_seconds;    /* device time in CG kernels (HIP events) */
+    double    asm_seconds;   /* device time in assembly kernels */
+    double    rm_seconds;    /* device time in return-map kernels */
+} cp_stats;
+
+/* ------------------------------------------------------------------ */
+/* context / device                                                     */
+/* ------------------------------------------------------------------ */
+/* Mesh::Mesh() device probe (Mesh.cuh:86-104). */
+int         cp_ctx_create(int device, cp_ctx** out);
+int         cp_ctx_destroy(cp_ctx* ctx);
+const char* cp_last_error(void);
+int         cp_device_info(cp_ctx* ctx, char* name, int name_len, size_t* total_mem, int* cu_count);
+int         cp_abi_version(void);
+
+/* ------------------------------------------------------------------ */
+/* host-side mesh generators (order 1) -- Mesh.cu                      */
+/* Sizes first (nn, ne, nb, blen[nb]); then the generator fills x, y    */
+/* (nn each), conn (4*ne, reference local order), bord (sum blen).     */
+/* ------------------------------------------------------------------ */
+/* Mesh::genRectangle (Mesh.cu:17-109) */
+int cp_mesh_rect_sizes(int N1, int N2, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_rect(double x1, double x2, double y1, double y2, int N1, int N2,
+                 double* x, double* y, int* conn, int* bord, int* blen);
+/* Mesh::genRectWithHole + renumerateRing (Mesh.cu:112-245, 332-384) */
+int cp_mesh_hole_sizes(double x1, double x2, double y1, double y2, double r, int N1, int N2,
+                       int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_hole(double x1, double x2, double y1, double y2, double r, int N1, int N2,
+                 double* x, double* y, int* conn, int* bord, int* blen);
+/* Mesh::genRing + renumerateRing (Mesh.cu:249-329) */
+int cp_mesh_ring_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_ring(double a, double b, int Nphi, int Nr,
+                 double* x, double* y, int* conn, int* bord, int* blen);
+/* New: quarter plate [0,L]^2 minus a quarter disc (borders: 0 y=0, 1 x=L, 2 y=L, 3 x=0, 4 hole) */
+int cp_mesh_quarter_plate_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_quarter_plate(double L, double r, int Nt, int Nr,
+                          double* x, double* y, int* conn, int* bord, int* blen);
+/* New: graded quarter ring (borders: 0 inner arc, 1 theta=pi/2, 2 outer arc, 3 theta=0) */
+int cp_mesh_quarter_ring_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
+                         double* x, double* y, int* conn, int* bord, int* blen);
+/* Mesh::findMaxIndexDiff (Mesh.cu:417-441) */
+int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
+
+/* ------------------------------------------------------------------ */
+/* host-side loads / boundary conditions -- PlasticitySolver.cu:1007-1216 */
+/* ------------------------------------------------------------------ */
+/* Border traction: fval holds the force value at every border node (2 each),
+ * as LoadConditions::setForce's function would return it (:1070-1103). */
+int cp_rhs_border_force(const double* x, const double* y, const int* border, int len,
+                        const double* fval, int norm_orient, double h, double* rp);
+/* Constant body force on 4-node elements (:1106-1117). */
+int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
+                      double gx, double gy, double rho, double h, double* rp);
+/* LoadConditions::fixVertAxis / fixHorAxis (:1165-1181). */
+int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask);
+/* LoadConditions::setDisplacement on a border (:1138-1164). */
+int cp_displacement_border(const int* border, int len, double ux, double uy,
+                           int fix_mid_x, int fix_mid_y, unsigned char* mask, double* u);
+
+/* ------------------------------------------------------------------ */
+/* CSR pattern in the reference layout -- CudaSparseSLAE.cuh:303-458    */
+/* ------------------------------------------------------------------ */
+int cp_csr_nnz(int nn, int ne, const int* conn, long long* nnz);
+int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols);
+
+/* ------------------------------------------------------------------ */
+/* device problems -- replaces the PlasticitySolver constructor         */
+/* (PlasticitySolver.cuh:236-338), CudaSparseSLAE / CudaSLAE /          */
+/* ConjGradCuda construction and the host-side initConditions_ upload.  */
+/* Inputs are copied; the library owns the device memory.               */
+/* rp: load vector at load factor 1; u0: prescribed displacements on    */
+/* fixed DOFs (scaled by the load factor too); mask: 1 free, 0 fixed.   */
+/* ------------------------------------------------------------------ */
+int cp_problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
+                      const double* rp, const double* u0, const unsigned char* mask,
+                      const cp_material* mat, cp_problem** out);
+int cp_problem_destroy(cp_problem* prob);
+/* Device bytes held by a problem of this size (serving admission). */
+int cp_problem_bytes(int nn, int ne, long long nnz, int precision, size_t* bytes);
+
+/* Integer CSR structure as the reference builds it (bit-exact target). */
+int cp_problem_csr_nnz(cp_problem* prob, long long* nnz);
+int cp_problem_get_csr(cp_problem* prob, int* rows, int* cols);
+/* FP64 values of the last FP64 assembly, in reference CSR order. */
+int cp_problem_get_values(cp_problem* prob, double* vals);
+
+/* Scale the load (rp, prescribed u) -- new: load stepping. */
+int cp_set_load(cp_problem* prob, double factor);
+/* Reset u and the secant moduli to the initial state (initPlastParams :385-410). */
+int cp_reset_state(cp_problem* prob);
+
+/* One assembly with the current secant moduli: fillGlobalStiffness
+ * (PlasticitySolver.cu:1492-1508) without the band matrix. */
+int cp_assemble(cp_problem* prob, int precision);
+/* One masked CG solve from the current u: ConjGradCuda::solve (ConjGradCuda.cu:185-231). */
+int cp_cg(cp_problem* prob, double tol, int maxit, int precision, double inner_tol,
+          long long* iters, double* relres);
+/* One secant update (updateParamDouble :1613-1625 + exitCondition :1722-1735)
+ * followed by the plain update of psi, E_c, nu_c; returns the exit measure. */
+int cp_return_map(cp_problem* prob, double* rel_err);
+/* calcPlastDeformCuda (:1655-1662). */
+int cp_plastic_strains(cp_problem* prob);
+/* f = K(E_c, nu_c) u computed matrix-free in FP64 (element internal force). */
+int cp_internal_force(cp_problem* prob, double* f_host);
+
+/* PlasticitySolver::solve / solveElast for one load level (solveCUDA :469-512). */
+int cp_solve(cp_problem* prob, const cp_options* opt, cp_stats* stats);
+int cp_solve_elastic(cp_problem* prob, const cp_options* opt, cp_stats* stats);
+/* nsteps load levels factor*n/nsteps, n = 1..nsteps, each warm-started. */
+int cp_solve_steps(cp_problem* prob, int nsteps, double final_factor, const cp_options* opt,
+                   cp_stats* per_step);
+
+/* dataToRAM (:1738-1781): one field (CP_F_*) to host memory. */
+int cp_download(cp_problem* prob, int field, double* host);
+/* Warm-start state from host: u (2*nn) and psi (ne); either may be NULL. */
+int cp_upload_state(cp_problem* prob, const double* u, const double* psi);
+
+/* Kernel timings of the last CG solve (HIP events on the problem's stream):
+ * average duration of one CG iteration kernel, and of the SpMV-only kernel
+ * when cp_bench_spmv was used. */
+int cp_last_cg_kernel_ms(cp_problem* prob, double* ms_per_iter);
+/* Times `reps` standalone SpMV launches y = A x (precision) and returns the
+ * average kernel ms, the stored-format bytes per launch and the algorithmic
+ * bytes per launch. */
+int cp_bench_spmv(cp_problem* prob, int precision, int reps, double* ms, double* bytes_stored,
+                  double* bytes_algo);
+/* Times `reps` fused CG iteration launches (precision) on the problem's
+ * current matrix (assembled here) with the stop test disabled; returns the
+ * average kernel ms and the algorithmic bytes of one iteration. */
+int cp_bench_cg(cp_problem* prob, int precision, int reps, double* ms, double* bytes_algo);
+/* Matrix storage info: nodes, stored blocks (with padding), true blocks. */
+int cp_problem_layout(cp_problem* prob, long long* nblk_stored, long long* nblk, int* max_width);
+
+/* ------------------------------------------------------------------ */
+/* serving: continuous batching of independent problems (new)          */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    size_t hbm_budget;    /* bytes of device memory the server may hold for problems */
+    int    max_slots;     /* problems resident at once */
+    int    batch_cg;      /* 1: batched multi-problem CG kernels; 0: one launch per problem */
+    int    reserved[5];
+} cp_server_config;
+
+typedef struct {
+    long long id;
+    int       status;          /* CP_OK / CP_ERR_* */
+    int       steps_done;
+    long long cg_iters;
+    int       outer_iters;
+    double    t_submit, t_admit, t_done;   /* seconds on the server clock */
+} cp_request_info;
+
+int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out);
+int cp_server_destroy(cp_server* srv);
+/* Submit one problem: mesh + loads + material + nsteps load levels to
+ * final_factor with options; inputs are copied. */
+int cp_server_submit(cp_server* srv, int nn, int ne, const double* x, const double* y, const int* conn,
+                     const double* rp, const double* u0, const unsigned char* mask,
+                     const cp_material* mat, int nsteps, double final_factor, const cp_options* opt,
+                     long long* id);
+/* Advance the server: admit queued requests that fit, run one scheduling
+ * round of device work.  Returns the number of requests still in flight. */
+int cp_server_step(cp_server* srv, int* in_flight);
+/* Completed request: info and (optional) results; returns CP_ERR_STATE if not done. */
+int cp_server_result(cp_server* srv, long long id, cp_request_info* info, double* u, double* sig_i);
+int cp_server_clock(cp_server* srv, double* now);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -6,6 +6,7 @@
  *
  * Citations are reference file:line, relative to the reference's src/.
  */
+#define _GNU_SOURCE
 #include "cp_oracle.h"
 
 #include <math.h>
@@ -190,9 +191,10 @@ void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double ho
     double cx = 0.5 * (x1 + x2), cy = 0.5 * (y1 + y2);
     double start = phi * (N1 + N2 * 1.5);
     for (int i = 0; i < borderN; ++i, ++nodeId) {
-        double a = start + (double)i * phi;
-        x[nodeId] = cx + cos(a) * holeRad;
-        y[nodeId] = cy + sin(a) * holeRad;
+        double a = start + (double)i * phi, sa, ca;
+        sincos(a, &sa, &ca);
+        x[nodeId] = cx + ca * holeRad;
+        y[nodeId] = cy + sa * holeRad;
         b4[borderN - i - 1] = nodeId;
     }
     b4[blen[4] - 1] = nodeCount - 1;
@@ -244,9 +246,10 @@ void or_gen_ring(double a, double b, int Nphi, int Nr,
     for (int j = 0; j < Nphi; ++j) {
         for (int i = 0; i <= Nr; ++i) {
             double r = b - h_r * i;
-            double phi = h_phi * j;
-            x[i * Nphi + j] = r * cos(phi);
-            y[i * Nphi + j] = r * sin(phi);
+            double phi = h_phi * j, sp, cp;
+            sincos(phi, &sp, &cp);
+            x[i * Nphi + j] = r * cp;
+            y[i * Nphi + j] = r * sp;
         }
         b1[Nphi - j] = j;
         b0[j] = Nr * Nphi + j;
@@ -298,8 +301,9 @@ void or_gen_quarter_plate(double L, double r, int Nt, int Nr,
         if (j == 0) { hx = r; hy = 0.; }
         else if (j == Nt) { hx = 0.; hy = r; }
         else {
-            double t = 0.5 * OR_PI * (double)j / (double)Nt;
-            hx = r * cos(t); hy = r * sin(t);
+            double t = 0.5 * OR_PI * (double)j / (double)Nt, st, ct;
+            sincos(t, &st, &ct);
+            hx = r * ct; hy = r * st;
         }
         double rhx = (hx - ox) / Nr, rhy = (hy - oy) / Nr;
         int base = j * (Nr + 1);
@@ -356,7 +360,7 @@ void or_gen_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
         double c, s;
         if (j == 0) { c = 1.; s = 0.; }
         else if (j == Nt) { c = 0.; s = 1.; }
-        else { double t = 0.5 * OR_PI * (double)j / (double)Nt; c = cos(t); s = sin(t); }
+        else { double t = 0.5 * OR_PI * (double)j / (double)Nt; sincos(t, &s, &c); }
         for (int i = 0; i <= Nr; ++i) {
             x[j * (Nr + 1) + i] = rho[i] * c;
             y[j * (Nr + 1) + i] = rho[i] * s;

@@ -1,0 +1,101 @@
+"""CPU tests of the product library's host side (no GPU calls): the C-ABI
+loads and exports everything include/cp_abi.h declares; its mesh generators,
+load/BC rules and reference CSR pattern agree with the oracle bit for bit."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cudaplasticity_amd import abi as A
+
+
+def test_library_exports_every_declared_symbol():
+    lib = A.lib()
+    names = A.declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.cp_abi_version() == 1
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    """Without a GPU the context cannot be created and no CPU path exists."""
+    from conftest import HAS_GPU
+    if HAS_GPU:
+        pytest.skip("a device is visible")
+    with pytest.raises(A.CPError) as e:
+        A.Context(0)
+    assert e.value.code == A.CP_ERR_NO_DEVICE
+
+
+MESHES = [
+    ("rect", (-1, 1, -1, 1, 4, 4)),
+    ("rect", (0, 3, -2, 1, 7, 5)),
+    ("hole", (-3, 3, -2, 2, 0.5, 48, 32)),
+    ("hole", (-1, 2, -1, 1, 0.2, 10, 6)),
+    ("ring", (0.5, 1.0, 40, 5)),
+    ("ring", (0.1, 0.2, 41, 3)),
+    ("qplate", (1.0, 0.1, 100, 49)),
+    ("qplate", (2.0, 0.3, 17, 4)),
+    ("qring", (0.5, 1.0, 64, 20, 1.05)),
+    ("qring", (0.5, 1.0, 16, 8, 1.0)),
+]
+
+
+def _pair(kind, args):
+    o = {"rect": O.gen_rect, "hole": O.gen_rect_with_hole, "ring": O.gen_ring,
+         "qplate": O.gen_quarter_plate, "qring": O.gen_quarter_ring}[kind](*args)
+    p = {"rect": A.gen_rect, "hole": A.gen_rect_with_hole, "ring": A.gen_ring,
+         "qplate": A.gen_quarter_plate, "qring": A.gen_quarter_ring}[kind](*args)
+    return o, p
+
+
+@pytest.mark.parametrize("kind,args", MESHES)
+def test_generators_match_oracle(kind, args):
+    o, p = _pair(kind, args)
+    assert np.array_equal(o.conn, p.conn)
+    assert len(o.borders) == len(p.borders)
+    for bo, bp in zip(o.borders, p.borders):
+        assert np.array_equal(bo, bp)
+    assert np.array_equal(o.x, p.x) and np.array_equal(o.y, p.y)
+
+
+@pytest.mark.parametrize("kind,args", MESHES)
+def test_csr_pattern_matches_oracle(kind, args):
+    o, p = _pair(kind, args)
+    ro, co = O.csr_pattern(o)
+    rp, cp = A.csr_pattern(p)
+    assert np.array_equal(ro, rp) and np.array_equal(co, cp)
+
+
+def test_loads_match_oracle():
+    o, p = _pair("hole", (-3, 3, -2, 2, 0.5, 24, 16))
+    lo = O.Problem(o).force(1, 170.0, 3.0).force(3, -170.0, 0.0).force(4, 5.0, 1.0, norm_orient=True)
+    lo = lo.gravity().fix_vert_axis(0).fix_hor_axis(0).displacement(2, 1e-3, 0.0, False, True)
+    lp = A.Loads(p).force(1, 170.0, 3.0).force(3, -170.0, 0.0).force(4, 5.0, 1.0, norm_orient=True)
+    lp = lp.gravity().fix_vert_axis(0).fix_hor_axis(0).displacement(2, 1e-3, 0.0, False, True)
+    assert np.array_equal(lo.rp, lp.rp)
+    assert np.array_equal(lo.mask, lp.mask)
+    assert np.array_equal(lo.u, lp.u)
+
+
+def test_degenerate_and_out_of_range_connectivity_rejected():
+    m = A.gen_rect(0, 1, 0, 1, 2, 2)
+    bad = m.conn.copy()
+    bad[1, 2] = bad[1, 0]
+    with pytest.raises(A.CPError):
+        A.csr_pattern(A.Mesh(m.x, m.y, bad, m.borders))
+    bad = m.conn.copy()
+    bad[0, 0] = m.nn
+    with pytest.raises(A.CPError):
+        A.csr_pattern(A.Mesh(m.x, m.y, bad, m.borders))
+
+
+def test_problem_bytes_budget_monotone():
+    import ctypes as C
+    b1, b2 = C.c_size_t(), C.c_size_t()
+    A.check(A.lib().cp_problem_bytes(5050, 4900, 180000, A.MIXED, C.byref(b1)))
+    A.check(A.lib().cp_problem_bytes(201000, 200000, 7200000, A.MIXED, C.byref(b2)))
+    assert 0 < b1.value < b2.value
+    b3 = C.c_size_t()
+    A.check(A.lib().cp_problem_bytes(201000, 200000, 7200000, A.FP64, C.byref(b3)))
+    assert b3.value > b2.value
