@@ -37,7 +37,8 @@ class Material(C.Structure):
 class Options(C.Structure):
     _fields_ = [("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("tol", C.c_double), ("max_outer", C.c_int),
                 ("fixed_outer", C.c_int), ("aitken", C.c_int), ("precision", C.c_int), ("warm", C.c_int),
-                ("inner_tol", C.c_double), ("reserved", C.c_int * 4)]
+                ("inner_tol", C.c_double), ("precond", C.c_int), ("predictor", C.c_int),
+                ("cg_forcing", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -108,10 +109,12 @@ def material(E, nu, sigmaT=1e300, law=LAW_LINEAR, K_T=None, A=0.0, m=0.0, n=0.0,
 
 
 def options(cg_tol=1e-7, cg_maxit=0, tol=1e-5, max_outer=1000, fixed_outer=0, aitken=False, precision=FP64,
-            warm=False, inner_tol=1e-3):
+            warm=False, inner_tol=1e-3, precond=False, predictor=0, cg_forcing=0.0):
+    """Defaults are the reference's (unpreconditioned CG to 1e-7, exit 1e-5)."""
     o = Options()
     o.cg_tol, o.cg_maxit, o.tol, o.max_outer, o.fixed_outer = cg_tol, cg_maxit, tol, max_outer, fixed_outer
     o.aitken, o.precision, o.warm, o.inner_tol = int(aitken), precision, int(warm), inner_tol
+    o.precond, o.predictor, o.cg_forcing = int(precond), int(predictor), cg_forcing
     return o
 
 
@@ -294,6 +297,9 @@ class Problem:
     def reset(self):
         check(lib().cp_reset_state(self.ptr))
 
+    def advance_load(self, factor, predictor=True):
+        check(lib().cp_advance_load(self.ptr, _f(factor), int(predictor)))
+
     def assemble(self, precision=FP64):
         check(lib().cp_assemble(self.ptr, precision))
 
@@ -362,3 +368,52 @@ def declared_functions(header=HEADER):
     import re
     txt = open(header).read()
     return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(cp_\w+)\s*\(", txt, re.M)))
+
+
+class Server:
+    """Continuous-batching serving loop (cp_server_*)."""
+
+    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True):
+        cfg = ServerConfig()
+        cfg.hbm_budget, cfg.max_slots, cfg.batch_cg = int(hbm_budget), int(max_slots), int(batch_cg)
+        self.ptr = C.c_void_p()
+        check(lib().cp_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)))
+        self.sizes = {}
+
+    def submit(self, mesh, loads, mat, nsteps, final_factor, opt):
+        rid = C.c_longlong()
+        check(lib().cp_server_submit(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
+                                     dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), int(nsteps),
+                                     _f(final_factor), C.byref(opt), C.byref(rid)))
+        self.sizes[rid.value] = (mesh.nn, mesh.ne)
+        return rid.value
+
+    def step(self):
+        n = C.c_int()
+        check(lib().cp_server_step(self.ptr, C.byref(n)))
+        return n.value
+
+    def clock(self):
+        t = C.c_double()
+        check(lib().cp_server_clock(self.ptr, C.byref(t)))
+        return t.value
+
+    def result(self, rid, want_fields=True):
+        nn, ne = self.sizes[rid]
+        info = RequestInfo()
+        u = np.zeros(2 * nn) if want_fields else None
+        sig = np.zeros(ne) if want_fields else None
+        code = lib().cp_server_result(self.ptr, C.c_longlong(rid), C.byref(info),
+                                      dp(u) if want_fields else None, dp(sig) if want_fields else None)
+        return code, info, u, sig
+
+    def run_until_done(self, max_rounds=10 ** 7):
+        for _ in range(max_rounds):
+            if self.step() == 0:
+                return
+        raise RuntimeError("server did not drain")
+
+    def close(self):
+        if self.ptr:
+            lib().cp_server_destroy(self.ptr)
+            self.ptr = C.c_void_p()

@@ -65,43 +65,40 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double* sh)
     __syncthreads();
 }
 
-// Grid-wide deterministic sum: every block publishes its partial, the last
-// block to arrive sums all partials in block order.  Returns true in the
-// last block; thread 0 of it holds the totals in v.
+// Cross-block sums without in-kernel synchronisation: each block writes its
+// partial with a plain store and a one-block kernel launched next sums the
+// partials in block order.  (Agent-scope fences inside a kernel would write
+// back and invalidate a whole XCD L2 per block on this multi-XCD part; the
+// kernel boundary provides the coherence once per launch instead.)
 template <int K>
-__device__ __forceinline__ bool grid_sum(double (&v)[K], double* partials, unsigned* counter, int nblocks, int lb)
+__device__ __forceinline__ void block_partial(double (&v)[K], double* partials, int lb)
 {
     __shared__ double sh[4 * K];
-    __shared__ int s_last;
     block_sum<K>(v, sh);
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) partials[(size_t)lb * K + k] = v[k];
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        s_last = (t == (unsigned)(nblocks - 1));
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double acc[K];
+}
+
+// Fixed-order sum of nblocks partials by one block; thread 0 gets the totals.
+template <int K>
+__device__ __forceinline__ void reduce_partials(const double* partials, int nblocks, double (&v)[K])
+{
+    __shared__ double sh[4 * K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = 0.;
+    for (int k = 0; k < K; ++k) v[k] = 0.;
     for (int j = threadIdx.x; j < nblocks; j += kBlock)
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] += partials[(size_t)j * K + k];
-    block_sum<K>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = acc[k];
-        *counter = 0u;
-    }
-    return true;
+        for (int k = 0; k < K; ++k) v[k] += partials[(size_t)j * K + k];
+    block_sum<K>(v, sh);
 }
 
 template <typename T> struct V2;
 template <> struct V2<double> { using type = double2; };
 template <> struct V2<float> { using type = float2; };
+template <typename T> struct V4;
+template <> struct V4<double> { using type = double4; };
+template <> struct V4<float> { using type = float4; };
 
 // ---------------------------------------------------------------- material
 
@@ -282,6 +279,7 @@ struct CgScal {
     double alpha, beta, gamma, delta;
     double bnorm2, tol2;
     double gamma0;
+    double rho;       // (r, r): the stop test uses the unpreconditioned residual
     int iters, done, status, maxit;
     unsigned counter, pad;
 };
@@ -300,10 +298,18 @@ struct CgArgs {
     V* s[2];
     const V* b;
     const uchar2* mask;
+    const typename V4<T>::type* minv;  // 2x2 block-Jacobi inverse (m00, m01, m11, 0) per node
     CgScal* sc;
     double* partials;
     int nn, nblocks;
 };
+
+template <typename T>
+__device__ __forceinline__ void apply_minv(const typename V4<T>::type& m, T rx, T ry, T& ux, T& uy)
+{
+    ux = m.x * rx + m.y * ry;
+    uy = m.y * rx + m.z * ry;
+}
 
 // Matrix values are streamed once per iteration: non-temporal loads keep
 // them from evicting the gathered vectors from L2.
@@ -354,44 +360,98 @@ __global__ __launch_bounds__(kBlock) void k_cg_init1(CgArgs<T> a, int zero_x)
     a.s[1][n] = V{0, 0};
 }
 
-// w_0 = mask (A r_0); gamma_0, delta_0 -> alpha_0.
-template <typename T>
+// Row of w = A z for node n, z at a neighbour m given by zfun(m).  W > 0:
+// the row is unrolled to W blocks with every load issued up front (indices
+// of padding blocks point at the node itself and their values are zeroed),
+// so a wave has all of its row's loads in flight at once.  W == 0: generic
+// loop for wide rows.
+template <int W, typename T, typename ZF>
+__device__ __forceinline__ void row_product(const CgArgs<T>& a, int n, const ZF& zfun, T& yx, T& yy)
+{
+    const int2 sl = a.slice[n >> 6];
+    const size_t base = (size_t)sl.x + (n & 63);
+    yx = 0;
+    yy = 0;
+    if constexpr (W > 0) {
+        int cidx[W];
+#pragma unroll
+        for (int k = 0; k < W; ++k) cidx[k] = a.col[base + (size_t)(k < sl.y ? k : 0) * kWave];
+        T m[W][4];
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            load_block(a, base + (size_t)(k < sl.y ? k : 0) * kWave, m[k][0], m[k][1], m[k][2], m[k][3]);
+            if (k >= sl.y) m[k][0] = m[k][1] = m[k][2] = m[k][3] = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+            T zx, zy;
+            zfun(cidx[k], zx, zy);
+            yx += m[k][0] * zx + m[k][1] * zy;
+            yy += m[k][2] * zx + m[k][3] * zy;
+        }
+    } else {
+        size_t q = base;
+        for (int k = 0; k < sl.y; ++k, q += kWave) {
+            T zx, zy, m00, m01, m10, m11;
+            zfun(a.col[q], zx, zy);
+            load_block(a, q, m00, m01, m10, m11);
+            yx += m00 * zx + m01 * zy;
+            yy += m10 * zx + m11 * zy;
+        }
+    }
+}
+
+// u_0 = M^-1 r_0 (M = I without preconditioning), w_0 = mask (A u_0);
+// gamma_0 = (r_0, u_0), delta_0 = (w_0, u_0), rho_0 = (r_0, r_0) -> alpha_0.
+template <int W, bool PRE, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_init2(CgArgs<T> a)
 {
     using V = typename V2<T>::type;
     const int lb = xcd_block(a.nblocks);
     const int n = lb * kBlock + threadIdx.x;
-    double v[2] = {0., 0.};
+    double v[3] = {0., 0., 0.};
     if (n < a.nn) {
-        const int2 sl = a.slice[n >> 6];
-        size_t q = (size_t)sl.x + (n & 63);
-        T yx = 0, yy = 0;
-        for (int k = 0; k < sl.y; ++k, q += kWave) {
-            const V rm = a.r[0][a.col[q]];
-            T m00, m01, m10, m11;
-            load_block(a, q, m00, m01, m10, m11);
-            yx += m00 * rm.x + m01 * rm.y;
-            yy += m10 * rm.x + m11 * rm.y;
-        }
+        const V* r0 = a.r[0];
+        T yx, yy;
+        row_product<W, T>(a, n,
+                          [&](int m, T& zx, T& zy) {
+                              const V rm = r0[m];
+                              if constexpr (PRE) apply_minv<T>(a.minv[m], rm.x, rm.y, zx, zy);
+                              else { zx = rm.x; zy = rm.y; }
+                          },
+                          yx, yy);
         const uchar2 mk = a.mask[n];
         const V wn = V{mk.x ? yx : (T)0, mk.y ? yy : (T)0};
         a.w[0][n] = wn;
-        const V rn = a.r[0][n];
-        v[0] = (double)rn.x * rn.x + (double)rn.y * rn.y;
-        v[1] = (double)wn.x * rn.x + (double)wn.y * rn.y;
+        const V rn = r0[n];
+        T ux = rn.x, uy = rn.y;
+        if constexpr (PRE) apply_minv<T>(a.minv[n], rn.x, rn.y, ux, uy);
+        v[0] = (double)rn.x * ux + (double)rn.y * uy;
+        v[1] = (double)wn.x * ux + (double)wn.y * uy;
+        v[2] = (double)rn.x * rn.x + (double)rn.y * rn.y;
     }
-    if (grid_sum<2>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
+    block_partial<3>(v, a.partials, lb);
+}
+
+// Scalars of the first iteration from the init2 partials.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs<T> a)
+{
+    double v[3];
+    reduce_partials<3>(a.partials, a.nblocks, v);
+    if (threadIdx.x == 0) {
         CgScal* sc = a.sc;
-        const double g = v[0], d = v[1];
-        if (sc->bnorm2 == 0.) sc->bnorm2 = g;
+        const double g = v[0], d = v[1], rho = v[2];
+        if (sc->bnorm2 == 0.) sc->bnorm2 = rho;
         sc->gamma0 = g;
         sc->gamma = g;
         sc->delta = d;
+        sc->rho = rho;
         sc->beta = 0.;
         sc->alpha = g / d;
         sc->iters = 0;
         sc->status = (g != g || d != d) ? 2 : 0;
-        sc->done = (g == 0. || sc->status != 0) ? 1 : 0;
+        sc->done = (rho == 0. || sc->status != 0) ? 1 : 0;
     }
 }
 
@@ -405,7 +465,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_init2(CgArgs<T> a)
 //   (delta_{i+1} - beta_{i+1} gamma_{i+1}/alpha_i) and the stop test
 //   gamma/||b||^2 <= tol^2 (ConjGradCuda.cu:203-222).
 // w and s carry the Dirichlet mask, so r stays zero on fixed DOFs.
-template <typename T>
+template <int W, bool PRE, typename T>
 __device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
 {
     using V = typename V2<T>::type;
@@ -420,41 +480,55 @@ __device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
     V* wout = a.w[par ^ 1];
     V* sout = a.s[par];
     const int n = lb * kBlock + threadIdx.x;
-    double v[2] = {0., 0.};
+    double v[3] = {0., 0., 0.};
     if (n < a.nn) {
+        T yx, yy;
+        row_product<W, T>(a, n,
+                          [&](int m, T& zx, T& zy) {
+                              const V rm = rin[m], wm = win[m], sm = sprev[m];
+                              const T qx = rm.x - alpha * (wm.x + beta * sm.x);
+                              const T qy = rm.y - alpha * (wm.y + beta * sm.y);
+                              if constexpr (PRE) apply_minv<T>(a.minv[m], qx, qy, zx, zy);
+                              else { zx = qx; zy = qy; }
+                          },
+                          yx, yy);
         const V ri = rin[n], wi = win[n], sp = sprev[n], pp = a.p[n], xi = a.x[n];
+        T uix = ri.x, uiy = ri.y;
+        typename V4<T>::type mn{};
+        if constexpr (PRE) {
+            mn = a.minv[n];
+            apply_minv<T>(mn, ri.x, ri.y, uix, uiy);
+        }
         const V si = V{wi.x + beta * sp.x, wi.y + beta * sp.y};
-        const V pi = V{ri.x + beta * pp.x, ri.y + beta * pp.y};
+        const V pi = V{uix + beta * pp.x, uiy + beta * pp.y};
         const V rn = V{ri.x - alpha * si.x, ri.y - alpha * si.y};
         a.p[n] = pi;
         sout[n] = si;
         a.x[n] = V{xi.x + alpha * pi.x, xi.y + alpha * pi.y};
         rout[n] = rn;
-
-        const int2 sl = a.slice[n >> 6];
-        size_t q = (size_t)sl.x + (n & 63);
-        T yx = 0, yy = 0;
-        for (int k = 0; k < sl.y; ++k, q += kWave) {
-            const int m = a.col[q];
-            const V rm = rin[m], wm = win[m], sm = sprev[m];
-            const T zx = rm.x - alpha * (wm.x + beta * sm.x);
-            const T zy = rm.y - alpha * (wm.y + beta * sm.y);
-            T m00, m01, m10, m11;
-            load_block(a, q, m00, m01, m10, m11);
-            yx += m00 * zx + m01 * zy;
-            yy += m10 * zx + m11 * zy;
-        }
         const uchar2 mk = a.mask[n];
         const V wn = V{mk.x ? yx : (T)0, mk.y ? yy : (T)0};
         wout[n] = wn;
-        v[0] = (double)rn.x * rn.x + (double)rn.y * rn.y;
-        v[1] = (double)wn.x * rn.x + (double)wn.y * rn.y;
+        T unx = rn.x, uny = rn.y;
+        if constexpr (PRE) apply_minv<T>(mn, rn.x, rn.y, unx, uny);
+        v[0] = (double)rn.x * unx + (double)rn.y * uny;
+        v[1] = (double)wn.x * unx + (double)wn.y * uny;
+        v[2] = (double)rn.x * rn.x + (double)rn.y * rn.y;
     }
-    if (grid_sum<2>(v, a.partials, &sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
-        const double g = v[0], d = v[1];
-        const int iters = it + 1;
+    block_partial<3>(v, a.partials, lb);
+}
+
+// Scalars of iteration i+1 from the iteration partials: beta, alpha of the
+// Chronopoulos-Gear recurrence and the stop test on (r, r) / ||b||^2.
+__device__ __forceinline__ void cg_step_scalars(CgScal* sc, const double* partials, int nblocks)
+{
+    double v[3];
+    reduce_partials<3>(partials, nblocks, v);
+    if (threadIdx.x == 0) {
+        const double g = v[0], d = v[1], rho = v[2];
+        const int iters = sc->iters + 1;
         sc->iters = iters;
-        if (g != g || d != d) {
+        if (g != g || d != d || rho != rho) {
             sc->status = 2;
             sc->done = 1;
             return;
@@ -463,9 +537,10 @@ __device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
         const double ap = g / (d - bp * g / sc->alpha);
         sc->gamma = g;
         sc->delta = d;
+        sc->rho = rho;
         sc->beta = bp;
         sc->alpha = ap;
-        if (!(g * (1. / sc->bnorm2) > sc->tol2)) {
+        if (!(rho * (1. / sc->bnorm2) > sc->tol2)) {
             sc->done = 1;
         } else if (iters >= sc->maxit) {
             sc->done = 1;
@@ -475,29 +550,72 @@ __device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
 }
 
 template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_step(CgArgs<T> a)
+{
+    if (a.sc->done) return;
+    cg_step_scalars(a.sc, a.partials, a.nblocks);
+}
+
+template <int W, bool PRE, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_iter(CgArgs<T> a)
 {
     if (a.sc->done) return;
-    cg_iter_body<T>(a, xcd_block(a.nblocks));
+    cg_iter_body<W, PRE, T>(a, xcd_block(a.nblocks));
 }
 
 // Batched form for the serving loop: one launch advances one CG iteration of
 // every problem in the table; tiles[t] = {problem, local block}.  Each
 // problem keeps its own scalars, partials and counter, so its result is the
 // same alone or in a batch.
-template <typename T>
+template <int W, bool PRE, typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_iter_batched(const CgArgs<T>* probs, const int2* tiles, int ntiles)
 {
     const int2 t = tiles[xcd_block(ntiles)];
     const CgArgs<T> a = probs[t.x];
     if (a.sc->done) return;
-    cg_iter_body<T>(a, t.y);
+    cg_iter_body<W, PRE, T>(a, t.y);
+}
+
+// Batched scalar step: block b handles problem b.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_step_batched(const CgArgs<T>* probs, int nprob)
+{
+    const CgArgs<T> a = probs[blockIdx.x];
+    if (a.sc->done) return;
+    cg_step_scalars(a.sc, a.partials, a.nblocks);
+}
+
+// 2x2 block-Jacobi inverse from each node's own (first) block, restricted
+// to the free DOFs so that M^-1 r stays zero on fixed ones.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_block_jacobi(CgArgs<T> a, typename V4<T>::type* minv)
+{
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    const int2 sl = a.slice[n >> 6];
+    T m00, m01, m10, m11;
+    load_block(a, (size_t)sl.x + (n & 63), m00, m01, m10, m11);
+    const double d00 = m00, d11 = m11, d01 = 0.5 * ((double)m01 + (double)m10);
+    const uchar2 mk = a.mask[n];
+    double i00 = 0., i01 = 0., i11 = 0.;
+    if (mk.x && mk.y) {
+        const double det = d00 * d11 - d01 * d01;
+        i00 = d11 / det;
+        i01 = -d01 / det;
+        i11 = d00 / det;
+    } else if (mk.x) {
+        i00 = 1. / d00;
+    } else if (mk.y) {
+        i11 = 1. / d11;
+    }
+    minv[n] = typename V4<T>::type{(T)i00, (T)i01, (T)i11, (T)0};
 }
 
 static __global__ void k_cg_setup(CgScal* sc, double bnorm2, double tol2, int maxit)
 {
     sc->bnorm2 = bnorm2;
     sc->tol2 = tol2;
+    sc->rho = 0.;
     sc->maxit = maxit;
     sc->iters = 0;
     sc->done = 0;
@@ -603,7 +721,14 @@ static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
         a.r64[n] = make_double2(rx, ry);
         if (a.r32) a.r32[n] = make_float2((float)rx, (float)ry);
     }
-    if (a.sc && grid_sum<1>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) a.sc->rnorm2 = v[0];
+    if (a.sc) block_partial<1>(v, a.partials, lb);
+}
+
+static __global__ __launch_bounds__(kBlock) void k_residual_total(RefScal* sc, const double* partials, int nblocks)
+{
+    double v[1];
+    reduce_partials<1>(partials, nblocks, v);
+    if (threadIdx.x == 0) sc->rnorm2 = v[0];
 }
 
 // u += (double) d on every node (the refinement update).
@@ -639,6 +764,7 @@ struct RmArgs {
     MatParams M;
     int ne, nblocks;
     int k, aitken, fixed_outer, max_outer, elastic;
+    int solved_tight;     // this iteration's linear solve ran at the final tolerance
     double tol;
 };
 
@@ -707,13 +833,25 @@ static __global__ __launch_bounds__(kBlock) void k_return_map(RmArgs a)
             v[4] = qd * qd;
         }
     }
-    if (grid_sum<5>(v, a.partials, &a.sc->counter, a.nblocks, lb) && threadIdx.x == 0) {
+    block_partial<5>(v, a.partials, lb);
+}
+
+// Exit measure relErr = ||sigma_i - sigma_tab|| / ||sigma_tab|| (exitCondition,
+// PlasticitySolver.cu:1722-1735) and the Irons-Tuck decision:
+//   plain step if last, no Aitken, first iteration or |r_k| > |r_{k-1}|;
+//   omega_k = -omega_{k-1} (r_{k-1}.(r_k - r_{k-1})) / |r_k - r_{k-1}|^2 in [0.05, 100].
+static __global__ __launch_bounds__(kBlock) void k_rm_decide(RmArgs a)
+{
+    double v[5];
+    reduce_partials<5>(a.partials, a.nblocks, v);
+    if (threadIdx.x == 0) {
         RmScal* sc = a.sc;
         const double rel = sqrt(v[0] / v[1]);
         sc->rel_err = rel;
         const int nan = rel != rel;
         const int last = a.elastic || nan ||
-                         (a.fixed_outer > 0 ? (a.k + 1 == a.fixed_outer) : (rel <= a.tol || a.k + 1 == a.max_outer));
+                         (a.fixed_outer > 0 ? (a.k + 1 == a.fixed_outer)
+                                            : ((rel <= a.tol && a.solved_tight) || a.k + 1 == a.max_outer));
         const int plain = last || !a.aitken || a.k == 0 || v[2] > sc->rr_prev;
         double om = 1.;
         if (!plain) {
@@ -792,6 +930,30 @@ static __global__ void k_set_load(double2* b, double2* u, const double2* rp0, co
     if (!mk.x) w.x = factor * p.x;
     if (!mk.y) w.y = factor * p.y;
     u[n] = w;
+}
+
+// Load-step predictor: u <- u + ratio (u - u_prev), log psi likewise (psi >= 1);
+// the previous state moves to *_prev.
+static __global__ void k_predict_u(double2* u, double2* u_prev, double ratio, int nn)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nn) return;
+    const double2 a = u[n], b = u_prev[n];
+    u_prev[n] = a;
+    u[n] = make_double2(a.x + ratio * (a.x - b.x), a.y + ratio * (a.y - b.y));
+}
+
+static __global__ void k_predict_psi(double* psi, double* psi_prev, double2* ecnu, double ratio, double E, double nu,
+                                     int ne)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const double a = psi[e], b = psi_prev[e];
+    psi_prev[e] = a;
+    double p = exp(log(a) + ratio * (log(a) - log(b)));
+    if (!(p >= 1.)) p = 1.;
+    psi[e] = p;
+    ecnu[e] = make_double2(E / p, 0.5 * (1. - (1. - 2. * nu) / p));
 }
 
 static __global__ void k_rm_reset(RmScal* sc)

@@ -60,10 +60,15 @@ struct cp_problem {
     long long last_cg_iters = 0;
     double last_cg_ms_per_iter = 0.;
     bool have_fp64_values = false;
+    int precond = 0;          // block-Jacobi PCG for the current solve
+    bool solved = false;      // u / psi hold a converged level at `load`
+    double lam_prev = 0.;     // load factor of u_prev / psi_prev
 
     cp::Arena common, a64, amx;
     // common
-    double2 *xy = nullptr, *ecnu = nullptr, *rp0d = nullptr, *u0d = nullptr, *b = nullptr, *u = nullptr;
+    double2 *xy = nullptr, *ecnu = nullptr, *rp0d = nullptr, *u0d = nullptr, *b = nullptr, *u = nullptr,
+            *u_prev = nullptr;
+    double* psi_prev = nullptr;
     int4* conn = nullptr;
     uchar2* mask = nullptr;
     double *psi = nullptr, *F = nullptr, *G = nullptr, *rcur = nullptr, *rprev = nullptr, *partials = nullptr;
@@ -74,8 +79,10 @@ struct cp_problem {
     cpk::RefScal* refs = nullptr;
     // FP64 solve
     double2 *va = nullptr, *vb = nullptr, *p64 = nullptr, *r64[2] = {}, *w64[2] = {}, *s64[2] = {};
+    double4* minv64 = nullptr;
     // mixed solve
     float4* vf = nullptr;
+    float4* minv32 = nullptr;
     float2 *x32 = nullptr, *p32 = nullptr, *r32[2] = {}, *w32[2] = {}, *s32[2] = {}, *b32 = nullptr;
     double2* res64 = nullptr;
     // pinned host mirrors of the device scalars
@@ -94,4 +101,19 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
                    const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
                    hipStream_t stream, cp_problem** out);
 int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st);
+int launch_assemble(cp_problem* P, int precision);
+double host_bnorm2(const cp_problem* P);
+int set_load(cp_problem* P, double factor, int reset);
+int advance_load(cp_problem* P, double factor, int predictor);
+int reset_state(cp_problem* P);
+cpk::ResArgs residual_args(cp_problem* P);
+int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int maxit);
+int enqueue_residual(cp_problem* P);
+int enqueue_correction(cp_problem* P);
+int enqueue_plastic(cp_problem* P);
+int enqueue_rm_reset(cp_problem* P);
+int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max_outer, int elastic, double tol,
+                      int solved_tight);
+template <typename T>
+void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st);
 }  // namespace cp

@@ -1,26 +1,562 @@
-// cp_server.hip -- serving loop (continuous batching of independent problems).
+// cp_server.hip -- serving loop: continuous batching of independent
+// plasticity problems on one GPU.
+//
+// Requests are admitted (FIFO) while a slot and the HBM budget allow; each
+// admitted request keeps its mesh, matrix, state and load level on the
+// device and advances through a small state machine
+//   level start -> assemble -> CG (or FP64 residual / FP32 CG refinement)
+//   -> return map -> next secant iteration / next level / done.
+// One scheduling round enqueues every request's next device work, then one
+// chunk of batched CG launches: a single launch advances one CG iteration of
+// every problem in its CG phase (tiles of all problems in one grid), each
+// problem with its own scalars and partial sums, so a result does not depend
+// on what else is in the batch.  One stream, one host synchronisation per
+// round; requests that arrive while others run are admitted at the next
+// round as slots and memory free up.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <vector>
 
 #include "cp_problem.h"
 
+namespace {
+
+enum Phase { PH_LEVEL, PH_CG, PH_WAIT_RES, PH_WAIT_RM, PH_DONE };
+enum State { RQ_QUEUED, RQ_ACTIVE, RQ_DONE, RQ_FAILED };
+
+struct Request {
+    long long id = 0;
+    int nn = 0, ne = 0;
+    std::vector<double> x, y, rp, u0;
+    std::vector<int> conn;
+    std::vector<unsigned char> mask;
+    cp_material mat{};
+    int nsteps = 1;
+    double final_factor = 1.;
+    cp_options opt{};
+    size_t bytes = 0;
+    State state = RQ_QUEUED;
+    cp_problem* P = nullptr;
+    int phase = PH_LEVEL;
+    int level = 0, k = 0, tight = 1;
+    double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1.;
+    int stalled = 0;
+    long long cg_iters = 0;
+    int outer_total = 0;
+    int status = CP_OK;
+    double t_submit = 0., t_admit = -1., t_done = -1.;
+    double cg_rho_prev = -1.;  // CG residual at the last observation
+    int cg_iters_seen = 0;      // CG iteration count at the last observation
+    int next_chunk = 16;        // predicted iterations left in the current CG
+    std::vector<double> u_out, sig_out;
+};
+
+// Host mirrors of the device scalars of one request, filled each round.
+struct Mirror {
+    cpk::CgScal cg;
+    cpk::RmScal rm;
+    cpk::RefScal ref;
+};
+
+struct PackItem {
+    const char* src;
+    int dst_off;
+    int nbytes;
+};
+
+__global__ void k_pack(const PackItem* items, int n, char* dst)
+{
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const PackItem it = items[i];
+    for (int b = threadIdx.x; b < it.nbytes; b += blockDim.x) dst[it.dst_off + b] = it.src[b];
+}
+
+double clock_now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct cp_server {
+    cp_ctx* ctx = nullptr;
+    cp_server_config cfg{};
+    hipStream_t stream = nullptr;
+    double t0 = 0.;
+    long long next_id = 1;
+    size_t used = 0;
+    std::map<long long, std::unique_ptr<Request>> reqs;
+    std::deque<Request*> queue;
+    std::vector<Request*> active;
+    // device tables for the batched CG and the scalar pack
+    char* dev = nullptr;
+    size_t dev_size = 0;
+    char* host = nullptr;  // pinned staging, same layout as dev
+    size_t host_size = 0;
+    double now() const { return clock_now() - t0; }
+};
+
+namespace {
+
+int ensure_buffers(cp_server* S, size_t bytes)
+{
+    if (bytes <= S->dev_size) return CP_OK;
+    size_t n = std::max(bytes, 2 * S->dev_size);
+    if (S->dev) (void)hipFree(S->dev);
+    if (S->host) (void)hipHostFree(S->host);
+    S->dev = nullptr;
+    S->host = nullptr;
+    CP_HIP(hipMalloc(&S->dev, n));
+    CP_HIP(hipHostMalloc((void**)&S->host, n, hipHostMallocDefault));
+    S->dev_size = S->host_size = n;
+    return CP_OK;
+}
+
+int fail_request(cp_server* S, Request* r, int code)
+{
+    r->status = code;
+    r->state = RQ_FAILED;
+    r->t_done = S->now();
+    if (r->P) {
+        S->used -= r->bytes;
+        cp_problem_destroy(r->P);
+        r->P = nullptr;
+    }
+    return CP_OK;
+}
+
+double level_factor(const Request* r, int level) { return r->final_factor * double(level) / double(r->nsteps); }
+
+// Enqueue the start of secant iteration k of the current level.
+int start_iteration(Request* r)
+{
+    cp_problem* P = r->P;
+    if (int s = cp::launch_assemble(P, r->opt.precision)) return s;
+    double ctol = r->opt.cg_tol;
+    if (r->opt.cg_forcing > 0. && r->opt.fixed_outer <= 0) ctol = std::max(r->opt.cg_tol, r->opt.cg_forcing * r->prev_rel);
+    r->ctol = ctol;
+    r->tight = ctol <= r->opt.cg_tol;
+    r->bnorm2 = cp::host_bnorm2(P);
+    const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : 1000000000;
+    if (r->opt.precision == CP_FP64) {
+        if (int s = cp::enqueue_cg_init(P, CP_FP64, r->bnorm2, ctol, maxit)) return s;
+        r->phase = PH_CG;
+        r->cg_rho_prev = -1.;
+        r->cg_iters_seen = 0;
+        r->next_chunk = 16;
+    } else {
+        r->ref_norm = r->bnorm2;
+        r->prev_rn = -1.;
+        r->stalled = 0;
+        if (int s = cp::enqueue_residual(P)) return s;
+        r->phase = PH_WAIT_RES;
+    }
+    return CP_OK;
+}
+
+int start_level(Request* r)
+{
+    cp_problem* P = r->P;
+    const double lam = level_factor(r, r->level);
+    if (r->level == 1) {
+        P->load = lam;
+        if (int s = cp::reset_state(P)) return s;
+    } else {
+        if (int s = cp::advance_load(P, lam, r->opt.predictor)) return s;
+    }
+    P->precond = r->opt.precond ? 1 : 0;
+    if (int s = cp::enqueue_rm_reset(P)) return s;
+    r->k = 0;
+    r->prev_rel = 1.;
+    return start_iteration(r);
+}
+
+int max_outer_of(const Request* r)
+{
+    return r->opt.fixed_outer > 0 ? r->opt.fixed_outer : std::max(1, r->opt.max_outer);
+}
+
+// Use the scalars read back this round; enqueue the request's next work.
+int advance(cp_server* S, Request* r, const Mirror& m)
+{
+    cp_problem* P = r->P;
+    switch (r->phase) {
+    case PH_CG: {
+        const cpk::CgScal& c = m.cg;
+        if (!c.done) return CP_OK;
+        r->cg_iters += c.iters;
+        if (c.status == 2) return fail_request(S, r, CP_ERR_NAN);
+        if (r->opt.precision == CP_FP64) {
+            if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
+                                              r->opt.tol, r->tight))
+                return s;
+            r->phase = PH_WAIT_RM;
+        } else {
+            if (int s = cp::enqueue_correction(P)) return s;
+            if (int s = cp::enqueue_residual(P)) return s;
+            r->phase = PH_WAIT_RES;
+        }
+        return CP_OK;
+    }
+    case PH_WAIT_RES: {
+        const double rn = m.ref.rnorm2;
+        if (rn != rn) return fail_request(S, r, CP_ERR_NAN);
+        if (r->ref_norm == 0.) r->ref_norm = rn;
+        if (rn == 0. || rn <= r->ctol * r->ctol * r->ref_norm || r->stalled >= 3) {
+            if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
+                                              r->opt.tol, r->tight && r->stalled < 3))
+                return s;
+            r->phase = PH_WAIT_RM;
+            return CP_OK;
+        }
+        if (r->prev_rn > 0. && rn > 0.25 * r->prev_rn) ++r->stalled;
+        r->prev_rn = rn;
+        const double inner = r->opt.inner_tol > 0 ? r->opt.inner_tol : 1e-3;
+        const double need = 0.5 * r->ctol * std::sqrt(r->ref_norm / rn);
+        const double itol = std::max(inner, std::min(need, 0.5));
+        const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : 1000000000;
+        if (int s = cp::enqueue_cg_init(P, CP_MIXED, rn, itol, maxit)) return s;
+        r->phase = PH_CG;
+        r->cg_rho_prev = -1.;
+        r->cg_iters_seen = 0;
+        r->next_chunk = 16;
+        return CP_OK;
+    }
+    case PH_WAIT_RM: {
+        const cpk::RmScal& h = m.rm;
+        ++r->outer_total;
+        if (h.status == 2) return fail_request(S, r, CP_ERR_NAN);
+        if (!h.last) {
+            ++r->k;
+            r->prev_rel = h.rel_err <= r->opt.tol ? 0. : h.rel_err;
+            return start_iteration(r);
+        }
+        if (!h.converged) return fail_request(S, r, CP_ERR_NOT_CONVERGED);
+        // end of a level, as in cp_solve_steps: plastic strains, then the next level
+        if (int s = cp::enqueue_plastic(P)) return s;
+        P->solved = true;
+        if (r->level < r->nsteps) {
+            ++r->level;
+            return start_level(r);
+        }
+        r->phase = PH_DONE;
+        return CP_OK;
+    }
+    default:
+        return CP_OK;
+    }
+}
+
+int admit(cp_server* S)
+{
+    while (!S->queue.empty() && (int)S->active.size() < S->cfg.max_slots) {
+        Request* r = S->queue.front();
+        if (r->bytes > S->cfg.hbm_budget) {
+            S->queue.pop_front();
+            fail_request(S, r, CP_ERR_BUDGET);
+            continue;
+        }
+        if (S->used + r->bytes > S->cfg.hbm_budget) break;
+        S->queue.pop_front();
+        cp_problem* P = nullptr;
+        int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
+                                   r->u0.data(), r->mask.data(), &r->mat, S->stream, &P);
+        if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
+        if (s != CP_OK) {
+            if (P) cp_problem_destroy(P);
+            fail_request(S, r, s);
+            continue;
+        }
+        r->P = P;
+        r->state = RQ_ACTIVE;
+        r->t_admit = S->now();
+        S->used += r->bytes;
+        // inputs now live on the device
+        std::vector<double>().swap(r->x);
+        std::vector<double>().swap(r->y);
+        std::vector<int>().swap(r->conn);
+        r->level = 1;
+        if (int q = start_level(r)) {
+            fail_request(S, r, q);
+            continue;
+        }
+        S->active.push_back(r);
+    }
+    return CP_OK;
+}
+
+// Batched CG over the requests currently in their CG phase.
+template <typename T>
+struct Group {
+    std::vector<Request*> reqs;
+    int pre = 0, maxw = 0;
+};
+
+template <int W, bool PRE, typename T>
+void launch_batch_w(const cpk::CgArgs<T>* d_args, const int2* d_tiles, int ntiles, int nprob, int chunk, hipStream_t st)
+{
+    for (int j = 0; j < chunk; ++j) {
+        hipLaunchKernelGGL((cpk::k_cg_iter_batched<W, PRE, T>), dim3(ntiles), dim3(cpk::kBlock), 0, st, d_args, d_tiles,
+                           ntiles);
+        hipLaunchKernelGGL((cpk::k_cg_step_batched<T>), dim3(nprob), dim3(cpk::kBlock), 0, st, d_args, nprob);
+    }
+}
+
+template <typename T>
+void launch_batch(int maxw, int pre, const cpk::CgArgs<T>* d_args, const int2* d_tiles, int ntiles, int nprob,
+                  int chunk, hipStream_t st)
+{
+    if (pre) {
+        if (maxw <= 9) launch_batch_w<9, true, T>(d_args, d_tiles, ntiles, nprob, chunk, st);
+        else launch_batch_w<0, true, T>(d_args, d_tiles, ntiles, nprob, chunk, st);
+    } else {
+        if (maxw <= 9) launch_batch_w<9, false, T>(d_args, d_tiles, ntiles, nprob, chunk, st);
+        else launch_batch_w<0, false, T>(d_args, d_tiles, ntiles, nprob, chunk, st);
+    }
+}
+
+}  // namespace
+
 extern "C" {
 
-int cp_server_create(cp_ctx*, const cp_server_config*, cp_server** out)
+int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out)
 {
-    if (out) *out = nullptr;
-    return cp::fail(CP_ERR_STATE, "serving loop not built yet");
+    if (!out) return cp::fail(CP_ERR_INVALID, "null out");
+    *out = nullptr;
+    if (!ctx || !cfg || cfg->max_slots < 1 || cfg->hbm_budget == 0)
+        return cp::fail(CP_ERR_INVALID, "cp_server_create: bad configuration");
+    CP_HIP(hipSetDevice(ctx->device));
+    std::unique_ptr<cp_server> S(new cp_server());
+    S->ctx = ctx;
+    S->cfg = *cfg;
+    S->t0 = clock_now();
+    CP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    if (int s = ensure_buffers(S.get(), 1 << 20)) return s;
+    *out = S.release();
+    return CP_OK;
 }
-int cp_server_destroy(cp_server*) { return CP_OK; }
-int cp_server_submit(cp_server*, int, int, const double*, const double*, const int*, const double*, const double*,
-                     const unsigned char*, const cp_material*, int, double, const cp_options*, long long*)
+
+int cp_server_destroy(cp_server* S)
 {
-    return cp::fail(CP_ERR_STATE, "serving loop not built yet");
+    if (!S) return CP_OK;
+    (void)hipStreamSynchronize(S->stream);
+    for (auto& kv : S->reqs)
+        if (kv.second->P) cp_problem_destroy(kv.second->P);
+    if (S->dev) (void)hipFree(S->dev);
+    if (S->host) (void)hipHostFree(S->host);
+    (void)hipStreamDestroy(S->stream);
+    delete S;
+    return CP_OK;
 }
-int cp_server_step(cp_server*, int*) { return cp::fail(CP_ERR_STATE, "serving loop not built yet"); }
-int cp_server_result(cp_server*, long long, cp_request_info*, double*, double*)
+
+int cp_server_clock(cp_server* S, double* now)
 {
-    return cp::fail(CP_ERR_STATE, "serving loop not built yet");
+    if (!S || !now) return cp::fail(CP_ERR_INVALID, "null argument");
+    *now = S->now();
+    return CP_OK;
 }
-int cp_server_clock(cp_server*, double*) { return cp::fail(CP_ERR_STATE, "serving loop not built yet"); }
+
+int cp_server_submit(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn, const double* rp,
+                     const double* u0, const unsigned char* mask, const cp_material* mat, int nsteps,
+                     double final_factor, const cp_options* opt, long long* id)
+{
+    if (!S || !x || !y || !conn || !rp || !mask || !mat || !opt || nsteps < 1 || nn < 1 || ne < 1)
+        return cp::fail(CP_ERR_INVALID, "cp_server_submit: bad arguments");
+    if (opt->precision != CP_FP64 && opt->precision != CP_MIXED) return cp::fail(CP_ERR_INVALID, "bad precision");
+    std::unique_ptr<Request> r(new Request());
+    r->id = S->next_id++;
+    r->nn = nn;
+    r->ne = ne;
+    r->x.assign(x, x + nn);
+    r->y.assign(y, y + nn);
+    r->conn.assign(conn, conn + 4 * (size_t)ne);
+    r->rp.assign(rp, rp + 2 * (size_t)nn);
+    r->u0.assign(2 * (size_t)nn, 0.);
+    if (u0) r->u0.assign(u0, u0 + 2 * (size_t)nn);
+    r->mask.assign(mask, mask + 2 * (size_t)nn);
+    r->mat = *mat;
+    r->nsteps = nsteps;
+    r->final_factor = final_factor;
+    r->opt = *opt;
+    // matrix blocks of a quad mesh ~ 9 per node: budget with the layout estimate
+    size_t bytes = 0;
+    if (int s = cp_problem_bytes(nn, ne, 4LL * 9 * nn, opt->precision, &bytes)) return s;
+    r->bytes = bytes;
+    r->t_submit = S->now();
+    Request* raw = r.get();
+    S->reqs[raw->id] = std::move(r);
+    S->queue.push_back(raw);
+    if (id) *id = raw->id;
+    return CP_OK;
+}
+
+int cp_server_step(cp_server* S, int* in_flight)
+{
+    if (!S) return cp::fail(CP_ERR_INVALID, "null server");
+    if (int s = admit(S)) return s;
+    hipStream_t st = S->stream;
+    std::vector<Request*>& act = S->active;
+    if (!act.empty()) {
+        // requests in their CG phase, grouped by (precision, preconditioner)
+        std::vector<Request*> g64[2], g32[2];
+        for (Request* r : act)
+            if (r->phase == PH_CG) (r->opt.precision == CP_FP64 ? g64 : g32)[r->opt.precond ? 1 : 0].push_back(r);
+        // device tables: [CgArgs groups][tiles][pack items] then the pack output
+        size_t off = 0;
+        auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+        struct Launch {
+            int prec, pre, maxw, ntiles, nprob;
+            size_t args_off, tiles_off;
+        };
+        std::vector<Launch> launches;
+        size_t need = 0;
+        for (int p = 0; p < 2; ++p)
+            for (int pre = 0; pre < 2; ++pre) {
+                auto& g = p == 0 ? g64[pre] : g32[pre];
+                if (g.empty()) continue;
+                int nt = 0;
+                for (Request* r : g) nt += r->P->nbn;
+                need += align(g.size() * sizeof(cpk::CgArgs<double>)) + align(nt * sizeof(int2));
+            }
+        need += align(act.size() * 3 * sizeof(PackItem)) + align(act.size() * sizeof(Mirror));
+        if (int s = ensure_buffers(S, need)) return s;
+        for (int p = 0; p < 2; ++p)
+            for (int pre = 0; pre < 2; ++pre) {
+                auto& g = p == 0 ? g64[pre] : g32[pre];
+                if (g.empty()) continue;
+                Launch L{p, pre, 0, 0, (int)g.size(), off, 0};
+                for (size_t i = 0; i < g.size(); ++i) {
+                    if (p == 0) reinterpret_cast<cpk::CgArgs<double>*>(S->host + off)[i] = g[i]->P->cg64_args();
+                    else reinterpret_cast<cpk::CgArgs<float>*>(S->host + off)[i] = g[i]->P->cg32_args();
+                    L.maxw = std::max(L.maxw, g[i]->P->max_w);
+                }
+                off += align(g.size() * (p == 0 ? sizeof(cpk::CgArgs<double>) : sizeof(cpk::CgArgs<float>)));
+                L.tiles_off = off;
+                int2* tiles = reinterpret_cast<int2*>(S->host + off);
+                for (size_t i = 0; i < g.size(); ++i)
+                    for (int b = 0; b < g[i]->P->nbn; ++b) tiles[L.ntiles++] = make_int2((int)i, b);
+                off += align(L.ntiles * sizeof(int2));
+                launches.push_back(L);
+            }
+        const size_t pack_off = off;
+        PackItem* items = reinterpret_cast<PackItem*>(S->host + pack_off);
+        off += align(act.size() * 3 * sizeof(PackItem));
+        const size_t mir_off = off;
+        int ni = 0;
+        for (size_t i = 0; i < act.size(); ++i) {
+            cp_problem* P = act[i]->P;
+            const int base = (int)(i * sizeof(Mirror));
+            const char* cg = (const char*)(act[i]->opt.precision == CP_FP64 ? P->sc64 : P->sc32);
+            items[ni++] = PackItem{cg, base + (int)offsetof(Mirror, cg), (int)sizeof(cpk::CgScal)};
+            items[ni++] = PackItem{(const char*)P->rms, base + (int)offsetof(Mirror, rm), (int)sizeof(cpk::RmScal)};
+            items[ni++] = PackItem{(const char*)P->refs, base + (int)offsetof(Mirror, ref), (int)sizeof(cpk::RefScal)};
+        }
+        off += align(act.size() * sizeof(Mirror));
+        CP_HIP(hipMemcpyAsync(S->dev, S->host, pack_off + align(ni * sizeof(PackItem)), hipMemcpyHostToDevice, st));
+
+        // one chunk of CG iterations for every request in its CG phase: as
+        // many as the member closest to convergence is predicted to need
+        int chunk = 0;
+        for (Request* r : act)
+            if (r->phase == PH_CG) chunk = chunk == 0 ? r->next_chunk : std::min(chunk, r->next_chunk);
+        chunk = std::max(4, chunk == 0 ? 16 : chunk);
+        for (const Launch& L : launches) {
+            if (S->cfg.batch_cg) {
+                if (L.prec == 0)
+                    launch_batch<double>(L.maxw, L.pre, reinterpret_cast<const cpk::CgArgs<double>*>(S->dev + L.args_off),
+                                         reinterpret_cast<const int2*>(S->dev + L.tiles_off), L.ntiles, L.nprob, chunk, st);
+                else
+                    launch_batch<float>(L.maxw, L.pre, reinterpret_cast<const cpk::CgArgs<float>*>(S->dev + L.args_off),
+                                        reinterpret_cast<const int2*>(S->dev + L.tiles_off), L.ntiles, L.nprob, chunk, st);
+            } else {
+                auto& g = L.prec == 0 ? g64[L.pre] : g32[L.pre];
+                for (Request* r : g) {
+                    cp_problem* P = r->P;
+                    if (L.prec == 0) cp::launch_cg_iters<double>(P->max_w, L.pre, P->cg64_args(), chunk, dim3(P->nbn), st);
+                    else cp::launch_cg_iters<float>(P->max_w, L.pre, P->cg32_args(), chunk, dim3(P->nbn), st);
+                }
+            }
+        }
+        CP_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_pack, dim3(ni), dim3(64), 0, st, reinterpret_cast<const PackItem*>(S->dev + pack_off), ni,
+                           S->dev + mir_off);
+        CP_HIP(hipGetLastError());
+        CP_HIP(hipMemcpyAsync(S->host + mir_off, S->dev + mir_off, act.size() * sizeof(Mirror), hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        const Mirror* mir = reinterpret_cast<const Mirror*>(S->host + mir_off);
+
+        // advance every request with this round's scalars
+        std::vector<Request*> keep;
+        std::vector<Mirror> mcopy(mir, mir + act.size());
+        for (size_t i = 0; i < act.size(); ++i) {
+            Request* r = act[i];
+            if (r->phase == PH_CG) {
+                // remaining-iteration estimate from the observed decay of (r, r)
+                const cpk::CgScal& g = mcopy[i].cg;
+                int next = std::min(1024, 2 * chunk);
+                if (r->cg_rho_prev > 0. && g.rho > 0. && g.rho < r->cg_rho_prev && g.iters > r->cg_iters_seen) {
+                    const double target = g.tol2 * g.bnorm2;
+                    const double rate = std::log(r->cg_rho_prev / g.rho) / double(g.iters - r->cg_iters_seen);
+                    const double rem = std::log(g.rho / target) / rate;
+                    if (rem > 0 && rem < 1e7) next = (int)std::min(1024., std::max(4., rem * 1.05 + 2.));
+                }
+                r->cg_rho_prev = g.rho;
+                r->cg_iters_seen = g.iters;
+                r->next_chunk = next;
+            }
+            if (int s = advance(S, r, mcopy[i])) fail_request(S, r, s);
+            if (r->state == RQ_ACTIVE && r->phase == PH_DONE) {
+                CP_HIP(hipStreamSynchronize(st));
+                r->u_out.resize(2 * (size_t)r->nn);
+                r->sig_out.resize(r->ne);
+                if (cp_download(r->P, CP_F_U, r->u_out.data()) || cp_download(r->P, CP_F_SIGI, r->sig_out.data())) {
+                    fail_request(S, r, CP_ERR_DEVICE);
+                } else {
+                    r->state = RQ_DONE;
+                    r->t_done = S->now();
+                    S->used -= r->bytes;
+                    cp_problem_destroy(r->P);
+                    r->P = nullptr;
+                }
+            }
+            if (r->state == RQ_ACTIVE) keep.push_back(r);
+        }
+        act.swap(keep);
+    }
+    if (in_flight) *in_flight = (int)(S->active.size() + S->queue.size());
+    return CP_OK;
+}
+
+int cp_server_result(cp_server* S, long long id, cp_request_info* info, double* u, double* sig_i)
+{
+    if (!S) return cp::fail(CP_ERR_INVALID, "null server");
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return cp::fail(CP_ERR_INVALID, "unknown request id");
+    Request* r = it->second.get();
+    if (info) {
+        info->id = r->id;
+        info->status = r->state == RQ_DONE ? CP_OK : (r->state == RQ_FAILED ? r->status : CP_ERR_STATE);
+        info->steps_done = r->state == RQ_DONE ? r->nsteps : std::max(0, r->level - 1);
+        info->cg_iters = r->cg_iters;
+        info->outer_iters = r->outer_total;
+        info->t_submit = r->t_submit;
+        info->t_admit = r->t_admit;
+        info->t_done = r->t_done;
+    }
+    if (r->state == RQ_FAILED) return cp::failf(r->status, "request %lld failed", id);
+    if (r->state != RQ_DONE) return cp::fail(CP_ERR_STATE, "request not finished");
+    if (u) std::memcpy(u, r->u_out.data(), sizeof(double) * r->u_out.size());
+    if (sig_i) std::memcpy(sig_i, r->sig_out.data(), sizeof(double) * r->sig_out.size());
+    return CP_OK;
+}
 
 }  // extern "C"

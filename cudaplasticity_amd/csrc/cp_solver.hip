@@ -51,16 +51,17 @@ Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int
 {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     Layout L{};
-    L.common = al(sizeof(double2) * nn) * 5 +      // xy, rp0, u0, b, u
+    L.common = al(sizeof(double2) * nn) * 6 +      // xy, rp0, u0, b, u, u_prev
                al(sizeof(uchar2) * nn) +
                al(sizeof(int4) * ne) + al(sizeof(double2) * ne) +
-               al(sizeof(double) * ne) * (4 + CP_NFIELDS) +   // psi, G, rcur, rprev, fields
+               al(sizeof(double) * ne) * (5 + CP_NFIELDS) +   // psi, psi_prev, G, rcur, rprev, fields
                al(sizeof(int) * (nn + 1)) + al(sizeof(int2) * ninc) +
                al(sizeof(int) * nblk_stored) + al(sizeof(int2) * nslices) +
                al(sizeof(double) * 5 * (size_t)nbmax) +
                4 * 256;                                        // scalars
-    L.fp64 = al(sizeof(double2) * nblk_stored) * 2 + al(sizeof(double2) * nn) * 7;
-    L.mixed = al(sizeof(float4) * nblk_stored) + al(sizeof(float2) * nn) * 9 + al(sizeof(double2) * nn);
+    L.fp64 = al(sizeof(double2) * nblk_stored) * 2 + al(sizeof(double2) * nn) * 7 + al(sizeof(double4) * nn);
+    L.mixed = al(sizeof(float4) * nblk_stored) + al(sizeof(float2) * nn) * 9 + al(sizeof(double2) * nn) +
+              al(sizeof(float4) * nn);
     return L;
 }
 
@@ -82,6 +83,7 @@ cpk::CgArgs<double> cp_problem::cg64_args() const
     }
     a.b = b;
     a.mask = mask;
+    a.minv = minv64;
     a.sc = sc64;
     a.partials = partials;
     a.nn = nn;
@@ -105,6 +107,7 @@ cpk::CgArgs<float> cp_problem::cg32_args() const
     }
     a.b = b32;
     a.mask = mask;
+    a.minv = minv32;
     a.sc = sc32;
     a.partials = partials;
     a.nn = nn;
@@ -134,6 +137,7 @@ int problem_alloc_solver(cp_problem* P, int precision)
             P->w64[i] = P->a64.take<double2>(P->nn);
             P->s64[i] = P->a64.take<double2>(P->nn);
         }
+        P->minv64 = P->a64.take<double4>(P->nn);
         return CP_OK;
     }
     if (precision == CP_MIXED) {
@@ -156,6 +160,7 @@ int problem_alloc_solver(cp_problem* P, int precision)
         }
         P->b32 = P->amx.take<float2>(P->nn);
         P->res64 = P->amx.take<double2>(P->nn);
+        P->minv32 = P->amx.take<float4>(P->nn);
         return CP_OK;
     }
     return failf(CP_ERR_INVALID, "unknown precision %d", precision);
@@ -255,10 +260,12 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->u0d = A.take<double2>(nn);
     P->b = A.take<double2>(nn);
     P->u = A.take<double2>(nn);
+    P->u_prev = A.take<double2>(nn);
     P->mask = A.take<uchar2>(nn);
     P->conn = A.take<int4>(ne);
     P->ecnu = A.take<double2>(ne);
     P->psi = A.take<double>(ne);
+    P->psi_prev = A.take<double>(ne);
     P->G = A.take<double>(ne);
     P->rcur = A.take<double>(ne);
     P->rprev = A.take<double>(ne);
@@ -312,6 +319,9 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     const int nbe = P->nbe, nbn = P->nbn;
     hipLaunchKernelGGL(cpk::k_init_state, dim3(nbe), dim3(cpk::kBlock), 0, st, P->psi, P->ecnu, P->rprev, P->M.E,
                        P->M.nu, ne);
+    hipLaunchKernelGGL(cpk::k_init_state, dim3(nbe), dim3(cpk::kBlock), 0, st, P->psi_prev, P->ecnu, P->rprev,
+                       P->M.E, P->M.nu, ne);
+    CP_HIP(hipMemsetAsync(P->u_prev, 0, sizeof(double2) * nn, st));
     hipLaunchKernelGGL(cpk::k_set_load, dim3(nbn), dim3(cpk::kBlock), 0, st, P->b, P->u, P->rp0d, P->u0d, P->mask,
                        1.0, 1, nn);
     CP_HIP(hipGetLastError());
@@ -325,7 +335,7 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
 
 // ---------------------------------------------------------------- kernels drivers
 
-static int launch_assemble(cp_problem* P, int precision)
+int launch_assemble(cp_problem* P, int precision)
 {
     if (int s = problem_alloc_solver(P, precision)) return s;
     cpk::AsmArgs a{};
@@ -360,15 +370,68 @@ static int launch_assemble(cp_problem* P, int precision)
         return fail(CP_ERR_INVALID, "row width too large");
     }
     CP_HIP(hipGetLastError());
+    if (P->precond) {
+        if (f64) hipLaunchKernelGGL(cpk::k_block_jacobi<double>, g, b, 0, P->stream, P->cg64_args(), P->minv64);
+        else hipLaunchKernelGGL(cpk::k_block_jacobi<float>, g, b, 0, P->stream, P->cg32_args(), P->minv32);
+        CP_HIP(hipGetLastError());
+    }
     if (f64) P->have_fp64_values = true;
     return CP_OK;
 }
 
-static double host_bnorm2(const cp_problem* P)
+double host_bnorm2(const cp_problem* P)
 {
     double s = 0.;
     for (double v : P->rp0) s += (P->load * v) * (P->load * v);
     return s;
+}
+
+// Launches `count` CG iterations with the row width unrolled to the mesh's
+// maximum (9 for the structured quad meshes: a node and its 8 neighbours).
+template <int W, bool PRE, typename T>
+static void launch_iters_w(const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st)
+{
+    for (int j = 0; j < count; ++j) {
+        hipLaunchKernelGGL((cpk::k_cg_iter<W, PRE, T>), g, dim3(cpk::kBlock), 0, st, a);
+        hipLaunchKernelGGL((cpk::k_cg_step<T>), dim3(1), dim3(cpk::kBlock), 0, st, a);
+    }
+}
+
+template <bool PRE, typename T>
+static void launch_iters_p(int max_w, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st)
+{
+    if (max_w <= 9) launch_iters_w<9, PRE, T>(a, count, g, st);
+    else if (max_w <= 12) launch_iters_w<12, PRE, T>(a, count, g, st);
+    else if (max_w <= 16) launch_iters_w<16, PRE, T>(a, count, g, st);
+    else launch_iters_w<0, PRE, T>(a, count, g, st);
+}
+
+template <typename T>
+void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st)
+{
+    if (pre) launch_iters_p<true, T>(max_w, a, count, g, st);
+    else launch_iters_p<false, T>(max_w, a, count, g, st);
+}
+template void launch_cg_iters<double>(int, int, const cpk::CgArgs<double>&, int, dim3, hipStream_t);
+template void launch_cg_iters<float>(int, int, const cpk::CgArgs<float>&, int, dim3, hipStream_t);
+
+template <int W, bool PRE, typename T>
+static void launch_init2_w(const cpk::CgArgs<T>& a, dim3 g, hipStream_t st)
+{
+    hipLaunchKernelGGL((cpk::k_cg_init2<W, PRE, T>), g, dim3(cpk::kBlock), 0, st, a);
+    hipLaunchKernelGGL((cpk::k_cg_start<T>), dim3(1), dim3(cpk::kBlock), 0, st, a);
+}
+
+template <typename T>
+static void launch_init2(int max_w, int pre, const cpk::CgArgs<T>& a, dim3 g, hipStream_t st)
+{
+    if (pre) {
+        if (max_w <= 9) launch_init2_w<9, true, T>(a, g, st);
+        else launch_init2_w<0, true, T>(a, g, st);
+    } else {
+        if (max_w <= 9) launch_init2_w<9, false, T>(a, g, st);
+        else launch_init2_w<0, false, T>(a, g, st);
+    }
 }
 
 // Runs a CG solve whose setup kernels have been launched: chunks of
@@ -378,14 +441,14 @@ static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal
                              long long* iters, int* status, double* ms)
 {
     hipStream_t st = P->stream;
-    const dim3 g(P->nbn), b(cpk::kBlock);
+    const dim3 g(P->nbn);
     CP_HIP(hipEventRecord(P->ev[6], st));
     long long launched = 0;
     int chunk = (int)std::min<long long>(4096, std::max<long long>(16, P->last_cg_iters / 2));
     double g_prev = -1.;
     long long it_prev = 0;
     for (;;) {
-        for (int j = 0; j < chunk; ++j) hipLaunchKernelGGL(cpk::k_cg_iter<T>, g, b, 0, st, a);
+        launch_cg_iters<T>(P->max_w, P->precond, a, chunk, g, st);
         CP_HIP(hipGetLastError());
         launched += chunk;
         CP_HIP(hipMemcpyAsync(P->h_cg, dsc, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
@@ -395,12 +458,12 @@ static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal
         // predict the remaining iterations from the observed decay of gamma
         const double target = tol2 * (bnorm2 > 0. ? bnorm2 : h.bnorm2);
         int next = std::min(4096, chunk * 2);
-        if (g_prev > 0. && h.gamma > 0. && h.gamma < g_prev && target > 0.) {
-            const double rate = std::log(g_prev / h.gamma) / double(h.iters - it_prev);
-            const double rem = std::log(h.gamma / target) / rate;
+        if (g_prev > 0. && h.rho > 0. && h.rho < g_prev && target > 0.) {
+            const double rate = std::log(g_prev / h.rho) / double(h.iters - it_prev);
+            const double rem = std::log(h.rho / target) / rate;
             if (rem > 0 && rem < 1e7) next = (int)std::min(4096., std::max(8., rem * 1.05 + 4.));
         }
-        g_prev = h.gamma;
+        g_prev = h.rho;
         it_prev = h.iters;
         chunk = next;
         if (launched > (long long)h.maxit + 8192) break;  // the kernel stops itself at maxit
@@ -418,18 +481,13 @@ static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal
 // One masked CG solve of K u = b from the current u (FP64).
 static int cg_fp64(cp_problem* P, double tol, int maxit, long long* iters, double* relres, double* ms)
 {
-    hipStream_t st = P->stream;
-    const dim3 g(P->nbn), b(cpk::kBlock);
     const double bn = host_bnorm2(P);
     auto a = P->cg64_args();
-    hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, bn, tol * tol, maxit);
-    hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
-    hipLaunchKernelGGL(cpk::k_cg_init2<double>, g, b, 0, st, a);
-    CP_HIP(hipGetLastError());
+    if (int s = enqueue_cg_init(P, CP_FP64, bn, tol, maxit)) return s;
     int status = 0;
     if (int s = run_cg_iterations<double>(P, a, P->sc64, tol * tol, bn, iters, &status, ms)) return s;
     const cpk::CgScal h = *P->h_cg;
-    if (relres) *relres = std::sqrt(h.gamma / (h.bnorm2 > 0 ? h.bnorm2 : 1.));
+    if (relres) *relres = std::sqrt(h.rho / (h.bnorm2 > 0 ? h.bnorm2 : 1.));
     P->last_cg_iters = *iters;
     if (*iters > 0) P->last_cg_ms_per_iter = *ms / double(*iters);
     if (status == 2) return fail(CP_ERR_NAN, "CG: NaN in the residual");
@@ -437,13 +495,8 @@ static int cg_fp64(cp_problem* P, double tol, int maxit, long long* iters, doubl
     return CP_OK;
 }
 
-// FP32 CG on K32 d = r64 with FP64 iterative refinement of u.
-static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long long* iters, long long* refs,
-                    double* relres, double* ms)
+cpk::ResArgs residual_args(cp_problem* P)
 {
-    hipStream_t st = P->stream;
-    const dim3 g(P->nbn), b(cpk::kBlock);
-    const double bn = host_bnorm2(P);
     cpk::ResArgs ra{};
     ra.incPtr = P->inc_ptr;
     ra.inc = P->inc;
@@ -461,6 +514,68 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
     ra.nblocks = P->nbn;
     ra.plane = P->M.plane;
     ra.h = P->M.h;
+    return ra;
+}
+
+// CG setup + r_0 / w_0 kernels (FP64 from the current u, or FP32 from zero
+// on the refinement residual).
+int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int maxit)
+{
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    hipStream_t st = P->stream;
+    if (precision == CP_FP64) {
+        auto a = P->cg64_args();
+        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, bnorm2, tol * tol, maxit);
+        hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
+        launch_init2<double>(P->max_w, P->precond, a, g, st);
+    } else {
+        auto a = P->cg32_args();
+        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, bnorm2, tol * tol, maxit);
+        hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
+        launch_init2<float>(P->max_w, P->precond, a, g, st);
+    }
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int enqueue_residual(cp_problem* P)
+{
+    hipLaunchKernelGGL(cpk::k_residual64, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, residual_args(P));
+    hipLaunchKernelGGL(cpk::k_residual_total, dim3(1), dim3(cpk::kBlock), 0, P->stream, P->refs,
+                       (const double*)P->partials, P->nbn);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int enqueue_correction(cp_problem* P)
+{
+    hipLaunchKernelGGL(cpk::k_add_correction, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, P->u,
+                       (const float2*)P->x32, P->nn, P->nbn);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int enqueue_plastic(cp_problem* P)
+{
+    hipLaunchKernelGGL(cpk::k_plastic_strains, dim3(P->nbe), dim3(cpk::kBlock), 0, P->stream, P->F, P->psi,
+                       (const double2*)P->ecnu, P->M.E, P->M.nu, P->ne, P->nbe);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int enqueue_rm_reset(cp_problem* P)
+{
+    hipLaunchKernelGGL(cpk::k_rm_reset, dim3(1), dim3(1), 0, P->stream, P->rms);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+// FP32 CG on K32 d = r64 with FP64 iterative refinement of u.
+static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long long* iters, long long* refs,
+                    double* relres, double* ms)
+{
+    hipStream_t st = P->stream;
+    const double bn = host_bnorm2(P);
     auto a = P->cg32_args();
     *iters = 0;
     *refs = 0;
@@ -469,8 +584,7 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
     int stalled = 0;
     double prev = -1.;
     for (int k = 0; k < 200; ++k) {
-        hipLaunchKernelGGL(cpk::k_residual64, g, b, 0, st, ra);
-        CP_HIP(hipGetLastError());
+        if (int s = enqueue_residual(P)) return s;
         CP_HIP(hipMemcpyAsync(P->h_ref, P->refs, sizeof(cpk::RefScal), hipMemcpyDeviceToHost, st));
         CP_HIP(hipStreamSynchronize(st));
         const double rn = P->h_ref->rnorm2;
@@ -484,27 +598,27 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
                                              std::sqrt(rn / ref_norm));
         }
         prev = rn;
-        hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, rn, inner_tol * inner_tol, maxit);
-        hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
-        hipLaunchKernelGGL(cpk::k_cg_init2<float>, g, b, 0, st, a);
-        CP_HIP(hipGetLastError());
+        // ask one FP32 solve for the reduction still needed (half of it, for
+        // margin), but never for less than inner_tol: FP32 cannot go further
+        const double need = 0.5 * tol * std::sqrt(ref_norm / rn);
+        const double itol = std::max(inner_tol, std::min(need, 0.5));
+        if (int s = enqueue_cg_init(P, CP_MIXED, rn, itol, maxit)) return s;
         long long it = 0;
         int status = 0;
         double t = 0.;
-        if (int s = run_cg_iterations<float>(P, a, P->sc32, inner_tol * inner_tol, rn, &it, &status, &t)) return s;
+        if (int s = run_cg_iterations<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t)) return s;
         *iters += it;
         *ms += t;
         P->last_cg_iters = it;
         if (it > 0) P->last_cg_ms_per_iter = t / double(it);
         if (status == 2) return fail(CP_ERR_NAN, "inner CG: NaN");
-        hipLaunchKernelGGL(cpk::k_add_correction, g, b, 0, st, P->u, (const float2*)P->x32, P->nn, P->nbn);
-        CP_HIP(hipGetLastError());
+        if (int s = enqueue_correction(P)) return s;
     }
     return fail(CP_ERR_NOT_CONVERGED, "refinement: 200 passes without convergence");
 }
 
-static int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max_outer, int elastic,
-                             double tol)
+int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max_outer, int elastic, double tol,
+                      int solved_tight)
 {
     cpk::RmArgs a{};
     a.conn = P->conn;
@@ -526,18 +640,19 @@ static int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, 
     a.fixed_outer = fixed_outer;
     a.max_outer = max_outer;
     a.elastic = elastic;
+    a.solved_tight = solved_tight;
     a.tol = tol;
     const dim3 g(P->nbe), b(cpk::kBlock);
     hipLaunchKernelGGL(cpk::k_return_map, g, b, 0, P->stream, a);
+    hipLaunchKernelGGL(cpk::k_rm_decide, dim3(1), b, 0, P->stream, a);
     hipLaunchKernelGGL(cpk::k_apply_update, g, b, 0, P->stream, P->psi, P->ecnu, (const double*)P->G,
                        (const double*)P->rcur, P->rprev, (const cpk::RmScal*)P->rms, P->M.E, P->M.nu, P->ne, P->nbe,
                        aitken);
     CP_HIP(hipGetLastError());
-    CP_HIP(hipMemcpyAsync(P->h_rm, P->rms, sizeof(cpk::RmScal), hipMemcpyDeviceToHost, P->stream));
     return CP_OK;
 }
 
-static int set_load(cp_problem* P, double factor, int reset)
+int set_load(cp_problem* P, double factor, int reset)
 {
     P->load = factor;
     hipLaunchKernelGGL(cpk::k_set_load, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, P->b, P->u, P->rp0d, P->u0d,
@@ -546,12 +661,41 @@ static int set_load(cp_problem* P, double factor, int reset)
     return CP_OK;
 }
 
-static int reset_state(cp_problem* P)
+int reset_state(cp_problem* P)
 {
     hipLaunchKernelGGL(cpk::k_init_state, dim3(P->nbe), dim3(cpk::kBlock), 0, P->stream, P->psi, P->ecnu, P->rprev,
                        P->M.E, P->M.nu, P->ne);
     CP_HIP(hipGetLastError());
+    CP_HIP(hipMemcpyAsync(P->psi_prev, P->psi, sizeof(double) * P->ne, hipMemcpyDeviceToDevice, P->stream));
+    CP_HIP(hipMemsetAsync(P->u_prev, 0, sizeof(double2) * P->nn, P->stream));
+    P->lam_prev = 0.;
+    P->solved = false;
     return set_load(P, P->load, 1);
+}
+
+// Next load level; with the predictor, extrapolate u and log(psi) from the
+// last two converged levels (the level before the first one is the
+// unloaded state u = 0, psi = 1 at factor 0).
+int advance_load(cp_problem* P, double factor, int predictor)
+{
+    if (P->solved && predictor && P->load != P->lam_prev) {
+        const double ratio = (factor - P->load) / (P->load - P->lam_prev);
+        hipLaunchKernelGGL(cpk::k_predict_u, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, P->u, P->u_prev, ratio,
+                           P->nn);
+        if (predictor >= 2)
+            hipLaunchKernelGGL(cpk::k_predict_psi, dim3(P->nbe), dim3(cpk::kBlock), 0, P->stream, P->psi,
+                               P->psi_prev, P->ecnu, ratio, P->M.E, P->M.nu, P->ne);
+        else
+            CP_HIP(hipMemcpyAsync(P->psi_prev, P->psi, sizeof(double) * P->ne, hipMemcpyDeviceToDevice, P->stream));
+        CP_HIP(hipGetLastError());
+        P->lam_prev = P->load;
+    } else if (P->solved) {
+        CP_HIP(hipMemcpyAsync(P->u_prev, P->u, sizeof(double2) * P->nn, hipMemcpyDeviceToDevice, P->stream));
+        CP_HIP(hipMemcpyAsync(P->psi_prev, P->psi, sizeof(double) * P->ne, hipMemcpyDeviceToDevice, P->stream));
+        P->lam_prev = P->load;
+    }
+    P->solved = false;
+    return set_load(P, factor, 0);
 }
 
 // One load level: the secant loop of solveCUDA (PlasticitySolver.cu:487-503)
@@ -563,6 +707,7 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     const int prec = opt->precision;
     if (prec != CP_FP64 && prec != CP_MIXED) return fail(CP_ERR_INVALID, "unknown precision");
     if (int s = problem_alloc_solver(P, prec)) return s;
+    P->precond = opt->precond ? 1 : 0;
     hipStream_t stream = P->stream;
     if (!opt->warm)
         if (int s = reset_state(P)) return s;
@@ -571,14 +716,21 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     const int maxit = opt->cg_maxit > 0 ? opt->cg_maxit : 1000000000;
     const double inner = opt->inner_tol > 0 ? opt->inner_tol : 1e-3;
     int rc = CP_OK;
+    double prev_rel = 1.0;
     for (int k = 0; k < max_outer; ++k) {
         CP_HIP(hipEventRecord(P->ev[0], stream));
         if (int s = launch_assemble(P, prec)) return s;
         CP_HIP(hipEventRecord(P->ev[1], stream));
         long long it = 0, refs = 0;
         double relres = 0., cgms = 0.;
-        int s = prec == CP_FP64 ? cg_fp64(P, opt->cg_tol, maxit, &it, &relres, &cgms)
-                                : cg_mixed(P, opt->cg_tol, maxit, inner, &it, &refs, &relres, &cgms);
+        // inexact secant iterations: the linear solve only needs to be as
+        // accurate as the current secant residual (cg_forcing > 0)
+        double ctol = opt->cg_tol;
+        if (opt->cg_forcing > 0. && !elastic && opt->fixed_outer <= 0)
+            ctol = std::max(opt->cg_tol, opt->cg_forcing * prev_rel);
+        const int tight = ctol <= opt->cg_tol;
+        int s = prec == CP_FP64 ? cg_fp64(P, ctol, maxit, &it, &relres, &cgms)
+                                : cg_mixed(P, ctol, maxit, inner, &it, &refs, &relres, &cgms);
         S.cg_iters += it;
         S.refinements += refs;
         S.cg_seconds += cgms * 1e-3;
@@ -590,7 +742,9 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
         }
         if (s && s != CP_ERR_NOT_CONVERGED) return s;
         CP_HIP(hipEventRecord(P->ev[2], stream));
-        if (int q = launch_return_map(P, k, opt->aitken, opt->fixed_outer, max_outer, elastic, opt->tol)) return q;
+        if (int q = launch_return_map(P, k, opt->aitken, opt->fixed_outer, max_outer, elastic, opt->tol, tight))
+            return q;
+        CP_HIP(hipMemcpyAsync(P->h_rm, P->rms, sizeof(cpk::RmScal), hipMemcpyDeviceToHost, stream));
         CP_HIP(hipEventRecord(P->ev[3], stream));
         CP_HIP(hipStreamSynchronize(stream));
         float ta = 0.f, tr = 0.f;
@@ -601,6 +755,8 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
         const cpk::RmScal h = *P->h_rm;
         S.outer_iters = k + 1;
         S.rel_err = h.rel_err;
+        // once the secant residual is below tol, the next solve is a tight one
+        prev_rel = h.rel_err <= opt->tol ? 0. : h.rel_err;
         if (h.status == 2) {
             rc = fail(CP_ERR_NAN, "NaN in the secant update");
             S.status = CP_ERR_NAN;
@@ -618,6 +774,7 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     }
     CP_HIP(hipStreamSynchronize(stream));
     S.seconds = now_s() - t0;
+    P->solved = (S.status == CP_OK);
     if (st) *st = S;
     if (rc) return rc;
     if (S.status == CP_ERR_NOT_CONVERGED) return fail(CP_ERR_NOT_CONVERGED, "secant loop hit its iteration cap");
@@ -759,6 +916,14 @@ int cp_set_load(cp_problem* P, double factor)
     return CP_OK;
 }
 
+int cp_advance_load(cp_problem* P, double factor, int predictor)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (int s = cp::advance_load(P, factor, predictor)) return s;
+    CP_HIP(hipStreamSynchronize(P->stream));
+    return CP_OK;
+}
+
 int cp_reset_state(cp_problem* P)
 {
     if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
@@ -790,7 +955,8 @@ int cp_return_map(cp_problem* P, double* rel_err)
 {
     if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
     hipLaunchKernelGGL(cpk::k_rm_reset, dim3(1), dim3(1), 0, P->stream, P->rms);
-    if (int s = cp::launch_return_map(P, 0, 0, 0, 1, 1, 0.)) return s;
+    if (int s = cp::launch_return_map(P, 0, 0, 0, 1, 1, 0., 1)) return s;
+    CP_HIP(hipMemcpyAsync(P->h_rm, P->rms, sizeof(cpk::RmScal), hipMemcpyDeviceToHost, P->stream));
     CP_HIP(hipStreamSynchronize(P->stream));
     if (rel_err) *rel_err = P->h_rm->rel_err;
     return P->h_rm->status == 2 ? cp::fail(CP_ERR_NAN, "NaN in the secant update") : CP_OK;
@@ -862,9 +1028,12 @@ int cp_solve_steps(cp_problem* P, int nsteps, double final_factor, const cp_opti
     const bool warm0 = o.warm != 0;
     for (int n = 1; n <= nsteps; ++n) {
         const double lam = final_factor * double(n) / double(nsteps);
-        if (int s = cp::set_load(P, lam, 0)) return s;
         o.warm = (n > 1 || warm0) ? 1 : 0;
-        if (!o.warm) P->load = lam;
+        if (o.warm) {
+            if (int s = cp::advance_load(P, lam, o.predictor)) return s;
+        } else {
+            P->load = lam;
+        }
         cp_stats st{};
         int s = cp::solve_level(P, &o, 0, &st);
         if (per_step) per_step[n - 1] = st;
@@ -973,9 +1142,9 @@ int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* byte
         // tol 0 and a huge cap keep every launch live
         hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, 1.0, 0.0, 2000000000);
         hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
-        hipLaunchKernelGGL(cpk::k_cg_init2<double>, g, b, 0, st, a);
+        cp::launch_init2<double>(P->max_w, P->precond, a, g, st);
         CP_HIP(hipEventRecord(P->ev[4], st));
-        for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(cpk::k_cg_iter<double>, g, b, 0, st, a);
+        cp::launch_cg_iters<double>(P->max_w, P->precond, a, reps, g, st);
     } else {
         auto a = P->cg32_args();
         // right-hand side of the inner solve: the FP64 residual of the current u
@@ -997,9 +1166,9 @@ int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* byte
         hipLaunchKernelGGL(cpk::k_residual64, g, b, 0, st, ra);
         hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, 1.0, 0.0, 2000000000);
         hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
-        hipLaunchKernelGGL(cpk::k_cg_init2<float>, g, b, 0, st, a);
+        cp::launch_init2<float>(P->max_w, P->precond, a, g, st);
         CP_HIP(hipEventRecord(P->ev[4], st));
-        for (int i = 0; i < reps; ++i) hipLaunchKernelGGL(cpk::k_cg_iter<float>, g, b, 0, st, a);
+        cp::launch_cg_iters<float>(P->max_w, P->precond, a, reps, g, st);
     }
     CP_HIP(hipGetLastError());
     CP_HIP(hipEventRecord(P->ev[5], st));
@@ -1008,7 +1177,10 @@ int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* byte
     CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
     if (ms) *ms = t / reps;
     // matrix once + 5 vector reads (r, w, s_prev, p, x) + 5 writes + mask
-    if (bytes_algo) *bytes_algo = P->nblk * (4. * s + 4.) + 8. * P->nslices + 10. * 2. * s * P->nn + 2. * P->nn;
+    // (+ the block-Jacobi inverse, 4 values per node, when preconditioned)
+    if (bytes_algo)
+        *bytes_algo = P->nblk * (4. * s + 4.) + 8. * P->nslices + 10. * 2. * s * P->nn + 2. * P->nn +
+                      (P->precond ? 4. * s * P->nn : 0.);
     return CP_OK;
 }
 

@@ -89,8 +89,14 @@ typedef struct {
     int    aitken;       /* 1: safeguarded Irons-Tuck acceleration of psi */
     int    precision;    /* CP_FP64 or CP_MIXED */
     int    warm;         /* 1: keep u and the secant moduli from the previous call */
-    double inner_tol;    /* CP_MIXED: relative reduction of each FP32 inner solve */
-    int    reserved[4];
+    double inner_tol;    /* CP_MIXED: smallest relative reduction one FP32 inner solve is asked
+                            for (each asks for what the FP64 residual test still needs) */
+    int    precond;      /* 0: unpreconditioned CG (the reference's); 1: 2x2 block-Jacobi PCG */
+    int    predictor;    /* load stepping initial guess from the last two levels: 0 none (previous
+                            level's state), 1 extrapolate u, 2 extrapolate u and log(psi) */
+    double cg_forcing;   /* > 0: inexact secant iterations -- secant iteration k solves to
+                            max(cg_tol, cg_forcing * rel_err_{k-1}) (rel_err_{-1} = 1); a level
+                            only exits after an iteration solved at cg_tol itself */
 } cp_options;
 
 typedef struct {
@@ -189,6 +195,11 @@ int cp_problem_get_values(cp_problem* prob, double* vals);
 
 /* Scale the load (rp, prescribed u) -- new: load stepping. */
 int cp_set_load(cp_problem* prob, double factor);
+/* Move to the next load level: with predictor >= 1 and a converged level
+ * behind, u (and with predictor = 2 also log(psi)) is extrapolated linearly
+ * in the load factor before the level's secant iterations start (initial
+ * guess only; the converged level does not depend on it). */
+int cp_advance_load(cp_problem* prob, double factor, int predictor);
 /* Reset u and the secant moduli to the initial state (initPlastParams :385-410). */
 int cp_reset_state(cp_problem* prob);
 

@@ -821,10 +821,17 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
     for (int e = 0; e < ne; ++e) rprev[e] = 0.;
     int maxOuter = opt->fixed_outer > 0 ? opt->fixed_outer : (opt->elastic ? 1 : opt->max_outer);
 
+    double prev_rel = 1.;
     for (int k = 0; k < maxOuter; ++k) {
         or_assemble(nn, ne, x, y, conn, rows, cols, F[OR_F_EC], F[OR_F_NUC], mat->plane, mat->h, vals);
         int it = 0;
-        int cgs = or_cg(N, rows, cols, vals, rp, mask, u, opt->cg_tol, opt->cg_maxit, opt->cg_quirk, &it);
+        double ctol = opt->cg_tol;
+        if (opt->cg_forcing > 0. && !opt->elastic && opt->fixed_outer <= 0) {
+            double f = opt->cg_forcing * prev_rel;
+            if (f > ctol) ctol = f;
+        }
+        int tight = ctol <= opt->cg_tol;
+        int cgs = or_cg(N, rows, cols, vals, rp, mask, u, ctol, opt->cg_maxit, opt->cg_quirk, &it);
         stats->cg_iters += it;
         stats->outer_iters = k + 1;
         if (cgs == OR_CG_NAN) { stats->status = 2; break; }
@@ -867,7 +874,8 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
         stats->rel_err = relErr;
         if (relErr != relErr) { stats->status = 2; break; }
         int last = opt->elastic || (opt->fixed_outer > 0 ? (k + 1 == opt->fixed_outer)
-                                                         : (relErr <= opt->tol || k + 1 == maxOuter));
+                                                         : ((relErr <= opt->tol && tight) || k + 1 == maxOuter));
+        prev_rel = relErr <= opt->tol ? 0. : relErr;
         int plain = last || !opt->aitken || k == 0 || rr > rr_prev;
         double omega = 1.;
         if (!plain) {

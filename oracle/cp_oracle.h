@@ -130,6 +130,8 @@ typedef struct {
     int    cg_quirk;    /* 1: reference GPU x_{k-1} behaviour */
     int    elastic;     /* 1: solveElast (one assembly, one CG, stresses) */
     int    warm;        /* 1: start from given u / psi state; 0: reset (initPlastParams) */
+    double cg_forcing;  /* > 0: secant iteration k solves to max(cg_tol, cg_forcing rel_err_{k-1}),
+                           rel_err_{-1} = 1; exit only after an iteration solved at cg_tol */
 } or_options;
 
 typedef struct {

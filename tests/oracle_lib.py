@@ -32,7 +32,7 @@ class Material(C.Structure):
 class Options(C.Structure):
     _fields_ = [("cg_tol", C.c_double), ("cg_maxit", C.c_int), ("tol", C.c_double),
                 ("max_outer", C.c_int), ("fixed_outer", C.c_int), ("aitken", C.c_int),
-                ("cg_quirk", C.c_int), ("elastic", C.c_int), ("warm", C.c_int)]
+                ("cg_quirk", C.c_int), ("elastic", C.c_int), ("warm", C.c_int), ("cg_forcing", C.c_double)]
 
 
 class Stats(C.Structure):
@@ -230,9 +230,9 @@ class Problem:
 
 
 def options(cg_tol=1e-7, cg_maxit=1000000, tol=1e-5, max_outer=50, fixed_outer=0, aitken=False,
-            cg_quirk=False, elastic=False, warm=False):
+            cg_quirk=False, elastic=False, warm=False, cg_forcing=0.0):
     return Options(cg_tol, cg_maxit, tol, max_outer, fixed_outer, int(aitken), int(cg_quirk),
-                   int(elastic), int(warm))
+                   int(elastic), int(warm), cg_forcing)
 
 
 def solve(prob, mat, opt, u=None, state=None, rp=None):
@@ -268,17 +268,36 @@ def elem_space4(mesh, e):
     return lib.or_elem_space4(dp(mesh.x), dp(mesh.y), ip(mesh.conn), e)
 
 
-def solve_steps(prob, mat, nsteps, opt, final_factor=1.0):
+def solve_steps(prob, mat, nsteps, opt, final_factor=1.0, predictor=False):
     """Incremental loading: level n applies n/nsteps of the load, warm-started
-    from level n-1's displacements and secant moduli."""
-    u = prob.u * (1.0 / nsteps * final_factor)
+    from level n-1's displacements and secant moduli; with `predictor`, u and
+    log(psi) are extrapolated linearly in the load factor from the last two
+    levels (the level before the first is the unloaded state); predictor 1
+    extrapolates u only, 2 also log(psi)."""
+    u = prob.u * 0.0
     st = None
     per = []
+    lam_prev, u_prev, psi_prev = 0.0, np.zeros(2 * prob.mesh.nn), np.ones(prob.mesh.ne)
+    lam_cur = 0.0
     for n in range(1, nsteps + 1):
         lam = final_factor * n / nsteps
+        if st is not None and predictor:
+            ratio = (lam - lam_cur) / (lam_cur - lam_prev)
+            psi = st[FIELDS.index("psi")].copy()
+            u_new = u + ratio * (u - u_prev)
+            psi_new = psi
+            if predictor >= 2:
+                psi_new = np.maximum(np.exp(np.log(psi) + ratio * (np.log(psi) - np.log(psi_prev))), 1.0)
+            u_prev, psi_prev = u.copy(), psi
+            u = u_new
+            st[FIELDS.index("psi")] = psi_new
+            st[FIELDS.index("Ec")] = mat.E / psi_new
+            st[FIELDS.index("nuc")] = 0.5 * (1 - (1 - 2 * mat.nu) / psi_new)
+            lam_prev = lam_cur
         u = np.where(prob.mask == 0, prob.u * lam, u)
         o = Options.from_buffer_copy(opt)
         o.warm = 1 if st is not None else 0
         u, st, s = solve(prob, mat, o, u=u, state=st, rp=prob.rp * lam)
+        lam_cur = lam
         per.append((s.outer_iters, s.cg_iters, s.rel_err, s.status))
     return u, st, per

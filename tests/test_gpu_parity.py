@@ -189,3 +189,82 @@ def test_deterministic_bitwise(ctx):
         outs.append((P.field("u"), P.field("sigI")))
         P.close()
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_block_jacobi_pcg_parity(ctx):
+    """2x2 block-Jacobi PCG reaches the oracle's (unpreconditioned) answer."""
+    pm, pl, pmat, om, ol, omat = quarter_plate_case(100, 49, 100.0)
+    P = A.Problem(ctx, pm, pl, pmat)
+    st_pre = P.solve(A.options(cg_tol=1e-12, precond=True), elastic=True)
+    u_o, s_o, _ = O.solve(ol, omat, O.options(cg_tol=1e-12, elastic=True))
+    assert relnorm(P.field("u"), u_o) < 1e-8
+    assert relnorm(P.field("sigI"), O.fields(s_o)["sigI"]) < 1e-8
+    st_plain = P.solve(A.options(cg_tol=1e-12), elastic=True)
+    print("pcg iterations", st_pre.cg_iters, "plain", st_plain.cg_iters, flush=True)
+    assert st_pre.cg_iters < 0.7 * st_plain.cg_iters
+    P.close()
+
+
+def test_graded_ring_pcg_lame(ctx):
+    """Graded quarter ring (config-4 geometry, small) under internal
+    pressure: block-Jacobi PCG in both precisions against the oracle."""
+    a, b, pr = 0.5, 1.0, 10.0
+    pm = A.gen_quarter_ring(a, b, 64, 24, 1.05)
+    om = O.gen_quarter_ring(a, b, 64, 24, 1.05)
+    pl = A.Loads(pm).force(0, -pr, 0.0, norm_orient=True).fix_vert_axis(0).fix_hor_axis(0)
+    ol = O.Problem(om).force(0, -pr, 0.0, norm_orient=True).fix_vert_axis(0).fix_hor_axis(0)
+    assert np.array_equal(pl.rp, ol.rp)
+    P = A.Problem(ctx, pm, pl, A.material(2.1e5, 0.3))
+    u_o, _, _ = O.solve(ol, O.material(2.1e5, 0.3), O.options(cg_tol=1e-13, elastic=True))
+    P.solve(A.options(cg_tol=1e-12, precond=True), elastic=True)
+    assert relnorm(P.field("u"), u_o) < 1e-8
+    P.solve(A.options(cg_tol=1e-10, precond=True, precision=A.MIXED, inner_tol=1e-4), elastic=True)
+    assert relnorm(P.field("u"), u_o) < 1e-7
+    P.close()
+
+
+def test_predictor_steps_parity(ctx):
+    """Load steps with the u / log(psi) predictor, block-Jacobi PCG and
+    Aitken: same converged path as the oracle with the same predictor."""
+    pm, pl, pmat, om, ol, omat = quarter_plate_case(40, 20, 300.0, "linear")
+    P = A.Problem(ctx, pm, pl, pmat)
+    opt = A.options(cg_tol=1e-13, tol=1e-11, aitken=True, max_outer=2000, precond=True, predictor=2)
+    per = P.solve_steps(6, 1.0, opt)
+    print("predictor steps:", [(s.outer_iters, s.cg_iters) for s in per], flush=True)
+    u_o, s_o, per_o = O.solve_steps(ol, omat, 6, O.options(cg_tol=1e-13, tol=1e-11, aitken=True, max_outer=2000),
+                                    predictor=2)
+    F_o = O.fields(s_o)
+    assert all(s.status == 0 for s in per)
+    assert relnorm(P.field("u"), u_o) < 1e-8
+    for f in ["sxx", "sigI", "Ec", "epsI_p"]:
+        assert relnorm(P.field(f), F_o[f]) < 1e-8, f
+    P.close()
+    # u-only predictor: same converged path
+    P2 = A.Problem(ctx, pm, pl, pmat)
+    per2 = P2.solve_steps(6, 1.0, A.options(cg_tol=1e-13, tol=1e-11, aitken=True, max_outer=2000, precond=True,
+                                            predictor=1))
+    print("u-predictor steps:", [(s.outer_iters, s.cg_iters) for s in per2], flush=True)
+    assert relnorm(P2.field("u"), u_o) < 1e-8
+    P2.close()
+
+
+def test_inexact_secant_iterations_parity(ctx):
+    """cg_forcing: loose linear solves while the secant residual is large,
+    a tight final one; same converged state as the oracle with the same rule,
+    and within the tolerance of the strict (fixed cg_tol) answer."""
+    pm, pl, pmat, om, ol, omat = quarter_plate_case(40, 20, 300.0, "linear")
+    kw = dict(cg_tol=1e-12, tol=1e-9, aitken=True, max_outer=2000)
+    P = A.Problem(ctx, pm, pl, pmat)
+    per = P.solve_steps(4, 1.0, A.options(precond=True, predictor=1, cg_forcing=0.05, **kw))
+    u_o, s_o, per_o = O.solve_steps(ol, omat, 4, O.options(cg_forcing=0.05, **kw), predictor=1)
+    P2 = A.Problem(ctx, pm, pl, pmat)
+    per2 = P2.solve_steps(4, 1.0, A.options(precond=True, predictor=1, **kw))
+    print("forced:", [(s.outer_iters, s.cg_iters) for s in per], "strict:",
+          [(s.outer_iters, s.cg_iters) for s in per2], flush=True)
+    assert all(s.status == 0 for s in per)
+    assert relnorm(P.field("u"), u_o) < 1e-8
+    assert relnorm(P.field("sigI"), O.fields(s_o)["sigI"]) < 1e-8
+    assert relnorm(P.field("u"), P2.field("u")) < 1e-7
+    assert sum(s.cg_iters for s in per) < sum(s.cg_iters for s in per2)
+    P.close()
+    P2.close()

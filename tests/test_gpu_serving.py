@@ -1,0 +1,92 @@
+"""Serving loop (continuous batching) on the GPU through the C-ABI: every
+result is the same alone and in the mix, admission respects the HBM budget
+and slot count, oversize requests are refused."""
+import numpy as np
+import pytest
+
+from cudaplasticity_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = A.Context(0)
+    yield c
+    c.close()
+
+
+def plate(Nt, Nr, r=0.1, S=300.0, sY=240.0, law=A.LAW_LINEAR):
+    mesh = A.gen_quarter_plate(1.0, r, Nt, Nr)
+    loads = A.Loads(mesh).force(1, S, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    E = 2.1e5
+    mat = A.material(E, 0.3, sY, law, K_T=E / 101, n=0.2)
+    return mesh, loads, mat
+
+
+def ring(Nt, Nr, p=180.0, sY=240.0):
+    mesh = A.gen_quarter_ring(0.5, 1.0, Nt, Nr, 1.05)
+    loads = A.Loads(mesh).force(0, -p, 0.0, norm_orient=True).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    E = 2.1e5
+    mat = A.material(E, 0.3, sY, A.LAW_LINEAR, K_T=E / 101)
+    return mesh, loads, mat
+
+
+CASES = [
+    (plate(40, 20), 4, A.MIXED),
+    (plate(24, 10, r=0.2, sY=300.0, law=A.LAW_SWIFT), 3, A.MIXED),
+    (ring(48, 12), 3, A.MIXED),
+    (plate(60, 30, r=0.15), 4, A.FP64),
+    (plate(30, 12), 2, A.FP64),
+]
+
+
+def opts(prec):
+    return A.options(cg_tol=1e-9, tol=1e-6, aitken=True, max_outer=500, precision=prec, precond=True,
+                     predictor=1, cg_forcing=0.1, inner_tol=1e-4)
+
+
+@pytest.mark.parametrize("batch", [True, False])
+def test_served_results_equal_alone(ctx, batch):
+    alone = []
+    for (mesh, loads, mat), nsteps, prec in CASES:
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.solve_steps(nsteps, 1.0, opts(prec))
+        alone.append((P.field("u"), P.field("sigI")))
+        P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 30, max_slots=3, batch_cg=batch)
+    ids = [srv.submit(m, l, mat, n, 1.0, opts(p)) for (m, l, mat), n, p in CASES]
+    srv.run_until_done()
+    for i, rid in enumerate(ids):
+        code, info, u, sig = srv.result(rid)
+        print("request", rid, "status", info.status, "outer", info.outer_iters, "cg", info.cg_iters,
+              "latency", info.t_done - info.t_submit, flush=True)
+        assert code == A.CP_OK
+        assert np.array_equal(u, alone[i][0]), (i, np.abs(u - alone[i][0]).max())
+        assert np.array_equal(sig, alone[i][1])
+        assert info.t_submit <= info.t_admit <= info.t_done
+    srv.close()
+
+
+def test_budget_and_slots(ctx):
+    (m, l, mat), _, _ = CASES[0]
+    import ctypes as C
+    need = C.c_size_t()
+    A.check(A.lib().cp_problem_bytes(m.nn, m.ne, 4 * 9 * m.nn, A.MIXED, C.byref(need)))
+    srv = A.Server(ctx, hbm_budget=int(need.value * 2.5), max_slots=8)
+    ids = [srv.submit(m, l, mat, 2, 1.0, opts(A.MIXED)) for _ in range(5)]
+    big = A.gen_quarter_plate(1.0, 0.1, 400, 200)
+    bl = A.Loads(big).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    big_id = srv.submit(big, bl, mat, 1, 1.0, opts(A.MIXED))
+    srv.run_until_done()
+    admits = []
+    for rid in ids:
+        code, info, u, sig = srv.result(rid)
+        assert code == A.CP_OK
+        admits.append((info.t_admit, info.t_done))
+    # at most two fit at once: the third is admitted only after one finished
+    admits.sort()
+    assert admits[2][0] >= min(admits[0][1], admits[1][1])
+    code, info, _, _ = srv.result(big_id)
+    assert code == A.CP_ERR_BUDGET and info.status == A.CP_ERR_BUDGET
+    srv.close()
