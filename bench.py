@@ -4,17 +4,21 @@ Workload (config 2): quarter plate [0,1]^2 with a hole of radius 0.1,
 N_theta x N_r = 632 x 316 bilinear quads (199 712 elements, 200 661 nodes,
 401 322 DOF), plane stress, E = 2.1e5 MPa, nu = 0.3, von Mises secant
 (deformation theory) update with linear hardening sigma_Y = 240 MPa,
-K_T = E/101 (i.e. H = E/100), 20 equal load increments of the far-edge
-traction up to 300 MPa, Aitken-accelerated secant iterations to a 1e-6
-stress tolerance, CG to ||r||/||b|| = cg_tol.
+K_T = E/101 (H = E/100), 20 equal load increments of the far-edge traction
+up to 300 MPa, Aitken-accelerated secant iterations to a 1e-6 stress
+tolerance, linear solves to ||r||/||b|| = 1e-9.
 
-A "step" is one load increment: warm-started from the previous increment,
-assemble -> CG -> return map until converged.  The timed region runs the
-first K increments of the 20-increment ramp on a fresh device problem with
-all inputs already in HBM; W warm-up increments run first on a separate
-instance.  Both the plain FP64 path and the mixed path (FP32 CG + FP64
-refinement, FP64 return map) are timed; `value` is the mixed path and the
-line carries the FP64 rate and the error between the two.
+A "step" is one load increment: warm-started from the previous one,
+assemble -> CG -> return map until the secant iteration converges.  The
+timed region runs the first K increments of the 20-increment ramp on a
+fresh device problem, all inputs already in HBM; W warm-up increments run
+first on a separate instance.  `value` is the mixed-precision path (FP32 CG
+with FP64 iterative refinement, FP64 return map and Aitken).  The same line
+reports the FP64 path with the same solver settings, the reference
+algorithm in FP64 (unpreconditioned CG to the full tolerance at every secant
+iteration, no predictor) over the first increments, the roofline of the
+dominant kernel, a served stream of independent problems (p50/p99 latency)
+and the CPU oracle.
 """
 import argparse
 import json
@@ -29,6 +33,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "load-steps/s + CG GB/s at 1e-6 rel err; p50/p99 latency (mixed+served vs FP64)"
 HBM_PEAK_GBS = 8000.0
+NSTEPS = 20
+# Fabric bytes per launch of the roofline kernel, measured with rocprofv3
+# --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --roofline-only`
+# (632x316 quarter plate), each scaled by what the counter reads for a plain
+# 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000):
+# 48.02 MB read + 8.05 MB written.
+MEASURED_TRAFFIC = {(632, 316): 56.07e6}
 
 
 def log(*a):
@@ -43,121 +54,266 @@ def build_case(A, Nt, Nr, S=300.0):
     return mesh, loads, mat
 
 
-def run_ramp(A, ctx, mesh, loads, mat, nsteps_total, ksteps, opt):
-    """First `ksteps` increments of an `nsteps_total` ramp; returns (seconds, stats list, problem)."""
+def solver_options(A, args, precision, reference=False):
+    if reference:
+        # the reference's algorithm: unpreconditioned CG to cg_tol at every
+        # secant iteration, previous level's state as the initial guess
+        return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision)
+    return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision, precond=True,
+                     predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
+
+
+def run_ramp(A, ctx, mesh, loads, mat, ksteps, opt, tag):
+    """First `ksteps` increments of the 20-increment ramp; (seconds, stats, problem)."""
+    t0 = time.perf_counter()
     P = A.Problem(ctx, mesh, loads, mat)
+    P.setup_seconds = time.perf_counter() - t0
     per = []
     t0 = time.perf_counter()
     for n in range(1, ksteps + 1):
         o = A.Options.from_buffer_copy(opt)
         o.warm = 1 if n > 1 else 0
-        P.set_load(n / nsteps_total)
+        if n == 1:
+            P.set_load(n / NSTEPS)
+        else:
+            P.advance_load(n / NSTEPS, int(opt.predictor))
         st = P.solve(o)
         per.append(st)
-        log(f"  increment {n}/{nsteps_total}: outer {st.outer_iters} cg {st.cg_iters} refinements {st.refinements} "
-            f"rel_err {st.rel_err:.2e} {st.seconds:.3f} s (cg {st.cg_seconds:.3f} asm {st.asm_seconds:.3f} "
-            f"rm {st.rm_seconds:.3f})")
-    dt = time.perf_counter() - t0
-    return dt, per, P
+        log(f"  {tag} increment {n}/{NSTEPS}: secant {st.outer_iters} cg {st.cg_iters} refinements "
+            f"{st.refinements} rel_err {st.rel_err:.2e} {st.seconds:.3f} s")
+    return time.perf_counter() - t0, per, P
 
 
-def cpu_baseline(mesh_args, gpu_per_step, cores):
-    """Oracle (CPU FP64 restatement of the reference solver) timed on the
-    host: per-outer-iteration cost (pattern, B, assembly, return map) and
-    per-CG-iteration cost on the same mesh, from two bounded runs of the
-    first increment with 60 and 160 CG iterations; the load-steps/s figure
-    applies them to the iteration counts of the FP64 GPU run."""
+def cg_iteration_bytes(nnz, N, s):
+    """Algorithmic bytes of one CG iteration over a CSR matrix with nnz scalar
+    entries and N unknowns, s-byte values: SpMV nnz(s+4) + 4(N+1) + 2Ns (values,
+    columns, row pointers, p read once, Ap written once) plus the vector
+    updates 9Ns and the one-byte Dirichlet mask."""
+    return nnz * (s + 4) + 4 * (N + 1) + 2 * N * s + 9 * N * s + N
+
+
+def scalar_nnz(A, P):
+    import ctypes as C
+    n = C.c_longlong()
+    A.check(A.lib().cp_problem_csr_nnz(P.ptr, C.byref(n)))
+    return n.value
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def bench_cg(A, P, precision, reps, precond=True):
+    import ctypes as C
+    ms, b = C.c_double(), C.c_double()
+    A.check(A.lib().cp_bench_cg(P.ptr, precision, int(precond), reps, C.byref(ms), C.byref(b)))
+    return ms.value, b.value
+
+
+def cpu_baseline(nt, nr, per_step, cores, what="the reference algorithm (FP64, unpreconditioned CG)"):
+    """The oracle (CPU FP64 restatement of the reference solver, OpenMP) on
+    the same mesh.  Two bounded runs of the first increment with 200 and 30200
+    CG iterations give the cost of one CG iteration and of the rest of one
+    secant iteration (pattern, B, assembly, return map); these are applied
+    to the secant and CG iteration counts `per_step` of the GPU run of the
+    reference algorithm (FP64, unpreconditioned CG) on the same increments."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
-    Nt, Nr = mesh_args
-    om = O.gen_quarter_plate(1.0, 0.1, Nt, Nr)
-    ol = O.Problem(om).force(1, 300.0 / 20, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    om = O.gen_quarter_plate(1.0, 0.1, nt, nr)
+    ol = O.Problem(om).force(1, 300.0 / NSTEPS, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
     E = 2.1e5
     mat = O.material(E, 0.3, 240.0, O.LAW_LINEAR, K_T=E / 101)
-    t = {}
-    for it in (60, 160):
+    t, n = {}, {}
+    for it in (200, 30200):
         t0 = time.perf_counter()
-        O.solve(ol, mat, O.options(cg_tol=1e-30, cg_maxit=it, fixed_outer=1))
+        _, _, st = O.solve(ol, mat, O.options(cg_tol=1e-30, cg_maxit=it, fixed_outer=1))
         t[it] = time.perf_counter() - t0
-    per_it = (t[160] - t[60]) / 100.0
-    per_outer = t[60] - 60 * per_it
-    outer = sum(s.outer_iters for s in gpu_per_step)
-    cg = sum(s.cg_iters for s in gpu_per_step)
+        n[it] = st.cg_iters
+        log(f"cpu baseline: first increment with {n[it]} CG iterations: {t[it]:.2f} s")
+    per_it = (t[30200] - t[200]) / (n[30200] - n[200])
+    per_outer = max(0.0, t[200] - n[200] * per_it)
+    outer = sum(s.outer_iters for s in per_step)
+    cg = sum(s.cg_iters for s in per_step)
     secs = outer * per_outer + cg * per_it
-    return {"value": len(gpu_per_step) / secs, "unit": "load-steps/s", "cores": cores, "kind": "port",
-            "sample": (f"oracle/cp_oracle.c on the same {Nt}x{Nr} mesh: first increment with 60 and 160 CG "
-                       f"iterations timed ({t[60] + t[160]:.1f} s total); {per_it * 1e3:.2f} ms per CG iteration, "
-                       f"{per_outer:.2f} s per secant iteration; applied to the FP64 GPU run's {outer} secant "
-                       f"and {cg} CG iterations")}
+    return {"value": len(per_step) / secs, "unit": "load-steps/s", "cores": cores, "kind": "port",
+            "sample": (f"oracle/cp_oracle.c (OpenMP, {cores} threads) on the same {nt}x{nr} mesh: the first "
+                       f"increment run with 200 and 30200 CG iterations ({t[200] + t[30200]:.1f} s) gives "
+                       f"{per_it * 1e3:.3f} ms per CG iteration and {per_outer:.3f} s per secant iteration "
+                       f"(pattern, B, assembly, return map); applied to the {outer} secant and {cg} CG "
+                       f"iterations of the FP64 GPU run of {what} on increments "
+                       f"1..{len(per_step)}: {secs:.1f} s projected")}
+
+
+def served_stream(A, ctx, args):
+    """Reduced config-5 stream: served with Poisson arrivals at `serve_load`
+    times the rate the same problems are solved back to back one at a time,
+    and every served result compared with the same problem solved alone."""
+    from cudaplasticity_amd import serving
+    stream = serving.make_stream(args.serve_n, qmin=2e3, qmax=args.serve_qmax)
+    opt = solver_options(A, args, A.MIXED)
+    t0 = time.perf_counter()
+    alone = []
+    for mesh, loads, mat, _ in stream:
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.solve_steps(args.serve_steps, 1.0, opt)
+        alone.append((P.field("u"), P.field("sigI")))
+        P.close()
+    t_seq = time.perf_counter() - t0
+    rate = args.serve_load * len(stream) / t_seq
+    log(f"served stream: {len(stream)} problems solved back to back in {t_seq:.2f} s; serving at {rate:.2f}/s")
+    budget = int(args.serve_budget_gb * (1 << 30))
+    recs = serving.serve(ctx, stream, args.serve_steps, opt, rate, budget, max_slots=args.serve_slots, log=log)
+    ok = all(r["code"] == A.CP_OK for r in recs)
+    same = ok and all(np.array_equal(r["u"], a[0]) and np.array_equal(r["sig"], a[1]) for r, a in zip(recs, alone))
+    lat = np.array([r["latency"] for r in recs])
+    span = max(r["t_done"] for r in recs)
+    return {"problems": len(stream), "load_steps_each": args.serve_steps,
+            "quads_range": [int(min(s[3]["quads"] for s in stream)), int(max(s[3]["quads"] for s in stream))],
+            "arrival_rate_per_s": rate, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "throughput_problems_per_s": len(stream) / span,
+            "throughput_load_steps_per_s": len(stream) * args.serve_steps / span,
+            "sequential_alone_load_steps_per_s": len(stream) * args.serve_steps / t_seq,
+            "all_ok": ok, "results_equal_alone_bitwise": bool(same)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=NSTEPS)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nt", type=int, default=632)
     ap.add_argument("--nr", type=int, default=316)
     ap.add_argument("--cg-tol", type=float, default=1e-9)
-    ap.add_argument("--inner-tol", type=float, default=1e-3)
+    ap.add_argument("--forcing", type=float, default=0.1)
+    ap.add_argument("--inner-tol", type=float, default=1e-5)
+    ap.add_argument("--ref-steps", type=int, default=8)
+    ap.add_argument("--large-nt", type=int, default=2000)
+    ap.add_argument("--large-nr", type=int, default=1000)
+    ap.add_argument("--serve-n", type=int, default=24)
+    ap.add_argument("--serve-qmax", type=float, default=5e4)
+    ap.add_argument("--serve-steps", type=int, default=5)
+    ap.add_argument("--serve-slots", type=int, default=16)
+    ap.add_argument("--serve-load", type=float, default=1.0)
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--reps", type=int, default=400, help="launches of the CG iteration kernel for the roofline")
+    ap.add_argument("--copy-calibration", type=float, default=0.0,
+                    help="MiB: only run a plain device copy of this size 10 times (HBM counter calibration)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only time the CG iteration kernel on the workload's matrix (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-ref", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build runs each problem on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print("bench.py measures one GPU: each problem runs on a single MI355X here; how a problem or a "
+              "stream would shard over several GPUs is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
-    nsteps_total = 20
-    K = max(1, min(args.steps, nsteps_total))
+    K = max(1, min(args.steps, NSTEPS))
+    if args.copy_calibration > 0:
+        import torch
+        n = int(args.copy_calibration * (1 << 20)) // 4
+        src = torch.rand(n, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}), flush=True)
+        return
     from cudaplasticity_amd import abi as A
 
     mesh, loads, mat = build_case(A, args.nt, args.nr)
     log(f"mesh {args.nt}x{args.nr}: {mesh.nn} nodes, {mesh.ne} quads")
     ctx = A.Context(0)
-    info = ctx.info()
-    log("device", info)
-    common = dict(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500)
-    opt64 = A.options(precision=A.FP64, **common)
-    optmx = A.options(precision=A.MIXED, inner_tol=args.inner_tol, **common)
+    log("device", ctx.info())
+    if args.roofline_only:
+        # the dominant kernel alone, for profiling: block-Jacobi FP32 CG
+        # iterations on the workload's matrix (initial secant moduli, full load)
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_load(1.0)
+        ms, fmt = bench_cg(A, P, A.MIXED, args.reps)
+        byt = cg_iteration_bytes(scalar_nnz(A, P), 2 * mesh.nn, 4)
+        print(json.dumps({"kernel": "cpk::k_cg_iter<9,true,float>", "launches": args.reps, "avg_launch_ms": ms,
+                          "bytes_per_launch": byt, "achieved_gbps": byt / ms / 1e6,
+                          "format_bytes_per_launch": fmt, "achieved_format_gbps": fmt / ms / 1e6}), flush=True)
+        P.close()
+        return
+    opt_mx = solver_options(A, args, A.MIXED)
+    opt_64 = solver_options(A, args, A.FP64)
 
     if args.warmup > 0:
-        t, per, P = run_ramp(A, ctx, mesh, loads, mat, nsteps_total, min(args.warmup, nsteps_total), optmx)
+        t, _, P = run_ramp(A, ctx, mesh, loads, mat, min(args.warmup, NSTEPS), opt_mx, "warmup")
         P.close()
-        log(f"warmup {args.warmup} increments: {t:.2f} s")
 
-    t_mx, per_mx, Pmx = run_ramp(A, ctx, mesh, loads, mat, nsteps_total, K, optmx)
-    log(f"mixed: {K} increments in {t_mx:.3f} s; per step (outer, cg, refinements):",
-        [(s.outer_iters, s.cg_iters, s.refinements) for s in per_mx])
-    res = {"mixed_seconds": t_mx}
-    if not args.no_fp64:
-        t_64, per_64, P64 = run_ramp(A, ctx, mesh, loads, mat, nsteps_total, K, opt64)
-        log(f"fp64: {K} increments in {t_64:.3f} s; per step (outer, cg):",
-            [(s.outer_iters, s.cg_iters) for s in per_64])
-        u64, umx = P64.field("u"), Pmx.field("u")
-        s64, smx = P64.field("sigI"), Pmx.field("sigI")
-        err_u = float(np.linalg.norm(umx - u64) / np.linalg.norm(u64))
-        err_s = float(np.linalg.norm(smx - s64) / np.linalg.norm(s64))
-        epsT = mat.sigmaT / mat.E
-        flags_equal = bool(np.array_equal(P64.field("epsI") >= epsT, Pmx.field("epsI") >= epsT))
-    else:
-        t_64, per_64, P64, err_u, err_s, flags_equal = None, None, None, None, None, None
+    t_mx, per_mx, Pmx = run_ramp(A, ctx, mesh, loads, mat, K, opt_mx, "mixed")
+    log(f"mixed: {K} increments in {t_mx:.3f} s")
+    t_64, per_64, P64 = run_ramp(A, ctx, mesh, loads, mat, K, opt_64, "fp64")
+    log(f"fp64: {K} increments in {t_64:.3f} s")
+    epsT = mat.sigmaT / mat.E
+    err = {"u": rel(Pmx.field("u"), P64.field("u")), "sigma_i": rel(Pmx.field("sigI"), P64.field("sigI")),
+           "plastic_flags_equal": bool(np.array_equal(Pmx.field("epsI") >= epsT, P64.field("epsI") >= epsT))}
 
-    # roofline of the dominant kernel: the fused CG iteration (FP32 in the mixed path)
-    reps = 200
-    ms32, bytes32 = (lambda r: (r[0], r[1]))(bench_cg(A, Pmx, A.MIXED, reps))
-    ms64, bytes64 = bench_cg(A, P64 or Pmx, A.FP64, reps)
+    ref = None
+    per_ref_steps = None
+    if not args.no_ref:
+        R = max(1, min(args.ref_steps, K))
+        t_ref, per_ref, Pref = run_ramp(A, ctx, mesh, loads, mat, R, solver_options(A, args, A.FP64, True), "fp64-ref")
+        Pchk = A.Problem(ctx, mesh, loads, mat)
+        Pchk.solve_steps(R, R / NSTEPS, opt_mx)
+        t_mx_R = sum(s.seconds for s in per_mx[:R])
+        per_ref_steps = per_ref
+        ref = {"increments": R, "seconds": t_ref, "load_steps_per_s": R / t_ref,
+               "mixed_same_increments_seconds": t_mx_R, "speedup_mixed_vs_reference_fp64": t_ref / t_mx_R,
+               "rel_err_mixed_vs_reference": {"u": rel(Pchk.field("u"), Pref.field("u")),
+                                              "sigma_i": rel(Pchk.field("sigI"), Pref.field("sigI"))},
+               "iterations": [[s.outer_iters, s.cg_iters] for s in per_ref]}
+        Pref.close()
+        Pchk.close()
+
+    # dominant kernel: the fused CG iteration of the mixed path (FP32 block-Jacobi PCG)
+    reps = args.reps
+    ms32, fmt32 = bench_cg(A, Pmx, A.MIXED, reps)
+    ms64, fmt64 = bench_cg(A, P64, A.FP64, reps)
+    sp_ms, sp_stored, sp_algo = P64.bench_spmv(A.FP64, reps)
+    nnz = scalar_nnz(A, Pmx)
+    bytes32, bytes64 = cg_iteration_bytes(nnz, 2 * mesh.nn, 4), cg_iteration_bytes(nnz, 2 * mesh.nn, 8)
     ach32 = bytes32 / (ms32 * 1e-3) / 1e9
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
-    cg_it_mx = sum(s.cg_iters for s in per_mx)
-    cg_s_mx = sum(s.cg_seconds for s in per_mx)
-    lay = Pmx.layout()
-    log(f"CG iteration kernel: FP32 {ms32 * 1e3:.1f} us, {bytes32 / 1e6:.1f} MB -> {ach32:.0f} GB/s; "
-        f"FP64 {ms64 * 1e3:.1f} us, {bytes64 / 1e6:.1f} MB -> {ach64:.0f} GB/s; layout {lay}")
+    cg_it = sum(s.cg_iters for s in per_mx)
+    cg_s = sum(s.cg_seconds for s in per_mx)
+    log(f"CG iteration kernel: FP32 {ms32 * 1e3:.1f} us {bytes32 / 1e6:.1f} MB ({fmt32 / 1e6:.1f} MB as stored) "
+        f"-> {ach32:.0f} GB/s; "
+        f"FP64 {ms64 * 1e3:.1f} us {bytes64 / 1e6:.1f} MB -> {ach64:.0f} GB/s; SpMV FP64 {sp_ms * 1e3:.1f} us "
+        f"-> {sp_algo / sp_ms / 1e6:.0f} GB/s; layout {Pmx.layout()}")
 
-    value = K / t_mx
+    large = None
+    if args.large_nt > 0:
+        # the same kernels on the 2000x1000 mesh, whose matrix and vectors
+        # (about 0.8 GB in FP32) do not fit the 256 MB last-level cache
+        lm, ll, lmat = build_case(A, args.large_nt, args.large_nr)
+        PL = A.Problem(ctx, lm, ll, lmat)
+        PL.set_load(1.0)
+        lnnz = scalar_nnz(A, PL)
+        l32, lf32 = bench_cg(A, PL, A.MIXED, max(20, reps // 4))
+        l64, lf64 = bench_cg(A, PL, A.FP64, max(20, reps // 4))
+        lsp_ms, _, lsp_algo = PL.bench_spmv(A.FP64, max(20, reps // 4))
+        b32, b64 = cg_iteration_bytes(lnnz, 2 * lm.nn, 4), cg_iteration_bytes(lnnz, 2 * lm.nn, 8)
+        large = {"mesh": f"{args.large_nt}x{args.large_nr}", "elements": lm.ne, "nodes": lm.nn,
+                 "mixed_iteration_kernel": {"ms": l32, "bytes": b32, "gbps": b32 / l32 / 1e6,
+                                            "format_gbps": lf32 / l32 / 1e6},
+                 "fp64_iteration_kernel": {"ms": l64, "bytes": b64, "gbps": b64 / l64 / 1e6,
+                                           "format_gbps": lf64 / l64 / 1e6},
+                 "fp64_spmv": {"ms": lsp_ms, "bytes": lsp_algo, "gbps": lsp_algo / lsp_ms / 1e6}}
+        log("large mesh kernels:", large)
+        PL.close()
+
+    serve = None
+    if not args.no_serve and args.serve_n > 0:
+        serve = served_stream(A, ctx, args)
+        log("served:", serve)
+
     line = {
         "metric": METRIC,
-        "value": value,
+        "value": K / t_mx,
         "unit": "load-steps/s",
         "n_gpus": 1,
         "steps": K,
@@ -168,35 +324,46 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": f"plate-with-hole quarter model {args.nt}x{args.nr} quads ({mesh.ne}), linear "
-                               f"hardening, {K} of {nsteps_total} increments to 300 MPa, Aitken, tol 1e-6, "
-                               f"cg_tol {args.cg_tol:g}",
-                   "nodes": mesh.nn, "elements": mesh.ne, "dof": 2 * mesh.nn, "path": "mixed (FP32 CG + FP64 refinement)"},
-        "fp64_load_steps_per_s": (K / t_64) if t_64 else None,
-        "speedup_vs_fp64": (t_64 / t_mx) if t_64 else None,
-        "rel_err_vs_fp64": {"u": err_u, "sigma_i": err_s, "plastic_flags_equal": flags_equal},
+        "config": {"workload": (f"plate with hole, quarter model {args.nt}x{args.nr} quads ({mesh.ne}), linear "
+                                f"hardening sigma_Y 240 MPa H E/100, increments 1..{K} of {NSTEPS} to 300 MPa, "
+                                f"Aitken to 1e-6, cg_tol {args.cg_tol:g}"),
+                   "nodes": mesh.nn, "elements": mesh.ne, "dof": 2 * mesh.nn,
+                   "path": "mixed: FP32 block-Jacobi CG + FP64 refinement, FP64 return map, load predictor, "
+                           f"inexact secant iterations (forcing {args.forcing:g})"},
+        "problem_setup_seconds": Pmx.setup_seconds,
+        "load_steps_per_s_incl_setup_and_upload": K / (t_mx + Pmx.setup_seconds),
+        "fp64_load_steps_per_s": K / t_64,
+        "speedup_vs_fp64": t_64 / t_mx,
+        "rel_err_vs_fp64": err,
+        "fp64_reference_algorithm": ref,
         "cg_gbps": {"mixed_iteration_kernel": ach32, "fp64_iteration_kernel": ach64,
-                    "mixed_in_solve": (cg_it_mx * bytes32 / cg_s_mx / 1e9) if cg_s_mx > 0 else None},
+                    "fp64_spmv": sp_algo / (sp_ms * 1e-3) / 1e9,
+                    "mixed_in_solve_incl_scalar_step": (cg_it * bytes32 / cg_s / 1e9) if cg_s > 0 else None},
         "iterations": {"mixed": [[s.outer_iters, s.cg_iters, s.refinements] for s in per_mx],
-                       "fp64": [[s.outer_iters, s.cg_iters] for s in per_64] if per_64 else None},
+                       "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
         "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cg_iter<float> (fused Chronopoulos-Gear CG iteration)",
-                     "bytes_per_launch": bytes32, "avg_launch_ms": ms32},
+                     "frac": ach32 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr)),
+                     "kernel": "cpk::k_cg_iter<9,true,float> (fused block-Jacobi Chronopoulos-Gear iteration)",
+                     "bytes_per_launch": bytes32, "avg_launch_ms": ms32, "reps": reps,
+                     "format_bytes_per_launch": fmt32, "achieved_format": fmt32 / (ms32 * 1e-3) / 1e9},
+        "kernels_2000x1000": large,
+        "served": serve,
         "cpu_baseline": None,
     }
-    if not args.no_cpu and per_64:
+    if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
         os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-        line["cpu_baseline"] = cpu_baseline((args.nt, args.nr), per_64, int(os.environ["OMP_NUM_THREADS"]))
+        threads = int(os.environ["OMP_NUM_THREADS"])
+        if per_ref_steps:
+            line["cpu_baseline"] = cpu_baseline(args.nt, args.nr, per_ref_steps, threads)
+        else:
+            line["cpu_baseline"] = cpu_baseline(args.nt, args.nr, per_64, threads, "the FP64 path")
+        line["cpu_baseline"]["gpu_same_increments"] = {
+            "fp64_reference_algorithm_load_steps_per_s": ref["load_steps_per_s"] if ref else None,
+            "mixed_load_steps_per_s": (ref["increments"] / ref["mixed_same_increments_seconds"]) if ref else None}
+    Pmx.close()
+    P64.close()
     print(json.dumps(line), flush=True)
-
-
-def bench_cg(A, P, precision, reps):
-    import ctypes as C
-    ms, b = C.c_double(), C.c_double()
-    A.check(A.lib().cp_bench_cg(P.ptr, precision, reps, C.byref(ms), C.byref(b)))
-    return ms.value, b.value
 
 
 if __name__ == "__main__":

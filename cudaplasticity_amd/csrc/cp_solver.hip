@@ -415,6 +415,21 @@ void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim
 template void launch_cg_iters<double>(int, int, const cpk::CgArgs<double>&, int, dim3, hipStream_t);
 template void launch_cg_iters<float>(int, int, const cpk::CgArgs<float>&, int, dim3, hipStream_t);
 
+// Iteration kernels only (no scalar step): times the kernel's own traffic.
+template <typename T>
+static void launch_iter_kernels_only(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st)
+{
+    for (int j = 0; j < count; ++j) {
+        if (max_w <= 9) {
+            if (pre) hipLaunchKernelGGL((cpk::k_cg_iter<9, true, T>), g, dim3(cpk::kBlock), 0, st, a);
+            else hipLaunchKernelGGL((cpk::k_cg_iter<9, false, T>), g, dim3(cpk::kBlock), 0, st, a);
+        } else {
+            if (pre) hipLaunchKernelGGL((cpk::k_cg_iter<0, true, T>), g, dim3(cpk::kBlock), 0, st, a);
+            else hipLaunchKernelGGL((cpk::k_cg_iter<0, false, T>), g, dim3(cpk::kBlock), 0, st, a);
+        }
+    }
+}
+
 template <int W, bool PRE, typename T>
 static void launch_init2_w(const cpk::CgArgs<T>& a, dim3 g, hipStream_t st)
 {
@@ -1130,9 +1145,10 @@ int cp_bench_spmv(cp_problem* P, int precision, int reps, double* ms, double* by
     return CP_OK;
 }
 
-int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* bytes_algo)
+int cp_bench_cg(cp_problem* P, int precision, int precond, int reps, double* ms, double* bytes_algo)
 {
-    if (!P || reps < 1) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED)) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    P->precond = precond ? 1 : 0;
     if (int s = cp::launch_assemble(P, precision)) return s;
     hipStream_t st = P->stream;
     const dim3 g(P->nbn), b(cpk::kBlock);
@@ -1144,7 +1160,7 @@ int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* byte
         hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
         cp::launch_init2<double>(P->max_w, P->precond, a, g, st);
         CP_HIP(hipEventRecord(P->ev[4], st));
-        cp::launch_cg_iters<double>(P->max_w, P->precond, a, reps, g, st);
+        cp::launch_iter_kernels_only<double>(P->max_w, P->precond, a, reps, g, st);
     } else {
         auto a = P->cg32_args();
         // right-hand side of the inner solve: the FP64 residual of the current u
@@ -1168,7 +1184,7 @@ int cp_bench_cg(cp_problem* P, int precision, int reps, double* ms, double* byte
         hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
         cp::launch_init2<float>(P->max_w, P->precond, a, g, st);
         CP_HIP(hipEventRecord(P->ev[4], st));
-        cp::launch_cg_iters<float>(P->max_w, P->precond, a, reps, g, st);
+        cp::launch_iter_kernels_only<float>(P->max_w, P->precond, a, reps, g, st);
     }
     CP_HIP(hipGetLastError());
     CP_HIP(hipEventRecord(P->ev[5], st));

@@ -238,10 +238,12 @@ int cp_last_cg_kernel_ms(cp_problem* prob, double* ms_per_iter);
  * bytes per launch. */
 int cp_bench_spmv(cp_problem* prob, int precision, int reps, double* ms, double* bytes_stored,
                   double* bytes_algo);
-/* Times `reps` fused CG iteration launches (precision) on the problem's
- * current matrix (assembled here) with the stop test disabled; returns the
- * average kernel ms and the algorithmic bytes of one iteration. */
-int cp_bench_cg(cp_problem* prob, int precision, int reps, double* ms, double* bytes_algo);
+/* Times `reps` launches of the fused CG iteration kernel alone (precision;
+ * precond 1: block-Jacobi variant) on the matrix of the current secant
+ * moduli (assembled here), HIP events on the problem's stream; returns the
+ * average kernel ms and the bytes one launch moves in this library's
+ * storage format. */
+int cp_bench_cg(cp_problem* prob, int precision, int precond, int reps, double* ms, double* bytes_format);
 /* Matrix storage info: nodes, stored blocks (with padding), true blocks. */
 int cp_problem_layout(cp_problem* prob, long long* nblk_stored, long long* nblk, int* max_width);
 

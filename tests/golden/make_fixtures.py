@@ -6,6 +6,8 @@ the GPU machine):  python tests/golden/make_fixtures.py [REFERENCE_ROOT]
 Inputs (reference data/, ASCII outputs of the reference's own experiments):
   solverTest.vtk         Main.cpp:14-38   elastic 4x4 rectangle, CPU path
   holePlatePlastPow.vtk  Main.cpp:252-296 48x32 hole plate, power law, S=170
+  ring.vtk               Main.cpp:75-80   genRing(0.1, 0.2, 40, 5) mesh with
+                         per-element space / aspect ratio / skew angle
   diagramLin.txt / diagramPow.txt  Main.cpp:929-994  uniaxial sigma(eps)
 Outputs: tests/golden/*.npz (data only: node coordinates, connectivity,
 per-element fields and nodal displacements exactly as printed).
@@ -51,7 +53,7 @@ def read_vtk(path):
 
 def main(ref_root):
     data = os.path.join(ref_root, "data")
-    for name in ("solverTest", "holePlatePlastPow"):
+    for name in ("solverTest", "holePlatePlastPow", "ring"):
         v = read_vtk(os.path.join(data, name + ".vtk"))
         arrays = {"points": v["points"], "polys": v["polys"]}
         for k, a in v["fields"].items():

@@ -44,6 +44,46 @@ def test_gen_rect_with_hole_matches_fixture_mesh():
     assert m.conn[0].tolist() == [67, 1, 0, 66]
 
 
+def _cyclic(quad):
+    """A quad as a cycle of nodes, independent of start node and direction."""
+    q = list(quad)
+    return min(tuple(s[k:] + s[:k]) for s in (q, q[::-1]) for k in range(4))
+
+
+def test_gen_ring_matches_ring_fixture():
+    """genRing + renumerateRing (Mesh.cu:249-384) against data/ring.vtk, the
+    mesh of genRing(0.1, 0.2, 40, 5) written by Main.cpp:80.  That file
+    predates two details of the current generator: its radial index ran from
+    the inner radius out (Mesh.cu:280 now has r = b - h_r i) and its elements
+    were stored ring by ring (the commented k = i N_phi + j at :299).  With
+    r mapped to a + b - r, every node sits where the fixture puts it (same
+    numbering, so renumerateRing is pinned) and the set of quads, as node
+    cycles, is the same; element order and radial direction follow the
+    current source."""
+    fx = load("ring")
+    a, b = 0.1, 0.2
+    m = O.gen_ring(a, b, 40, 5)
+    assert (m.nn, m.ne) == (240, 200)
+    r = np.hypot(m.x, m.y)
+    mirrored = np.c_[m.x, m.y] * ((a + b - r) / r)[:, None]
+    assert np.abs(mirrored - fx["points"]).max() < 1e-6
+    assert sorted(map(_cyclic, m.conn.tolist())) == sorted(map(_cyclic, fx["polys"].tolist()))
+    # current source: node 0 on the outer radius at phi = 0, element 0 in
+    # the outermost ring, first radial column
+    assert np.isclose(r[0], b) and m.y[0] == 0.0
+    assert sorted(r[m.conn[0]].round(12).tolist()) == [0.18, 0.18, 0.2, 0.2]
+
+
+def test_elem_space_matches_ring_fixture():
+    """elemSpace4 (Mesh.cuh:258-267) on the fixture's own points and quads
+    reproduces its Element_space column (points and areas printed to 6
+    digits, so the areas agree to about 1e-4)."""
+    fx = load("ring")
+    fm = O.Mesh(fx["points"][:, 0], fx["points"][:, 1], fx["polys"], [])
+    sp = np.array([O.elem_space4(fm, e) for e in range(fm.ne)])
+    assert np.abs(sp / fx["f_Element_space"] - 1).max() < 1e-4
+
+
 def test_csr_pattern_fixture_mesh():
     """CudaSparseSLAE.cuh:303-458 on the fixture mesh: nnz and the row-0
     column order (node 0 adjacency [67, 1, 66, 34, 33] in first-appearance order)."""
@@ -346,3 +386,36 @@ def test_quarter_plate_generator():
     for e in range(0, m.ne, 101):
         B, dJ = O.calc_b4(m, e)
         assert np.all(dJ > 0)
+
+
+def test_deform_energy_closed_form():
+    """deformEnergy (PlasticitySolver.cuh:415-445): elastic elements add
+    eps_i sigma_i V / 2; plastic ones sigma_T eps_T V / 2 plus the 100-panel
+    trapezoid rule of f from eps_T to eps_i, which is exact for the linear
+    law and within the trapezoid error bound of the integral for the power
+    law."""
+    from scipy.integrate import quad
+    m = O.gen_rect(0, 2, 0, 1, 4, 2)
+    areas = np.array([O.elem_space4(m, e) for e in range(m.ne)])
+    assert np.allclose(areas, 0.25)
+    E, sT = 2.0e5, 240.0
+    eT = sT / E
+    rng = np.random.default_rng(5)
+    epsI = eT * rng.uniform(0.2, 4.0, m.ne)
+    for mat in (O.material(E, 0.3, sT, O.LAW_LINEAR, K_T=E / 101, h=0.5),
+                O.material(E, 0.3, sT, O.LAW_POWER, A=1500.0, m=0.4, h=0.5)):
+        f = np.vectorize(lambda e: O.material_f(mat, e))
+        sigI = f(epsI)
+        st = np.zeros((len(O.FIELDS), m.ne))
+        st[O.FIELDS.index("epsI")] = epsI
+        st[O.FIELDS.index("sigI")] = sigI
+        want = 0.0
+        for e in range(m.ne):
+            V = 0.5 * areas[e]
+            if sigI[e] > sT:
+                want += (0.5 * sT * eT + quad(lambda t: O.material_f(mat, t), eT, epsI[e], epsabs=0, epsrel=1e-13)[0]) * V
+            else:
+                want += 0.5 * epsI[e] * sigI[e] * V
+        got = O.deform_energy(m, mat, st)
+        tol = 1e-13 if mat.law == O.LAW_LINEAR else 1e-5
+        assert abs(got - want) <= tol * want, (mat.law, got, want)
