@@ -1,0 +1,272 @@
+// Mesh.cpp -- the drop-in Mesh class over the C-ABI generators.
+#include "cudaplasticity/Mesh.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cp_abi.h"
+
+namespace {
+
+void check(int status, const char* what)
+{
+    if (status != CP_OK) throw std::runtime_error(std::string(what) + ": " + cp_last_error());
+}
+
+int as_int(size_t n, const char* what)
+{
+    if (n > 0x3fffffff) throw std::invalid_argument(std::string(what) + ": size out of range");
+    return (int)n;
+}
+
+void first_order_only(int order)
+{
+    if (order != 1) throw std::invalid_argument("Mesh: only 4-node (order 1) elements are provided");
+}
+
+}  // namespace
+
+Mesh::Mesh()
+{
+    cp_ctx* ctx = nullptr;
+    if (cp_ctx_create(0, &ctx) == CP_OK) {
+        char name[256] = {};
+        size_t mem = 0;
+        int cus = 0;
+        cp_device_info(ctx, name, sizeof(name), &mem, &cus);
+        cp_ctx_destroy(ctx);
+        useCuda = true;
+        std::cout << "Detected HIP device" << name << " " << std::llround((double)mem / 1048576.) << " MB, " << cus
+                  << " CUs\n\n";
+    } else {
+        useCuda = false;
+        std::cout << "No HIP device: meshes can be built, solving needs a GPU\n\n";
+    }
+}
+
+Mesh::Mesh(const Mesh& o) { *this = o; }
+
+Mesh& Mesh::operator=(const Mesh& o)
+{
+    if (this == &o) return *this;
+    clear();
+    useCuda = o.useCuda;
+    ramSaved = o.ramSaved;
+    analysed = o.analysed;
+    count3 = o.count3;
+    count4 = o.count4;
+    count8 = o.count8;
+    std::copy(o.elemPos, o.elemPos + 3, elemPos);
+    nodeCount = o.nodeCount;
+    if (o.node) {
+        node = new vec2[nodeCount];
+        std::copy(o.node, o.node + nodeCount, node);
+        secOrdNodes = new bool[nodeCount];
+        std::copy(o.secOrdNodes, o.secOrdNodes + nodeCount, secOrdNodes);
+    }
+    if (o.elem4) {
+        elem4 = new int[4 * (size_t)count4];
+        std::copy(o.elem4, o.elem4 + 4 * (size_t)count4, elem4);
+    }
+    bordersCount = o.bordersCount;
+    if (bordersCount) {
+        borderLength = new int[bordersCount];
+        borders = new int*[bordersCount];
+        for (int b = 0; b < bordersCount; ++b) {
+            borderLength[b] = o.borderLength[b];
+            borders[b] = new int[borderLength[b]];
+            std::copy(o.borders[b], o.borders[b] + borderLength[b], borders[b]);
+        }
+    }
+    return *this;
+}
+
+Mesh::~Mesh() { clear(); }
+
+void Mesh::clear()
+{
+    delete[] node;
+    delete[] secOrdNodes;
+    delete[] elem3;
+    delete[] elem4;
+    delete[] elem8;
+    for (int b = 0; b < bordersCount; ++b) delete[] borders[b];
+    delete[] borders;
+    delete[] borderLength;
+    node = nullptr;
+    secOrdNodes = nullptr;
+    elem3 = elem4 = elem8 = nullptr;
+    borders = nullptr;
+    borderLength = nullptr;
+    bordersCount = 0;
+    nodeCount = 0;
+    count3 = count4 = count8 = 0;
+    elemPos[0] = elemPos[1] = elemPos[2] = 0;
+    analysed = false;
+}
+
+// sizes(nn, ne, nb, blen) then gen(x, y, conn, bord, blen) as the C-ABI
+// generators define them; fills the reference's public arrays.
+template <typename Sizes, typename Gen>
+void Mesh::generate(Sizes sizes, Gen gen)
+{
+    int nn = 0, ne = 0, nb = 0;
+    int blen[16] = {};
+    check(sizes(&nn, &ne, &nb, blen), "mesh sizes");
+    if (nb > 16) throw std::runtime_error("mesh generator: too many borders");
+    std::vector<double> x(nn), y(nn);
+    std::vector<int> conn(4 * (size_t)ne);
+    size_t nbord = 0;
+    for (int b = 0; b < nb; ++b) nbord += (size_t)blen[b];
+    std::vector<int> bord(nbord);
+    check(gen(x.data(), y.data(), conn.data(), bord.data(), blen), "mesh generator");
+
+    clear();
+    nodeCount = nn;
+    node = new vec2[nn];
+    secOrdNodes = new bool[nn];
+    for (int i = 0; i < nn; ++i) {
+        node[i] = vec2(x[i], y[i]);
+        secOrdNodes[i] = false;
+    }
+    count4 = ne;
+    elem4 = new int[4 * (size_t)ne];
+    std::copy(conn.begin(), conn.end(), elem4);
+    elemPos[0] = 0;
+    elemPos[1] = ne;
+    elemPos[2] = ne;
+    bordersCount = nb;
+    borderLength = new int[nb];
+    borders = new int*[nb];
+    size_t off = 0;
+    for (int b = 0; b < nb; ++b) {
+        borderLength[b] = blen[b];
+        borders[b] = new int[blen[b]];
+        std::copy(bord.begin() + off, bord.begin() + off + blen[b], borders[b]);
+        off += blen[b];
+    }
+    ramSaved = true;
+}
+
+void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, size_t N2, int order)
+{
+    first_order_only(order);
+    const int n1 = as_int(N1, "genRectangle"), n2 = as_int(N2, "genRectangle");
+    generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect_sizes(n1, n2, nn, ne, nb, bl); },
+             [&](double* x, double* y, int* c, int* b, int* bl) {
+                 return cp_mesh_rect(x1, x2, y1, y2, n1, n2, x, y, c, b, bl);
+             });
+}
+
+void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double holeRad, size_t N1, size_t N2,
+                           int order)
+{
+    first_order_only(order);
+    const int n1 = as_int(N1, "genRectWithHole"), n2 = as_int(N2, "genRectWithHole");
+    generate(
+        [&](int* nn, int* ne, int* nb, int* bl) {
+            return cp_mesh_hole_sizes(x1, x2, y1, y2, holeRad, n1, n2, nn, ne, nb, bl);
+        },
+        [&](double* x, double* y, int* c, int* b, int* bl) {
+            return cp_mesh_hole(x1, x2, y1, y2, holeRad, n1, n2, x, y, c, b, bl);
+        });
+}
+
+void Mesh::genRing(double a, double b, size_t N_phi, size_t N_r, int order)
+{
+    first_order_only(order);
+    const int np = as_int(N_phi, "genRing"), nr = as_int(N_r, "genRing");
+    generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring_sizes(np, nr, nn, ne, nb, bl); },
+             [&](double* x, double* y, int* c, int* bd, int* bl) {
+                 return cp_mesh_ring(a, b, np, nr, x, y, c, bd, bl);
+             });
+}
+
+void Mesh::genQuarterPlate(double L, double r, size_t N_theta, size_t N_r)
+{
+    const int nt = as_int(N_theta, "genQuarterPlate"), nr = as_int(N_r, "genQuarterPlate");
+    generate(
+        [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_quarter_plate_sizes(nt, nr, nn, ne, nb, bl); },
+        [&](double* x, double* y, int* c, int* b, int* bl) {
+            return cp_mesh_quarter_plate(L, r, nt, nr, x, y, c, b, bl);
+        });
+}
+
+void Mesh::genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double ratio)
+{
+    const int nt = as_int(N_theta, "genQuarterRing"), nr = as_int(N_r, "genQuarterRing");
+    generate(
+        [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_quarter_ring_sizes(nt, nr, nn, ne, nb, bl); },
+        [&](double* x, double* y, int* c, int* bd, int* bl) {
+            return cp_mesh_quarter_ring(a, b, nt, nr, ratio, x, y, c, bd, bl);
+        });
+}
+
+int Mesh::findMaxIndexDiff() const
+{
+    int d = 0;
+    check(cp_mesh_max_index_diff(count4, elem4, &d), "findMaxIndexDiff");
+    return d;
+}
+
+double Mesh::elemSpace4(int e) const
+{
+    const int* c = elem4 + 4 * (size_t)e;
+    double s1 = 0., s2 = 0.;
+    for (int i = 0; i < 4; ++i) {
+        const vec2& p = node[c[i]];
+        const vec2& q = node[c[(i + 1) & 3]];
+        s1 += p.x * q.y;
+        s2 += p.y * q.x;
+    }
+    return std::fabs(s1 - s2) * 0.5;
+}
+
+double Mesh::aspectRatio(int e) const
+{
+    const int* c = elem4 + 4 * (size_t)e;
+    double lo = 0., hi = 0.;
+    for (int i = 0; i < 4; ++i) {
+        const double len = (node[c[(i + 1) & 3]] - node[c[i]]).norm();
+        lo = i == 0 ? len : std::min(lo, len);
+        hi = i == 0 ? len : std::max(hi, len);
+    }
+    return hi / lo;
+}
+
+double Mesh::skewAngleSin(int e) const
+{
+    const int* c = elem4 + 4 * (size_t)e;
+    const vec2 v1 = (node[c[0]] + node[c[1]] - node[c[2]] - node[c[3]]) * 0.5;
+    const vec2 v2 = (node[c[1]] + node[c[2]] - node[c[3]] - node[c[0]]) * 0.5;
+    const double d = v1 * v2;
+    return 1. - d * d / ((v1 * v1) * (v2 * v2));
+}
+
+double Mesh::avrElemSize() const
+{
+    if (count4 == 0) return 0.;
+    double s = 0.;
+    for (int e = 0; e < count4; ++e) s += elemSpace4(e);
+    return s / count4;
+}
+
+void Mesh::print() const
+{
+    std::cout << "Nodes: " << nodeCount << "\n";
+    for (int i = 0; i < nodeCount; ++i) std::cout << i << ": " << node[i].x << " " << node[i].y << "\n";
+    std::cout << "Quads: " << count4 << "\n";
+    for (int e = 0; e < count4; ++e)
+        std::cout << e << ": " << elem4[4 * e] << " " << elem4[4 * e + 1] << " " << elem4[4 * e + 2] << " "
+                  << elem4[4 * e + 3] << "\n";
+    for (int b = 0; b < bordersCount; ++b) {
+        std::cout << "Border " << b << ":";
+        for (int i = 0; i < borderLength[b]; ++i) std::cout << " " << borders[b][i];
+        std::cout << "\n";
+    }
+}

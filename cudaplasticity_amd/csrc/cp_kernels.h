@@ -109,7 +109,8 @@ struct MatParams {
 };
 
 // Material.cu:3-23 (linear / power); Swift law solved by Newton for sigma.
-__device__ __forceinline__ double mat_f(const MatParams& M, double eps)
+// Host and device share this one definition (cp_material_f is the host call).
+__host__ __device__ inline double mat_f(const MatParams& M, double eps)
 {
     const double epsT = M.sigmaT / M.E;
     if (eps < epsT) return eps * M.E;

@@ -436,6 +436,27 @@ int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
     return CP_OK;
 }
 
+int cp_rhs_body_force_nodal(int ne, const double* x, const double* y, const int* conn, const double* rnode,
+                            double rho, double h, double* rp)
+{
+    for (int e = 0; e < ne; ++e) {
+        const int* c = conn + 4 * e;
+        double s1 = 0., s2 = 0.;
+        for (int i = 1; i < 4; ++i) {
+            s1 += x[c[i - 1]] * y[c[i]];
+            s2 += y[c[i - 1]] * x[c[i]];
+        }
+        s1 += x[c[3]] * y[c[0]];
+        s2 += y[c[3]] * x[c[0]];
+        const double w = 0.25 * rho * h * (std::fabs(s1 - s2) * 0.5);
+        for (int i = 0; i < 4; ++i) {
+            rp[2 * c[i]] += w * rnode[2 * c[i]];
+            rp[2 * c[i] + 1] += w * rnode[2 * c[i] + 1];
+        }
+    }
+    return CP_OK;
+}
+
 int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask)
 {
     const double* v = vertical ? x : y;
@@ -462,6 +483,28 @@ int cp_displacement_border(const int* border, int len, double ux, double uy, int
                 const int d = 2 * border[i] + comp;
                 mask[d] = 0;
                 u[d] = val[comp];
+            }
+        }
+    }
+    return CP_OK;
+}
+
+int cp_displacement_border_values(const int* border, int len, const double* uval, int fix_mid_x, int fix_mid_y,
+                                  unsigned char* mask, double* u)
+{
+    if (len < 1) return cp::fail(CP_ERR_INVALID, "cp_displacement_border_values: empty border");
+    const int end = len - (border[0] == border[len - 1] ? 1 : 0);
+    const int mid[2] = {fix_mid_x, fix_mid_y};
+    for (int comp = 0; comp < 2; ++comp) {
+        if (mid[comp]) {
+            const int d = 2 * border[len / 2] + comp;
+            mask[d] = 0;
+            u[d] = uval[2 * (len / 2) + comp];
+        } else {
+            for (int i = 0; i < end; ++i) {
+                const int d = 2 * border[i] + comp;
+                mask[d] = 0;
+                u[d] = uval[2 * i + comp];
             }
         }
     }

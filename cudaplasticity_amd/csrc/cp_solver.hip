@@ -806,6 +806,15 @@ const char* cp_last_error(void) { return g_err.c_str(); }
 
 int cp_abi_version(void) { return CP_ABI_VERSION; }
 
+int cp_material_f(const cp_material* mat, double eps, double* sigma)
+{
+    if (!mat || !sigma) return cp::fail(CP_ERR_INVALID, "cp_material_f: null argument");
+    if (mat->law < CP_LAW_LINEAR || mat->law > CP_LAW_SWIFT) return cp::fail(CP_ERR_INVALID, "unknown material law");
+    const cpk::MatParams M{mat->E, mat->nu, mat->h, mat->sigmaT, mat->law, mat->plane, mat->K_T, mat->A, mat->m, mat->n};
+    *sigma = cpk::mat_f(M, eps);
+    return CP_OK;
+}
+
 int cp_ctx_create(int device, cp_ctx** out)
 {
     if (!out) return cp::fail(CP_ERR_INVALID, "cp_ctx_create: null out");

@@ -120,6 +120,9 @@ int         cp_ctx_destroy(cp_ctx* ctx);
 const char* cp_last_error(void);
 int         cp_device_info(cp_ctx* ctx, char* name, int name_len, size_t* total_mem, int* cu_count);
 int         cp_abi_version(void);
+/* Material::f (Material.cu:3-23; Swift law new): the stress intensity the
+ * device kernels use for strain intensity eps, evaluated on the host. */
+int         cp_material_f(const cp_material* mat, double eps, double* sigma);
 
 /* ------------------------------------------------------------------ */
 /* host-side mesh generators (order 1) -- Mesh.cu                      */
@@ -160,11 +163,20 @@ int cp_rhs_border_force(const double* x, const double* y, const int* border, int
 /* Constant body force on 4-node elements (:1106-1117). */
 int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
                       double gx, double gy, double rho, double h, double* rp);
+/* Body force given at every node (rnode: 2 per node), 4-node elements
+ * (:1106-1117; R is evaluated at the element's own nodes -- the reference
+ * indexes node[2 * elem] there, which only constant R survives). */
+int cp_rhs_body_force_nodal(int ne, const double* x, const double* y, const int* conn, const double* rnode,
+                            double rho, double h, double* rp);
 /* LoadConditions::fixVertAxis / fixHorAxis (:1165-1181). */
 int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask);
 /* LoadConditions::setDisplacement on a border (:1138-1164). */
 int cp_displacement_border(const int* border, int len, double ux, double uy,
                            int fix_mid_x, int fix_mid_y, unsigned char* mask, double* u);
+/* The same with a displacement per border entry (uval: 2 per entry), as
+ * LoadConditions::setDisplacement's function would return it. */
+int cp_displacement_border_values(const int* border, int len, const double* uval,
+                                  int fix_mid_x, int fix_mid_y, unsigned char* mask, double* u);
 
 /* ------------------------------------------------------------------ */
 /* CSR pattern in the reference layout -- CudaSparseSLAE.cuh:303-458    */

@@ -1,0 +1,81 @@
+// Mesh.hpp -- mesh of the drop-in C++ API (Mesh.cuh:21-244).
+//
+// Same public members (node, elem4, borders, counts, flags) and generator
+// signatures as the reference; the generators run in libcp_hip (host code)
+// and produce the reference's node numbering and element order.  Meshes are
+// host data here: the solver copies them to the device when it is built, so
+// meshToGPU / meshToRAM are no-ops kept for source compatibility.  Only
+// 4-node quads exist in this library (order 2 is refused).
+#pragma once
+
+#include <cstddef>
+#include <string>
+
+#include "vec2.hpp"
+
+class Mesh {
+public:
+    bool useCuda = false;   // a HIP device was found at construction
+    bool ramSaved = true;   // host arrays are current (always, here)
+    bool analysed = false;
+
+    int count3 = 0;
+    int count4 = 0;
+    int count8 = 0;
+
+    int* elem3 = nullptr;
+    int* elem4 = nullptr;   // 4 node indices per quad, reference local order
+    int* elem8 = nullptr;
+
+    int elemPos[3] = {};    // first index of triangles / quads / 8-node elements
+
+    int nodeCount = 0;
+    vec2* node = nullptr;
+    bool* secOrdNodes = nullptr;   // all false: first-order meshes only
+
+    int bordersCount = 0;
+    int* borderLength = nullptr;
+    int** borders = nullptr;
+
+    Mesh();
+    Mesh(const Mesh& other);
+    Mesh& operator=(const Mesh& other);
+    ~Mesh();
+
+    int elemCount() const { return elemPos[2]; }
+
+    // Mesh.cu:17-109
+    void genRectangle(double x1, double x2, double y1, double y2, size_t N1, size_t N2, int order = 1);
+    // Mesh.cu:112-245 (+ renumerateRing)
+    void genRectWithHole(double x1, double x2, double y1, double y2, double holeRad, size_t N1, size_t N2,
+                         int order = 1);
+    // Mesh.cu:249-329 (+ renumerateRing)
+    void genRing(double a, double b, size_t N_phi, size_t N_r, int order = 1);
+    // New: quarter plate [0,L]^2 minus a quarter disc of radius r at the origin.
+    // Borders: 0 y=0, 1 x=L, 2 y=L, 3 x=0, 4 hole arc.
+    void genQuarterPlate(double L, double r, size_t N_theta, size_t N_r);
+    // New: quarter ring a <= rho <= b, radial sizes growing by `ratio` from the
+    // inner radius.  Borders: 0 inner arc, 1 theta=pi/2, 2 outer arc, 3 theta=0.
+    void genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double ratio);
+
+    // Mesh.cu:417-441
+    int findMaxIndexDiff() const;
+
+    // Mesh.cuh:258-267 (shoelace area of quad e)
+    double elemSpace4(int e) const;
+    // Mesh.cuh:281-293
+    double aspectRatio(int e) const;
+    // Mesh.cuh:296-305
+    double skewAngleSin(int e) const;
+    // mean quad area
+    double avrElemSize() const;
+
+    void meshToRAM() {}
+    void meshToGPU() {}
+    void print() const;
+
+private:
+    void clear();
+    template <typename Sizes, typename Gen>
+    void generate(Sizes sizes, Gen gen);
+};

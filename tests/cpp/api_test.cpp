@@ -1,0 +1,293 @@
+// api_test.cpp -- drives the drop-in C++ API (include/cudaplasticity/) the
+// way the reference's experiments drive its classes, and writes raw arrays
+// into OUTDIR for tests/test_cpp_api.py to check.
+//
+//   api_test cpu OUTDIR   meshes, material laws and load conditions (no device)
+//   api_test gpu OUTDIR   solves through PlasticitySolver on the GPU
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "cudaplasticity/cudaplasticity.hpp"
+
+namespace {
+
+std::string outdir;
+
+template <typename T>
+void dump(const std::string& name, const T* data, size_t n)
+{
+    std::ofstream f(outdir + "/" + name, std::ios::binary);
+    f.write(reinterpret_cast<const char*>(data), sizeof(T) * n);
+}
+
+template <typename T>
+void dump(const std::string& name, const std::vector<T>& v)
+{
+    dump(name, v.data(), v.size());
+}
+
+void dump_mesh(const std::string& tag, const Mesh& m)
+{
+    std::vector<double> xy(2 * (size_t)m.nodeCount);
+    for (int i = 0; i < m.nodeCount; ++i) {
+        xy[2 * i] = m.node[i].x;
+        xy[2 * i + 1] = m.node[i].y;
+    }
+    dump(tag + "_xy.f64", xy);
+    dump(tag + "_elem4.i32", m.elem4, 4 * (size_t)m.count4);
+    std::vector<double> q(3 * (size_t)m.count4);
+    for (int e = 0; e < m.count4; ++e) {
+        q[3 * e] = m.elemSpace4(e);
+        q[3 * e + 1] = m.aspectRatio(e);
+        q[3 * e + 2] = m.skewAngleSin(e);
+    }
+    dump(tag + "_quality.f64", q);
+    std::vector<int> meta = {m.nodeCount, m.count4, m.bordersCount, m.findMaxIndexDiff()};
+    dump(tag + "_meta.i32", meta);
+}
+
+void dump_conditions(const std::string& tag, const PlasticitySolver& ps)
+{
+    std::vector<double> rp, u0;
+    std::vector<unsigned char> mask;
+    ps.initConditions(rp, u0, mask);
+    dump(tag + "_rp.f64", rp);
+    dump(tag + "_u0.f64", u0);
+    dump(tag + "_mask.u8", mask);
+}
+
+// plasticityTest (Main.cpp:252-296): 304L steel plate with a hole.
+const double E_304L = 193e9, S_T_304L = 170e6, NU_304L = 0.29;
+
+void hole_plate(Mesh& mesh, LoadConditions& lc, Material& m)
+{
+    mesh.genRectWithHole(-3., 3., -2., 2., 0.5, 48, 32, 1);
+    const double S = 170;
+    lc.setForce(1, vec2(S, 0.));
+    lc.setForce(3, vec2(-S, 0.));
+    lc.fixVertAxis(0.);
+    lc.fixHorAxis(0.);
+    m.h = 1.;
+    m.setPowerPlast(E_304L * 1e-6, S_T_304L * 1e-6, E_304L * 0.01 * 1e-6, 0.015);
+    m.nu = NU_304L;
+}
+
+int run_cpu()
+{
+    {
+        Mesh mesh;
+        LoadConditions lc;
+        Material m;
+        hole_plate(mesh, lc, m);
+        dump_mesh("hole", mesh);
+        PlasticitySolver ps(m, mesh, lc);
+        dump_conditions("hole", ps);
+        // the solver deep-copies the mesh
+        Mesh copy(mesh);
+        dump_mesh("holecopy", copy);
+    }
+    {
+        // testRing (Main.cpp:75-106): position-dependent border force, h = 0.1
+        Mesh mesh;
+        mesh.genRing(0.1, 0.2, 40, 5);
+        LoadConditions lc;
+        lc.setForce(0, [](vec2 r) { return vec2(1e6 * std::fabs(r.x), 0); });
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.E = 1.1e10;
+        m.nu = 0.3;
+        m.h = 0.1;
+        dump_mesh("ring", mesh);
+        PlasticitySolver ps(m, mesh, lc);
+        dump_conditions("ring", ps);
+    }
+    {
+        // diagramTest (Main.cpp:929-994) level: gravity and an end displacement
+        Mesh mesh;
+        mesh.genRectangle(-1, 1, -1, 1, 2, 2);
+        LoadConditions lc;
+        lc.setGravity();
+        lc.setDisplacement(2, vec2(0, 0), false, true);
+        lc.setDisplacement(0, vec2(0.0015, 0), false, true);
+        Material m;
+        m.setLinearPlast(E_304L, S_T_304L, 0.1 * E_304L);
+        m.nu = NU_304L;
+        dump_mesh("diag", mesh);
+        PlasticitySolver ps(m, mesh, lc);
+        dump_conditions("diag", ps);
+    }
+    {
+        // graded quarter ring, normal-oriented internal pressure
+        Mesh mesh;
+        mesh.genQuarterRing(0.5, 1.0, 48, 12, 1.05);
+        LoadConditions lc;
+        lc.setForce(0, vec2(-180., 0.), true);
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.setSwiftPlast(2.1e5, 240., 0.2);
+        dump_mesh("qring", mesh);
+        PlasticitySolver ps(m, mesh, lc);
+        dump_conditions("qring", ps);
+    }
+    {
+        // point conditions, fixed border, rotation body force, displacement function
+        Mesh mesh;
+        mesh.genRectangle(0., 1., 0., 1., 4, 4);
+        LoadConditions lc;
+        lc.setForcePoint(vec2(1., 1.), vec2(3., 4.));
+        lc.setDisplPoint(vec2(0., 0.), vec2(0.1, 0.2));
+        lc.fixPoint(vec2(1., 0.));
+        lc.fixBorderVert(3);
+        lc.setRotation(2.0, vec2(0.5, 0.));
+        lc.setDisplacement(2, [](vec2 r) { return vec2(0.01 * r.x, -0.02 * r.x * r.x); });
+        Material m;
+        m.rho = 3.;
+        m.h = 0.5;
+        dump_mesh("pts", mesh);
+        PlasticitySolver ps(m, mesh, lc);
+        dump_conditions("pts", ps);
+    }
+    {
+        // material laws on the host (the device's own law code)
+        Material lin, pw, sw;
+        lin.setLinearPlast(2.1e5, 240., 2.1e5 / 101);
+        pw.setPowerPlast(193e3, 170., 1930., 0.015);
+        sw.setSwiftPlast(2.1e5, 240., 0.2);
+        std::vector<double> out;
+        for (double eps : {1e-4, 1e-3, 2e-3, 5e-3, 2e-2, 0.1}) {
+            out.push_back(lin.f(eps));
+            out.push_back(pw.f(eps));
+            out.push_back(sw.f(eps));
+        }
+        dump("laws.f64", out);
+    }
+    {
+        // order-2 meshes and the CPU solver are refused, loudly
+        int refused = 0;
+        Mesh mesh;
+        try {
+            mesh.genRectangle(0, 1, 0, 1, 2, 2, 2);
+        } catch (const std::invalid_argument&) {
+            ++refused;
+        }
+        mesh.genRectangle(0, 1, 0, 1, 2, 2);
+        PlasticitySolver ps(Material(), mesh, LoadConditions());
+        try {
+            ps.solveCPU();
+        } catch (const std::runtime_error&) {
+            ++refused;
+        }
+        std::vector<int> r = {refused};
+        dump("refused.i32", r);
+    }
+    std::cout << "cpu cases written to " << outdir << "\n";
+    return 0;
+}
+
+int run_gpu()
+{
+    {
+        // the reference's plasticityTest through the drop-in API, reference defaults
+        Mesh mesh;
+        LoadConditions lc;
+        Material m;
+        hole_plate(mesh, lc, m);
+        PlasticitySolver ps(m, mesh, lc);
+        const double t = ps.solve();
+        const cp_stats& st = ps.lastStats();
+        dump("plast_u.f64", ps.displacements());
+        std::vector<double> F;
+        for (int f = 0; f < CP_NFIELDS; ++f) F.insert(F.end(), ps.field(f).begin(), ps.field(f).end());
+        dump("plast_fields.f64", F);
+        std::vector<double> s = {(double)st.outer_iters, st.rel_err, (double)st.status, ps.deformEnergy(),
+                                 ps.getSxx0(), ps.getExx0(), t};
+        dump("plast_stats.f64", s);
+        ps.saveDisplsToFile(outdir + "/plast_displs.txt");
+    }
+    {
+        // config-1 style elastic solve with the Kirsch check of Main.cpp:65-68
+        Mesh mesh;
+        mesh.genQuarterPlate(1.0, 0.1, 100, 49);
+        LoadConditions lc;
+        lc.setForce(1, vec2(100., 0.));
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.E = 2.1e5;
+        m.nu = 0.3;
+        PlasticitySolver ps(m, mesh, lc);
+        ps.options.cgTol = 1e-10;
+        ps.iterOutput = false;
+        ps.solveElast();
+        dump("elast_u.f64", ps.displacements());
+        dump("elast_sxx.f64", ps.field(CP_F_SXX));
+        const double a = 0.1, S = 100.;
+        auto kirsch_r = [&](vec2 p) {
+            const double r = p.x, th = p.y, q = a * a / (r * r);
+            return 0.5 * S * (1 - q) + 0.5 * S * (1 - 4 * q + 3 * q * q) * std::cos(2 * th);
+        };
+        auto kirsch_t = [&](vec2 p) {
+            const double r = p.x, th = p.y, q = a * a / (r * r);
+            return 0.5 * S * (1 + q) - 0.5 * S * (1 + 3 * q * q) * std::cos(2 * th);
+        };
+        auto kirsch_rt = [&](vec2 p) {
+            const double r = p.x, th = p.y, q = a * a / (r * r);
+            return -0.5 * S * (1 + 2 * q - 3 * q * q) * std::sin(2 * th);
+        };
+        std::vector<double> e = {ps.polarSigmaError(kirsch_r, kirsch_t, kirsch_rt, 0.3)};
+        dump("elast_polar_err.f64", e);
+    }
+    {
+        // additive API: load steps, mixed precision, block-Jacobi, predictor, forcing
+        Mesh mesh;
+        mesh.genQuarterPlate(1.0, 0.1, 40, 20);
+        LoadConditions lc;
+        lc.setForce(1, vec2(300., 0.));
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.setLinearPlast(2.1e5, 240., 2.1e5 / 101);
+        PlasticitySolver ps(m, mesh, lc);
+        ps.iterOutput = false;
+        ps.options.cgTol = 1e-9;
+        ps.options.tol = 1e-6;
+        ps.options.aitken = true;
+        ps.options.mixedPrecision = true;
+        ps.options.blockJacobi = true;
+        ps.options.predictor = 1;
+        ps.options.cgForcing = 0.1;
+        ps.options.innerTol = 1e-4;
+        ps.solveSteps(4, 1.0);
+        dump("steps_u.f64", ps.displacements());
+        dump("steps_sigi.f64", ps.field(CP_F_SIGI));
+    }
+    std::cout << "gpu cases written to " << outdir << "\n";
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    if (argc != 3) {
+        std::cerr << "usage: api_test cpu|gpu OUTDIR\n";
+        return 2;
+    }
+    outdir = argv[2];
+    const std::string mode = argv[1];
+    try {
+        if (mode == "cpu") return run_cpu();
+        if (mode == "gpu") return run_gpu();
+    } catch (const std::exception& e) {
+        std::cerr << "api_test: " << e.what() << "\n";
+        return 1;
+    }
+    std::cerr << "unknown mode " << mode << "\n";
+    return 2;
+}

@@ -1,0 +1,201 @@
+"""The drop-in C++ API (include/cudaplasticity/, libcudaplasticity.so),
+driven by tests/cpp/api_test.cpp the way the reference's experiments drive
+its classes (Main.cpp).  CPU cases check meshes, material laws and the load
+conditions the solver builds against the oracle; the GPU case runs the
+reference's plasticityTest through PlasticitySolver::solve() with the
+reference's defaults and compares with the reference's own output file."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cudaplasticity_amd import abi as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "_build", "api_test")
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def run(mode, outdir):
+    assert os.path.exists(BIN), "tests/cpp/_build/api_test is not built: run __graft_entry__.build()"
+    print("api_test", mode, flush=True)
+    r = subprocess.run([BIN, mode, str(outdir)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-3000:], flush=True)
+    assert r.returncode == 0, r.stderr
+
+
+def rd(d, name):
+    kind = {"f64": np.float64, "i32": np.int32, "u8": np.uint8}[name.rsplit(".", 1)[1]]
+    return np.fromfile(os.path.join(d, name), dtype=kind)
+
+
+@pytest.fixture(scope="module")
+def cpu_out(tmp_path_factory):
+    d = tmp_path_factory.mktemp("api_cpu")
+    run("cpu", d)
+    return d
+
+
+def check_mesh(d, tag, om):
+    xy = rd(d, f"{tag}_xy.f64").reshape(-1, 2)
+    assert np.array_equal(xy[:, 0], om.x) and np.array_equal(xy[:, 1], om.y)
+    assert np.array_equal(rd(d, f"{tag}_elem4.i32").reshape(-1, 4), om.conn)
+    meta = rd(d, f"{tag}_meta.i32")
+    assert meta[0] == om.nn and meta[1] == om.ne and meta[2] == len(om.borders)
+    assert meta[3] == O.lib.or_max_index_diff(om.ne, O.ip(om.conn))
+    q = rd(d, f"{tag}_quality.f64").reshape(-1, 3)
+    area = np.array([O.elem_space4(om, e) for e in range(om.ne)])
+    assert np.array_equal(q[:, 0], area)
+    P = np.stack([om.x[om.conn], om.y[om.conn]], axis=-1)          # ne x 4 x 2
+    sides = np.linalg.norm(np.roll(P, -1, axis=1) - P, axis=-1)
+    np.testing.assert_allclose(q[:, 1], sides.max(1) / sides.min(1), rtol=1e-14)
+    v1 = 0.5 * (P[:, 0] + P[:, 1] - P[:, 2] - P[:, 3])
+    v2 = 0.5 * (P[:, 1] + P[:, 2] - P[:, 3] - P[:, 0])
+    d12 = (v1 * v2).sum(1)
+    np.testing.assert_allclose(q[:, 2], 1 - d12 ** 2 / ((v1 * v1).sum(1) * (v2 * v2).sum(1)), rtol=1e-12,
+                               atol=1e-15)
+
+
+def check_conditions(d, tag, prob):
+    assert np.array_equal(rd(d, f"{tag}_rp.f64"), prob.rp), tag
+    assert np.array_equal(rd(d, f"{tag}_mask.u8"), prob.mask), tag
+    assert np.array_equal(rd(d, f"{tag}_u0.f64"), prob.u), tag
+
+
+def test_hole_plate_mesh_and_conditions(cpu_out):
+    """plasticityTest's mesh (Main.cpp:256) and loads (:266-272)."""
+    om = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)
+    check_mesh(cpu_out, "hole", om)
+    check_mesh(cpu_out, "holecopy", om)
+    fx = np.load(os.path.join(GOLD, "holePlatePlastPow.npz"))
+    assert np.array_equal(rd(cpu_out, "hole_elem4.i32").reshape(-1, 4), fx["polys"])
+    check_conditions(cpu_out, "hole",
+                     O.Problem(om).force(1, 170.0, 0.0).force(3, -170.0, 0.0).fix_vert_axis(0).fix_hor_axis(0))
+
+
+def test_ring_function_force(cpu_out):
+    """testRing (Main.cpp:75-106): setForce with a lambda, thickness 0.1."""
+    om = O.gen_ring(0.1, 0.2, 40, 5)
+    check_mesh(cpu_out, "ring", om)
+    b = om.borders[0]
+    fv = np.stack([1e6 * np.abs(om.x[b]), np.zeros(b.size)], axis=1)
+    check_conditions(cpu_out, "ring", O.Problem(om, h=0.1).force_values(0, fv).fix_vert_axis(0).fix_hor_axis(0))
+
+
+def test_gravity_and_displacements(cpu_out):
+    """diagramTest level (Main.cpp:929-994): setGravity + setDisplacement."""
+    om = O.gen_rect(-1, 1, -1, 1, 2, 2)
+    check_mesh(cpu_out, "diag", om)
+    check_conditions(cpu_out, "diag", O.Problem(om).gravity().displacement(0, 0.0015, 0.0, False, True)
+                     .displacement(2, 0.0, 0.0, False, True))
+
+
+def test_normal_pressure_quarter_ring(cpu_out):
+    om = O.gen_quarter_ring(0.5, 1.0, 48, 12, 1.05)
+    check_mesh(cpu_out, "qring", om)
+    check_conditions(cpu_out, "qring",
+                     O.Problem(om).force(0, -180.0, 0.0, norm_orient=True).fix_vert_axis(0).fix_hor_axis(0))
+
+
+def test_points_fixed_border_rotation_function_displacement(cpu_out):
+    """setRotation, fixBorderVert, setForcePoint, setDisplPoint, fixPoint and
+    a displacement lambda (LoadConditions.cuh:69-179; their effect on the
+    load vector and mask as PlasticitySolver.cu:895-1003 applies them)."""
+    om = O.gen_rect(0, 1, 0, 1, 4, 4)
+    check_mesh(cpu_out, "pts", om)
+    nn = om.nn
+    rp = np.zeros(2 * nn)
+    u = np.zeros(2 * nn)
+    mask = np.ones(2 * nn, dtype=np.uint8)
+    rho, h, w2, axis = 3.0, 0.5, 4.0, np.array([0.5, 0.0])
+    for e in range(om.ne):
+        c = om.conn[e]
+        wgt = 0.25 * rho * h * O.elem_space4(om, e)
+        for n in c:
+            R = w2 * (np.array([om.x[n], om.y[n]]) - axis)
+            rp[2 * n] += wgt * R[0]
+            rp[2 * n + 1] += wgt * R[1]
+    b2 = om.borders[2]
+    end2 = b2.size - (1 if b2[0] == b2[-1] else 0)
+    for n in b2[:end2]:
+        u[2 * n], u[2 * n + 1] = 0.01 * om.x[n], -0.02 * om.x[n] * om.x[n]
+        mask[2 * n] = mask[2 * n + 1] = 0
+    b3 = om.borders[3]
+    end3 = b3.size - (1 if b3[0] == b3[-1] else 0)
+    mask[2 * b3[:end3]] = 0
+    node = lambda x, y: int(np.flatnonzero((om.x == x) & (om.y == y))[0])
+    n11 = node(1.0, 1.0)
+    rp[2 * n11] += 3.0
+    rp[2 * n11 + 1] += 4.0
+    for (x, y), val in (((0.0, 0.0), (0.1, 0.2)), ((1.0, 0.0), (0.0, 0.0))):
+        n = node(x, y)
+        u[2 * n], u[2 * n + 1] = val
+        mask[2 * n] = mask[2 * n + 1] = 0
+    np.testing.assert_allclose(rd(cpu_out, "pts_rp.f64"), rp, rtol=1e-14, atol=1e-15)
+    assert np.array_equal(rd(cpu_out, "pts_mask.u8"), mask)
+    assert np.array_equal(rd(cpu_out, "pts_u0.f64"), u)
+
+
+def test_material_laws_on_host(cpu_out):
+    got = rd(cpu_out, "laws.f64").reshape(-1, 3)
+    mats = [O.material(2.1e5, 0.3, 240.0, O.LAW_LINEAR, K_T=2.1e5 / 101),
+            O.material(193e3, 0.3, 170.0, O.LAW_POWER, A=1930.0, m=0.015),
+            O.material(2.1e5, 0.3, 240.0, O.LAW_SWIFT, n=0.2)]
+    for i, eps in enumerate([1e-4, 1e-3, 2e-3, 5e-3, 2e-2, 0.1]):
+        for j, m in enumerate(mats):
+            assert got[i, j] == O.material_f(m, eps), (i, j)
+
+
+def test_unsupported_paths_refuse(cpu_out):
+    """order-2 meshes and solveCPU throw instead of silently degrading."""
+    assert rd(cpu_out, "refused.i32")[0] == 2
+
+
+@pytest.mark.gpu
+def test_plasticity_test_through_cpp_api(tmp_path):
+    """The reference's plasticityTest (Main.cpp:252-296) through
+    PlasticitySolver::solve() with the reference's defaults reproduces
+    data/holePlatePlastPow.vtk; config-1 elastic solve within 1e-8 of the
+    oracle; load steps with the additive options equal the C-ABI path."""
+    run("gpu", tmp_path)
+    fx = np.load(os.path.join(GOLD, "holePlatePlastPow.npz"))
+    om = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)
+    u = rd(tmp_path, "plast_u.f64")
+    F = rd(tmp_path, "plast_fields.f64").reshape(len(A.FIELDS), -1)
+    st = rd(tmp_path, "plast_stats.f64")
+    rel = lambda a, b: np.linalg.norm(a - b) / np.linalg.norm(b)
+    assert rel(u, fx["f_Displacement"][:, :2].reshape(-1)) < 2e-7
+    for k, n in [(O.FIELDS.index("sxx"), "Sigma_xx"), (O.FIELDS.index("sigI"), "Sigma_intensity"),
+                 (O.FIELDS.index("psi"), "Psi"), (O.FIELDS.index("epsI_p"), "Eps_p_intensity")]:
+        assert rel(F[k], fx["f_" + n]) < 2e-7, n
+    E = 193e9 * 1e-6
+    mat = O.material(E, 0.29, 170.0, O.LAW_POWER, A=E * 0.01, m=0.015)
+    flags = F[O.FIELDS.index("epsI")] >= mat.sigmaT / mat.E
+    assert flags.sum() == 3528
+    assert np.array_equal(flags, fx["f_Eps_intensity"] >= mat.sigmaT / mat.E)
+    assert st[2] == 0 and abs(st[0] - 41) <= 1 and st[1] <= 1e-5
+    assert abs(st[3] - O.deform_energy(om, mat, F)) <= 1e-12 * abs(st[3])
+    assert st[4] == F[O.FIELDS.index("sxx"), 0] and st[5] == F[O.FIELDS.index("exx"), 0]
+    assert os.path.getsize(os.path.join(tmp_path, "plast_displs.txt")) > 0
+
+    # config-1 elastic
+    qm = O.gen_quarter_plate(1.0, 0.1, 100, 49)
+    qp = O.Problem(qm).force(1, 100.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
+    uo, so, _ = O.solve(qp, O.material(2.1e5, 0.3), O.options(cg_tol=1e-10, elastic=True))
+    assert rel(rd(tmp_path, "elast_u.f64"), uo) < 1e-8
+    assert rel(rd(tmp_path, "elast_sxx.f64"), O.fields(so)["sxx"]) < 1e-8
+    assert rd(tmp_path, "elast_polar_err.f64")[0] < 0.2   # Kirsch (infinite plate) near the hole
+
+    # additive API == C-ABI path
+    pm = A.gen_quarter_plate(1.0, 0.1, 40, 20)
+    pl = A.Loads(pm).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    ctx = A.Context(0)
+    P = A.Problem(ctx, pm, pl, A.material(2.1e5, 0.3, 240.0, A.LAW_LINEAR, K_T=2.1e5 / 101))
+    P.solve_steps(4, 1.0, A.options(cg_tol=1e-9, tol=1e-6, aitken=True, max_outer=1000, precision=A.MIXED,
+                                    precond=True, predictor=1, cg_forcing=0.1, inner_tol=1e-4))
+    assert np.array_equal(rd(tmp_path, "steps_u.f64"), P.field("u"))
+    assert np.array_equal(rd(tmp_path, "steps_sigi.f64"), P.field("sigI"))
+    P.close()
+    ctx.close()

@@ -99,3 +99,30 @@ def test_problem_bytes_budget_monotone():
     b3 = C.c_size_t()
     A.check(A.lib().cp_problem_bytes(201000, 200000, 7200000, A.FP64, C.byref(b3)))
     assert b3.value > b2.value
+
+
+def test_nodal_load_entry_points_match_constant_ones():
+    """cp_rhs_body_force_nodal with a constant field equals cp_rhs_body_force;
+    cp_displacement_border_values with constant values equals
+    cp_displacement_border (both bit-exact, every fix-mid combination)."""
+    m = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 24, 16)
+    lib = A.lib()
+    import ctypes as C
+    rp1 = np.zeros(2 * m.nn)
+    rp2 = np.zeros(2 * m.nn)
+    A.check(lib.cp_rhs_body_force(m.ne, A.dp(m.x), A.dp(m.y), A.ip(m.conn), C.c_double(0.5), C.c_double(-9.81),
+                                  C.c_double(2.0), C.c_double(0.3), A.dp(rp1)))
+    rn = np.tile([0.5, -9.81], m.nn)
+    A.check(lib.cp_rhs_body_force_nodal(m.ne, A.dp(m.x), A.dp(m.y), A.ip(m.conn), A.dp(rn), C.c_double(2.0),
+                                        C.c_double(0.3), A.dp(rp2)))
+    assert np.array_equal(rp1, rp2)
+    b = m.borders[2]
+    for mx in (0, 1):
+        for my in (0, 1):
+            k1, u1 = np.ones(2 * m.nn, np.uint8), np.zeros(2 * m.nn)
+            k2, u2 = np.ones(2 * m.nn, np.uint8), np.zeros(2 * m.nn)
+            A.check(lib.cp_displacement_border(A.ip(b), b.size, C.c_double(1e-3), C.c_double(-2e-3), mx, my,
+                                               A.u8p(k1), A.dp(u1)))
+            uv = np.tile([1e-3, -2e-3], b.size)
+            A.check(lib.cp_displacement_border_values(A.ip(b), b.size, A.dp(uv), mx, my, A.u8p(k2), A.dp(u2)))
+            assert np.array_equal(k1, k2) and np.array_equal(u1, u2)
