@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <stdexcept>
 #include <string>
@@ -73,6 +74,14 @@ Mesh& Mesh::operator=(const Mesh& o)
         elem4 = new int[4 * (size_t)count4];
         std::copy(o.elem4, o.elem4 + 4 * (size_t)count4, elem4);
     }
+    double* const* src[3] = {&o.spaces, &o.aspects, &o.skewAngles};
+    double** dst[3] = {&spaces, &aspects, &skewAngles};
+    for (int k = 0; k < 3; ++k)
+        if (*src[k]) {
+            *dst[k] = new double[count4];
+            std::copy(*src[k], *src[k] + count4, *dst[k]);
+        }
+    if (!spaces) analysed = false;
     bordersCount = o.bordersCount;
     if (bordersCount) {
         borderLength = new int[bordersCount];
@@ -90,6 +99,10 @@ Mesh::~Mesh() { clear(); }
 
 void Mesh::clear()
 {
+    delete[] spaces;
+    delete[] aspects;
+    delete[] skewAngles;
+    spaces = aspects = skewAngles = nullptr;
     delete[] node;
     delete[] secOrdNodes;
     delete[] elem3;
@@ -254,6 +267,73 @@ double Mesh::avrElemSize() const
     double s = 0.;
     for (int e = 0; e < count4; ++e) s += elemSpace4(e);
     return s / count4;
+}
+
+void Mesh::printAnalysis()
+{
+    delete[] spaces;
+    delete[] aspects;
+    delete[] skewAngles;
+    spaces = new double[count4];
+    aspects = new double[count4];
+    skewAngles = new double[count4];
+    double sa = 0., smin = 1e300, smax = 0., aa = 0., amin = 1e300, amax = 0., ka = 0., kmin = 1e300, kmax = 0.;
+    for (int e = 0; e < count4; ++e) {
+        const double s = elemSpace4(e), a = aspectRatio(e), k = skewAngleSin(e);
+        spaces[e] = s;
+        aspects[e] = a;
+        skewAngles[e] = std::acos(k);   // the file's Element_skew_angle column
+        sa += s;
+        aa += a;
+        ka += k;
+        smin = std::min(smin, s);
+        smax = std::max(smax, s);
+        amin = std::min(amin, a);
+        amax = std::max(amax, a);
+        kmin = std::min(kmin, k);
+        kmax = std::max(kmax, k);
+    }
+    if (count4) {
+        sa /= count4;
+        aa /= count4;
+        ka /= count4;
+    }
+    std::cout << "Mesh analysis:\nNodes max index difference in element: " << findMaxIndexDiff()
+              << "\nSpace: avr = " << sa << ", min = " << smin << ", max = " << smax << "\nAspect ratio "
+              << 100. / aa << "%: avr = " << aa << ", min = " << amin << ", max = " << amax
+              << "\nSkew angle sin " << 100 * ka << "%: avr = " << ka << ", min = " << kmin << ", max = " << kmax
+              << "\n"
+              << std::endl;
+    analysed = true;
+}
+
+// Legacy VTK polydata: points (6 significant digits), quads, element ids,
+// the quality fields when analysed, node ids.
+void Mesh::saveAsVtk(const std::string& fileName) const
+{
+    std::ofstream f(fileName);
+    if (!f) throw std::runtime_error("Mesh::saveAsVtk: cannot open " + fileName);
+    f << "# vtk DataFile Version 2.0\nMesh\nASCII\nDATASET POLYDATA\nPOINTS " << nodeCount << " float\n";
+    for (int i = 0; i < nodeCount; ++i) f << node[i].x << " " << node[i].y << " " << 0 << "\n";
+    f << "POLYGONS " << elemCount() << " " << 5 * count4;
+    for (int e = 0; e < count4; ++e) {
+        f << "\n4 ";
+        for (int j = 0; j < 4; ++j) f << elem4[4 * e + j] << " ";
+    }
+    f << "\nCELL_DATA " << elemCount() << "\nSCALARS ElementID int 1\nLOOKUP_TABLE default\n";
+    for (int e = 0; e < elemCount(); ++e) f << e << "\n";
+    if (analysed) {
+        const double* cols[3] = {spaces, aspects, skewAngles};
+        const char* names[3] = {"Element_space", "Element_aspect_ratio", "Element_skew_angle"};
+        f << "FIELD FieldData 3\n";
+        for (int c = 0; c < 3; ++c) {
+            if (c) f << "\n";
+            f << names[c] << " 1 " << elemCount() << " float\n";
+            for (int e = 0; e < elemCount(); ++e) f << cols[c][e] << " ";
+        }
+    }
+    f << "\nPOINT_DATA " << nodeCount << "\nSCALARS NodeID int 1\nLOOKUP_TABLE my_table";
+    for (int i = 0; i < nodeCount; ++i) f << "\n" << i;
 }
 
 void Mesh::print() const

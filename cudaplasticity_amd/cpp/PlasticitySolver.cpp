@@ -182,6 +182,7 @@ double PlasticitySolver::run(bool elastic, bool mixed)
     const cp_options o = abiOptions(mixed);
     const int s = elastic ? cp_solve_elastic(prob_, &o, &stats_) : cp_solve(prob_, &o, &stats_);
     ramSaved_ = false;
+    plastSolved_ = !elastic;
     stepStats_.assign(1, stats_);
     if (s != CP_OK && s != CP_ERR_NOT_CONVERGED) check(s, elastic ? "solveElast" : "solve");
     if (iterOutput || s == CP_ERR_NOT_CONVERGED)
@@ -215,6 +216,7 @@ double PlasticitySolver::solveSteps(int nsteps, double finalFactor)
     stepStats_.assign(nsteps, cp_stats{});
     const int s = cp_solve_steps(prob_, nsteps, finalFactor, &o, stepStats_.data());
     ramSaved_ = false;
+    plastSolved_ = true;
     stats_ = stepStats_.back();
     if (s != CP_OK && s != CP_ERR_NOT_CONVERGED) check(s, "solveSteps");
     if (iterOutput || s == CP_ERR_NOT_CONVERGED)
@@ -298,6 +300,75 @@ void PlasticitySolver::saveDisplsToFile(const std::string& fileName)
     for (int i = 0; i < mesh_.nodeCount; ++i)
         out << mesh_.node[i].x << " " << mesh_.node[i].y << " " << u_[2 * (size_t)i] << " " << u_[2 * (size_t)i + 1]
             << "\n";
+}
+
+void PlasticitySolver::saveAsVtk(const std::string& fileName)
+{
+    dataToRAM();
+    const int nn = mesh_.nodeCount, ne = mesh_.count4;
+    std::ofstream f(fileName);
+    if (!f) throw std::runtime_error("saveAsVtk: cannot open " + fileName);
+    f << "# vtk DataFile Version 2.0\nSolution\nASCII\nDATASET POLYDATA\nPOINTS " << nn << " float\n";
+    for (int i = 0; i < nn; ++i) f << mesh_.node[i].x << " " << mesh_.node[i].y << " " << 0 << "\n";
+    f << "POLYGONS " << ne << " " << 5 * ne;
+    for (int e = 0; e < ne; ++e) {
+        f << "\n4 ";
+        for (int j = 0; j < 4; ++j) f << mesh_.elem4[4 * (size_t)e + j] << " ";
+    }
+    f << "\nCELL_DATA " << ne << "\nSCALARS ElementID int 1\nLOOKUP_TABLE default\n";
+    for (int e = 0; e < ne; ++e) f << e << "\n";
+    f.precision(16);
+    f << "FIELD FieldData " << 8 + (plastSolved_ ? 7 : 0) + (mesh_.analysed ? 3 : 0);
+
+    auto column = [&](const char* name, auto value) {
+        f << "\n" << name << " 1 " << ne << " float\n";
+        for (int e = 0; e < ne; ++e) f << value(e) << " ";
+    };
+    // polar components of a symmetric tensor (x, y, xy) at the element centre
+    auto polar = [&](int e, double x, double y, double xy, int comp) {
+        const vec2 c = elemCenter(e);
+        const double r = c.norm(), co = c.x / r, si = c.y / r;
+        if (comp == 0) return co * co * x + si * si * y + 2. * co * si * xy;
+        if (comp == 1) return si * si * x + co * co * y - 2. * co * si * xy;
+        return co * si * (y - x) + xy * (co * co - si * si);
+    };
+    auto tensor = [&](const char* const (&names)[3], int fx, int fy, int fxy, double xy_scale) {
+        const std::vector<double>&X = fields_[fx], &Y = fields_[fy], &XY = fields_[fxy];
+        for (int comp = 0; comp < 3; ++comp) {
+            if (polarCoord)
+                column(names[comp], [&](int e) { return polar(e, X[e], Y[e], xy_scale * XY[e], comp); });
+            else
+                column(names[comp], [&](int e) { return (comp == 0 ? X : comp == 1 ? Y : XY)[e]; });
+        }
+    };
+    const std::vector<std::vector<double>>& F = fields_;
+    static const char* const strain_xy[3] = {"Eps_xx", "Eps_yy", "Gamma_xy"};
+    static const char* const strain_rp[3] = {"Eps_rr", "Eps_phiphi", "Gamma_rphi"};
+    static const char* const pstrain_xy[3] = {"Eps_p_xx", "Eps_p_yy", "Gamma_p_xy"};
+    static const char* const pstrain_rp[3] = {"Eps_p_rr", "Eps_p_phiphi", "Gamma_p_rphi"};
+    static const char* const stress_xy[3] = {"Sigma_xx", "Sigma_yy", "Tau_xy"};
+    static const char* const stress_rp[3] = {"Sigma_rr", "Sigma_phiphi", "Tau_rphi"};
+    // engineering shear strain: its tensor component is half of it
+    tensor(polarCoord ? strain_rp : strain_xy, CP_F_EXX, CP_F_EYY, CP_F_GXY, 0.5);
+    column("Eps_intensity", [&](int e) { return F[CP_F_EPSI][e]; });
+    if (plastSolved_) {
+        tensor(polarCoord ? pstrain_rp : pstrain_xy, CP_F_EXXP, CP_F_EYYP, CP_F_GXYP, 0.5);
+        column("Eps_p_intensity", [&](int e) { return F[CP_F_EPSIP][e]; });
+        column("Psi", [&](int e) { return F[CP_F_PSI][e]; });
+        column("E_c", [&](int e) { return F[CP_F_EC][e]; });
+        column("nu_c", [&](int e) { return F[CP_F_NUC][e]; });
+    }
+    tensor(polarCoord ? stress_rp : stress_xy, CP_F_SXX, CP_F_SYY, CP_F_TAU, 1.0);
+    column("Sigma_intensity", [&](int e) { return F[CP_F_SIGI][e]; });
+    if (mesh_.analysed) {
+        column("Element_space", [&](int e) { return mesh_.spaces[e]; });
+        column("Element_aspect_ratio", [&](int e) { return mesh_.aspects[e]; });
+        column("Element_skew_angle", [&](int e) { return mesh_.skewAngles[e]; });
+    }
+    f << "\nPOINT_DATA " << nn << "\nSCALARS NodeID int 1\nLOOKUP_TABLE my_table";
+    for (int i = 0; i < nn; ++i) f << "\n" << i;
+    f << "\nFIELD FieldData2 1\nDisplacement 3 " << nn << " float\n";
+    for (int i = 0; i < nn; ++i) f << u_[2 * (size_t)i] << " " << u_[2 * (size_t)i + 1] << " 0 ";
 }
 
 vec2 PlasticitySolver::elemCenter(int e) const

@@ -300,8 +300,9 @@ struct CgArgs {
     const V* b;
     const uchar2* mask;
     const typename V4<T>::type* minv;  // 2x2 block-Jacobi inverse (m00, m01, m11, 0) per node
-    CgScal* sc;
+    CgScal* sc;              // k_cg_iter_fused: two slots sc[0], sc[1]
     double* partials;
+    double* partials_alt;    // second partials buffer of k_cg_iter_fused
     int nn, nblocks;
 };
 
@@ -467,12 +468,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs<T> a)
 //   gamma/||b||^2 <= tol^2 (ConjGradCuda.cu:203-222).
 // w and s carry the Dirichlet mask, so r stays zero on fixed DOFs.
 template <int W, bool PRE, typename T>
-__device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
+__device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb, int it, T alpha, T beta, double* partials)
 {
     using V = typename V2<T>::type;
-    CgScal* sc = a.sc;
-    const int it = sc->iters;
-    const T alpha = (T)sc->alpha, beta = (T)sc->beta;
     const int par = it & 1;
     const V* rin = a.r[par];
     const V* win = a.w[par];
@@ -516,37 +514,52 @@ __device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
         v[1] = (double)wn.x * unx + (double)wn.y * uny;
         v[2] = (double)rn.x * rn.x + (double)rn.y * rn.y;
     }
-    block_partial<3>(v, a.partials, lb);
+    block_partial<3>(v, partials, lb);
 }
 
-// Scalars of iteration i+1 from the iteration partials: beta, alpha of the
-// Chronopoulos-Gear recurrence and the stop test on (r, r) / ||b||^2.
+template <int W, bool PRE, typename T>
+__device__ __forceinline__ void cg_iter_body(const CgArgs<T>& a, int lb)
+{
+    const CgScal* sc = a.sc;
+    cg_iter_body<W, PRE, T>(a, lb, sc->iters, (T)sc->alpha, (T)sc->beta, a.partials);
+}
+
+// Scalars of iteration i+1 from those of iteration i and the iteration's
+// totals v = {gamma, delta, rho}: beta, alpha of the Chronopoulos-Gear
+// recurrence and the stop test on (r, r) / ||b||^2.
+__device__ __forceinline__ void cg_next_scalars(CgScal& s, const double (&v)[3])
+{
+    const double g = v[0], d = v[1], rho = v[2];
+    const int iters = s.iters + 1;
+    s.iters = iters;
+    if (g != g || d != d || rho != rho) {
+        s.status = 2;
+        s.done = 1;
+        return;
+    }
+    const double bp = g / s.gamma;
+    const double ap = g / (d - bp * g / s.alpha);
+    s.gamma = g;
+    s.delta = d;
+    s.rho = rho;
+    s.beta = bp;
+    s.alpha = ap;
+    if (!(rho * (1. / s.bnorm2) > s.tol2)) {
+        s.done = 1;
+    } else if (iters >= s.maxit) {
+        s.done = 1;
+        s.status = 1;
+    }
+}
+
 __device__ __forceinline__ void cg_step_scalars(CgScal* sc, const double* partials, int nblocks)
 {
     double v[3];
     reduce_partials<3>(partials, nblocks, v);
     if (threadIdx.x == 0) {
-        const double g = v[0], d = v[1], rho = v[2];
-        const int iters = sc->iters + 1;
-        sc->iters = iters;
-        if (g != g || d != d || rho != rho) {
-            sc->status = 2;
-            sc->done = 1;
-            return;
-        }
-        const double bp = g / sc->gamma;
-        const double ap = g / (d - bp * g / sc->alpha);
-        sc->gamma = g;
-        sc->delta = d;
-        sc->rho = rho;
-        sc->beta = bp;
-        sc->alpha = ap;
-        if (!(rho * (1. / sc->bnorm2) > sc->tol2)) {
-            sc->done = 1;
-        } else if (iters >= sc->maxit) {
-            sc->done = 1;
-            sc->status = 1;
-        }
+        CgScal s = *sc;
+        cg_next_scalars(s, v);
+        *sc = s;
     }
 }
 
@@ -562,6 +575,55 @@ __global__ __launch_bounds__(kBlock) void k_cg_iter(CgArgs<T> a)
 {
     if (a.sc->done) return;
     cg_iter_body<W, PRE, T>(a, xcd_block(a.nblocks));
+}
+
+// Largest grid for which k_cg_iter_fused pays: every block re-reads all
+// 3 x nblocks partials of the previous iteration.
+constexpr int kFusedMaxBlocks = 2048;
+
+// Iteration k with the scalar step of iteration k-1 folded in, so that one
+// launch per iteration suffices.  Scalars live in two slots sc[0], sc[1]
+// and per-block partials in two buffers partials, partials_alt: launch k
+// reads slot (k-1)&1 and the partials of launch k-1, forms the scalars of
+// iteration k in every block (same fixed-order sum everywhere, so every
+// block gets the same bits), block 0 stores them in slot k&1, and the
+// iteration writes its partials to buffer k&1.  Launch 0 takes slot 0 from
+// k_cg_start.  A converged or stopped slot is copied forward unchanged, so
+// slot (L-1)&1 is current after L launches.
+// At least 4 waves per SIMD (at most 128 VGPRs): the FP64 form otherwise
+// takes 130 and a 784-block grid (the 632x316 plate) no longer fits on the
+// chip in one round.
+template <int W, bool PRE, typename T>
+__global__ __launch_bounds__(kBlock, 4) void k_cg_iter_fused(CgArgs<T> a, int k)
+{
+    __shared__ double s_ab[2];
+    __shared__ int s_state[2];
+    double* parts[2] = {a.partials, a.partials_alt};
+    if (k == 0) {
+        if (a.sc[0].done) return;
+        cg_iter_body<W, PRE, T>(a, xcd_block(a.nblocks), 0, (T)a.sc[0].alpha, (T)a.sc[0].beta, parts[0]);
+        return;
+    }
+    const CgScal* prev = a.sc + ((k - 1) & 1);
+    CgScal* cur = a.sc + (k & 1);
+    if (prev->done) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *cur = *prev;
+        return;
+    }
+    double v[3];
+    reduce_partials<3>(parts[(k - 1) & 1], a.nblocks, v);
+    if (threadIdx.x == 0) {
+        CgScal s = *prev;
+        cg_next_scalars(s, v);
+        if (blockIdx.x == 0) *cur = s;
+        s_ab[0] = s.alpha;
+        s_ab[1] = s.beta;
+        s_state[0] = s.done;
+        s_state[1] = s.iters;
+    }
+    __syncthreads();
+    if (s_state[0]) return;
+    cg_iter_body<W, PRE, T>(a, xcd_block(a.nblocks), s_state[1], (T)s_ab[0], (T)s_ab[1], parts[k & 1]);
 }
 
 // Batched form for the serving loop: one launch advances one CG iteration of

@@ -71,7 +71,8 @@ struct cp_problem {
     double* psi_prev = nullptr;
     int4* conn = nullptr;
     uchar2* mask = nullptr;
-    double *psi = nullptr, *F = nullptr, *G = nullptr, *rcur = nullptr, *rprev = nullptr, *partials = nullptr;
+    double *psi = nullptr, *F = nullptr, *G = nullptr, *rcur = nullptr, *rprev = nullptr, *partials = nullptr,
+           *partials_alt = nullptr;
     int *inc_ptr = nullptr, *col = nullptr;
     int2 *inc = nullptr, *slice = nullptr;
     cpk::CgScal *sc64 = nullptr, *sc32 = nullptr;

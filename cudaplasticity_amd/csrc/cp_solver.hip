@@ -57,7 +57,7 @@ Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int
                al(sizeof(double) * ne) * (5 + CP_NFIELDS) +   // psi, psi_prev, G, rcur, rprev, fields
                al(sizeof(int) * (nn + 1)) + al(sizeof(int2) * ninc) +
                al(sizeof(int) * nblk_stored) + al(sizeof(int2) * nslices) +
-               al(sizeof(double) * 5 * (size_t)nbmax) +
+               al(sizeof(double) * 5 * (size_t)nbmax) + al(sizeof(double) * 3 * (size_t)nbmax) +
                4 * 256;                                        // scalars
     L.fp64 = al(sizeof(double2) * nblk_stored) * 2 + al(sizeof(double2) * nn) * 7 + al(sizeof(double4) * nn);
     L.mixed = al(sizeof(float4) * nblk_stored) + al(sizeof(float2) * nn) * 9 + al(sizeof(double2) * nn) +
@@ -86,6 +86,7 @@ cpk::CgArgs<double> cp_problem::cg64_args() const
     a.minv = minv64;
     a.sc = sc64;
     a.partials = partials;
+    a.partials_alt = partials_alt;
     a.nn = nn;
     a.nblocks = nbn;
     return a;
@@ -110,6 +111,7 @@ cpk::CgArgs<float> cp_problem::cg32_args() const
     a.minv = minv32;
     a.sc = sc32;
     a.partials = partials;
+    a.partials_alt = partials_alt;
     a.nn = nn;
     a.nblocks = nbn;
     return a;
@@ -275,8 +277,9 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->col = A.take<int>(P->nblk_stored);
     P->slice = A.take<int2>(P->nslices);
     P->partials = A.take<double>(5 * (size_t)std::max(P->nbn, P->nbe));
-    P->sc64 = A.take<cpk::CgScal>(1);
-    P->sc32 = A.take<cpk::CgScal>(1);
+    P->partials_alt = A.take<double>(3 * (size_t)std::max(P->nbn, P->nbe));
+    P->sc64 = A.take<cpk::CgScal>(2);
+    P->sc32 = A.take<cpk::CgScal>(2);
     P->rms = A.take<cpk::RmScal>(1);
     P->refs = A.take<cpk::RefScal>(1);
 
@@ -312,8 +315,8 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     CP_HIP(hipMemcpyAsync(P->col, hcol.data(), sizeof(int) * off, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemcpyAsync(P->slice, slices.data(), sizeof(int2) * P->nslices, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemsetAsync(P->F, 0, sizeof(double) * (size_t)ne * CP_NFIELDS, st));
-    CP_HIP(hipMemsetAsync(P->sc64, 0, sizeof(cpk::CgScal), st));
-    CP_HIP(hipMemsetAsync(P->sc32, 0, sizeof(cpk::CgScal), st));
+    CP_HIP(hipMemsetAsync(P->sc64, 0, 2 * sizeof(cpk::CgScal), st));
+    CP_HIP(hipMemsetAsync(P->sc32, 0, 2 * sizeof(cpk::CgScal), st));
     CP_HIP(hipMemsetAsync(P->rms, 0, sizeof(cpk::RmScal), st));
     CP_HIP(hipMemsetAsync(P->refs, 0, sizeof(cpk::RefScal), st));
     const int nbe = P->nbe, nbn = P->nbn;
@@ -415,10 +418,42 @@ void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim
 template void launch_cg_iters<double>(int, int, const cpk::CgArgs<double>&, int, dim3, hipStream_t);
 template void launch_cg_iters<float>(int, int, const cpk::CgArgs<float>&, int, dim3, hipStream_t);
 
-// Iteration kernels only (no scalar step): times the kernel's own traffic.
+// One launch per iteration (k_cg_iter_fused), iterations k0 .. k0+count-1.
+template <int W, bool PRE, typename T>
+static void launch_fused_w(const cpk::CgArgs<T>& a, int k0, int count, dim3 g, hipStream_t st)
+{
+    for (int j = 0; j < count; ++j)
+        hipLaunchKernelGGL((cpk::k_cg_iter_fused<W, PRE, T>), g, dim3(cpk::kBlock), 0, st, a, k0 + j);
+}
+
+template <typename T>
+static void launch_fused(int max_w, int pre, const cpk::CgArgs<T>& a, int k0, int count, dim3 g, hipStream_t st)
+{
+    if (pre) {
+        if (max_w <= 9) launch_fused_w<9, true, T>(a, k0, count, g, st);
+        else if (max_w <= 12) launch_fused_w<12, true, T>(a, k0, count, g, st);
+        else if (max_w <= 16) launch_fused_w<16, true, T>(a, k0, count, g, st);
+        else launch_fused_w<0, true, T>(a, k0, count, g, st);
+    } else {
+        if (max_w <= 9) launch_fused_w<9, false, T>(a, k0, count, g, st);
+        else if (max_w <= 12) launch_fused_w<12, false, T>(a, k0, count, g, st);
+        else if (max_w <= 16) launch_fused_w<16, false, T>(a, k0, count, g, st);
+        else launch_fused_w<0, false, T>(a, k0, count, g, st);
+    }
+}
+
+static bool use_fused(const cp_problem* P) { return P->nbn <= cpk::kFusedMaxBlocks; }
+
+// The iteration kernel the solver uses, `count` times: the fused form (its
+// scalar step included) on grids up to kFusedMaxBlocks, else the iteration
+// kernel alone without its separate one-block step.
 template <typename T>
 static void launch_iter_kernels_only(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st)
 {
+    if ((int)g.x <= cpk::kFusedMaxBlocks) {
+        launch_fused<T>(max_w, pre, a, 0, count, g, st);
+        return;
+    }
     for (int j = 0; j < count; ++j) {
         if (max_w <= 9) {
             if (pre) hipLaunchKernelGGL((cpk::k_cg_iter<9, true, T>), g, dim3(cpk::kBlock), 0, st, a);
@@ -462,11 +497,15 @@ static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal
     int chunk = (int)std::min<long long>(4096, std::max<long long>(16, P->last_cg_iters / 2));
     double g_prev = -1.;
     long long it_prev = 0;
+    const bool fused = use_fused(P);
     for (;;) {
-        launch_cg_iters<T>(P->max_w, P->precond, a, chunk, g, st);
+        if (fused) launch_fused<T>(P->max_w, P->precond, a, (int)launched, chunk, g, st);
+        else launch_cg_iters<T>(P->max_w, P->precond, a, chunk, g, st);
         CP_HIP(hipGetLastError());
         launched += chunk;
-        CP_HIP(hipMemcpyAsync(P->h_cg, dsc, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
+        // fused launches keep the scalars in two slots; the last launch's is current
+        const cpk::CgScal* cur = fused ? dsc + ((launched - 1) & 1) : dsc;
+        CP_HIP(hipMemcpyAsync(P->h_cg, cur, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
         CP_HIP(hipStreamSynchronize(st));
         const cpk::CgScal h = *P->h_cg;
         if (h.done) break;
@@ -1201,6 +1240,12 @@ int cp_bench_cg(cp_problem* P, int precision, int precond, int reps, double* ms,
     float t = 0.f;
     CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
     if (ms) *ms = t / reps;
+    if (P->nbn <= cpk::kFusedMaxBlocks) {
+        // every timed launch must have iterated (tolerance 0: only a NaN stops)
+        const cpk::CgScal* last = (precision == CP_FP64 ? P->sc64 : P->sc32) + ((reps - 1) & 1);
+        CP_HIP(hipMemcpy(P->h_cg, last, sizeof(cpk::CgScal), hipMemcpyDeviceToHost));
+        if (P->h_cg->done) return cp::fail(CP_ERR_NAN, "cp_bench_cg: the iteration stopped early");
+    }
     // matrix once + 5 vector reads (r, w, s_prev, p, x) + 5 writes + mask
     // (+ the block-Jacobi inverse, 4 values per node, when preconditioned)
     if (bytes_algo)

@@ -70,6 +70,17 @@ public:
     // mean quad area
     double avrElemSize() const;
 
+    // Element quality arrays (area, aspect ratio, skew sine per quad) filled
+    // by printAnalysis (Mesh.cu:1032-1080), which also prints their ranges
+    // and sets `analysed`.
+    double* spaces = nullptr;
+    double* aspects = nullptr;
+    double* skewAngles = nullptr;
+    void printAnalysis();
+
+    // Legacy-VTK polydata of the mesh, the layout of Mesh.cu:939-989.
+    void saveAsVtk(const std::string& fileName) const;
+
     void meshToRAM() {}
     void meshToGPU() {}
     void print() const;

@@ -73,6 +73,12 @@ public:
     double getSxx0() const;
     double getExx0() const;
     void saveDisplsToFile(const std::string& fileName);
+    // Legacy-VTK solution file in the reference's layout
+    // (PlasticitySolver.cu:1784-1945): mesh, element ids, strain, plastic
+    // (after solve) and stress fields, Cartesian or, with polarCoord, polar
+    // components about the origin, the mesh quality fields when the mesh was
+    // analysed, node ids and displacements.
+    void saveAsVtk(const std::string& fileName);
     double sigmaError(std::function<double(vec2)> s_x, std::function<double(vec2)> s_y,
                       std::function<double(vec2)> s_xy);
     double polarSigmaError(std::function<double(vec2)> s_r, std::function<double(vec2)> s_phi,
@@ -102,6 +108,7 @@ private:
     std::vector<std::vector<double>> fields_;
     std::vector<double> u_;
     bool ramSaved_ = false;
+    bool plastSolved_ = false;
 
     void buildProblem();
     cp_options abiOptions(bool mixed) const;

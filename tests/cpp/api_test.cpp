@@ -2,8 +2,10 @@
 // way the reference's experiments drive its classes, and writes raw arrays
 // into OUTDIR for tests/test_cpp_api.py to check.
 //
-//   api_test cpu OUTDIR   meshes, material laws and load conditions (no device)
-//   api_test gpu OUTDIR   solves through PlasticitySolver on the GPU
+//   api_test cpu OUTDIR [INDIR]  meshes, material laws, load conditions and
+//                                (with INDIR/ring_points.f64, ring_polys.i32) the
+//                                mesh analysis and VTK writer; no device needed
+//   api_test gpu OUTDIR          solves through PlasticitySolver on the GPU
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -15,7 +17,7 @@
 
 namespace {
 
-std::string outdir;
+std::string outdir, indir;
 
 template <typename T>
 void dump(const std::string& name, const T* data, size_t n)
@@ -186,6 +188,31 @@ int run_cpu()
         std::vector<int> r = {refused};
         dump("refused.i32", r);
     }
+    if (!indir.empty()) {
+        // a mesh given through the public arrays (the reference's data/ring.vtk
+        // points and quads): printAnalysis and the mesh VTK writer
+        std::ifstream fp(indir + "/ring_points.f64", std::ios::binary), fq(indir + "/ring_polys.i32", std::ios::binary);
+        fp.seekg(0, std::ios::end);
+        const size_t np = (size_t)fp.tellg() / (2 * sizeof(double));
+        fp.seekg(0);
+        fq.seekg(0, std::ios::end);
+        const size_t nq = (size_t)fq.tellg() / (4 * sizeof(int));
+        fq.seekg(0);
+        Mesh mesh;
+        mesh.nodeCount = (int)np;
+        mesh.node = new vec2[np];
+        for (size_t i = 0; i < np; ++i) {
+            double p[2];
+            fp.read(reinterpret_cast<char*>(p), sizeof(p));
+            mesh.node[i] = vec2(p[0], p[1]);
+        }
+        mesh.count4 = (int)nq;
+        mesh.elem4 = new int[4 * nq];
+        fq.read(reinterpret_cast<char*>(mesh.elem4), 4 * nq * sizeof(int));
+        mesh.elemPos[1] = mesh.elemPos[2] = (int)nq;
+        mesh.printAnalysis();
+        mesh.saveAsVtk(outdir + "/ring_mesh.vtk");
+    }
     std::cout << "cpu cases written to " << outdir << "\n";
     return 0;
 }
@@ -209,6 +236,25 @@ int run_gpu()
                                  ps.getSxx0(), ps.getExx0(), t};
         dump("plast_stats.f64", s);
         ps.saveDisplsToFile(outdir + "/plast_displs.txt");
+        ps.saveAsVtk(outdir + "/holePlatePlastPow.vtk");
+        ps.polarCoord = true;
+        ps.saveAsVtk(outdir + "/holePlatePlastPow_polar.vtk");
+    }
+    {
+        // solverTest (Main.cpp:14-38): elastic 4x4 block, end traction, mid fixings
+        Mesh mesh;
+        mesh.genRectangle(-1., 1., -1., 1., 4, 4);
+        LoadConditions lc;
+        lc.setDisplacement(2, vec2(0, 0.), false, true);
+        lc.setForce(0, vec2(1, 0));
+        Material m;
+        m.setLinearPlast(E_304L * 1e-6, S_T_304L, E_304L * 1e-6);
+        m.E = E_304L * 1e-6;
+        m.nu = NU_304L * 1e-6;
+        m.h = 1.;
+        PlasticitySolver ps(m, mesh, lc);
+        ps.solveElast();
+        ps.saveAsVtk(outdir + "/solverTest.vtk");
     }
     {
         // config-1 style elastic solve with the Kirsch check of Main.cpp:65-68
@@ -275,11 +321,12 @@ int run_gpu()
 
 int main(int argc, char** argv)
 {
-    if (argc != 3) {
-        std::cerr << "usage: api_test cpu|gpu OUTDIR\n";
+    if (argc != 3 && argc != 4) {
+        std::cerr << "usage: api_test cpu|gpu OUTDIR [INDIR]\n";
         return 2;
     }
     outdir = argv[2];
+    if (argc == 4) indir = argv[3];
     const std::string mode = argv[1];
     try {
         if (mode == "cpu") return run_cpu();

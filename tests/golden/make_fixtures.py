@@ -10,8 +10,11 @@ Inputs (reference data/, ASCII outputs of the reference's own experiments):
                          per-element space / aspect ratio / skew angle
   diagramLin.txt / diagramPow.txt  Main.cpp:929-994  uniaxial sigma(eps)
 Outputs: tests/golden/*.npz (data only: node coordinates, connectivity,
-per-element fields and nodal displacements exactly as printed).
+per-element fields and nodal displacements exactly as printed) and
+tests/golden/vtk_layout.json (the section header lines of the three VTK
+files, in order).
 """
+import json
 import os
 import sys
 
@@ -60,6 +63,16 @@ def main(ref_root):
             arrays["f_" + k] = a
         np.savez_compressed(os.path.join(HERE, name + ".npz"), **arrays)
         print(name, {k: a.shape for k, a in arrays.items()})
+    # the section layout (every non-numeric line, in order) of the reference's
+    # VTK writers: PlasticitySolver::saveAsVtk after solveElast / solve, and
+    # Mesh::saveAsVtk after printAnalysis
+    layout = {}
+    for name in ("solverTest", "holePlatePlastPow", "ring"):
+        with open(os.path.join(data, name + ".vtk")) as f:
+            layout[name] = [ln.rstrip("\n") for ln in f if ln[:1].isalpha() or ln.startswith("#")]
+    with open(os.path.join(HERE, "vtk_layout.json"), "w") as f:
+        json.dump(layout, f, indent=0)
+    print("vtk_layout", {k: len(v) for k, v in layout.items()})
     lin = np.loadtxt(os.path.join(data, "diagramLin.txt"))
     pw = np.loadtxt(os.path.join(data, "diagramPow.txt"))
     np.savez_compressed(os.path.join(HERE, "diagram.npz"), lin=lin, pow=pw)

@@ -31,11 +31,72 @@ def rd(d, name):
     return np.fromfile(os.path.join(d, name), dtype=kind)
 
 
+def run_in(mode, outdir, indir):
+    assert os.path.exists(BIN), "tests/cpp/_build/api_test is not built: run __graft_entry__.build()"
+    r = subprocess.run([BIN, mode, str(outdir), str(indir)], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-3000:], flush=True)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.fixture(scope="module")
 def cpu_out(tmp_path_factory):
     d = tmp_path_factory.mktemp("api_cpu")
-    run("cpu", d)
+    src = tmp_path_factory.mktemp("api_in")
+    ring = np.load(os.path.join(GOLD, "ring.npz"))
+    np.ascontiguousarray(ring["points"], dtype=np.float64).tofile(os.path.join(src, "ring_points.f64"))
+    np.ascontiguousarray(ring["polys"], dtype=np.int32).tofile(os.path.join(src, "ring_polys.i32"))
+    run_in("cpu", d, src)
     return d
+
+
+def read_vtk(path):
+    """Section header lines (every non-numeric line, in order) and the
+    numeric blocks of a legacy VTK polydata file."""
+    layout, fields, points, polys = [], {}, None, None
+    with open(path) as f:
+        lines = f.read().split("\n")
+    i = 0
+    while i < len(lines):
+        ln = lines[i]
+        if ln[:1].isalpha() or ln.startswith("#"):
+            layout.append(ln)
+            parts = ln.split()
+            if parts[0] == "POINTS":
+                n = int(parts[1])
+                points = np.array([list(map(float, lines[i + 1 + k].split()))[:2] for k in range(n)])
+                i += n + 1
+                continue
+            if parts[0] == "POLYGONS":
+                n = int(parts[1])
+                polys = np.array([list(map(int, lines[i + 1 + k].split()))[1:] for k in range(n)])
+                i += n + 1
+                continue
+            if len(parts) == 4 and parts[3] == "float":
+                fields[parts[0]] = np.array(list(map(float, lines[i + 1].split())))
+                i += 2
+                continue
+        i += 1
+    return layout, fields, points, polys
+
+
+def layouts():
+    import json
+    with open(os.path.join(GOLD, "vtk_layout.json")) as f:
+        return json.load(f)
+
+
+def test_mesh_analysis_and_vtk_writer(cpu_out):
+    """Mesh::printAnalysis + Mesh::saveAsVtk (Mesh.cu:939-989, 1032-1080) on
+    the points and quads of the reference's data/ring.vtk: the same section
+    layout, and the quality columns the reference printed (6 digits)."""
+    fx = np.load(os.path.join(GOLD, "ring.npz"))
+    lay, F, pts, polys = read_vtk(os.path.join(cpu_out, "ring_mesh.vtk"))
+    assert lay == layouts()["ring"]
+    assert np.array_equal(polys, fx["polys"])
+    assert np.abs(pts - fx["points"]).max() < 1e-6
+    np.testing.assert_allclose(F["Element_space"], fx["f_Element_space"], rtol=1e-4)
+    np.testing.assert_allclose(F["Element_aspect_ratio"], fx["f_Element_aspect_ratio"], rtol=1e-4)
+    np.testing.assert_allclose(F["Element_skew_angle"], fx["f_Element_skew_angle"], atol=1e-4)
 
 
 def check_mesh(d, tag, om):
@@ -179,6 +240,41 @@ def test_plasticity_test_through_cpp_api(tmp_path):
     assert abs(st[3] - O.deform_energy(om, mat, F)) <= 1e-12 * abs(st[3])
     assert st[4] == F[O.FIELDS.index("sxx"), 0] and st[5] == F[O.FIELDS.index("exx"), 0]
     assert os.path.getsize(os.path.join(tmp_path, "plast_displs.txt")) > 0
+
+    # saveAsVtk (PlasticitySolver.cu:1784-1945): same layout as the reference's files
+    lay, VF, pts, polys = read_vtk(os.path.join(tmp_path, "holePlatePlastPow.vtk"))
+    assert lay == layouts()["holePlatePlastPow"]
+    assert np.array_equal(polys, fx["polys"]) and np.abs(pts - fx["points"]).max() < 2e-5
+    for name in ("Eps_xx", "Gamma_xy", "Eps_p_intensity", "Psi", "E_c", "nu_c", "Sigma_xx", "Tau_xy",
+                 "Sigma_intensity"):
+        assert rel(VF[name], fx["f_" + name]) < 2e-7, name
+    assert rel(VF["Displacement"].reshape(-1, 3)[:, :2].ravel(), fx["f_Displacement"][:, :2].ravel()) < 2e-7
+    play, PF, _, _ = read_vtk(os.path.join(tmp_path, "holePlatePlastPow_polar.vtk"))
+    rename = {"Eps_xx": "Eps_rr", "Eps_yy": "Eps_phiphi", "Gamma_xy": "Gamma_rphi", "Eps_p_xx": "Eps_p_rr",
+              "Eps_p_yy": "Eps_p_phiphi", "Gamma_p_xy": "Gamma_p_rphi", "Sigma_xx": "Sigma_rr",
+              "Sigma_yy": "Sigma_phiphi", "Tau_xy": "Tau_rphi"}
+    want = [" ".join([rename.get(w.split()[0], w.split()[0])] + w.split()[1:]) for w in lay]
+    assert play == want
+    P = np.stack([om.x[om.conn], om.y[om.conn]], axis=-1).sum(1) / 4
+    c, s_ = P[:, 0] / np.hypot(P[:, 0], P[:, 1]), P[:, 1] / np.hypot(P[:, 0], P[:, 1])
+    sx, sy, txy = VF["Sigma_xx"], VF["Sigma_yy"], VF["Tau_xy"]
+    np.testing.assert_allclose(PF["Sigma_rr"], c * c * sx + s_ * s_ * sy + 2 * c * s_ * txy, rtol=1e-12, atol=1e-9)
+    np.testing.assert_allclose(PF["Tau_rphi"], c * s_ * (sy - sx) + txy * (c * c - s_ * s_), rtol=1e-12, atol=1e-9)
+    ex, ey, g = VF["Eps_xx"], VF["Eps_yy"], VF["Gamma_xy"]
+    np.testing.assert_allclose(PF["Gamma_rphi"], c * s_ * (ey - ex) + 0.5 * g * (c * c - s_ * s_), rtol=1e-12,
+                               atol=1e-15)
+
+    # solverTest (Main.cpp:14-38) through solveElast + saveAsVtk against data/solverTest.vtk
+    st_fx = np.load(os.path.join(GOLD, "solverTest.npz"))
+    lay, SF, pts, polys = read_vtk(os.path.join(tmp_path, "solverTest.vtk"))
+    assert lay == layouts()["solverTest"]
+    assert np.array_equal(polys, st_fx["polys"]) and np.array_equal(pts, st_fx["points"])
+    for name in ("Eps_xx", "Sigma_xx", "Sigma_intensity"):
+        assert rel(SF[name], st_fx["f_" + name]) < 1e-9, name
+    # nu = 2.9e-7 here: the lateral strain is 1e-6 of the axial one, so it is
+    # measured against the axial strain's size
+    assert np.linalg.norm(SF["Eps_yy"] - st_fx["f_Eps_yy"]) < 1e-9 * np.linalg.norm(st_fx["f_Eps_xx"])
+    assert np.abs(SF["Sigma_xx"] - 1.0).max() < 1e-9
 
     # config-1 elastic
     qm = O.gen_quarter_plate(1.0, 0.1, 100, 49)
