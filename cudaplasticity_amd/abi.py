@@ -291,6 +291,20 @@ class Problem:
         check(lib().cp_problem_layout(self.ptr, C.byref(a), C.byref(b), C.byref(w)))
         return {"nblk_stored": a.value, "nblk": b.value, "max_width": w.value}
 
+    def set_grid(self, n1, n2, node_of=None):
+        """Logical lattice of a structured mesh (for precond=2, multigrid)."""
+        no = None if node_of is None else np.ascontiguousarray(node_of, dtype=np.int32)
+        check(lib().cp_problem_set_grid(self.ptr, int(n1), int(n2), ip(no) if no is not None else None))
+
+    def set_graphs(self, enable):
+        check(lib().cp_problem_set_graphs(self.ptr, int(enable)))
+
+    def mg_levels(self):
+        n = C.c_int()
+        dims = (C.c_int * 128)()
+        check(lib().cp_problem_mg_levels(self.ptr, C.byref(n), dims))
+        return [(dims[2 * i], dims[2 * i + 1]) for i in range(n.value)]
+
     def set_load(self, factor):
         check(lib().cp_set_load(self.ptr, _f(factor)))
 

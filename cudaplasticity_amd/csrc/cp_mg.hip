@@ -1,0 +1,528 @@
+// cp_mg.hip -- host side of the geometric multigrid preconditioner
+// (precond = 2): lattice set-up, level hierarchy, set-up after each
+// assembly, the V-cycle and CG preconditioned by it.
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+#include <vector>
+
+#include "cp_mg.h"
+#include "cp_problem.h"
+
+namespace cp {
+
+struct MgLevel {
+    int n1 = 0, n2 = 0, nn = 0, nb = 0, nslices = 0;
+    int2* slice = nullptr;   // levels >= 1 (level 0 uses the problem's matrix)
+    int* col = nullptr;
+    uchar2* mask = nullptr;
+    double2 *va = nullptr, *vb = nullptr;
+    float4* vf = nullptr;
+    double4* minv64 = nullptr;
+    float4* minv32 = nullptr;
+    double* lam = nullptr;   // [0] FP64 operator, [1] FP32 operator
+    void* vec[2][6] = {};    // per precision: b, xa, xb, da, db, res
+};
+
+struct MgState {
+    int nlev = 0;
+    std::vector<MgLevel> L;
+    int* lg0 = nullptr;      // node -> lattice index of level 0 (nullptr: identity)
+    int* node0 = nullptr;    // lattice index -> node of level 0 (nullptr: identity)
+    double* dense = nullptr; // coarsest operator, inverted in place
+    int nd = 0;
+    bool alloc[2] = {false, false};
+    std::vector<void*> blocks;
+    // one CG iteration (update, V-cycle, dots, step) captured per precision
+    hipGraphExec_t iter[2] = {nullptr, nullptr};
+    bool use_graphs = true;
+
+    template <typename X>
+    int get(X** p, size_t count)
+    {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(X)) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(CP_ERR_NOMEM, "multigrid: device allocation failed");
+        }
+        blocks.push_back(q);
+        *p = reinterpret_cast<X*>(q);
+        return CP_OK;
+    }
+    ~MgState()
+    {
+        for (hipGraphExec_t& e : iter)
+            if (e) (void)hipGraphExecDestroy(e);
+        for (void* q : blocks) (void)hipFree(q);
+    }
+};
+
+void mg_destroy(cp_problem* P)
+{
+    delete P->mg;
+    P->mg = nullptr;
+}
+
+static int blocks_of(int n) { return (n + cpk::kBlock - 1) / cpk::kBlock; }
+
+// Per-precision value and vector arrays, allocated on first use.
+static int mg_alloc(cp_problem* P, int prec)
+{
+    MgState& G = *P->mg;
+    const int pi = prec == CP_FP64 ? 0 : 1;
+    if (G.alloc[pi]) return CP_OK;
+    for (int l = 0; l < G.nlev; ++l) {
+        MgLevel& L = G.L[l];
+        const size_t stored = (size_t)L.nslices * cpk::kWave * 9;
+        if (l > 0) {
+            if (prec == CP_FP64) {
+                if (int s = G.get(&L.va, stored)) return s;
+                if (int s = G.get(&L.vb, stored)) return s;
+                if (int s = G.get(&L.minv64, L.nn)) return s;
+            } else {
+                if (int s = G.get(&L.vf, stored)) return s;
+                if (int s = G.get(&L.minv32, L.nn)) return s;
+            }
+        }
+        const size_t vb = prec == CP_FP64 ? sizeof(double2) : sizeof(float2);
+        for (int k = 0; k < 6; ++k) {
+            if (l == 0 && k == 0) continue;   // level 0's right-hand side is the CG residual
+            char* q = nullptr;
+            if (int s = G.get(&q, vb * L.nn)) return s;
+            L.vec[pi][k] = q;
+        }
+    }
+    G.alloc[pi] = true;
+    return CP_OK;
+}
+
+template <typename T>
+static cpk::CgArgs<T> level_args(cp_problem* P, int l);
+
+template <>
+cpk::CgArgs<double> level_args<double>(cp_problem* P, int l)
+{
+    if (l == 0) return P->cg64_args();
+    const MgLevel& L = P->mg->L[l];
+    cpk::CgArgs<double> a{};
+    a.col = L.col;
+    a.slice = L.slice;
+    a.va = L.va;
+    a.vb = L.vb;
+    a.mask = L.mask;
+    a.minv = L.minv64;
+    a.nn = L.nn;
+    a.nblocks = L.nb;
+    return a;
+}
+
+template <>
+cpk::CgArgs<float> level_args<float>(cp_problem* P, int l)
+{
+    if (l == 0) return P->cg32_args();
+    const MgLevel& L = P->mg->L[l];
+    cpk::CgArgs<float> a{};
+    a.col = L.col;
+    a.slice = L.slice;
+    a.va = L.vf;
+    a.vb = nullptr;
+    a.mask = L.mask;
+    a.minv = L.minv32;
+    a.nn = L.nn;
+    a.nblocks = L.nb;
+    return a;
+}
+
+template <typename T>
+static typename cpk::V4<T>::type* level_minv(cp_problem* P, int l);
+template <>
+double4* level_minv<double>(cp_problem* P, int l)
+{
+    return l == 0 ? P->minv64 : P->mg->L[l].minv64;
+}
+template <>
+float4* level_minv<float>(cp_problem* P, int l)
+{
+    return l == 0 ? P->minv32 : P->mg->L[l].minv32;
+}
+
+template <typename T>
+static int mg_setup_t(cp_problem* P)
+{
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    hipStream_t st = P->stream;
+    for (int l = 0; l < G.nlev; ++l) {
+        const MgLevel& L = G.L[l];
+        cpk::CgArgs<T> a = level_args<T>(P, l);
+        if (l > 0) {
+            const MgLevel& F = G.L[l - 1];
+            cpk::GalArgs<T, T> g{};
+            g.f = level_args<T>(P, l - 1);
+            g.lg = l == 1 ? G.lg0 : nullptr;
+            g.node_of = l == 1 ? G.node0 : nullptr;
+            g.n1f = F.n1;
+            g.n2f = F.n2;
+            g.n1c = L.n1;
+            g.n2c = L.n2;
+            g.nnc = L.nn;
+            g.nbc = L.nb;
+            g.slice_c = L.slice;
+            if (pi == 0) {
+                g.va_c = L.va;
+                g.vb_c = L.vb;
+            } else {
+                g.vf_c = L.vf;
+            }
+            g.mask_c = L.mask;
+            hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(L.nb), dim3(cpk::kBlock), 0, st, g);
+        }
+        if (l < G.nlev - 1) {
+            hipLaunchKernelGGL(cpk::k_mg_smooth_setup<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a,
+                               level_minv<T>(P, l), P->partials);
+            hipLaunchKernelGGL(cpk::k_mg_lam, dim3(1), dim3(cpk::kBlock), 0, st, (const double*)P->partials, L.nb,
+                               L.lam + pi);
+        } else {
+            hipLaunchKernelGGL(cpk::k_mg_dense<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a, G.dense, G.nd);
+            hipLaunchKernelGGL(cpk::k_mg_invert, dim3(1), dim3(1024), 0, st, G.dense, G.nd);
+        }
+    }
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int mg_setup(cp_problem* P, int precision)
+{
+    if (!P->mg) return fail(CP_ERR_STATE, "multigrid needs the mesh lattice: call cp_problem_set_grid first");
+    if (int s = mg_alloc(P, precision)) return s;
+    return precision == CP_FP64 ? mg_setup_t<double>(P) : mg_setup_t<float>(P);
+}
+
+template <int W, int MODE, typename T>
+static void smooth(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi, const void* di, void* xo,
+                   void* dout, int second, int last, hipStream_t st)
+{
+    using V = typename cpk::V2<T>::type;
+    hipLaunchKernelGGL((cpk::k_mg_smooth<W, MODE, T>), dim3(a.nblocks), dim3(cpk::kBlock), 0, st, a, v, (const V*)xi,
+                       (const V*)di, (V*)xo, (V*)dout, second, last);
+}
+
+template <int W, typename T>
+static void smooth_level_down(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, hipStream_t st)
+{
+    smooth<W, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
+    smooth<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
+    smooth<W, cpk::MG_RESID, T>(a, v, v.xb, v.db, v.xa, nullptr, 0, 0, st);
+}
+
+template <int W, typename T>
+static void smooth_level_up(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, hipStream_t st)
+{
+    smooth<W, cpk::MG_POST1, T>(a, v, v.xb, nullptr, nullptr, v.da, 0, 0, st);
+    smooth<W, cpk::MG_CHEB, T>(a, v, v.xb, v.da, v.xa, v.db, 0, 0, st);
+    smooth<W, cpk::MG_CHEB, T>(a, v, v.xa, v.db, v.xb, nullptr, 1, 1, st);
+}
+
+template <typename T, typename F>
+static void by_width(int w, const F& f)
+{
+    if (w <= 9) f(std::integral_constant<int, 9>());
+    else if (w <= 12) f(std::integral_constant<int, 12>());
+    else if (w <= 16) f(std::integral_constant<int, 16>());
+    else f(std::integral_constant<int, 0>());
+}
+
+// u = M r: one V-cycle; the result lands in level 0's xb.
+template <typename T>
+static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const cpk::CgScal* sc)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    hipStream_t st = P->stream;
+    auto vec = [&](int l) {
+        MgLevel& L = G.L[l];
+        cpk::MgVec<T> v{};
+        v.b = l == 0 ? rhs : (const V*)L.vec[pi][0];
+        v.xa = (V*)L.vec[pi][1];
+        v.xb = (V*)L.vec[pi][2];
+        v.da = (V*)L.vec[pi][3];
+        v.db = (V*)L.vec[pi][4];
+        v.res = (V*)L.vec[pi][5];
+        v.lam = L.lam + pi;
+        v.sc = sc;
+        return v;
+    };
+    auto grid = [&](int l) {
+        cpk::MgGrid g{};
+        g.lg = l == 0 ? G.lg0 : nullptr;
+        g.node_of = l == 0 ? G.node0 : nullptr;
+        g.n1f = G.L[l].n1;
+        g.n2f = G.L[l].n2;
+        g.n1c = G.L[l + 1].n1;
+        g.n2c = G.L[l + 1].n2;
+        return g;
+    };
+    const int last = G.nlev - 1;
+    for (int l = 0; l < last; ++l) {
+        const cpk::CgArgs<T> a = level_args<T>(P, l);
+        const cpk::MgVec<T> v = vec(l);
+        by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
+        const MgLevel& C = G.L[l + 1];
+        hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(C.nb), dim3(cpk::kBlock), 0, st, grid(l), (const V*)v.res,
+                           (V*)C.vec[pi][0], (const uchar2*)C.mask, C.nn, C.nb, sc);
+    }
+    {
+        const MgLevel& C = G.L[last];
+        hipLaunchKernelGGL(cpk::k_mg_coarse_solve<T>, dim3(1), dim3(cpk::kBlock), 0, st, (const double*)G.dense, G.nd,
+                           (const V*)C.vec[pi][0], (V*)C.vec[pi][2], sc);
+    }
+    for (int l = last - 1; l >= 0; --l) {
+        const cpk::CgArgs<T> a = level_args<T>(P, l);
+        const cpk::MgVec<T> v = vec(l);
+        const MgLevel& L = G.L[l];
+        const MgLevel& C = G.L[l + 1];
+        hipLaunchKernelGGL(cpk::k_mg_prolong<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, grid(l),
+                           (const V*)C.vec[pi][2], (const V*)v.xa, v.xb, (const uchar2*)a.mask, L.nn, L.nb, sc);
+        by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
+    }
+}
+
+// One iteration of CG preconditioned by the V-cycle: p, s, x, r updates,
+// u = M r, w = A u with the three dot products, scalars of the next step.
+template <typename T>
+static void mg_cg_iteration(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc,
+                            const typename cpk::V2<T>::type* u)
+{
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    hipLaunchKernelGGL(cpk::k_mg_cg_update<T>, g, b, 0, st, a, u);
+    mg_vcycle<T>(P, a.r[0], dsc);
+    by_width<T>(P->max_w, [&](auto w) {
+        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
+    });
+    hipLaunchKernelGGL(cpk::k_cg_step<T>, dim3(1), b, 0, st, a);
+}
+
+// The iteration above recorded once as a HIP graph (all its arguments are
+// fixed device addresses, the scalars live on the device) and replayed:
+// about 50 small launches per iteration cost more on the host than on the
+// device otherwise.
+template <typename T>
+static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, const typename cpk::V2<T>::type* u,
+                    hipGraphExec_t* out)
+{
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    if (!G.iter[pi]) {
+        hipGraph_t graph = nullptr;
+        CP_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+        mg_cg_iteration<T>(P, a, dsc, u);
+        const hipError_t e = hipStreamEndCapture(P->stream, &graph);
+        if (e != hipSuccess) return failf(CP_ERR_DEVICE, "multigrid: graph capture failed: %s", hipGetErrorString(e));
+        const hipError_t e2 = hipGraphInstantiate(&G.iter[pi], graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e2 != hipSuccess) {
+            G.iter[pi] = nullptr;
+            return failf(CP_ERR_DEVICE, "multigrid: graph instantiation failed: %s", hipGetErrorString(e2));
+        }
+    }
+    *out = G.iter[pi];
+    return CP_OK;
+}
+
+// CG preconditioned by the V-cycle (Chronopoulos-Gear form, one reduction
+// per iteration): the caller has launched the scalar set-up and r_0.
+template <typename T>
+int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
+              long long* iters, int* status, double* ms)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    hipStream_t st = P->stream;
+    const V* u = (const V*)G.L[0].vec[pi][2];
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    CP_HIP(hipEventRecord(P->ev[6], st));
+    hipLaunchKernelGGL(cpk::k_mg_cg_clear<T>, g, b, 0, st, a);
+    mg_vcycle<T>(P, a.r[0], dsc);
+    by_width<T>(P->max_w, [&](auto w) {
+        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
+    });
+    hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
+    CP_HIP(hipGetLastError());
+    long long launched = 0;
+    int chunk = (int)std::min<long long>(64, std::max<long long>(4, P->last_cg_iters / 2));
+    double g_prev = -1.;
+    long long it_prev = 0;
+    hipGraphExec_t exec = nullptr;
+    if (G.use_graphs)
+        if (int s = mg_graph<T>(P, a, dsc, u, &exec)) return s;
+    for (;;) {
+        for (int j = 0; j < chunk; ++j) {
+            if (exec) CP_HIP(hipGraphLaunch(exec, st));
+            else mg_cg_iteration<T>(P, a, dsc, u);
+        }
+        CP_HIP(hipGetLastError());
+        launched += chunk;
+        CP_HIP(hipMemcpyAsync(P->h_cg, dsc, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
+        CP_HIP(hipStreamSynchronize(st));
+        const cpk::CgScal h = *P->h_cg;
+        if (h.done) break;
+        const double target = tol2 * (bnorm2 > 0. ? bnorm2 : h.bnorm2);
+        int next = std::min(64, chunk * 2);
+        if (g_prev > 0. && h.rho > 0. && h.rho < g_prev && target > 0.) {
+            const double rate = std::log(g_prev / h.rho) / double(h.iters - it_prev);
+            const double rem = std::log(h.rho / target) / rate;
+            if (rem > 0 && rem < 1e7) next = (int)std::min(64., std::max(2., rem * 1.05 + 2.));
+        }
+        g_prev = h.rho;
+        it_prev = h.iters;
+        chunk = next;
+        if (launched > (long long)h.maxit + 256) break;
+    }
+    CP_HIP(hipEventRecord(P->ev[7], st));
+    CP_HIP(hipEventSynchronize(P->ev[7]));
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[6], P->ev[7]));
+    *ms = t;
+    *iters = P->h_cg->iters;
+    *status = P->h_cg->status;
+    return CP_OK;
+}
+template int run_mg_cg<double>(cp_problem*, const cpk::CgArgs<double>&, cpk::CgScal*, double, double, long long*,
+                               int*, double*);
+template int run_mg_cg<float>(cp_problem*, const cpk::CgArgs<float>&, cpk::CgScal*, double, double, long long*,
+                              int*, double*);
+
+}  // namespace cp
+
+extern "C" {
+
+int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
+{
+    using namespace cp;
+    if (!P || n1 < 2 || n2 < 2) return fail(CP_ERR_INVALID, "cp_problem_set_grid: bad arguments");
+    if ((long long)n1 * n2 != P->nn)
+        return failf(CP_ERR_INVALID, "cp_problem_set_grid: %d x %d lattice for %d nodes", n1, n2, P->nn);
+    const int nn = P->nn;
+    std::vector<int> lg(nn, -1);
+    if (node_of) {
+        for (int l = 0; l < nn; ++l) {
+            const int n = node_of[l];
+            if (n < 0 || n >= nn || lg[n] >= 0) return fail(CP_ERR_INVALID, "cp_problem_set_grid: node_of is not a permutation");
+            lg[n] = l;
+        }
+    } else {
+        for (int n = 0; n < nn; ++n) lg[n] = n;
+    }
+    // every coupling of the matrix must stay within one lattice step
+    for (int n = 0; n < nn; ++n) {
+        const int a1 = lg[n] / n2, a2 = lg[n] % n2;
+        for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
+            const int m = P->adj_node[q];
+            const int b1 = lg[m] / n2, b2 = lg[m] % n2;
+            if (std::abs(a1 - b1) > 1 || std::abs(a2 - b2) > 1)
+                return failf(CP_ERR_INVALID, "cp_problem_set_grid: nodes %d and %d share an element but are not lattice neighbours",
+                             n, m);
+        }
+    }
+    CP_HIP(hipStreamSynchronize(P->stream));
+    mg_destroy(P);
+    auto* G = new MgState();
+    P->mg = G;
+    // levels: halve both directions until the coarsest has at most kMgDenseMax DOFs
+    int c1 = n1, c2 = n2;
+    for (;;) {
+        MgLevel L;
+        L.n1 = c1;
+        L.n2 = c2;
+        L.nn = c1 * c2;
+        L.nb = blocks_of(L.nn);
+        L.nslices = (L.nn + cpk::kWave - 1) / cpk::kWave;
+        G->L.push_back(L);
+        if (2 * c1 * c2 <= cpk::kMgDenseMax) break;
+        const int d1 = cpk::mg_coarse_n(c1), d2 = cpk::mg_coarse_n(c2);
+        if (d1 == c1 && d2 == c2) break;
+        c1 = d1;
+        c2 = d2;
+    }
+    G->nlev = (int)G->L.size();
+    if (G->nlev < 2 || 2 * G->L.back().nn > cpk::kMgDenseMax) {
+        mg_destroy(P);
+        return fail(CP_ERR_INVALID, "cp_problem_set_grid: lattice too small or too thin for multigrid");
+    }
+    int s = CP_OK;
+    for (int l = 0; l < G->nlev && !s; ++l) {
+        MgLevel& L = G->L[l];
+        if ((s = G->get(&L.lam, 2))) break;
+        if (l == 0) continue;
+        if ((s = G->get(&L.mask, L.nn))) break;
+        if ((s = G->get(&L.slice, L.nslices))) break;
+        if ((s = G->get(&L.col, (size_t)L.nslices * cpk::kWave * 9))) break;
+        std::vector<int2> hs(L.nslices);
+        std::vector<int> hc((size_t)L.nslices * cpk::kWave * 9, 0);
+        for (int sl = 0; sl < L.nslices; ++sl) {
+            hs[sl] = make_int2(sl * cpk::kWave * 9, 9);
+            for (int lane = 0; lane < cpk::kWave; ++lane) {
+                const int C = sl * cpk::kWave + lane;
+                if (C >= L.nn) continue;
+                const int K1 = C / L.n2, K2 = C % L.n2;
+                for (int d1 = -1; d1 <= 1; ++d1)
+                    for (int d2 = -1; d2 <= 1; ++d2) {
+                        const int D1 = K1 + d1, D2 = K2 + d2;
+                        const bool in = D1 >= 0 && D1 < L.n1 && D2 >= 0 && D2 < L.n2;
+                        hc[(size_t)hs[sl].x + (size_t)cpk::mg_slot(d1, d2) * cpk::kWave + lane] = in ? D1 * L.n2 + D2 : C;
+                    }
+            }
+        }
+        if (hipMemcpy(L.slice, hs.data(), sizeof(int2) * hs.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(L.col, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice) != hipSuccess)
+            s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
+    }
+    bool identity = true;
+    for (int n = 0; n < nn; ++n) identity = identity && lg[n] == n;
+    if (!s && !identity) {
+        std::vector<int> no(nn);
+        for (int n = 0; n < nn; ++n) no[lg[n]] = n;
+        if (!(s = G->get(&G->lg0, nn)) && !(s = G->get(&G->node0, nn)))
+            if (hipMemcpy(G->lg0, lg.data(), sizeof(int) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(G->node0, no.data(), sizeof(int) * nn, hipMemcpyHostToDevice) != hipSuccess)
+                s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
+    }
+    if (!s) {
+        G->nd = 2 * G->L.back().nn;
+        s = G->get(&G->dense, (size_t)G->nd * G->nd);
+    }
+    if (s) {
+        mg_destroy(P);
+        return s;
+    }
+    return CP_OK;
+}
+
+int cp_problem_set_graphs(cp_problem* P, int enable)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_problem_set_graphs: no multigrid lattice set");
+    P->mg->use_graphs = enable != 0;
+    return CP_OK;
+}
+
+int cp_problem_mg_levels(cp_problem* P, int* levels, int* dims)
+{
+    if (!P) return cp::fail(CP_ERR_INVALID, "null problem");
+    if (!P->mg) {
+        if (levels) *levels = 0;
+        return CP_OK;
+    }
+    if (levels) *levels = P->mg->nlev;
+    if (dims)
+        for (int l = 0; l < P->mg->nlev; ++l) {
+            dims[2 * l] = P->mg->L[l].n1;
+            dims[2 * l + 1] = P->mg->L[l].n2;
+        }
+    return CP_OK;
+}
+
+}  // extern "C"

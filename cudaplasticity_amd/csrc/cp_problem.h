@@ -44,6 +44,8 @@ struct Layout {
 
 Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax);
 
+struct MgState;   // multigrid hierarchy (cp_mg.hip)
+
 }  // namespace cp
 
 struct cp_problem {
@@ -60,7 +62,8 @@ struct cp_problem {
     long long last_cg_iters = 0;
     double last_cg_ms_per_iter = 0.;
     bool have_fp64_values = false;
-    int precond = 0;          // block-Jacobi PCG for the current solve
+    int precond = 0;          // current solve: 0 plain CG, 1 block-Jacobi, 2 multigrid
+    cp::MgState* mg = nullptr;  // set by cp_problem_set_grid
     bool solved = false;      // u / psi hold a converged level at `load`
     double lam_prev = 0.;     // load factor of u_prev / psi_prev
 
@@ -117,4 +120,9 @@ int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max
                       int solved_tight);
 template <typename T>
 void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st);
+int mg_setup(cp_problem* P, int precision);
+void mg_destroy(cp_problem* P);
+template <typename T>
+int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
+              long long* iters, int* status, double* ms);
 }  // namespace cp

@@ -372,6 +372,8 @@ int cp_server_submit(cp_server* S, int nn, int ne, const double* x, const double
     if (!S || !x || !y || !conn || !rp || !mask || !mat || !opt || nsteps < 1 || nn < 1 || ne < 1)
         return cp::fail(CP_ERR_INVALID, "cp_server_submit: bad arguments");
     if (opt->precision != CP_FP64 && opt->precision != CP_MIXED) return cp::fail(CP_ERR_INVALID, "bad precision");
+    if (opt->precond < 0 || opt->precond > 1)
+        return cp::fail(CP_ERR_INVALID, "cp_server_submit: the serving loop runs plain or block-Jacobi CG (precond 0 or 1)");
     std::unique_ptr<Request> r(new Request());
     r->id = S->next_id++;
     r->nn = nn;

@@ -373,10 +373,12 @@ int launch_assemble(cp_problem* P, int precision)
         return fail(CP_ERR_INVALID, "row width too large");
     }
     CP_HIP(hipGetLastError());
-    if (P->precond) {
+    if (P->precond == 1) {
         if (f64) hipLaunchKernelGGL(cpk::k_block_jacobi<double>, g, b, 0, P->stream, P->cg64_args(), P->minv64);
         else hipLaunchKernelGGL(cpk::k_block_jacobi<float>, g, b, 0, P->stream, P->cg32_args(), P->minv32);
         CP_HIP(hipGetLastError());
+    } else if (P->precond == 2) {
+        if (int s = mg_setup(P, precision)) return s;
     }
     if (f64) P->have_fp64_values = true;
     return CP_OK;
@@ -539,7 +541,9 @@ static int cg_fp64(cp_problem* P, double tol, int maxit, long long* iters, doubl
     auto a = P->cg64_args();
     if (int s = enqueue_cg_init(P, CP_FP64, bn, tol, maxit)) return s;
     int status = 0;
-    if (int s = run_cg_iterations<double>(P, a, P->sc64, tol * tol, bn, iters, &status, ms)) return s;
+    if (int s = P->precond == 2 ? run_mg_cg<double>(P, a, P->sc64, tol * tol, bn, iters, &status, ms)
+                                : run_cg_iterations<double>(P, a, P->sc64, tol * tol, bn, iters, &status, ms))
+        return s;
     const cpk::CgScal h = *P->h_cg;
     if (relres) *relres = std::sqrt(h.rho / (h.bnorm2 > 0 ? h.bnorm2 : 1.));
     P->last_cg_iters = *iters;
@@ -581,12 +585,12 @@ int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int
         auto a = P->cg64_args();
         hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc64, bnorm2, tol * tol, maxit);
         hipLaunchKernelGGL(cpk::k_cg_init1<double>, g, b, 0, st, a, 0);
-        launch_init2<double>(P->max_w, P->precond, a, g, st);
+        if (P->precond != 2) launch_init2<double>(P->max_w, P->precond, a, g, st);
     } else {
         auto a = P->cg32_args();
         hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, bnorm2, tol * tol, maxit);
         hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
-        launch_init2<float>(P->max_w, P->precond, a, g, st);
+        if (P->precond != 2) launch_init2<float>(P->max_w, P->precond, a, g, st);
     }
     CP_HIP(hipGetLastError());
     return CP_OK;
@@ -660,7 +664,9 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
         long long it = 0;
         int status = 0;
         double t = 0.;
-        if (int s = run_cg_iterations<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t)) return s;
+        if (int s = P->precond == 2 ? run_mg_cg<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t)
+                                    : run_cg_iterations<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t))
+            return s;
         *iters += it;
         *ms += t;
         P->last_cg_iters = it;
@@ -761,7 +767,8 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     const int prec = opt->precision;
     if (prec != CP_FP64 && prec != CP_MIXED) return fail(CP_ERR_INVALID, "unknown precision");
     if (int s = problem_alloc_solver(P, prec)) return s;
-    P->precond = opt->precond ? 1 : 0;
+    if (opt->precond < 0 || opt->precond > 2) return fail(CP_ERR_INVALID, "precond must be 0, 1 or 2");
+    P->precond = opt->precond;
     hipStream_t stream = P->stream;
     if (!opt->warm)
         if (int s = reset_state(P)) return s;
@@ -907,6 +914,7 @@ int cp_problem_destroy(cp_problem* P)
     if (P->stream) (void)hipStreamSynchronize(P->stream);
     for (auto& e : P->ev)
         if (e) (void)hipEventDestroy(e);
+    cp::mg_destroy(P);
     if (P->h_cg) (void)hipHostFree(P->h_cg);
     if (P->common.base) (void)hipFree(P->common.base);
     if (P->a64.base) (void)hipFree(P->a64.base);
@@ -1195,7 +1203,8 @@ int cp_bench_spmv(cp_problem* P, int precision, int reps, double* ms, double* by
 
 int cp_bench_cg(cp_problem* P, int precision, int precond, int reps, double* ms, double* bytes_algo)
 {
-    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED)) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED) || precond < 0 || precond > 1)
+        return cp::fail(CP_ERR_INVALID, "bad arguments");
     P->precond = precond ? 1 : 0;
     if (int s = cp::launch_assemble(P, precision)) return s;
     hipStream_t st = P->stream;

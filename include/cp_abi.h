@@ -91,7 +91,9 @@ typedef struct {
     int    warm;         /* 1: keep u and the secant moduli from the previous call */
     double inner_tol;    /* CP_MIXED: smallest relative reduction one FP32 inner solve is asked
                             for (each asks for what the FP64 residual test still needs) */
-    int    precond;      /* 0: unpreconditioned CG (the reference's); 1: 2x2 block-Jacobi PCG */
+    int    precond;      /* 0: unpreconditioned CG (the reference's); 1: 2x2 block-Jacobi PCG;
+                            2: CG preconditioned by a geometric multigrid V-cycle (needs
+                            cp_problem_set_grid; not in the serving loop) */
     int    predictor;    /* load stepping initial guess from the last two levels: 0 none (previous
                             level's state), 1 extrapolate u, 2 extrapolate u and log(psi) */
     double cg_forcing;   /* > 0: inexact secant iterations -- secant iteration k solves to
@@ -204,6 +206,20 @@ int cp_problem_csr_nnz(cp_problem* prob, long long* nnz);
 int cp_problem_get_csr(cp_problem* prob, int* rows, int* cols);
 /* FP64 values of the last FP64 assembly, in reference CSR order. */
 int cp_problem_get_values(cp_problem* prob, double* vals);
+
+/* New: the logical lattice of a structured quad mesh, for the multigrid
+ * preconditioner (precond = 2): node_of[i1 * n2 + i2] is the node at lattice
+ * point (i1, i2) (NULL: node = i1 * n2 + i2, the numbering of the rectangle,
+ * quarter-plate and quarter-ring generators, with n1 = N1 + 1 or N_theta + 1
+ * and n2 = N2 + 1 or N_r + 1).  Every pair of nodes sharing an element must
+ * be lattice neighbours; the call builds the level hierarchy. */
+int cp_problem_set_grid(cp_problem* prob, int n1, int n2, const int* node_of);
+/* Replay of the multigrid CG iteration as a HIP graph (default 1); 0 launches
+ * its kernels one by one (same kernels, same results). */
+int cp_problem_set_graphs(cp_problem* prob, int enable);
+/* Number of multigrid levels (0 without a lattice) and their lattice sizes
+ * (dims: 2 per level, may be NULL). */
+int cp_problem_mg_levels(cp_problem* prob, int* levels, int* dims);
 
 /* Scale the load (rp, prescribed u) -- new: load stepping. */
 int cp_set_load(cp_problem* prob, double factor);
