@@ -12,13 +12,14 @@ A "step" is one load increment: warm-started from the previous one,
 assemble -> CG -> return map until the secant iteration converges.  The
 timed region runs the first K increments of the 20-increment ramp on a
 fresh device problem, all inputs already in HBM; W warm-up increments run
-first on a separate instance.  `value` is the mixed-precision path (FP32 CG
-with FP64 iterative refinement, FP64 return map and Aitken).  The same line
-reports the FP64 path with the same solver settings, the reference
-algorithm in FP64 (unpreconditioned CG to the full tolerance at every secant
-iteration, no predictor) over the first increments, the roofline of the
-dominant kernel, a served stream of independent problems (p50/p99 latency)
-and the CPU oracle.
+first on a separate instance.  `value` is the mixed-precision path: FP32 CG
+preconditioned by a geometric multigrid V-cycle, FP64 iterative refinement,
+FP64 return map and Aitken.  The same line reports the FP64 path with the
+same solver settings, the mixed path with block-Jacobi CG instead of
+multigrid, the reference algorithm in FP64 (unpreconditioned CG to the full
+tolerance at every secant iteration, no predictor) over the first
+increments, the roofline of the dominant kernel, a served stream of
+independent problems (p50/p99 latency) and the CPU oracle.
 """
 import argparse
 import json
@@ -38,8 +39,8 @@ NSTEPS = 20
 # --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --roofline-only`
 # (632x316 quarter plate), each scaled by what the counter reads for a plain
 # 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000):
-# 48.19 MB read + 8.05 MB written.
-MEASURED_TRAFFIC = {(632, 316): 56.24e6}
+# 48.19 MB read + 8.05 MB written for the fused CG iteration.
+MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6}
 
 
 def log(*a):
@@ -54,19 +55,22 @@ def build_case(A, Nt, Nr, S=300.0):
     return mesh, loads, mat
 
 
-def solver_options(A, args, precision, reference=False):
+def solver_options(A, args, precision, reference=False, precond=None):
     if reference:
         # the reference's algorithm: unpreconditioned CG to cg_tol at every
         # secant iteration, previous level's state as the initial guess
         return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision)
-    return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision, precond=True,
-                     predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
+    return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision,
+                     precond=args.precond if precond is None else precond, predictor=1, cg_forcing=args.forcing,
+                     inner_tol=args.inner_tol)
 
 
-def run_ramp(A, ctx, mesh, loads, mat, ksteps, opt, tag):
+def run_ramp(A, ctx, mesh, loads, mat, ksteps, opt, tag, grid=None):
     """First `ksteps` increments of the 20-increment ramp; (seconds, stats, problem)."""
     t0 = time.perf_counter()
     P = A.Problem(ctx, mesh, loads, mat)
+    if opt.precond == 2:
+        P.set_grid(*grid)
     P.setup_seconds = time.perf_counter() - t0
     per = []
     t0 = time.perf_counter()
@@ -97,6 +101,21 @@ def scalar_nnz(A, P):
     n = C.c_longlong()
     A.check(A.lib().cp_problem_csr_nnz(P.ptr, C.byref(n)))
     return n.value
+
+
+def cheb_step_bytes(nnz, N, s):
+    """Algorithmic bytes of one Chebyshev smoothing step of the finest level
+    in the reference's CSR structure: the SpMV nnz(s+4) + 4(N+1) + 2Ns, the
+    second input vector (the direction) and the right-hand side Ns each, the
+    2x2 block-diagonal inverse 1.5Ns, the second output Ns and the mask N."""
+    return nnz * (s + 4) + 4 * (N + 1) + 2 * N * s + N * s + N * s + 1.5 * N * s + N * s + N
+
+
+def bench_mg(A, P, precision, reps):
+    import ctypes as C
+    v, sm, b = C.c_double(), C.c_double(), C.c_double()
+    A.check(A.lib().cp_bench_mg(P.ptr, precision, reps, C.byref(v), C.byref(sm), C.byref(b)))
+    return v.value, sm.value, b.value
 
 
 def rel(a, b):
@@ -150,7 +169,7 @@ def served_stream(A, ctx, args):
     and every served result compared with the same problem solved alone."""
     from cudaplasticity_amd import serving
     stream = serving.make_stream(args.serve_n, qmin=2e3, qmax=args.serve_qmax)
-    opt = solver_options(A, args, A.MIXED)
+    opt = solver_options(A, args, A.MIXED, precond=1)   # the serving loop batches block-Jacobi CG
     t0 = time.perf_counter()
     alone = []
     for mesh, loads, mat, _ in stream:
@@ -185,6 +204,8 @@ def main():
     ap.add_argument("--nr", type=int, default=316)
     ap.add_argument("--cg-tol", type=float, default=1e-9)
     ap.add_argument("--forcing", type=float, default=0.1)
+    ap.add_argument("--precond", type=int, default=2, choices=(1, 2),
+                    help="1 block-Jacobi CG, 2 multigrid-preconditioned CG (headline)")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
     ap.add_argument("--ref-steps", type=int, default=8)
     ap.add_argument("--large-nt", type=int, default=2000)
@@ -201,6 +222,7 @@ def main():
     ap.add_argument("--roofline-only", action="store_true",
                     help="only time the CG iteration kernel on the workload's matrix (for profiler runs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bj", action="store_true", help="skip the block-Jacobi comparison ramp")
     ap.add_argument("--no-ref", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
@@ -225,29 +247,45 @@ def main():
     log(f"mesh {args.nt}x{args.nr}: {mesh.nn} nodes, {mesh.ne} quads")
     ctx = A.Context(0)
     log("device", ctx.info())
+    grid = (args.nt + 1, args.nr + 1)
     if args.roofline_only:
-        # the dominant kernel alone, for profiling: block-Jacobi FP32 CG
-        # iterations on the workload's matrix (initial secant moduli, full load)
+        # the dominant kernel alone, for profiling, on the workload's matrix
+        # (initial secant moduli, full load): the finest level's Chebyshev
+        # smoothing step (multigrid) or the block-Jacobi CG iteration
         P = A.Problem(ctx, mesh, loads, mat)
         P.set_load(1.0)
-        ms, fmt = bench_cg(A, P, A.MIXED, args.reps)
-        byt = cg_iteration_bytes(scalar_nnz(A, P), 2 * mesh.nn, 4)
-        print(json.dumps({"kernel": "cpk::k_cg_iter_fused<9,true,float>", "launches": args.reps, "avg_launch_ms": ms,
-                          "bytes_per_launch": byt, "achieved_gbps": byt / ms / 1e6,
-                          "format_bytes_per_launch": fmt, "achieved_format_gbps": fmt / ms / 1e6}), flush=True)
+        N, nnz = 2 * mesh.nn, scalar_nnz(A, P)
+        if args.precond == 2:
+            P.set_grid(*grid)
+            _, ms, fmt = bench_mg(A, P, A.MIXED, args.reps)
+            name, byt = "cpk::k_mg_smooth<9,1,float>", cheb_step_bytes(nnz, N, 4)
+        else:
+            ms, fmt = bench_cg(A, P, A.MIXED, args.reps)
+            name, byt = "cpk::k_cg_iter_fused<9,true,float>", cg_iteration_bytes(nnz, N, 4)
+        print(json.dumps({"kernel": name, "launches": args.reps, "avg_launch_ms": ms, "bytes_per_launch": byt,
+                          "achieved_gbps": byt / ms / 1e6, "format_bytes_per_launch": fmt,
+                          "achieved_format_gbps": fmt / ms / 1e6}), flush=True)
         P.close()
         return
     opt_mx = solver_options(A, args, A.MIXED)
     opt_64 = solver_options(A, args, A.FP64)
 
     if args.warmup > 0:
-        t, _, P = run_ramp(A, ctx, mesh, loads, mat, min(args.warmup, NSTEPS), opt_mx, "warmup")
+        t, _, P = run_ramp(A, ctx, mesh, loads, mat, min(args.warmup, NSTEPS), opt_mx, "warmup", grid)
         P.close()
 
-    t_mx, per_mx, Pmx = run_ramp(A, ctx, mesh, loads, mat, K, opt_mx, "mixed")
+    t_mx, per_mx, Pmx = run_ramp(A, ctx, mesh, loads, mat, K, opt_mx, "mixed", grid)
     log(f"mixed: {K} increments in {t_mx:.3f} s")
-    t_64, per_64, P64 = run_ramp(A, ctx, mesh, loads, mat, K, opt_64, "fp64")
+    t_64, per_64, P64 = run_ramp(A, ctx, mesh, loads, mat, K, opt_64, "fp64", grid)
     log(f"fp64: {K} increments in {t_64:.3f} s")
+    bj = None
+    if args.precond == 2 and not args.no_bj:
+        t_bj, per_bj, Pbj = run_ramp(A, ctx, mesh, loads, mat, K, solver_options(A, args, A.MIXED, precond=1),
+                                     "mixed-block-jacobi")
+        bj = {"load_steps_per_s": K / t_bj, "cg_iterations": sum(s.cg_iters for s in per_bj),
+              "rel_err_vs_multigrid": {"u": rel(Pbj.field("u"), Pmx.field("u")),
+                                       "sigma_i": rel(Pbj.field("sigI"), Pmx.field("sigI"))}}
+        Pbj.close()
     epsT = mat.sigmaT / mat.E
     err = {"u": rel(Pmx.field("u"), P64.field("u")), "sigma_i": rel(Pmx.field("sigI"), P64.field("sigI")),
            "plastic_flags_equal": bool(np.array_equal(Pmx.field("epsI") >= epsT, P64.field("epsI") >= epsT))}
@@ -258,6 +296,8 @@ def main():
         R = max(1, min(args.ref_steps, K))
         t_ref, per_ref, Pref = run_ramp(A, ctx, mesh, loads, mat, R, solver_options(A, args, A.FP64, True), "fp64-ref")
         Pchk = A.Problem(ctx, mesh, loads, mat)
+        if opt_mx.precond == 2:
+            Pchk.set_grid(*grid)
         Pchk.solve_steps(R, R / NSTEPS, opt_mx)
         t_mx_R = sum(s.seconds for s in per_mx[:R])
         per_ref_steps = per_ref
@@ -281,6 +321,19 @@ def main():
     ach64 = bytes64 / (ms64 * 1e-3) / 1e9
     cg_it = sum(s.cg_iters for s in per_mx)
     cg_s = sum(s.cg_seconds for s in per_mx)
+    mg = None
+    if args.precond == 2:
+        # multigrid path: the finest level's Chebyshev smoothing step (three
+        # per V-cycle each way) and the whole V-cycle
+        mv32, ms_s32, fmt_s32 = bench_mg(A, Pmx, A.MIXED, reps)
+        mv64, ms_s64, fmt_s64 = bench_mg(A, P64, A.FP64, reps)
+        bs32, bs64 = cheb_step_bytes(nnz, 2 * mesh.nn, 4), cheb_step_bytes(nnz, 2 * mesh.nn, 8)
+        mg = {"levels": Pmx.mg_levels(), "vcycle_ms": {"mixed": mv32, "fp64": mv64},
+              "smooth_step_ms": {"mixed": ms_s32, "fp64": ms_s64},
+              "smooth_step_gbps": {"mixed": bs32 / ms_s32 / 1e6, "fp64": bs64 / ms_s64 / 1e6},
+              "smooth_step_format_gbps": {"mixed": fmt_s32 / ms_s32 / 1e6, "fp64": fmt_s64 / ms_s64 / 1e6},
+              "cg_iterations": {"mixed": sum(s.cg_iters for s in per_mx), "fp64": sum(s.cg_iters for s in per_64)}}
+        log("multigrid:", mg)
     log(f"CG iteration kernel: FP32 {ms32 * 1e3:.1f} us {bytes32 / 1e6:.1f} MB ({fmt32 / 1e6:.1f} MB as stored) "
         f"-> {ach32:.0f} GB/s; "
         f"FP64 {ms64 * 1e3:.1f} us {bytes64 / 1e6:.1f} MB -> {ach64:.0f} GB/s; SpMV FP64 {sp_ms * 1e3:.1f} us "
@@ -297,13 +350,19 @@ def main():
         l32, lf32 = bench_cg(A, PL, A.MIXED, max(20, reps // 4))
         l64, lf64 = bench_cg(A, PL, A.FP64, max(20, reps // 4))
         lsp_ms, _, lsp_algo = PL.bench_spmv(A.FP64, max(20, reps // 4))
+        PL.set_grid(args.large_nt + 1, args.large_nr + 1)
+        lv32, lsm32, lfs32 = bench_mg(A, PL, A.MIXED, max(20, reps // 4))
+        lbs32 = cheb_step_bytes(lnnz, 2 * lm.nn, 4)
         b32, b64 = cg_iteration_bytes(lnnz, 2 * lm.nn, 4), cg_iteration_bytes(lnnz, 2 * lm.nn, 8)
         large = {"mesh": f"{args.large_nt}x{args.large_nr}", "elements": lm.ne, "nodes": lm.nn,
                  "mixed_iteration_kernel": {"ms": l32, "bytes": b32, "gbps": b32 / l32 / 1e6,
                                             "format_gbps": lf32 / l32 / 1e6},
                  "fp64_iteration_kernel": {"ms": l64, "bytes": b64, "gbps": b64 / l64 / 1e6,
                                            "format_gbps": lf64 / l64 / 1e6},
-                 "fp64_spmv": {"ms": lsp_ms, "bytes": lsp_algo, "gbps": lsp_algo / lsp_ms / 1e6}}
+                 "fp64_spmv": {"ms": lsp_ms, "bytes": lsp_algo, "gbps": lsp_algo / lsp_ms / 1e6},
+                 "mixed_mg_smooth_step": {"ms": lsm32, "bytes": lbs32, "gbps": lbs32 / lsm32 / 1e6,
+                                          "format_gbps": lfs32 / lsm32 / 1e6},
+                 "mixed_mg_vcycle_ms": lv32, "mg_levels": PL.mg_levels()}
         log("large mesh kernels:", large)
         PL.close()
 
@@ -329,8 +388,9 @@ def main():
                                 f"hardening sigma_Y 240 MPa H E/100, increments 1..{K} of {NSTEPS} to 300 MPa, "
                                 f"Aitken to 1e-6, cg_tol {args.cg_tol:g}"),
                    "nodes": mesh.nn, "elements": mesh.ne, "dof": 2 * mesh.nn,
-                   "path": "mixed: FP32 block-Jacobi CG + FP64 refinement, FP64 return map, load predictor, "
-                           f"inexact secant iterations (forcing {args.forcing:g})"},
+                   "path": (("mixed: FP32 CG preconditioned by a geometric multigrid V-cycle" if args.precond == 2
+                             else "mixed: FP32 block-Jacobi CG") + " + FP64 refinement, FP64 return map, load "
+                            f"predictor, inexact secant iterations (forcing {args.forcing:g})")},
         "problem_setup_seconds": Pmx.setup_seconds,
         "load_steps_per_s_incl_setup_and_upload": K / (t_mx + Pmx.setup_seconds),
         "fp64_load_steps_per_s": K / t_64,
@@ -342,12 +402,21 @@ def main():
                     "mixed_in_solve_incl_scalar_step": (cg_it * bytes32 / cg_s / 1e9) if cg_s > 0 else None},
         "iterations": {"mixed": [[s.outer_iters, s.cg_iters, s.refinements] for s in per_mx],
                        "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
-        "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": ach32 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr)),
-                     "kernel": ("cpk::k_cg_iter_fused<9,true,float> (block-Jacobi Chronopoulos-Gear iteration, "
-                                "scalar step included)" if fused else "cpk::k_cg_iter<9,true,float>"),
-                     "bytes_per_launch": bytes32, "avg_launch_ms": ms32, "reps": reps,
-                     "format_bytes_per_launch": fmt32, "achieved_format": fmt32 / (ms32 * 1e-3) / 1e9},
+        "roofline": ({"bound": "hbm", "achieved": bs32 / ms_s32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": bs32 / ms_s32 / 1e6 / HBM_PEAK_GBS,
+                      "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, "mg_smooth")),
+                      "kernel": "cpk::k_mg_smooth<9,1,float> (finest-level Chebyshev smoothing step)",
+                      "bytes_per_launch": bs32, "avg_launch_ms": ms_s32, "reps": reps,
+                      "format_bytes_per_launch": fmt_s32, "achieved_format": fmt_s32 / ms_s32 / 1e6}
+                     if mg else
+                     {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": ach32 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, "cg")),
+                      "kernel": ("cpk::k_cg_iter_fused<9,true,float> (block-Jacobi Chronopoulos-Gear iteration, "
+                                 "scalar step included)" if fused else "cpk::k_cg_iter<9,true,float>"),
+                      "bytes_per_launch": bytes32, "avg_launch_ms": ms32, "reps": reps,
+                      "format_bytes_per_launch": fmt32, "achieved_format": fmt32 / (ms32 * 1e-3) / 1e9}),
+        "multigrid": mg,
+        "mixed_block_jacobi": bj,
         "kernels_2000x1000": large,
         "served": serve,
         "cpu_baseline": None,

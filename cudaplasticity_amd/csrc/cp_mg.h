@@ -26,7 +26,7 @@
 
 namespace cpk {
 
-constexpr int kMgDenseMax = 256;   // DOFs of the coarsest level
+constexpr int kMgDenseMax = 128;   // DOFs of the coarsest level (its inverse stays in registers)
 constexpr double kMgLowFrac = 1. / 30.;
 
 // 1-D coarsening: coarse point count of n fine points.
@@ -114,6 +114,8 @@ struct GalArgs {
     CgArgs<TS> f;          // fine level: matrix, mask
     const int* lg;         // fine node -> lattice index (nullptr: identity)
     const int* node_of;    // fine lattice index -> node (nullptr: identity)
+    const signed char* slot_pos;   // fine node x 9 lattice offsets -> row position, -1 if absent
+                                   // (nullptr: the coarse-level layout, position = slot)
     int n1f, n2f;
     int n1c, n2c, nnc, nbc;
     const int2* slice_c;
@@ -135,85 +137,90 @@ __device__ __forceinline__ void load_block_d(const CgArgs<T>& a, size_t q, doubl
     m11 = b11;
 }
 
-// One coarse node per thread: (P^T A P)(C, D) for the 3x3 coarse neighbours
-// D, summed over C's fine children f (in lattice order) and f's row blocks
-// (in storage order).
+// Children of coarse point K in an n-point direction: fine points and
+// their interpolation weights (at most 3).
+__device__ __forceinline__ int mg_children(int n, int K, int (&f)[3], double (&w)[3])
+{
+    const int p = mg_pos(n, K);
+    int c = 0;
+    for (int a = -1; a <= 1; ++a) {
+        const int q = p + a;
+        if (q < 0 || q >= n) continue;
+        const double wq = mg_weight(n, q, K);
+        if (wq == 0.) continue;
+        f[c] = q;
+        w[c] = wq;
+        ++c;
+    }
+    return c;
+}
+
+// One (coarse node C, stencil slot t) per thread: (P^T A P)(C, D_t), D_t
+// the coarse neighbour of slot t: the sum over the fine children f of C and
+// g of D_t that are lattice neighbours of w_f w_g A_fg (in lattice order).
+// A_fg is found through the fine row's slot table (offset (g - f) -> storage
+// position; level 0 carries one, coarser levels store slots in offset order).
+// nbc counts blocks over 9 x nnc threads.
 template <typename TS, typename TD>
 __global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
 {
-    const int C = xcd_block(g.nbc) * kBlock + threadIdx.x;
-    if (C >= g.nnc) return;
+    const int id = xcd_block(g.nbc) * kBlock + threadIdx.x;
+    if (id >= 9 * g.nnc) return;
+    const int C = id / 9, t = id - 9 * C;
     const int K1 = C / g.n2c, K2 = C - K1 * g.n2c;
-    double acc[9][4];
-#pragma unroll
-    for (int s = 0; s < 9; ++s) acc[s][0] = acc[s][1] = acc[s][2] = acc[s][3] = 0.;
-    const int p1 = mg_pos(g.n1f, K1), p2 = mg_pos(g.n2f, K2);
-    for (int a1 = -1; a1 <= 1; ++a1) {
-        const int f1 = p1 + a1;
-        if (f1 < 0 || f1 >= g.n1f) continue;
-        const double w1 = mg_weight(g.n1f, f1, K1);
-        if (w1 == 0.) continue;
-        for (int a2 = -1; a2 <= 1; ++a2) {
-            const int f2 = p2 + a2;
-            if (f2 < 0 || f2 >= g.n2f) continue;
-            const double w2 = mg_weight(g.n2f, f2, K2);
-            if (w2 == 0.) continue;
-            const int lf = f1 * g.n2f + f2;
-            const int fn = g.node_of ? g.node_of[lf] : lf;
-            const double wf = w1 * w2;
-            const uchar2 mf = g.f.mask[fn];
-            const int2 sl = g.f.slice[fn >> 6];
-            const size_t base = (size_t)sl.x + (fn & 63);
-            for (int k = 0; k < sl.y; ++k) {
-                const size_t q = base + (size_t)k * kWave;
-                const int gn = g.f.col[q];
-                double m00, m01, m10, m11;
-                load_block_d(g.f, q, m00, m01, m10, m11);
-                const uchar2 mg = g.f.mask[gn];
-                m00 = (mf.x && mg.x) ? m00 : 0.;
-                m01 = (mf.x && mg.y) ? m01 : 0.;
-                m10 = (mf.y && mg.x) ? m10 : 0.;
-                m11 = (mf.y && mg.y) ? m11 : 0.;
-                const int lgg = g.lg ? g.lg[gn] : gn;
-                const int g1 = lgg / g.n2f, g2 = lgg - g1 * g.n2f;
-                int Q1[2], Q2[2];
-                double V1[2], V2[2];
-                const int nq1 = mg_parents(g.n1f, g1, Q1, V1), nq2 = mg_parents(g.n2f, g2, Q2, V2);
-                for (int b1 = 0; b1 < nq1; ++b1)
-                    for (int b2 = 0; b2 < nq2; ++b2) {
-                        const int d1 = Q1[b1] - K1, d2 = Q2[b2] - K2;
-                        if (d1 < -1 || d1 > 1 || d2 < -1 || d2 > 1) continue;
-                        const int s = mg_slot(d1, d2);
-                        const double ww = wf * V1[b1] * V2[b2];
-#pragma unroll
-                        for (int t = 0; t < 9; ++t)
-                            if (t == s) {
-                                acc[t][0] += ww * m00;
-                                acc[t][1] += ww * m01;
-                                acc[t][2] += ww * m10;
-                                acc[t][3] += ww * m11;
-                            }
+    // slot t -> coarse offset (d1, d2)
+    const int idx = t == 0 ? 4 : (t <= 4 ? t - 1 : t);
+    const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
+    const int D1 = K1 + d1, D2 = K2 + d2;
+    double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
+    if (D1 >= 0 && D1 < g.n1c && D2 >= 0 && D2 < g.n2c) {
+        int fa[3], fb[3], ga[3], gb[3];
+        double wfa[3], wfb[3], wga[3], wgb[3];
+        const int nfa = mg_children(g.n1f, K1, fa, wfa), nfb = mg_children(g.n2f, K2, fb, wfb);
+        const int nga = mg_children(g.n1f, D1, ga, wga), ngb = mg_children(g.n2f, D2, gb, wgb);
+        for (int i = 0; i < nfa; ++i)
+            for (int j = 0; j < nfb; ++j) {
+                const int lf = fa[i] * g.n2f + fb[j];
+                const int fn = g.node_of ? g.node_of[lf] : lf;
+                const uchar2 mf = g.f.mask[fn];
+                const size_t base = (size_t)g.f.slice[fn >> 6].x + (fn & 63);
+                for (int k = 0; k < nga; ++k) {
+                    const int e1 = ga[k] - fa[i];
+                    if (e1 < -1 || e1 > 1) continue;
+                    for (int l = 0; l < ngb; ++l) {
+                        const int e2 = gb[l] - fb[j];
+                        if (e2 < -1 || e2 > 1) continue;
+                        const int so = mg_slot(e1, e2);
+                        const int pos = g.slot_pos ? g.slot_pos[(size_t)fn * 9 + so] : so;
+                        if (pos < 0) continue;   // not coupled in the fine matrix
+                        const size_t q = base + (size_t)pos * kWave;
+                        const int gn = g.f.col[q];
+                        const uchar2 mg = g.f.mask[gn];
+                        double m00, m01, m10, m11;
+                        load_block_d(g.f, q, m00, m01, m10, m11);
+                        const double ww = wfa[i] * wfb[j] * wga[k] * wgb[l];
+                        acc0 += (mf.x && mg.x) ? ww * m00 : 0.;
+                        acc1 += (mf.x && mg.y) ? ww * m01 : 0.;
+                        acc2 += (mf.y && mg.x) ? ww * m10 : 0.;
+                        acc3 += (mf.y && mg.y) ? ww * m11 : 0.;
                     }
+                }
             }
-        }
     }
-    uchar2 mc;
-    mc.x = acc[0][0] > 0. ? 1 : 0;
-    mc.y = acc[0][3] > 0. ? 1 : 0;
-    if (!mc.x) acc[0][0] = 1.;
-    if (!mc.y) acc[0][3] = 1.;
-    g.mask_c[C] = mc;
-    const int2 sl = g.slice_c[C >> 6];
-    const size_t base = (size_t)sl.x + (C & 63);
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
-        const size_t q = base + (size_t)t * kWave;
-        if (g.va_c) {
-            g.va_c[q] = make_double2(acc[t][0], acc[t][1]);
-            g.vb_c[q] = make_double2(acc[t][2], acc[t][3]);
-        } else {
-            g.vf_c[q] = make_float4((float)acc[t][0], (float)acc[t][1], (float)acc[t][2], (float)acc[t][3]);
-        }
+    if (t == 0) {
+        uchar2 mc;
+        mc.x = acc0 > 0. ? 1 : 0;
+        mc.y = acc3 > 0. ? 1 : 0;
+        if (!mc.x) acc0 = 1.;
+        if (!mc.y) acc3 = 1.;
+        g.mask_c[C] = mc;
+    }
+    const size_t q = (size_t)g.slice_c[C >> 6].x + (C & 63) + (size_t)t * kWave;
+    if (g.va_c) {
+        g.va_c[q] = make_double2(acc0, acc1);
+        g.vb_c[q] = make_double2(acc2, acc3);
+    } else {
+        g.vf_c[q] = make_float4((float)acc0, (float)acc1, (float)acc2, (float)acc3);
     }
 }
 
@@ -296,41 +303,53 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(CgArgs<T> a, double* D, int
     }
 }
 
-// In-place inverse of the SPD matrix D (nd x nd) by Gauss-Jordan
-// elimination without pivoting, one workgroup.
-// Launch with 1024 threads: thread t owns column t & 255 of rows t >> 8,
-// (t >> 8) + 4, ...
+// In-place inverse of the SPD matrix D (nd x nd, nd <= kMgDenseMax) by
+// Gauss-Jordan elimination without pivoting, in one workgroup of 1024
+// threads: thread (j = t & 255, i0 = t >> 8) keeps D[i][j] for the rows
+// i = i0, i0 + 4, ... in registers; each step broadcasts the pivot row and
+// column through LDS.
 static __global__ __launch_bounds__(1024) void k_mg_invert(double* D, int nd)
 {
+    constexpr int R = kMgDenseMax / 4;
     __shared__ double colk[kMgDenseMax];
-    __shared__ double piv;
+    __shared__ double rowk[kMgDenseMax];
     const int j = threadIdx.x & 255, i0 = threadIdx.x >> 8;
+    const bool active = j < nd;
+    double v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = i0 + 4 * r;
+        v[r] = (active && i < nd) ? D[(size_t)i * nd + j] : 0.;
+    }
     for (int k = 0; k < nd; ++k) {
-        if (threadIdx.x == 0) {
-            const double d = D[(size_t)k * nd + k];
-            piv = d != 0. ? 1. / d : 0.;
-        }
-        for (int i = threadIdx.x; i < nd; i += blockDim.x) colk[i] = D[(size_t)i * nd + k];
-        __syncthreads();
-        const double p = piv;
-        // every row but k eliminated against row k (row k and column k are
-        // read here and rewritten after the barrier)
-        if (j < nd && j != k) {
-            const double rkj = p * D[(size_t)k * nd + j];
-            for (int i = i0; i < nd; i += 4)
-                if (i != k) D[(size_t)i * nd + j] -= colk[i] * rkj;
+        const int rk = k >> 2, ik = k & 3;
+        // column k and row k of the current matrix to LDS
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = i0 + 4 * r;
+            if (active && j == k && i < nd) colk[i] = v[r];
+            if (active && i0 == ik && r == rk) rowk[j] = v[r];
         }
         __syncthreads();
-        for (int jj = threadIdx.x; jj < nd; jj += blockDim.x) {
-            if (jj == k) continue;
-            D[(size_t)k * nd + jj] *= p;
+        const double dk = colk[k];
+        const double p = dk != 0. ? 1. / dk : 0.;
+        if (active) {
+            const double rkj = rowk[j] * p;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = i0 + 4 * r;
+                if (i >= nd) continue;
+                if (i == k) v[r] = j == k ? p : rkj;                 // row k: scaled; pivot: 1 / d
+                else if (j == k) v[r] = -colk[i] * p;                 // column k
+                else v[r] -= colk[i] * rkj;                           // the rest
+            }
         }
-        for (int i = threadIdx.x; i < nd; i += blockDim.x) {
-            if (i == k) continue;
-            D[(size_t)i * nd + k] = -colk[i] * p;
-        }
-        if (threadIdx.x == 0) D[(size_t)k * nd + k] = p;
         __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = i0 + 4 * r;
+        if (active && i < nd) D[(size_t)i * nd + j] = v[r];
     }
 }
 
@@ -358,14 +377,11 @@ enum { MG_PRE1 = 0, MG_CHEB = 1, MG_RESID = 2, MG_POST1 = 3 };
 //   MG_RESID x = xi + di                       -> xo = x, res = b - A x
 //   MG_POST1 x = xi                            -> do = c0 D^-1 (b - A x)
 template <int W, int MODE, typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_smooth(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
-                                                      const typename V2<T>::type* di, typename V2<T>::type* xo,
-                                                      typename V2<T>::type* dout, int second, int last)
+__device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T>& v, const typename V2<T>::type* xi,
+                                               const typename V2<T>::type* di, typename V2<T>::type* xo,
+                                               typename V2<T>::type* dout, int second, int last, int n)
 {
     using V = typename V2<T>::type;
-    if (v.sc && v.sc->done) return;
-    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
-    if (n >= a.nn) return;
     const ChebCoef<T> c = mg_cheb<T>(*v.lam);
     const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
     T yx, yy;
@@ -419,6 +435,17 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(CgArgs<T> a, MgVec<T> v, c
     }
 }
 
+template <int W, int MODE, typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
+                                                      const typename V2<T>::type* di, typename V2<T>::type* xo,
+                                                      typename V2<T>::type* dout, int second, int last)
+{
+    if (v.sc && v.sc->done) return;
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    mg_smooth_node<W, MODE, T>(a, v, xi, di, xo, dout, second, last, n);
+}
+
 struct MgGrid {
     const int* lg;        // fine node -> lattice index (nullptr: identity)
     const int* node_of;   // fine lattice index -> node (nullptr: identity)
@@ -428,14 +455,10 @@ struct MgGrid {
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
 // w_f * res_f (fine residuals are already zero on fixed DOFs).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typename V2<T>::type* res,
-                                                        typename V2<T>::type* bc, const uchar2* mask_c, int nnc,
-                                                        int nbc, const CgScal* sc)
+__device__ __forceinline__ void mg_restrict_node(const MgGrid& gr, const typename V2<T>::type* res,
+                                                 typename V2<T>::type* bc, const uchar2* mask_c, int C)
 {
     using V = typename V2<T>::type;
-    if (sc && sc->done) return;
-    const int C = xcd_block(nbc) * kBlock + threadIdx.x;
-    if (C >= nnc) return;
     const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
     const int p1 = mg_pos(gr.n1f, K1), p2 = mg_pos(gr.n2f, K2);
     double sx = 0., sy = 0.;
@@ -459,16 +482,24 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typenam
     bc[C] = V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typename V2<T>::type* res,
+                                                        typename V2<T>::type* bc, const uchar2* mask_c, int nnc,
+                                                        int nbc, const CgScal* sc)
+{
+    if (sc && sc->done) return;
+    const int C = xcd_block(nbc) * kBlock + threadIdx.x;
+    if (C >= nnc) return;
+    mg_restrict_node<T>(gr, res, bc, mask_c, C);
+}
+
 // Coarse-grid correction: x_f = xin_f + mask_f * sum over f's parents of w * xc.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename V2<T>::type* xc,
-                                                       const typename V2<T>::type* xin, typename V2<T>::type* xout,
-                                                       const uchar2* mask_f, int nnf, int nbf, const CgScal* sc)
+__device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename V2<T>::type* xc,
+                                                const typename V2<T>::type* xin, typename V2<T>::type* xout,
+                                                const uchar2* mask_f, int n)
 {
     using V = typename V2<T>::type;
-    if (sc && sc->done) return;
-    const int n = xcd_block(nbf) * kBlock + threadIdx.x;
-    if (n >= nnf) return;
     const int lf = gr.lg ? gr.lg[n] : n;
     const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
     int Q1[2], Q2[2];
@@ -486,7 +517,35 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename
     xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
 }
 
-// Coarsest level: x = D^-1 b with the explicit inverse, one workgroup.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename V2<T>::type* xc,
+                                                       const typename V2<T>::type* xin, typename V2<T>::type* xout,
+                                                       const uchar2* mask_f, int nnf, int nbf, const CgScal* sc)
+{
+    if (sc && sc->done) return;
+    const int n = xcd_block(nbf) * kBlock + threadIdx.x;
+    if (n >= nnf) return;
+    mg_prolong_node<T>(gr, xc, xin, xout, mask_f, n);
+}
+
+// Coarsest level: x = D^-1 b with the explicit inverse (bs: b in LDS).
+template <typename T>
+__device__ __forceinline__ void mg_coarse_rows(const double* Dinv, int nd, const double* bs, typename V2<T>::type* x,
+                                               int i0, int stride)
+{
+    using V = typename V2<T>::type;
+    for (int i = i0; i < nd / 2; i += stride) {
+        double s0 = 0., s1 = 0.;
+        const double* r0 = Dinv + (size_t)(2 * i) * nd;
+        const double* r1 = r0 + nd;
+        for (int j = 0; j < nd; ++j) {
+            s0 += r0[j] * bs[j];
+            s1 += r1[j] * bs[j];
+        }
+        x[i] = V{(T)s0, (T)s1};
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_coarse_solve(const double* Dinv, int nd, const typename V2<T>::type* b,
                                                             typename V2<T>::type* x, const CgScal* sc)
@@ -500,15 +559,70 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse_solve(const double* Dinv, 
         bs[2 * i + 1] = v.y;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < nd / 2; i += kBlock) {
-        double s0 = 0., s1 = 0.;
-        const double* r0 = Dinv + (size_t)(2 * i) * nd;
-        const double* r1 = r0 + nd;
-        for (int j = 0; j < nd; ++j) {
-            s0 += r0[j] * bs[j];
-            s1 += r1[j] * bs[j];
+    mg_coarse_rows<T>(Dinv, nd, bs, x, threadIdx.x, kBlock);
+}
+
+// ------------------------------------------------------------ coarse tail
+
+constexpr int kTailThreads = 1024;   // levels of at most this many nodes run in one workgroup
+
+template <typename T>
+struct TailLevel {
+    CgArgs<T> a;       // matrix view
+    MgVec<T> v;        // vectors (v.b: right-hand side)
+    MgGrid grid;       // to the next coarser level
+};
+
+// Every level from the first one of at most kTailThreads nodes down to the
+// coarsest and back, in one workgroup: the same per-node work as the
+// level kernels, with workgroup barriers instead of launches between the
+// steps (global memory written by the workgroup is visible to it after the
+// barrier).  lv[0] is the tail's finest level, whose right-hand side the
+// level above has restricted; its result lands in lv[0].v.xb.
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
+                                                          const CgScal* sc)
+{
+    using V = typename V2<T>::type;
+    if (sc && sc->done) return;
+    const int tid = threadIdx.x;
+    for (int l = 0; l < nl - 1; ++l) {
+        const TailLevel<T> L = lv[l];
+        const int nn = L.a.nn;
+        if (tid < nn) mg_smooth_node<9, MG_PRE1, T>(L.a, L.v, nullptr, nullptr, L.v.xa, L.v.da, 0, 0, tid);
+        __syncthreads();
+        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.da, L.v.xb, L.v.db, 1, 0, tid);
+        __syncthreads();
+        if (tid < nn) mg_smooth_node<9, MG_RESID, T>(L.a, L.v, L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0, tid);
+        __syncthreads();
+        const TailLevel<T> Cn = lv[l + 1];
+        if (tid < Cn.a.nn) mg_restrict_node<T>(L.grid, L.v.res, const_cast<V*>(Cn.v.b), Cn.a.mask, tid);
+        __syncthreads();
+    }
+    {
+        __shared__ double bs[kMgDenseMax];
+        const TailLevel<T> Cn = lv[nl - 1];
+        for (int i = tid; i < nd / 2; i += kTailThreads) {
+            const V b = Cn.v.b[i];
+            bs[2 * i] = b.x;
+            bs[2 * i + 1] = b.y;
         }
-        x[i] = V{(T)s0, (T)s1};
+        __syncthreads();
+        mg_coarse_rows<T>(Dinv, nd, bs, Cn.v.xb, tid, kTailThreads);
+        __syncthreads();
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        const TailLevel<T> L = lv[l];
+        const TailLevel<T> Cn = lv[l + 1];
+        const int nn = L.a.nn;
+        if (tid < nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
+        __syncthreads();
+        if (tid < nn) mg_smooth_node<9, MG_POST1, T>(L.a, L.v, L.v.xb, nullptr, nullptr, L.v.da, 0, 0, tid);
+        __syncthreads();
+        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xb, L.v.da, L.v.xa, L.v.db, 0, 0, tid);
+        __syncthreads();
+        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1, tid);
+        __syncthreads();
     }
 }
 

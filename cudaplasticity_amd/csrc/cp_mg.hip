@@ -29,6 +29,7 @@ struct MgState {
     std::vector<MgLevel> L;
     int* lg0 = nullptr;      // node -> lattice index of level 0 (nullptr: identity)
     int* node0 = nullptr;    // lattice index -> node of level 0 (nullptr: identity)
+    signed char* slot0 = nullptr;   // level 0: node x 9 lattice offsets -> position in its row
     double* dense = nullptr; // coarsest operator, inverted in place
     int nd = 0;
     bool alloc[2] = {false, false};
@@ -36,6 +37,9 @@ struct MgState {
     // one CG iteration (update, V-cycle, dots, step) captured per precision
     hipGraphExec_t iter[2] = {nullptr, nullptr};
     bool use_graphs = true;
+    // levels lt.. (at most kTailThreads nodes each) run in one workgroup
+    int lt = 0;
+    void* tail[2] = {nullptr, nullptr};
 
     template <typename X>
     int get(X** p, size_t count)
@@ -99,6 +103,41 @@ static int mg_alloc(cp_problem* P, int prec)
 template <typename T>
 static cpk::CgArgs<T> level_args(cp_problem* P, int l);
 
+// Device table of the coarse-tail levels for precision T (fixed addresses).
+template <typename T>
+static int mg_tail_table(cp_problem* P)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    if (G.tail[pi]) return CP_OK;
+    std::vector<cpk::TailLevel<T>> tab;
+    for (int l = G.lt; l < G.nlev; ++l) {
+        MgLevel& L = G.L[l];
+        cpk::TailLevel<T> t{};
+        t.a = level_args<T>(P, l);
+        t.v.b = (const V*)L.vec[pi][0];
+        t.v.xa = (V*)L.vec[pi][1];
+        t.v.xb = (V*)L.vec[pi][2];
+        t.v.da = (V*)L.vec[pi][3];
+        t.v.db = (V*)L.vec[pi][4];
+        t.v.res = (V*)L.vec[pi][5];
+        t.v.lam = L.lam + pi;
+        if (l + 1 < G.nlev) {
+            t.grid.n1f = L.n1;
+            t.grid.n2f = L.n2;
+            t.grid.n1c = G.L[l + 1].n1;
+            t.grid.n2c = G.L[l + 1].n2;
+        }
+        tab.push_back(t);
+    }
+    cpk::TailLevel<T>* d = nullptr;
+    if (int s = G.get(&d, tab.size())) return s;
+    CP_HIP(hipMemcpy(d, tab.data(), sizeof(cpk::TailLevel<T>) * tab.size(), hipMemcpyHostToDevice));
+    G.tail[pi] = d;
+    return CP_OK;
+}
+
 template <>
 cpk::CgArgs<double> level_args<double>(cp_problem* P, int l)
 {
@@ -161,12 +200,13 @@ static int mg_setup_t(cp_problem* P)
             g.f = level_args<T>(P, l - 1);
             g.lg = l == 1 ? G.lg0 : nullptr;
             g.node_of = l == 1 ? G.node0 : nullptr;
+            g.slot_pos = l == 1 ? G.slot0 : nullptr;
             g.n1f = F.n1;
             g.n2f = F.n2;
             g.n1c = L.n1;
             g.n2c = L.n2;
             g.nnc = L.nn;
-            g.nbc = L.nb;
+            g.nbc = blocks_of(9 * L.nn);
             g.slice_c = L.slice;
             if (pi == 0) {
                 g.va_c = L.va;
@@ -175,7 +215,7 @@ static int mg_setup_t(cp_problem* P)
                 g.vf_c = L.vf;
             }
             g.mask_c = L.mask;
-            hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(L.nb), dim3(cpk::kBlock), 0, st, g);
+            hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
         }
         if (l < G.nlev - 1) {
             hipLaunchKernelGGL(cpk::k_mg_smooth_setup<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a,
@@ -195,6 +235,7 @@ int mg_setup(cp_problem* P, int precision)
 {
     if (!P->mg) return fail(CP_ERR_STATE, "multigrid needs the mesh lattice: call cp_problem_set_grid first");
     if (int s = mg_alloc(P, precision)) return s;
+    if (int s = precision == CP_FP64 ? mg_tail_table<double>(P) : mg_tail_table<float>(P)) return s;
     return precision == CP_FP64 ? mg_setup_t<double>(P) : mg_setup_t<float>(P);
 }
 
@@ -263,7 +304,7 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         g.n2c = G.L[l + 1].n2;
         return g;
     };
-    const int last = G.nlev - 1;
+    const int last = G.lt;   // levels lt.. run in the one-workgroup tail
     for (int l = 0; l < last; ++l) {
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
@@ -272,11 +313,8 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(C.nb), dim3(cpk::kBlock), 0, st, grid(l), (const V*)v.res,
                            (V*)C.vec[pi][0], (const uchar2*)C.mask, C.nn, C.nb, sc);
     }
-    {
-        const MgLevel& C = G.L[last];
-        hipLaunchKernelGGL(cpk::k_mg_coarse_solve<T>, dim3(1), dim3(cpk::kBlock), 0, st, (const double*)G.dense, G.nd,
-                           (const V*)C.vec[pi][0], (V*)C.vec[pi][2], sc);
-    }
+    hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st,
+                       (const cpk::TailLevel<T>*)G.tail[pi], G.nlev - G.lt, (const double*)G.dense, G.nd, sc);
     for (int l = last - 1; l >= 0; --l) {
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
@@ -395,9 +433,68 @@ template int run_mg_cg<double>(cp_problem*, const cpk::CgArgs<double>&, cpk::CgS
 template int run_mg_cg<float>(cp_problem*, const cpk::CgArgs<float>&, cpk::CgScal*, double, double, long long*,
                               int*, double*);
 
+template <typename T>
+static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smooth, double* bytes_smooth)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    hipStream_t st = P->stream;
+    const cpk::CgArgs<T> a = level_args<T>(P, 0);
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    // right-hand side: the residual of the current state (the FP32 solve
+    // works on the FP64 residual, as in the refinement loop)
+    if (pi == 1)
+        if (int s = enqueue_residual(P)) return s;
+    hipLaunchKernelGGL(cpk::k_cg_init1<T>, g, b, 0, st, a, pi);
+    MgLevel& L = G.L[0];
+    cpk::MgVec<T> v{};
+    v.b = a.r[0];
+    v.xa = (V*)L.vec[pi][1];
+    v.xb = (V*)L.vec[pi][2];
+    v.da = (V*)L.vec[pi][3];
+    v.db = (V*)L.vec[pi][4];
+    v.res = (V*)L.vec[pi][5];
+    v.lam = L.lam + pi;
+    by_width<T>(P->max_w, [&](auto w) {
+        smooth<decltype(w)::value, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
+    });
+    CP_HIP(hipEventRecord(P->ev[4], st));
+    for (int i = 0; i < reps; ++i)
+        by_width<T>(P->max_w, [&](auto w) {
+            smooth<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
+        });
+    CP_HIP(hipEventRecord(P->ev[5], st));
+    for (int i = 0; i < reps; ++i) mg_vcycle<T>(P, a.r[0], nullptr);
+    CP_HIP(hipEventRecord(P->ev[6], st));
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipEventSynchronize(P->ev[6]));
+    float t1 = 0.f, t2 = 0.f;
+    CP_HIP(hipEventElapsedTime(&t1, P->ev[4], P->ev[5]));
+    CP_HIP(hipEventElapsedTime(&t2, P->ev[5], P->ev[6]));
+    if (ms_smooth) *ms_smooth = t1 / reps;
+    if (ms_vcycle) *ms_vcycle = t2 / reps;
+    const double s = std::is_same<T, double>::value ? 8. : 4.;
+    // matrix + columns, slices; x, d, b read, x, d written (2 values per
+    // node each), the block-diagonal inverse (4 values per node), the mask
+    if (bytes_smooth)
+        *bytes_smooth = P->nblk_stored * (4. * s + 4.) + 8. * P->nslices + (5. * 2. * s + 4. * s) * P->nn + 2. * P->nn;
+    return CP_OK;
+}
+
 }  // namespace cp
 
 extern "C" {
+
+int cp_bench_mg(cp_problem* P, int precision, int reps, double* ms_vcycle, double* ms_smooth, double* bytes_smooth)
+{
+    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED)) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_bench_mg: no multigrid lattice set");
+    P->precond = 2;
+    if (int s = cp::launch_assemble(P, precision)) return s;
+    return precision == CP_FP64 ? cp::mg_bench_t<double>(P, reps, ms_vcycle, ms_smooth, bytes_smooth)
+                                : cp::mg_bench_t<float>(P, reps, ms_vcycle, ms_smooth, bytes_smooth);
+}
 
 int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
 {
@@ -448,6 +545,8 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         c2 = d2;
     }
     G->nlev = (int)G->L.size();
+    G->lt = G->nlev - 1;
+    while (G->lt > 1 && G->L[G->lt - 1].nn <= cpk::kTailThreads) --G->lt;
     if (G->nlev < 2 || 2 * G->L.back().nn > cpk::kMgDenseMax) {
         mg_destroy(P);
         return fail(CP_ERR_INVALID, "cp_problem_set_grid: lattice too small or too thin for multigrid");
@@ -479,6 +578,22 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         if (hipMemcpy(L.slice, hs.data(), sizeof(int2) * hs.size(), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(L.col, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice) != hipSuccess)
             s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
+    }
+    // level 0 rows: position of each lattice offset in the node's row (the
+    // row lists the node itself, then its neighbours, in adjacency order)
+    if (!s) {
+        std::vector<signed char> sp((size_t)nn * 9, -1);
+        for (int n = 0; n < nn; ++n) {
+            const int a1 = lg[n] / n2, a2 = lg[n] % n2;
+            for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
+                const int m = P->adj_node[q];
+                const int e1 = lg[m] / n2 - a1, e2 = lg[m] % n2 - a2;
+                sp[(size_t)n * 9 + cpk::mg_slot(e1, e2)] = (signed char)(q - P->adj_ptr[n]);
+            }
+        }
+        if (!(s = G->get(&G->slot0, sp.size())))
+            if (hipMemcpy(G->slot0, sp.data(), sp.size(), hipMemcpyHostToDevice) != hipSuccess)
+                s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
     }
     bool identity = true;
     for (int n = 0; n < nn; ++n) identity = identity && lg[n] == n;

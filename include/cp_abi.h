@@ -272,6 +272,13 @@ int cp_bench_spmv(cp_problem* prob, int precision, int reps, double* ms, double*
  * average kernel ms and the bytes one launch moves in this library's
  * storage format. */
 int cp_bench_cg(cp_problem* prob, int precision, int precond, int reps, double* ms, double* bytes_format);
+/* Multigrid timing (precond 2, lattice set): assembles and sets up the
+ * hierarchy, then times `reps` launches of the finest level's Chebyshev
+ * smoothing step and `reps` V-cycles (HIP events on the problem's stream);
+ * returns their average ms and the bytes one smoothing step moves in this
+ * library's storage format. */
+int cp_bench_mg(cp_problem* prob, int precision, int reps, double* ms_vcycle, double* ms_smooth,
+                double* bytes_smooth);
 /* Matrix storage info: nodes, stored blocks (with padding), true blocks. */
 int cp_problem_layout(cp_problem* prob, long long* nblk_stored, long long* nblk, int* max_width);
 
