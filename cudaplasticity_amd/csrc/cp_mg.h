@@ -2,9 +2,10 @@
 //
 // Structured quad meshes carry a logical grid: node = node_of[i1 * n2 + i2]
 // on an n1 x n2 lattice, every matrix coupling within one lattice step.  Each
-// coarser level halves both lattice directions (coarse k sits on fine
-// min(2k, n-1); a direction of 2 or fewer points is kept) with bilinear
-// interpolation P, and carries the Galerkin operator P^T A P of the level
+// coarser level halves the lattice directions chosen for it (coarse k sits
+// on fine min(2k, n-1); a direction of 2 or fewer points is kept; on
+// stretched meshes only the direction of the short element edges, which
+// couples strongly, is halved) with bilinear interpolation P, and carries the Galerkin operator P^T A P of the level
 // above, formed on the device as a 3x3-node stencil of 2x2 blocks in the same
 // sliced-ELL layout as the fine matrix (slot 0 the node itself, slots 1..8
 // the neighbours in raster order; slots outside the lattice point at the
@@ -29,15 +30,16 @@ namespace cpk {
 constexpr int kMgDenseMax = 128;   // DOFs of the coarsest level (its inverse stays in registers)
 constexpr double kMgLowFrac = 1. / 30.;
 
-// 1-D coarsening: coarse point count of n fine points.
-__host__ __device__ inline int mg_coarse_n(int n) { return n <= 2 ? n : n / 2 + 1; }
+// 1-D coarsening: coarse point count of n fine points (co = 0: the
+// direction is kept, as when a level is semi-coarsened along the other one).
+__host__ __device__ inline int mg_coarse_n(int n, int co = 1) { return (n <= 2 || !co) ? n : n / 2 + 1; }
 
 // Coarse parents of fine point f of an n-point direction: count (1 or 2),
 // coarse indices and bilinear weights.
-__host__ __device__ inline int mg_parents(int n, int f, int (&K)[2], double (&w)[2])
+__host__ __device__ inline int mg_parents(int n, int co, int f, int (&K)[2], double (&w)[2])
 {
-    if (n <= 2 || (f & 1) == 0) {
-        K[0] = n <= 2 ? f : (f >> 1);
+    if (n <= 2 || !co || (f & 1) == 0) {
+        K[0] = (n <= 2 || !co) ? f : (f >> 1);
         w[0] = 1.;
         return 1;
     }
@@ -53,11 +55,11 @@ __host__ __device__ inline int mg_parents(int n, int f, int (&K)[2], double (&w)
 }
 
 // Weight of coarse point K in the interpolation to fine point f (0 if none).
-__device__ __forceinline__ double mg_weight(int n, int f, int K)
+__device__ __forceinline__ double mg_weight(int n, int co, int f, int K)
 {
     int P[2];
     double w[2];
-    const int np = mg_parents(n, f, P, w);
+    const int np = mg_parents(n, co, f, P, w);
     double r = 0.;
     for (int p = 0; p < np; ++p)
         if (P[p] == K) r = w[p];
@@ -65,7 +67,10 @@ __device__ __forceinline__ double mg_weight(int n, int f, int K)
 }
 
 // Fine position of coarse point K.
-__device__ __forceinline__ int mg_pos(int n, int K) { return n <= 2 ? K : (2 * K < n - 1 ? 2 * K : n - 1); }
+__device__ __forceinline__ int mg_pos(int n, int co, int K)
+{
+    return (n <= 2 || !co) ? K : (2 * K < n - 1 ? 2 * K : n - 1);
+}
 
 // Stencil slot of the coarse neighbour at offset (d1, d2), |d| <= 1.
 __host__ __device__ inline int mg_slot(int d1, int d2)
@@ -117,6 +122,7 @@ struct GalArgs {
     const signed char* slot_pos;   // fine node x 9 lattice offsets -> row position, -1 if absent
                                    // (nullptr: the coarse-level layout, position = slot)
     int n1f, n2f;
+    int co1, co2;          // directions coarsened to the next level
     int n1c, n2c, nnc, nbc;
     const int2* slice_c;
     double2* va_c;         // FP64 destination planes
@@ -139,14 +145,14 @@ __device__ __forceinline__ void load_block_d(const CgArgs<T>& a, size_t q, doubl
 
 // Children of coarse point K in an n-point direction: fine points and
 // their interpolation weights (at most 3).
-__device__ __forceinline__ int mg_children(int n, int K, int (&f)[3], double (&w)[3])
+__device__ __forceinline__ int mg_children(int n, int co, int K, int (&f)[3], double (&w)[3])
 {
-    const int p = mg_pos(n, K);
+    const int p = mg_pos(n, co, K);
     int c = 0;
     for (int a = -1; a <= 1; ++a) {
         const int q = p + a;
         if (q < 0 || q >= n) continue;
-        const double wq = mg_weight(n, q, K);
+        const double wq = mg_weight(n, co, q, K);
         if (wq == 0.) continue;
         f[c] = q;
         w[c] = wq;
@@ -176,8 +182,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
     if (D1 >= 0 && D1 < g.n1c && D2 >= 0 && D2 < g.n2c) {
         int fa[3], fb[3], ga[3], gb[3];
         double wfa[3], wfb[3], wga[3], wgb[3];
-        const int nfa = mg_children(g.n1f, K1, fa, wfa), nfb = mg_children(g.n2f, K2, fb, wfb);
-        const int nga = mg_children(g.n1f, D1, ga, wga), ngb = mg_children(g.n2f, D2, gb, wgb);
+        const int nfa = mg_children(g.n1f, g.co1, K1, fa, wfa), nfb = mg_children(g.n2f, g.co2, K2, fb, wfb);
+        const int nga = mg_children(g.n1f, g.co1, D1, ga, wga), ngb = mg_children(g.n2f, g.co2, D2, gb, wgb);
         for (int i = 0; i < nfa; ++i)
             for (int j = 0; j < nfb; ++j) {
                 const int lf = fa[i] * g.n2f + fb[j];
@@ -450,6 +456,7 @@ struct MgGrid {
     const int* lg;        // fine node -> lattice index (nullptr: identity)
     const int* node_of;   // fine lattice index -> node (nullptr: identity)
     int n1f, n2f, n1c, n2c;
+    int co1, co2;         // directions coarsened to the next level
 };
 
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
@@ -460,17 +467,17 @@ __device__ __forceinline__ void mg_restrict_node(const MgGrid& gr, const typenam
 {
     using V = typename V2<T>::type;
     const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
-    const int p1 = mg_pos(gr.n1f, K1), p2 = mg_pos(gr.n2f, K2);
+    const int p1 = mg_pos(gr.n1f, gr.co1, K1), p2 = mg_pos(gr.n2f, gr.co2, K2);
     double sx = 0., sy = 0.;
     for (int a1 = -1; a1 <= 1; ++a1) {
         const int f1 = p1 + a1;
         if (f1 < 0 || f1 >= gr.n1f) continue;
-        const double w1 = mg_weight(gr.n1f, f1, K1);
+        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1);
         if (w1 == 0.) continue;
         for (int a2 = -1; a2 <= 1; ++a2) {
             const int f2 = p2 + a2;
             if (f2 < 0 || f2 >= gr.n2f) continue;
-            const double w2 = mg_weight(gr.n2f, f2, K2);
+            const double w2 = mg_weight(gr.n2f, gr.co2, f2, K2);
             if (w2 == 0.) continue;
             const int lf = f1 * gr.n2f + f2;
             const V r = res[gr.node_of ? gr.node_of[lf] : lf];
@@ -504,7 +511,7 @@ __device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename
     const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
     int Q1[2], Q2[2];
     double W1[2], W2[2];
-    const int nq1 = mg_parents(gr.n1f, f1, Q1, W1), nq2 = mg_parents(gr.n2f, f2, Q2, W2);
+    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
     double sx = 0., sy = 0.;
     for (int b1 = 0; b1 < nq1; ++b1)
         for (int b2 = 0; b2 < nq2; ++b2) {

@@ -13,6 +13,7 @@ namespace cp {
 
 struct MgLevel {
     int n1 = 0, n2 = 0, nn = 0, nb = 0, nslices = 0;
+    int co1 = 1, co2 = 1;    // directions halved from this level to the next
     int2* slice = nullptr;   // levels >= 1 (level 0 uses the problem's matrix)
     int* col = nullptr;
     uchar2* mask = nullptr;
@@ -128,6 +129,8 @@ static int mg_tail_table(cp_problem* P)
             t.grid.n2f = L.n2;
             t.grid.n1c = G.L[l + 1].n1;
             t.grid.n2c = G.L[l + 1].n2;
+            t.grid.co1 = L.co1;
+            t.grid.co2 = L.co2;
         }
         tab.push_back(t);
     }
@@ -203,6 +206,8 @@ static int mg_setup_t(cp_problem* P)
             g.slot_pos = l == 1 ? G.slot0 : nullptr;
             g.n1f = F.n1;
             g.n2f = F.n2;
+            g.co1 = F.co1;
+            g.co2 = F.co2;
             g.n1c = L.n1;
             g.n2c = L.n2;
             g.nnc = L.nn;
@@ -302,6 +307,8 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         g.n2f = G.L[l].n2;
         g.n1c = G.L[l + 1].n1;
         g.n2c = G.L[l + 1].n2;
+        g.co1 = G.L[l].co1;
+        g.co2 = G.L[l].co2;
         return g;
     };
     const int last = G.lt;   // levels lt.. run in the one-workgroup tail
@@ -528,7 +535,34 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     mg_destroy(P);
     auto* G = new MgState();
     P->mg = G;
-    // levels: halve both directions until the coarsest has at most kMgDenseMax DOFs
+    // mean lattice edge lengths along the two directions: a stretched mesh
+    // couples strongly across its short edges, so only that direction is
+    // halved until the two are within a factor 2 (semi-coarsening)
+    double len1 = 0., len2 = 0.;
+    {
+        std::vector<double2> xy(nn);
+        CP_HIP(hipMemcpy(xy.data(), P->xy, sizeof(double2) * nn, hipMemcpyDeviceToHost));
+        std::vector<int> no(nn);
+        for (int n = 0; n < nn; ++n) no[lg[n]] = n;
+        long long c1n = 0, c2n = 0;
+        for (int i1 = 0; i1 < n1; ++i1)
+            for (int i2 = 0; i2 < n2; ++i2) {
+                const double2 p = xy[no[i1 * n2 + i2]];
+                if (i1 + 1 < n1) {
+                    const double2 q = xy[no[(i1 + 1) * n2 + i2]];
+                    len1 += std::hypot(q.x - p.x, q.y - p.y);
+                    ++c1n;
+                }
+                if (i2 + 1 < n2) {
+                    const double2 q = xy[no[i1 * n2 + i2 + 1]];
+                    len2 += std::hypot(q.x - p.x, q.y - p.y);
+                    ++c2n;
+                }
+            }
+        len1 /= std::max<long long>(c1n, 1);
+        len2 /= std::max<long long>(c2n, 1);
+    }
+    // levels until the coarsest has at most kMgDenseMax DOFs
     int c1 = n1, c2 = n2;
     for (;;) {
         MgLevel L;
@@ -537,10 +571,16 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         L.nn = c1 * c2;
         L.nb = blocks_of(L.nn);
         L.nslices = (L.nn + cpk::kWave - 1) / cpk::kWave;
+        const bool done = 2 * c1 * c2 <= cpk::kMgDenseMax;
+        L.co1 = !(len1 > 2. * len2) && c1 > 2;
+        L.co2 = !(len2 > 2. * len1) && c2 > 2;
+        if (!L.co1 && !L.co2) L.co1 = c1 > 2, L.co2 = c2 > 2;
         G->L.push_back(L);
-        if (2 * c1 * c2 <= cpk::kMgDenseMax) break;
-        const int d1 = cpk::mg_coarse_n(c1), d2 = cpk::mg_coarse_n(c2);
+        if (done) break;
+        const int d1 = cpk::mg_coarse_n(c1, L.co1), d2 = cpk::mg_coarse_n(c2, L.co2);
         if (d1 == c1 && d2 == c2) break;
+        if (d1 != c1) len1 *= 2.;
+        if (d2 != c2) len2 *= 2.;
         c1 = d1;
         c2 = d2;
     }

@@ -774,7 +774,9 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
         if (int s = reset_state(P)) return s;
     hipLaunchKernelGGL(cpk::k_rm_reset, dim3(1), dim3(1), 0, stream, P->rms);
     const int max_outer = opt->fixed_outer > 0 ? opt->fixed_outer : (elastic ? 1 : std::max(1, opt->max_outer));
-    const int maxit = opt->cg_maxit > 0 ? opt->cg_maxit : 1000000000;
+    // multigrid CG needs tens of iterations: a cap keeps a stagnating solve
+    // (FP32 on a badly conditioned level) from running on
+    const int maxit = opt->cg_maxit > 0 ? opt->cg_maxit : (P->precond == 2 ? 2000 : 1000000000);
     const double inner = opt->inner_tol > 0 ? opt->inner_tol : 1e-3;
     int rc = CP_OK;
     double prev_rel = 1.0;

@@ -1,0 +1,103 @@
+"""The 2M-quad plate (Swift hardening) and the 1M-quad graded ring at their
+full sizes, multigrid CG, mixed and FP64, through the C-ABI.  At these
+sizes the oracle would take hours, so the checks are size-independent:
+mixed against FP64 within 1e-6, global equilibrium of the internal forces
+against the applied load, the secant exit test met at every level, and the
+same plastic zone in both precisions.  The load paths are shortened (the
+plate to half its final load) to keep the test to seconds."""
+import math
+
+import numpy as np
+import pytest
+
+from cudaplasticity_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+
+E = 2.1e5
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = A.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def ramp(ctx, mesh, loads, mat, grid, nsteps, upto, final, prec, tol=1e-6):
+    P = A.Problem(ctx, mesh, loads, mat)
+    P.set_grid(*grid)
+    opt = A.options(cg_tol=1e-9, tol=tol, aitken=True, max_outer=500, precision=prec, precond=2, predictor=1,
+                    cg_forcing=0.1, inner_tol=1e-5)
+    per = []
+    for n in range(1, upto + 1):
+        o = A.Options.from_buffer_copy(opt)
+        o.warm = 1 if n > 1 else 0
+        if n == 1:
+            P.set_load(final * n / nsteps)
+        else:
+            P.advance_load(final * n / nsteps, 1)
+        per.append(P.solve(o))
+    print("increments", [(s.outer_iters, s.cg_iters, round(s.seconds, 3)) for s in per], flush=True)
+    return P, per
+
+
+def check_pair(mesh, mat, Pm, per_m, Pf, per_f, load_factor, loads, fixed_x_dofs, tol=1e-6):
+    assert all(s.status == A.CP_OK and s.rel_err <= tol for s in per_m + per_f)
+    eu, es = rel(Pm.field("u"), Pf.field("u")), rel(Pm.field("sigI"), Pf.field("sigI"))
+    print("mixed vs fp64: u %.2e sigma_i %.2e" % (eu, es), flush=True)
+    assert eu < 1e-6 and es < 1e-6
+    epsT = mat.sigmaT / mat.E
+    fm, ff = Pm.field("epsI") >= epsT, Pf.field("epsI") >= epsT
+    print("plastic elements", int(fm.sum()), "of", mesh.ne, "differing", int((fm != ff).sum()), flush=True)
+    assert (fm != ff).sum() <= 1e-4 * mesh.ne
+    # equilibrium: K(E_c, nu_c) u against the applied load on free DOFs, and
+    # the reactions on the fixed x DOFs against the applied x load
+    f = Pf.internal_force()
+    fext = load_factor * loads.rp
+    free = loads.mask.astype(bool)
+    r = np.linalg.norm((f - fext)[free]) / np.linalg.norm(fext[free])
+    assert not free[fixed_x_dofs].any()
+    react = f[fixed_x_dofs].sum()
+    xfree = np.zeros_like(free)
+    xfree[0::2] = free[0::2]
+    applied = fext[xfree].sum()
+    print("free-DOF residual %.2e, x reactions %.6e, applied x load %.6e" % (r, react, applied), flush=True)
+    assert r < 1e-5
+    assert abs(react + applied) <= 1e-5 * np.abs(fext[xfree]).sum()
+
+
+def test_plate_2000x1000_swift(ctx):
+    Nt, Nr = 2000, 1000
+    mesh = A.gen_quarter_plate(1.0, 0.1, Nt, Nr)
+    loads = A.Loads(mesh).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    mat = A.material(E, 0.3, 240.0, A.LAW_SWIFT, n=0.2)
+    upto = 20                                     # increments 1..20 of 40: half the final 300 MPa
+    Pm, per_m = ramp(ctx, mesh, loads, mat, (Nt + 1, Nr + 1), 40, upto, 1.0, A.MIXED)
+    Pf, per_f = ramp(ctx, mesh, loads, mat, (Nt + 1, Nr + 1), 40, upto, 1.0, A.FP64)
+    fixed_x = 2 * mesh.borders[3]                 # x = 0 edge: u_x fixed
+    check_pair(mesh, mat, Pm, per_m, Pf, per_f, upto / 40, loads, fixed_x)
+    Pm.close()
+    Pf.close()
+
+
+def test_ring_16700x60_graded(ctx):
+    Nt, Nr = 16700, 60
+    mesh = A.gen_quarter_ring(0.5, 1.0, Nt, Nr, 1.05)
+    p = 1.2 * 240.0 * math.log(2.0)               # 1.2 x the limit pressure sigma_Y ln(b/a)
+    loads = A.Loads(mesh).force(0, -p, 0.0, norm_orient=True).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    mat = A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101)
+    # past the limit pressure the whole ring flows on the hardening slope
+    # (E/101), so a stress-level difference of the exit tolerance shows up
+    # about a hundredfold in u: the secant loop runs to 1e-7 here for the
+    # 1e-6 comparison of u
+    Pm, per_m = ramp(ctx, mesh, loads, mat, (Nt + 1, Nr + 1), 30, 30, 1.0, A.MIXED, tol=1e-7)
+    Pf, per_f = ramp(ctx, mesh, loads, mat, (Nt + 1, Nr + 1), 30, 30, 1.0, A.FP64, tol=1e-7)
+    fixed_x = 2 * mesh.borders[1]                 # theta = pi/2 edge (x = 0): u_x fixed
+    check_pair(mesh, mat, Pm, per_m, Pf, per_f, 1.0, loads, fixed_x, tol=1e-7)
+    Pm.close()
+    Pf.close()
