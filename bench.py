@@ -195,9 +195,83 @@ def served_stream(A, ctx, args):
             "all_ok": ok, "results_equal_alone_bitwise": bool(same)}
 
 
+def other_workload(A, args):
+    """The 2M-quad plate (Swift law, 40 increments to 300 MPa) or the 1M-quad
+    graded quarter ring (linear hardening, 30 increments to 1.2 sigma_Y
+    ln(b/a) internal pressure): mixed and FP64 multigrid ramps, their
+    difference and the finest-level smoothing step; one JSON line."""
+    import math
+    E = 2.1e5
+    ctx = A.Context(0)
+    if args.workload == "c3":
+        nt, nr, nsteps, tol = 2000, 1000, 40, 1e-6
+        mesh = A.gen_quarter_plate(1.0, 0.1, nt, nr)
+        loads = A.Loads(mesh).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+        mat = A.material(E, 0.3, 240.0, A.LAW_SWIFT, n=0.2)
+        name = (f"plate with hole, quarter model {nt}x{nr} quads ({mesh.ne}), Swift hardening sigma_Y 240 MPa n 0.2, "
+                f"{nsteps} increments to 300 MPa, Aitken to {tol:g}")
+    else:
+        nt, nr, nsteps, tol = 16700, 60, 30, 1e-7
+        mesh = A.gen_quarter_ring(0.5, 1.0, nt, nr, 1.05)
+        p = 1.2 * 240.0 * math.log(2.0)
+        loads = A.Loads(mesh).force(0, -p, 0.0, norm_orient=True).fix_vert_axis(0.0).fix_hor_axis(0.0)
+        mat = A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101)
+        name = (f"thick ring a 0.5 b 1.0, quarter model {nt}x{nr} quads ({mesh.ne}) graded 1.05 per element radially "
+                f"(overall 1.05^59 = {1.05 ** 59:.1f}), linear hardening H E/100, {nsteps} increments to "
+                f"1.2 sigma_Y ln(b/a) = {p:.1f} MPa, Aitken to {tol:g}")
+    log(name)
+    grid = (nt + 1, nr + 1)
+    out = {}
+    for tag, prec in (("mixed", A.MIXED), ("fp64", A.FP64)):
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(*grid)
+        opt = A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=prec, precond=2,
+                        predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
+        t0 = time.perf_counter()
+        per = []
+        for n in range(1, nsteps + 1):
+            o = A.Options.from_buffer_copy(opt)
+            o.warm = 1 if n > 1 else 0
+            if n == 1:
+                P.set_load(n / nsteps)
+            else:
+                P.advance_load(n / nsteps, 1)
+            per.append(P.solve(o))
+            log(f"  {tag} increment {n}/{nsteps}: secant {per[-1].outer_iters} cg {per[-1].cg_iters} "
+                f"{per[-1].seconds:.3f} s")
+        out[tag] = (time.perf_counter() - t0, per, P)
+    (tm, per_m, Pm), (tf, per_f, Pf) = out["mixed"], out["fp64"]
+    nnz = scalar_nnz(A, Pm)
+    _, ms_s, fmt = bench_mg(A, Pm, A.MIXED, args.reps // 4)
+    byt = cheb_step_bytes(nnz, 2 * mesh.nn, 4)
+    epsT = mat.sigmaT / mat.E
+    line = {"metric": METRIC, "value": nsteps / tm, "unit": "load-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+            "ms_per_step": tm / nsteps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": name, "nodes": mesh.nn, "elements": mesh.ne, "dof": 2 * mesh.nn,
+                       "mg_levels": Pm.mg_levels()},
+            "fp64_load_steps_per_s": nsteps / tf, "speedup_vs_fp64": tf / tm,
+            "rel_err_vs_fp64": {"u": rel(Pm.field("u"), Pf.field("u")), "sigma_i": rel(Pm.field("sigI"), Pf.field("sigI")),
+                                "plastic_flags_equal": bool(np.array_equal(Pm.field("epsI") >= epsT,
+                                                                           Pf.field("epsI") >= epsT))},
+            "plastic_elements": int((Pm.field("epsI") >= epsT).sum()),
+            "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
+                           "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
+            "roofline": {"bound": "hbm", "achieved": byt / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": byt / ms_s / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "cpk::k_mg_smooth<9,1,float> (finest-level Chebyshev smoothing step)",
+                         "bytes_per_launch": byt, "avg_launch_ms": ms_s, "format_bytes_per_launch": fmt,
+                         "achieved_format": fmt / ms_s / 1e6},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
+                    help="c2: the bench line (632x316 plate); c3: 2000x1000 plate, Swift law, 40 increments; "
+                         "c4: 16700x60 graded quarter ring, 30 increments")
     ap.add_argument("--steps", type=int, default=NSTEPS)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nt", type=int, default=632)
@@ -231,6 +305,10 @@ def main():
               "stream would shard over several GPUs is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
     K = max(1, min(args.steps, NSTEPS))
+    if args.workload != "c2":
+        from cudaplasticity_amd import abi as A
+        other_workload(A, args)
+        return
     if args.copy_calibration > 0:
         import torch
         n = int(args.copy_calibration * (1 << 20)) // 4
