@@ -111,6 +111,13 @@ def cheb_step_bytes(nnz, N, s):
     return nnz * (s + 4) + 4 * (N + 1) + 2 * N * s + N * s + N * s + 1.5 * N * s + N * s + N
 
 
+def smooth_kernel(P):
+    """Name and traffic key of the finest level's Chebyshev step kernel."""
+    if P.mg_smoothers()[0] == 1:
+        return "cpk::k_mg_smooth_seg<9,1,float>", "mg_smooth_seg"
+    return "cpk::k_mg_smooth<9,1,float>", "mg_smooth"
+
+
 def bench_mg(A, P, precision, reps):
     import ctypes as C
     v, sm, b = C.c_double(), C.c_double(), C.c_double()
@@ -259,7 +266,7 @@ def other_workload(A, args):
                            "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
             "roofline": {"bound": "hbm", "achieved": byt / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": byt / ms_s / 1e6 / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "cpk::k_mg_smooth<9,1,float> (finest-level Chebyshev smoothing step)",
+                         "kernel": smooth_kernel(Pm)[0] + " (finest-level Chebyshev smoothing step)",
                          "bytes_per_launch": byt, "avg_launch_ms": ms_s, "format_bytes_per_launch": fmt,
                          "achieved_format": fmt / ms_s / 1e6},
             "cpu_baseline": None}
@@ -336,7 +343,7 @@ def main():
         if args.precond == 2:
             P.set_grid(*grid)
             _, ms, fmt = bench_mg(A, P, A.MIXED, args.reps)
-            name, byt = "cpk::k_mg_smooth<9,1,float>", cheb_step_bytes(nnz, N, 4)
+            name, byt = smooth_kernel(P)[0], cheb_step_bytes(nnz, N, 4)
         else:
             ms, fmt = bench_cg(A, P, A.MIXED, args.reps)
             name, byt = "cpk::k_cg_iter_fused<9,true,float>", cg_iteration_bytes(nnz, N, 4)
@@ -406,7 +413,9 @@ def main():
         mv32, ms_s32, fmt_s32 = bench_mg(A, Pmx, A.MIXED, reps)
         mv64, ms_s64, fmt_s64 = bench_mg(A, P64, A.FP64, reps)
         bs32, bs64 = cheb_step_bytes(nnz, 2 * mesh.nn, 4), cheb_step_bytes(nnz, 2 * mesh.nn, 8)
-        mg = {"levels": Pmx.mg_levels(), "vcycle_ms": {"mixed": mv32, "fp64": mv64},
+        mg = {"levels": Pmx.mg_levels(),
+              "smoothers": ["segments" if k else "block-Jacobi" for k in Pmx.mg_smoothers()[:-1]] + ["exact"],
+              "vcycle_ms": {"mixed": mv32, "fp64": mv64},
               "smooth_step_ms": {"mixed": ms_s32, "fp64": ms_s64},
               "smooth_step_gbps": {"mixed": bs32 / ms_s32 / 1e6, "fp64": bs64 / ms_s64 / 1e6},
               "smooth_step_format_gbps": {"mixed": fmt_s32 / ms_s32 / 1e6, "fp64": fmt_s64 / ms_s64 / 1e6},
@@ -482,8 +491,8 @@ def main():
                        "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
         "roofline": ({"bound": "hbm", "achieved": bs32 / ms_s32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                       "frac": bs32 / ms_s32 / 1e6 / HBM_PEAK_GBS,
-                      "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, "mg_smooth")),
-                      "kernel": "cpk::k_mg_smooth<9,1,float> (finest-level Chebyshev smoothing step)",
+                      "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, smooth_kernel(Pmx)[1])),
+                      "kernel": smooth_kernel(Pmx)[0] + " (finest-level Chebyshev smoothing step)",
                       "bytes_per_launch": bs32, "avg_launch_ms": ms_s32, "reps": reps,
                       "format_bytes_per_launch": fmt_s32, "achieved_format": fmt_s32 / ms_s32 / 1e6}
                      if mg else

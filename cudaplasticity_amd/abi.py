@@ -305,6 +305,16 @@ class Problem:
         check(lib().cp_problem_mg_levels(self.ptr, C.byref(n), dims))
         return [(dims[2 * i], dims[2 * i + 1]) for i in range(n.value)]
 
+    def set_mg_rebuild(self, every):
+        check(lib().cp_problem_set_mg_rebuild(self.ptr, int(every)))
+
+    def mg_smoothers(self):
+        """Per level: 0 = 2x2 block Jacobi, 1 = segments along direction 1."""
+        n = len(self.mg_levels())
+        kinds = (C.c_int * max(n, 1))()
+        check(lib().cp_problem_mg_smoothers(self.ptr, kinds))
+        return [kinds[i] for i in range(n)]
+
     def set_load(self, factor):
         check(lib().cp_set_load(self.ptr, _f(factor)))
 

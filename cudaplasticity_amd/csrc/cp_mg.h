@@ -13,10 +13,13 @@
 // on them, and a coarse DOF that only interpolates to fixed DOFs gets a unit
 // diagonal and mask 0.
 //
-// Smoothing: Chebyshev iteration of degree 3 on the block-Jacobi scaled
-// operator over [lam/30, 1.1 lam], lam = the Gershgorin bound of
-// |D^-1 A| (no eigenvalue iteration).  Pre- and post-smoothing are the same
-// polynomial, so the V-cycle is a symmetric operator and can precondition CG.
+// Smoothing: Chebyshev iteration of degree 3 on D^-1 A over
+// [lam/30, 1.1 lam], D the 2x2 block diagonal with lam the Gershgorin bound
+// of |D^-1 A| (no eigenvalue iteration), or, on levels whose cells are
+// stretched along direction 1, D the block-tridiagonal part along segments
+// of that direction with lam = 4 (segment smoother below).  Pre- and
+// post-smoothing are the same polynomial, so the V-cycle is a symmetric
+// operator and can precondition CG.
 // The coarsest level (at most kMgDenseMax DOFs) is solved with an explicit
 // inverse formed by Gauss-Jordan elimination in one workgroup.
 //
@@ -67,7 +70,7 @@ __device__ __forceinline__ double mg_weight(int n, int co, int f, int K)
 }
 
 // Fine position of coarse point K.
-__device__ __forceinline__ int mg_pos(int n, int co, int K)
+__host__ __device__ inline int mg_pos(int n, int co, int K)
 {
     return (n <= 2 || !co) ? K : (2 * K < n - 1 ? 2 * K : n - 1);
 }
@@ -311,50 +314,58 @@ __global__ __launch_bounds__(kBlock) void k_mg_dense(CgArgs<T> a, double* D, int
 
 // In-place inverse of the SPD matrix D (nd x nd, nd <= kMgDenseMax) by
 // Gauss-Jordan elimination without pivoting, in one workgroup of 1024
-// threads: thread (j = t & 255, i0 = t >> 8) keeps D[i][j] for the rows
-// i = i0, i0 + 4, ... in registers; each step broadcasts the pivot row and
-// column through LDS.
+// threads: thread (j = t % 128, i0 = t / 128) keeps D[i][j] for the rows
+// i = i0, i0 + 8, ... in registers; each step broadcasts the pivot row and
+// column through LDS.  The update is written without branches (selects),
+// so the LDS reads of a step are issued together.
 static __global__ __launch_bounds__(1024) void k_mg_invert(double* D, int nd)
 {
-    constexpr int R = kMgDenseMax / 4;
+    constexpr int G = 8, R = kMgDenseMax / G;
     __shared__ double colk[kMgDenseMax];
     __shared__ double rowk[kMgDenseMax];
-    const int j = threadIdx.x & 255, i0 = threadIdx.x >> 8;
+    const int j = threadIdx.x % kMgDenseMax, i0 = threadIdx.x / kMgDenseMax;
     const bool active = j < nd;
+    if (threadIdx.x < kMgDenseMax) colk[threadIdx.x] = rowk[threadIdx.x] = 0.;
     double v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int i = i0 + 4 * r;
+        const int i = i0 + G * r;
         v[r] = (active && i < nd) ? D[(size_t)i * nd + j] : 0.;
     }
+    __syncthreads();
     for (int k = 0; k < nd; ++k) {
-        const int rk = k >> 2, ik = k & 3;
         // column k and row k of the current matrix to LDS
+        if (j == k) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = i0 + 4 * r;
-            if (active && j == k && i < nd) colk[i] = v[r];
-            if (active && i0 == ik && r == rk) rowk[j] = v[r];
+            for (int r = 0; r < R; ++r) colk[i0 + G * r] = v[r];
+        }
+        if (active && i0 == k % G) {
+            const int rk = k / G;
+            double t = 0.;
+#pragma unroll
+            for (int r = 0; r < R; ++r) t = r == rk ? v[r] : t;
+            rowk[j] = t;
         }
         __syncthreads();
+        double c[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) c[r] = colk[i0 + G * r];
         const double dk = colk[k];
         const double p = dk != 0. ? 1. / dk : 0.;
-        if (active) {
-            const double rkj = rowk[j] * p;
+        const double rkj = rowk[j] * p;
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int i = i0 + 4 * r;
-                if (i >= nd) continue;
-                if (i == k) v[r] = j == k ? p : rkj;                 // row k: scaled; pivot: 1 / d
-                else if (j == k) v[r] = -colk[i] * p;                 // column k
-                else v[r] -= colk[i] * rkj;                           // the rest
-            }
+        for (int r = 0; r < R; ++r) {
+            const int i = i0 + G * r;
+            double nv = v[r] - c[r] * rkj;              // the rest
+            nv = j == k ? -c[r] * p : nv;               // column k
+            nv = i == k ? (j == k ? p : rkj) : nv;      // row k: scaled; pivot: 1 / d
+            v[r] = (active && i < nd) ? nv : v[r];
         }
         __syncthreads();
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-        const int i = i0 + 4 * r;
+        const int i = i0 + G * r;
         if (active && i < nd) D[(size_t)i * nd + j] = v[r];
     }
 }
@@ -450,6 +461,260 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(CgArgs<T> a, MgVec<T> v, c
     const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
     if (n >= a.nn) return;
     mg_smooth_node<W, MODE, T>(a, v, xi, di, xo, dout, second, last, n);
+}
+
+// ------------------------------------------------------------ segment smoother
+//
+// Where elements are stretched (short edges along lattice direction 1) the
+// matrix couples strongly along that direction and a point smoother leaves
+// errors that are smooth across the weak direction untouched.  On such
+// levels D is the block-tridiagonal part of A along segments of kSeg nodes
+// of direction 1 (segment s of lattice column i2: nodes i1 = s kSeg ..
+// s kSeg + kSeg - 1), so one application of D^-1 solves kSeg coupled nodes
+// at once.  Per node the set-up stores the inverse of the pivot block of
+// the block-LU factorisation of its segment,
+//   Dt_0 = D_0,  Dt_k = D_k - L_k Dt_{k-1}^-1 L_k^T   (L_k = A(k, k-1)),
+// in the minv slot, and the solve D z = rho is
+//   y_k = Dt_k^-1 rho_k - (Dt_k^-1 L_k) y_{k-1},   z_k = y_k - (Dt_k^-1 A(k, k+1)) z_{k+1}.
+// The segments coloured by (s mod 2, i2 mod 2) are uncoupled within a
+// colour, so lambda_max(D^-1 A) <= 4: the Chebyshev interval is
+// [4/30, 4.4] with no eigenvalue estimate.
+constexpr int kSeg = 16;           // segment length (direction 1) and tile height
+constexpr int kSegCols = kBlock / kSeg;   // tile width (direction 2): 16 columns
+constexpr double kSegLam = 4.;
+
+struct SegGrid {
+    int n1, n2;              // lattice of the level
+    int nt2, ntiles;         // tiles along direction 2, tiles in all
+    const int* node_of;      // lattice index -> node (level 0 only; nullptr: identity)
+    const signed char* slot_pos;   // level 0: node x 9 lattice offsets -> row position
+};
+
+__device__ __forceinline__ int seg_node(const SegGrid& g, int i1, int i2)
+{
+    const int li = i1 * g.n2 + i2;
+    return g.node_of ? g.node_of[li] : li;
+}
+
+// Row position of the coupling of node n to its lattice neighbour at
+// (d1, d2) (always present for lattice neighbours along direction 1).
+__device__ __forceinline__ int seg_pos(const SegGrid& g, int n, int d1, int d2)
+{
+    const int so = mg_slot(d1, d2);
+    return g.slot_pos ? g.slot_pos[(size_t)n * 9 + so] : so;
+}
+
+template <typename T>
+__device__ __forceinline__ void seg_block(const CgArgs<T>& a, const SegGrid& g, int n, int d1, double (&m)[4])
+{
+    const int pos = seg_pos(g, n, d1, 0);
+    if (pos < 0) {   // not coupled in the matrix
+        m[0] = m[1] = m[2] = m[3] = 0.;
+        return;
+    }
+    const int2 sl = a.slice[n >> 6];
+    load_block_d(a, (size_t)sl.x + (n & 63) + (size_t)pos * kWave, m[0], m[1], m[2], m[3]);
+}
+
+// Pivot inverses of every segment (one thread per segment column), and
+// lambda = 4 for the Chebyshev coefficients.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g, typename V4<T>::type* minv,
+                                                         double* lam)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *lam = kSegLam;
+    const int idx = blockIdx.x * kBlock + threadIdx.x;
+    const int nseg = (g.n1 + kSeg - 1) / kSeg;
+    if (idx >= nseg * g.n2) return;
+    const int s = idx / g.n2, i2 = idx - s * g.n2;
+    double p00 = 0., p01 = 0., p11 = 0.;   // Dt_{k-1}^-1 (zero on fixed DOFs)
+    uchar2 mp = make_uchar2(0, 0);
+    // rows in chunks of kChunk: the loads of a chunk are issued before its
+    // (dependent) eliminations and stores
+    constexpr int kChunk = 4;
+    for (int k0 = 0; k0 < kSeg; k0 += kChunk) {
+        double d[kChunk][4], l[kChunk][4];
+        uchar2 mk[kChunk];
+        int nodes[kChunk];
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            const int i1 = s * kSeg + k0 + c;
+            nodes[c] = i1 < g.n1 ? seg_node(g, i1, i2) : -1;
+            if (nodes[c] < 0) continue;
+            mk[c] = a.mask[nodes[c]];
+            seg_block(a, g, nodes[c], 0, d[c]);
+            if (k0 + c > 0) seg_block(a, g, nodes[c], -1, l[c]);
+            else l[c][0] = l[c][1] = l[c][2] = l[c][3] = 0.;
+        }
+#pragma unroll
+        for (int c = 0; c < kChunk; ++c) {
+            const int n = nodes[c];
+            if (n < 0) break;
+            const uchar2 m = mk[c];
+            double t00 = d[c][0], t01 = 0.5 * (d[c][1] + d[c][2]), t11 = d[c][3];
+            // L masked by this node's (rows) and the previous node's (columns) fixed DOFs
+            const double l00 = (m.x && mp.x) ? l[c][0] : 0., l01 = (m.x && mp.y) ? l[c][1] : 0.;
+            const double l10 = (m.y && mp.x) ? l[c][2] : 0., l11 = (m.y && mp.y) ? l[c][3] : 0.;
+            // L P L^T with P = Dt_{k-1}^-1 symmetric
+            const double q00 = l00 * p00 + l01 * p01, q01 = l00 * p01 + l01 * p11;
+            const double q10 = l10 * p00 + l11 * p01, q11 = l10 * p01 + l11 * p11;
+            t00 -= q00 * l00 + q01 * l01;
+            t01 -= q00 * l10 + q01 * l11;
+            t11 -= q10 * l10 + q11 * l11;
+            double i00 = 0., i01 = 0., i11 = 0.;
+            if (m.x && m.y) {
+                const double det = t00 * t11 - t01 * t01;
+                i00 = t11 / det;
+                i01 = -t01 / det;
+                i11 = t00 / det;
+            } else if (m.x) {
+                i00 = 1. / t00;
+            } else if (m.y) {
+                i11 = 1. / t11;
+            }
+            minv[n] = typename V4<T>::type{(T)i00, (T)i01, (T)i11, (T)0};
+            p00 = i00;
+            p01 = i01;
+            p11 = i11;
+            mp = m;
+        }
+    }
+}
+
+// Smoothing steps of a segment level (same Chebyshev recurrences as
+// mg_smooth_node, D^-1 applied by the segment solve):
+//   SEG_X0    xo = c0 D^-1 b                      (x1 of the pre-smoothing)
+//   SEG_FIRST x = xi                              -> do = a1 x + b1 D^-1 (b - A x)
+//   MG_CHEB   x = xi + di                         -> xo = x (or x + d if last), do = a di + c D^-1 (b - A x)
+//   MG_POST1  x = xi                              -> do = c0 D^-1 (b - A x)
+// (the residual step MG_RESID needs no D^-1 and runs in k_mg_smooth).
+// One workgroup per tile of kSeg x kSegCols lattice nodes: every thread
+// forms its node's residual and the per-node factors, then one thread per
+// tile column runs the two short recurrences of its segment in LDS.
+enum { SEG_X0 = 4, SEG_FIRST = 5 };
+
+template <int W, int MODE, typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
+                                                          const typename V2<T>::type* di, typename V2<T>::type* xo,
+                                                          typename V2<T>::type* dout, int second, int last, SegGrid g)
+{
+    using V = typename V2<T>::type;
+    if (v.sc && v.sc->done) return;
+    __shared__ T sy[2][kBlock];    // Dt^-1 rho, then y, then z
+    __shared__ T sf[4][kBlock];    // Dt^-1 L
+    __shared__ T sh[4][kBlock];    // Dt^-1 U
+    const int tile = xcd_block(g.ntiles);
+    const int T1 = tile / g.nt2, T2 = tile - T1 * g.nt2;
+    const int t1 = threadIdx.x / kSegCols, t2 = threadIdx.x - t1 * kSegCols;
+    const int i1 = T1 * kSeg + t1, i2 = T2 * kSegCols + t2;
+    const bool in = i1 < g.n1 && i2 < g.n2;
+    const int n = in ? seg_node(g, i1, i2) : 0;
+    const ChebCoef<T> c = mg_cheb<T>(*v.lam);
+    T rx = 0, ry = 0;
+    V xn = V{0, 0}, dn = V{0, 0};
+    T f[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
+    T ax = 0, ay = 0;
+    if (in) {
+        const uchar2 mk = a.mask[n];
+        const V bn = v.b[n];
+        if constexpr (MODE == SEG_X0) {
+            rx = mk.x ? bn.x : (T)0;
+            ry = mk.y ? bn.y : (T)0;
+        } else {
+            T yx, yy;
+            row_product<W, T>(a, n,
+                              [&](int m, T& zx, T& zy) {
+                                  if constexpr (MODE == MG_CHEB) {
+                                      const V xm = xi[m], dm = di[m];
+                                      zx = xm.x + dm.x;
+                                      zy = xm.y + dm.y;
+                                  } else {
+                                      const V xm = xi[m];
+                                      zx = xm.x;
+                                      zy = xm.y;
+                                  }
+                              },
+                              yx, yy);
+            rx = mk.x ? bn.x - yx : (T)0;
+            ry = mk.y ? bn.y - yy : (T)0;
+            xn = xi[n];
+            if constexpr (MODE == MG_CHEB) dn = di[n];
+        }
+        const typename V4<T>::type mi = a.minv[n];
+        apply_minv<T>(mi, rx, ry, ax, ay);
+        // Dt^-1 times the couplings to the segment neighbours.  Rows of
+        // fixed DOFs are zero in Dt^-1, so y and z vanish on fixed DOFs and
+        // the columns of the neighbours' fixed DOFs need no masking.
+        if (t1 > 0) {
+            double l[4];
+            seg_block(a, g, n, -1, l);
+            f[0] = mi.x * (T)l[0] + mi.y * (T)l[2];
+            f[1] = mi.x * (T)l[1] + mi.y * (T)l[3];
+            f[2] = mi.y * (T)l[0] + mi.z * (T)l[2];
+            f[3] = mi.y * (T)l[1] + mi.z * (T)l[3];
+        }
+        if (t1 < kSeg - 1 && i1 + 1 < g.n1) {
+            double u[4];
+            seg_block(a, g, n, 1, u);
+            h[0] = mi.x * (T)u[0] + mi.y * (T)u[2];
+            h[1] = mi.x * (T)u[1] + mi.y * (T)u[3];
+            h[2] = mi.y * (T)u[0] + mi.z * (T)u[2];
+            h[3] = mi.y * (T)u[1] + mi.z * (T)u[3];
+        }
+    }
+    const int tid = threadIdx.x;
+    sy[0][tid] = ax;
+    sy[1][tid] = ay;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        sf[j][tid] = f[j];
+        sh[j][tid] = h[j];
+    }
+    __syncthreads();
+    if (tid < kSegCols) {
+        T yx = 0, yy = 0;
+#pragma unroll 2
+        for (int k = 0; k < kSeg; ++k) {
+            const int q = k * kSegCols + tid;
+            const T nx = sy[0][q] - (sf[0][q] * yx + sf[1][q] * yy);
+            const T ny = sy[1][q] - (sf[2][q] * yx + sf[3][q] * yy);
+            yx = nx;
+            yy = ny;
+            sy[0][q] = yx;
+            sy[1][q] = yy;
+        }
+        T zx = 0, zy = 0;
+#pragma unroll 2
+        for (int k = kSeg - 1; k >= 0; --k) {
+            const int q = k * kSegCols + tid;
+            const T nx = sy[0][q] - (sh[0][q] * zx + sh[1][q] * zy);
+            const T ny = sy[1][q] - (sh[2][q] * zx + sh[3][q] * zy);
+            zx = nx;
+            zy = ny;
+            sy[0][q] = zx;
+            sy[1][q] = zy;
+        }
+    }
+    __syncthreads();
+    if (!in) return;
+    const T gx = sy[0][tid], gy = sy[1][tid];
+    if constexpr (MODE == SEG_X0) {
+        xo[n] = V{c.c0 * gx, c.c0 * gy};
+    } else if constexpr (MODE == SEG_FIRST) {
+        dout[n] = V{c.a1 * xn.x + c.b1 * gx, c.a1 * xn.y + c.b1 * gy};
+    } else if constexpr (MODE == MG_POST1) {
+        dout[n] = V{c.c0 * gx, c.c0 * gy};
+    } else {
+        const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
+        const V x = V{xn.x + dn.x, xn.y + dn.y};
+        const V d = V{ca * dn.x + cb * gx, ca * dn.y + cb * gy};
+        if (last) {
+            xo[n] = V{x.x + d.x, x.y + d.y};
+        } else {
+            xo[n] = x;
+            dout[n] = d;
+        }
+    }
 }
 
 struct MgGrid {

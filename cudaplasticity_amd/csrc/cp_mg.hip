@@ -14,6 +14,7 @@ namespace cp {
 struct MgLevel {
     int n1 = 0, n2 = 0, nn = 0, nb = 0, nslices = 0;
     int co1 = 1, co2 = 1;    // directions halved from this level to the next
+    int seg = 0;             // 1: segment smoother along direction 1, else 2x2 block Jacobi
     int2* slice = nullptr;   // levels >= 1 (level 0 uses the problem's matrix)
     int* col = nullptr;
     uchar2* mask = nullptr;
@@ -40,6 +41,12 @@ struct MgState {
     bool use_graphs = true;
     // levels lt.. (at most kTailThreads nodes each) run in one workgroup
     int lt = 0;
+    // coarse levels rebuilt every `rebuild` set-ups (1: every assembly; a
+    // plate ramp runs 15 % faster at 8 than at 1 with 3 % more CG iterations);
+    // set-ups since the last rebuild, per precision
+    int rebuild = 8;
+    int since[2] = {0, 0};
+    bool built[2] = {false, false};
     void* tail[2] = {nullptr, nullptr};
 
     template <typename X>
@@ -69,6 +76,8 @@ void mg_destroy(cp_problem* P)
 }
 
 static int blocks_of(int n) { return (n + cpk::kBlock - 1) / cpk::kBlock; }
+
+constexpr double kSegFrac = 0.05;   // share of stretched cells that selects the segment smoother
 
 // Per-precision value and vector arrays, allocated on first use.
 static int mg_alloc(cp_problem* P, int prec)
@@ -188,13 +197,26 @@ float4* level_minv<float>(cp_problem* P, int l)
     return l == 0 ? P->minv32 : P->mg->L[l].minv32;
 }
 
+static cpk::SegGrid seg_grid(const MgState& G, int l)
+{
+    const MgLevel& L = G.L[l];
+    cpk::SegGrid g{};
+    g.n1 = L.n1;
+    g.n2 = L.n2;
+    g.nt2 = (L.n2 + cpk::kSegCols - 1) / cpk::kSegCols;
+    g.ntiles = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * g.nt2;
+    g.node_of = l == 0 ? G.node0 : nullptr;
+    g.slot_pos = l == 0 ? G.slot0 : nullptr;
+    return g;
+}
+
 template <typename T>
-static int mg_setup_t(cp_problem* P)
+static int mg_setup_t(cp_problem* P, int nlev)
 {
     MgState& G = *P->mg;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     hipStream_t st = P->stream;
-    for (int l = 0; l < G.nlev; ++l) {
+    for (int l = 0; l < nlev; ++l) {
         const MgLevel& L = G.L[l];
         cpk::CgArgs<T> a = level_args<T>(P, l);
         if (l > 0) {
@@ -222,7 +244,11 @@ static int mg_setup_t(cp_problem* P)
             g.mask_c = L.mask;
             hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
         }
-        if (l < G.nlev - 1) {
+        if (l < G.nlev - 1 && L.seg) {
+            const int cols = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * L.n2;
+            hipLaunchKernelGGL(cpk::k_mg_seg_setup<T>, dim3(blocks_of(cols)), dim3(cpk::kBlock), 0, st, a,
+                               seg_grid(G, l), level_minv<T>(P, l), L.lam + pi);
+        } else if (l < G.nlev - 1) {
             hipLaunchKernelGGL(cpk::k_mg_smooth_setup<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a,
                                level_minv<T>(P, l), P->partials);
             hipLaunchKernelGGL(cpk::k_mg_lam, dim3(1), dim3(cpk::kBlock), 0, st, (const double*)P->partials, L.nb,
@@ -236,12 +262,25 @@ static int mg_setup_t(cp_problem* P)
     return CP_OK;
 }
 
-int mg_setup(cp_problem* P, int precision)
+int mg_setup(cp_problem* P, int precision, int full)
 {
     if (!P->mg) return fail(CP_ERR_STATE, "multigrid needs the mesh lattice: call cp_problem_set_grid first");
     if (int s = mg_alloc(P, precision)) return s;
     if (int s = precision == CP_FP64 ? mg_tail_table<double>(P) : mg_tail_table<float>(P)) return s;
-    return precision == CP_FP64 ? mg_setup_t<double>(P) : mg_setup_t<float>(P);
+    MgState& G = *P->mg;
+    const int pi = precision == CP_FP64 ? 0 : 1;
+    // a reused coarse hierarchy stays a symmetric positive definite
+    // preconditioner (each level's smoother is set up from that level's own
+    // operator); only its quality depends on how far the moduli have moved
+    const bool all = full || !G.built[pi] || G.since[pi] + 1 >= G.rebuild;
+    if (all) {
+        G.built[pi] = true;
+        G.since[pi] = 0;
+    } else {
+        ++G.since[pi];
+    }
+    const int nl = all ? G.nlev : 1;
+    return precision == CP_FP64 ? mg_setup_t<double>(P, nl) : mg_setup_t<float>(P, nl);
 }
 
 template <int W, int MODE, typename T>
@@ -267,6 +306,36 @@ static void smooth_level_up(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, hip
     smooth<W, cpk::MG_POST1, T>(a, v, v.xb, nullptr, nullptr, v.da, 0, 0, st);
     smooth<W, cpk::MG_CHEB, T>(a, v, v.xb, v.da, v.xa, v.db, 0, 0, st);
     smooth<W, cpk::MG_CHEB, T>(a, v, v.xa, v.db, v.xb, nullptr, 1, 1, st);
+}
+
+template <int W, int MODE, typename T>
+static void smooth_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi, const void* di, void* xo,
+                       void* dout, int second, int last, const cpk::SegGrid& g, hipStream_t st)
+{
+    using V = typename cpk::V2<T>::type;
+    hipLaunchKernelGGL((cpk::k_mg_smooth_seg<W, MODE, T>), dim3(g.ntiles), dim3(cpk::kBlock), 0, st, a, v,
+                       (const V*)xi, (const V*)di, (V*)xo, (V*)dout, second, last, g);
+}
+
+// The same steps with the segment smoother: x1 = c0 D^-1 b takes a launch
+// of its own (D^-1 b at a neighbour is not local any more).
+template <int W, typename T>
+static void smooth_level_down_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g,
+                                  hipStream_t st)
+{
+    smooth_seg<W, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
+    smooth_seg<W, cpk::SEG_FIRST, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g, st);
+    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, g, st);
+    smooth<W, cpk::MG_RESID, T>(a, v, v.xb, v.db, v.xa, nullptr, 0, 0, st);
+}
+
+template <int W, typename T>
+static void smooth_level_up_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g,
+                                hipStream_t st)
+{
+    smooth_seg<W, cpk::MG_POST1, T>(a, v, v.xb, nullptr, nullptr, v.da, 0, 0, g, st);
+    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xb, v.da, v.xa, v.db, 0, 0, g, st);
+    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.db, v.xb, nullptr, 1, 1, g, st);
 }
 
 template <typename T, typename F>
@@ -315,7 +384,13 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
     for (int l = 0; l < last; ++l) {
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
-        by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
+        if (G.L[l].seg) {
+            const cpk::SegGrid sg = seg_grid(G, l);
+            by_width<T>(l == 0 ? P->max_w : 9,
+                        [&](auto w) { smooth_level_down_seg<decltype(w)::value, T>(a, v, sg, st); });
+        } else {
+            by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
+        }
         const MgLevel& C = G.L[l + 1];
         hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(C.nb), dim3(cpk::kBlock), 0, st, grid(l), (const V*)v.res,
                            (V*)C.vec[pi][0], (const uchar2*)C.mask, C.nn, C.nb, sc);
@@ -329,7 +404,13 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         const MgLevel& C = G.L[l + 1];
         hipLaunchKernelGGL(cpk::k_mg_prolong<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, grid(l),
                            (const V*)C.vec[pi][2], (const V*)v.xa, v.xb, (const uchar2*)a.mask, L.nn, L.nb, sc);
-        by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
+        if (L.seg) {
+            const cpk::SegGrid sg = seg_grid(G, l);
+            by_width<T>(l == 0 ? P->max_w : 9,
+                        [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(a, v, sg, st); });
+        } else {
+            by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
+        }
     }
 }
 
@@ -463,13 +544,21 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     v.db = (V*)L.vec[pi][4];
     v.res = (V*)L.vec[pi][5];
     v.lam = L.lam + pi;
+    const bool seg = L.seg != 0;
+    const cpk::SegGrid sg = seg_grid(G, 0);
     by_width<T>(P->max_w, [&](auto w) {
-        smooth<decltype(w)::value, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
+        if (seg) {
+            smooth_seg<decltype(w)::value, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, sg, st);
+            smooth_seg<decltype(w)::value, cpk::SEG_FIRST, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, sg, st);
+        } else {
+            smooth<decltype(w)::value, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
+        }
     });
     CP_HIP(hipEventRecord(P->ev[4], st));
     for (int i = 0; i < reps; ++i)
         by_width<T>(P->max_w, [&](auto w) {
-            smooth<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
+            if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg, st);
+            else smooth<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
         });
     CP_HIP(hipEventRecord(P->ev[5], st));
     for (int i = 0; i < reps; ++i) mg_vcycle<T>(P, a.r[0], nullptr);
@@ -539,22 +628,22 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     // couples strongly across its short edges, so only that direction is
     // halved until the two are within a factor 2 (semi-coarsening)
     double len1 = 0., len2 = 0.;
+    std::vector<double2> pos(nn);   // node coordinates in lattice order
     {
         std::vector<double2> xy(nn);
         CP_HIP(hipMemcpy(xy.data(), P->xy, sizeof(double2) * nn, hipMemcpyDeviceToHost));
-        std::vector<int> no(nn);
-        for (int n = 0; n < nn; ++n) no[lg[n]] = n;
+        for (int n = 0; n < nn; ++n) pos[lg[n]] = xy[n];
         long long c1n = 0, c2n = 0;
         for (int i1 = 0; i1 < n1; ++i1)
             for (int i2 = 0; i2 < n2; ++i2) {
-                const double2 p = xy[no[i1 * n2 + i2]];
+                const double2 p = pos[i1 * n2 + i2];
                 if (i1 + 1 < n1) {
-                    const double2 q = xy[no[(i1 + 1) * n2 + i2]];
+                    const double2 q = pos[(i1 + 1) * n2 + i2];
                     len1 += std::hypot(q.x - p.x, q.y - p.y);
                     ++c1n;
                 }
                 if (i2 + 1 < n2) {
-                    const double2 q = xy[no[i1 * n2 + i2 + 1]];
+                    const double2 q = pos[i1 * n2 + i2 + 1];
                     len2 += std::hypot(q.x - p.x, q.y - p.y);
                     ++c2n;
                 }
@@ -587,6 +676,34 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     G->nlev = (int)G->L.size();
     G->lt = G->nlev - 1;
     while (G->lt > 1 && G->L[G->lt - 1].nn <= cpk::kTailThreads) --G->lt;
+    // smoother of each level: fully coarsened levels (outside the one-
+    // workgroup tail) where at least kSegFrac of the cells are stretched
+    // along direction 1 (mean direction-1 edge below half the direction-2
+    // edge) use the segment smoother; coarse nodes sit on their fine
+    // positions, so the cells of every level are measured on the mesh
+    for (int l = 0; l < G->lt; ++l) {
+        MgLevel& L = G->L[l];
+        long long cells = 0, stretched = 0;
+        for (int i1 = 0; i1 + 1 < L.n1; ++i1)
+            for (int i2 = 0; i2 + 1 < L.n2; ++i2) {
+                auto at = [&](int a1, int a2) { return pos[a1 * L.n2 + a2]; };
+                auto len = [](double2 p, double2 q) { return std::hypot(q.x - p.x, q.y - p.y); };
+                const double h1 = 0.5 * (len(at(i1, i2), at(i1 + 1, i2)) + len(at(i1, i2 + 1), at(i1 + 1, i2 + 1)));
+                const double h2 = 0.5 * (len(at(i1, i2), at(i1, i2 + 1)) + len(at(i1 + 1, i2), at(i1 + 1, i2 + 1)));
+                ++cells;
+                stretched += h1 < 0.5 * h2;
+            }
+        L.seg = L.co1 && L.co2 && cells > 0 && stretched >= kSegFrac * cells;
+        if (l + 1 < G->nlev) {
+            const MgLevel& C = G->L[l + 1];
+            std::vector<double2> pc((size_t)C.nn);
+            for (int K1 = 0; K1 < C.n1; ++K1)
+                for (int K2 = 0; K2 < C.n2; ++K2)
+                    pc[(size_t)K1 * C.n2 + K2] =
+                        pos[cpk::mg_pos(L.n1, L.co1, K1) * L.n2 + cpk::mg_pos(L.n2, L.co2, K2)];
+            pos.swap(pc);
+        }
+    }
     if (G->nlev < 2 || 2 * G->L.back().nn > cpk::kMgDenseMax) {
         mg_destroy(P);
         return fail(CP_ERR_INVALID, "cp_problem_set_grid: lattice too small or too thin for multigrid");
@@ -677,6 +794,22 @@ int cp_problem_mg_levels(cp_problem* P, int* levels, int* dims)
             dims[2 * l] = P->mg->L[l].n1;
             dims[2 * l + 1] = P->mg->L[l].n2;
         }
+    return CP_OK;
+}
+
+int cp_problem_set_mg_rebuild(cp_problem* P, int every)
+{
+    if (!P || every < 1) return cp::fail(CP_ERR_INVALID, "cp_problem_set_mg_rebuild: every must be >= 1");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_problem_set_mg_rebuild: no multigrid lattice set");
+    P->mg->rebuild = every;
+    return CP_OK;
+}
+
+int cp_problem_mg_smoothers(cp_problem* P, int* kinds)
+{
+    if (!P || !kinds) return cp::fail(CP_ERR_INVALID, "cp_problem_mg_smoothers: bad arguments");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_problem_mg_smoothers: no multigrid lattice set");
+    for (int l = 0; l < P->mg->nlev; ++l) kinds[l] = P->mg->L[l].seg;
     return CP_OK;
 }
 

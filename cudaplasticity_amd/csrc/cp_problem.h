@@ -105,7 +105,8 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
                    const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
                    hipStream_t stream, cp_problem** out);
 int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st);
-int launch_assemble(cp_problem* P, int precision);
+// mg_full (multigrid only): 0 lets the coarse levels be reused (see mg_setup).
+int launch_assemble(cp_problem* P, int precision, int mg_full = 1);
 double host_bnorm2(const cp_problem* P);
 int set_load(cp_problem* P, double factor, int reset);
 int advance_load(cp_problem* P, double factor, int predictor);
@@ -120,7 +121,11 @@ int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max
                       int solved_tight);
 template <typename T>
 void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim3 g, hipStream_t st);
-int mg_setup(cp_problem* P, int precision);
+// Multigrid set-up after an assembly: the finest level's smoother always;
+// the coarse levels (Galerkin operators, smoothers, coarsest inverse) when
+// full is set, on the first set-up of a precision, or every
+// MgState::rebuild-th set-up, else the previous ones are kept.
+int mg_setup(cp_problem* P, int precision, int full);
 void mg_destroy(cp_problem* P);
 template <typename T>
 int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,

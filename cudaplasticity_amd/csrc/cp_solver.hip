@@ -338,7 +338,7 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
 
 // ---------------------------------------------------------------- kernels drivers
 
-int launch_assemble(cp_problem* P, int precision)
+int launch_assemble(cp_problem* P, int precision, int mg_full)
 {
     if (int s = problem_alloc_solver(P, precision)) return s;
     cpk::AsmArgs a{};
@@ -378,7 +378,7 @@ int launch_assemble(cp_problem* P, int precision)
         else hipLaunchKernelGGL(cpk::k_block_jacobi<float>, g, b, 0, P->stream, P->cg32_args(), P->minv32);
         CP_HIP(hipGetLastError());
     } else if (P->precond == 2) {
-        if (int s = mg_setup(P, precision)) return s;
+        if (int s = mg_setup(P, precision, mg_full)) return s;
     }
     if (f64) P->have_fp64_values = true;
     return CP_OK;
@@ -782,7 +782,9 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     double prev_rel = 1.0;
     for (int k = 0; k < max_outer; ++k) {
         CP_HIP(hipEventRecord(P->ev[0], stream));
-        if (int s = launch_assemble(P, prec)) return s;
+        // multigrid: a fresh hierarchy at the first secant iteration of the
+        // level, later ones may reuse the coarse levels
+        if (int s = launch_assemble(P, prec, k == 0)) return s;
         CP_HIP(hipEventRecord(P->ev[1], stream));
         long long it = 0, refs = 0;
         double relres = 0., cgms = 0.;

@@ -220,6 +220,15 @@ int cp_problem_set_graphs(cp_problem* prob, int enable);
 /* Number of multigrid levels (0 without a lattice) and their lattice sizes
  * (dims: 2 per level, may be NULL). */
 int cp_problem_mg_levels(cp_problem* prob, int* levels, int* dims);
+/* Smoother of each multigrid level except the coarsest (kinds: one per
+ * level, 0 = 2x2 block Jacobi, 1 = block-tridiagonal segments along lattice
+ * direction 1; the coarsest level, solved exactly, reports 0). */
+int cp_problem_mg_smoothers(cp_problem* prob, int* kinds);
+/* Multigrid set-up policy: the coarse levels are rebuilt at the first secant
+ * iteration of every load level and then every `every`-th assembly (1 =
+ * every assembly, default 8); in between only the finest level's smoother
+ * follows the new secant moduli. */
+int cp_problem_set_mg_rebuild(cp_problem* prob, int every);
 
 /* Scale the load (rp, prescribed u) -- new: load stepping. */
 int cp_set_load(cp_problem* prob, double factor);

@@ -44,10 +44,14 @@ def test_elastic_plate_matches_oracle(ctx):
     om = O.gen_quarter_plate(1.0, 0.1, 100, 49)
     ol = O.Problem(om).force(1, 100.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
     uo, so, _ = O.solve(ol, O.material(E, 0.3), O.options(cg_tol=1e-10, elastic=True))
-    print("mg iterations", st.cg_iters, "block-Jacobi", st_bj.cg_iters, flush=True)
+    print("mg iterations", st.cg_iters, "block-Jacobi", st_bj.cg_iters, "smoothers", P.mg_smoothers(), flush=True)
     assert rel(P.field("u"), uo) < 1e-8
     assert rel(P.field("sxx"), O.fields(so)["sxx"]) < 1e-8
     assert st.cg_iters * 10 < st_bj.cg_iters
+    # the cells near the hole are stretched along the lattice direction 1
+    # (about 1:16): the segment smoother keeps the V-cycle's rate there
+    assert P.mg_smoothers()[0] == 1
+    assert st.cg_iters <= 30
 
 
 @pytest.mark.parametrize("prec", [A.FP64, A.MIXED])
@@ -142,3 +146,50 @@ def test_graph_replay_equals_eager(ctx, prec):
     print("eager", out[0][2], "graph", out[1][2], flush=True)
     assert out[0][2] == out[1][2]
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
+def test_permuted_lattice_segment_smoother(ctx):
+    """The same plate with its nodes renumbered at random: the lattice is
+    handed over through node_of, the segment smoother and the Galerkin
+    levels gather through it, and the solution is the renumbered one."""
+    Nt, Nr = 96, 48
+    m = A.gen_quarter_plate(1.0, 0.1, Nt, Nr)
+    perm = np.random.default_rng(7).permutation(m.nn).astype(np.int32)   # old node -> new node
+    x, y = np.empty(m.nn), np.empty(m.nn)
+    x[perm], y[perm] = m.x, m.y
+    pm = A.Mesh(x, y, perm[m.conn], [perm[b] for b in m.borders])
+    mat = A.material(E, 0.3)
+    out = []
+    for mesh, node_of in ((m, None), (pm, perm)):
+        loads = A.Loads(mesh).force(1, 100.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(Nt + 1, Nr + 1, node_of)
+        st = P.solve(A.options(cg_tol=1e-10, precond=2), elastic=True)
+        print("cg", st.cg_iters, "smoothers", P.mg_smoothers(), flush=True)
+        assert P.mg_smoothers()[:2] == [1, 1]
+        out.append((P.field("u").reshape(-1, 2), st.cg_iters))
+        P.close()
+    u0, u1 = out[0][0], out[1][0][perm]
+    assert rel(u1, u0) < 1e-8
+    assert abs(out[0][1] - out[1][1]) <= 2
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.MIXED])
+def test_coarse_rebuild_interval(ctx, prec):
+    """Reusing the coarse levels between rebuilds changes the iteration
+    path, not the converged load levels."""
+    res = []
+    for every in (1, 8):
+        mesh, loads, mat, P = plate(ctx, 60, 30)
+        P.set_mg_rebuild(every)
+        per = P.solve_steps(6, 1.0, A.options(cg_tol=1e-9, tol=1e-7, aitken=True, precision=prec, precond=2,
+                                              predictor=1, cg_forcing=0.1, inner_tol=1e-4))
+        assert all(s.status == A.CP_OK for s in per)
+        res.append((P.field("u"), P.field("sigI"), sum(s.cg_iters for s in per)))
+        P.close()
+    print("cg iterations, rebuild 1 / 8:", res[0][2], res[1][2], flush=True)
+    assert rel(res[1][0], res[0][0]) < 1e-6 and rel(res[1][1], res[0][1]) < 1e-6
+    mesh, loads, mat, P = plate(ctx, 20, 10)
+    with pytest.raises(A.CPError) as e:
+        P.set_mg_rebuild(0)
+    assert e.value.code == A.CP_ERR_INVALID
