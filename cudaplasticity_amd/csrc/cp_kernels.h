@@ -281,6 +281,7 @@ struct CgScal {
     double bnorm2, tol2;
     double gamma0;
     double rho;       // (r, r): the stop test uses the unpreconditioned residual
+    double rho0;      // (r_0, r_0)
     int iters, done, status, maxit;
     unsigned counter, pad;
 };
@@ -449,6 +450,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_start(CgArgs<T> a)
         sc->gamma = g;
         sc->delta = d;
         sc->rho = rho;
+        sc->rho0 = rho;
         sc->beta = 0.;
         sc->alpha = g / d;
         sc->iters = 0;

@@ -461,7 +461,7 @@ static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, co
 // per iteration): the caller has launched the scalar set-up and r_0.
 template <typename T>
 int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
-              long long* iters, int* status, double* ms)
+              long long* iters, int* status, double* ms, bool zero_start)
 {
     using V = typename cpk::V2<T>::type;
     MgState& G = *P->mg;
@@ -478,7 +478,17 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
     hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
     CP_HIP(hipGetLastError());
     long long launched = 0;
-    int chunk = (int)std::min<long long>(64, std::max<long long>(4, P->last_cg_iters / 2));
+    // iterations after convergence still cost their (empty) launches, so
+    // the first chunk is the predicted count: from the reduction still
+    // needed when the solve starts from zero (the refinement passes, where
+    // (r_0, r_0) is bnorm2) and the rate of the previous solves, else the
+    // previous solve's count
+    int chunk;
+    if (zero_start && P->mg_lrate < 0. && bnorm2 > 0. && tol2 > 0. && tol2 < 1.)
+        chunk = (int)std::lround(std::log(tol2) / P->mg_lrate);
+    else
+        chunk = (int)P->last_cg_iters;
+    chunk = std::min(64, std::max(1, chunk));
     double g_prev = -1.;
     long long it_prev = 0;
     hipGraphExec_t exec = nullptr;
@@ -497,10 +507,12 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
         if (h.done) break;
         const double target = tol2 * (bnorm2 > 0. ? bnorm2 : h.bnorm2);
         int next = std::min(64, chunk * 2);
-        if (g_prev > 0. && h.rho > 0. && h.rho < g_prev && target > 0.) {
-            const double rate = std::log(g_prev / h.rho) / double(h.iters - it_prev);
+        const double r0 = g_prev > 0. ? g_prev : h.rho0;
+        const long long i0 = g_prev > 0. ? it_prev : 0;
+        if (r0 > 0. && h.rho > 0. && h.rho < r0 && target > 0. && h.iters > i0) {
+            const double rate = std::log(r0 / h.rho) / double(h.iters - i0);
             const double rem = std::log(h.rho / target) / rate;
-            if (rem > 0 && rem < 1e7) next = (int)std::min(64., std::max(2., rem * 1.05 + 2.));
+            if (rem > 0 && rem < 1e7) next = (int)std::min(64., std::max(1., std::ceil(rem)));
         }
         g_prev = h.rho;
         it_prev = h.iters;
@@ -514,12 +526,17 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
     *ms = t;
     *iters = P->h_cg->iters;
     *status = P->h_cg->status;
+    const cpk::CgScal& h = *P->h_cg;
+    if (h.iters > 0 && h.rho > 0. && h.rho0 > h.rho) {
+        const double lr = std::log(h.rho / h.rho0) / double(h.iters);
+        P->mg_lrate = P->mg_lrate < 0. ? 0.7 * P->mg_lrate + 0.3 * lr : lr;
+    }
     return CP_OK;
 }
 template int run_mg_cg<double>(cp_problem*, const cpk::CgArgs<double>&, cpk::CgScal*, double, double, long long*,
-                               int*, double*);
+                               int*, double*, bool);
 template int run_mg_cg<float>(cp_problem*, const cpk::CgArgs<float>&, cpk::CgScal*, double, double, long long*,
-                              int*, double*);
+                              int*, double*, bool);
 
 template <typename T>
 static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smooth, double* bytes_smooth)

@@ -60,6 +60,7 @@ struct cp_problem {
     std::vector<double> rp0;   // host copy for ||b||^2
     std::vector<int> adj_ptr, adj_node;
     long long last_cg_iters = 0;
+    double mg_lrate = 0.;   // multigrid CG: mean log reduction of (r, r) per iteration (0: none yet)
     double last_cg_ms_per_iter = 0.;
     bool have_fp64_values = false;
     int precond = 0;          // current solve: 0 plain CG, 1 block-Jacobi, 2 multigrid
@@ -127,7 +128,8 @@ void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim
 // MgState::rebuild-th set-up, else the previous ones are kept.
 int mg_setup(cp_problem* P, int precision, int full);
 void mg_destroy(cp_problem* P);
+// zero_start: x_0 = 0, so (r_0, r_0) = bnorm2 (the refinement passes).
 template <typename T>
 int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
-              long long* iters, int* status, double* ms);
+              long long* iters, int* status, double* ms, bool zero_start = false);
 }  // namespace cp

@@ -664,7 +664,7 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
         long long it = 0;
         int status = 0;
         double t = 0.;
-        if (int s = P->precond == 2 ? run_mg_cg<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t)
+        if (int s = P->precond == 2 ? run_mg_cg<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t, true)
                                     : run_cg_iterations<float>(P, a, P->sc32, itol * itol, rn, &it, &status, &t))
             return s;
         *iters += it;
