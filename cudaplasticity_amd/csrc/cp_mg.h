@@ -490,6 +490,16 @@ struct SegGrid {
     const signed char* slot_pos;   // level 0: node x 9 lattice offsets -> row position
 };
 
+// Row positions of the couplings to the previous and next node of the
+// segment, packed into the unused fourth component of the node's minv entry
+// (bit pattern only; kSegNone: no such coupling).
+constexpr unsigned kSegNone = 0xff;
+template <typename T> __device__ __forceinline__ T seg_pack(unsigned b);
+template <> __device__ __forceinline__ float seg_pack<float>(unsigned b) { return __uint_as_float(b); }
+template <> __device__ __forceinline__ double seg_pack<double>(unsigned b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned seg_unpack(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ unsigned seg_unpack(double v) { return (unsigned)__double_as_longlong(v); }
+
 __device__ __forceinline__ int seg_node(const SegGrid& g, int i1, int i2)
 {
     const int li = i1 * g.n2 + i2;
@@ -572,7 +582,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
             } else if (m.y) {
                 i11 = 1. / t11;
             }
-            minv[n] = typename V4<T>::type{(T)i00, (T)i01, (T)i11, (T)0};
+            // the pad carries the row positions of the in-segment couplings
+            const int k = k0 + c;
+            const int i1 = s * kSeg + k;
+            const int ql = k > 0 ? seg_pos(g, n, -1, 0) : -1;
+            const int qu = (k < kSeg - 1 && i1 + 1 < g.n1) ? seg_pos(g, n, 1, 0) : -1;
+            const unsigned pl = ql >= 0 ? (unsigned)ql : kSegNone, pu = qu >= 0 ? (unsigned)qu : kSegNone;
+            minv[n] = typename V4<T>::type{(T)i00, (T)i01, (T)i11, seg_pack<T>(pl | (pu << 8))};
             p00 = i00;
             p01 = i01;
             p11 = i11;
@@ -617,50 +633,83 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
     if (in) {
         const uchar2 mk = a.mask[n];
         const V bn = v.b[n];
+        const typename V4<T>::type mi = a.minv[n];
+        const unsigned pp = seg_unpack(mi.w);
+        const int pl = pp & 0xff, pu = (pp >> 8) & 0xff;
+        const int2 sl = a.slice[n >> 6];
+        const size_t base = (size_t)sl.x + (n & 63);
+        T bl[4] = {0, 0, 0, 0}, bu[4] = {0, 0, 0, 0};   // couplings to the previous / next segment node
         if constexpr (MODE == SEG_X0) {
             rx = mk.x ? bn.x : (T)0;
             ry = mk.y ? bn.y : (T)0;
+            if (pl != (int)kSegNone) load_block(a, base + (size_t)pl * kWave, bl[0], bl[1], bl[2], bl[3]);
+            if (pu != (int)kSegNone) load_block(a, base + (size_t)pu * kWave, bu[0], bu[1], bu[2], bu[3]);
         } else {
-            T yx, yy;
-            row_product<W, T>(a, n,
-                              [&](int m, T& zx, T& zy) {
-                                  if constexpr (MODE == MG_CHEB) {
-                                      const V xm = xi[m], dm = di[m];
-                                      zx = xm.x + dm.x;
-                                      zy = xm.y + dm.y;
-                                  } else {
-                                      const V xm = xi[m];
-                                      zx = xm.x;
-                                      zy = xm.y;
-                                  }
-                              },
-                              yx, yy);
+            // the row product of mg_smooth_node, keeping the two blocks of
+            // the segment couplings as they pass
+            auto zfun = [&](int m, T& zx, T& zy) {
+                if constexpr (MODE == MG_CHEB) {
+                    const V xm = xi[m], dm = di[m];
+                    zx = xm.x + dm.x;
+                    zy = xm.y + dm.y;
+                } else {
+                    const V xm = xi[m];
+                    zx = xm.x;
+                    zy = xm.y;
+                }
+            };
+            T yx = 0, yy = 0;
+            if constexpr (W > 0) {
+                int cidx[W];
+#pragma unroll
+                for (int k = 0; k < W; ++k) cidx[k] = a.col[base + (size_t)(k < sl.y ? k : 0) * kWave];
+                T m[W][4];
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    load_block(a, base + (size_t)(k < sl.y ? k : 0) * kWave, m[k][0], m[k][1], m[k][2], m[k][3]);
+                    if (k >= sl.y) m[k][0] = m[k][1] = m[k][2] = m[k][3] = 0;
+                }
+#pragma unroll
+                for (int k = 0; k < W; ++k) {
+                    T zx, zy;
+                    zfun(cidx[k], zx, zy);
+                    yx += m[k][0] * zx + m[k][1] * zy;
+                    yy += m[k][2] * zx + m[k][3] * zy;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        bl[j] = k == pl ? m[k][j] : bl[j];
+                        bu[j] = k == pu ? m[k][j] : bu[j];
+                    }
+                }
+            } else {
+                size_t q = base;
+                for (int k = 0; k < sl.y; ++k, q += kWave) {
+                    T zx, zy, m00, m01, m10, m11;
+                    zfun(a.col[q], zx, zy);
+                    load_block(a, q, m00, m01, m10, m11);
+                    yx += m00 * zx + m01 * zy;
+                    yy += m10 * zx + m11 * zy;
+                    if (k == pl) bl[0] = m00, bl[1] = m01, bl[2] = m10, bl[3] = m11;
+                    if (k == pu) bu[0] = m00, bu[1] = m01, bu[2] = m10, bu[3] = m11;
+                }
+            }
             rx = mk.x ? bn.x - yx : (T)0;
             ry = mk.y ? bn.y - yy : (T)0;
             xn = xi[n];
             if constexpr (MODE == MG_CHEB) dn = di[n];
         }
-        const typename V4<T>::type mi = a.minv[n];
         apply_minv<T>(mi, rx, ry, ax, ay);
         // Dt^-1 times the couplings to the segment neighbours.  Rows of
         // fixed DOFs are zero in Dt^-1, so y and z vanish on fixed DOFs and
         // the columns of the neighbours' fixed DOFs need no masking.
-        if (t1 > 0) {
-            double l[4];
-            seg_block(a, g, n, -1, l);
-            f[0] = mi.x * (T)l[0] + mi.y * (T)l[2];
-            f[1] = mi.x * (T)l[1] + mi.y * (T)l[3];
-            f[2] = mi.y * (T)l[0] + mi.z * (T)l[2];
-            f[3] = mi.y * (T)l[1] + mi.z * (T)l[3];
-        }
-        if (t1 < kSeg - 1 && i1 + 1 < g.n1) {
-            double u[4];
-            seg_block(a, g, n, 1, u);
-            h[0] = mi.x * (T)u[0] + mi.y * (T)u[2];
-            h[1] = mi.x * (T)u[1] + mi.y * (T)u[3];
-            h[2] = mi.y * (T)u[0] + mi.z * (T)u[2];
-            h[3] = mi.y * (T)u[1] + mi.z * (T)u[3];
-        }
+        f[0] = mi.x * bl[0] + mi.y * bl[2];
+        f[1] = mi.x * bl[1] + mi.y * bl[3];
+        f[2] = mi.y * bl[0] + mi.z * bl[2];
+        f[3] = mi.y * bl[1] + mi.z * bl[3];
+        h[0] = mi.x * bu[0] + mi.y * bu[2];
+        h[1] = mi.x * bu[1] + mi.y * bu[3];
+        h[2] = mi.y * bu[0] + mi.z * bu[2];
+        h[3] = mi.y * bu[1] + mi.z * bu[3];
     }
     const int tid = threadIdx.x;
     sy[0][tid] = ax;

@@ -36,8 +36,10 @@ struct MgState {
     int nd = 0;
     bool alloc[2] = {false, false};
     std::vector<void*> blocks;
-    // one CG iteration (update, V-cycle, dots, step) captured per precision
+    // one CG iteration (update, V-cycle, dots, step) and the start of a
+    // solve (clear, V-cycle, dots, scalars) captured per precision
     hipGraphExec_t iter[2] = {nullptr, nullptr};
+    hipGraphExec_t start[2] = {nullptr, nullptr};
     bool use_graphs = true;
     // levels lt.. (at most kTailThreads nodes each) run in one workgroup
     int lt = 0;
@@ -64,6 +66,8 @@ struct MgState {
     ~MgState()
     {
         for (hipGraphExec_t& e : iter)
+            if (e) (void)hipGraphExecDestroy(e);
+        for (hipGraphExec_t& e : start)
             if (e) (void)hipGraphExecDestroy(e);
         for (void* q : blocks) (void)hipFree(q);
     }
@@ -430,30 +434,54 @@ static void mg_cg_iteration(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal*
     hipLaunchKernelGGL(cpk::k_cg_step<T>, dim3(1), b, 0, st, a);
 }
 
-// The iteration above recorded once as a HIP graph (all its arguments are
-// fixed device addresses, the scalars live on the device) and replayed:
-// about 50 small launches per iteration cost more on the host than on the
-// device otherwise.
+// The first V-cycle of a solve: u_0 = M r_0, w_0 = A u_0 with the dots,
+// and the scalars of the first step.
+template <typename T>
+static void mg_cg_start(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, const typename cpk::V2<T>::type* u)
+{
+    hipStream_t st = P->stream;
+    const dim3 g(P->nbn), b(cpk::kBlock);
+    hipLaunchKernelGGL(cpk::k_mg_cg_clear<T>, g, b, 0, st, a);
+    mg_vcycle<T>(P, a.r[0], dsc);
+    by_width<T>(P->max_w, [&](auto w) {
+        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
+    });
+    hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
+}
+
+template <typename F>
+static int capture(cp_problem* P, hipGraphExec_t* exec, const F& record)
+{
+    hipGraph_t graph = nullptr;
+    CP_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+    record();
+    const hipError_t e = hipStreamEndCapture(P->stream, &graph);
+    if (e != hipSuccess) return failf(CP_ERR_DEVICE, "multigrid: graph capture failed: %s", hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+        *exec = nullptr;
+        return failf(CP_ERR_DEVICE, "multigrid: graph instantiation failed: %s", hipGetErrorString(e2));
+    }
+    return CP_OK;
+}
+
+// The iteration and the start of a solve recorded once each as a HIP graph
+// (all their arguments are fixed device addresses, the scalars live on the
+// device) and replayed: about 50 small launches per V-cycle cost more on
+// the host than on the device otherwise.
 template <typename T>
 static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, const typename cpk::V2<T>::type* u,
-                    hipGraphExec_t* out)
+                    hipGraphExec_t* start, hipGraphExec_t* iter)
 {
     MgState& G = *P->mg;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
-    if (!G.iter[pi]) {
-        hipGraph_t graph = nullptr;
-        CP_HIP(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
-        mg_cg_iteration<T>(P, a, dsc, u);
-        const hipError_t e = hipStreamEndCapture(P->stream, &graph);
-        if (e != hipSuccess) return failf(CP_ERR_DEVICE, "multigrid: graph capture failed: %s", hipGetErrorString(e));
-        const hipError_t e2 = hipGraphInstantiate(&G.iter[pi], graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e2 != hipSuccess) {
-            G.iter[pi] = nullptr;
-            return failf(CP_ERR_DEVICE, "multigrid: graph instantiation failed: %s", hipGetErrorString(e2));
-        }
-    }
-    *out = G.iter[pi];
+    if (!G.iter[pi])
+        if (int s = capture(P, &G.iter[pi], [&] { mg_cg_iteration<T>(P, a, dsc, u); })) return s;
+    if (!G.start[pi])
+        if (int s = capture(P, &G.start[pi], [&] { mg_cg_start<T>(P, a, dsc, u); })) return s;
+    *iter = G.iter[pi];
+    *start = G.start[pi];
     return CP_OK;
 }
 
@@ -468,14 +496,12 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     hipStream_t st = P->stream;
     const V* u = (const V*)G.L[0].vec[pi][2];
-    const dim3 g(P->nbn), b(cpk::kBlock);
+    hipGraphExec_t exec = nullptr, start = nullptr;
+    if (G.use_graphs)
+        if (int s = mg_graph<T>(P, a, dsc, u, &start, &exec)) return s;
     CP_HIP(hipEventRecord(P->ev[6], st));
-    hipLaunchKernelGGL(cpk::k_mg_cg_clear<T>, g, b, 0, st, a);
-    mg_vcycle<T>(P, a.r[0], dsc);
-    by_width<T>(P->max_w, [&](auto w) {
-        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
-    });
-    hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
+    if (start) CP_HIP(hipGraphLaunch(start, st));
+    else mg_cg_start<T>(P, a, dsc, u);
     CP_HIP(hipGetLastError());
     long long launched = 0;
     // iterations after convergence still cost their (empty) launches, so
@@ -491,9 +517,6 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
     chunk = std::min(64, std::max(1, chunk));
     double g_prev = -1.;
     long long it_prev = 0;
-    hipGraphExec_t exec = nullptr;
-    if (G.use_graphs)
-        if (int s = mg_graph<T>(P, a, dsc, u, &exec)) return s;
     for (;;) {
         for (int j = 0; j < chunk; ++j) {
             if (exec) CP_HIP(hipGraphLaunch(exec, st));
