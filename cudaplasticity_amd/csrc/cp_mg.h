@@ -886,29 +886,59 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse_solve(const double* Dinv, 
 // ------------------------------------------------------------ coarse tail
 
 constexpr int kTailThreads = 1024;   // levels of at most this many nodes run in one workgroup
+// LDS of the tail: the six vectors of every tail level live there (the
+// host keeps the tail within it: at most 1536 nodes in FP64)
+constexpr int kTailLds = 144 * 1024;
 
 template <typename T>
 struct TailLevel {
     CgArgs<T> a;       // matrix view
-    MgVec<T> v;        // vectors (v.b: right-hand side)
+    MgVec<T> v;        // vectors in global memory (v.b: right-hand side)
     MgGrid grid;       // to the next coarser level
+    int lds;           // byte offset of the level's b, xa, xb, da, db, res in the tail's LDS
 };
+
+// Level l of the tail with its vectors pointed into the LDS arena.
+template <typename T>
+__device__ __forceinline__ TailLevel<T> tail_level(const TailLevel<T>* lv, int l, char* lds)
+{
+    using V = typename V2<T>::type;
+    TailLevel<T> L = lv[l];
+    V* base = reinterpret_cast<V*>(lds + L.lds);
+    const int nn = L.a.nn;
+    L.v.b = base;
+    L.v.xa = base + nn;
+    L.v.xb = base + 2 * nn;
+    L.v.da = base + 3 * nn;
+    L.v.db = base + 4 * nn;
+    L.v.res = base + 5 * nn;
+    return L;
+}
 
 // Every level from the first one of at most kTailThreads nodes down to the
 // coarsest and back, in one workgroup: the same per-node work as the
 // level kernels, with workgroup barriers instead of launches between the
-// steps (global memory written by the workgroup is visible to it after the
-// barrier).  lv[0] is the tail's finest level, whose right-hand side the
-// level above has restricted; its result lands in lv[0].v.xb.
+// steps, and every vector of these levels in LDS (the matrices are read
+// from global memory).  lv[0] is the tail's finest level: its right-hand
+// side, restricted by the level above, is read from lv[0].v.b and its
+// result is written to lv[0].v.xb.
 template <typename T>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
                                                           const CgScal* sc)
 {
     using V = typename V2<T>::type;
     if (sc && sc->done) return;
+    __shared__ __align__(16) char lds[kTailLds];
     const int tid = threadIdx.x;
+    {
+        // right-hand side of the first tail level (restricted by the level above)
+        const TailLevel<T> L0 = lv[0];
+        V* b0 = reinterpret_cast<V*>(lds + L0.lds);
+        for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
+        __syncthreads();
+    }
     for (int l = 0; l < nl - 1; ++l) {
-        const TailLevel<T> L = lv[l];
+        const TailLevel<T> L = tail_level<T>(lv, l, lds);
         const int nn = L.a.nn;
         if (tid < nn) mg_smooth_node<9, MG_PRE1, T>(L.a, L.v, nullptr, nullptr, L.v.xa, L.v.da, 0, 0, tid);
         __syncthreads();
@@ -916,13 +946,13 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
         __syncthreads();
         if (tid < nn) mg_smooth_node<9, MG_RESID, T>(L.a, L.v, L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0, tid);
         __syncthreads();
-        const TailLevel<T> Cn = lv[l + 1];
+        const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         if (tid < Cn.a.nn) mg_restrict_node<T>(L.grid, L.v.res, const_cast<V*>(Cn.v.b), Cn.a.mask, tid);
         __syncthreads();
     }
     {
         __shared__ double bs[kMgDenseMax];
-        const TailLevel<T> Cn = lv[nl - 1];
+        const TailLevel<T> Cn = tail_level<T>(lv, nl - 1, lds);
         for (int i = tid; i < nd / 2; i += kTailThreads) {
             const V b = Cn.v.b[i];
             bs[2 * i] = b.x;
@@ -933,8 +963,8 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
-        const TailLevel<T> L = lv[l];
-        const TailLevel<T> Cn = lv[l + 1];
+        const TailLevel<T> L = tail_level<T>(lv, l, lds);
+        const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         const int nn = L.a.nn;
         if (tid < nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
         __syncthreads();
@@ -944,6 +974,12 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
         __syncthreads();
         if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1, tid);
         __syncthreads();
+    }
+    {
+        // the result of the first tail level, for the prolongation above
+        const TailLevel<T> L0 = lv[0];
+        const V* x0 = reinterpret_cast<const V*>(lds + L0.lds) + 2 * L0.a.nn;
+        for (int n = tid; n < L0.a.nn; n += kTailThreads) L0.v.xb[n] = x0[n];
     }
 }
 

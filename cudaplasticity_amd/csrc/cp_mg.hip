@@ -117,6 +117,9 @@ static int mg_alloc(cp_problem* P, int prec)
 template <typename T>
 static cpk::CgArgs<T> level_args(cp_problem* P, int l);
 
+// LDS bytes of one tail level's six vectors (16-byte aligned).
+static int tail_lds_bytes(int nn, size_t vbytes) { return (int)((6 * (size_t)nn * vbytes + 15) / 16 * 16); }
+
 // Device table of the coarse-tail levels for precision T (fixed addresses).
 template <typename T>
 static int mg_tail_table(cp_problem* P)
@@ -126,9 +129,12 @@ static int mg_tail_table(cp_problem* P)
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     if (G.tail[pi]) return CP_OK;
     std::vector<cpk::TailLevel<T>> tab;
+    int lds = 0;
     for (int l = G.lt; l < G.nlev; ++l) {
         MgLevel& L = G.L[l];
         cpk::TailLevel<T> t{};
+        t.lds = lds;
+        lds += tail_lds_bytes(L.nn, sizeof(V));
         t.a = level_args<T>(P, l);
         t.v.b = (const V*)L.vec[pi][0];
         t.v.xa = (V*)L.vec[pi][1];
@@ -147,6 +153,7 @@ static int mg_tail_table(cp_problem* P)
         }
         tab.push_back(t);
     }
+    if (lds > cpk::kTailLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
     cpk::TailLevel<T>* d = nullptr;
     if (int s = G.get(&d, tab.size())) return s;
     CP_HIP(hipMemcpy(d, tab.data(), sizeof(cpk::TailLevel<T>) * tab.size(), hipMemcpyHostToDevice));
@@ -715,7 +722,16 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     }
     G->nlev = (int)G->L.size();
     G->lt = G->nlev - 1;
-    while (G->lt > 1 && G->L[G->lt - 1].nn <= cpk::kTailThreads) --G->lt;
+    // the tail: the coarsest levels of at most kTailThreads nodes whose
+    // vectors fit its LDS in FP64
+    {
+        int lds = tail_lds_bytes(G->L[G->lt].nn, sizeof(double2));
+        while (G->lt > 1 && G->L[G->lt - 1].nn <= cpk::kTailThreads &&
+               lds + tail_lds_bytes(G->L[G->lt - 1].nn, sizeof(double2)) <= cpk::kTailLds) {
+            --G->lt;
+            lds += tail_lds_bytes(G->L[G->lt].nn, sizeof(double2));
+        }
+    }
     // smoother of each level: fully coarsened levels (outside the one-
     // workgroup tail) where at least kSegFrac of the cells are stretched
     // along direction 1 (mean direction-1 edge below half the direction-2
