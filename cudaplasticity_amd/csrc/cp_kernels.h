@@ -305,6 +305,9 @@ struct CgArgs {
     double* partials;
     double* partials_alt;    // second partials buffer of k_cg_iter_fused
     int nn, nblocks;
+    // coarse multigrid levels: the lattice of their 9-slot stencil rows,
+    // whose addresses follow from the node (0: rows through col / slice)
+    int lat_n1 = 0, lat_n2 = 0;
 };
 
 template <typename T>

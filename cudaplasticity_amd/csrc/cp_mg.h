@@ -26,6 +26,8 @@
 // All sums run in a fixed order: results do not depend on scheduling.
 #pragma once
 
+#include <type_traits>
+
 #include "cp_kernels.h"
 
 namespace cpk {
@@ -387,6 +389,55 @@ struct MgVec {
 
 enum { MG_PRE1 = 0, MG_CHEB = 1, MG_RESID = 2, MG_POST1 = 3 };
 
+// Coarse levels store lattice node n's row in slice n >> 6 (9 slots of 64
+// lanes each) with slot k holding the neighbour at a fixed lattice offset
+// (itself where the lattice ends, with zero values), so the row's
+// addresses and neighbours follow from (i1, i2) without loading col / slice.
+__device__ __forceinline__ int lat_neighbour(int k, int i1, int i2, int n1, int n2, int n)
+{
+    const int idx = k == 0 ? 4 : (k <= 4 ? k - 1 : k);
+    const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
+    const int j1 = i1 + d1, j2 = i2 + d2;
+    return (j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2) ? n + d1 * n2 + d2 : n;
+}
+
+__device__ __forceinline__ size_t lat_base(int n) { return (size_t)(n >> 6) * (9 * kWave) + (n & 63); }
+
+template <typename T, typename ZF>
+__device__ __forceinline__ void row_product_lat(const CgArgs<T>& a, int n, int i1, int i2, const ZF& zfun, T& yx,
+                                                T& yy)
+{
+    const size_t base = lat_base(n);
+    T m[9][4];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) load_block(a, base + (size_t)k * kWave, m[k][0], m[k][1], m[k][2], m[k][3]);
+    yx = 0;
+    yy = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        T zx, zy;
+        zfun(lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n), zx, zy);
+        yx += m[k][0] * zx + m[k][1] * zy;
+        yy += m[k][2] * zx + m[k][3] * zy;
+    }
+}
+
+// Row width template value of the coarse levels: lattice addressing.
+constexpr int kLatW = -1;
+
+// Row product of a multigrid level: lattice addressing on coarse levels
+// (W == kLatW), else the sliced-ELL row of width W.
+template <int W, typename T, typename ZF>
+__device__ __forceinline__ void row_apply(const CgArgs<T>& a, int n, const ZF& zfun, T& yx, T& yy)
+{
+    if constexpr (W == kLatW) {
+        const int i1 = n / a.lat_n2;
+        row_product_lat<T>(a, n, i1, n - i1 * a.lat_n2, zfun, yx, yy);
+    } else {
+        row_product<W, T>(a, n, zfun, yx, yy);
+    }
+}
+
 // One smoothing step of a level.  The iterate at the neighbours is formed
 // on the fly from the previous step's buffers, so each step is one launch:
 //   MG_PRE1  x1 = c0 D^-1 b                    -> xo = x1, do = a1 x1 + b1 D^-1 (b - A x1)
@@ -402,7 +453,7 @@ __device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T
     const ChebCoef<T> c = mg_cheb<T>(*v.lam);
     const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
     T yx, yy;
-    row_product<W, T>(a, n,
+    row_apply<W, T>(a, n,
                       [&](int m, T& zx, T& zy) {
                           if constexpr (MODE == MG_PRE1) {
                               const V bm = v.b[m];
@@ -463,6 +514,95 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth(CgArgs<T> a, MgVec<T> v, c
     mg_smooth_node<W, MODE, T>(a, v, xi, di, xo, dout, second, last, n);
 }
 
+struct MgGrid {
+    const int* lg;        // fine node -> lattice index (nullptr: identity)
+    const int* node_of;   // fine lattice index -> node (nullptr: identity)
+    int n1f, n2f, n1c, n2c;
+    int co1, co2;         // directions coarsened to the next level
+};
+
+// Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
+// w_f * res_f (fine residuals are already zero on fixed DOFs).
+template <typename T>
+__device__ __forceinline__ typename V2<T>::type mg_restrict_value(const MgGrid& gr, const typename V2<T>::type* res,
+                                                                  uchar2 mc, int C)
+{
+    using V = typename V2<T>::type;
+    const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
+    const int p1 = mg_pos(gr.n1f, gr.co1, K1), p2 = mg_pos(gr.n2f, gr.co2, K2);
+    double sx = 0., sy = 0.;
+    for (int a1 = -1; a1 <= 1; ++a1) {
+        const int f1 = p1 + a1;
+        if (f1 < 0 || f1 >= gr.n1f) continue;
+        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1);
+        if (w1 == 0.) continue;
+        for (int a2 = -1; a2 <= 1; ++a2) {
+            const int f2 = p2 + a2;
+            if (f2 < 0 || f2 >= gr.n2f) continue;
+            const double w2 = mg_weight(gr.n2f, gr.co2, f2, K2);
+            if (w2 == 0.) continue;
+            const int lf = f1 * gr.n2f + f2;
+            const V r = res[gr.node_of ? gr.node_of[lf] : lf];
+            sx += w1 * w2 * (double)r.x;
+            sy += w1 * w2 * (double)r.y;
+        }
+    }
+    return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
+}
+
+template <typename T>
+__device__ __forceinline__ void mg_restrict_node(const MgGrid& gr, const typename V2<T>::type* res,
+                                                 typename V2<T>::type* bc, const uchar2* mask_c, int C)
+{
+    bc[C] = mg_restrict_value<T>(gr, res, mask_c[C], C);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typename V2<T>::type* res,
+                                                        typename V2<T>::type* bc, const uchar2* mask_c, int nnc,
+                                                        int nbc, const CgScal* sc)
+{
+    if (sc && sc->done) return;
+    const int C = xcd_block(nbc) * kBlock + threadIdx.x;
+    if (C >= nnc) return;
+    mg_restrict_node<T>(gr, res, bc, mask_c, C);
+}
+
+// Coarse-grid correction: x_f = xin_f + mask_f * sum over f's parents of w * xc.
+template <typename T>
+__device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename V2<T>::type* xc,
+                                                const typename V2<T>::type* xin, typename V2<T>::type* xout,
+                                                const uchar2* mask_f, int n)
+{
+    using V = typename V2<T>::type;
+    const int lf = gr.lg ? gr.lg[n] : n;
+    const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
+    int Q1[2], Q2[2];
+    double W1[2], W2[2];
+    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
+    double sx = 0., sy = 0.;
+    for (int b1 = 0; b1 < nq1; ++b1)
+        for (int b2 = 0; b2 < nq2; ++b2) {
+            const V c = xc[Q1[b1] * gr.n2c + Q2[b2]];
+            sx += W1[b1] * W2[b2] * (double)c.x;
+            sy += W1[b1] * W2[b2] * (double)c.y;
+        }
+    const uchar2 mk = mask_f[n];
+    const V x = xin[n];
+    xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename V2<T>::type* xc,
+                                                       const typename V2<T>::type* xin, typename V2<T>::type* xout,
+                                                       const uchar2* mask_f, int nnf, int nbf, const CgScal* sc)
+{
+    if (sc && sc->done) return;
+    const int n = xcd_block(nbf) * kBlock + threadIdx.x;
+    if (n >= nnf) return;
+    mg_prolong_node<T>(gr, xc, xin, xout, mask_f, n);
+}
+
 // ------------------------------------------------------------ segment smoother
 //
 // Where elements are stretched (short edges along lattice direction 1) the
@@ -488,6 +628,8 @@ struct SegGrid {
     int nt2, ntiles;         // tiles along direction 2, tiles in all
     const int* node_of;      // lattice index -> node (level 0 only; nullptr: identity)
     const signed char* slot_pos;   // level 0: node x 9 lattice offsets -> row position
+    MgGrid rg;               // SEG_RX0: the restriction from the finer level
+    const void* rres;        // SEG_RX0: the finer level's residual
 };
 
 // Row positions of the couplings to the previous and next node of the
@@ -600,6 +742,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 // Smoothing steps of a segment level (same Chebyshev recurrences as
 // mg_smooth_node, D^-1 applied by the segment solve):
 //   SEG_X0    xo = c0 D^-1 b                      (x1 of the pre-smoothing)
+//   SEG_RX0   b = R res (restriction of the finer level's residual), then SEG_X0
 //   SEG_FIRST x = xi                              -> do = a1 x + b1 D^-1 (b - A x)
 //   MG_CHEB   x = xi + di                         -> xo = x (or x + d if last), do = a di + c D^-1 (b - A x)
 //   MG_POST1  x = xi                              -> do = c0 D^-1 (b - A x)
@@ -607,7 +750,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 // One workgroup per tile of kSeg x kSegCols lattice nodes: every thread
 // forms its node's residual and the per-node factors, then one thread per
 // tile column runs the two short recurrences of its segment in LDS.
-enum { SEG_X0 = 4, SEG_FIRST = 5 };
+enum { SEG_X0 = 4, SEG_FIRST = 5, SEG_RX0 = 6 };
 
 template <int W, int MODE, typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
@@ -632,14 +775,21 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
     T ax = 0, ay = 0;
     if (in) {
         const uchar2 mk = a.mask[n];
-        const V bn = v.b[n];
+        V bn;
+        if constexpr (MODE == SEG_RX0) {
+            bn = mg_restrict_value<T>(g.rg, reinterpret_cast<const V*>(g.rres), mk, n);
+            const_cast<V*>(v.b)[n] = bn;
+        } else {
+            bn = v.b[n];
+        }
         const typename V4<T>::type mi = a.minv[n];
         const unsigned pp = seg_unpack(mi.w);
         const int pl = pp & 0xff, pu = (pp >> 8) & 0xff;
-        const int2 sl = a.slice[n >> 6];
-        const size_t base = (size_t)sl.x + (n & 63);
+        constexpr bool lat = W == kLatW;
+        const int2 sl = lat ? make_int2(0, 9) : a.slice[n >> 6];
+        const size_t base = lat ? lat_base(n) : (size_t)sl.x + (n & 63);
         T bl[4] = {0, 0, 0, 0}, bu[4] = {0, 0, 0, 0};   // couplings to the previous / next segment node
-        if constexpr (MODE == SEG_X0) {
+        if constexpr (MODE == SEG_X0 || MODE == SEG_RX0) {
             rx = mk.x ? bn.x : (T)0;
             ry = mk.y ? bn.y : (T)0;
             if (pl != (int)kSegNone) load_block(a, base + (size_t)pl * kWave, bl[0], bl[1], bl[2], bl[3]);
@@ -659,7 +809,25 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
                 }
             };
             T yx = 0, yy = 0;
-            if constexpr (W > 0) {
+            if constexpr (lat) {
+                // coarse level: lattice addressing (the coupling slots are 2 and 7)
+                T m[9][4];
+#pragma unroll
+                for (int k = 0; k < 9; ++k)
+                    load_block(a, base + (size_t)k * kWave, m[k][0], m[k][1], m[k][2], m[k][3]);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    T zx, zy;
+                    zfun(lat_neighbour(k, i1, i2, g.n1, g.n2, n), zx, zy);
+                    yx += m[k][0] * zx + m[k][1] * zy;
+                    yy += m[k][2] * zx + m[k][3] * zy;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        bl[j] = k == pl ? m[k][j] : bl[j];
+                        bu[j] = k == pu ? m[k][j] : bu[j];
+                    }
+                }
+            } else if constexpr (W > 0) {
                 int cidx[W];
 #pragma unroll
                 for (int k = 0; k < W; ++k) cidx[k] = a.col[base + (size_t)(k < sl.y ? k : 0) * kWave];
@@ -747,7 +915,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
     __syncthreads();
     if (!in) return;
     const T gx = sy[0][tid], gy = sy[1][tid];
-    if constexpr (MODE == SEG_X0) {
+    if constexpr (MODE == SEG_X0 || MODE == SEG_RX0) {
         xo[n] = V{c.c0 * gx, c.c0 * gy};
     } else if constexpr (MODE == SEG_FIRST) {
         dout[n] = V{c.a1 * xn.x + c.b1 * gx, c.a1 * xn.y + c.b1 * gy};
@@ -764,89 +932,6 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
             dout[n] = d;
         }
     }
-}
-
-struct MgGrid {
-    const int* lg;        // fine node -> lattice index (nullptr: identity)
-    const int* node_of;   // fine lattice index -> node (nullptr: identity)
-    int n1f, n2f, n1c, n2c;
-    int co1, co2;         // directions coarsened to the next level
-};
-
-// Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
-// w_f * res_f (fine residuals are already zero on fixed DOFs).
-template <typename T>
-__device__ __forceinline__ void mg_restrict_node(const MgGrid& gr, const typename V2<T>::type* res,
-                                                 typename V2<T>::type* bc, const uchar2* mask_c, int C)
-{
-    using V = typename V2<T>::type;
-    const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
-    const int p1 = mg_pos(gr.n1f, gr.co1, K1), p2 = mg_pos(gr.n2f, gr.co2, K2);
-    double sx = 0., sy = 0.;
-    for (int a1 = -1; a1 <= 1; ++a1) {
-        const int f1 = p1 + a1;
-        if (f1 < 0 || f1 >= gr.n1f) continue;
-        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1);
-        if (w1 == 0.) continue;
-        for (int a2 = -1; a2 <= 1; ++a2) {
-            const int f2 = p2 + a2;
-            if (f2 < 0 || f2 >= gr.n2f) continue;
-            const double w2 = mg_weight(gr.n2f, gr.co2, f2, K2);
-            if (w2 == 0.) continue;
-            const int lf = f1 * gr.n2f + f2;
-            const V r = res[gr.node_of ? gr.node_of[lf] : lf];
-            sx += w1 * w2 * (double)r.x;
-            sy += w1 * w2 * (double)r.y;
-        }
-    }
-    const uchar2 mc = mask_c[C];
-    bc[C] = V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typename V2<T>::type* res,
-                                                        typename V2<T>::type* bc, const uchar2* mask_c, int nnc,
-                                                        int nbc, const CgScal* sc)
-{
-    if (sc && sc->done) return;
-    const int C = xcd_block(nbc) * kBlock + threadIdx.x;
-    if (C >= nnc) return;
-    mg_restrict_node<T>(gr, res, bc, mask_c, C);
-}
-
-// Coarse-grid correction: x_f = xin_f + mask_f * sum over f's parents of w * xc.
-template <typename T>
-__device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename V2<T>::type* xc,
-                                                const typename V2<T>::type* xin, typename V2<T>::type* xout,
-                                                const uchar2* mask_f, int n)
-{
-    using V = typename V2<T>::type;
-    const int lf = gr.lg ? gr.lg[n] : n;
-    const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
-    int Q1[2], Q2[2];
-    double W1[2], W2[2];
-    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
-    double sx = 0., sy = 0.;
-    for (int b1 = 0; b1 < nq1; ++b1)
-        for (int b2 = 0; b2 < nq2; ++b2) {
-            const V c = xc[Q1[b1] * gr.n2c + Q2[b2]];
-            sx += W1[b1] * W2[b2] * (double)c.x;
-            sy += W1[b1] * W2[b2] * (double)c.y;
-        }
-    const uchar2 mk = mask_f[n];
-    const V x = xin[n];
-    xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename V2<T>::type* xc,
-                                                       const typename V2<T>::type* xin, typename V2<T>::type* xout,
-                                                       const uchar2* mask_f, int nnf, int nbf, const CgScal* sc)
-{
-    if (sc && sc->done) return;
-    const int n = xcd_block(nbf) * kBlock + threadIdx.x;
-    if (n >= nnf) return;
-    mg_prolong_node<T>(gr, xc, xin, xout, mask_f, n);
 }
 
 // Coarsest level: x = D^-1 b with the explicit inverse (bs: b in LDS).
@@ -927,6 +1012,9 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
                                                           const CgScal* sc)
 {
     using V = typename V2<T>::type;
+    // rows through lattice addressing in FP32; in FP64 (at most 128 VGPRs
+    // with 1024 threads) through col / slice, which needs fewer registers
+    constexpr int TW = std::is_same<T, double>::value ? 9 : kLatW;
     if (sc && sc->done) return;
     __shared__ __align__(16) char lds[kTailLds];
     const int tid = threadIdx.x;
@@ -940,11 +1028,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
     for (int l = 0; l < nl - 1; ++l) {
         const TailLevel<T> L = tail_level<T>(lv, l, lds);
         const int nn = L.a.nn;
-        if (tid < nn) mg_smooth_node<9, MG_PRE1, T>(L.a, L.v, nullptr, nullptr, L.v.xa, L.v.da, 0, 0, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_PRE1, T>(L.a, L.v, nullptr, nullptr, L.v.xa, L.v.da, 0, 0, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.da, L.v.xb, L.v.db, 1, 0, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.da, L.v.xb, L.v.db, 1, 0, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<9, MG_RESID, T>(L.a, L.v, L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_RESID, T>(L.a, L.v, L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0, tid);
         __syncthreads();
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         if (tid < Cn.a.nn) mg_restrict_node<T>(L.grid, L.v.res, const_cast<V*>(Cn.v.b), Cn.a.mask, tid);
@@ -968,11 +1056,11 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
         const int nn = L.a.nn;
         if (tid < nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<9, MG_POST1, T>(L.a, L.v, L.v.xb, nullptr, nullptr, L.v.da, 0, 0, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_POST1, T>(L.a, L.v, L.v.xb, nullptr, nullptr, L.v.da, 0, 0, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xb, L.v.da, L.v.xa, L.v.db, 0, 0, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xb, L.v.da, L.v.xa, L.v.db, 0, 0, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<9, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1, tid);
+        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1, tid);
         __syncthreads();
     }
     {

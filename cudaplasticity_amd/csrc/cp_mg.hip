@@ -175,6 +175,8 @@ cpk::CgArgs<double> level_args<double>(cp_problem* P, int l)
     a.minv = L.minv64;
     a.nn = L.nn;
     a.nblocks = L.nb;
+    a.lat_n1 = L.n1;
+    a.lat_n2 = L.n2;
     return a;
 }
 
@@ -192,6 +194,8 @@ cpk::CgArgs<float> level_args<float>(cp_problem* P, int l)
     a.minv = L.minv32;
     a.nn = L.nn;
     a.nblocks = L.nb;
+    a.lat_n1 = L.n1;
+    a.lat_n2 = L.n2;
     return a;
 }
 
@@ -329,12 +333,15 @@ static void smooth_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const vo
 }
 
 // The same steps with the segment smoother: x1 = c0 D^-1 b takes a launch
-// of its own (D^-1 b at a neighbour is not local any more).
+// of its own (D^-1 b at a neighbour is not local any more); on coarse
+// levels that launch also restricts the finer level's residual into b
+// (g.rres set).
 template <int W, typename T>
 static void smooth_level_down_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g,
                                   hipStream_t st)
 {
-    smooth_seg<W, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
+    if (g.rres) smooth_seg<W, cpk::SEG_RX0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
+    else smooth_seg<W, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
     smooth_seg<W, cpk::SEG_FIRST, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g, st);
     smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, g, st);
     smooth<W, cpk::MG_RESID, T>(a, v, v.xb, v.db, v.xa, nullptr, 0, 0, st);
@@ -356,6 +363,15 @@ static void by_width(int w, const F& f)
     else if (w <= 12) f(std::integral_constant<int, 12>());
     else if (w <= 16) f(std::integral_constant<int, 16>());
     else f(std::integral_constant<int, 0>());
+}
+
+// Row instantiation of level l: the problem's row width on level 0, lattice
+// addressing on the coarse levels.
+template <typename T, typename F>
+static void by_level(const cp_problem* P, int l, const F& f)
+{
+    if (l == 0) by_width<T>(P->max_w, f);
+    else f(std::integral_constant<int, cpk::kLatW>());
 }
 
 // u = M r: one V-cycle; the result lands in level 0's xb.
@@ -392,16 +408,23 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         return g;
     };
     const int last = G.lt;   // levels lt.. run in the one-workgroup tail
+    // a launched segment level below level 0 restricts its right-hand side
+    // in its first smoothing launch
+    auto fused_restrict = [&](int l) { return l > 0 && l < last && G.L[l].seg; };
     for (int l = 0; l < last; ++l) {
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
         if (G.L[l].seg) {
-            const cpk::SegGrid sg = seg_grid(G, l);
-            by_width<T>(l == 0 ? P->max_w : 9,
-                        [&](auto w) { smooth_level_down_seg<decltype(w)::value, T>(a, v, sg, st); });
+            cpk::SegGrid sg = seg_grid(G, l);
+            if (fused_restrict(l)) {
+                sg.rg = grid(l - 1);
+                sg.rres = G.L[l - 1].vec[pi][5];
+            }
+            by_level<T>(P, l, [&](auto w) { smooth_level_down_seg<decltype(w)::value, T>(a, v, sg, st); });
         } else {
-            by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
         }
+        if (fused_restrict(l + 1)) continue;
         const MgLevel& C = G.L[l + 1];
         hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(C.nb), dim3(cpk::kBlock), 0, st, grid(l), (const V*)v.res,
                            (V*)C.vec[pi][0], (const uchar2*)C.mask, C.nn, C.nb, sc);
@@ -417,10 +440,9 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
                            (const V*)C.vec[pi][2], (const V*)v.xa, v.xb, (const uchar2*)a.mask, L.nn, L.nb, sc);
         if (L.seg) {
             const cpk::SegGrid sg = seg_grid(G, l);
-            by_width<T>(l == 0 ? P->max_w : 9,
-                        [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(a, v, sg, st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(a, v, sg, st); });
         } else {
-            by_width<T>(l == 0 ? P->max_w : 9, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
         }
     }
 }
