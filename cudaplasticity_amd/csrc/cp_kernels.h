@@ -684,6 +684,7 @@ static __global__ void k_cg_setup(CgScal* sc, double bnorm2, double tol2, int ma
     sc->bnorm2 = bnorm2;
     sc->tol2 = tol2;
     sc->rho = 0.;
+    sc->rho0 = 0.;
     sc->maxit = maxit;
     sc->iters = 0;
     sc->done = 0;
