@@ -177,18 +177,26 @@ def cpu_baseline(nt, nr, per_step, cores, what="the reference algorithm (FP64, u
                        f"1..{len(per_step)}: {secs:.1f} s projected")}
 
 
-def served_stream(A, ctx, args):
+def served_stream(A, ctx, args, precond=2, n=None, qmax=None):
     """Reduced config-5 stream: served with Poisson arrivals at `serve_load`
     times the rate the same problems are solved back to back one at a time,
-    and every served result compared with the same problem solved alone."""
+    and every served result compared with the same problem solved alone.
+    precond 2: multigrid CG (each request replays its own V-cycle graphs);
+    precond 1: block-Jacobi CG batched across requests."""
     from cudaplasticity_amd import serving
-    stream = serving.make_stream(args.serve_n, qmin=2e3, qmax=args.serve_qmax)
-    opt = solver_options(A, args, A.MIXED, precond=1)   # the serving loop batches block-Jacobi CG
+    n = args.serve_n if n is None else n
+    stream = serving.make_stream(n, qmin=2e3, qmax=args.serve_qmax if qmax is None else qmax)
+    opt = solver_options(A, args, A.MIXED, precond=precond)
     t0 = time.perf_counter()
     alone = []
-    for mesh, loads, mat, _ in stream:
+    for mesh, loads, mat, desc in stream:
         P = A.Problem(ctx, mesh, loads, mat)
-        P.solve_steps(args.serve_steps, 1.0, opt)
+        if precond == 2:
+            P.set_grid(*serving.lattice(desc))
+        ts = time.perf_counter()
+        per = P.solve_steps(args.serve_steps, 1.0, opt)
+        log(f"  alone: {desc['kind']} {desc['quads']} quads, secant {sum(s.outer_iters for s in per)}, "
+            f"cg {sum(s.cg_iters for s in per)}, {time.perf_counter() - ts:.3f} s")
         alone.append((P.field("u"), P.field("sigI")))
         P.close()
     t_seq = time.perf_counter() - t0
@@ -304,8 +312,8 @@ def main():
     ap.add_argument("--ref-steps", type=int, default=8)
     ap.add_argument("--large-nt", type=int, default=2000)
     ap.add_argument("--large-nr", type=int, default=1000)
-    ap.add_argument("--serve-n", type=int, default=24)
-    ap.add_argument("--serve-qmax", type=float, default=5e4)
+    ap.add_argument("--serve-n", type=int, default=64)
+    ap.add_argument("--serve-qmax", type=float, default=5e5)
     ap.add_argument("--serve-steps", type=int, default=5)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-load", type=float, default=1.0)
@@ -470,10 +478,15 @@ def main():
         log("large mesh kernels:", large)
         PL.close()
 
-    serve = None
+    serve = serve_bj = None
     if not args.no_serve and args.serve_n > 0:
-        serve = served_stream(A, ctx, args)
+        serve = served_stream(A, ctx, args, precond=2)
         log("served:", serve)
+        if not args.no_bj:
+            # the block-Jacobi serving path, batched across requests, on a
+            # smaller stream (2k-50k quads)
+            serve_bj = served_stream(A, ctx, args, precond=1, n=24, qmax=5e4)
+            log("served, block-Jacobi:", serve_bj)
 
     line = {
         "metric": METRIC,
@@ -523,6 +536,7 @@ def main():
         "mixed_block_jacobi": bj,
         "kernels_2000x1000": large,
         "served": serve,
+        "served_block_jacobi": serve_bj,
         "cpu_baseline": None,
     }
     if not args.no_cpu:

@@ -404,11 +404,19 @@ class Server:
         check(lib().cp_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)))
         self.sizes = {}
 
-    def submit(self, mesh, loads, mat, nsteps, final_factor, opt):
+    def submit(self, mesh, loads, mat, nsteps, final_factor, opt, grid=None, node_of=None):
+        """grid = (n1, n2): the mesh's lattice, needed for multigrid CG (precond 2)."""
         rid = C.c_longlong()
-        check(lib().cp_server_submit(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
-                                     dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), int(nsteps),
-                                     _f(final_factor), C.byref(opt), C.byref(rid)))
+        if grid is None:
+            check(lib().cp_server_submit(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
+                                         dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), int(nsteps),
+                                         _f(final_factor), C.byref(opt), C.byref(rid)))
+        else:
+            no = None if node_of is None else np.ascontiguousarray(node_of, dtype=np.int32)
+            check(lib().cp_server_submit_grid(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
+                                              dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), int(nsteps),
+                                              _f(final_factor), C.byref(opt), int(grid[0]), int(grid[1]),
+                                              ip(no) if no is not None else None, C.byref(rid)))
         self.sizes[rid.value] = (mesh.nn, mesh.ne)
         return rid.value
 

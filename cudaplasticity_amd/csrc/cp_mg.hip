@@ -514,44 +514,104 @@ static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, co
     return CP_OK;
 }
 
+template <typename T>
+static int mg_start_t(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, hipGraphExec_t* exec)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    const V* u = (const V*)G.L[0].vec[pi][2];
+    hipGraphExec_t start = nullptr;
+    *exec = nullptr;
+    if (G.use_graphs)
+        if (int s = mg_graph<T>(P, a, dsc, u, &start, exec)) return s;
+    if (start) CP_HIP(hipGraphLaunch(start, P->stream));
+    else mg_cg_start<T>(P, a, dsc, u);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+template <typename T>
+static int mg_iterations_t(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, hipGraphExec_t exec, int n)
+{
+    using V = typename cpk::V2<T>::type;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    const V* u = (const V*)P->mg->L[0].vec[pi][2];
+    for (int j = 0; j < n; ++j) {
+        if (exec) CP_HIP(hipGraphLaunch(exec, P->stream));
+        else mg_cg_iteration<T>(P, a, dsc, u);
+    }
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+// Iterations after convergence still cost their (empty) launches, so the
+// first chunk is the predicted count: from the reduction still needed when
+// the solve starts from zero (the refinement passes, where (r_0, r_0) is
+// bnorm2) and the rate of the previous solves, else the previous solve's
+// count.
+int mg_cg_first_chunk(const cp_problem* P, double tol2, bool zero_start)
+{
+    int chunk;
+    if (zero_start && P->mg_lrate < 0. && tol2 > 0. && tol2 < 1.)
+        chunk = (int)std::lround(std::log(tol2) / P->mg_lrate);
+    else
+        chunk = (int)P->last_cg_iters;
+    return std::min(64, std::max(1, chunk));
+}
+
+void mg_cg_observe(cp_problem* P, const cpk::CgScal& h)
+{
+    if (h.iters > 0 && h.rho > 0. && h.rho0 > h.rho) {
+        const double lr = std::log(h.rho / h.rho0) / double(h.iters);
+        P->mg_lrate = P->mg_lrate < 0. ? 0.7 * P->mg_lrate + 0.3 * lr : lr;
+    }
+}
+
+size_t mg_bytes_estimate(int nn, int precision)
+{
+    // per coarse node: 9 columns, the stencil values (FP64: two double2
+    // planes; FP32: float4), pivot inverse, mask, six vectors; per finest
+    // node: five vectors, the slot table and the lattice maps; the
+    // coarsest inverse
+    const bool f64 = precision == CP_FP64;
+    const double coarse = 36. + (f64 ? 288. : 144.) + (f64 ? 32. : 16.) + 2. + 6. * (f64 ? 16. : 8.);
+    const double fine = 5. * (f64 ? 16. : 8.) + 9. + 8.;
+    return (size_t)(nn * (coarse + fine)) + (size_t)cpk::kMgDenseMax * cpk::kMgDenseMax * 8 + (1 << 20);
+}
+
+int mg_cg_enqueue_start(cp_problem* P, int precision)
+{
+    hipGraphExec_t exec = nullptr;
+    return precision == CP_FP64 ? mg_start_t<double>(P, P->cg64_args(), P->sc64, &exec)
+                                : mg_start_t<float>(P, P->cg32_args(), P->sc32, &exec);
+}
+
+int mg_cg_enqueue_iterations(cp_problem* P, int precision, int n)
+{
+    MgState& G = *P->mg;
+    const int pi = precision == CP_FP64 ? 0 : 1;
+    hipGraphExec_t exec = G.use_graphs ? G.iter[pi] : nullptr;
+    return precision == CP_FP64 ? mg_iterations_t<double>(P, P->cg64_args(), P->sc64, exec, n)
+                                : mg_iterations_t<float>(P, P->cg32_args(), P->sc32, exec, n);
+}
+
 // CG preconditioned by the V-cycle (Chronopoulos-Gear form, one reduction
 // per iteration): the caller has launched the scalar set-up and r_0.
 template <typename T>
 int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
               long long* iters, int* status, double* ms, bool zero_start)
 {
-    using V = typename cpk::V2<T>::type;
-    MgState& G = *P->mg;
-    const int pi = std::is_same<T, double>::value ? 0 : 1;
     hipStream_t st = P->stream;
-    const V* u = (const V*)G.L[0].vec[pi][2];
-    hipGraphExec_t exec = nullptr, start = nullptr;
-    if (G.use_graphs)
-        if (int s = mg_graph<T>(P, a, dsc, u, &start, &exec)) return s;
     CP_HIP(hipEventRecord(P->ev[6], st));
-    if (start) CP_HIP(hipGraphLaunch(start, st));
-    else mg_cg_start<T>(P, a, dsc, u);
-    CP_HIP(hipGetLastError());
+    hipGraphExec_t exec = nullptr;
+    if (int s = mg_start_t<T>(P, a, dsc, &exec)) return s;
     long long launched = 0;
-    // iterations after convergence still cost their (empty) launches, so
-    // the first chunk is the predicted count: from the reduction still
-    // needed when the solve starts from zero (the refinement passes, where
-    // (r_0, r_0) is bnorm2) and the rate of the previous solves, else the
-    // previous solve's count
-    int chunk;
-    if (zero_start && P->mg_lrate < 0. && bnorm2 > 0. && tol2 > 0. && tol2 < 1.)
-        chunk = (int)std::lround(std::log(tol2) / P->mg_lrate);
-    else
-        chunk = (int)P->last_cg_iters;
-    chunk = std::min(64, std::max(1, chunk));
+    int chunk = mg_cg_first_chunk(P, tol2, zero_start && bnorm2 > 0.);
     double g_prev = -1.;
     long long it_prev = 0;
     for (;;) {
-        for (int j = 0; j < chunk; ++j) {
-            if (exec) CP_HIP(hipGraphLaunch(exec, st));
-            else mg_cg_iteration<T>(P, a, dsc, u);
-        }
-        CP_HIP(hipGetLastError());
+        if (int s = mg_iterations_t<T>(P, a, dsc, exec, chunk)) return s;
         launched += chunk;
         CP_HIP(hipMemcpyAsync(P->h_cg, dsc, sizeof(cpk::CgScal), hipMemcpyDeviceToHost, st));
         CP_HIP(hipStreamSynchronize(st));
@@ -578,11 +638,7 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
     *ms = t;
     *iters = P->h_cg->iters;
     *status = P->h_cg->status;
-    const cpk::CgScal& h = *P->h_cg;
-    if (h.iters > 0 && h.rho > 0. && h.rho0 > h.rho) {
-        const double lr = std::log(h.rho / h.rho0) / double(h.iters);
-        P->mg_lrate = P->mg_lrate < 0. ? 0.7 * P->mg_lrate + 0.3 * lr : lr;
-    }
+    mg_cg_observe(P, *P->h_cg);
     return CP_OK;
 }
 template int run_mg_cg<double>(cp_problem*, const cpk::CgArgs<double>&, cpk::CgScal*, double, double, long long*,

@@ -132,4 +132,15 @@ void mg_destroy(cp_problem* P);
 template <typename T>
 int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double tol2, double bnorm2,
               long long* iters, int* status, double* ms, bool zero_start = false);
+// The same solve in pieces, for callers that drive it without waiting
+// (the serving loop): enqueue the start (after enqueue_cg_init), enqueue n
+// iterations, the predicted first chunk, and the bookkeeping once done.
+int mg_cg_enqueue_start(cp_problem* P, int precision);
+int mg_cg_enqueue_iterations(cp_problem* P, int precision, int n);
+int mg_cg_first_chunk(const cp_problem* P, double tol2, bool zero_start);
+// Upper bound of the device memory cp_problem_set_grid and the multigrid
+// solves add to a problem of nn nodes (the coarse levels hold at most nn
+// nodes together).
+size_t mg_bytes_estimate(int nn, int precision);
+void mg_cg_observe(cp_problem* P, const cpk::CgScal& h);
 }  // namespace cp

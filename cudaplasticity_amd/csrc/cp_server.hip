@@ -38,6 +38,8 @@ struct Request {
     std::vector<double> x, y, rp, u0;
     std::vector<int> conn;
     std::vector<unsigned char> mask;
+    int n1 = 0, n2 = 0;              // multigrid lattice (precond 2)
+    std::vector<int> node_of;        // its node numbering (empty: identity)
     cp_material mat{};
     int nsteps = 1;
     double final_factor = 1.;
@@ -47,7 +49,7 @@ struct Request {
     cp_problem* P = nullptr;
     int phase = PH_LEVEL;
     int level = 0, k = 0, tight = 1;
-    double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1.;
+    double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1., itol = 0., rn = 0.;
     int stalled = 0;
     long long cg_iters = 0;
     int outer_total = 0;
@@ -140,19 +142,23 @@ double level_factor(const Request* r, int level) { return r->final_factor * doub
 int start_iteration(Request* r)
 {
     cp_problem* P = r->P;
-    if (int s = cp::launch_assemble(P, r->opt.precision)) return s;
+    // multigrid: a fresh hierarchy at the first secant iteration of a level,
+    // as cp_solve does
+    if (int s = cp::launch_assemble(P, r->opt.precision, r->k == 0)) return s;
     double ctol = r->opt.cg_tol;
     if (r->opt.cg_forcing > 0. && r->opt.fixed_outer <= 0) ctol = std::max(r->opt.cg_tol, r->opt.cg_forcing * r->prev_rel);
     r->ctol = ctol;
     r->tight = ctol <= r->opt.cg_tol;
     r->bnorm2 = cp::host_bnorm2(P);
-    const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : 1000000000;
+    const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : (P->precond == 2 ? 2000 : 1000000000);
     if (r->opt.precision == CP_FP64) {
         if (int s = cp::enqueue_cg_init(P, CP_FP64, r->bnorm2, ctol, maxit)) return s;
+        if (P->precond == 2)
+            if (int s = cp::mg_cg_enqueue_start(P, CP_FP64)) return s;
         r->phase = PH_CG;
         r->cg_rho_prev = -1.;
         r->cg_iters_seen = 0;
-        r->next_chunk = 16;
+        r->next_chunk = P->precond == 2 ? cp::mg_cg_first_chunk(P, ctol * ctol, false) : 16;
     } else {
         r->ref_norm = r->bnorm2;
         r->prev_rn = -1.;
@@ -173,7 +179,7 @@ int start_level(Request* r)
     } else {
         if (int s = cp::advance_load(P, lam, r->opt.predictor)) return s;
     }
-    P->precond = r->opt.precond ? 1 : 0;
+    P->precond = r->opt.precond;
     if (int s = cp::enqueue_rm_reset(P)) return s;
     r->k = 0;
     r->prev_rel = 1.;
@@ -195,6 +201,8 @@ int advance(cp_server* S, Request* r, const Mirror& m)
         if (!c.done) return CP_OK;
         r->cg_iters += c.iters;
         if (c.status == 2) return fail_request(S, r, CP_ERR_NAN);
+        P->last_cg_iters = c.iters;
+        if (P->precond == 2) cp::mg_cg_observe(P, c);
         if (r->opt.precision == CP_FP64) {
             if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
                                               r->opt.tol, r->tight))
@@ -202,6 +210,16 @@ int advance(cp_server* S, Request* r, const Mirror& m)
             r->phase = PH_WAIT_RM;
         } else {
             if (int s = cp::enqueue_correction(P)) return s;
+            // as cp_solve: an inexact secant iteration's FP32 solve that met
+            // the whole reduction ends the refinement
+            const double inner = r->opt.inner_tol > 0 ? r->opt.inner_tol : 1e-3;
+            if (!r->tight && r->itol > inner && c.status == 0 && c.rho0 > 0. && c.rho <= r->itol * r->itol * c.rho0) {
+                if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
+                                                  r->opt.tol, r->tight))
+                    return s;
+                r->phase = PH_WAIT_RM;
+                return CP_OK;
+            }
             if (int s = cp::enqueue_residual(P)) return s;
             r->phase = PH_WAIT_RES;
         }
@@ -223,12 +241,16 @@ int advance(cp_server* S, Request* r, const Mirror& m)
         const double inner = r->opt.inner_tol > 0 ? r->opt.inner_tol : 1e-3;
         const double need = 0.5 * r->ctol * std::sqrt(r->ref_norm / rn);
         const double itol = std::max(inner, std::min(need, 0.5));
-        const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : 1000000000;
+        r->itol = itol;
+        r->rn = rn;
+        const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : (P->precond == 2 ? 2000 : 1000000000);
         if (int s = cp::enqueue_cg_init(P, CP_MIXED, rn, itol, maxit)) return s;
+        if (P->precond == 2)
+            if (int s = cp::mg_cg_enqueue_start(P, CP_MIXED)) return s;
         r->phase = PH_CG;
         r->cg_rho_prev = -1.;
         r->cg_iters_seen = 0;
-        r->next_chunk = 16;
+        r->next_chunk = P->precond == 2 ? cp::mg_cg_first_chunk(P, itol * itol, true) : 16;
         return CP_OK;
     }
     case PH_WAIT_RM: {
@@ -271,6 +293,8 @@ int admit(cp_server* S)
         int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
                                    r->u0.data(), r->mask.data(), &r->mat, S->stream, &P);
         if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
+        if (s == CP_OK && r->opt.precond == 2)
+            s = cp_problem_set_grid(P, r->n1, r->n2, r->node_of.empty() ? nullptr : r->node_of.data());
         if (s != CP_OK) {
             if (P) cp_problem_destroy(P);
             fail_request(S, r, s);
@@ -369,12 +393,28 @@ int cp_server_submit(cp_server* S, int nn, int ne, const double* x, const double
                      const double* u0, const unsigned char* mask, const cp_material* mat, int nsteps,
                      double final_factor, const cp_options* opt, long long* id)
 {
+    if (opt && opt->precond == 2)
+        return cp::fail(CP_ERR_INVALID, "cp_server_submit: multigrid (precond 2) needs the lattice: cp_server_submit_grid");
+    return cp_server_submit_grid(S, nn, ne, x, y, conn, rp, u0, mask, mat, nsteps, final_factor, opt, 0, 0, nullptr, id);
+}
+
+int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn,
+                          const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                          int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
+                          long long* id)
+{
     if (!S || !x || !y || !conn || !rp || !mask || !mat || !opt || nsteps < 1 || nn < 1 || ne < 1)
         return cp::fail(CP_ERR_INVALID, "cp_server_submit: bad arguments");
     if (opt->precision != CP_FP64 && opt->precision != CP_MIXED) return cp::fail(CP_ERR_INVALID, "bad precision");
-    if (opt->precond < 0 || opt->precond > 1)
-        return cp::fail(CP_ERR_INVALID, "cp_server_submit: the serving loop runs plain or block-Jacobi CG (precond 0 or 1)");
+    if (opt->precond < 0 || opt->precond > 2) return cp::fail(CP_ERR_INVALID, "cp_server_submit: precond must be 0, 1 or 2");
+    if (opt->precond == 2 && (n1 < 2 || n2 < 2 || (long long)n1 * n2 != nn))
+        return cp::failf(CP_ERR_INVALID, "cp_server_submit_grid: %d x %d lattice for %d nodes", n1, n2, nn);
     std::unique_ptr<Request> r(new Request());
+    if (opt->precond == 2) {
+        r->n1 = n1;
+        r->n2 = n2;
+        if (node_of) r->node_of.assign(node_of, node_of + nn);
+    }
     r->id = S->next_id++;
     r->nn = nn;
     r->ne = ne;
@@ -392,6 +432,7 @@ int cp_server_submit(cp_server* S, int nn, int ne, const double* x, const double
     // matrix blocks of a quad mesh ~ 9 per node: budget with the layout estimate
     size_t bytes = 0;
     if (int s = cp_problem_bytes(nn, ne, 4LL * 9 * nn, opt->precision, &bytes)) return s;
+    if (opt->precond == 2) bytes += cp::mg_bytes_estimate(nn, opt->precision);
     r->bytes = bytes;
     r->t_submit = S->now();
     Request* raw = r.get();
@@ -408,10 +449,14 @@ int cp_server_step(cp_server* S, int* in_flight)
     hipStream_t st = S->stream;
     std::vector<Request*>& act = S->active;
     if (!act.empty()) {
-        // requests in their CG phase, grouped by (precision, preconditioner)
-        std::vector<Request*> g64[2], g32[2];
-        for (Request* r : act)
-            if (r->phase == PH_CG) (r->opt.precision == CP_FP64 ? g64 : g32)[r->opt.precond ? 1 : 0].push_back(r);
+        // requests in their CG phase, grouped by (precision, preconditioner);
+        // multigrid requests replay their own iteration graphs instead
+        std::vector<Request*> g64[2], g32[2], gmg;
+        for (Request* r : act) {
+            if (r->phase != PH_CG) continue;
+            if (r->opt.precond == 2) gmg.push_back(r);
+            else (r->opt.precision == CP_FP64 ? g64 : g32)[r->opt.precond ? 1 : 0].push_back(r);
+        }
         // device tables: [CgArgs groups][tiles][pack items] then the pack output
         size_t off = 0;
         auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
@@ -469,7 +514,7 @@ int cp_server_step(cp_server* S, int* in_flight)
         // many as the member closest to convergence is predicted to need
         int chunk = 0;
         for (Request* r : act)
-            if (r->phase == PH_CG) chunk = chunk == 0 ? r->next_chunk : std::min(chunk, r->next_chunk);
+            if (r->phase == PH_CG && r->opt.precond != 2) chunk = chunk == 0 ? r->next_chunk : std::min(chunk, r->next_chunk);
         chunk = std::max(4, chunk == 0 ? 16 : chunk);
         for (const Launch& L : launches) {
             if (S->cfg.batch_cg) {
@@ -488,6 +533,8 @@ int cp_server_step(cp_server* S, int* in_flight)
                 }
             }
         }
+        for (Request* r : gmg)
+            if (int s = cp::mg_cg_enqueue_iterations(r->P, r->opt.precision, r->next_chunk)) return s;
         CP_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_pack, dim3(ni), dim3(64), 0, st, reinterpret_cast<const PackItem*>(S->dev + pack_off), ni,
                            S->dev + mir_off);
@@ -504,12 +551,16 @@ int cp_server_step(cp_server* S, int* in_flight)
             if (r->phase == PH_CG) {
                 // remaining-iteration estimate from the observed decay of (r, r)
                 const cpk::CgScal& g = mcopy[i].cg;
-                int next = std::min(1024, 2 * chunk);
-                if (r->cg_rho_prev > 0. && g.rho > 0. && g.rho < r->cg_rho_prev && g.iters > r->cg_iters_seen) {
+                const bool mg = r->opt.precond == 2;
+                int next = mg ? std::min(64, 2 * r->next_chunk) : std::min(1024, 2 * chunk);
+                const double r0 = r->cg_rho_prev > 0. ? r->cg_rho_prev : (mg ? g.rho0 : -1.);
+                if (r0 > 0. && g.rho > 0. && g.rho < r0 && g.iters > r->cg_iters_seen) {
                     const double target = g.tol2 * g.bnorm2;
-                    const double rate = std::log(r->cg_rho_prev / g.rho) / double(g.iters - r->cg_iters_seen);
+                    const double rate = std::log(r0 / g.rho) / double(g.iters - r->cg_iters_seen);
                     const double rem = std::log(g.rho / target) / rate;
-                    if (rem > 0 && rem < 1e7) next = (int)std::min(1024., std::max(4., rem * 1.05 + 2.));
+                    if (rem > 0 && rem < 1e7)
+                        next = mg ? (int)std::min(64., std::max(1., std::ceil(rem)))
+                                  : (int)std::min(1024., std::max(4., rem * 1.05 + 2.));
                 }
                 r->cg_rho_prev = g.rho;
                 r->cg_iters_seen = g.iters;

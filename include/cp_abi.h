@@ -313,11 +313,18 @@ typedef struct {
 int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out);
 int cp_server_destroy(cp_server* srv);
 /* Submit one problem: mesh + loads + material + nsteps load levels to
- * final_factor with options; inputs are copied. */
+ * final_factor with options (precond 0 or 1); inputs are copied. */
 int cp_server_submit(cp_server* srv, int nn, int ne, const double* x, const double* y, const int* conn,
                      const double* rp, const double* u0, const unsigned char* mask,
                      const cp_material* mat, int nsteps, double final_factor, const cp_options* opt,
                      long long* id);
+/* The same with the mesh's lattice for multigrid CG (opt->precond = 2; n1 x
+ * n2 nodes, node_of as in cp_problem_set_grid, may be NULL); precond 0 and 1
+ * ignore the lattice.  New: the reference has no serving loop. */
+int cp_server_submit_grid(cp_server* srv, int nn, int ne, const double* x, const double* y, const int* conn,
+                          const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                          int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
+                          long long* id);
 /* Advance the server: admit queued requests that fit, run one scheduling
  * round of device work.  Returns the number of requests still in flight. */
 int cp_server_step(cp_server* srv, int* in_flight);

@@ -68,6 +68,50 @@ def test_served_results_equal_alone(ctx, batch):
     srv.close()
 
 
+MG_CASES = [
+    (plate(40, 20), (41, 21), 4, A.MIXED),
+    (ring(48, 12), (49, 13), 3, A.MIXED),
+    (plate(60, 30, r=0.15), (61, 31), 4, A.FP64),
+    (plate(24, 10, r=0.2, sY=300.0, law=A.LAW_SWIFT), (25, 11), 3, A.MIXED),
+]
+
+
+def mg_opts(prec):
+    return A.options(cg_tol=1e-9, tol=1e-6, aitken=True, max_outer=500, precision=prec, precond=2,
+                     predictor=1, cg_forcing=0.3, inner_tol=1e-5)
+
+
+def test_served_multigrid_equal_alone(ctx):
+    """Multigrid requests (each replays its own V-cycle graphs inside the
+    serving rounds, next to block-Jacobi requests in the batched kernels)
+    give bitwise the results of the same problems solved alone."""
+    alone = []
+    for (mesh, loads, mat), grid, nsteps, prec in MG_CASES:
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(*grid)
+        P.solve_steps(nsteps, 1.0, mg_opts(prec))
+        alone.append((P.field("u"), P.field("sigI")))
+        P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 30, max_slots=3, batch_cg=True)
+    ids = [srv.submit(m, l, mat, n, 1.0, mg_opts(p), grid=g) for (m, l, mat), g, n, p in MG_CASES]
+    (bm, bl, bmat), bn, bp = CASES[0]
+    bj = srv.submit(bm, bl, bmat, bn, 1.0, opts(bp))          # a block-Jacobi request in the same rounds
+    srv.run_until_done()
+    for i, rid in enumerate(ids):
+        code, info, u, sig = srv.result(rid)
+        print("request", rid, "status", info.status, "outer", info.outer_iters, "cg", info.cg_iters,
+              "latency", info.t_done - info.t_submit, flush=True)
+        assert code == A.CP_OK
+        assert np.array_equal(u, alone[i][0]), (i, np.abs(u - alone[i][0]).max())
+        assert np.array_equal(sig, alone[i][1])
+    assert srv.result(bj)[0] == A.CP_OK
+    with pytest.raises(A.CPError) as e:                        # lattice of the wrong size
+        (m, l, mat), g, n, p = MG_CASES[0]
+        srv.submit(m, l, mat, n, 1.0, mg_opts(p), grid=(g[0], g[1] + 1))
+    assert e.value.code == A.CP_ERR_INVALID
+    srv.close()
+
+
 def test_budget_and_slots(ctx):
     (m, l, mat), _, _ = CASES[0]
     import ctypes as C
