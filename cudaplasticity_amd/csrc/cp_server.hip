@@ -47,6 +47,7 @@ struct Request {
     size_t bytes = 0;
     State state = RQ_QUEUED;
     cp_problem* P = nullptr;
+    hipEvent_t ev = nullptr;         // multigrid requests: end of this round's work on their own stream
     int phase = PH_LEVEL;
     int level = 0, k = 0, tight = 1;
     double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1., itol = 0., rn = 0.;
@@ -123,16 +124,25 @@ int ensure_buffers(cp_server* S, size_t bytes)
     return CP_OK;
 }
 
-int fail_request(cp_server* S, Request* r, int code)
+void release(cp_server* S, Request* r)
 {
-    r->status = code;
-    r->state = RQ_FAILED;
-    r->t_done = S->now();
     if (r->P) {
         S->used -= r->bytes;
         cp_problem_destroy(r->P);
         r->P = nullptr;
     }
+    if (r->ev) {
+        (void)hipEventDestroy(r->ev);
+        r->ev = nullptr;
+    }
+}
+
+int fail_request(cp_server* S, Request* r, int code)
+{
+    r->status = code;
+    r->state = RQ_FAILED;
+    r->t_done = S->now();
+    release(S, r);
     return CP_OK;
 }
 
@@ -290,8 +300,15 @@ int admit(cp_server* S)
         if (S->used + r->bytes > S->cfg.hbm_budget) break;
         S->queue.pop_front();
         cp_problem* P = nullptr;
+        // a multigrid request works on a stream of its own, so the V-cycles
+        // of several requests (small problems leave most of the GPU idle)
+        // run side by side; block-Jacobi requests share the server's stream
+        // with the batched CG kernels
+        const bool mg = r->opt.precond == 2;
         int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
-                                   r->u0.data(), r->mask.data(), &r->mat, S->stream, &P);
+                                   r->u0.data(), r->mask.data(), &r->mat, mg ? nullptr : S->stream, &P);
+        if (s == CP_OK && mg && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+            s = cp::fail(CP_ERR_DEVICE, "cp_server: event creation failed");
         if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
         if (s == CP_OK && r->opt.precond == 2)
             s = cp_problem_set_grid(P, r->n1, r->n2, r->node_of.empty() ? nullptr : r->node_of.data());
@@ -373,8 +390,10 @@ int cp_server_destroy(cp_server* S)
 {
     if (!S) return CP_OK;
     (void)hipStreamSynchronize(S->stream);
-    for (auto& kv : S->reqs)
+    for (auto& kv : S->reqs) {
         if (kv.second->P) cp_problem_destroy(kv.second->P);
+        if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+    }
     if (S->dev) (void)hipFree(S->dev);
     if (S->host) (void)hipHostFree(S->host);
     (void)hipStreamDestroy(S->stream);
@@ -535,6 +554,12 @@ int cp_server_step(cp_server* S, int* in_flight)
         }
         for (Request* r : gmg)
             if (int s = cp::mg_cg_enqueue_iterations(r->P, r->opt.precision, r->next_chunk)) return s;
+        // the scalar pack waits for every request working on its own stream
+        for (Request* r : act)
+            if (r->ev) {
+                CP_HIP(hipEventRecord(r->ev, r->P->stream));
+                CP_HIP(hipStreamWaitEvent(st, r->ev, 0));
+            }
         CP_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_pack, dim3(ni), dim3(64), 0, st, reinterpret_cast<const PackItem*>(S->dev + pack_off), ni,
                            S->dev + mir_off);
@@ -576,9 +601,7 @@ int cp_server_step(cp_server* S, int* in_flight)
                 } else {
                     r->state = RQ_DONE;
                     r->t_done = S->now();
-                    S->used -= r->bytes;
-                    cp_problem_destroy(r->P);
-                    r->P = nullptr;
+                    release(S, r);
                 }
             }
             if (r->state == RQ_ACTIVE) keep.push_back(r);
