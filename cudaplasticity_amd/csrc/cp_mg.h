@@ -621,6 +621,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename
 // [4/30, 4.4] with no eigenvalue estimate.
 constexpr int kSeg = 16;           // segment length (direction 1) and tile height
 constexpr int kSegCols = kBlock / kSeg;   // tile width (direction 2): 16 columns
+constexpr int kSegPitch = kSegCols + 1;   // LDS row pitch of a tile
 constexpr double kSegLam = 4.;
 
 struct SegGrid {
@@ -759,9 +760,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
 {
     using V = typename V2<T>::type;
     if (v.sc && v.sc->done) return;
-    __shared__ T sy[2][kBlock];    // Dt^-1 rho, then y, then z
-    __shared__ T sf[4][kBlock];    // Dt^-1 L
-    __shared__ T sh[4][kBlock];    // Dt^-1 U
+    // per node (row k of the tile, column c at k * kSegPitch + c: padded
+    // against LDS bank conflicts in the column-wise scans)
+    __shared__ T sy[2][kSeg * kSegPitch];    // Dt^-1 rho, then z
+    __shared__ T sf[4][kSeg * kSegPitch];    // Dt^-1 L
+    __shared__ T sh[4][kSeg * kSegPitch];    // Dt^-1 U
     const int tile = xcd_block(g.ntiles);
     const int T1 = tile / g.nt2, T2 = tile - T1 * g.nt2;
     const int t1 = threadIdx.x / kSegCols, t2 = threadIdx.x - t1 * kSegCols;
@@ -880,41 +883,66 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
         h[3] = mi.y * bu[1] + mi.z * bu[3];
     }
     const int tid = threadIdx.x;
-    sy[0][tid] = ax;
-    sy[1][tid] = ay;
+    const int q1 = t1 * kSegPitch + t2;
+    sy[0][q1] = ax;
+    sy[1][q1] = ay;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        sf[j][tid] = f[j];
-        sh[j][tid] = h[j];
+        sf[j][q1] = f[j];
+        sh[j][q1] = h[j];
     }
     __syncthreads();
-    if (tid < kSegCols) {
-        T yx = 0, yy = 0;
-#pragma unroll 2
-        for (int k = 0; k < kSeg; ++k) {
-            const int q = k * kSegCols + tid;
-            const T nx = sy[0][q] - (sf[0][q] * yx + sf[1][q] * yy);
-            const T ny = sy[1][q] - (sf[2][q] * yx + sf[3][q] * yy);
-            yx = nx;
-            yy = ny;
-            sy[0][q] = yx;
-            sy[1][q] = yy;
+    {
+        // the two recurrences of each segment as parallel scans over its 16
+        // nodes, held by 16 consecutive lanes (column c, row k):
+        //   y_k = a_k - F_k y_{k-1}: prefix compositions of y -> v + M y,
+        //   z_k = y_k - H_k z_{k+1}: suffix compositions of z -> w + N z
+        const int c = tid / kSeg, k = tid - c * kSeg;
+        const int q = k * kSegPitch + c;
+        T v0 = sy[0][q], v1 = sy[1][q];
+        T m00 = -sf[0][q], m01 = -sf[1][q], m10 = -sf[2][q], m11 = -sf[3][q];
+#pragma unroll
+        for (int d = 1; d < kSeg; d <<= 1) {
+            const T pv0 = __shfl_up(v0, d, kSeg), pv1 = __shfl_up(v1, d, kSeg);
+            const T p00 = __shfl_up(m00, d, kSeg), p01 = __shfl_up(m01, d, kSeg);
+            const T p10 = __shfl_up(m10, d, kSeg), p11 = __shfl_up(m11, d, kSeg);
+            if (k >= d) {
+                const T n0 = v0 + m00 * pv0 + m01 * pv1, n1 = v1 + m10 * pv0 + m11 * pv1;
+                const T a00 = m00 * p00 + m01 * p10, a01 = m00 * p01 + m01 * p11;
+                const T a10 = m10 * p00 + m11 * p10, a11 = m10 * p01 + m11 * p11;
+                v0 = n0;
+                v1 = n1;
+                m00 = a00;
+                m01 = a01;
+                m10 = a10;
+                m11 = a11;
+            }
         }
-        T zx = 0, zy = 0;
-#pragma unroll 2
-        for (int k = kSeg - 1; k >= 0; --k) {
-            const int q = k * kSegCols + tid;
-            const T nx = sy[0][q] - (sh[0][q] * zx + sh[1][q] * zy);
-            const T ny = sy[1][q] - (sh[2][q] * zx + sh[3][q] * zy);
-            zx = nx;
-            zy = ny;
-            sy[0][q] = zx;
-            sy[1][q] = zy;
+        // v = y_k
+        T n00 = -sh[0][q], n01 = -sh[1][q], n10 = -sh[2][q], n11 = -sh[3][q];
+#pragma unroll
+        for (int d = 1; d < kSeg; d <<= 1) {
+            const T pw0 = __shfl_down(v0, d, kSeg), pw1 = __shfl_down(v1, d, kSeg);
+            const T p00 = __shfl_down(n00, d, kSeg), p01 = __shfl_down(n01, d, kSeg);
+            const T p10 = __shfl_down(n10, d, kSeg), p11 = __shfl_down(n11, d, kSeg);
+            if (k + d < kSeg) {
+                const T w0 = v0 + n00 * pw0 + n01 * pw1, w1 = v1 + n10 * pw0 + n11 * pw1;
+                const T a00 = n00 * p00 + n01 * p10, a01 = n00 * p01 + n01 * p11;
+                const T a10 = n10 * p00 + n11 * p10, a11 = n10 * p01 + n11 * p11;
+                v0 = w0;
+                v1 = w1;
+                n00 = a00;
+                n01 = a01;
+                n10 = a10;
+                n11 = a11;
+            }
         }
+        sy[0][q] = v0;
+        sy[1][q] = v1;
     }
     __syncthreads();
     if (!in) return;
-    const T gx = sy[0][tid], gy = sy[1][tid];
+    const T gx = sy[0][q1], gy = sy[1][q1];
     if constexpr (MODE == SEG_X0 || MODE == SEG_RX0) {
         xo[n] = V{c.c0 * gx, c.c0 * gy};
     } else if constexpr (MODE == SEG_FIRST) {
