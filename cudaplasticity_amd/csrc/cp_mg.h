@@ -620,8 +620,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename
 // colour, so lambda_max(D^-1 A) <= 4: the Chebyshev interval is
 // [4/30, 4.4] with no eigenvalue estimate.
 constexpr int kSeg = 16;           // segment length (direction 1) and tile height
-constexpr int kSegCols = kBlock / kSeg;   // tile width (direction 2): 16 columns
-constexpr int kSegPitch = kSegCols + 1;   // LDS row pitch of a tile
+// Tile width (direction 2) of a smoothing tile: 32 columns (512 threads) on
+// the finest level, where the wider rows of consecutive nodes stream the
+// matrix better; 16 (256 threads) on the coarse levels, which need the
+// workgroups for parallelism.
+template <int W> constexpr int seg_cols() { return W == kLatW ? 16 : 32; }
 constexpr double kSegLam = 4.;
 
 struct SegGrid {
@@ -754,7 +757,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 enum { SEG_X0 = 4, SEG_FIRST = 5, SEG_RX0 = 6 };
 
 template <int W, int MODE, typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
+__global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
                                                           const typename V2<T>::type* di, typename V2<T>::type* xo,
                                                           typename V2<T>::type* dout, int second, int last, SegGrid g)
 {
@@ -762,6 +765,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> 
     if (v.sc && v.sc->done) return;
     // per node (row k of the tile, column c at k * kSegPitch + c: padded
     // against LDS bank conflicts in the column-wise scans)
+    constexpr int kSegCols = seg_cols<W>(), kSegPitch = kSegCols + 1;
     __shared__ T sy[2][kSeg * kSegPitch];    // Dt^-1 rho, then z
     __shared__ T sf[4][kSeg * kSegPitch];    // Dt^-1 L
     __shared__ T sh[4][kSeg * kSegPitch];    // Dt^-1 U

@@ -218,7 +218,8 @@ static cpk::SegGrid seg_grid(const MgState& G, int l)
     cpk::SegGrid g{};
     g.n1 = L.n1;
     g.n2 = L.n2;
-    g.nt2 = (L.n2 + cpk::kSegCols - 1) / cpk::kSegCols;
+    const int cols = l == 0 ? cpk::seg_cols<0>() : cpk::seg_cols<cpk::kLatW>();
+    g.nt2 = (L.n2 + cols - 1) / cols;
     g.ntiles = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * g.nt2;
     g.node_of = l == 0 ? G.node0 : nullptr;
     g.slot_pos = l == 0 ? G.slot0 : nullptr;
@@ -328,7 +329,7 @@ static void smooth_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const vo
                        void* dout, int second, int last, const cpk::SegGrid& g, hipStream_t st)
 {
     using V = typename cpk::V2<T>::type;
-    hipLaunchKernelGGL((cpk::k_mg_smooth_seg<W, MODE, T>), dim3(g.ntiles), dim3(cpk::kBlock), 0, st, a, v,
+    hipLaunchKernelGGL((cpk::k_mg_smooth_seg<W, MODE, T>), dim3(g.ntiles), dim3(cpk::kSeg * cpk::seg_cols<W>()), 0, st, a, v,
                        (const V*)xi, (const V*)di, (V*)xo, (V*)dout, second, last, g);
 }
 
