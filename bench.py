@@ -41,8 +41,8 @@ NSTEPS = 20
 # 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000):
 # the fused block-Jacobi CG iteration (--precond 1) 48.19 MB read + 8.05 MB
 # written; the finest level's segment smoothing step (multigrid, the
-# default) 45.16 MB read + 3.21 MB written.
-MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.37e6}
+# default) 44.93 MB read + 3.21 MB written.
+MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6}
 
 
 def log(*a):
