@@ -159,12 +159,12 @@ void or_renumerate_ring(int nn, int ne, double* x, double* y, int* conn,
     free(tx); free(ty); free(nw);
 }
 
-/* genRectWithHole, order 1 -- Mesh.cu:112-245 */
-void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double holeRad, int N1, int N2,
-                           double* x, double* y, int* conn, int* bord, int* blen)
+/* genRectWithHole, order 1 -- Mesh.cu:112-245, with the radial segment
+ * count given (order 2 doubles N1, N2 and radN of the order-1 mesh, :117-121). */
+void or_gen_rect_with_hole_r(double x1, double x2, double y1, double y2, double holeRad, int N1, int N2,
+                             int radN, double* x, double* y, int* conn, int* bord, int* blen)
 {
     int borderN = (N1 + N2) * 2;
-    int radN = or_hole_radN(x1, x2, y1, y2, holeRad, N1, N2);
     int nodeCount = borderN * (radN + 1);
     blen[0] = blen[2] = N1 + 1;
     blen[1] = blen[3] = N2 + 1;
@@ -225,6 +225,13 @@ void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double ho
         conn[b + 0] = (i + 1) * borderN;
     }
     or_renumerate_ring(nodeCount, borderN * radN, x, y, conn, 5, bord, blen, borderN);
+}
+
+void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double holeRad, int N1, int N2,
+                           double* x, double* y, int* conn, int* bord, int* blen)
+{
+    or_gen_rect_with_hole_r(x1, x2, y1, y2, holeRad, N1, N2, or_hole_radN(x1, x2, y1, y2, holeRad, N1, N2), x, y,
+                            conn, bord, blen);
 }
 
 /* genRing, order 1 -- Mesh.cu:249-329 */
@@ -412,24 +419,82 @@ double or_elem_space4(const double* x, const double* y, const int* conn, int e)
     return fabs(s1 - s2) * 0.5;
 }
 
+/* elemSpace8 -- Mesh.cuh:269-278: shoelace area of the octagon corner,
+ * mid-side, corner, ... */
+double or_elem_space8(const double* x, const double* y, const int* conn, int e)
+{
+    const int* c = conn + 8 * (size_t)e;
+    double s1 = 0., s2 = 0.;
+    for (int i = 0; i < 3; ++i) {
+        s1 += x[c[i]] * y[c[i + 4]] + x[c[i + 4]] * y[c[i + 1]];
+        s2 += y[c[i]] * x[c[i + 4]] + y[c[i + 4]] * x[c[i + 1]];
+    }
+    s1 += x[c[3]] * y[c[7]] + x[c[7]] * y[c[0]];
+    s2 += y[c[3]] * x[c[7]] + y[c[7]] * x[c[0]];
+    return fabs(s1 - s2) * 0.5;
+}
+
+/* remapOrder -- Mesh.cu:443-503.  An order-2 mesh is the order-1 mesh of
+ * twice the resolution regrouped: in the strip numbering k = i*width + j of
+ * the order-1 elements, the 2x2 patch (i, j), (i, j+1), (i+1, j),
+ * (i+1, j+1) with i, j even becomes 8-node element (i/2)(width/2) + j/2;
+ * its corners are local nodes 0, 1, 2, 3 of quads (i+1, j+1), (i, j+1),
+ * (i, j), (i+1, j), its mid-sides local nodes 1, 2, 3, 0 of the same quads;
+ * the patch centre (local node 0 of quad (i, j)) is dropped and the other
+ * nodes keep their order.  Border lists (nbord entries in bord) are
+ * renumbered in place; sec[n] marks the mid-side nodes.  Returns the new
+ * node count. */
+int or_remap_order(int nn, int ne4, const double* x, const double* y, const int* conn4, int width,
+                   int nbord, int* bord, double* x8, double* y8, int* conn8, unsigned char* sec)
+{
+    const int ne8 = ne4 / 4;
+    unsigned char* keep = (unsigned char*)malloc((size_t)nn);
+    int* newId = (int*)malloc(sizeof(int) * (size_t)nn);
+    memset(keep, 1, (size_t)nn);
+    for (int i = 0; i < ne4 / width; i += 2)
+        for (int j = 0; j < width; j += 2) {
+            const int* qa = conn4 + 4 * ((size_t)(i + 1) * width + j + 1);
+            const int* qb = conn4 + 4 * ((size_t)i * width + j + 1);
+            const int* qc = conn4 + 4 * ((size_t)i * width + j);
+            const int* qd = conn4 + 4 * ((size_t)(i + 1) * width + j);
+            int* c = conn8 + 8 * ((size_t)(i / 2) * (width / 2) + j / 2);
+            c[0] = qa[0]; c[1] = qb[1]; c[2] = qc[2]; c[3] = qd[3];
+            c[4] = qa[1]; c[5] = qb[2]; c[6] = qc[3]; c[7] = qd[0];
+            keep[qc[0]] = 0;
+        }
+    int m = 0;
+    for (int n = 0; n < nn; ++n) {
+        newId[n] = keep[n] ? m : -1;
+        if (keep[n]) { x8[m] = x[n]; y8[m] = y[n]; ++m; }
+    }
+    for (size_t k = 0; k < 8 * (size_t)ne8; ++k) conn8[k] = newId[conn8[k]];
+    for (int k = 0; k < nbord; ++k) bord[k] = newId[bord[k]];
+    memset(sec, 0, (size_t)m);
+    for (size_t e = 0; e < (size_t)ne8; ++e)
+        for (int a = 4; a < 8; ++a) sec[conn8[8 * e + a]] = 1;
+    free(keep); free(newId);
+    return m;
+}
+
 /* ====================================================================== */
-/* CSR pattern -- CudaSparseSLAE.cuh:303-458 (dim = 2, 4-node elements).   */
+/* CSR pattern -- CudaSparseSLAE.cuh:303-458 (dim = 2; 4-node elements, or   */
+/* 8-node elements with the scan over elem8, :339-391).                    */
 /* adj(n) = other nodes of n's elements in first-appearance order scanning */
 /* elements in storage order and local nodes in local order.               */
 /* ====================================================================== */
-static int* build_adjacency(int nn, int ne, const int* conn, int** adjPos_out)
+static int* build_adjacency(int nn, int ne, int npe, const int* conn, int** adjPos_out)
 {
     int* cnt = (int*)calloc((size_t)nn + 1, sizeof(int));
-    for (int k = 0; k < 4 * ne; ++k) cnt[conn[k] + 1] += 3;
+    for (long long k = 0; k < (long long)npe * ne; ++k) cnt[conn[k] + 1] += npe - 1;
     for (int n = 0; n < nn; ++n) cnt[n + 1] += cnt[n];
     int* all = (int*)malloc(sizeof(int) * ((size_t)cnt[nn] + 1));
     int* fill = (int*)malloc(sizeof(int) * ((size_t)nn + 1));
     memcpy(fill, cnt, sizeof(int) * (size_t)nn);
     for (int e = 0; e < ne; ++e)
-        for (int i = 0; i < 4; ++i) {
-            int node = conn[4 * e + i];
-            for (int j = 0; j < 4; ++j)
-                if (i != j) all[fill[node]++] = conn[4 * e + j];
+        for (int i = 0; i < npe; ++i) {
+            int node = conn[(size_t)npe * e + i];
+            for (int j = 0; j < npe; ++j)
+                if (i != j) all[fill[node]++] = conn[(size_t)npe * e + j];
         }
     /* unique, first appearance */
     int* ucnt = (int*)calloc((size_t)nn + 1, sizeof(int));
@@ -454,19 +519,21 @@ static int* build_adjacency(int nn, int ne, const int* conn, int** adjPos_out)
     return adj;
 }
 
-long long or_csr_nnz(int nn, int ne, const int* conn)
+long long or_csr_nnz_n(int nn, int ne, int npe, const int* conn)
 {
     int* adjPos;
-    int* adj = build_adjacency(nn, ne, conn, &adjPos);
+    int* adj = build_adjacency(nn, ne, npe, conn, &adjPos);
     long long nnz = 4LL * ((long long)adjPos[nn] + nn);
     free(adj); free(adjPos);
     return nnz;
 }
 
-void or_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+long long or_csr_nnz(int nn, int ne, const int* conn) { return or_csr_nnz_n(nn, ne, 4, conn); }
+
+void or_csr_pattern_n(int nn, int ne, int npe, const int* conn, int* rows, int* cols)
 {
     int* adjPos;
-    int* adj = build_adjacency(nn, ne, conn, &adjPos);
+    int* adj = build_adjacency(nn, ne, npe, conn, &adjPos);
     /* rows[2n+i+1] = 4*begin + 2(i+1) size + 2(2n+i+1)   (CudaSparseSLAE.cuh:414-422) */
     rows[0] = 0;
     for (int n = 0; n < nn; ++n) {
@@ -487,6 +554,11 @@ void or_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
         }
     }
     free(adj); free(adjPos);
+}
+
+void or_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+{
+    or_csr_pattern_n(nn, ne, 4, conn, rows, cols);
 }
 
 /* ====================================================================== */
@@ -569,6 +641,91 @@ void or_form_ke4(const double B[4][2][4], const double detJ[4], const double C[9
     for (int q = 0; q < 36; ++q) Ke[q] *= h;
 }
 
+/* 3-point Gauss rule of the 8-node element as the reference writes it:
+ * points and weights truncated to 15 decimals (calcB8 :70, formKe8 :190,
+ * strain8 :1555), k = 3i + j for (r, s) = (p_i, p_j). */
+static const double OR_GS3_P[3] = {-0.774596669241483, 0., 0.774596669241483};
+static const double OR_GS3_W[3] = {0.555555555555555, 0.888888888888888, 0.555555555555555};
+
+/* Serendipity shape-function derivatives at (r, s).  Local order: corners
+ * (1,1), (-1,1), (-1,-1), (1,-1), then the mid-sides of edges 1-2, 2-3, 3-4,
+ * 4-1 (the order of calcB8 :66-134 and of remapOrder's elem8). */
+static void q8_dshape(double r, double s, double dr[8], double ds[8])
+{
+    static const double R[8] = {1., -1., -1., 1., 0., -1., 0., 1.};
+    static const double S[8] = {1., 1., -1., -1., 1., 0., -1., 0.};
+    for (int a = 0; a < 8; ++a) {
+        const double ra = R[a], sa = S[a];
+        if (a < 4) {          /* N = (1 + r ra)(1 + s sa)(r ra + s sa - 1)/4 */
+            dr[a] = 0.25 * ra * (1. + s * sa) * (2. * r * ra + s * sa);
+            ds[a] = 0.25 * sa * (1. + r * ra) * (r * ra + 2. * s * sa);
+        } else if (ra == 0.) { /* N = (1 - r^2)(1 + s sa)/2 */
+            dr[a] = -r * (1. + s * sa);
+            ds[a] = 0.5 * sa * (1. - r * r);
+        } else {              /* N = (1 + r ra)(1 - s^2)/2 */
+            dr[a] = 0.5 * ra * (1. - s * s);
+            ds[a] = -s * (1. + r * ra);
+        }
+    }
+}
+
+/* calcB8 -- PlasticitySolver.cu:66-134: J, detJ, B = [dN/dx; dN/dy] of the
+ * 8-node quad at the 9 Gauss points. */
+void or_calc_b8(const double* xx, const double* yy, const int* conn, int e,
+                double B[9][2][8], double detJ[9])
+{
+    double x[8], y[8];
+    for (int a = 0; a < 8; ++a) { x[a] = xx[conn[8 * (size_t)e + a]]; y[a] = yy[conn[8 * (size_t)e + a]]; }
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int k = 3 * i + j;
+            double dr[8], ds[8];
+            q8_dshape(OR_GS3_P[i], OR_GS3_P[j], dr, ds);
+            double J0 = 0., J1 = 0., J2 = 0., J3 = 0.;   /* x_r, y_r, x_s, y_s */
+            for (int a = 0; a < 8; ++a) {
+                J0 += dr[a] * x[a]; J1 += dr[a] * y[a];
+                J2 += ds[a] * x[a]; J3 += ds[a] * y[a];
+            }
+            detJ[k] = J0 * J3 - J1 * J2;
+            const double inv = 1. / detJ[k];
+            const double i0 = J3 * inv, i1 = -J1 * inv, i2 = -J2 * inv, i3 = J0 * inv;
+            for (int a = 0; a < 8; ++a) {
+                B[k][0][a] = i0 * dr[a] + i1 * ds[a];
+                B[k][1][a] = i2 * dr[a] + i3 * ds[a];
+            }
+        }
+}
+
+/* formKe8 -- PlasticitySolver.cu:188-224: Ke = h sum_k w_k detJ_k B^T C B, packed lower triangle (136). */
+void or_form_ke8(const double B[9][2][8], const double detJ[9], const double C[9], double h, double Ke[136])
+{
+    for (int q = 0; q < 136; ++q) Ke[q] = 0.;
+    for (int ig = 0; ig < 3; ++ig)
+        for (int jg = 0; jg < 3; ++jg) {
+            const int k = 3 * ig + jg;
+            const double dJ = detJ[k], w = OR_GS3_W[ig] * OR_GS3_W[jg];
+            for (int i = 0; i < 8; ++i) {
+                int i2 = i * 2;
+                double BC0[3], BC1[3];
+                for (int j = 0; j < 3; ++j) {
+                    BC0[j] = (B[k][0][i] * C[0 * 3 + j] + B[k][1][i] * C[2 * 3 + j]) * dJ;
+                    BC1[j] = (B[k][1][i] * C[1 * 3 + j] + B[k][0][i] * C[2 * 3 + j]) * dJ;
+                }
+                KE(i2, i2) += (BC0[0] * B[k][0][i] + BC0[2] * B[k][1][i]) * w;
+                KE(i2 + 1, i2) += (BC1[0] * B[k][0][i] + BC1[2] * B[k][1][i]) * w;
+                KE(i2 + 1, i2 + 1) += (BC1[1] * B[k][1][i] + BC1[2] * B[k][0][i]) * w;
+                for (int j = 0; j < i; ++j) {
+                    int j2 = j * 2;
+                    KE(i2, j2) += (BC0[0] * B[k][0][j] + BC0[2] * B[k][1][j]) * w;
+                    KE(i2, j2 + 1) += (BC0[1] * B[k][1][j] + BC0[2] * B[k][0][j]) * w;
+                    KE(i2 + 1, j2) += (BC1[0] * B[k][0][j] + BC1[2] * B[k][1][j]) * w;
+                    KE(i2 + 1, j2 + 1) += (BC1[1] * B[k][1][j] + BC1[2] * B[k][0][j]) * w;
+                }
+            }
+        }
+    for (int q = 0; q < 136; ++q) Ke[q] *= h;
+}
+
 static int csr_find(const int* rows, const int* cols, int r, int c)
 {
     for (int p = rows[r]; p < rows[r + 1]; ++p)
@@ -576,20 +733,22 @@ static int csr_find(const int* rows, const int* cols, int r, int c)
     return -1;
 }
 
-/* distribKe4 -- PlasticitySolver.cu:711-746 (sequential, element order) */
-static void distrib_ke4(const int* rows, const int* cols, const int* conn, int e, const double* Ke, double* vals)
+/* distribKe4 / distribKe8 -- PlasticitySolver.cu:711-784 (sequential, element order) */
+static void distrib_ke(int npe, const int* rows, const int* cols, const int* conn, int e, const double* Ke,
+                       double* vals)
 {
 #define ADD(r, c, v) vals[csr_find(rows, cols, (r), (c))] += (v)
-    for (int i = 0; i < 4; ++i) {
-        int gi = 2 * conn[4 * e + i];
+    const int* c = conn + (size_t)npe * e;
+    for (int i = 0; i < npe; ++i) {
+        int gi = 2 * c[i];
         ADD(gi, gi, KE(2 * i, 2 * i));
         ADD(gi + 1, gi, KE(2 * i + 1, 2 * i));
         ADD(gi, gi + 1, KE(2 * i + 1, 2 * i));
         ADD(gi + 1, gi + 1, KE(2 * i + 1, 2 * i + 1));
     }
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < npe; ++i)
         for (int j = 0; j < i; ++j) {
-            int gi = 2 * conn[4 * e + i], gj = 2 * conn[4 * e + j];
+            int gi = 2 * c[i], gj = 2 * c[j];
             ADD(gi, gj, KE(2 * i, 2 * j));
             ADD(gi + 1, gj, KE(2 * i + 1, 2 * j));
             ADD(gi, gj + 1, KE(2 * i, 2 * j + 1));
@@ -602,32 +761,88 @@ static void distrib_ke4(const int* rows, const int* cols, const int* conn, int e
 #undef ADD
 }
 
-/* fillGlobalStiffness (CPU) -- PlasticitySolver.cu:786-810: C from (E_c, nu_c), Ke, scatter. */
+/* fillGlobalStiffness (CPU) -- PlasticitySolver.cu:786-810: C from (E_c, nu_c), Ke, scatter.
+ * npe = 4 (formKe4) or 8 (formKe8). */
+void or_assemble_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+                   const int* rows, const int* cols, const double* Ec, const double* nuc,
+                   int plane, double h, double* vals)
+{
+    long long nnz = rows[2 * nn];
+    const int nke = npe == 8 ? 136 : 36;
+    memset(vals, 0, sizeof(double) * (size_t)nnz);
+    double* Kes = (double*)malloc(sizeof(double) * nke * (size_t)ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        double C[9];
+        or_form_c(plane, Ec[e], nuc[e], C);
+        if (npe == 8) {
+            double B[9][2][8], dJ[9];
+            or_calc_b8(x, y, conn, e, B, dJ);
+            or_form_ke8(B, dJ, C, h, Kes + nke * (size_t)e);
+        } else {
+            double B[4][2][4], dJ[4];
+            or_calc_b4(x, y, conn, e, B, dJ);
+            or_form_ke4(B, dJ, C, h, Kes + nke * (size_t)e);
+        }
+    }
+    for (int e = 0; e < ne; ++e) distrib_ke(npe, rows, cols, conn, e, Kes + nke * (size_t)e, vals);
+    free(Kes);
+}
+
 void or_assemble(int nn, int ne, const double* x, const double* y, const int* conn,
                  const int* rows, const int* cols, const double* Ec, const double* nuc,
                  int plane, double h, double* vals)
 {
-    long long nnz = rows[2 * nn];
-    memset(vals, 0, sizeof(double) * (size_t)nnz);
-    double* Kes = (double*)malloc(sizeof(double) * 36 * (size_t)ne);
-#pragma omp parallel for schedule(static)
-    for (int e = 0; e < ne; ++e) {
-        double B[4][2][4], dJ[4], C[9];
-        or_calc_b4(x, y, conn, e, B, dJ);
-        or_form_c(plane, Ec[e], nuc[e], C);
-        or_form_ke4(B, dJ, C, h, Kes + 36 * (size_t)e);
-    }
-    for (int e = 0; e < ne; ++e) distrib_ke4(rows, cols, conn, e, Kes + 36 * (size_t)e, vals);
-    free(Kes);
+    or_assemble_n(nn, ne, 4, x, y, conn, rows, cols, Ec, nuc, plane, h, vals);
 }
 
 /* ====================================================================== */
 /* Loads and boundary conditions -- PlasticitySolver.cu:812-1005           */
 /* ====================================================================== */
-void or_rhs_border_force(const double* x, const double* y, const int* border, int len,
-                         const double* fval, int normOrient, double h, double* rp)
+/* Border traction -- initConditions_ PlasticitySolver.cu:1022-1103.  The
+ * force is given per border position (fval[2p], fval[2p+1]).  A segment
+ * [n1, n2] adds f(n) h |n2 - n1| / 2 to both end nodes.  On an order-2
+ * border (sec[border[i]] set: border[i] is a mid-side node) the segment is
+ * [n1, n2, n3] and takes the consistent quadratic loads f1 L/6, f2 2L/3,
+ * f3 L/6 in the reference's form (:1025-1069); with normOrient the mid-side
+ * value uses the mean of the two half-segment frames.  sec may be NULL. */
+void or_rhs_border_force_q(const double* x, const double* y, const int* border, int len,
+                           const unsigned char* sec, const double* fval, int normOrient, double h, double* rp)
 {
     for (int i = 1; i < len; ++i) {
+        if (sec && sec[border[i]] && i + 1 < len) {
+            ++i;
+            int n1 = border[i - 2], n2 = border[i - 1], n3 = border[i];
+            double a1 = 0.5 * (x[n1] - x[n2]), b1 = 0.5 * (y[n1] - y[n2]);
+            double a2 = 0.5 * (x[n2] - x[n3]), b2 = 0.5 * (y[n2] - y[n3]);
+            double dl1 = sqrt(a1 * a1 + b1 * b1), dl2 = sqrt(a2 * a2 + b2 * b2);
+            double f1x = fval[2 * (i - 2)], f1y = fval[2 * (i - 2) + 1];
+            double f2x = fval[2 * (i - 1)], f2y = fval[2 * (i - 1) + 1];
+            double f3x = fval[2 * i], f3y = fval[2 * i + 1];
+            if (normOrient) {
+                double t1x = x[n2] - x[n1], t1y = y[n2] - y[n1], q = 1. / sqrt(t1x * t1x + t1y * t1y);
+                t1x *= q; t1y *= q;
+                double m1x = t1y, m1y = -t1x;
+                q = 1. / sqrt(m1x * m1x + m1y * m1y); m1x *= q; m1y *= q;
+                double t2x = x[n3] - x[n2], t2y = y[n3] - y[n2];
+                q = 1. / sqrt(t2x * t2x + t2y * t2y);
+                t2x *= q; t2y *= q;
+                double m2x = t2y, m2y = -t2x;
+                q = 1. / sqrt(m2x * m2x + m2y * m2y); m2x *= q; m2y *= q;
+                double g1x = m1x * f1x + t1x * f1y, g1y = m1y * f1x + t1y * f1y;
+                double g2x = 0.5 * ((m1x + m2x) * f2x + (t1x + t2x) * f2y);
+                double g2y = 0.5 * ((m1y + m2y) * f2x + (t1y + t2y) * f2y);
+                double g3x = m2x * f3x + t2x * f3y, g3y = m2y * f3x + t2y * f3y;
+                f1x = g1x; f1y = g1y; f2x = g2x; f2y = g2y; f3x = g3x; f3y = g3y;
+            }
+            rp[2 * n1] += f1x / 3. * (h * dl1 * 2.);
+            rp[2 * n1 + 1] += f1y / 3. * (h * dl1 * 2.);
+            rp[2 * n2] += f2x * 4. / 3. * (h * (dl1 + dl2));
+            rp[2 * n2 + 1] += f2y * 4. / 3. * (h * (dl1 + dl2));
+            rp[2 * n3] += f3x / 3. * (h * dl2 * 2.);
+            rp[2 * n3 + 1] += f3y / 3. * (h * dl2 * 2.);
+            continue;
+        }
         int n1 = border[i - 1], n2 = border[i];
         double hd1 = 0.5 * (x[n1] - x[n2]), hd2 = 0.5 * (y[n1] - y[n2]);
         double dl = sqrt(hd1 * hd1 + hd2 * hd2);
@@ -650,6 +865,12 @@ void or_rhs_border_force(const double* x, const double* y, const int* border, in
         rp[2 * n2] += f2x * c;
         rp[2 * n2 + 1] += f2y * c;
     }
+}
+
+void or_rhs_border_force(const double* x, const double* y, const int* border, int len,
+                         const double* fval, int normOrient, double h, double* rp)
+{
+    or_rhs_border_force_q(x, y, border, len, NULL, fval, normOrient, h, rp);
 }
 
 /* 4-node body force with a constant field (PlasticitySolver.cu:899-910) */
@@ -788,21 +1009,62 @@ void or_secant_update(const or_material* m, double Ec, double nuc,
 #define OR_OMEGA_MIN 0.05
 #define OR_OMEGA_MAX 100.0
 
-int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
-             const double* rp, const unsigned char* mask, double* u,
-             const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
+/* Element strains: strain4 (:1517-1549) = mean of B u over the 4 Gauss
+ * points; strain8 (:1551-1588) = (1/4) sum_k w_k B_k u over the 9 points. */
+static void element_strain(int npe, const double* Bs, const int* c, const double* u, double* exx, double* eyy,
+                           double* gxy)
+{
+    double ex = 0., ey = 0., gx = 0.;
+    if (npe == 8) {
+        const double (*B)[2][8] = (const double (*)[2][8])Bs;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                const double (*b)[8] = B[3 * i + j];
+                double el = 0., yl = 0., gl = 0.;
+                for (int k = 0; k < 8; ++k) {
+                    double uk = u[2 * c[k]], vk = u[2 * c[k] + 1];
+                    el += uk * b[0][k];
+                    yl += vk * b[1][k];
+                    gl += uk * b[1][k] + vk * b[0][k];
+                }
+                double w = OR_GS3_W[i] * OR_GS3_W[j];
+                ex += el * w; ey += yl * w; gx += gl * w;
+            }
+    } else {
+        const double (*B)[2][4] = (const double (*)[2][4])Bs;
+        double u1 = u[2 * c[0]], v1 = u[2 * c[0] + 1], u2 = u[2 * c[1]], v2 = u[2 * c[1] + 1],
+               u3 = u[2 * c[2]], v3 = u[2 * c[2] + 1], u4 = u[2 * c[3]], v4 = u[2 * c[3] + 1];
+        for (int g = 0; g < 4; ++g) {
+            const double (*b)[4] = B[g];
+            ex += u1 * b[0][0] + u2 * b[0][1] + u3 * b[0][2] + u4 * b[0][3];
+            ey += v1 * b[1][0] + v2 * b[1][1] + v3 * b[1][2] + v4 * b[1][3];
+            gx += u1 * b[1][0] + u2 * b[1][1] + u3 * b[1][2] + u4 * b[1][3] +
+                  v1 * b[0][0] + v2 * b[0][1] + v3 * b[0][2] + v4 * b[0][3];
+        }
+    }
+    *exx = ex * 0.25; *eyy = ey * 0.25; *gxy = gx * 0.25;
+}
+
+int or_solve_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+               const double* rp, const unsigned char* mask, double* u,
+               const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
 {
     int N = 2 * nn;
-    long long nnz = or_csr_nnz(nn, ne, conn);
+    long long nnz = or_csr_nnz_n(nn, ne, npe, conn);
     int* rows = (int*)malloc(sizeof(int) * (N + 1));
     int* cols = (int*)malloc(sizeof(int) * (size_t)nnz);
     double* vals = (double*)malloc(sizeof(double) * (size_t)nnz);
-    or_csr_pattern(nn, ne, conn, rows, cols);
+    or_csr_pattern_n(nn, ne, npe, conn, rows, cols);
 
-    double (*B)[2][4] = (double (*)[2][4])malloc(sizeof(double) * 32 * (size_t)ne);
-    double* dJ = (double*)malloc(sizeof(double) * 4 * (size_t)ne);
+    const int ngp = npe == 8 ? 9 : 4;
+    const size_t bstride = (size_t)ngp * 2 * npe;
+    double* B = (double*)malloc(sizeof(double) * bstride * (size_t)ne);
+    double* dJ = (double*)malloc(sizeof(double) * ngp * (size_t)ne);
 #pragma omp parallel for schedule(static)
-    for (int e = 0; e < ne; ++e) or_calc_b4(x, y, conn, e, (double (*)[2][4])(B + 4 * (size_t)e), dJ + 4 * (size_t)e);
+    for (int e = 0; e < ne; ++e) {
+        if (npe == 8) or_calc_b8(x, y, conn, e, (double (*)[2][8])(B + bstride * e), dJ + ngp * (size_t)e);
+        else or_calc_b4(x, y, conn, e, (double (*)[2][4])(B + bstride * e), dJ + ngp * (size_t)e);
+    }
 
     double* F[OR_NFIELDS];
     for (int f = 0; f < OR_NFIELDS; ++f) F[f] = st_ + (size_t)f * ne;
@@ -823,7 +1085,7 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
 
     double prev_rel = 1.;
     for (int k = 0; k < maxOuter; ++k) {
-        or_assemble(nn, ne, x, y, conn, rows, cols, F[OR_F_EC], F[OR_F_NUC], mat->plane, mat->h, vals);
+        or_assemble_n(nn, ne, npe, x, y, conn, rows, cols, F[OR_F_EC], F[OR_F_NUC], mat->plane, mat->h, vals);
         int it = 0;
         double ctol = opt->cg_tol;
         if (opt->cg_forcing > 0. && !opt->elastic && opt->fixed_outer <= 0) {
@@ -836,22 +1098,12 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
         stats->outer_iters = k + 1;
         if (cgs == OR_CG_NAN) { stats->status = 2; break; }
 
-        /* strain4 (:1517-1549) + plastStress (:1590-1610) */
+        /* strain4 / strain8 + plastStress (:1590-1610) */
         double num = 0., den = 0., rr = 0., dr = 0., dd = 0.;
 #pragma omp parallel for reduction(+ : num, den, rr, dr, dd) schedule(static)
         for (int e = 0; e < ne; ++e) {
-            const int* c = conn + 4 * e;
-            double u1 = u[2 * c[0]], v1 = u[2 * c[0] + 1], u2 = u[2 * c[1]], v2 = u[2 * c[1] + 1],
-                   u3 = u[2 * c[2]], v3 = u[2 * c[2] + 1], u4 = u[2 * c[3]], v4 = u[2 * c[3] + 1];
-            double exx = 0., eyy = 0., gxy = 0.;
-            for (int g = 0; g < 4; ++g) {
-                double (*b)[4] = B[4 * (size_t)e + g];
-                exx += u1 * b[0][0] + u2 * b[0][1] + u3 * b[0][2] + u4 * b[0][3];
-                eyy += v1 * b[1][0] + v2 * b[1][1] + v3 * b[1][2] + v4 * b[1][3];
-                gxy += u1 * b[1][0] + u2 * b[1][1] + u3 * b[1][2] + u4 * b[1][3] +
-                       v1 * b[0][0] + v2 * b[0][1] + v3 * b[0][2] + v4 * b[0][3];
-            }
-            exx *= 0.25; eyy *= 0.25; gxy *= 0.25;
+            double exx, eyy, gxy;
+            element_strain(npe, B + bstride * e, conn + (size_t)npe * e, u, &exx, &eyy, &gxy);
             double o[8];
             or_secant_update(mat, F[OR_F_EC][e], F[OR_F_NUC][e], exx, eyy, gxy, o);
             F[OR_F_EXX][e] = exx; F[OR_F_EYY][e] = eyy; F[OR_F_GXY][e] = gxy;
@@ -921,14 +1173,29 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
     return stats->status;
 }
 
-/* deformEnergy -- PlasticitySolver.cuh:415-445 (4-node part) */
-double or_deform_energy(int ne, const double* x, const double* y, const int* conn,
-                        const or_material* m, const double* st)
+int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
+             const double* rp, const unsigned char* mask, double* u,
+             const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
+{
+    return or_solve_n(nn, ne, 4, x, y, conn, rp, mask, u, mat, opt, st_, stats);
+}
+
+/* deformEnergy -- PlasticitySolver.cuh:415-445.  4-node elements: the
+ * elastic energy below sigmaT, above it the trapezoid integral of f over
+ * 100 steps; 8-node elements: epsI sigmaI / 2 times the element area in
+ * every case (the reference's second loop, :437-443). */
+double or_deform_energy_n(int ne, int npe, const double* x, const double* y, const int* conn,
+                          const or_material* m, const double* st)
 {
     const double* sigI = st + (size_t)OR_F_SIGI * ne;
     const double* epsI = st + (size_t)OR_F_EPSI * ne;
     double energy = 0.;
     for (int e = 0; e < ne; ++e) {
+        if (npe == 8) {
+            double vol = m->h * or_elem_space8(x, y, conn, e);
+            energy += epsI[e] * sigI[e] * 0.5 * vol;
+            continue;
+        }
         double vol = m->h * or_elem_space4(x, y, conn, e);
         if (sigI[e] > m->sigmaT) {
             double ET = m->sigmaT / m->E;
@@ -940,4 +1207,10 @@ double or_deform_energy(int ne, const double* x, const double* y, const int* con
             energy += epsI[e] * sigI[e] * 0.5 * vol;
     }
     return energy;
+}
+
+double or_deform_energy(int ne, const double* x, const double* y, const int* conn,
+                        const or_material* m, const double* st)
+{
+    return or_deform_energy_n(ne, 4, x, y, conn, m, st);
 }

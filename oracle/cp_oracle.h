@@ -56,6 +56,9 @@ void or_hole_sizes(double x1, double x2, double y1, double y2, double r, int N1,
                    int* nn, int* ne, int* nb, int* blen);
 void or_gen_rect_with_hole(double x1, double x2, double y1, double y2, double r, int N1, int N2,
                            double* x, double* y, int* conn, int* bord, int* blen);
+/* the same with the radial segment count given (sizes: borderN (radN + 1) nodes) */
+void or_gen_rect_with_hole_r(double x1, double x2, double y1, double y2, double r, int N1, int N2, int radN,
+                             double* x, double* y, int* conn, int* bord, int* blen);
 
 void or_ring_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen);
 void or_gen_ring(double a, double b, int Nphi, int Nr,
@@ -81,10 +84,18 @@ void or_gen_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
 
 int    or_max_index_diff(int ne, const int* conn);             /* Mesh.cu:417-441 */
 double or_elem_space4(const double* x, const double* y, const int* conn, int e); /* Mesh.cuh:258-267 */
+double or_elem_space8(const double* x, const double* y, const int* conn, int e); /* Mesh.cuh:269-278 */
+/* remapOrder (Mesh.cu:443-503): order-1 mesh of doubled resolution -> 8-node
+ * mesh; returns the new node count (nn - ne4/4). */
+int or_remap_order(int nn, int ne4, const double* x, const double* y, const int* conn4, int width,
+                   int nbord, int* bord, double* x8, double* y8, int* conn8, unsigned char* sec);
 
 /* ---------------- CSR pattern (CudaSparseSLAE.cuh:303-458) ---------------- */
 long long or_csr_nnz(int nn, int ne, const int* conn);
 void      or_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols);
+/* npe nodes per element (4 or 8). */
+long long or_csr_nnz_n(int nn, int ne, int npe, const int* conn);
+void      or_csr_pattern_n(int nn, int ne, int npe, const int* conn, int* rows, int* cols);
 
 /* ---------------- element math (PlasticitySolver.cu) ---------------- */
 void or_calc_b4(const double* x, const double* y, const int* conn, int e,
@@ -92,15 +103,27 @@ void or_calc_b4(const double* x, const double* y, const int* conn, int e,
 void or_form_c(int plane, double E, double nu, double C[9]);          /* :226-242 */
 void or_form_ke4(const double B[4][2][4], const double detJ[4], const double C[9],
                  double h, double Ke[36]);                              /* :156-186 */
+void or_calc_b8(const double* x, const double* y, const int* conn, int e,
+                double B[9][2][8], double detJ[9]);                     /* :66-134 */
+void or_form_ke8(const double B[9][2][8], const double detJ[9], const double C[9],
+                 double h, double Ke[136]);                             /* :188-224 */
 /* Assemble K(E_c, nu_c) into CSR values (fixed element order). */
 void or_assemble(int nn, int ne, const double* x, const double* y, const int* conn,
                  const int* rows, const int* cols, const double* Ec, const double* nuc,
                  int plane, double h, double* vals);
+void or_assemble_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+                   const int* rows, const int* cols, const double* Ec, const double* nuc,
+                   int plane, double h, double* vals);
 
 /* ---------------- loads / boundary conditions (PlasticitySolver.cu:812-1005) ---------------- */
 /* Border traction: force values given per border node (2 per node). */
 void or_rhs_border_force(const double* x, const double* y, const int* border, int len,
                          const double* fval, int normOrient, double h, double* rp);
+/* The same with order-2 (3-node) border segments where sec[border[i]] marks
+ * a mid-side node (:1025-1069); sec may be NULL. */
+void or_rhs_border_force_q(const double* x, const double* y, const int* border, int len,
+                           const unsigned char* sec, const double* fval, int normOrient, double h,
+                           double* rp);
 void or_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
                        double gx, double gy, double rho, double h, double* rp);
 void or_fix_axis(int nn, const double* x, const double* y, int vertical, double coord,
@@ -147,6 +170,11 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
              const double* rp, const unsigned char* mask, double* u,
              const or_material* mat, const or_options* opt, double* state, or_stats* st);
 
+/* The same for npe = 4 or 8 nodes per element (strain4 / strain8). */
+int or_solve_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+               const double* rp, const unsigned char* mask, double* u,
+               const or_material* mat, const or_options* opt, double* state, or_stats* st);
+
 /* Secant update for one element given its strains (updateParameters :1320-1335). */
 void or_secant_update(const or_material* mat, double Ec, double nuc,
                       double exx, double eyy, double gxy, double out[8]);
@@ -154,6 +182,8 @@ void or_secant_update(const or_material* mat, double Ec, double nuc,
 /* Strain energy (PlasticitySolver.cuh:415-445). */
 double or_deform_energy(int ne, const double* x, const double* y, const int* conn,
                         const or_material* mat, const double* state);
+double or_deform_energy_n(int ne, int npe, const double* x, const double* y, const int* conn,
+                          const or_material* mat, const double* state);
 
 /* y = A x over CSR (diagnostic). */
 void or_spmv(int N, const int* rows, const int* cols, const double* vals, const double* x, double* y);

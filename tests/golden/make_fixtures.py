@@ -9,6 +9,11 @@ Inputs (reference data/, ASCII outputs of the reference's own experiments):
   ring.vtk               Main.cpp:75-80   genRing(0.1, 0.2, 40, 5) mesh with
                          per-element space / aspect ratio / skew angle
   diagramLin.txt / diagramPow.txt  Main.cpp:929-994  uniaxial sigma(eps)
+  rect1.vtk, hole1.vtk, ring1.vtk  Main.cpp:1173-1190  order-2 (8-node) meshes of
+                         genRectangle / genRectWithHole / genRing
+  renumMesh.vtk          the 48x32 order-2 hole plate mesh (5120 8-node quads)
+  aitken_*.txt           Main.cpp:105-178 energyConvTest: deformEnergy per mesh
+                         of genRectWithHole(-3, 3, -2, 2, 0.5, N1, N2, order)
 Outputs: tests/golden/*.npz (data only: node coordinates, connectivity,
 per-element fields and nodal displacements exactly as printed) and
 tests/golden/vtk_layout.json (the section header lines of the three VTK
@@ -73,6 +78,19 @@ def main(ref_root):
     with open(os.path.join(HERE, "vtk_layout.json"), "w") as f:
         json.dump(layout, f, indent=0)
     print("vtk_layout", {k: len(v) for k, v in layout.items()})
+    for name in ("rect1", "hole1", "ring1", "renumMesh"):
+        v = read_vtk(os.path.join(data, name + ".vtk"))
+        np.savez_compressed(os.path.join(HERE, "mesh_" + name + ".npz"), points=v["points"], polys=v["polys"])
+        print(name, v["points"].shape, v["polys"].shape)
+    tables = {}
+    for fn in sorted(os.listdir(data)):
+        if fn.startswith("aitken") and fn.endswith(".txt"):
+            with open(os.path.join(data, fn)) as f:
+                rows = [ln.split() for ln in f.read().splitlines()[1:] if ln.strip()]
+            tables[fn[:-4]] = [[int(r[0]), r[1]] for r in rows]   # elements, energy as printed
+    with open(os.path.join(HERE, "aitken_energy.json"), "w") as f:
+        json.dump(tables, f, indent=0)
+    print("aitken tables", sorted(tables))
     lin = np.loadtxt(os.path.join(data, "diagramLin.txt"))
     pw = np.loadtxt(os.path.join(data, "diagramPow.txt"))
     np.savez_compressed(os.path.join(HERE, "diagram.npz"), lin=lin, pow=pw)

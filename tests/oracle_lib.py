@@ -50,6 +50,10 @@ def _load():
     lib.or_csr_nnz.restype = C.c_longlong
     lib.or_max_index_diff.restype = C.c_int
     lib.or_elem_space4.restype = C.c_double
+    lib.or_elem_space8.restype = C.c_double
+    lib.or_csr_nnz_n.restype = C.c_longlong
+    lib.or_deform_energy_n.restype = C.c_double
+    lib.or_solve_n.restype = C.c_int
     lib.or_deform_energy.restype = C.c_double
     lib.or_solve.restype = C.c_int
     lib.or_cg.restype = C.c_int
@@ -72,13 +76,17 @@ def u8p(a):
 
 
 class Mesh:
-    """Plain arrays: x, y (nn), conn (ne, 4) int32, borders (list of int32 arrays)."""
+    """Plain arrays: x, y (nn), conn (ne, npe) int32 with npe = 4 (bilinear
+    quads) or 8 (8-node quads), borders (list of int32 arrays), sec (nn
+    uint8, 1 on mid-side nodes; None for order 1)."""
 
-    def __init__(self, x, y, conn, borders):
+    def __init__(self, x, y, conn, borders, npe=4, sec=None):
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
-        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 4)
+        self.npe = npe
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, npe)
         self.borders = [np.ascontiguousarray(b, dtype=np.int32) for b in borders]
+        self.sec = None if sec is None else np.ascontiguousarray(sec, dtype=np.uint8)
 
     @property
     def nn(self):
@@ -105,23 +113,63 @@ def _gen(sizes_fn, sizes_args, gen_fn, gen_args):
     return Mesh(x, y, conn, borders)
 
 
-def gen_rect(x1, x2, y1, y2, N1, N2):
-    """genRectangle (Mesh.cu:17-109)."""
+def remap_order(m, width):
+    """remapOrder (Mesh.cu:443-503): order-1 mesh of doubled resolution ->
+    8-node mesh; `width` = elements per strip of the order-1 numbering."""
+    ne8 = m.ne // 4
+    nn8 = m.nn - ne8
+    x8, y8 = np.zeros(nn8), np.zeros(nn8)
+    conn8 = np.zeros(8 * ne8, dtype=np.int32)
+    sec = np.zeros(nn8, dtype=np.uint8)
+    bord = np.concatenate(m.borders).astype(np.int32) if m.borders else np.zeros(0, dtype=np.int32)
+    got = lib.or_remap_order(m.nn, m.ne, dp(m.x), dp(m.y), ip(m.conn), width, bord.size, ip(bord), dp(x8),
+                             dp(y8), ip(conn8), u8p(sec))
+    assert got == nn8
+    borders, off = [], 0
+    for b in m.borders:
+        borders.append(bord[off: off + b.size].copy())
+        off += b.size
+    return Mesh(x8, y8, conn8, borders, npe=8, sec=sec)
+
+
+def gen_rect(x1, x2, y1, y2, N1, N2, order=1):
+    """genRectangle (Mesh.cu:17-109); order 2 doubles N1, N2, then remapOrder(2 N2)."""
     f = C.c_double
-    return _gen(lib.or_rect_sizes, (N1, N2), lib.or_gen_rect, (f(x1), f(x2), f(y1), f(y2), N1, N2))
+    k = 2 if order == 2 else 1
+    m = _gen(lib.or_rect_sizes, (k * N1, k * N2), lib.or_gen_rect, (f(x1), f(x2), f(y1), f(y2), k * N1, k * N2))
+    return remap_order(m, 2 * N2) if order == 2 else m
 
 
-def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2):
-    """genRectWithHole + renumerateRing (Mesh.cu:112-245, 332-384)."""
+def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2, order=1):
+    """genRectWithHole + renumerateRing (Mesh.cu:112-245, 332-384); order 2
+    doubles N1, N2 and the radial count, then remapOrder(2 radN)."""
     f = C.c_double
-    return _gen(lib.or_hole_sizes, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2),
-                lib.or_gen_rect_with_hole, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2))
+    if order != 2:
+        return _gen(lib.or_hole_sizes, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2),
+                    lib.or_gen_rect_with_hole, (f(x1), f(x2), f(y1), f(y2), f(r), N1, N2))
+    radN = 2 * lib.or_hole_radN(f(x1), f(x2), f(y1), f(y2), f(r), N1, N2)
+    borderN = 4 * (N1 + N2)
+    nn, ne = borderN * (radN + 1), borderN * radN
+    blen = np.array([2 * N1 + 1, 2 * N2 + 1, 2 * N1 + 1, 2 * N2 + 1, borderN + 1, 0, 0, 0], dtype=np.int32)
+    x, y = np.zeros(nn), np.zeros(nn)
+    conn = np.zeros(4 * ne, dtype=np.int32)
+    bord = np.zeros(int(blen[:5].sum()), dtype=np.int32)
+    lib.or_gen_rect_with_hole_r(f(x1), f(x2), f(y1), f(y2), f(r), 2 * N1, 2 * N2, radN, dp(x), dp(y), ip(conn),
+                                ip(bord), ip(blen))
+    borders, off = [], 0
+    for b in range(5):
+        borders.append(bord[off: off + blen[b]].copy())
+        off += blen[b]
+    return remap_order(Mesh(x, y, conn, borders), radN)
 
 
-def gen_ring(a, b, Nphi, Nr):
-    """genRing + renumerateRing (Mesh.cu:249-329)."""
+def gen_ring(a, b, Nphi, Nr, order=1):
+    """genRing + renumerateRing (Mesh.cu:249-329); order 2 doubles Nphi, Nr,
+    then remapOrder(2 Nr)."""
     f = C.c_double
-    return _gen(lib.or_ring_sizes, (Nphi, Nr), lib.or_gen_ring, (f(a), f(b), Nphi, Nr))
+    k = 2 if order == 2 else 1
+    m = _gen(lib.or_ring_sizes, (k * Nphi, k * Nr), lib.or_gen_ring, (f(a), f(b), k * Nphi, k * Nr))
+    return remap_order(m, 2 * Nr) if order == 2 else m
 
 
 def gen_quarter_plate(L, r, Nt, Nr):
@@ -144,19 +192,42 @@ def material_f(mat, eps):
 
 
 def csr_pattern(mesh):
-    nnz = lib.or_csr_nnz(mesh.nn, mesh.ne, ip(mesh.conn))
+    nnz = lib.or_csr_nnz_n(mesh.nn, mesh.ne, mesh.npe, ip(mesh.conn))
     rows = np.zeros(2 * mesh.nn + 1, dtype=np.int32)
     cols = np.zeros(nnz, dtype=np.int32)
-    lib.or_csr_pattern(mesh.nn, mesh.ne, ip(mesh.conn), ip(rows), ip(cols))
+    lib.or_csr_pattern_n(mesh.nn, mesh.ne, mesh.npe, ip(mesh.conn), ip(rows), ip(cols))
     return rows, cols
 
 
 def assemble(mesh, rows, cols, Ec, nuc, plane=PLANE_STRESS, h=1.0):
     vals = np.zeros(cols.size)
-    lib.or_assemble(mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn), ip(rows), ip(cols),
-                    dp(np.ascontiguousarray(Ec, dtype=np.float64)),
-                    dp(np.ascontiguousarray(nuc, dtype=np.float64)), plane, C.c_double(h), dp(vals))
+    lib.or_assemble_n(mesh.nn, mesh.ne, mesh.npe, dp(mesh.x), dp(mesh.y), ip(mesh.conn), ip(rows), ip(cols),
+                      dp(np.ascontiguousarray(Ec, dtype=np.float64)),
+                      dp(np.ascontiguousarray(nuc, dtype=np.float64)), plane, C.c_double(h), dp(vals))
     return vals
+
+
+def calc_b8(mesh, e):
+    B = np.zeros((9, 2, 8))
+    dJ = np.zeros(9)
+    lib.or_calc_b8(dp(mesh.x), dp(mesh.y), ip(mesh.conn), e, dp(B), dp(dJ))
+    return B, dJ
+
+
+def form_ke8(B, dJ, Cm, h=1.0):
+    Ke = np.zeros(136)
+    lib.or_form_ke8(dp(np.ascontiguousarray(B)), dp(np.ascontiguousarray(dJ)),
+                    dp(np.ascontiguousarray(Cm, dtype=np.float64).reshape(9)), C.c_double(h), dp(Ke))
+    full = np.zeros((16, 16))
+    for a in range(16):
+        for b in range(a + 1):
+            full[a, b] = full[b, a] = Ke[a * (a + 1) // 2 + b]
+    return full
+
+
+def elem_space(mesh, e):
+    f = lib.or_elem_space8 if mesh.npe == 8 else lib.or_elem_space4
+    return f(dp(mesh.x), dp(mesh.y), ip(mesh.conn), e)
 
 
 def calc_b4(mesh, e):
@@ -194,18 +265,21 @@ class Problem:
         self.mask = np.ones(2 * mesh.nn, dtype=np.uint8)
         self.u = np.zeros(2 * mesh.nn)
 
+    def _sec(self):
+        return None if self.mesh.sec is None else u8p(self.mesh.sec)
+
     def force(self, border_id, fx, fy, norm_orient=False):
         b = self.mesh.borders[border_id]
         fv = np.tile(np.array([fx, fy], dtype=np.float64), b.size)
-        lib.or_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
-                                int(norm_orient), C.c_double(self.h), dp(self.rp))
+        lib.or_rhs_border_force_q(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, self._sec(), dp(fv),
+                                  int(norm_orient), C.c_double(self.h), dp(self.rp))
         return self
 
     def force_values(self, border_id, fv, norm_orient=False):
         b = self.mesh.borders[border_id]
         fv = np.ascontiguousarray(fv, dtype=np.float64).reshape(-1)
-        lib.or_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
-                                int(norm_orient), C.c_double(self.h), dp(self.rp))
+        lib.or_rhs_border_force_q(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, self._sec(), dp(fv),
+                                  int(norm_orient), C.c_double(self.h), dp(self.rp))
         return self
 
     def gravity(self, g=9.81, rho=1.0):
@@ -242,8 +316,8 @@ def solve(prob, mat, opt, u=None, state=None, rp=None):
     st = np.zeros((len(FIELDS), m.ne)) if state is None else np.ascontiguousarray(state).copy()
     rp = prob.rp if rp is None else np.ascontiguousarray(rp, dtype=np.float64)
     stats = Stats()
-    lib.or_solve(m.nn, m.ne, dp(m.x), dp(m.y), ip(m.conn), dp(rp), u8p(prob.mask), dp(u),
-                 C.byref(mat), C.byref(opt), dp(st), C.byref(stats))
+    lib.or_solve_n(m.nn, m.ne, m.npe, dp(m.x), dp(m.y), ip(m.conn), dp(rp), u8p(prob.mask), dp(u),
+                   C.byref(mat), C.byref(opt), dp(st), C.byref(stats))
     return u, st, stats
 
 
@@ -260,7 +334,7 @@ def cg(rows, cols, vals, b, mask, x0, tol, maxit=1000000, quirk=False):
 
 
 def deform_energy(mesh, mat, st):
-    return lib.or_deform_energy(mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn), C.byref(mat),
+    return lib.or_deform_energy_n(mesh.ne, mesh.npe, dp(mesh.x), dp(mesh.y), ip(mesh.conn), C.byref(mat),
                                 dp(np.ascontiguousarray(st)))
 
 
