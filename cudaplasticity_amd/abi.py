@@ -121,13 +121,18 @@ def options(cg_tol=1e-7, cg_maxit=0, tol=1e-5, max_outer=1000, fixed_outer=0, ai
 # ------------------------------------------------------------------ host-side setup (library code)
 
 class Mesh:
-    """Node coordinates, connectivity (ne x 4, reference local order) and borders."""
+    """Node coordinates, connectivity (ne x npe, reference local order) and
+    borders.  npe = 4 (bilinear quads, elem4) or 8 (8-node quads, elem8:
+    corners, then the mid-sides of edges 1-2, 2-3, 3-4, 4-1); sec marks the
+    mid-side nodes of an 8-node mesh (Mesh::secOrdNodes)."""
 
-    def __init__(self, x, y, conn, borders):
+    def __init__(self, x, y, conn, borders, npe=4, sec=None):
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.y = np.ascontiguousarray(y, dtype=np.float64)
-        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 4)
+        self.npe = npe
+        self.conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, npe)
         self.borders = [np.ascontiguousarray(b, dtype=np.int32) for b in borders]
+        self.sec = None if sec is None else np.ascontiguousarray(sec, dtype=np.uint8)
 
     @property
     def nn(self):
@@ -154,22 +159,47 @@ def _gen(sizes_name, sizes_args, gen_name, gen_args):
     return Mesh(x, y, conn, borders)
 
 
+def _gen2(sizes_name, sizes_args, gen_name, gen_args):
+    """Order-2 generators: 8 nodes per element and the mid-side flags."""
+    L = lib()
+    nn, ne, nb = C.c_int(), C.c_int(), C.c_int()
+    blen = np.zeros(8, dtype=np.int32)
+    check(getattr(L, sizes_name)(*sizes_args, C.byref(nn), C.byref(ne), C.byref(nb), ip(blen)))
+    x, y = np.zeros(nn.value), np.zeros(nn.value)
+    conn = np.zeros(8 * ne.value, dtype=np.int32)
+    sec = np.zeros(nn.value, dtype=np.uint8)
+    bord = np.zeros(int(blen[:nb.value].sum()), dtype=np.int32)
+    check(getattr(L, gen_name)(*gen_args, dp(x), dp(y), ip(conn), u8p(sec), ip(bord), ip(blen)))
+    borders, off = [], 0
+    for b in range(nb.value):
+        borders.append(bord[off:off + blen[b]].copy())
+        off += blen[b]
+    return Mesh(x, y, conn, borders, npe=8, sec=sec)
+
+
 _f = C.c_double
 
 
-def gen_rect(x1, x2, y1, y2, N1, N2):
-    """Mesh::genRectangle (Mesh.cu:17-109)."""
-    return _gen("cp_mesh_rect_sizes", (N1, N2), "cp_mesh_rect", (_f(x1), _f(x2), _f(y1), _f(y2), N1, N2))
+def gen_rect(x1, x2, y1, y2, N1, N2, order=1):
+    """Mesh::genRectangle (Mesh.cu:17-109); order 2: 8-node quads (remapOrder)."""
+    a = (_f(x1), _f(x2), _f(y1), _f(y2), N1, N2)
+    if order == 2:
+        return _gen2("cp_mesh_rect2_sizes", (N1, N2), "cp_mesh_rect2", a)
+    return _gen("cp_mesh_rect_sizes", (N1, N2), "cp_mesh_rect", a)
 
 
-def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2):
-    """Mesh::genRectWithHole (Mesh.cu:112-245)."""
+def gen_rect_with_hole(x1, x2, y1, y2, r, N1, N2, order=1):
+    """Mesh::genRectWithHole (Mesh.cu:112-245); order 2: 8-node quads."""
     a = (_f(x1), _f(x2), _f(y1), _f(y2), _f(r), N1, N2)
+    if order == 2:
+        return _gen2("cp_mesh_hole2_sizes", a, "cp_mesh_hole2", a)
     return _gen("cp_mesh_hole_sizes", a, "cp_mesh_hole", a)
 
 
-def gen_ring(a, b, Nphi, Nr):
-    """Mesh::genRing (Mesh.cu:249-329)."""
+def gen_ring(a, b, Nphi, Nr, order=1):
+    """Mesh::genRing (Mesh.cu:249-329); order 2: 8-node quads."""
+    if order == 2:
+        return _gen2("cp_mesh_ring2_sizes", (Nphi, Nr), "cp_mesh_ring2", (_f(a), _f(b), Nphi, Nr))
     return _gen("cp_mesh_ring_sizes", (Nphi, Nr), "cp_mesh_ring", (_f(a), _f(b), Nphi, Nr))
 
 
@@ -183,10 +213,10 @@ def gen_quarter_ring(a, b, Nt, Nr, ratio):
 
 def csr_pattern(mesh):
     nnz = C.c_longlong()
-    check(lib().cp_csr_nnz(mesh.nn, mesh.ne, ip(mesh.conn), C.byref(nnz)))
+    check(lib().cp_csr_nnz_n(mesh.nn, mesh.ne, mesh.npe, ip(mesh.conn), C.byref(nnz)))
     rows = np.zeros(2 * mesh.nn + 1, dtype=np.int32)
     cols = np.zeros(nnz.value, dtype=np.int32)
-    check(lib().cp_csr_pattern(mesh.nn, mesh.ne, ip(mesh.conn), ip(rows), ip(cols)))
+    check(lib().cp_csr_pattern_n(mesh.nn, mesh.ne, mesh.npe, ip(mesh.conn), ip(rows), ip(cols)))
     return rows, cols
 
 
@@ -208,11 +238,14 @@ class Loads:
     def force_values(self, border_id, fv, norm_orient=False):
         b = self.mesh.borders[border_id]
         fv = np.ascontiguousarray(fv, dtype=np.float64).reshape(-1)
-        check(lib().cp_rhs_border_force(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, dp(fv),
-                                        int(norm_orient), _f(self.h), dp(self.rp)))
+        sec = None if self.mesh.sec is None else u8p(self.mesh.sec)
+        check(lib().cp_rhs_border_force_q(dp(self.mesh.x), dp(self.mesh.y), ip(b), b.size, sec, dp(fv),
+                                          int(norm_orient), _f(self.h), dp(self.rp)))
         return self
 
     def gravity(self, g=9.81, rho=1.0):
+        if self.mesh.npe != 4:
+            raise ValueError("body forces are provided for 4-node meshes")
         check(lib().cp_rhs_body_force(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
                                       _f(0.0), _f(-g), _f(rho), _f(self.h), dp(self.rp)))
         return self
@@ -257,8 +290,9 @@ class Problem:
     def __init__(self, ctx, mesh, loads, mat):
         self.mesh, self.mat = mesh, mat
         self.ptr = C.c_void_p()
-        check(lib().cp_problem_create(ctx.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
-                                      dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), C.byref(self.ptr)))
+        check(lib().cp_problem_create_n(ctx.ptr, mesh.npe, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y),
+                                        ip(mesh.conn), dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat),
+                                        C.byref(self.ptr)))
 
     def close(self):
         if self.ptr:

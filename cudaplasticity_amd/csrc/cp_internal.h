@@ -12,13 +12,14 @@ int failf(int code, const char* fmt, ...);
 
 // Node adjacency in the reference CSR order (self first, then first
 // appearance while scanning elements) plus the element-ordered incidence
-// list node -> (element << 2 | local index).
+// list node -> element * npe + local index (npe = 4 or 8 nodes per element).
 struct Adjacency {
     std::vector<int> ptr, node;
     std::vector<int> incPtr, inc;
+    int npe = 4;
 };
 
-int build_adjacency(int nn, int ne, const int* conn, Adjacency& adj);
+int build_adjacency(int nn, int ne, int npe, const int* conn, Adjacency& adj);
 void export_csr(const Adjacency& adj, int nn, int* rows, int* cols);
 
 }  // namespace cp

@@ -180,6 +180,48 @@ __device__ __forceinline__ double pick4(const double (&v)[4], int i)
     return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
+// 8-node (serendipity) quad, 3 x 3 Gauss rule with the reference's
+// constants (sqrt(0.6), 5/9 and 8/9 truncated to 15 decimals: calcB8
+// PlasticitySolver.cu:70, formKe8 :190, strain8 :1555).  Gauss point
+// g = 3i + j sits at (r, s) = (p_i, p_j).
+__device__ __forceinline__ double q8_point(int i) { return i == 0 ? -0.774596669241483 : (i == 1 ? 0. : 0.774596669241483); }
+__device__ __forceinline__ double q8_weight(int i) { return i == 1 ? 0.888888888888888 : 0.555555555555555; }
+
+// calcB8 (PlasticitySolver.cu:66-134) at (r, s): B0 = dN/dx, B1 = dN/dy of
+// the nodes in local order (corners (1,1), (-1,1), (-1,-1), (1,-1), then
+// the mid-sides of edges 1-2, 2-3, 3-4, 4-1), and detJ.
+__device__ __forceinline__ void q8_grad(const double (&x)[8], const double (&y)[8], double r, double s,
+                                        double (&B0)[8], double (&B1)[8], double& detJ)
+{
+    double dr[8], ds[8];
+    // corners: N = (1 + r ra)(1 + s sa)(r ra + s sa - 1) / 4
+    dr[0] = 0.25 * (1. + s) * (2. * r + s);       ds[0] = 0.25 * (1. + r) * (r + 2. * s);
+    dr[1] = -0.25 * (1. + s) * (-2. * r + s);     ds[1] = 0.25 * (1. - r) * (-r + 2. * s);
+    dr[2] = -0.25 * (1. - s) * (-2. * r - s);     ds[2] = -0.25 * (1. - r) * (-r - 2. * s);
+    dr[3] = 0.25 * (1. - s) * (2. * r - s);       ds[3] = -0.25 * (1. + r) * (r - 2. * s);
+    // mid-sides
+    dr[4] = -r * (1. + s);                        ds[4] = 0.5 * (1. - r * r);
+    dr[5] = -0.5 * (1. - s * s);                  ds[5] = -s * (1. - r);
+    dr[6] = -r * (1. - s);                        ds[6] = -0.5 * (1. - r * r);
+    dr[7] = 0.5 * (1. - s * s);                   ds[7] = -s * (1. + r);
+    double J0 = 0., J1 = 0., J2 = 0., J3 = 0.;    // x_r, y_r, x_s, y_s
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        J0 += dr[a] * x[a];
+        J1 += dr[a] * y[a];
+        J2 += ds[a] * x[a];
+        J3 += ds[a] * y[a];
+    }
+    detJ = J0 * J3 - J1 * J2;
+    const double inv = 1. / detJ;
+    const double i0 = J3 * inv, i1 = -J1 * inv, i2 = -J2 * inv, i3 = J0 * inv;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        B0[a] = i0 * dr[a] + i1 * ds[a];
+        B1[a] = i2 * dr[a] + i3 * ds[a];
+    }
+}
+
 // ---------------------------------------------------------------- assembly
 
 struct AsmArgs {
@@ -837,25 +879,24 @@ struct RmArgs {
     double tol;
 };
 
-// strain4 (PlasticitySolver.cu:1517-1549) + plastStress (:1590-1610) +
-// exitNorm2 (:1664-1691) for one element per thread, always in FP64,
-// followed by the decision of the safeguarded Irons-Tuck step.
-static __global__ __launch_bounds__(kBlock) void k_return_map(RmArgs a)
+// Mean element strain: strain4 (PlasticitySolver.cu:1517-1549), the mean of
+// B u over the 4 Gauss points, or strain8 (:1551-1588), (1/4) sum_g w_g B_g u
+// over the 9 points.  conn holds NPE / 4 int4 per element.
+template <int NPE>
+__device__ __forceinline__ void element_strain(const int4* conn, const double2* xy, const double2* u, int e,
+                                               double& exx, double& eyy, double& gxy)
 {
-    const int lb = xcd_block(a.nblocks);
-    const int e = lb * kBlock + threadIdx.x;
-    double v[5] = {0., 0., 0., 0., 0.};
-    if (e < a.ne) {
-        const int4 c = a.conn[e];
+    exx = eyy = gxy = 0.;
+    if constexpr (NPE == 4) {
+        const int4 c = conn[e];
         const int cc[4] = {c.x, c.y, c.z, c.w};
         double x[4], y[4], ux[4], uy[4];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const double2 p = a.xy[cc[j]];
-            const double2 w = a.u[cc[j]];
+            const double2 p = xy[cc[j]];
+            const double2 w = u[cc[j]];
             x[j] = p.x; y[j] = p.y; ux[j] = w.x; uy[j] = w.y;
         }
-        double exx = 0., eyy = 0., gxy = 0.;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
             double B0[4], B1[4], dJ;
@@ -865,9 +906,50 @@ static __global__ __launch_bounds__(kBlock) void k_return_map(RmArgs a)
             gxy += ux[0] * B1[0] + ux[1] * B1[1] + ux[2] * B1[2] + ux[3] * B1[3] + uy[0] * B0[0] + uy[1] * B0[1] +
                    uy[2] * B0[2] + uy[3] * B0[3];
         }
-        exx *= 0.25;
-        eyy *= 0.25;
-        gxy *= 0.25;
+    } else {
+        const int4 c0 = conn[2 * e], c1 = conn[2 * e + 1];
+        const int cc[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+        double x[8], y[8], ux[8], uy[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double2 p = xy[cc[j]];
+            const double2 w = u[cc[j]];
+            x[j] = p.x; y[j] = p.y; ux[j] = w.x; uy[j] = w.y;
+        }
+        for (int gi = 0; gi < 3; ++gi)
+            for (int gj = 0; gj < 3; ++gj) {
+                double B0[8], B1[8], dJ;
+                q8_grad(x, y, q8_point(gi), q8_point(gj), B0, B1, dJ);
+                double el = 0., yl = 0., gl = 0.;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    el += ux[k] * B0[k];
+                    yl += uy[k] * B1[k];
+                    gl += ux[k] * B1[k] + uy[k] * B0[k];
+                }
+                const double w = q8_weight(gi) * q8_weight(gj);
+                exx += el * w;
+                eyy += yl * w;
+                gxy += gl * w;
+            }
+    }
+    exx *= 0.25;
+    eyy *= 0.25;
+    gxy *= 0.25;
+}
+
+// strain4 / strain8 + plastStress (:1590-1610) + exitNorm2 (:1664-1691) for
+// one element per thread, always in FP64, followed by the decision of the
+// safeguarded Irons-Tuck step.
+template <int NPE>
+__global__ __launch_bounds__(kBlock) void k_return_map(RmArgs a)
+{
+    const int lb = xcd_block(a.nblocks);
+    const int e = lb * kBlock + threadIdx.x;
+    double v[5] = {0., 0., 0., 0., 0.};
+    if (e < a.ne) {
+        double exx, eyy, gxy;
+        element_strain<NPE>(a.conn, a.xy, a.u, e, exx, eyy, gxy);
         const double2 en = a.ecnu[e];
         double c00, c01, c22;
         elastic_c(a.M.plane, en.x, en.y, c00, c01, c22);

@@ -63,6 +63,66 @@ int radial_segments(double x1, double x2, double y1, double y2, double r, int N1
     return (int)std::round(double(radN) * (x2 - x1 + y2 - y1 - 4. * r) * 2. / (x2 - x1 + y2 - y1));
 }
 
+// Mesh::genRectWithHole (Mesh.cu:112-245), order 1, radial segment count
+// given: outer rectangle nodes counter-clockwise from (x1, y1), hole nodes
+// on a circle, straight rays between outer node j and hole node j, then
+// renumerateRing.  Order 2 doubles N1, N2 and radN (:117-121).
+int hole_mesh(double x1, double x2, double y1, double y2, double r, int N1, int N2, int radN, double* x,
+              double* y, int* conn, int* bord, int* blen)
+{
+    const int borderN = 2 * (N1 + N2);
+    const int nn = borderN * (radN + 1), ne = borderN * radN, nb = 5;
+    blen[0] = blen[2] = N1 + 1;
+    blen[1] = blen[3] = N2 + 1;
+    blen[4] = borderN + 1;
+    BorderWriter bw{bord, blen};
+    int *bBottom = bw.at(0), *bRight = bw.at(1), *bTop = bw.at(2), *bLeft = bw.at(3), *bHole = bw.at(4);
+
+    const double hx = (x2 - x1) / N1, hy = (y2 - y1) / N2;
+    int id = 0;
+    for (int i = 0; i < N1; ++i, ++id) { x[id] = x1 + i * hx; y[id] = y1; bBottom[i] = id; }
+    bBottom[N1] = id;
+    for (int j = 0; j < N2; ++j, ++id) { x[id] = x2; y[id] = y1 + j * hy; bRight[j] = id; }
+    bRight[N2] = id;
+    for (int i = 0; i < N1; ++i, ++id) { x[id] = x2 - i * hx; y[id] = y2; bTop[i] = id; }
+    bTop[N1] = id;
+    for (int j = 0; j < N2; ++j, ++id) { x[id] = x1; y[id] = y2 - j * hy; bLeft[j] = id; }
+    bLeft[N2] = 0;
+
+    const double dphi = 2. * kPi / borderN;
+    const double cx = 0.5 * (x1 + x2), cy = 0.5 * (y1 + y2);
+    const double phi0 = dphi * (N1 + N2 * 1.5);
+    const int holeBase = radN * borderN;
+    for (int k = 0; k < borderN; ++k) {
+        const double a = phi0 + double(k) * dphi;
+        double sa, ca;
+        ::sincos(a, &sa, &ca);
+        x[holeBase + k] = cx + ca * r;
+        y[holeBase + k] = cy + sa * r;
+        bHole[borderN - 1 - k] = holeBase + k;
+    }
+    bHole[borderN] = nn - 1;
+    for (int j = 0; j < borderN; ++j) {
+        const double sx = (x[holeBase + j] - x[j]) / radN, sy = (y[holeBase + j] - y[j]) / radN;
+        for (int i = 1; i < radN; ++i) {
+            x[i * borderN + j] = x[j] + sx * double(i);
+            y[i * borderN + j] = y[j] + sy * double(i);
+        }
+    }
+    for (int j = 0; j < borderN; ++j) {
+        const int jn = (j + 1 == borderN) ? 0 : j + 1;
+        for (int i = 0; i < radN; ++i) {
+            int* c = conn + 4 * (j * radN + i);
+            c[0] = (i + 1) * borderN + jn;
+            c[1] = (i + 1) * borderN + j;
+            c[2] = i * borderN + j;
+            c[3] = i * borderN + jn;
+        }
+    }
+    renumber_ring(nn, ne, x, y, conn, nb, bord, blen, borderN);
+    return CP_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -132,62 +192,14 @@ int cp_mesh_hole_sizes(double x1, double x2, double y1, double y2, double r, int
     return CP_OK;
 }
 
-// Mesh::genRectWithHole (Mesh.cu:112-245), order 1: outer rectangle nodes
-// counter-clockwise from (x1, y1), hole nodes on a circle, straight rays
-// between outer node j and hole node j, then renumerateRing.
+// Mesh::genRectWithHole (Mesh.cu:112-245), order 1.
 int cp_mesh_hole(double x1, double x2, double y1, double y2, double r, int N1, int N2, double* x, double* y,
                  int* conn, int* bord, int* blen)
 {
     int nn, ne, nb;
     if (int s = cp_mesh_hole_sizes(x1, x2, y1, y2, r, N1, N2, &nn, &ne, &nb, blen)) return s;
-    const int borderN = 2 * (N1 + N2);
-    const int radN = radial_segments(x1, x2, y1, y2, r, N1, N2);
-    BorderWriter bw{bord, blen};
-    int *bBottom = bw.at(0), *bRight = bw.at(1), *bTop = bw.at(2), *bLeft = bw.at(3), *bHole = bw.at(4);
-
-    const double hx = (x2 - x1) / N1, hy = (y2 - y1) / N2;
-    int id = 0;
-    for (int i = 0; i < N1; ++i, ++id) { x[id] = x1 + i * hx; y[id] = y1; bBottom[i] = id; }
-    bBottom[N1] = id;
-    for (int j = 0; j < N2; ++j, ++id) { x[id] = x2; y[id] = y1 + j * hy; bRight[j] = id; }
-    bRight[N2] = id;
-    for (int i = 0; i < N1; ++i, ++id) { x[id] = x2 - i * hx; y[id] = y2; bTop[i] = id; }
-    bTop[N1] = id;
-    for (int j = 0; j < N2; ++j, ++id) { x[id] = x1; y[id] = y2 - j * hy; bLeft[j] = id; }
-    bLeft[N2] = 0;
-
-    const double dphi = 2. * kPi / borderN;
-    const double cx = 0.5 * (x1 + x2), cy = 0.5 * (y1 + y2);
-    const double phi0 = dphi * (N1 + N2 * 1.5);
-    const int holeBase = radN * borderN;
-    for (int k = 0; k < borderN; ++k) {
-        const double a = phi0 + double(k) * dphi;
-        double sa, ca;
-        ::sincos(a, &sa, &ca);
-        x[holeBase + k] = cx + ca * r;
-        y[holeBase + k] = cy + sa * r;
-        bHole[borderN - 1 - k] = holeBase + k;
-    }
-    bHole[borderN] = nn - 1;
-    for (int j = 0; j < borderN; ++j) {
-        const double sx = (x[holeBase + j] - x[j]) / radN, sy = (y[holeBase + j] - y[j]) / radN;
-        for (int i = 1; i < radN; ++i) {
-            x[i * borderN + j] = x[j] + sx * double(i);
-            y[i * borderN + j] = y[j] + sy * double(i);
-        }
-    }
-    for (int j = 0; j < borderN; ++j) {
-        const int jn = (j + 1 == borderN) ? 0 : j + 1;
-        for (int i = 0; i < radN; ++i) {
-            int* c = conn + 4 * (j * radN + i);
-            c[0] = (i + 1) * borderN + jn;
-            c[1] = (i + 1) * borderN + j;
-            c[2] = i * borderN + j;
-            c[3] = i * borderN + jn;
-        }
-    }
-    renumber_ring(nn, ne, x, y, conn, nb, bord, blen, borderN);
-    return CP_OK;
+    return hole_mesh(x1, x2, y1, y2, r, N1, N2, radial_segments(x1, x2, y1, y2, r, N1, N2), x, y, conn, bord,
+                     blen);
 }
 
 int cp_mesh_ring_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen)
@@ -380,27 +392,64 @@ int cp_mesh_max_index_diff(int ne, const int* conn, int* out)
     return CP_OK;
 }
 
-// Border traction (PlasticitySolver.cu:1070-1103): each segment [n1, n2]
+// Border traction (PlasticitySolver.cu:1022-1103): each segment [n1, n2]
 // adds f(n) * h * |n2 - n1| / 2 to both end nodes; with norm_orient the
-// value is (normal, tangential) with normal = (t.y, -t.x).
-int cp_rhs_border_force(const double* x, const double* y, const int* border, int len, const double* fval,
-                        int norm_orient, double h, double* rp)
+// value is (normal, tangential) with normal = (t.y, -t.x).  Where sec marks
+// border[k] as a mid-side node (order-2 meshes) the segment is [n1, n2, n3]
+// with the quadratic split f1 L/6, f2 2L/3, f3 L/6 (:1025-1069), the mid
+// node's frame being the mean of the two half segments' frames.
+int cp_rhs_border_force_q(const double* x, const double* y, const int* border, int len, const unsigned char* sec,
+                          const double* fval, int norm_orient, double h, double* rp)
 {
+    auto frame = [&](int a, int b, double& tx, double& ty, double& nx, double& ny) {
+        tx = x[b] - x[a];
+        ty = y[b] - y[a];
+        const double it = 1. / std::sqrt(tx * tx + ty * ty);
+        tx *= it;
+        ty *= it;
+        nx = ty;
+        ny = -tx;
+        const double in = 1. / std::sqrt(nx * nx + ny * ny);
+        nx *= in;
+        ny *= in;
+    };
     for (int k = 1; k < len; ++k) {
+        if (sec && sec[border[k]] && k + 1 < len) {
+            ++k;
+            const int a = border[k - 2], m = border[k - 1], b = border[k];
+            const double d1x = 0.5 * (x[a] - x[m]), d1y = 0.5 * (y[a] - y[m]);
+            const double d2x = 0.5 * (x[m] - x[b]), d2y = 0.5 * (y[m] - y[b]);
+            const double dl1 = std::sqrt(d1x * d1x + d1y * d1y), dl2 = std::sqrt(d2x * d2x + d2y * d2y);
+            double fa[2] = {fval[2 * (k - 2)], fval[2 * (k - 2) + 1]};
+            double fm[2] = {fval[2 * (k - 1)], fval[2 * (k - 1) + 1]};
+            double fb[2] = {fval[2 * k], fval[2 * k + 1]};
+            if (norm_orient) {
+                double t1x, t1y, n1x, n1y, t2x, t2y, n2x, n2y;
+                frame(a, m, t1x, t1y, n1x, n1y);
+                frame(m, b, t2x, t2y, n2x, n2y);
+                const double ax = n1x * fa[0] + t1x * fa[1], ay = n1y * fa[0] + t1y * fa[1];
+                const double mx = 0.5 * ((n1x + n2x) * fm[0] + (t1x + t2x) * fm[1]);
+                const double my = 0.5 * ((n1y + n2y) * fm[0] + (t1y + t2y) * fm[1]);
+                const double bx = n2x * fb[0] + t2x * fb[1], by = n2y * fb[0] + t2y * fb[1];
+                fa[0] = ax; fa[1] = ay; fm[0] = mx; fm[1] = my; fb[0] = bx; fb[1] = by;
+            }
+            const double wa = h * dl1 * 2., wm = h * (dl1 + dl2), wb = h * dl2 * 2.;
+            rp[2 * a] += fa[0] / 3. * wa;
+            rp[2 * a + 1] += fa[1] / 3. * wa;
+            rp[2 * m] += fm[0] * 4. / 3. * wm;
+            rp[2 * m + 1] += fm[1] * 4. / 3. * wm;
+            rp[2 * b] += fb[0] / 3. * wb;
+            rp[2 * b + 1] += fb[1] / 3. * wb;
+            continue;
+        }
         const int a = border[k - 1], b = border[k];
         const double hdx = 0.5 * (x[a] - x[b]), hdy = 0.5 * (y[a] - y[b]);
         const double dl = std::sqrt(hdx * hdx + hdy * hdy);
         double fa[2] = {fval[2 * (k - 1)], fval[2 * (k - 1) + 1]};
         double fb[2] = {fval[2 * k], fval[2 * k + 1]};
         if (norm_orient) {
-            double tx = x[b] - x[a], ty = y[b] - y[a];
-            const double it = 1. / std::sqrt(tx * tx + ty * ty);
-            tx *= it;
-            ty *= it;
-            double nx = ty, ny = -tx;
-            const double in = 1. / std::sqrt(nx * nx + ny * ny);
-            nx *= in;
-            ny *= in;
+            double tx, ty, nx, ny;
+            frame(a, b, tx, ty, nx, ny);
             const double ax = nx * fa[0] + tx * fa[1], ay = ny * fa[0] + ty * fa[1];
             const double bx = nx * fb[0] + tx * fb[1], by = ny * fb[0] + ty * fb[1];
             fa[0] = ax; fa[1] = ay; fb[0] = bx; fb[1] = by;
@@ -412,6 +461,12 @@ int cp_rhs_border_force(const double* x, const double* y, const int* border, int
         rp[2 * b + 1] += fb[1] * w;
     }
     return CP_OK;
+}
+
+int cp_rhs_border_force(const double* x, const double* y, const int* border, int len, const double* fval,
+                        int norm_orient, double h, double* rp)
+{
+    return cp_rhs_border_force_q(x, y, border, len, nullptr, fval, norm_orient, h, rp);
 }
 
 int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn, double gx, double gy,
@@ -511,20 +566,165 @@ int cp_displacement_border_values(const int* border, int len, const double* uval
     return CP_OK;
 }
 
-int cp_csr_nnz(int nn, int ne, const int* conn, long long* nnz)
+int cp_csr_nnz_n(int nn, int ne, int npe, const int* conn, long long* nnz)
 {
     cp::Adjacency adj;
-    if (int s = cp::build_adjacency(nn, ne, conn, adj)) return s;
+    if (int s = cp::build_adjacency(nn, ne, npe, conn, adj)) return s;
     *nnz = 4LL * (long long)adj.ptr[nn];
     return CP_OK;
 }
 
-int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+int cp_csr_pattern_n(int nn, int ne, int npe, const int* conn, int* rows, int* cols)
 {
     cp::Adjacency adj;
-    if (int s = cp::build_adjacency(nn, ne, conn, adj)) return s;
+    if (int s = cp::build_adjacency(nn, ne, npe, conn, adj)) return s;
     cp::export_csr(adj, nn, rows, cols);
     return CP_OK;
+}
+
+int cp_csr_nnz(int nn, int ne, const int* conn, long long* nnz) { return cp_csr_nnz_n(nn, ne, 4, conn, nnz); }
+
+int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols)
+{
+    return cp_csr_pattern_n(nn, ne, 4, conn, rows, cols);
+}
+
+// Mesh::remapOrder (Mesh.cu:443-503).  In the strip numbering k = i*width + j
+// of the order-1 quads, the 2x2 patch with i, j even becomes 8-node quad
+// (i/2)(width/2) + j/2: corners = local nodes 0, 1, 2, 3 of quads (i+1, j+1),
+// (i, j+1), (i, j), (i+1, j); mid-sides = local nodes 1, 2, 3, 0 of the same
+// quads.  The patch centre (local node 0 of quad (i, j)) is dropped and the
+// remaining nodes are renumbered in their order; bord (nbord entries) is
+// renumbered in place and sec marks the mid-side nodes.
+int cp_mesh_remap_order(int nn, int ne4, const double* x, const double* y, const int* conn4, int width, int nbord,
+                        int* bord, double* x8, double* y8, int* conn8, unsigned char* sec, int* nn8)
+{
+    if (nn < 1 || width < 2 || (width & 1) || ne4 % width || ((ne4 / width) & 1) || !x || !y || !conn4 || !x8 ||
+        !y8 || !conn8 || !sec || !nn8 || (nbord > 0 && !bord))
+        return cp::fail(CP_ERR_INVALID, "cp_mesh_remap_order: the order-1 mesh must be 2m x 2n strips of quads");
+    const int strips = ne4 / width;
+    std::vector<unsigned char> keep(nn, 1);
+    for (int i = 0; i < strips; i += 2)
+        for (int j = 0; j < width; j += 2) {
+            const int* qa = conn4 + 4 * ((size_t)(i + 1) * width + j + 1);
+            const int* qb = conn4 + 4 * ((size_t)i * width + j + 1);
+            const int* qc = conn4 + 4 * ((size_t)i * width + j);
+            const int* qd = conn4 + 4 * ((size_t)(i + 1) * width + j);
+            int* c = conn8 + 8 * ((size_t)(i / 2) * (width / 2) + j / 2);
+            const int v[8] = {qa[0], qb[1], qc[2], qd[3], qa[1], qb[2], qc[3], qd[0]};
+            for (int t = 0; t < 8; ++t) {
+                if (v[t] < 0 || v[t] >= nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_remap_order: bad node index");
+                c[t] = v[t];
+            }
+            keep[qc[0]] = 0;
+        }
+    std::vector<int> id(nn, -1);
+    int m = 0;
+    for (int n = 0; n < nn; ++n)
+        if (keep[n]) {
+            id[n] = m;
+            x8[m] = x[n];
+            y8[m] = y[n];
+            ++m;
+        }
+    const size_t ne8 = (size_t)ne4 / 4;
+    for (size_t k = 0; k < 8 * ne8; ++k) conn8[k] = id[conn8[k]];
+    for (int k = 0; k < nbord; ++k) {
+        if (bord[k] < 0 || bord[k] >= nn || id[bord[k]] < 0)
+            return cp::fail(CP_ERR_INVALID, "cp_mesh_remap_order: border through a removed node");
+        bord[k] = id[bord[k]];
+    }
+    std::memset(sec, 0, (size_t)m);
+    for (size_t e = 0; e < ne8; ++e)
+        for (int t = 4; t < 8; ++t) sec[conn8[8 * e + t]] = 1;
+    *nn8 = m;
+    return CP_OK;
+}
+
+// Order-2 generators (genRectangle / genRectWithHole / genRing with order = 2):
+// the order-1 mesh of doubled resolution regrouped by remapOrder.
+int cp_mesh_rect2_sizes(int N1, int N2, int* nn, int* ne, int* nb, int* blen)
+{
+    int n1, e1;
+    if (int s = cp_mesh_rect_sizes(2 * N1, 2 * N2, &n1, &e1, nb, blen)) return s;
+    *ne = e1 / 4;
+    *nn = n1 - *ne;
+    return CP_OK;
+}
+
+static int remap_into(int nn1, int ne1, const std::vector<double>& x1, const std::vector<double>& y1,
+                      const std::vector<int>& c1, int width, std::vector<int>& b1, int* bord, double* x, double* y,
+                      int* conn, unsigned char* sec)
+{
+    int nn8 = 0;
+    if (int s = cp_mesh_remap_order(nn1, ne1, x1.data(), y1.data(), c1.data(), width, (int)b1.size(), b1.data(), x, y,
+                                    conn, sec, &nn8))
+        return s;
+    std::copy(b1.begin(), b1.end(), bord);
+    return CP_OK;
+}
+
+int cp_mesh_rect2(double x1, double x2, double y1, double y2, int N1, int N2, double* x, double* y, int* conn,
+                  unsigned char* sec, int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_rect_sizes(2 * N1, 2 * N2, &nn, &ne, &nb, blen)) return s;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(2 * (blen[0] + blen[1]));
+    if (int s = cp_mesh_rect(x1, x2, y1, y2, 2 * N1, 2 * N2, X.data(), Y.data(), C.data(), B.data(), blen)) return s;
+    return remap_into(nn, ne, X, Y, C, 2 * N2, B, bord, x, y, conn, sec);
+}
+
+int cp_mesh_hole2_sizes(double x1, double x2, double y1, double y2, double r, int N1, int N2, int* nn, int* ne,
+                        int* nb, int* blen)
+{
+    if (N1 < 1 || N2 < 1 || !(r > 0)) return cp::fail(CP_ERR_INVALID, "cp_mesh_hole2_sizes: bad arguments");
+    const int radN = 2 * radial_segments(x1, x2, y1, y2, r, N1, N2), borderN = 4 * (N1 + N2);
+    if (radN < 2) return cp::fail(CP_ERR_INVALID, "cp_mesh_hole2_sizes: hole too large for the plate");
+    const int n1 = borderN * (radN + 1), e1 = borderN * radN;
+    *ne = e1 / 4;
+    *nn = n1 - *ne;
+    *nb = 5;
+    if (blen) {
+        blen[0] = blen[2] = 2 * N1 + 1;
+        blen[1] = blen[3] = 2 * N2 + 1;
+        blen[4] = borderN + 1;
+    }
+    return CP_OK;
+}
+
+int cp_mesh_hole2(double x1, double x2, double y1, double y2, double r, int N1, int N2, double* x, double* y,
+                  int* conn, unsigned char* sec, int* bord, int* blen)
+{
+    int nn8, ne8, nb;
+    if (int s = cp_mesh_hole2_sizes(x1, x2, y1, y2, r, N1, N2, &nn8, &ne8, &nb, blen)) return s;
+    const int radN = 2 * radial_segments(x1, x2, y1, y2, r, N1, N2), borderN = 4 * (N1 + N2);
+    const int nn = borderN * (radN + 1), ne = borderN * radN;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(2 * (blen[0] + blen[1]) + blen[4]);
+    if (int s = hole_mesh(x1, x2, y1, y2, r, 2 * N1, 2 * N2, radN, X.data(), Y.data(), C.data(), B.data(), blen))
+        return s;
+    return remap_into(nn, ne, X, Y, C, radN, B, bord, x, y, conn, sec);
+}
+
+int cp_mesh_ring2_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen)
+{
+    int n1, e1;
+    if (int s = cp_mesh_ring_sizes(2 * Nphi, 2 * Nr, &n1, &e1, nb, blen)) return s;
+    *ne = e1 / 4;
+    *nn = n1 - *ne;
+    return CP_OK;
+}
+
+int cp_mesh_ring2(double a, double b, int Nphi, int Nr, double* x, double* y, int* conn, unsigned char* sec,
+                  int* bord, int* blen)
+{
+    int nn, ne, nb;
+    if (int s = cp_mesh_ring_sizes(2 * Nphi, 2 * Nr, &nn, &ne, &nb, blen)) return s;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(blen[0] + blen[1]);
+    if (int s = cp_mesh_ring(a, b, 2 * Nphi, 2 * Nr, X.data(), Y.data(), C.data(), B.data(), blen)) return s;
+    return remap_into(nn, ne, X, Y, C, 2 * Nr, B, bord, x, y, conn, sec);
 }
 
 }  // extern "C"
@@ -536,36 +736,41 @@ namespace cp {
 // are met scanning elements in storage order (local order inside an
 // element), duplicates dropped.  Built per node from an element-ordered
 // incidence list, so the per-node work is independent (OpenMP).
-int build_adjacency(int nn, int ne, const int* conn, Adjacency& adj)
+int build_adjacency(int nn, int ne, int npe, const int* conn, Adjacency& adj)
 {
-    for (long long k = 0; k < 4LL * ne; ++k)
+    if (npe != 4 && npe != 8) return fail(CP_ERR_INVALID, "elements must have 4 or 8 nodes");
+    if ((long long)ne * npe >= (1LL << 31)) return fail(CP_ERR_INVALID, "too many elements");
+    const long long nc = (long long)npe * ne;
+    for (long long k = 0; k < nc; ++k)
         if (conn[k] < 0 || conn[k] >= nn) return fail(CP_ERR_INVALID, "connectivity index out of range");
     for (int e = 0; e < ne; ++e) {
-        const int* c = conn + 4 * e;
-        if (c[0] == c[1] || c[0] == c[2] || c[0] == c[3] || c[1] == c[2] || c[1] == c[3] || c[2] == c[3])
-            return fail(CP_ERR_INVALID, "degenerate element (repeated node)");
+        const int* c = conn + (size_t)npe * e;
+        for (int i = 0; i < npe; ++i)
+            for (int j = i + 1; j < npe; ++j)
+                if (c[i] == c[j]) return fail(CP_ERR_INVALID, "degenerate element (repeated node)");
     }
     std::vector<int> incPtr(nn + 1, 0);
-    for (long long k = 0; k < 4LL * ne; ++k) ++incPtr[conn[k] + 1];
+    for (long long k = 0; k < nc; ++k) ++incPtr[conn[k] + 1];
     for (int n = 0; n < nn; ++n) incPtr[n + 1] += incPtr[n];
-    std::vector<int> inc(incPtr[nn]);  // (element << 2) | local index, element order per node
+    std::vector<int> inc(incPtr[nn]);  // element * npe + local index, element order per node
     {
         std::vector<int> fill(incPtr.begin(), incPtr.end() - 1);
         for (int e = 0; e < ne; ++e)
-            for (int i = 0; i < 4; ++i) inc[fill[conn[4 * e + i]]++] = (e << 2) | i;
+            for (int i = 0; i < npe; ++i) inc[fill[conn[(size_t)npe * e + i]]++] = npe * e + i;
     }
     std::vector<int> cnt(nn + 1, 0);
-    std::vector<int> scratch(incPtr[nn] * 3 + nn + 1);
+    const int others = npe - 1;
+    std::vector<int> scratch((size_t)incPtr[nn] * others + nn + 1);
 #pragma omp parallel for schedule(static)
     for (int n = 0; n < nn; ++n) {
-        int* list = scratch.data() + 3 * incPtr[n] + n;
+        int* list = scratch.data() + (size_t)others * incPtr[n] + n;
         int u = 0;
         list[u++] = n;
         for (int q = incPtr[n]; q < incPtr[n + 1]; ++q) {
-            const int e = inc[q] >> 2, i = inc[q] & 3;
-            for (int j = 0; j < 4; ++j) {
+            const int e = inc[q] / npe, i = inc[q] % npe;
+            for (int j = 0; j < npe; ++j) {
                 if (j == i) continue;
-                const int v = conn[4 * e + j];
+                const int v = conn[(size_t)npe * e + j];
                 bool seen = false;
                 for (int t = 1; t < u; ++t)
                     if (list[t] == v) { seen = true; break; }
@@ -579,8 +784,9 @@ int build_adjacency(int nn, int ne, const int* conn, Adjacency& adj)
     adj.node.resize(adj.ptr[nn]);
 #pragma omp parallel for schedule(static)
     for (int n = 0; n < nn; ++n)
-        std::memcpy(adj.node.data() + adj.ptr[n], scratch.data() + 3 * incPtr[n] + n,
+        std::memcpy(adj.node.data() + adj.ptr[n], scratch.data() + (size_t)others * incPtr[n] + n,
                     sizeof(int) * (adj.ptr[n + 1] - adj.ptr[n]));
+    adj.npe = npe;
     adj.incPtr.swap(incPtr);
     adj.inc.swap(inc);
     return CP_OK;

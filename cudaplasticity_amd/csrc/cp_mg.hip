@@ -722,6 +722,7 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
 {
     using namespace cp;
     if (!P || n1 < 2 || n2 < 2) return fail(CP_ERR_INVALID, "cp_problem_set_grid: bad arguments");
+    if (P->npe != 4) return fail(CP_ERR_INVALID, "cp_problem_set_grid: multigrid needs 4-node elements");
     if ((long long)n1 * n2 != P->nn)
         return failf(CP_ERR_INVALID, "cp_problem_set_grid: %d x %d lattice for %d nodes", n1, n2, P->nn);
     const int nn = P->nn;

@@ -42,7 +42,7 @@ struct Layout {
     size_t common, fp64, mixed;
 };
 
-Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax);
+Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax, int npe = 4);
 
 struct MgState;   // multigrid hierarchy (cp_mg.hip)
 
@@ -53,6 +53,7 @@ struct cp_problem {
     hipStream_t stream = nullptr;
     bool own_stream = true;
     int nn = 0, ne = 0, nslices = 0, max_w = 0;
+    int npe = 4;           // nodes per element: 4 (bilinear) or 8 (8-node quadratic)
     int nbn = 0, nbe = 0;  // blocks over nodes / elements
     long long nblk = 0, nblk_stored = 0;
     cpk::MatParams M{};
@@ -68,7 +69,12 @@ struct cp_problem {
     bool solved = false;      // u / psi hold a converged level at `load`
     double lam_prev = 0.;     // load factor of u_prev / psi_prev
 
-    cp::Arena common, a64, amx;
+    cp::Arena common, a64, amx, aq8;
+    // 8-node elements: element blocks and their gather lists (cp_q8.h)
+    int* q8_inc = nullptr;      // node -> 8 e + i, element order
+    int *q8_bptr = nullptr, *q8_bsrc = nullptr, *q8_bdst = nullptr;
+    double4* q8_keb = nullptr;
+    double2* q8_fe = nullptr;
     // common
     double2 *xy = nullptr, *ecnu = nullptr, *rp0d = nullptr, *u0d = nullptr, *b = nullptr, *u = nullptr,
             *u_prev = nullptr;
@@ -104,7 +110,7 @@ namespace cp {
 int problem_alloc_solver(cp_problem* P, int precision);
 int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
                    const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
-                   hipStream_t stream, cp_problem** out);
+                   hipStream_t stream, cp_problem** out, int npe = 4);
 int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st);
 // mg_full (multigrid only): 0 lets the coarse levels be reused (see mg_setup).
 int launch_assemble(cp_problem* P, int precision, int mg_full = 1);
@@ -113,6 +119,9 @@ int set_load(cp_problem* P, double factor, int reset);
 int advance_load(cp_problem* P, double factor, int predictor);
 int reset_state(cp_problem* P);
 cpk::ResArgs residual_args(cp_problem* P);
+// r = mask (b - f(u)) (b NULL: f(u)) in FP64 with the element kernels of the
+// problem's element type; partials for ||r||^2 when ra.sc is set.
+int launch_residual(cp_problem* P, const cpk::ResArgs& ra, hipStream_t st);
 int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int maxit);
 int enqueue_residual(cp_problem* P);
 int enqueue_correction(cp_problem* P);

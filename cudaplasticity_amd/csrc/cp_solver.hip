@@ -16,6 +16,7 @@
 #include <string>
 
 #include "cp_problem.h"
+#include "cp_q8.h"
 
 namespace {
 thread_local std::string g_err;
@@ -47,13 +48,13 @@ int failf(int code, const char* fmt, ...)
     return code;
 }
 
-Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax)
+Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax, int npe)
 {
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     Layout L{};
     L.common = al(sizeof(double2) * nn) * 6 +      // xy, rp0, u0, b, u, u_prev
                al(sizeof(uchar2) * nn) +
-               al(sizeof(int4) * ne) + al(sizeof(double2) * ne) +
+               al(sizeof(int4) * (size_t)ne * (npe / 4)) + al(sizeof(double2) * ne) +
                al(sizeof(double) * ne) * (5 + CP_NFIELDS) +   // psi, psi_prev, G, rcur, rprev, fields
                al(sizeof(int) * (nn + 1)) + al(sizeof(int2) * ninc) +
                al(sizeof(int) * nblk_stored) + al(sizeof(int2) * nslices) +
@@ -133,6 +134,10 @@ int problem_alloc_solver(cp_problem* P, int precision)
         P->a64.size = L.fp64;
         P->va = P->a64.take<double2>(P->nblk_stored);
         P->vb = P->a64.take<double2>(P->nblk_stored);
+        if (P->npe == 8) {   // the gather writes stored blocks only: padding stays zero
+            CP_HIP(hipMemsetAsync(P->va, 0, sizeof(double2) * P->nblk_stored, P->stream));
+            CP_HIP(hipMemsetAsync(P->vb, 0, sizeof(double2) * P->nblk_stored, P->stream));
+        }
         P->p64 = P->a64.take<double2>(P->nn);
         for (int i = 0; i < 2; ++i) {
             P->r64[i] = P->a64.take<double2>(P->nn);
@@ -153,6 +158,7 @@ int problem_alloc_solver(cp_problem* P, int precision)
         P->amx.base = (char*)mem;
         P->amx.size = L.mixed;
         P->vf = P->amx.take<float4>(P->nblk_stored);
+        if (P->npe == 8) CP_HIP(hipMemsetAsync(P->vf, 0, sizeof(float4) * P->nblk_stored, P->stream));
         P->x32 = P->amx.take<float2>(P->nn);
         P->p32 = P->amx.take<float2>(P->nn);
         for (int i = 0; i < 2; ++i) {
@@ -180,21 +186,24 @@ static int max_width_instantiated(int w)
 
 int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
                    const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
-                   hipStream_t stream, cp_problem** out)
+                   hipStream_t stream, cp_problem** out, int npe)
 {
     *out = nullptr;
     if (!ctx || nn < 1 || ne < 1 || !x || !y || !conn || !rp || !mask || !mat)
         return fail(CP_ERR_INVALID, "cp_problem_create: null argument or empty mesh");
+    if (npe != 4 && npe != 8) return fail(CP_ERR_INVALID, "cp_problem_create: elements must have 4 or 8 nodes");
     if (!(mat->E > 0) || !(mat->sigmaT > 0) || mat->law < 0 || mat->law > 2 || mat->plane < 0 || mat->plane > 1)
         return fail(CP_ERR_INVALID, "cp_problem_create: invalid material");
-    if (nn > (1 << 30) / 2 || ne > (1 << 29)) return fail(CP_ERR_INVALID, "cp_problem_create: mesh too large");
+    if (nn > (1 << 30) / 2 || ne > (npe == 8 ? (1 << 24) : (1 << 29)))
+        return fail(CP_ERR_INVALID, "cp_problem_create: mesh too large");
     Adjacency adj;
-    if (int s = build_adjacency(nn, ne, conn, adj)) return s;
+    if (int s = build_adjacency(nn, ne, npe, conn, adj)) return s;
 
     std::unique_ptr<cp_problem> P(new cp_problem());
     P->ctx = ctx;
     P->nn = nn;
     P->ne = ne;
+    P->npe = npe;
     P->M = cpk::MatParams{mat->E, mat->nu, mat->h, mat->sigmaT, mat->law, mat->plane, mat->K_T, mat->A, mat->m, mat->n};
     P->nslices = (nn + cpk::kWave - 1) / cpk::kWave;
     P->nbn = blocks_for(nn);
@@ -231,24 +240,59 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
         }
     // node -> (element, slots of K_e(i, j) in the node's row)
     const long long ninc = adj.incPtr[nn];
-    std::vector<int2> hinc(ninc);
-    for (int n = 0; n < nn; ++n) {
-        const int* row = adj.node.data() + adj.ptr[n];
-        const int deg = adj.ptr[n + 1] - adj.ptr[n];
-        for (int q = adj.incPtr[n]; q < adj.incPtr[n + 1]; ++q) {
-            const int e = adj.inc[q] >> 2;
-            unsigned packed = 0;
-            for (int j = 0; j < 4; ++j) {
-                const int v = conn[4 * e + j];
-                int k = 0;
-                while (k < deg && row[k] != v) ++k;
-                packed |= (unsigned)k << (8 * j);
+    std::vector<int2> hinc(npe == 4 ? ninc : 0);
+    // 8-node elements: per stored block (compact order adj.ptr[n] + k) the
+    // element blocks 64 e + 8 i + j that land there, in element order
+    std::vector<int> bptr, bsrc, bdst;
+    if (npe == 4) {
+        for (int n = 0; n < nn; ++n) {
+            const int* row = adj.node.data() + adj.ptr[n];
+            const int deg = adj.ptr[n + 1] - adj.ptr[n];
+            for (int q = adj.incPtr[n]; q < adj.incPtr[n + 1]; ++q) {
+                const int e = adj.inc[q] >> 2;
+                unsigned packed = 0;
+                for (int j = 0; j < 4; ++j) {
+                    const int v = conn[4 * e + j];
+                    int k = 0;
+                    while (k < deg && row[k] != v) ++k;
+                    packed |= (unsigned)k << (8 * j);
+                }
+                hinc[q] = make_int2(e, (int)packed);
             }
-            hinc[q] = make_int2(e, (int)packed);
         }
+    } else {
+        const long long nblk = adj.ptr[nn];
+        bptr.assign(nblk + 1, 0);
+        bsrc.resize(64 * (size_t)ne);
+        bdst.resize(nblk);
+        std::vector<int> slot(8 * (size_t)ninc);
+        for (int n = 0; n < nn; ++n) {
+            const int* row = adj.node.data() + adj.ptr[n];
+            const int deg = adj.ptr[n + 1] - adj.ptr[n];
+            for (int k = 0; k < deg; ++k)
+                bdst[adj.ptr[n] + k] = slices[n / cpk::kWave].x + k * cpk::kWave + (n % cpk::kWave);
+            for (int q = adj.incPtr[n]; q < adj.incPtr[n + 1]; ++q) {
+                const int e = adj.inc[q] / 8;
+                for (int j = 0; j < 8; ++j) {
+                    const int v = conn[8 * (size_t)e + j];
+                    int k = 0;
+                    while (k < deg && row[k] != v) ++k;
+                    slot[8 * (size_t)q + j] = k;
+                    ++bptr[adj.ptr[n] + k + 1];
+                }
+            }
+        }
+        for (long long b = 0; b < nblk; ++b) bptr[b + 1] += bptr[b];
+        std::vector<int> fill(bptr.begin(), bptr.end() - 1);
+        for (int n = 0; n < nn; ++n)
+            for (int q = adj.incPtr[n]; q < adj.incPtr[n + 1]; ++q) {
+                const int e = adj.inc[q] / 8, i = adj.inc[q] % 8;
+                for (int j = 0; j < 8; ++j) bsrc[fill[adj.ptr[n] + slot[8 * (size_t)q + j]]++] = 64 * e + 8 * i + j;
+            }
     }
 
-    Layout L = problem_layout(nn, ne, P->nblk_stored, ninc, P->nslices, std::max(P->nbn, P->nbe));
+    Layout L = problem_layout(nn, ne, P->nblk_stored, npe == 4 ? ninc : 0, P->nslices, std::max(P->nbn, P->nbe),
+                              npe);
     void* mem = nullptr;
     if (hipMalloc(&mem, L.common) != hipSuccess) {
         (void)hipGetLastError();
@@ -264,7 +308,7 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->u = A.take<double2>(nn);
     P->u_prev = A.take<double2>(nn);
     P->mask = A.take<uchar2>(nn);
-    P->conn = A.take<int4>(ne);
+    P->conn = A.take<int4>((size_t)ne * (npe / 4));
     P->ecnu = A.take<double2>(ne);
     P->psi = A.take<double>(ne);
     P->psi_prev = A.take<double>(ne);
@@ -273,7 +317,7 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->rprev = A.take<double>(ne);
     P->F = A.take<double>((size_t)ne * CP_NFIELDS);
     P->inc_ptr = A.take<int>(nn + 1);
-    P->inc = A.take<int2>(ninc);
+    P->inc = A.take<int2>(npe == 4 ? ninc : 0);
     P->col = A.take<int>(P->nblk_stored);
     P->slice = A.take<int2>(P->nslices);
     P->partials = A.take<double>(5 * (size_t)std::max(P->nbn, P->nbe));
@@ -296,6 +340,27 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->h_rm = (cpk::RmScal*)((char*)pinned + 256);
     P->h_ref = (cpk::RefScal*)((char*)pinned + 512);
 
+    if (npe == 8) {
+        auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
+        const size_t bytes = al(sizeof(int) * ninc) + al(sizeof(int) * bptr.size()) + al(sizeof(int) * bsrc.size()) +
+                             al(sizeof(int) * bdst.size()) + al(sizeof(double4) * 64 * (size_t)ne) +
+                             al(sizeof(double2) * 8 * (size_t)ne);
+        void* m8 = nullptr;
+        if (hipMalloc(&m8, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(P->common.base);
+            return failf(CP_ERR_NOMEM, "hipMalloc of %zu bytes (8-node element blocks) failed", bytes);
+        }
+        P->aq8.base = (char*)m8;
+        P->aq8.size = bytes;
+        P->q8_inc = P->aq8.take<int>(ninc);
+        P->q8_bptr = P->aq8.take<int>(bptr.size());
+        P->q8_bsrc = P->aq8.take<int>(bsrc.size());
+        P->q8_bdst = P->aq8.take<int>(bdst.size());
+        P->q8_keb = P->aq8.take<double4>(64 * (size_t)ne);
+        P->q8_fe = P->aq8.take<double2>(8 * (size_t)ne);
+    }
+
     std::vector<double2> hxy(nn), hrp(nn), hu0(nn);
     std::vector<uchar2> hmask(nn);
     for (int n = 0; n < nn; ++n) {
@@ -309,9 +374,16 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     CP_HIP(hipMemcpyAsync(P->rp0d, hrp.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemcpyAsync(P->u0d, hu0.data(), sizeof(double2) * nn, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemcpyAsync(P->mask, hmask.data(), sizeof(uchar2) * nn, hipMemcpyHostToDevice, st));
-    CP_HIP(hipMemcpyAsync(P->conn, conn, sizeof(int4) * ne, hipMemcpyHostToDevice, st));
+    CP_HIP(hipMemcpyAsync(P->conn, conn, sizeof(int) * npe * (size_t)ne, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemcpyAsync(P->inc_ptr, adj.incPtr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice, st));
-    CP_HIP(hipMemcpyAsync(P->inc, hinc.data(), sizeof(int2) * ninc, hipMemcpyHostToDevice, st));
+    if (npe == 4) {
+        CP_HIP(hipMemcpyAsync(P->inc, hinc.data(), sizeof(int2) * ninc, hipMemcpyHostToDevice, st));
+    } else {
+        CP_HIP(hipMemcpyAsync(P->q8_inc, adj.inc.data(), sizeof(int) * ninc, hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(P->q8_bptr, bptr.data(), sizeof(int) * bptr.size(), hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(P->q8_bsrc, bsrc.data(), sizeof(int) * bsrc.size(), hipMemcpyHostToDevice, st));
+        CP_HIP(hipMemcpyAsync(P->q8_bdst, bdst.data(), sizeof(int) * bdst.size(), hipMemcpyHostToDevice, st));
+    }
     CP_HIP(hipMemcpyAsync(P->col, hcol.data(), sizeof(int) * off, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemcpyAsync(P->slice, slices.data(), sizeof(int2) * P->nslices, hipMemcpyHostToDevice, st));
     CP_HIP(hipMemsetAsync(P->F, 0, sizeof(double) * (size_t)ne * CP_NFIELDS, st));
@@ -338,9 +410,82 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
 
 // ---------------------------------------------------------------- kernels drivers
 
+static int launch_assemble_q8(cp_problem* P, bool f64)
+{
+    cpk::Q8Args a{};
+    a.conn = P->conn;
+    a.xy = P->xy;
+    a.ecnu = P->ecnu;
+    a.keb = P->q8_keb;
+    a.bptr = P->q8_bptr;
+    a.bsrc = P->q8_bsrc;
+    a.bdst = P->q8_bdst;
+    a.va = P->va;
+    a.vb = P->vb;
+    a.vf = P->vf;
+    a.ne = P->ne;
+    a.nblk = (int)P->nblk;
+    a.nbe8 = blocks_for(8LL * P->ne);
+    a.nbb = blocks_for(P->nblk);
+    a.plane = P->M.plane;
+    a.h = P->M.h;
+    hipLaunchKernelGGL(cpk::k_q8_ke, dim3(a.nbe8), dim3(cpk::kBlock), 0, P->stream, a);
+    if (f64) hipLaunchKernelGGL(cpk::k_q8_gather<true>, dim3(a.nbb), dim3(cpk::kBlock), 0, P->stream, a);
+    else hipLaunchKernelGGL(cpk::k_q8_gather<false>, dim3(a.nbb), dim3(cpk::kBlock), 0, P->stream, a);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
+int launch_residual(cp_problem* P, const cpk::ResArgs& ra, hipStream_t st)
+{
+    if (P->npe == 4) {
+        hipLaunchKernelGGL(cpk::k_residual64, dim3(P->nbn), dim3(cpk::kBlock), 0, st, ra);
+        CP_HIP(hipGetLastError());
+        return CP_OK;
+    }
+    cpk::Q8ResArgs q{};
+    q.conn = P->conn;
+    q.xy = P->xy;
+    q.ecnu = P->ecnu;
+    q.u = ra.u;
+    q.fe = P->q8_fe;
+    q.incPtr = P->inc_ptr;
+    q.inc = P->q8_inc;
+    q.b = ra.b;
+    q.mask = ra.mask;
+    q.r64 = ra.r64;
+    q.r32 = ra.r32;
+    q.sc = ra.sc;
+    q.partials = ra.partials;
+    q.ne = P->ne;
+    q.nn = P->nn;
+    q.nbe = P->nbe;
+    q.nbn = P->nbn;
+    q.plane = P->M.plane;
+    q.h = P->M.h;
+    hipLaunchKernelGGL(cpk::k_q8_force, dim3(P->nbe), dim3(cpk::kBlock), 0, st, q);
+    hipLaunchKernelGGL(cpk::k_q8_fgather, dim3(P->nbn), dim3(cpk::kBlock), 0, st, q);
+    CP_HIP(hipGetLastError());
+    return CP_OK;
+}
+
 int launch_assemble(cp_problem* P, int precision, int mg_full)
 {
     if (int s = problem_alloc_solver(P, precision)) return s;
+    if (P->npe == 8) {
+        if (int s = launch_assemble_q8(P, precision == CP_FP64)) return s;
+        const dim3 g(P->nbn), b(cpk::kBlock);
+        if (P->precond == 1) {
+            if (precision == CP_FP64)
+                hipLaunchKernelGGL(cpk::k_block_jacobi<double>, g, b, 0, P->stream, P->cg64_args(), P->minv64);
+            else hipLaunchKernelGGL(cpk::k_block_jacobi<float>, g, b, 0, P->stream, P->cg32_args(), P->minv32);
+            CP_HIP(hipGetLastError());
+        } else if (P->precond == 2) {
+            return fail(CP_ERR_INVALID, "multigrid preconditioning needs a 4-node lattice mesh");
+        }
+        if (precision == CP_FP64) P->have_fp64_values = true;
+        return CP_OK;
+    }
     cpk::AsmArgs a{};
     a.incPtr = P->inc_ptr;
     a.inc = P->inc;
@@ -598,7 +743,7 @@ int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int
 
 int enqueue_residual(cp_problem* P)
 {
-    hipLaunchKernelGGL(cpk::k_residual64, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, residual_args(P));
+    if (int s = launch_residual(P, residual_args(P), P->stream)) return s;
     hipLaunchKernelGGL(cpk::k_residual_total, dim3(1), dim3(cpk::kBlock), 0, P->stream, P->refs,
                        (const double*)P->partials, P->nbn);
     CP_HIP(hipGetLastError());
@@ -712,7 +857,8 @@ int launch_return_map(cp_problem* P, int k, int aitken, int fixed_outer, int max
     a.solved_tight = solved_tight;
     a.tol = tol;
     const dim3 g(P->nbe), b(cpk::kBlock);
-    hipLaunchKernelGGL(cpk::k_return_map, g, b, 0, P->stream, a);
+    if (P->npe == 8) hipLaunchKernelGGL(cpk::k_return_map<8>, g, b, 0, P->stream, a);
+    else hipLaunchKernelGGL(cpk::k_return_map<4>, g, b, 0, P->stream, a);
     hipLaunchKernelGGL(cpk::k_rm_decide, dim3(1), b, 0, P->stream, a);
     hipLaunchKernelGGL(cpk::k_apply_update, g, b, 0, P->stream, P->psi, P->ecnu, (const double*)P->G,
                        (const double*)P->rcur, P->rprev, (const cpk::RmScal*)P->rms, P->M.E, P->M.nu, P->ne, P->nbe,
@@ -921,6 +1067,15 @@ int cp_problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double
     return cp::problem_create(ctx, nn, ne, x, y, conn, rp, u0, mask, mat, nullptr, out);
 }
 
+int cp_problem_create_n(cp_ctx* ctx, int npe, int nn, int ne, const double* x, const double* y, const int* conn,
+                        const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                        cp_problem** out)
+{
+    if (!out) return cp::fail(CP_ERR_INVALID, "null out");
+    if (ctx) CP_HIP(hipSetDevice(ctx->device));
+    return cp::problem_create(ctx, nn, ne, x, y, conn, rp, u0, mask, mat, nullptr, out, npe);
+}
+
 int cp_problem_destroy(cp_problem* P)
 {
     if (!P) return CP_OK;
@@ -932,6 +1087,7 @@ int cp_problem_destroy(cp_problem* P)
     if (P->common.base) (void)hipFree(P->common.base);
     if (P->a64.base) (void)hipFree(P->a64.base);
     if (P->amx.base) (void)hipFree(P->amx.base);
+    if (P->aq8.base) (void)hipFree(P->aq8.base);
     if (P->own_stream && P->stream) (void)hipStreamDestroy(P->stream);
     delete P;
     return CP_OK;
@@ -1072,8 +1228,7 @@ int cp_internal_force(cp_problem* P, double* f_host)
     ra.nblocks = P->nbn;
     ra.plane = P->M.plane;
     ra.h = P->M.h;
-    hipLaunchKernelGGL(cpk::k_residual64, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, ra);
-    CP_HIP(hipGetLastError());
+    if (int s = cp::launch_residual(P, ra, P->stream)) return s;
     CP_HIP(hipMemcpyAsync(f_host, P->res64, sizeof(double2) * P->nn, hipMemcpyDeviceToHost, P->stream));
     CP_HIP(hipStreamSynchronize(P->stream));
     return CP_OK;
@@ -1249,7 +1404,7 @@ int cp_bench_cg(cp_problem* P, int precision, int precond, int reps, double* ms,
         ra.nblocks = P->nbn;
         ra.plane = P->M.plane;
         ra.h = P->M.h;
-        hipLaunchKernelGGL(cpk::k_residual64, g, b, 0, st, ra);
+        if (int s = cp::launch_residual(P, ra, st)) return s;
         hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, P->sc32, 1.0, 0.0, 2000000000);
         hipLaunchKernelGGL(cpk::k_cg_init1<float>, g, b, 0, st, a, 1);
         cp::launch_init2<float>(P->max_w, P->precond, a, g, st);

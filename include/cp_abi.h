@@ -154,6 +154,27 @@ int cp_mesh_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
                          double* x, double* y, int* conn, int* bord, int* blen);
 /* Mesh::findMaxIndexDiff (Mesh.cu:417-441) */
 int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
+/* Mesh::remapOrder (Mesh.cu:443-503): the order-1 mesh of doubled resolution
+ * (ne4 quads in strips of `width`, both counts even) regrouped into ne4 / 4
+ * 8-node quads; the patch centres are dropped (nn8 = nn - ne4 / 4 nodes),
+ * bord (nbord entries) is renumbered in place, sec[n] = 1 on mid-side nodes. */
+int cp_mesh_remap_order(int nn, int ne4, const double* x, const double* y, const int* conn4, int width,
+                        int nbord, int* bord, double* x8, double* y8, int* conn8, unsigned char* sec,
+                        int* nn8);
+/* genRectangle / genRectWithHole / genRing with order = 2 (Mesh.cu:17-23,
+ * 112-121, 249-253 + remapOrder): 8 nodes per element in conn (corners, then
+ * the mid-sides of edges 1-2, 2-3, 3-4, 4-1), sec marks the mid-side nodes;
+ * border lists run through the mid-side nodes. */
+int cp_mesh_rect2_sizes(int N1, int N2, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_rect2(double x1, double x2, double y1, double y2, int N1, int N2,
+                  double* x, double* y, int* conn, unsigned char* sec, int* bord, int* blen);
+int cp_mesh_hole2_sizes(double x1, double x2, double y1, double y2, double r, int N1, int N2,
+                        int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_hole2(double x1, double x2, double y1, double y2, double r, int N1, int N2,
+                  double* x, double* y, int* conn, unsigned char* sec, int* bord, int* blen);
+int cp_mesh_ring2_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen);
+int cp_mesh_ring2(double a, double b, int Nphi, int Nr,
+                  double* x, double* y, int* conn, unsigned char* sec, int* bord, int* blen);
 
 /* ------------------------------------------------------------------ */
 /* host-side loads / boundary conditions -- PlasticitySolver.cu:1007-1216 */
@@ -162,6 +183,12 @@ int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
  * as LoadConditions::setForce's function would return it (:1070-1103). */
 int cp_rhs_border_force(const double* x, const double* y, const int* border, int len,
                         const double* fval, int norm_orient, double h, double* rp);
+/* The same on order-2 borders: where sec[border[k]] is set, border[k] is a
+ * mid-side node and the segment [border[k-1], border[k], border[k+1]] takes
+ * the quadratic loads f L/6, 2 f L/3, f L/6 (:1025-1069).  sec may be NULL. */
+int cp_rhs_border_force_q(const double* x, const double* y, const int* border, int len,
+                          const unsigned char* sec, const double* fval, int norm_orient, double h,
+                          double* rp);
 /* Constant body force on 4-node elements (:1106-1117). */
 int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
                       double gx, double gy, double rho, double h, double* rp);
@@ -185,6 +212,9 @@ int cp_displacement_border_values(const int* border, int len, const double* uval
 /* ------------------------------------------------------------------ */
 int cp_csr_nnz(int nn, int ne, const int* conn, long long* nnz);
 int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols);
+/* npe = 4 or 8 nodes per element (the elem8 scan of :339-391). */
+int cp_csr_nnz_n(int nn, int ne, int npe, const int* conn, long long* nnz);
+int cp_csr_pattern_n(int nn, int ne, int npe, const int* conn, int* rows, int* cols);
 
 /* ------------------------------------------------------------------ */
 /* device problems -- replaces the PlasticitySolver constructor         */
@@ -197,6 +227,13 @@ int cp_csr_pattern(int nn, int ne, const int* conn, int* rows, int* cols);
 int cp_problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y, const int* conn,
                       const double* rp, const double* u0, const unsigned char* mask,
                       const cp_material* mat, cp_problem** out);
+/* The same for npe = 4 (bilinear quads, elem4) or npe = 8 (8-node quads,
+ * elem8: formKe8 / insertKe8 / strain8, PlasticitySolver.cu:66-224,
+ * 1452-1490, 1551-1588).  8-node problems take block-Jacobi or plain CG
+ * (precond 0 / 1); multigrid and the serving loop take 4-node meshes. */
+int cp_problem_create_n(cp_ctx* ctx, int npe, int nn, int ne, const double* x, const double* y,
+                        const int* conn, const double* rp, const double* u0, const unsigned char* mask,
+                        const cp_material* mat, cp_problem** out);
 int cp_problem_destroy(cp_problem* prob);
 /* Device bytes held by a problem of this size (serving admission). */
 int cp_problem_bytes(int nn, int ne, long long nnz, int precision, size_t* bytes);

@@ -1,0 +1,131 @@
+"""GPU parity of the 8-node (order-2) element path through the C-ABI.
+
+Inputs: the reference's energyConvTest hole plate, genRectWithHole(-3, 3,
+-2, 2, 0.5, 3k, 2k, order 2), 304L material (Main.cpp:8-10, 105-178).
+Bars: FP64 assembly within 1e-13 of the oracle's CSR values; FP64 solutions
+within relative L2 1e-9 (elastic) and 1e-6 (plastic, secant exit 1e-8) of
+the oracle; the mixed path (FP32 CG with block-Jacobi + FP64 refinement)
+within 1e-6 of the FP64 path; and deformEnergy of the GPU solution on the
+reference's printed 6 digits (data/aitken__elast2.txt, aitken__plast2.txt).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cudaplasticity_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+E304, ST304, NU304 = 193e9, 170e6, 0.29
+
+
+def relnorm(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = A.Context(0)
+    yield c
+    c.close()
+
+
+def case(k, law, S):
+    pm = A.gen_rect_with_hole(-3., 3., -2., 2., 0.5, 3 * k, 2 * k, order=2)
+    om = O.gen_rect_with_hole(-3., 3., -2., 2., 0.5, 3 * k, 2 * k, order=2)
+    pl = A.Loads(pm).force(1, S, 0.).force(3, -S, 0.).fix_vert_axis(0.).fix_hor_axis(0.)
+    ol = O.Problem(om).force(1, S, 0.).force(3, -S, 0.).fix_vert_axis(0.).fix_hor_axis(0.)
+    assert np.array_equal(pl.rp, ol.rp) and np.array_equal(pl.mask, ol.mask)
+    if law == "elastic":
+        kw = dict(E=E304, nu=NU304)
+    else:
+        kw = dict(E=E304, nu=NU304, sigmaT=ST304, law=A.LAW_POWER, A=0.01 * E304, m=0.015)
+    print("8-node hole plate k=%d: %d nodes, %d elements, %s" % (k, pm.nn, pm.ne, law), flush=True)
+    return pm, pl, A.material(**kw), om, ol, O.material(**kw)
+
+
+def energy(om, omat, P):
+    F = P.fields()
+    st = np.stack([F[name] for name in O.FIELDS])
+    return O.deform_energy(om, omat, st)
+
+
+def within_printed(value, printed):
+    unit = 10.0 ** (math.floor(math.log10(abs(printed))) - 5)
+    return abs(value - printed) <= 0.5 * unit
+
+
+def test_q8_assembly_matches_oracle(ctx):
+    pm, pl, pmat, om, ol, omat = case(4, "power", 170e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    rows, cols = P.csr()
+    ro, co = O.csr_pattern(om)
+    assert np.array_equal(rows, ro) and np.array_equal(cols, co)
+    P.assemble(A.FP64)
+    got = P.values()
+    want = O.assemble(om, ro, co, np.full(om.ne, E304), np.full(om.ne, NU304))
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print("assembly max rel err %.2e" % err, flush=True)
+    assert err < 1e-13
+    P.close()
+
+
+def test_q8_elastic_matches_oracle_and_energy_table(ctx):
+    pm, pl, pmat, om, ol, omat = case(6, "elastic", 1e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    P.solve(A.options(cg_tol=1e-12), elastic=True)
+    u_o, st_o, _ = O.solve(ol, omat, O.options(cg_tol=1e-12, elastic=True))
+    e_u = relnorm(P.field("u"), u_o)
+    e_s = relnorm(P.field("sigI"), st_o[O.FIELDS.index("sigI")])
+    en = energy(om, omat, P)
+    print("elastic u %.2e sigma_i %.2e energy %.7f (table 68.6683)" % (e_u, e_s, en), flush=True)
+    assert e_u < 1e-9 and e_s < 1e-9
+    assert within_printed(en, 68.6683)
+    # equilibrium of the internal forces on the free DOFs
+    f = P.internal_force()
+    free = pl.mask.astype(bool)
+    assert np.linalg.norm((f - pl.rp)[free]) <= 1e-9 * np.linalg.norm(pl.rp[free])
+    P.close()
+
+
+@pytest.mark.parametrize("precision,precond", [(A.FP64, 0), (A.FP64, 1), (A.MIXED, 1)])
+def test_q8_plastic_matches_oracle(ctx, precision, precond):
+    pm, pl, pmat, om, ol, omat = case(4, "power", 170e6)
+    opt = dict(cg_tol=1e-10, tol=1e-8, aitken=True)
+    P = A.Problem(ctx, pm, pl, pmat)
+    st = P.solve(A.options(precision=precision, precond=precond, inner_tol=1e-5, **opt))
+    u_o, st_o, so = O.solve(ol, omat, O.options(max_outer=1000, **opt))
+    assert st.status == A.CP_OK and so.status == 0
+    e_u = relnorm(P.field("u"), u_o)
+    e_s = relnorm(P.field("sigI"), st_o[O.FIELDS.index("sigI")])
+    print("precision %d precond %d: outer %d (oracle %d), cg %d; u %.2e sigma_i %.2e" %
+          (precision, precond, st.outer_iters, so.outer_iters, st.cg_iters, e_u, e_s), flush=True)
+    assert e_u < 1e-6 and e_s < 1e-6
+    P.close()
+
+
+def test_q8_energy_table_at_reference_tolerances(ctx):
+    """The reference's loop: FP64, unpreconditioned CG to 1e-7, plain secant
+    iterations to relErr 1e-5; aitken__plast2.txt prints 2.45491e+06 for the
+    660-element mesh."""
+    pm, pl, pmat, om, ol, omat = case(6, "power", 170e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    st = P.solve(A.options())
+    en = energy(om, omat, P)
+    print("outer %d cg %d energy %.1f (table 2.45491e+06)" % (st.outer_iters, st.cg_iters, en), flush=True)
+    assert st.status == A.CP_OK
+    assert within_printed(en, 2.45491e6)
+    P.close()
+
+
+def test_q8_multigrid_is_refused(ctx):
+    pm, pl, pmat, om, ol, omat = case(2, "elastic", 1e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    n1 = next(d for d in range(2, pm.nn) if pm.nn % d == 0)
+    with pytest.raises(A.CPError) as e:
+        P.set_grid(n1, pm.nn // n1)
+    assert e.value.code == A.CP_ERR_INVALID and "4-node" in str(e.value)
+    with pytest.raises(A.CPError):
+        P.solve(A.options(precond=2))
+    P.close()

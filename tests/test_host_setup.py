@@ -126,3 +126,59 @@ def test_nodal_load_entry_points_match_constant_ones():
             uv = np.tile([1e-3, -2e-3], b.size)
             A.check(lib.cp_displacement_border_values(A.ip(b), b.size, A.dp(uv), mx, my, A.u8p(k2), A.dp(u2)))
             assert np.array_equal(k1, k2) and np.array_equal(u1, u2)
+
+
+MESHES2 = [
+    ("rect", (-2, 2, -1, 1, 12, 6)),
+    ("rect", (0, 3, -2, 1, 5, 3)),
+    ("hole", (-3, 3, -2, 2, 0.5, 12, 8)),
+    ("hole", (-3, 3, -2, 2, 0.5, 18, 12)),
+    ("ring", (1, 2, 40, 6)),
+    ("ring", (0.1, 0.2, 9, 2)),
+]
+
+
+@pytest.mark.parametrize("kind,args", MESHES2)
+def test_order2_generators_match_oracle(kind, args):
+    o = {"rect": O.gen_rect, "hole": O.gen_rect_with_hole, "ring": O.gen_ring}[kind](*args, order=2)
+    p = {"rect": A.gen_rect, "hole": A.gen_rect_with_hole, "ring": A.gen_ring}[kind](*args, order=2)
+    assert p.npe == 8 and np.array_equal(o.conn, p.conn)
+    assert np.array_equal(o.x, p.x) and np.array_equal(o.y, p.y)
+    assert np.array_equal(o.sec, p.sec)
+    assert len(o.borders) == len(p.borders)
+    for bo, bp in zip(o.borders, p.borders):
+        assert np.array_equal(bo, bp)
+    # reference CSR pattern over elem8
+    ro, co = O.csr_pattern(o)
+    rp, cp = A.csr_pattern(p)
+    assert np.array_equal(ro, rp) and np.array_equal(co, cp)
+
+
+def test_order2_generator_matches_reference_mesh_file():
+    """hole1.vtk: genRectWithHole(-3, 3, -2, 2, 0.5, 12, 8, 2) saved by Mesh::saveAsVtk."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mesh_hole1.npz"))
+    p = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2)
+    assert np.array_equal(p.conn[:, [0, 4, 1, 5, 2, 6, 3, 7]], g["polys"])
+
+
+def test_order2_loads_match_oracle():
+    o = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2)
+    p = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2)
+    po = O.Problem(o, h=0.7).force(1, 170.0, 3.0).force(3, -170.0, 0.0).force(4, 5.0, 1.0, norm_orient=True)
+    pa = A.Loads(p, h=0.7).force(1, 170.0, 3.0).force(3, -170.0, 0.0).force(4, 5.0, 1.0, norm_orient=True)
+    po.fix_vert_axis(0.0).fix_hor_axis(0.0)
+    pa.fix_vert_axis(0.0).fix_hor_axis(0.0)
+    assert np.array_equal(po.rp, pa.rp)
+    assert np.array_equal(po.mask, pa.mask)
+
+
+def test_remap_rejects_odd_strips():
+    m = A.gen_rect(0, 1, 0, 1, 3, 2)
+    x8, y8 = np.zeros(m.nn), np.zeros(m.nn)
+    c8 = np.zeros(2 * m.conn.size, dtype=np.int32)
+    sec = np.zeros(m.nn, dtype=np.uint8)
+    nn8 = A.C.c_int()
+    code = A.lib().cp_mesh_remap_order(m.nn, m.ne, A.dp(m.x), A.dp(m.y), A.ip(m.conn), 2, 0, None, A.dp(x8),
+                                       A.dp(y8), A.ip(c8), A.u8p(sec), A.C.byref(nn8))
+    assert code == A.CP_ERR_INVALID
