@@ -291,12 +291,83 @@ def other_workload(A, args):
     print(json.dumps(line), flush=True)
 
 
+def q8_workload(A, args):
+    """8-node quads: the reference's energyConvTest plate (Main.cpp:105-178),
+    genRectWithHole(-3, 3, -2, 2, 0.5, 3k, 2k, order 2), 304L power law,
+    traction 170 MPa on x = +-3 in 20 increments, Aitken to 1e-6;
+    block-Jacobi CG (the multigrid preconditioner takes 4-node lattices),
+    mixed and FP64; the CG iteration kernel on the 8-node matrix and the
+    element-block assembly.  One JSON line."""
+    k, nsteps, tol, S = args.q8_k, NSTEPS, 1e-6, 170.0
+    E = 193e3
+    mesh = A.gen_rect_with_hole(-3., 3., -2., 2., 0.5, 3 * k, 2 * k, order=2)
+    loads = A.Loads(mesh).force(1, S, 0.0).force(3, -S, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    mat = A.material(E, 0.29, 170.0, A.LAW_POWER, A=0.01 * E, m=0.015)
+    name = (f"plate with hole 6x4, r 0.5, 8-node quads ({mesh.ne}, genRectWithHole {3 * k}x{2 * k} order 2), "
+            f"304L power law, {nsteps} increments to {S:g} MPa, Aitken to {tol:g}")
+    log(name)
+    ctx = A.Context(0)
+    out = {}
+    for tag, prec in (("mixed", A.MIXED), ("fp64", A.FP64)):
+        P = A.Problem(ctx, mesh, loads, mat)
+        opt = A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=prec, precond=1,
+                        predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
+        t0 = time.perf_counter()
+        per = []
+        for n in range(1, nsteps + 1):
+            o = A.Options.from_buffer_copy(opt)
+            o.warm = 1 if n > 1 else 0
+            if n == 1:
+                P.set_load(n / nsteps)
+            else:
+                P.advance_load(n / nsteps, 1)
+            per.append(P.solve(o))
+            log(f"  {tag} increment {n}/{nsteps}: secant {per[-1].outer_iters} cg {per[-1].cg_iters} "
+                f"{per[-1].seconds:.3f} s")
+        out[tag] = (time.perf_counter() - t0, per, P)
+    (tm, per_m, Pm), (tf, per_f, Pf) = out["mixed"], out["fp64"]
+    nnz = scalar_nnz(A, Pm)
+    N = 2 * mesh.nn
+    ms, fmt = bench_cg(A, Pm, A.MIXED, args.reps, precond=True)
+    byt = cg_iteration_bytes(nnz, N, 4)
+    import ctypes as C
+    asm = {}
+    for tag, prec in (("fp64", A.FP64), ("fp32", A.MIXED)):
+        t = C.c_double()
+        A.check(A.lib().cp_bench_assemble(Pm.ptr, prec, 20, C.byref(t)))
+        asm[tag] = {"ms": t.value, "elements_per_s": mesh.ne / (t.value * 1e-3)}
+    epsT = mat.sigmaT / mat.E
+    line = {"metric": METRIC, "value": nsteps / tm, "unit": "load-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": 0,
+            "ms_per_step": tm / nsteps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": name, "nodes": mesh.nn, "elements": mesh.ne, "dof": N, "nnz": nnz,
+                       "nodes_per_element": 8},
+            "fp64_load_steps_per_s": nsteps / tf, "speedup_vs_fp64": tf / tm,
+            "rel_err_vs_fp64": {"u": rel(Pm.field("u"), Pf.field("u")), "sigma_i": rel(Pm.field("sigI"), Pf.field("sigI")),
+                                "plastic_flags_equal": bool(np.array_equal(Pm.field("epsI") >= epsT,
+                                                                           Pf.field("epsI") >= epsT))},
+            "plastic_elements": int((Pm.field("epsI") >= epsT).sum()),
+            "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
+                           "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
+            "assembly": asm,
+            "roofline": {"bound": "hbm", "achieved": byt / ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": byt / ms / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "cpk::k_cg_iter_fused<0,true,float> (block-Jacobi CG iteration, 8-node matrix)",
+                         "bytes_per_launch": byt, "avg_launch_ms": ms, "format_bytes_per_launch": fmt,
+                         "achieved_format": fmt / ms / 1e6},
+            "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "q8"),
                     help="c2: the bench line (632x316 plate); c3: 2000x1000 plate, Swift law, 40 increments; "
-                         "c4: 16700x60 graded quarter ring, 30 increments")
+                         "c4: 16700x60 graded quarter ring, 30 increments; q8: 8-node quads on the reference's "
+                         "energyConvTest plate")
+    ap.add_argument("--q8-k", type=int, default=64,
+                    help="q8 mesh: genRectWithHole(-3, 3, -2, 2, 0.5, 3k, 2k, order 2)")
     ap.add_argument("--steps", type=int, default=NSTEPS)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nt", type=int, default=632)
@@ -337,6 +408,10 @@ def main():
               "stream would shard over several GPUs is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
     K = max(1, min(args.steps, NSTEPS))
+    if args.workload == "q8":
+        import cudaplasticity_amd.abi as A
+        q8_workload(A, args)
+        return
     if args.workload != "c2":
         from cudaplasticity_amd import abi as A
         other_workload(A, args)

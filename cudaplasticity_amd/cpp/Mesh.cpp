@@ -25,9 +25,9 @@ int as_int(size_t n, const char* what)
     return (int)n;
 }
 
-void first_order_only(int order)
+void check_order(int order)
 {
-    if (order != 1) throw std::invalid_argument("Mesh: only 4-node (order 1) elements are provided");
+    if (order != 1 && order != 2) throw std::invalid_argument("Mesh: order must be 1 (4-node) or 2 (8-node quads)");
 }
 
 }  // namespace
@@ -73,6 +73,10 @@ Mesh& Mesh::operator=(const Mesh& o)
     if (o.elem4) {
         elem4 = new int[4 * (size_t)count4];
         std::copy(o.elem4, o.elem4 + 4 * (size_t)count4, elem4);
+    }
+    if (o.elem8) {
+        elem8 = new int[8 * (size_t)count8];
+        std::copy(o.elem8, o.elem8 + 8 * (size_t)count8, elem8);
     }
     double* const* src[3] = {&o.spaces, &o.aspects, &o.skewAngles};
     double** dst[3] = {&spaces, &aspects, &skewAngles};
@@ -153,6 +157,44 @@ void Mesh::generate(Sizes sizes, Gen gen)
     elemPos[0] = 0;
     elemPos[1] = ne;
     elemPos[2] = ne;
+    setBorders(nb, blen, bord);
+}
+
+// The same for the order-2 generators: gen(x, y, conn8, sec, bord, blen).
+template <typename Sizes, typename Gen>
+void Mesh::generate2(Sizes sizes, Gen gen)
+{
+    int nn = 0, ne = 0, nb = 0;
+    int blen[16] = {};
+    check(sizes(&nn, &ne, &nb, blen), "mesh sizes");
+    if (nb > 16) throw std::runtime_error("mesh generator: too many borders");
+    std::vector<double> x(nn), y(nn);
+    std::vector<int> conn(8 * (size_t)ne);
+    std::vector<unsigned char> sec(nn);
+    size_t nbord = 0;
+    for (int b = 0; b < nb; ++b) nbord += (size_t)blen[b];
+    std::vector<int> bord(nbord);
+    check(gen(x.data(), y.data(), conn.data(), sec.data(), bord.data(), blen), "mesh generator");
+
+    clear();
+    nodeCount = nn;
+    node = new vec2[nn];
+    secOrdNodes = new bool[nn];
+    for (int i = 0; i < nn; ++i) {
+        node[i] = vec2(x[i], y[i]);
+        secOrdNodes[i] = sec[i] != 0;
+    }
+    count8 = ne;
+    elem8 = new int[8 * (size_t)ne];
+    std::copy(conn.begin(), conn.end(), elem8);
+    elemPos[0] = 0;
+    elemPos[1] = 0;
+    elemPos[2] = ne;
+    setBorders(nb, blen, bord);
+}
+
+void Mesh::setBorders(int nb, const int* blen, const std::vector<int>& bord)
+{
     bordersCount = nb;
     borderLength = new int[nb];
     borders = new int*[nb];
@@ -168,8 +210,14 @@ void Mesh::generate(Sizes sizes, Gen gen)
 
 void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, size_t N2, int order)
 {
-    first_order_only(order);
+    check_order(order);
     const int n1 = as_int(N1, "genRectangle"), n2 = as_int(N2, "genRectangle");
+    if (order == 2)
+        return generate2(
+            [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect2_sizes(n1, n2, nn, ne, nb, bl); },
+            [&](double* x, double* y, int* c, unsigned char* sc, int* b, int* bl) {
+                return cp_mesh_rect2(x1, x2, y1, y2, n1, n2, x, y, c, sc, b, bl);
+            });
     generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect_sizes(n1, n2, nn, ne, nb, bl); },
              [&](double* x, double* y, int* c, int* b, int* bl) {
                  return cp_mesh_rect(x1, x2, y1, y2, n1, n2, x, y, c, b, bl);
@@ -179,8 +227,16 @@ void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, s
 void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double holeRad, size_t N1, size_t N2,
                            int order)
 {
-    first_order_only(order);
+    check_order(order);
     const int n1 = as_int(N1, "genRectWithHole"), n2 = as_int(N2, "genRectWithHole");
+    if (order == 2)
+        return generate2(
+            [&](int* nn, int* ne, int* nb, int* bl) {
+                return cp_mesh_hole2_sizes(x1, x2, y1, y2, holeRad, n1, n2, nn, ne, nb, bl);
+            },
+            [&](double* x, double* y, int* c, unsigned char* sc, int* b, int* bl) {
+                return cp_mesh_hole2(x1, x2, y1, y2, holeRad, n1, n2, x, y, c, sc, b, bl);
+            });
     generate(
         [&](int* nn, int* ne, int* nb, int* bl) {
             return cp_mesh_hole_sizes(x1, x2, y1, y2, holeRad, n1, n2, nn, ne, nb, bl);
@@ -192,8 +248,14 @@ void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double ho
 
 void Mesh::genRing(double a, double b, size_t N_phi, size_t N_r, int order)
 {
-    first_order_only(order);
+    check_order(order);
     const int np = as_int(N_phi, "genRing"), nr = as_int(N_r, "genRing");
+    if (order == 2)
+        return generate2(
+            [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring2_sizes(np, nr, nn, ne, nb, bl); },
+            [&](double* x, double* y, int* c, unsigned char* sc, int* bd, int* bl) {
+                return cp_mesh_ring2(a, b, np, nr, x, y, c, sc, bd, bl);
+            });
     generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring_sizes(np, nr, nn, ne, nb, bl); },
              [&](double* x, double* y, int* c, int* bd, int* bl) {
                  return cp_mesh_ring(a, b, np, nr, x, y, c, bd, bl);
@@ -223,8 +285,25 @@ void Mesh::genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double
 int Mesh::findMaxIndexDiff() const
 {
     int d = 0;
-    check(cp_mesh_max_index_diff(count4, elem4, &d), "findMaxIndexDiff");
+    if (count4) check(cp_mesh_max_index_diff(count4, elem4, &d), "findMaxIndexDiff");
+    for (int e = 0; e < count8; ++e)   // Mesh.cu:433-439
+        for (int i = 0; i < 8; ++i)
+            for (int j = i + 1; j < 8; ++j) d = std::max(d, std::abs(elem8[8 * e + i] - elem8[8 * e + j]));
     return d;
+}
+
+double Mesh::elemSpace8(int e) const
+{
+    const int* c = elem8 + 8 * (size_t)e;
+    double s1 = 0., s2 = 0.;
+    for (int i = 0; i < 4; ++i) {   // corner i, mid-side i + 4, next corner
+        const vec2& p = node[c[i]];
+        const vec2& m = node[c[i + 4]];
+        const vec2& q = node[c[(i + 1) & 3]];
+        s1 += p.x * m.y + m.x * q.y;
+        s2 += p.y * m.x + m.y * q.x;
+    }
+    return std::fabs(s1 - s2) * 0.5;
 }
 
 double Mesh::elemSpace4(int e) const
@@ -263,14 +342,17 @@ double Mesh::skewAngleSin(int e) const
 
 double Mesh::avrElemSize() const
 {
-    if (count4 == 0) return 0.;
+    if (count4 + count8 == 0) return 0.;
     double s = 0.;
     for (int e = 0; e < count4; ++e) s += elemSpace4(e);
-    return s / count4;
+    for (int e = 0; e < count8; ++e) s += elemSpace8(e);
+    return s / (count4 + count8);
 }
 
 void Mesh::printAnalysis()
 {
+    // the quality measures are those of 4-node quads (Mesh.cu:1032-1080)
+    if (count8) throw std::invalid_argument("Mesh::printAnalysis: provided for 4-node meshes");
     delete[] spaces;
     delete[] aspects;
     delete[] skewAngles;
@@ -315,10 +397,14 @@ void Mesh::saveAsVtk(const std::string& fileName) const
     if (!f) throw std::runtime_error("Mesh::saveAsVtk: cannot open " + fileName);
     f << "# vtk DataFile Version 2.0\nMesh\nASCII\nDATASET POLYDATA\nPOINTS " << nodeCount << " float\n";
     for (int i = 0; i < nodeCount; ++i) f << node[i].x << " " << node[i].y << " " << 0 << "\n";
-    f << "POLYGONS " << elemCount() << " " << 5 * count4;
+    f << "POLYGONS " << elemCount() << " " << 5 * count4 + 9 * count8;
     for (int e = 0; e < count4; ++e) {
         f << "\n4 ";
         for (int j = 0; j < 4; ++j) f << elem4[4 * e + j] << " ";
+    }
+    for (int e = 0; e < count8; ++e) {   // corner, mid-side, corner, ... (Mesh.cu:959-963)
+        f << "\n8 ";
+        for (int j = 0; j < 4; ++j) f << elem8[8 * e + j] << " " << elem8[8 * e + j + 4] << " ";
     }
     f << "\nCELL_DATA " << elemCount() << "\nSCALARS ElementID int 1\nLOOKUP_TABLE default\n";
     for (int e = 0; e < elemCount(); ++e) f << e << "\n";
@@ -344,6 +430,12 @@ void Mesh::print() const
     for (int e = 0; e < count4; ++e)
         std::cout << e << ": " << elem4[4 * e] << " " << elem4[4 * e + 1] << " " << elem4[4 * e + 2] << " "
                   << elem4[4 * e + 3] << "\n";
+    if (count8) std::cout << "8-node quads: " << count8 << "\n";
+    for (int e = 0; e < count8; ++e) {
+        std::cout << e << ":";
+        for (int j = 0; j < 8; ++j) std::cout << " " << elem8[8 * e + j];
+        std::cout << "\n";
+    }
     for (int b = 0; b < bordersCount; ++b) {
         std::cout << "Border " << b << ":";
         for (int i = 0; i < borderLength[b]; ++i) std::cout << " " << borders[b][i];

@@ -67,9 +67,16 @@ void PlasticitySolver::setPlainCondition(plainCondition pc)
 void PlasticitySolver::initConditions(std::vector<double>& rp, std::vector<double>& u0,
                                       std::vector<unsigned char>& mask) const
 {
-    const int nn = mesh_.nodeCount, ne = mesh_.count4;
-    if (nn <= 0 || ne <= 0 || !mesh_.elem4) throw std::invalid_argument("PlasticitySolver: empty mesh");
-    if (mesh_.count3 || mesh_.count8) throw std::invalid_argument("PlasticitySolver: only 4-node quads are provided");
+    const int nn = mesh_.nodeCount, ne = mesh_.count4 + mesh_.count8;
+    if (nn <= 0 || ne <= 0 || (mesh_.count4 && !mesh_.elem4) || (mesh_.count8 && !mesh_.elem8))
+        throw std::invalid_argument("PlasticitySolver: empty mesh");
+    if (mesh_.count3 || (mesh_.count4 && mesh_.count8))
+        throw std::invalid_argument("PlasticitySolver: the mesh must hold 4-node or 8-node quads only");
+    std::vector<unsigned char> sec;
+    if (mesh_.count8) {
+        sec.assign(nn, 0);
+        for (int i = 0; i < nn; ++i) sec[i] = mesh_.secOrdNodes && mesh_.secOrdNodes[i] ? 1 : 0;
+    }
     std::vector<double> x(nn), y(nn);
     rp.assign(2 * (size_t)nn, 0.);
     u0.assign(2 * (size_t)nn, 0.);
@@ -90,10 +97,12 @@ void PlasticitySolver::initConditions(std::vector<double>& rp, std::vector<doubl
             fval[2 * k] = f.x;
             fval[2 * k + 1] = f.y;
         }
-        check(cp_rhs_border_force(x.data(), y.data(), b, len, fval.data(), fc.normOrient ? 1 : 0, m_.h, rp.data()),
+        check(cp_rhs_border_force_q(x.data(), y.data(), b, len, sec.empty() ? nullptr : sec.data(), fval.data(),
+                                    fc.normOrient ? 1 : 0, m_.h, rp.data()),
               "border force");
     }
     if (cond_.Rset) {
+        if (mesh_.count8) throw std::invalid_argument("PlasticitySolver: body forces are provided for 4-node quads");
         std::vector<double> rnode(2 * (size_t)nn);
         for (int i = 0; i < nn; ++i) {
             const vec2 r = cond_.R(mesh_.node[i]);
@@ -151,8 +160,9 @@ void PlasticitySolver::buildProblem()
         y[i] = mesh_.node[i].y;
     }
     const cp_material mat = m_.toAbi(pc_ == strain ? CP_PLANE_STRAIN : CP_PLANE_STRESS);
-    check(cp_problem_create(shared_context(), nn, mesh_.count4, x.data(), y.data(), mesh_.elem4, rp.data(),
-                            u0.data(), mask.data(), &mat, &prob_),
+    const bool q8 = mesh_.count8 > 0;
+    check(cp_problem_create_n(shared_context(), q8 ? 8 : 4, nn, q8 ? mesh_.count8 : mesh_.count4, x.data(), y.data(),
+                              q8 ? mesh_.elem8 : mesh_.elem4, rp.data(), u0.data(), mask.data(), &mat, &prob_),
           "problem setup");
 }
 
@@ -231,7 +241,7 @@ void PlasticitySolver::dataToRAM()
 {
     if (ramSaved_) return;
     if (!prob_) throw std::runtime_error("PlasticitySolver: nothing solved yet");
-    fields_.assign(CP_NFIELDS, std::vector<double>(mesh_.count4));
+    fields_.assign(CP_NFIELDS, std::vector<double>(mesh_.elemCount()));
     for (int f = 0; f < CP_NFIELDS; ++f) check(cp_download(prob_, f, fields_[f].data()), "download");
     u_.assign(2 * (size_t)mesh_.nodeCount, 0.);
     check(cp_download(prob_, CP_F_U, u_.data()), "download");
@@ -255,7 +265,7 @@ double PlasticitySolver::getSxx0() const
 {
     double v = 0.;
     if (!prob_) return 0.;
-    std::vector<double> s(mesh_.count4);
+    std::vector<double> s(mesh_.elemCount());
     check(cp_download(prob_, CP_F_SXX, s.data()), "download");
     v = s[0];
     return v;
@@ -264,14 +274,15 @@ double PlasticitySolver::getSxx0() const
 double PlasticitySolver::getExx0() const
 {
     if (!prob_) return 0.;
-    std::vector<double> s(mesh_.count4);
+    std::vector<double> s(mesh_.elemCount());
     check(cp_download(prob_, CP_F_EXX, s.data()), "download");
     return s[0];
 }
 
 // PlasticitySolver.cuh:415-445: elastic elements add eps_i sigma_i V / 2;
 // yielded ones sigma_T eps_T V / 2 plus the 100-panel trapezoid rule of f
-// from eps_T to eps_i.
+// from eps_T to eps_i.  8-node elements add eps_i sigma_i V / 2 in every
+// case (:437-443).
 double PlasticitySolver::deformEnergy()
 {
     dataToRAM();
@@ -289,6 +300,7 @@ double PlasticitySolver::deformEnergy()
             energy += 0.5 * eI[e] * sI[e] * V;
         }
     }
+    for (int e = 0; e < mesh_.count8; ++e) energy += eI[e] * sI[e] * 0.5 * (m_.h * mesh_.elemSpace8(e));
     return energy;
 }
 
@@ -305,15 +317,19 @@ void PlasticitySolver::saveDisplsToFile(const std::string& fileName)
 void PlasticitySolver::saveAsVtk(const std::string& fileName)
 {
     dataToRAM();
-    const int nn = mesh_.nodeCount, ne = mesh_.count4;
+    const int nn = mesh_.nodeCount, ne = mesh_.elemCount();
     std::ofstream f(fileName);
     if (!f) throw std::runtime_error("saveAsVtk: cannot open " + fileName);
     f << "# vtk DataFile Version 2.0\nSolution\nASCII\nDATASET POLYDATA\nPOINTS " << nn << " float\n";
     for (int i = 0; i < nn; ++i) f << mesh_.node[i].x << " " << mesh_.node[i].y << " " << 0 << "\n";
-    f << "POLYGONS " << ne << " " << 5 * ne;
-    for (int e = 0; e < ne; ++e) {
+    f << "POLYGONS " << ne << " " << 5 * mesh_.count4 + 9 * mesh_.count8;
+    for (int e = 0; e < mesh_.count4; ++e) {
         f << "\n4 ";
         for (int j = 0; j < 4; ++j) f << mesh_.elem4[4 * (size_t)e + j] << " ";
+    }
+    for (int e = 0; e < mesh_.count8; ++e) {   // corner, mid-side, ... (PlasticitySolver.cu:1809-1813)
+        f << "\n8 ";
+        for (int j = 0; j < 4; ++j) f << mesh_.elem8[8 * (size_t)e + j] << " " << mesh_.elem8[8 * (size_t)e + j + 4] << " ";
     }
     f << "\nCELL_DATA " << ne << "\nSCALARS ElementID int 1\nLOOKUP_TABLE default\n";
     for (int e = 0; e < ne; ++e) f << e << "\n";
@@ -371,11 +387,18 @@ void PlasticitySolver::saveAsVtk(const std::string& fileName)
     for (int i = 0; i < nn; ++i) f << u_[2 * (size_t)i] << " " << u_[2 * (size_t)i + 1] << " 0 ";
 }
 
+// Mean of the element's corner nodes.
 vec2 PlasticitySolver::elemCenter(int e) const
 {
     vec2 c;
-    for (int i = 0; i < 4; ++i) c += mesh_.node[mesh_.elem4[4 * (size_t)e + i]];
+    const bool q8 = mesh_.count8 > 0;
+    for (int i = 0; i < 4; ++i) c += mesh_.node[q8 ? mesh_.elem8[8 * (size_t)e + i] : mesh_.elem4[4 * (size_t)e + i]];
     return c / 4.;
+}
+
+double PlasticitySolver::elemArea(int e) const
+{
+    return mesh_.count8 > 0 ? mesh_.elemSpace8(e) : mesh_.elemSpace4(e);
 }
 
 // PlasticitySolver.cuh:464-494: largest nodal-centre error of each stress
@@ -386,8 +409,8 @@ double PlasticitySolver::sigmaError(std::function<double(vec2)> s_x, std::functi
     dataToRAM();
     const std::vector<double>&sx = fields_[CP_F_SXX], &sy = fields_[CP_F_SYY], &txy = fields_[CP_F_TAU];
     double mean[3] = {0., 0., 0.}, worst[3] = {0., 0., 0.}, vol = 0.;
-    for (int e = 0; e < mesh_.count4; ++e) {
-        const double a = mesh_.elemSpace4(e);
+    for (int e = 0; e < mesh_.elemCount(); ++e) {
+        const double a = elemArea(e);
         mean[0] += sx[e] * a;
         mean[1] += sy[e] * a;
         mean[2] += txy[e] * a;
@@ -410,7 +433,7 @@ double PlasticitySolver::polarSigmaError(std::function<double(vec2)> s_r, std::f
     const std::vector<double>&sx = fields_[CP_F_SXX], &sy = fields_[CP_F_SYY], &txy = fields_[CP_F_TAU];
     const double pi = 3.141592653589793;
     double mean[3] = {0., 0., 0.}, worst[3] = {0., 0., 0.}, vol = 0.;
-    for (int e = 0; e < mesh_.count4; ++e) {
+    for (int e = 0; e < mesh_.elemCount(); ++e) {
         const vec2 c = elemCenter(e);
         const double r = c.norm();
         const vec2 rp(r, std::atan(c.y / c.x) + (c.x < 0. ? pi : 0.));
@@ -419,7 +442,7 @@ double PlasticitySolver::polarSigmaError(std::function<double(vec2)> s_r, std::f
         const double srr = co * co * sx[e] + si * si * sy[e] + 2. * co * si * txy[e];
         const double spp = si * si * sx[e] + co * co * sy[e] - 2. * co * si * txy[e];
         const double srp = co * si * (sy[e] - sx[e]) + txy[e] * (co * co - si * si);
-        const double a = mesh_.elemSpace4(e);
+        const double a = elemArea(e);
         mean[0] += std::fabs(srr) * a;
         mean[1] += std::fabs(spp) * a;
         mean[2] += std::fabs(srp) * a;

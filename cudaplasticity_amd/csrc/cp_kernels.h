@@ -411,8 +411,9 @@ __global__ __launch_bounds__(kBlock) void k_cg_init1(CgArgs<T> a, int zero_x)
 // Row of w = A z for node n, z at a neighbour m given by zfun(m).  W > 0:
 // the row is unrolled to W blocks with every load issued up front (indices
 // of padding blocks point at the node itself and their values are zeroed),
-// so a wave has all of its row's loads in flight at once.  W == 0: generic
-// loop for wide rows.
+// so a wave has all of its row's loads in flight at once.  W == 0: wide rows
+// (8-node meshes: 13 or 21 blocks) in chunks of 8 blocks, each chunk's loads
+// issued together; blocks past the slice width are zero.
 template <int W, typename T, typename ZF>
 __device__ __forceinline__ void row_product(const CgArgs<T>& a, int n, const ZF& zfun, T& yx, T& yy)
 {
@@ -438,13 +439,24 @@ __device__ __forceinline__ void row_product(const CgArgs<T>& a, int n, const ZF&
             yy += m[k][2] * zx + m[k][3] * zy;
         }
     } else {
-        size_t q = base;
-        for (int k = 0; k < sl.y; ++k, q += kWave) {
-            T zx, zy, m00, m01, m10, m11;
-            zfun(a.col[q], zx, zy);
-            load_block(a, q, m00, m01, m10, m11);
-            yx += m00 * zx + m01 * zy;
-            yy += m10 * zx + m11 * zy;
+        constexpr int C = 8;
+        for (int k0 = 0; k0 < sl.y; k0 += C) {
+            int cidx[C];
+            T m[C][4];
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const size_t q = base + (size_t)(k0 + j < sl.y ? k0 + j : k0) * kWave;
+                cidx[j] = a.col[q];
+                load_block(a, q, m[j][0], m[j][1], m[j][2], m[j][3]);
+                if (k0 + j >= sl.y) m[j][0] = m[j][1] = m[j][2] = m[j][3] = 0;
+            }
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                T zx, zy;
+                zfun(cidx[j], zx, zy);
+                yx += m[j][0] * zx + m[j][1] * zy;
+                yy += m[j][2] * zx + m[j][3] * zy;
+            }
         }
     }
 }

@@ -1369,6 +1369,31 @@ int cp_bench_spmv(cp_problem* P, int precision, int reps, double* ms, double* by
     return CP_OK;
 }
 
+int cp_bench_assemble(cp_problem* P, int precision, int reps, double* ms)
+{
+    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED))
+        return cp::fail(CP_ERR_INVALID, "bad arguments");
+    const int pre = P->precond;
+    P->precond = 0;   // the element stiffness and its assembly only
+    if (int s = cp::launch_assemble(P, precision)) {
+        P->precond = pre;
+        return s;
+    }
+    CP_HIP(hipEventRecord(P->ev[4], P->stream));
+    for (int i = 0; i < reps; ++i)
+        if (int s = cp::launch_assemble(P, precision)) {
+            P->precond = pre;
+            return s;
+        }
+    CP_HIP(hipEventRecord(P->ev[5], P->stream));
+    CP_HIP(hipEventSynchronize(P->ev[5]));
+    P->precond = pre;
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
+    if (ms) *ms = t / reps;
+    return CP_OK;
+}
+
 int cp_bench_cg(cp_problem* P, int precision, int precond, int reps, double* ms, double* bytes_algo)
 {
     if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED) || precond < 0 || precond > 1)

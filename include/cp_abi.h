@@ -318,6 +318,10 @@ int cp_bench_spmv(cp_problem* prob, int precision, int reps, double* ms, double*
  * average kernel ms and the bytes one launch moves in this library's
  * storage format. */
 int cp_bench_cg(cp_problem* prob, int precision, int precond, int reps, double* ms, double* bytes_format);
+/* Times `reps` assemblies (element stiffness + deterministic assembly into
+ * the FP64 or FP32 matrix, no preconditioner set-up) at the current secant
+ * moduli; returns the average ms of one assembly. */
+int cp_bench_assemble(cp_problem* prob, int precision, int reps, double* ms);
 /* Multigrid timing (precond 2, lattice set): assembles and sets up the
  * hierarchy, then times `reps` launches of the finest level's Chebyshev
  * smoothing step and `reps` V-cycles (HIP events on the problem's stream);

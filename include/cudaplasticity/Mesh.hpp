@@ -4,12 +4,14 @@
 // signatures as the reference; the generators run in libcp_hip (host code)
 // and produce the reference's node numbering and element order.  Meshes are
 // host data here: the solver copies them to the device when it is built, so
-// meshToGPU / meshToRAM are no-ops kept for source compatibility.  Only
-// 4-node quads exist in this library (order 2 is refused).
+// meshToGPU / meshToRAM are no-ops kept for source compatibility.  order = 2
+// builds 8-node quads (elem8, count8) as the reference's remapOrder does;
+// a mesh holds one element type.
 #pragma once
 
 #include <cstddef>
 #include <string>
+#include <vector>
 
 #include "vec2.hpp"
 
@@ -25,13 +27,13 @@ public:
 
     int* elem3 = nullptr;
     int* elem4 = nullptr;   // 4 node indices per quad, reference local order
-    int* elem8 = nullptr;
+    int* elem8 = nullptr;   // 8 node indices: corners, then mid-sides of edges 1-2, 2-3, 3-4, 4-1
 
     int elemPos[3] = {};    // first index of triangles / quads / 8-node elements
 
     int nodeCount = 0;
     vec2* node = nullptr;
-    bool* secOrdNodes = nullptr;   // all false: first-order meshes only
+    bool* secOrdNodes = nullptr;   // true on the mid-side nodes of 8-node quads
 
     int bordersCount = 0;
     int* borderLength = nullptr;
@@ -63,11 +65,13 @@ public:
 
     // Mesh.cuh:258-267 (shoelace area of quad e)
     double elemSpace4(int e) const;
+    // Mesh.cuh:269-278 (shoelace area of the 8-node quad's octagon)
+    double elemSpace8(int e) const;
     // Mesh.cuh:281-293
     double aspectRatio(int e) const;
     // Mesh.cuh:296-305
     double skewAngleSin(int e) const;
-    // mean quad area
+    // mean element area (Mesh.cuh:308-318)
     double avrElemSize() const;
 
     // Element quality arrays (area, aspect ratio, skew sine per quad) filled
@@ -89,4 +93,7 @@ private:
     void clear();
     template <typename Sizes, typename Gen>
     void generate(Sizes sizes, Gen gen);
+    template <typename Sizes, typename Gen>
+    void generate2(Sizes sizes, Gen gen);
+    void setBorders(int nb, const int* blen, const std::vector<int>& bord);
 };

@@ -115,4 +115,5 @@ private:
     double run(bool elastic, bool mixed);
     void dataToRAM();
     vec2 elemCenter(int e) const;
+    double elemArea(int e) const;
 };

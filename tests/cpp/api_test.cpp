@@ -170,11 +170,41 @@ int run_cpu()
         dump("laws.f64", out);
     }
     {
-        // order-2 meshes and the CPU solver are refused, loudly
+        // order-2 meshes of the reference's mesh files (Main.cpp:1173-1190):
+        // generators, the mesh VTK writer, the 8-node conditions
+        Mesh rect, hole, ring;
+        rect.genRectangle(-2, 2, -1, 1, 12, 6, 2);
+        hole.genRectWithHole(-3, 3, -2, 2, 0.5, 12, 8, 2);
+        ring.genRing(1, 2, 40, 6, 2);
+        rect.saveAsVtk(outdir + "/rect1.vtk");
+        hole.saveAsVtk(outdir + "/hole1.vtk");
+        ring.saveAsVtk(outdir + "/ring1.vtk");
+        std::vector<unsigned char> sec(hole.nodeCount);
+        for (int i = 0; i < hole.nodeCount; ++i) sec[i] = hole.secOrdNodes[i] ? 1 : 0;
+        dump("hole1_sec.u8", sec);
+        std::vector<double> q8 = {hole.avrElemSize(), (double)hole.findMaxIndexDiff(), hole.elemSpace8(0)};
+        dump("hole1_q8.f64", q8);
+        LoadConditions lc;
+        lc.setForce(1, vec2(170., 3.));
+        lc.setForce(3, vec2(-170., 0.));
+        lc.setForce(4, vec2(5., 1.), true);
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.h = 0.7;
+        PlasticitySolver ps(m, hole, lc);
+        dump_conditions("hole1", ps);
+        Mesh copy(hole);
+        std::vector<int> c8(copy.elem8, copy.elem8 + 8 * (size_t)copy.count8);
+        dump("hole1copy_elem8.i32", c8);
+    }
+    {
+        // the CPU solver and the 4-node mesh analysis of an 8-node mesh are refused, loudly
         int refused = 0;
         Mesh mesh;
+        mesh.genRectangle(0, 1, 0, 1, 2, 2, 2);
         try {
-            mesh.genRectangle(0, 1, 0, 1, 2, 2, 2);
+            mesh.printAnalysis();
         } catch (const std::invalid_argument&) {
             ++refused;
         }
@@ -312,6 +342,27 @@ int run_gpu()
         ps.solveSteps(4, 1.0);
         dump("steps_u.f64", ps.displacements());
         dump("steps_sigi.f64", ps.field(CP_F_SIGI));
+    }
+    {
+        // energyConvTest (Main.cpp:105-178) on the 660-element 8-node mesh,
+        // power law, the reference's defaults: data/aitken__plast2.txt
+        Mesh mesh;
+        mesh.genRectWithHole(-3., 3., -2., 2., 0.5, 18, 12, 2);
+        LoadConditions lc;
+        lc.setForce(1, vec2(170e6, 0));
+        lc.setForce(3, vec2(-170e6, 0));
+        lc.fixVertAxis(0.);
+        lc.fixHorAxis(0.);
+        Material m;
+        m.setPowerPlast(E_304L, S_T_304L, E_304L * 0.01, 0.015);
+        m.nu = NU_304L;
+        m.h = 1.;
+        PlasticitySolver ps(m, mesh, lc);
+        ps.iterOutput = false;
+        ps.solve();
+        std::vector<double> e = {ps.deformEnergy(), (double)mesh.elemCount(), (double)ps.lastStats().status};
+        dump("q8_energy.f64", e);
+        ps.saveAsVtk(outdir + "/q8_plate.vtk");
     }
     std::cout << "gpu cases written to " << outdir << "\n";
     return 0;

@@ -210,8 +210,49 @@ def test_material_laws_on_host(cpu_out):
 
 
 def test_unsupported_paths_refuse(cpu_out):
-    """order-2 meshes and solveCPU throw instead of silently degrading."""
+    """solveCPU and the 4-node mesh analysis of an 8-node mesh throw instead
+    of silently degrading."""
     assert rd(cpu_out, "refused.i32")[0] == 2
+
+
+@pytest.mark.parametrize("name", ["rect1", "hole1", "ring1"])
+def test_order2_meshes_through_cpp_api(cpu_out, name):
+    """Mesh::genRectangle / genRectWithHole / genRing with order 2 and
+    Mesh::saveAsVtk write the reference's data/rect1.vtk, hole1.vtk and
+    ring1.vtk meshes: same polygons, points to the printed digits."""
+    g = np.load(os.path.join(GOLD, "mesh_%s.npz" % name))
+    lay, _, pts, polys = read_vtk(os.path.join(cpu_out, name + ".vtk"))
+    assert np.array_equal(polys, g["polys"])
+    assert np.abs(pts - g["points"]).max() <= 5e-6 * max(1.0, np.abs(g["points"]).max())
+    assert "POLYGONS %d %d" % (g["polys"].shape[0], 9 * g["polys"].shape[0]) in lay
+
+
+def test_order2_conditions_through_cpp_api(cpu_out):
+    om = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2)
+    op = O.Problem(om, h=0.7).force(1, 170.0, 3.0).force(3, -170.0, 0.0).force(4, 5.0, 1.0, norm_orient=True)
+    op.fix_vert_axis(0.0).fix_hor_axis(0.0)
+    assert np.array_equal(rd(cpu_out, "hole1_sec.u8"), om.sec)
+    assert np.array_equal(rd(cpu_out, "hole1_rp.f64"), op.rp)
+    assert np.array_equal(rd(cpu_out, "hole1_mask.u8"), op.mask)
+    assert np.array_equal(rd(cpu_out, "hole1copy_elem8.i32"), om.conn.ravel())
+    avr, maxdiff, s0 = rd(cpu_out, "hole1_q8.f64")
+    areas = np.array([O.elem_space(om, e) for e in range(om.ne)])
+    assert abs(avr - areas.mean()) <= 1e-12 * avr and abs(s0 - areas[0]) <= 1e-12 * s0
+    assert maxdiff == max(int(np.ptp(c)) for c in om.conn)
+
+
+@pytest.mark.gpu
+def test_order2_energy_through_cpp_api(tmp_path):
+    """energyConvTest on the 660-element 8-node plate through
+    PlasticitySolver::solve() with the reference's defaults and
+    deformEnergy(): data/aitken__plast2.txt prints 2.45491e+06."""
+    run("gpu", tmp_path)
+    en, ne, status = rd(tmp_path, "q8_energy.f64")
+    print("8-node energy %.1f on %d elements" % (en, ne), flush=True)
+    assert ne == 660 and status == 0
+    assert abs(en - 2.45491e6) <= 5.0
+    lay, VF, _, polys = read_vtk(os.path.join(tmp_path, "q8_plate.vtk"))
+    assert polys.shape == (660, 8) and VF["Sigma_intensity"].size == 660
 
 
 @pytest.mark.gpu
