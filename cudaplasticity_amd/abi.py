@@ -203,6 +203,34 @@ def gen_ring(a, b, Nphi, Nr, order=1):
     return _gen("cp_mesh_ring_sizes", (Nphi, Nr), "cp_mesh_ring", (_f(a), _f(b), Nphi, Nr))
 
 
+def load_inp(path):
+    """Mesh::loadFromFile (Mesh.cu:725-937): an .inp file of S4 (4-node) or
+    S8R (8-node) quads with *MPC borders."""
+    L = lib()
+    p = os.fsencode(path)
+    nn, n3, n4, n8, nb = (C.c_int() for _ in range(5))
+    check(L.cp_mesh_inp_sizes(p, C.byref(nn), C.byref(n3), C.byref(n4), C.byref(n8), C.byref(nb), None, 0))
+    blen = np.zeros(max(nb.value, 1), dtype=np.int32)
+    check(L.cp_mesh_inp_sizes(p, C.byref(nn), C.byref(n3), C.byref(n4), C.byref(n8), C.byref(nb), ip(blen),
+                              nb.value))
+    if n3.value or (n4.value and n8.value):
+        raise ValueError("%s: the solver takes meshes of 4-node or of 8-node quads only" % path)
+    x, y = np.zeros(nn.value), np.zeros(nn.value)
+    c3 = np.zeros(1, dtype=np.int32)
+    c4 = np.zeros(max(4 * n4.value, 1), dtype=np.int32)
+    c8 = np.zeros(max(8 * n8.value, 1), dtype=np.int32)
+    sec = np.zeros(max(nn.value, 1), dtype=np.uint8)
+    bord = np.zeros(max(int(blen[:nb.value].sum()), 1), dtype=np.int32)
+    check(L.cp_mesh_load_inp(p, dp(x), dp(y), ip(c3), ip(c4), ip(c8), u8p(sec), ip(bord)))
+    borders, off = [], 0
+    for b in range(nb.value):
+        borders.append(bord[off:off + blen[b]].copy())
+        off += blen[b]
+    if n8.value:
+        return Mesh(x, y, c8[:8 * n8.value], borders, npe=8, sec=sec[:nn.value])
+    return Mesh(x, y, c4[:4 * n4.value], borders)
+
+
 def gen_quarter_plate(L, r, Nt, Nr):
     return _gen("cp_mesh_quarter_plate_sizes", (Nt, Nr), "cp_mesh_quarter_plate", (_f(L), _f(r), Nt, Nr))
 
@@ -244,8 +272,11 @@ class Loads:
         return self
 
     def gravity(self, g=9.81, rho=1.0):
-        if self.mesh.npe != 4:
-            raise ValueError("body forces are provided for 4-node meshes")
+        if self.mesh.npe == 8:
+            rn = np.tile(np.array([0.0, -g]), self.mesh.nn)
+            check(lib().cp_rhs_body_force_nodal8(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
+                                                 dp(rn), _f(rho), _f(self.h), dp(self.rp)))
+            return self
         check(lib().cp_rhs_body_force(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
                                       _f(0.0), _f(-g), _f(rho), _f(self.h), dp(self.rp)))
         return self

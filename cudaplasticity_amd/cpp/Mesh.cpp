@@ -282,6 +282,49 @@ void Mesh::genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double
         });
 }
 
+bool Mesh::loadFromFile(const std::string& fileName)
+{
+    if (fileName.size() < 4 || fileName.compare(fileName.size() - 4, 4, ".inp") != 0)
+        throw "Unknown mesh file format";
+    if (!std::ifstream(fileName)) throw "There is no such file";
+    int nn = 0, n3 = 0, n4 = 0, n8 = 0, nb = 0;
+    check(cp_mesh_inp_sizes(fileName.c_str(), &nn, &n3, &n4, &n8, &nb, nullptr, 0), "loadFromFile");
+    std::vector<int> blen(nb);
+    check(cp_mesh_inp_sizes(fileName.c_str(), &nn, &n3, &n4, &n8, &nb, blen.data(), nb), "loadFromFile");
+    std::vector<double> x(nn), y(nn);
+    std::vector<int> c3(3 * (size_t)n3), c4(4 * (size_t)n4), c8(8 * (size_t)n8);
+    std::vector<unsigned char> sec(nn);
+    size_t nbord = 0;
+    for (int b : blen) nbord += (size_t)b;
+    std::vector<int> bord(nbord);
+    check(cp_mesh_load_inp(fileName.c_str(), x.data(), y.data(), c3.data(), c4.data(), c8.data(), sec.data(),
+                           bord.data()),
+          "loadFromFile");
+    clear();
+    nodeCount = nn;
+    node = new vec2[nn];
+    secOrdNodes = new bool[nn];
+    for (int i = 0; i < nn; ++i) {
+        node[i] = vec2(x[i], y[i]);
+        secOrdNodes[i] = sec[i] != 0;
+    }
+    count3 = n3;
+    count4 = n4;
+    count8 = n8;
+    elem3 = new int[c3.size()];
+    elem4 = new int[c4.size()];
+    elem8 = new int[c8.size()];
+    std::copy(c3.begin(), c3.end(), elem3);
+    std::copy(c4.begin(), c4.end(), elem4);
+    std::copy(c8.begin(), c8.end(), elem8);
+    elemPos[0] = n3;   // fillPos: triangles, then quads, then 8-node quads
+    elemPos[1] = n3 + n4;
+    elemPos[2] = n3 + n4 + n8;
+    setBorders(nb, blen.data(), bord);
+    analysed = false;
+    return true;
+}
+
 int Mesh::findMaxIndexDiff() const
 {
     int d = 0;

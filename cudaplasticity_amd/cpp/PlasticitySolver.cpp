@@ -102,15 +102,20 @@ void PlasticitySolver::initConditions(std::vector<double>& rp, std::vector<doubl
               "border force");
     }
     if (cond_.Rset) {
-        if (mesh_.count8) throw std::invalid_argument("PlasticitySolver: body forces are provided for 4-node quads");
         std::vector<double> rnode(2 * (size_t)nn);
         for (int i = 0; i < nn; ++i) {
             const vec2 r = cond_.R(mesh_.node[i]);
             rnode[2 * i] = r.x;
             rnode[2 * i + 1] = r.y;
         }
-        check(cp_rhs_body_force_nodal(ne, x.data(), y.data(), mesh_.elem4, rnode.data(), m_.rho, m_.h, rp.data()),
-              "body force");
+        if (mesh_.count8)
+            check(cp_rhs_body_force_nodal8(ne, x.data(), y.data(), mesh_.elem8, rnode.data(), m_.rho, m_.h,
+                                           rp.data()),
+                  "body force");
+        else
+            check(cp_rhs_body_force_nodal(ne, x.data(), y.data(), mesh_.elem4, rnode.data(), m_.rho, m_.h,
+                                          rp.data()),
+                  "body force");
     }
     for (const auto& [id, dc] : cond_.displCond) {
         auto [b, len] = border(id, "setDisplacement");

@@ -7,6 +7,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
 #include <vector>
 
 #include "cp_abi.h"
@@ -123,6 +126,65 @@ int hole_mesh(double x1, double x2, double y1, double y2, double r, int N1, int 
     return CP_OK;
 }
 
+}  // namespace
+
+namespace {
+struct InpCounts {
+    int nodeLines = 0, n3 = 0, n4 = 0, n8 = 0;
+    std::vector<int> blen;
+};
+
+void blank_commas(std::string& s)
+{
+    for (char& c : s)
+        if (c == ',') c = ' ';
+}
+
+bool body_line(const std::string& l) { return !l.empty() && l[0] != '*'; }
+
+int elem_kind(const std::string& line)
+{
+    if (line.compare(0, 8, "*ELEMENT") != 0) return 0;
+    const size_t t = line.find("TYPE=");
+    if (t == std::string::npos) return 0;
+    if (line.compare(t + 5, 3, "S8R") == 0) return 8;
+    if (line.compare(t + 5, 2, "S4") == 0) return 4;
+    if (line.compare(t + 5, 2, "S3") == 0) return 3;
+    return 0;
+}
+
+// Walks the file once; on every block line calls fn(kind, index-in-block,
+// block-number, line): kind 1 node, 3/4/8 element, 9 border.
+template <typename Fn>
+int walk_inp(const char* path, Fn fn)
+{
+    const std::string name(path ? path : "");
+    if (name.size() < 4 || name.compare(name.size() - 4, 4, ".inp") != 0)
+        return cp::fail(CP_ERR_INVALID, "Unknown mesh file format");
+    std::ifstream f(name);
+    if (!f) return cp::fail(CP_ERR_INVALID, "There is no such file");
+    std::string line;
+    bool have = false;   // `line` already holds the next line to dispatch
+    int nb = 0;
+    while (have || std::getline(f, line)) {
+        have = false;
+        int kind = 0;
+        if (line == "*NODE") kind = 1;
+        else if (line == "*MPC") kind = 9;
+        else kind = elem_kind(line);
+        if (!kind) continue;
+        int k = 0;
+        while (std::getline(f, line)) {
+            if (!body_line(line)) {
+                have = true;
+                break;
+            }
+            if (int s = fn(kind, k++, nb, line)) return s;
+        }
+        if (kind == 9) ++nb;
+    }
+    return CP_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -512,6 +574,34 @@ int cp_rhs_body_force_nodal(int ne, const double* x, const double* y, const int*
     return CP_OK;
 }
 
+// Body force on 8-node quads (PlasticitySolver.cu:1118-1136): corner nodes
+// take -rho h A / 12, mid-side nodes rho h A / 3 of R at the node, A the
+// octagon area of elemSpace8 (Mesh.cuh:269-278).  R is evaluated at the
+// element's own nodes (the reference reads node[2 * index] there, which
+// only a constant R survives).
+int cp_rhs_body_force_nodal8(int ne, const double* x, const double* y, const int* conn, const double* rnode,
+                             double rho, double h, double* rp)
+{
+    for (int e = 0; e < ne; ++e) {
+        const int* c = conn + 8 * (size_t)e;
+        double s1 = 0., s2 = 0.;
+        for (int i = 0; i < 3; ++i) {
+            s1 += x[c[i]] * y[c[i + 4]] + x[c[i + 4]] * y[c[i + 1]];
+            s2 += y[c[i]] * x[c[i + 4]] + y[c[i + 4]] * x[c[i + 1]];
+        }
+        s1 += x[c[3]] * y[c[7]] + x[c[7]] * y[c[0]];
+        s2 += y[c[3]] * x[c[7]] + y[c[7]] * x[c[0]];
+        const double space = std::fabs(s1 - s2) * 0.5;
+        const double w1 = -rho * h * space / 12., w2 = rho * h * space / 3.;
+        for (int i = 0; i < 8; ++i) {
+            const double w = i < 4 ? w1 : w2;
+            rp[2 * c[i]] += w * rnode[2 * c[i]];
+            rp[2 * c[i] + 1] += w * rnode[2 * c[i] + 1];
+        }
+    }
+    return CP_OK;
+}
+
 int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask)
 {
     const double* v = vertical ? x : y;
@@ -639,6 +729,107 @@ int cp_mesh_remap_order(int nn, int ne4, const double* x, const double* y, const
         for (int t = 4; t < 8; ++t) sec[conn8[8 * e + t]] = 1;
     *nn8 = m;
     return CP_OK;
+}
+
+// Mesh::loadFromFile (Mesh.cu:725-937): an Abaqus-style .inp file.  Lines
+// are matched as the reference matches them: a block starts at a line equal
+// to "*NODE" or "*MPC", or beginning with "*ELEMENT" and holding "TYPE=S3",
+// "TYPE=S4" or "TYPE=S8R", and runs to the next empty line or line starting
+// with '*'.  Commas count as blanks.  Node lines are "id x y ..." with 1-based
+// ids; element lines "id n1 n2 ..." (1-based); every *MPC block is one border,
+// its lines holding the border node after the first four characters.  One
+// node per border (the MPC reference node) is not a mesh node: the node count
+// is the number of node lines less the number of borders, and node lines with
+// a larger id are skipped.
+int cp_mesh_inp_sizes(const char* path, int* nn, int* n3, int* n4, int* n8, int* nb, int* blen, int max_borders)
+{
+    if (!nn || !n3 || !n4 || !n8 || !nb) return cp::fail(CP_ERR_INVALID, "cp_mesh_inp_sizes: null argument");
+    InpCounts c;
+    int s = walk_inp(path, [&](int kind, int, int b, const std::string&) {
+        if (kind == 1) ++c.nodeLines;
+        else if (kind == 3) ++c.n3;
+        else if (kind == 4) ++c.n4;
+        else if (kind == 8) ++c.n8;
+        else {
+            if ((int)c.blen.size() <= b) c.blen.resize(b + 1, 0);
+            ++c.blen[b];
+        }
+        return CP_OK;
+    });
+    if (s) return s;
+    *nn = c.nodeLines - (int)c.blen.size();
+    *n3 = c.n3;
+    *n4 = c.n4;
+    *n8 = c.n8;
+    *nb = (int)c.blen.size();
+    if (*nn < 0) return cp::fail(CP_ERR_INVALID, "cp_mesh_inp_sizes: fewer nodes than borders");
+    if (blen) {
+        if (*nb > max_borders) return cp::fail(CP_ERR_INVALID, "cp_mesh_inp_sizes: more borders than blen holds");
+        for (int b = 0; b < *nb; ++b) blen[b] = c.blen[b];
+    }
+    return CP_OK;
+}
+
+int cp_mesh_load_inp(const char* path, double* x, double* y, int* conn3, int* conn4, int* conn8, unsigned char* sec,
+                     int* bord)
+{
+    int nn, n3, n4, n8, nb;
+    if (int s = cp_mesh_inp_sizes(path, &nn, &n3, &n4, &n8, &nb, nullptr, 0)) return s;
+    std::vector<int> boff;   // start of each border in bord
+    {
+        InpCounts c;
+        walk_inp(path, [&](int kind, int, int b, const std::string&) {
+            if (kind == 9) {
+                if ((int)c.blen.size() <= b) c.blen.resize(b + 1, 0);
+                ++c.blen[b];
+            }
+            return CP_OK;
+        });
+        int off = 0;
+        for (int b = 0; b < nb; ++b) {
+            boff.push_back(off);
+            off += c.blen[b];
+        }
+    }
+    if (sec) std::memset(sec, 0, (size_t)std::max(nn, 0));
+    int e3 = 0, e4 = 0, e8 = 0;
+    auto node_index = [&](long v, int& out) {
+        if (v < 1 || v > nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_load_inp: node index out of range");
+        out = (int)(v - 1);
+        return (int)CP_OK;
+    };
+    return walk_inp(path, [&](int kind, int k, int b, const std::string& raw) {
+        std::string line = raw;
+        blank_commas(line);
+        if (kind == 9) {
+            for (size_t i = 0; i < 4 && i < line.size(); ++i) line[i] = ' ';
+            std::istringstream ss(line);
+            long v = 0;
+            if (!(ss >> v)) return cp::fail(CP_ERR_INVALID, "cp_mesh_load_inp: bad *MPC line");
+            return node_index(v, bord[boff[b] + k]);
+        }
+        std::istringstream ss(line);
+        long id = 0;
+        if (!(ss >> id)) return cp::fail(CP_ERR_INVALID, "cp_mesh_load_inp: bad line");
+        if (kind == 1) {
+            double px = 0., py = 0.;
+            if (!(ss >> px >> py)) return cp::fail(CP_ERR_INVALID, "cp_mesh_load_inp: bad *NODE line");
+            if (id >= 1 && id <= nn) {   // larger ids: the borders' reference nodes
+                x[id - 1] = px;
+                y[id - 1] = py;
+            }
+            return (int)CP_OK;
+        }
+        int* c = kind == 3 ? conn3 + 3 * (size_t)e3++ : kind == 4 ? conn4 + 4 * (size_t)e4++ : conn8 + 8 * (size_t)e8++;
+        for (int i = 0; i < kind; ++i) {
+            long v = 0;
+            if (!(ss >> v)) return cp::fail(CP_ERR_INVALID, "cp_mesh_load_inp: short element line");
+            if (int s = node_index(v, c[i])) return s;
+        }
+        if (kind == 8 && sec)
+            for (int i = 4; i < 8; ++i) sec[c[i]] = 1;
+        return (int)CP_OK;
+    });
 }
 
 // Order-2 generators (genRectangle / genRectWithHole / genRing with order = 2):

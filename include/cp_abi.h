@@ -161,6 +161,18 @@ int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
 int cp_mesh_remap_order(int nn, int ne4, const double* x, const double* y, const int* conn4, int width,
                         int nbord, int* bord, double* x8, double* y8, int* conn8, unsigned char* sec,
                         int* nn8);
+/* Mesh::loadFromFile (Mesh.cu:725-937): an Abaqus-style .inp file with
+ * *NODE (1-based "id, x, y"), *ELEMENT TYPE=S3 / S4 / S8R and *MPC blocks
+ * (one border per block, its node after the first four characters of each
+ * line).  nn = node lines less one reference node per border.  Sizes first
+ * (blen: max_borders entries), then the arrays: conn3 (3 n3), conn4 (4 n4),
+ * conn8 (8 n8), sec (nn: mid-side nodes of the S8R elements), bord (sum of
+ * blen), all 0-based.  A path not ending in ".inp" or a missing file is
+ * CP_ERR_INVALID. */
+int cp_mesh_inp_sizes(const char* path, int* nn, int* n3, int* n4, int* n8, int* nb, int* blen,
+                      int max_borders);
+int cp_mesh_load_inp(const char* path, double* x, double* y, int* conn3, int* conn4, int* conn8,
+                     unsigned char* sec, int* bord);
 /* genRectangle / genRectWithHole / genRing with order = 2 (Mesh.cu:17-23,
  * 112-121, 249-253 + remapOrder): 8 nodes per element in conn (corners, then
  * the mid-sides of edges 1-2, 2-3, 3-4, 4-1), sec marks the mid-side nodes;
@@ -197,6 +209,10 @@ int cp_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
  * indexes node[2 * elem] there, which only constant R survives). */
 int cp_rhs_body_force_nodal(int ne, const double* x, const double* y, const int* conn, const double* rnode,
                             double rho, double h, double* rp);
+/* Body force given at every node on 8-node quads (:1118-1136): corners
+ * -rho h A / 12, mid-sides rho h A / 3 of R at the node, A from elemSpace8. */
+int cp_rhs_body_force_nodal8(int ne, const double* x, const double* y, const int* conn, const double* rnode,
+                             double rho, double h, double* rp);
 /* LoadConditions::fixVertAxis / fixHorAxis (:1165-1181). */
 int cp_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask);
 /* LoadConditions::setDisplacement on a border (:1138-1164). */

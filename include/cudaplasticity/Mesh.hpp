@@ -60,6 +60,11 @@ public:
     // inner radius.  Borders: 0 inner arc, 1 theta=pi/2, 2 outer arc, 3 theta=0.
     void genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double ratio);
 
+    // Mesh.cu:725-937: *NODE, *ELEMENT TYPE=S3/S4/S8R and *MPC (borders)
+    // blocks of an .inp file.  As the reference does, throws a const char*
+    // ("Unknown mesh file format", "There is no such file").
+    bool loadFromFile(const std::string& fileName);
+
     // Mesh.cu:417-441
     int findMaxIndexDiff() const;
 

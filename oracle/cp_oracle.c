@@ -888,6 +888,23 @@ void or_rhs_body_force(int ne, const double* x, const double* y, const int* conn
     }
 }
 
+/* 8-node body force with a constant field (PlasticitySolver.cu:1118-1136):
+ * corners -rho h A / 12, mid-sides rho h A / 3, A = elemSpace8. */
+void or_rhs_body_force8(int ne, const double* x, const double* y, const int* conn,
+                        double gx, double gy, double rho, double h, double* rp)
+{
+    for (int e = 0; e < ne; ++e) {
+        double space = or_elem_space8(x, y, conn, e);
+        double c1 = -rho * h * space / 12., c2 = rho * h * space / 3.;
+        for (int i = 0; i < 8; ++i) {
+            int gi = 2 * conn[8 * (size_t)e + i];
+            double c = i < 4 ? c1 : c2;
+            rp[gi] += c * gx;
+            rp[gi + 1] += c * gy;
+        }
+    }
+}
+
 void or_fix_axis(int nn, const double* x, const double* y, int vertical, double coord, unsigned char* mask)
 {
     for (int i = 0; i < nn; ++i) {

@@ -126,6 +126,8 @@ void or_rhs_border_force_q(const double* x, const double* y, const int* border, 
                            double* rp);
 void or_rhs_body_force(int ne, const double* x, const double* y, const int* conn,
                        double gx, double gy, double rho, double h, double* rp);
+void or_rhs_body_force8(int ne, const double* x, const double* y, const int* conn,
+                        double gx, double gy, double rho, double h, double* rp);
 void or_fix_axis(int nn, const double* x, const double* y, int vertical, double coord,
                  unsigned char* mask);
 void or_displacement_border(const int* border, int len, double ux, double uy,

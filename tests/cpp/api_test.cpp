@@ -219,6 +219,39 @@ int run_cpu()
         dump("refused.i32", r);
     }
     if (!indir.empty()) {
+        // Mesh::loadFromFile on an .inp deck of 8-node quads; the reference's
+        // errors are thrown as const char*
+        Mesh mesh;
+        mesh.loadFromFile(indir + "/plate8.inp");
+        std::vector<double> xy(2 * (size_t)mesh.nodeCount);
+        std::vector<unsigned char> sec(mesh.nodeCount);
+        for (int i = 0; i < mesh.nodeCount; ++i) {
+            xy[2 * i] = mesh.node[i].x;
+            xy[2 * i + 1] = mesh.node[i].y;
+            sec[i] = mesh.secOrdNodes[i] ? 1 : 0;
+        }
+        dump("inp_xy.f64", xy);
+        dump("inp_sec.u8", sec);
+        dump("inp_elem8.i32", mesh.elem8, 8 * (size_t)mesh.count8);
+        std::vector<int> bl(mesh.borderLength, mesh.borderLength + mesh.bordersCount);
+        dump("inp_blen.i32", bl);
+        std::vector<int> b1(mesh.borders[1], mesh.borders[1] + mesh.borderLength[1]);
+        dump("inp_border1.i32", b1);
+        std::vector<int> counts = {mesh.count3, mesh.count4, mesh.count8, mesh.elemCount()};
+        int thrown = 0;
+        for (const char* bad : {"mesh.msh", "no_such_mesh.inp"}) {
+            try {
+                Mesh other;
+                other.loadFromFile(indir + "/" + bad);
+            } catch (const char* msg) {
+                ++thrown;
+                std::cout << "loadFromFile(" << bad << "): " << msg << "\n";
+            }
+        }
+        counts.push_back(thrown);
+        dump("inp_counts.i32", counts);
+    }
+    if (!indir.empty()) {
         // a mesh given through the public arrays (the reference's data/ring.vtk
         // points and quads): printAnalysis and the mesh VTK writer
         std::ifstream fp(indir + "/ring_points.f64", std::ios::binary), fq(indir + "/ring_polys.i32", std::ios::binary);

@@ -283,6 +283,11 @@ class Problem:
         return self
 
     def gravity(self, g=9.81, rho=1.0):
+        if self.mesh.npe == 8:
+            lib.or_rhs_body_force8(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
+                                   C.c_double(0.0 * g), C.c_double(-g), C.c_double(rho), C.c_double(self.h),
+                                   dp(self.rp))
+            return self
         lib.or_rhs_body_force(self.mesh.ne, dp(self.mesh.x), dp(self.mesh.y), ip(self.mesh.conn),
                               C.c_double(0.0 * g), C.c_double(-g), C.c_double(rho), C.c_double(self.h),
                               dp(self.rp))

@@ -45,6 +45,8 @@ def cpu_out(tmp_path_factory):
     ring = np.load(os.path.join(GOLD, "ring.npz"))
     np.ascontiguousarray(ring["points"], dtype=np.float64).tofile(os.path.join(src, "ring_points.f64"))
     np.ascontiguousarray(ring["polys"], dtype=np.int32).tofile(os.path.join(src, "ring_polys.i32"))
+    from test_host_setup import write_inp
+    write_inp(os.path.join(src, "plate8.inp"), A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2), "S8R")
     run_in("cpu", d, src)
     return d
 
@@ -213,6 +215,18 @@ def test_unsupported_paths_refuse(cpu_out):
     """solveCPU and the 4-node mesh analysis of an 8-node mesh throw instead
     of silently degrading."""
     assert rd(cpu_out, "refused.i32")[0] == 2
+
+
+def test_load_from_file_through_cpp_api(cpu_out):
+    """Mesh::loadFromFile (Mesh.cu:725-937) on an .inp deck of the order-2
+    hole plate; a wrong extension and a missing file throw const char*."""
+    m = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=2)
+    assert np.array_equal(rd(cpu_out, "inp_xy.f64"), np.stack([m.x, m.y], 1).ravel())
+    assert np.array_equal(rd(cpu_out, "inp_sec.u8"), m.sec)
+    assert np.array_equal(rd(cpu_out, "inp_elem8.i32"), m.conn.ravel())
+    assert list(rd(cpu_out, "inp_blen.i32")) == [b.size for b in m.borders]
+    assert np.array_equal(rd(cpu_out, "inp_border1.i32"), m.borders[1])
+    assert list(rd(cpu_out, "inp_counts.i32")) == [0, 0, m.ne, m.ne, 2]
 
 
 @pytest.mark.parametrize("name", ["rect1", "hole1", "ring1"])

@@ -129,3 +129,25 @@ def test_q8_multigrid_is_refused(ctx):
     with pytest.raises(A.CPError):
         P.solve(A.options(precond=2))
     P.close()
+
+
+def test_q8_gravity_equilibrium(ctx):
+    """An 8-node column under its own weight (consistent body-force loads,
+    corners -1/12 and mid-sides 1/3 of an element's weight), top edge fixed:
+    the reactions on the fixed DOFs carry the whole weight."""
+    pm = A.gen_rect(0.0, 1.0, 0.0, 4.0, 3, 8, order=2)
+    top = pm.borders[1]
+    assert np.allclose(pm.y[top], 4.0)
+    pl = A.Loads(pm, h=0.2).gravity(9.81, 7.8).displacement(1, 0.0, 0.0)
+    P = A.Problem(ctx, pm, pl, A.material(2.1e5, 0.3, h=0.2))
+    P.solve(A.options(cg_tol=1e-12), elastic=True)
+    f = P.internal_force()
+    weight = 7.8 * 9.81 * 0.2 * 4.0
+    fixed_y = 2 * top + 1
+    reaction = (f[fixed_y] - pl.rp[fixed_y]).sum()   # support force: internal force less the lumped load
+    print("reaction %.12e weight %.12e" % (reaction, weight), flush=True)
+    assert abs(reaction - weight) <= 1e-9 * weight
+    assert abs(pl.rp[1::2].sum() + weight) <= 1e-12 * weight
+    free = pl.mask.astype(bool)
+    assert np.linalg.norm((f - pl.rp)[free]) <= 1e-9 * np.linalg.norm(pl.rp[free])
+    P.close()

@@ -1,6 +1,8 @@
 """CPU tests of the product library's host side (no GPU calls): the C-ABI
 loads and exports everything include/cp_abi.h declares; its mesh generators,
 load/BC rules and reference CSR pattern agree with the oracle bit for bit."""
+import os
+
 import numpy as np
 import pytest
 
@@ -182,3 +184,83 @@ def test_remap_rejects_odd_strips():
     code = A.lib().cp_mesh_remap_order(m.nn, m.ne, A.dp(m.x), A.dp(m.y), A.ip(m.conn), 2, 0, None, A.dp(x8),
                                        A.dp(y8), A.ip(c8), A.u8p(sec), A.C.byref(nn8))
     assert code == A.CP_ERR_INVALID
+
+
+def write_inp(path, m, elem_type, extra_nodes=True, blank_between=True):
+    """An Abaqus-style deck of mesh m: *NODE (1-based, then one MPC reference
+    node per border), the elements, one *MPC block per border."""
+    lines = ["*HEADING", "test deck", "*NODE"]
+    for i in range(m.nn):
+        lines.append("%d, %.17g, %.17g, 0" % (i + 1, m.x[i], m.y[i]))
+    if extra_nodes:
+        for b in range(len(m.borders)):
+            lines.append("%d, 0, 0, 0" % (m.nn + b + 1))
+    if blank_between:
+        lines.append("")
+    lines.append("*ELEMENT, TYPE=%s, ELSET=PLATE" % elem_type)
+    for e, c in enumerate(m.conn):
+        lines.append("%d, " % (e + 1) + ", ".join(str(v + 1) for v in c))
+    lines.append("**")
+    for b, bd in enumerate(m.borders):
+        lines.append("*MPC")
+        for v in bd:
+            lines.append("TIE, %d, %d" % (v + 1, m.nn + b + 1))
+    lines.append("*END")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_inp_loader_round_trip(tmp_path, order):
+    """Mesh::loadFromFile (Mesh.cu:725-937): a deck written from a generated
+    mesh loads back to the same nodes, elements, mid-side flags and borders."""
+    m = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8, order=order)
+    p = os.path.join(str(tmp_path), "plate.inp")
+    write_inp(p, m, "S8R" if order == 2 else "S4")
+    g = A.load_inp(p)
+    assert g.npe == m.npe and np.array_equal(g.conn, m.conn)
+    assert np.array_equal(g.x, m.x) and np.array_equal(g.y, m.y)
+    assert len(g.borders) == len(m.borders)
+    for a, b in zip(g.borders, m.borders):
+        assert np.array_equal(a, b)
+    if order == 2:
+        assert np.array_equal(g.sec, m.sec)
+    # the load vector built on the loaded mesh equals the generated one's
+    la = A.Loads(g).force(1, 170.0, 0.0).force(3, -170.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    lb = A.Loads(m).force(1, 170.0, 0.0).force(3, -170.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    assert np.array_equal(la.rp, lb.rp) and np.array_equal(la.mask, lb.mask)
+
+
+def test_inp_loader_errors(tmp_path):
+    with pytest.raises(A.CPError) as e:
+        A.load_inp(os.path.join(str(tmp_path), "mesh.msh"))
+    assert "Unknown mesh file format" in str(e.value)
+    with pytest.raises(A.CPError) as e:
+        A.load_inp(os.path.join(str(tmp_path), "missing.inp"))
+    assert "There is no such file" in str(e.value)
+    # an element through a border's reference node (an id past the mesh nodes)
+    m = A.gen_rect(0, 1, 0, 1, 2, 2)
+    bad = A.Mesh(m.x, m.y, m.conn.copy(), m.borders)
+    bad.conn[0, 2] = m.nn + 1
+    p = os.path.join(str(tmp_path), "bad.inp")
+    write_inp(p, bad, "S4")
+    with pytest.raises(A.CPError) as e:
+        A.load_inp(p)
+    assert "out of range" in str(e.value)
+
+
+def test_body_force_8node_matches_oracle_and_closed_form():
+    m = A.gen_ring(1, 2, 12, 3, order=2)
+    o = O.gen_ring(1, 2, 12, 3, order=2)
+    la = A.Loads(m, h=0.5).gravity(9.81, 7.8)
+    lo = O.Problem(o, h=0.5).gravity(9.81, 7.8)
+    assert np.array_equal(la.rp, lo.rp)
+    area = sum(O.elem_space(o, e) for e in range(o.ne))
+    assert abs(la.rp[1::2].sum() + 7.8 * 0.5 * 9.81 * area) <= 1e-12 * 7.8 * 0.5 * 9.81 * area
+    assert la.rp[0::2].max() == 0.0 and la.rp[0::2].min() == 0.0
+    # consistent loads of one element: corners -1/12, mid-sides 1/3 of its weight
+    one = A.Mesh(m.x, m.y, m.conn[:1].ravel(), [], npe=8, sec=m.sec)
+    l1 = A.Loads(one).gravity(1.0, 1.0)
+    w = O.elem_space(o, 0)
+    np.testing.assert_allclose(l1.rp[2 * m.conn[0, :4] + 1], w / 12, rtol=1e-14)
+    np.testing.assert_allclose(l1.rp[2 * m.conn[0, 4:] + 1], -w / 3, rtol=1e-14)
