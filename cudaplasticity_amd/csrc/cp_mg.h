@@ -86,7 +86,9 @@ __host__ __device__ inline int mg_slot(int d1, int d2)
 
 template <typename T> struct ChebCoef { T c0, a1, b1, a2, b2; };
 
-// Degree-3 Chebyshev coefficients on [lam/30, 1.1 lam].
+// Coefficients of the first three steps of the Chebyshev three-term
+// recurrence on [lam/30, 1.1 lam]; step k's coefficients do not depend on
+// the degree, so degree 2 uses c0, a1, b1 only.
 template <typename T>
 __device__ __forceinline__ ChebCoef<T> mg_cheb(double lam)
 {
