@@ -1138,20 +1138,50 @@ __global__ __launch_bounds__(kBlock) void k_mg_cg_dots(CgArgs<T> a, const typena
 
 // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s (own node only).
 template <typename T>
+// The updated residual's (r, r) goes to per-block partials (partials_alt),
+// so that k_mg_cg_check can stop the solve before its V-cycle.
 __global__ __launch_bounds__(kBlock) void k_mg_cg_update(CgArgs<T> a, const typename V2<T>::type* u)
 {
     using V = typename V2<T>::type;
     if (a.sc->done) return;
-    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
-    if (n >= a.nn) return;
-    const T alpha = (T)a.sc->alpha, beta = (T)a.sc->beta;
-    const V un = u[n], wn = a.w[0][n], pp = a.p[n], sp = a.s[0][n], xn = a.x[n], rn = a.r[0][n];
-    const V p = V{un.x + beta * pp.x, un.y + beta * pp.y};
-    const V s = V{wn.x + beta * sp.x, wn.y + beta * sp.y};
-    a.p[n] = p;
-    a.s[0][n] = s;
-    a.x[n] = V{xn.x + alpha * p.x, xn.y + alpha * p.y};
-    a.r[0][n] = V{rn.x - alpha * s.x, rn.y - alpha * s.y};
+    const int lb = xcd_block(a.nblocks);
+    const int n = lb * kBlock + threadIdx.x;
+    double v[1] = {0.};
+    if (n < a.nn) {
+        const T alpha = (T)a.sc->alpha, beta = (T)a.sc->beta;
+        const V un = u[n], wn = a.w[0][n], pp = a.p[n], sp = a.s[0][n], xn = a.x[n], rn = a.r[0][n];
+        const V p = V{un.x + beta * pp.x, un.y + beta * pp.y};
+        const V s = V{wn.x + beta * sp.x, wn.y + beta * sp.y};
+        a.p[n] = p;
+        a.s[0][n] = s;
+        a.x[n] = V{xn.x + alpha * p.x, xn.y + alpha * p.y};
+        const V r = V{rn.x - alpha * s.x, rn.y - alpha * s.y};
+        a.r[0][n] = r;
+        v[0] = (double)r.x * r.x + (double)r.y * r.y;
+    }
+    block_partial<1>(v, a.partials_alt, lb);
+}
+
+// Stop test of the iteration on the updated residual, before the V-cycle:
+// a solve that has converged skips its last preconditioner application
+// (the V-cycle, dots and step kernels return at once once done is set).
+// The (r, r) summed here in block order equals the one the dots kernel
+// would sum, so the stop decision is the one the step would take.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_cg_check(CgArgs<T> a)
+{
+    if (a.sc->done) return;
+    double v[1];
+    reduce_partials<1>(a.partials_alt, a.nblocks, v);
+    if (threadIdx.x == 0) {
+        CgScal* sc = a.sc;
+        const double rho = v[0];
+        if (rho == rho && !(rho * (1. / sc->bnorm2) > sc->tol2)) {
+            sc->iters += 1;
+            sc->rho = rho;
+            sc->done = 1;
+        }
+    }
 }
 
 // Zero p and s before the first update (beta = 0 multiplies them).

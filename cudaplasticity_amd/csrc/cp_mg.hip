@@ -475,7 +475,8 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
 }
 
 // One iteration of CG preconditioned by the V-cycle: p, s, x, r updates,
-// u = M r, w = A u with the three dot products, scalars of the next step.
+// the stop test on the new r, u = M r, w = A u with the three dot products,
+// scalars of the next step.
 template <typename T>
 static void mg_cg_iteration(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc,
                             const typename cpk::V2<T>::type* u)
@@ -483,6 +484,7 @@ static void mg_cg_iteration(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal*
     hipStream_t st = P->stream;
     const dim3 g(P->nbn), b(cpk::kBlock);
     hipLaunchKernelGGL(cpk::k_mg_cg_update<T>, g, b, 0, st, a, u);
+    hipLaunchKernelGGL(cpk::k_mg_cg_check<T>, dim3(1), b, 0, st, a);
     mg_vcycle<T>(P, a.r[0], dsc);
     by_width<T>(P->max_w, [&](auto w) {
         hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
