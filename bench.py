@@ -41,8 +41,10 @@ NSTEPS = 20
 # 256 MiB device copy (FETCH_SIZE x 2.000, WRITE_SIZE x 1.000):
 # the fused block-Jacobi CG iteration (--precond 1) 48.19 MB read + 8.05 MB
 # written; the finest level's segment smoothing step (multigrid, the
-# default) 44.93 MB read + 3.21 MB written.
-MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6}
+# default) 44.93 MB read + 3.21 MB written; on the 2000x1000 plate
+# (HBM-resident) the same step 475.8 MB read + 33.0 MB written.
+MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6,
+                    (2000, 1000, "mg_smooth_seg"): 508.8e6}
 
 
 def log(*a):
@@ -283,7 +285,8 @@ def other_workload(A, args):
             "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
                            "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
             "roofline": {"bound": "hbm", "achieved": byt / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": byt / ms_s / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                         "frac": byt / ms_s / 1e6 / HBM_PEAK_GBS,
+                         "traffic": MEASURED_TRAFFIC.get((nt, nr, smooth_kernel(Pm)[1])),
                          "kernel": smooth_kernel(Pm)[0] + " (finest-level Chebyshev smoothing step)",
                          "bytes_per_launch": byt, "avg_launch_ms": ms_s, "format_bytes_per_launch": fmt,
                          "achieved_format": fmt / ms_s / 1e6},
