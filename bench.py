@@ -22,6 +22,7 @@ increments, the roofline of the dominant kernel, a served stream of
 independent problems (p50/p99 latency) and the CPU oracle.
 """
 import argparse
+import hashlib
 import json
 import os
 import sys
@@ -179,6 +180,52 @@ def cpu_baseline(nt, nr, per_step, cores, what="the reference algorithm (FP64, u
                        f"1..{len(per_step)}: {secs:.1f} s projected")}
 
 
+def digest(*arrays):
+    """sha256 over the arrays' bytes (-0.0 read as 0.0): equal digests are
+    bitwise-equal results."""
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update((np.ascontiguousarray(a, dtype=np.float64) + 0.0).tobytes())
+    return h.hexdigest()
+
+
+def solve_alone(A, ctx, stream, opt, nsteps):
+    """Every problem of the stream solved alone, back to back: total seconds
+    and the digest of each result (u, sigma_i)."""
+    from cudaplasticity_amd import serving
+    t0 = time.perf_counter()
+    sums = []
+    for k, (mesh, loads, mat, desc) in enumerate(stream):
+        P = A.Problem(ctx, mesh, loads, mat)
+        if opt.precond == 2:
+            P.set_grid(*serving.lattice(desc))
+        ts = time.perf_counter()
+        per = P.solve_steps(nsteps, 1.0, opt)
+        log(f"  alone {k + 1}/{len(stream)}: {desc['kind']} {desc['quads']} quads, secant "
+            f"{sum(s.outer_iters for s in per)}, cg {sum(s.cg_iters for s in per)}, {time.perf_counter() - ts:.3f} s")
+        sums.append(digest(P.field("u"), P.field("sigI")))
+        P.close()
+    return time.perf_counter() - t0, sums
+
+
+def serve_once(A, ctx, args, stream, opt, rate, sums):
+    """The stream served at Poisson rate `rate`; latency percentiles,
+    throughput and the bitwise check of every result against `sums`."""
+    from cudaplasticity_amd import serving
+    budget = int(args.serve_budget_gb * (1 << 30))
+    recs = serving.serve(ctx, stream, args.serve_steps, opt, rate, budget, max_slots=args.serve_slots, log=log)
+    ok = all(r["code"] == A.CP_OK for r in recs)
+    same = ok and all(digest(r["u"], r["sig"]) == d for r, d in zip(recs, sums))
+    lat = np.array([r["latency"] for r in recs])
+    queue = np.array([r["queue"] for r in recs])
+    span = max(r["t_done"] for r in recs)
+    return {"arrival_rate_per_s": rate, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "p99_queue_s": float(np.percentile(queue, 99)),
+            "throughput_problems_per_s": len(stream) / span,
+            "throughput_load_steps_per_s": len(stream) * args.serve_steps / span,
+            "all_ok": ok, "results_equal_alone_bitwise": bool(same)}
+
+
 def served_stream(A, ctx, args, precond=2, n=None, qmax=None):
     """Reduced config-5 stream: served with Poisson arrivals at `serve_load`
     times the rate the same problems are solved back to back one at a time,
@@ -189,34 +236,59 @@ def served_stream(A, ctx, args, precond=2, n=None, qmax=None):
     n = args.serve_n if n is None else n
     stream = serving.make_stream(n, qmin=2e3, qmax=args.serve_qmax if qmax is None else qmax)
     opt = solver_options(A, args, A.MIXED, precond=precond)
-    t0 = time.perf_counter()
-    alone = []
-    for mesh, loads, mat, desc in stream:
-        P = A.Problem(ctx, mesh, loads, mat)
-        if precond == 2:
-            P.set_grid(*serving.lattice(desc))
-        ts = time.perf_counter()
-        per = P.solve_steps(args.serve_steps, 1.0, opt)
-        log(f"  alone: {desc['kind']} {desc['quads']} quads, secant {sum(s.outer_iters for s in per)}, "
-            f"cg {sum(s.cg_iters for s in per)}, {time.perf_counter() - ts:.3f} s")
-        alone.append((P.field("u"), P.field("sigI")))
-        P.close()
-    t_seq = time.perf_counter() - t0
+    t_seq, sums = solve_alone(A, ctx, stream, opt, args.serve_steps)
     rate = args.serve_load * len(stream) / t_seq
     log(f"served stream: {len(stream)} problems solved back to back in {t_seq:.2f} s; serving at {rate:.2f}/s")
-    budget = int(args.serve_budget_gb * (1 << 30))
-    recs = serving.serve(ctx, stream, args.serve_steps, opt, rate, budget, max_slots=args.serve_slots, log=log)
-    ok = all(r["code"] == A.CP_OK for r in recs)
-    same = ok and all(np.array_equal(r["u"], a[0]) and np.array_equal(r["sig"], a[1]) for r, a in zip(recs, alone))
-    lat = np.array([r["latency"] for r in recs])
-    span = max(r["t_done"] for r in recs)
-    return {"problems": len(stream), "load_steps_each": args.serve_steps,
-            "quads_range": [int(min(s[3]["quads"] for s in stream)), int(max(s[3]["quads"] for s in stream))],
-            "arrival_rate_per_s": rate, "p50_latency_s": float(np.percentile(lat, 50)),
-            "p99_latency_s": float(np.percentile(lat, 99)), "throughput_problems_per_s": len(stream) / span,
-            "throughput_load_steps_per_s": len(stream) * args.serve_steps / span,
-            "sequential_alone_load_steps_per_s": len(stream) * args.serve_steps / t_seq,
-            "all_ok": ok, "results_equal_alone_bitwise": bool(same)}
+    rec = serve_once(A, ctx, args, stream, opt, rate, sums)
+    rec.pop("p99_queue_s")
+    out = {"problems": len(stream), "load_steps_each": args.serve_steps,
+           "quads_range": [int(min(s[3]["quads"] for s in stream)), int(max(s[3]["quads"] for s in stream))]}
+    out.update(rec)
+    out["sequential_alone_load_steps_per_s"] = len(stream) * args.serve_steps / t_seq
+    return out
+
+
+def c5_workload(A, args):
+    """Config 5 at full size: a stream of 512 problems (plate or graded
+    ring, 2k-500k quads log-uniform, random hole radius, yield and hardening
+    law, 5 increments each, mixed precision, multigrid CG), solved alone back
+    to back, then served at Poisson arrival rates swept over multiples of the
+    back-to-back rate.  `value` is the served throughput at the highest
+    rate; every served result of every rate is checked bitwise against the
+    problem solved alone."""
+    from cudaplasticity_amd import serving
+    ctx = A.Context(0)
+    n = args.c5_n
+    stream = serving.make_stream(n, qmin=2e3, qmax=args.serve_qmax)
+    quads = np.array([s[3]["quads"] for s in stream])
+    log(f"stream of {n} problems: {int(quads.sum())} quads in all, {quads.min()}-{quads.max()} each")
+    opt = solver_options(A, args, A.MIXED, precond=2)
+    t_seq, sums = solve_alone(A, ctx, stream, opt, args.serve_steps)
+    base = n / t_seq
+    log(f"back to back: {t_seq:.2f} s, {base:.2f} problems/s")
+    sweep = []
+    for load in args.serve_loads:
+        log(f"serving at {load:g} x the back-to-back rate ({load * base:.2f}/s)")
+        rec = serve_once(A, ctx, args, stream, opt, load * base, sums)
+        rec["load"] = load
+        log("  " + json.dumps(rec))
+        sweep.append(rec)
+    top = sweep[-1]
+    ok = all(r["all_ok"] and r["results_equal_alone_bitwise"] for r in sweep)
+    line = {"metric": METRIC, "value": top["throughput_load_steps_per_s"], "unit": "load-steps/s", "n_gpus": 1,
+            "steps": n * args.serve_steps, "warmup": 0, "ms_per_step": 1e3 / top["throughput_load_steps_per_s"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+            "config": {"workload": (f"served stream of {n} problems, plate with hole or graded ring uniformly, "
+                                    f"2k-{args.serve_qmax:.0f} quads log-uniform, hole radius U(0.05,0.3), yield "
+                                    f"U(200,400) MPa, linear or Swift hardening, {args.serve_steps} increments each, "
+                                    f"Poisson arrivals; mixed precision, multigrid CG"),
+                       "problems": n, "quads_total": int(quads.sum()),
+                       "quads_range": [int(quads.min()), int(quads.max())], "max_slots": args.serve_slots,
+                       "hbm_budget_gb": args.serve_budget_gb},
+            "sequential_alone_load_steps_per_s": n * args.serve_steps / t_seq,
+            "sweep": sweep, "all_results_equal_alone_bitwise": bool(ok),
+            "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
 
 
 def other_workload(A, args):
@@ -365,10 +437,10 @@ def q8_workload(A, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "q8"),
+    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5", "q8"),
                     help="c2: the bench line (632x316 plate); c3: 2000x1000 plate, Swift law, 40 increments; "
-                         "c4: 16700x60 graded quarter ring, 30 increments; q8: 8-node quads on the reference's "
-                         "energyConvTest plate")
+                         "c4: 16700x60 graded quarter ring, 30 increments; c5: served stream of --c5-n problems "
+                         "at swept arrival rates; q8: 8-node quads on the reference's energyConvTest plate")
     ap.add_argument("--q8-k", type=int, default=64,
                     help="q8 mesh: genRectWithHole(-3, 3, -2, 2, 0.5, 3k, 2k, order 2)")
     ap.add_argument("--steps", type=int, default=NSTEPS)
@@ -391,6 +463,9 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=5)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-load", type=float, default=1.0)
+    ap.add_argument("--c5-n", type=int, default=512, help="--workload c5: problems in the stream")
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0],
+                    help="--workload c5: arrival rates swept, as multiples of the back-to-back rate")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--reps", type=int, default=400, help="launches of the CG iteration kernel for the roofline")
     ap.add_argument("--copy-calibration", type=float, default=0.0,
@@ -414,6 +489,10 @@ def main():
     if args.workload == "q8":
         import cudaplasticity_amd.abi as A
         q8_workload(A, args)
+        return
+    if args.workload == "c5":
+        from cudaplasticity_amd import abi as A
+        c5_workload(A, args)
         return
     if args.workload != "c2":
         from cudaplasticity_amd import abi as A
