@@ -464,7 +464,7 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-load", type=float, default=1.0)
     ap.add_argument("--c5-n", type=int, default=512, help="--workload c5: problems in the stream")
-    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0],
+    ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0, 4.0],
                     help="--workload c5: arrival rates swept, as multiples of the back-to-back rate")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
     ap.add_argument("--reps", type=int, default=400, help="launches of the CG iteration kernel for the roofline")

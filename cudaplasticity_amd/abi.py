@@ -468,6 +468,7 @@ class Server:
         self.ptr = C.c_void_p()
         check(lib().cp_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)))
         self.sizes = {}
+        self.last_message = ""   # message of the last failed result()
 
     def submit(self, mesh, loads, mat, nsteps, final_factor, opt, grid=None, node_of=None):
         """grid = (n1, n2): the mesh's lattice, needed for multigrid CG (precond 2)."""
@@ -502,6 +503,7 @@ class Server:
         sig = np.zeros(ne) if want_fields else None
         code = lib().cp_server_result(self.ptr, C.c_longlong(rid), C.byref(info),
                                       dp(u) if want_fields else None, dp(sig) if want_fields else None)
+        self.last_message = lib().cp_last_error().decode() if code != CP_OK else ""
         return code, info, u, sig
 
     def run_until_done(self, max_rounds=10 ** 7):

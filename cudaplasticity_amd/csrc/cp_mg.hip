@@ -543,6 +543,18 @@ static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, co
     return CP_OK;
 }
 
+int mg_prepare(cp_problem* P, int precision)
+{
+    if (!P->mg) return fail(CP_ERR_STATE, "multigrid needs the mesh lattice: call cp_problem_set_grid first");
+    if (int s = mg_alloc(P, precision)) return s;
+    if (int s = precision == CP_FP64 ? mg_tail_table<double>(P) : mg_tail_table<float>(P)) return s;
+    if (!P->mg->use_graphs) return CP_OK;
+    hipGraphExec_t start = nullptr, iter = nullptr;
+    if (precision == CP_FP64)
+        return mg_graph<double>(P, P->cg64_args(), P->sc64, (const double2*)P->mg->L[0].vec[0][2], &start, &iter);
+    return mg_graph<float>(P, P->cg32_args(), P->sc32, (const float2*)P->mg->L[0].vec[1][2], &start, &iter);
+}
+
 template <typename T>
 static int mg_start_t(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, hipGraphExec_t* exec)
 {

@@ -145,6 +145,11 @@ int run_mg_cg(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, double t
 // (the serving loop): enqueue the start (after enqueue_cg_init), enqueue n
 // iterations, the predicted first chunk, and the bookkeeping once done.
 int mg_cg_enqueue_start(cp_problem* P, int precision);
+// The buffers of `precision` and the two CG graphs (start, iteration) ahead
+// of the first solve, so that no later call captures a graph (the serving
+// loop prepares problems on a worker thread, where a capture cannot overlap
+// another thread's device calls).
+int mg_prepare(cp_problem* P, int precision);
 int mg_cg_enqueue_iterations(cp_problem* P, int precision, int n);
 int mg_cg_first_chunk(const cp_problem* P, double tol2, bool zero_start);
 // Upper bound of the device memory cp_problem_set_grid and the multigrid

@@ -6,23 +6,34 @@
 // device and advances through a small state machine
 //   level start -> assemble -> CG (or FP64 residual / FP32 CG refinement)
 //   -> return map -> next secant iteration / next level / done.
-// One scheduling round enqueues every request's next device work, then one
-// chunk of batched CG launches: a single launch advances one CG iteration of
-// every problem in its CG phase (tiles of all problems in one grid), each
-// problem with its own scalars and partial sums, so a result does not depend
-// on what else is in the batch.  One stream, one host synchronisation per
-// round; requests that arrive while others run are admitted at the next
-// round as slots and memory free up.
+// Block-Jacobi requests share the server's stream and advance in rounds: one
+// round enqueues every such request's next device work, then one chunk of
+// batched CG launches (a single launch advances one CG iteration of every
+// problem in its CG phase, tiles of all problems in one grid, each problem
+// with its own scalars and partial sums, so a result does not depend on what
+// else is in the batch), then one host synchronisation.  Multigrid requests
+// work on streams of their own and advance independently: each one's device
+// work ends with a readback of its scalars into a pinned host slot and an
+// event, and every step advances the requests whose event has completed, so
+// a small problem never waits for a large one's V-cycles.  Requests that
+// arrive while others run are admitted (FIFO) at the next step as slots and
+// memory free up; a worker thread builds an admitted request's device problem
+// (CSR pattern, upload, multigrid lattice) while the steps go on advancing
+// the others.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstddef>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "cp_problem.h"
@@ -47,7 +58,9 @@ struct Request {
     size_t bytes = 0;
     State state = RQ_QUEUED;
     cp_problem* P = nullptr;
-    hipEvent_t ev = nullptr;         // multigrid requests: end of this round's work on their own stream
+    hipEvent_t ev = nullptr;         // multigrid requests: their scalars' readback has landed
+    int slot = -1;                   // multigrid requests: mirror slot of the readback
+    bool pending = false;            // multigrid requests: a readback is in flight
     int phase = PH_LEVEL;
     int level = 0, k = 0, tight = 1;
     double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1., itol = 0., rn = 0.;
@@ -55,6 +68,8 @@ struct Request {
     long long cg_iters = 0;
     int outer_total = 0;
     int status = CP_OK;
+    int prep_status = CP_OK;         // result of the worker's preparation
+    std::string err;                 // message of the failure that ended the request
     double t_submit = 0., t_admit = -1., t_done = -1.;
     double cg_rho_prev = -1.;  // CG residual at the last observation
     int cg_iters_seen = 0;      // CG iteration count at the last observation
@@ -83,6 +98,19 @@ __global__ void k_pack(const PackItem* items, int n, char* dst)
     for (int b = threadIdx.x; b < it.nbytes; b += blockDim.x) dst[it.dst_off + b] = it.src[b];
 }
 
+// The three scalar blocks of one request into its mirror slot (grid of 3).
+struct Pack3 {
+    PackItem it[3];
+};
+
+__global__ void k_pack3(Pack3 p, char* dst)
+{
+    const PackItem it = p.it[blockIdx.x];
+    for (int b = threadIdx.x; b < it.nbytes; b += blockDim.x) dst[it.dst_off + b] = it.src[b];
+}
+
+constexpr size_t kMirStride = (sizeof(Mirror) + 255) / 256 * 256;
+
 double clock_now()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -100,6 +128,26 @@ struct cp_server {
     std::map<long long, std::unique_ptr<Request>> reqs;
     std::deque<Request*> queue;
     std::vector<Request*> active;
+    // preparation of admitted requests on the worker thread: to_prep and
+    // prepared are shared with it under mu; `preparing` counts both plus the
+    // one in the worker's hands (main thread only)
+    std::thread worker;
+    // held by every step and by the worker's graph captures: a capture must
+    // not overlap another thread's device calls (a synchronous copy, an
+    // allocation, a free), which would invalidate it
+    std::mutex step_mu;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request*> to_prep, prepared;
+    bool stop = false;
+    int preparing = 0;
+    // finished problems whose memory is not yet returned (a free may wait for
+    // the whole device, so frees are batched): problem and budgeted bytes
+    std::vector<std::pair<cp_problem*, size_t>> grave;
+    // readback slots of the multigrid requests, one per admission slot
+    char* mir_dev = nullptr;
+    char* mir_host = nullptr;
+    std::vector<int> free_slots;
     // device tables for the batched CG and the scalar pack
     char* dev = nullptr;
     size_t dev_size = 0;
@@ -124,21 +172,37 @@ int ensure_buffers(cp_server* S, size_t bytes)
     return CP_OK;
 }
 
+// Return the memory of every finished problem to the device and the budget.
+void flush_grave(cp_server* S)
+{
+    for (auto& g : S->grave) {
+        cp_problem_destroy(g.first);
+        S->used -= g.second;
+    }
+    S->grave.clear();
+}
+
 void release(cp_server* S, Request* r)
 {
     if (r->P) {
-        S->used -= r->bytes;
-        cp_problem_destroy(r->P);
+        S->grave.emplace_back(r->P, r->bytes);
         r->P = nullptr;
     }
     if (r->ev) {
         (void)hipEventDestroy(r->ev);
         r->ev = nullptr;
     }
+    if (r->slot >= 0) {
+        S->free_slots.push_back(r->slot);
+        r->slot = -1;
+    }
+    r->pending = false;
+    if ((int)S->grave.size() >= S->cfg.max_slots) flush_grave(S);
 }
 
 int fail_request(cp_server* S, Request* r, int code)
 {
+    if (r->err.empty()) r->err = cp_last_error();
     r->status = code;
     r->state = RQ_FAILED;
     r->t_done = S->now();
@@ -210,7 +274,7 @@ int advance(cp_server* S, Request* r, const Mirror& m)
         const cpk::CgScal& c = m.cg;
         if (!c.done) return CP_OK;
         r->cg_iters += c.iters;
-        if (c.status == 2) return fail_request(S, r, CP_ERR_NAN);
+        if (c.status == 2) return fail_request(S, r, cp::fail(CP_ERR_NAN, "CG: NaN in the residual"));
         P->last_cg_iters = c.iters;
         if (P->precond == 2) cp::mg_cg_observe(P, c);
         if (r->opt.precision == CP_FP64) {
@@ -237,7 +301,7 @@ int advance(cp_server* S, Request* r, const Mirror& m)
     }
     case PH_WAIT_RES: {
         const double rn = m.ref.rnorm2;
-        if (rn != rn) return fail_request(S, r, CP_ERR_NAN);
+        if (rn != rn) return fail_request(S, r, cp::fail(CP_ERR_NAN, "refinement: NaN in the residual"));
         if (r->ref_norm == 0.) r->ref_norm = rn;
         if (rn == 0. || rn <= r->ctol * r->ctol * r->ref_norm || r->stalled >= 3) {
             if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
@@ -266,13 +330,13 @@ int advance(cp_server* S, Request* r, const Mirror& m)
     case PH_WAIT_RM: {
         const cpk::RmScal& h = m.rm;
         ++r->outer_total;
-        if (h.status == 2) return fail_request(S, r, CP_ERR_NAN);
+        if (h.status == 2) return fail_request(S, r, cp::fail(CP_ERR_NAN, "return map: NaN in the secant update"));
         if (!h.last) {
             ++r->k;
             r->prev_rel = h.rel_err <= r->opt.tol ? 0. : h.rel_err;
             return start_iteration(r);
         }
-        if (!h.converged) return fail_request(S, r, CP_ERR_NOT_CONVERGED);
+        if (!h.converged) return fail_request(S, r, cp::fail(CP_ERR_NOT_CONVERGED, "secant iterations did not converge"));
         // end of a level, as in cp_solve_steps: plastic strains, then the next level
         if (int s = cp::enqueue_plastic(P)) return s;
         P->solved = true;
@@ -288,49 +352,167 @@ int advance(cp_server* S, Request* r, const Mirror& m)
     }
 }
 
+// Remaining-iteration estimate of a request in its CG phase from the
+// observed decay of (r, r) since the last observation; `chunk` is the
+// iteration count of the batched round that just ran (block-Jacobi).
+void update_chunk(Request* r, const cpk::CgScal& g, int chunk)
+{
+    const bool mg = r->opt.precond == 2;
+    int next = mg ? std::min(64, 2 * r->next_chunk) : std::min(1024, 2 * chunk);
+    const double r0 = r->cg_rho_prev > 0. ? r->cg_rho_prev : (mg ? g.rho0 : -1.);
+    if (r0 > 0. && g.rho > 0. && g.rho < r0 && g.iters > r->cg_iters_seen) {
+        const double target = g.tol2 * g.bnorm2;
+        const double rate = std::log(r0 / g.rho) / double(g.iters - r->cg_iters_seen);
+        const double rem = std::log(g.rho / target) / rate;
+        if (rem > 0 && rem < 1e7)
+            next = mg ? (int)std::min(64., std::max(1., std::ceil(rem)))
+                      : (int)std::min(1024., std::max(4., rem * 1.05 + 2.));
+    }
+    r->cg_rho_prev = g.rho;
+    r->cg_iters_seen = g.iters;
+    r->next_chunk = next;
+}
+
+// Multigrid request: its CG iterations if it is in the CG phase, then the
+// readback of its scalars into its mirror slot and the event that says so.
+int enqueue_mg_round(cp_server* S, Request* r)
+{
+    cp_problem* P = r->P;
+    if (r->phase == PH_CG)
+        if (int s = cp::mg_cg_enqueue_iterations(P, r->opt.precision, r->next_chunk)) return s;
+    Pack3 pk;
+    pk.it[0] = PackItem{(const char*)(r->opt.precision == CP_FP64 ? (const void*)P->sc64 : (const void*)P->sc32),
+                        (int)offsetof(Mirror, cg), (int)sizeof(cpk::CgScal)};
+    pk.it[1] = PackItem{(const char*)P->rms, (int)offsetof(Mirror, rm), (int)sizeof(cpk::RmScal)};
+    pk.it[2] = PackItem{(const char*)P->refs, (int)offsetof(Mirror, ref), (int)sizeof(cpk::RefScal)};
+    char* d = S->mir_dev + (size_t)r->slot * kMirStride;
+    hipLaunchKernelGGL(k_pack3, dim3(3), dim3(64), 0, P->stream, pk, d);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipMemcpyAsync(S->mir_host + (size_t)r->slot * kMirStride, d, sizeof(Mirror), hipMemcpyDeviceToHost,
+                          P->stream));
+    CP_HIP(hipEventRecord(r->ev, P->stream));
+    r->pending = true;
+    return CP_OK;
+}
+
+// A request whose work has finished: its results to the host, its memory to
+// the deferred frees.
+void finish(cp_server* S, Request* r)
+{
+    r->u_out.resize(2 * (size_t)r->nn);
+    r->sig_out.resize(r->ne);
+    if (cp_download(r->P, CP_F_U, r->u_out.data()) || cp_download(r->P, CP_F_SIGI, r->sig_out.data())) {
+        fail_request(S, r, CP_ERR_DEVICE);
+    } else {
+        r->state = RQ_DONE;
+        r->t_done = S->now();
+        release(S, r);
+    }
+}
+
+// Worker thread: the device problem of an admitted request (host CSR
+// pattern and incidences, upload, solver vectors, multigrid lattice and
+// hierarchy); the request's host copies of its mesh are dropped after.
+void prepare(cp_server* S, Request* r)
+{
+    const bool mg = r->opt.precond == 2;
+    cp_problem* P = nullptr;
+    // a multigrid request works on a stream of its own, so the V-cycles of
+    // several requests (small problems leave most of the GPU idle) run side
+    // by side; block-Jacobi requests share the server's stream with the
+    // batched CG kernels
+    int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
+                               r->u0.data(), r->mask.data(), &r->mat, mg ? nullptr : S->stream, &P);
+    if (s == CP_OK && mg && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+        s = cp::fail(CP_ERR_DEVICE, "cp_server: event creation failed");
+    if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
+    if (s == CP_OK && mg) s = cp_problem_set_grid(P, r->n1, r->n2, r->node_of.empty() ? nullptr : r->node_of.data());
+    if (s == CP_OK && mg) {
+        std::lock_guard<std::mutex> lk(S->step_mu);
+        s = cp::mg_prepare(P, r->opt.precision);
+    }
+    if (s != CP_OK && P) {
+        cp_problem_destroy(P);
+        P = nullptr;
+    }
+    r->P = P;
+    r->prep_status = s;
+    if (s != CP_OK) r->err = cp_last_error();
+    std::vector<double>().swap(r->x);
+    std::vector<double>().swap(r->y);
+    std::vector<int>().swap(r->conn);
+}
+
+void worker_main(cp_server* S)
+{
+    (void)hipSetDevice(S->ctx->device);
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [S] { return S->stop || !S->to_prep.empty(); });
+            if (S->stop) return;
+            r = S->to_prep.front();
+            S->to_prep.pop_front();
+        }
+        prepare(S, r);
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->prepared.push_back(r);
+    }
+}
+
 int admit(cp_server* S)
 {
-    while (!S->queue.empty() && (int)S->active.size() < S->cfg.max_slots) {
-        Request* r = S->queue.front();
-        if (r->bytes > S->cfg.hbm_budget) {
-            S->queue.pop_front();
-            fail_request(S, r, CP_ERR_BUDGET);
+    // prepared requests start their first level
+    std::deque<Request*> ready;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        ready.swap(S->prepared);
+    }
+    for (Request* r : ready) {
+        --S->preparing;
+        if (r->prep_status != CP_OK) {
+            S->used -= r->bytes;
+            fail_request(S, r, r->prep_status);
             continue;
         }
-        if (S->used + r->bytes > S->cfg.hbm_budget) break;
-        S->queue.pop_front();
-        cp_problem* P = nullptr;
-        // a multigrid request works on a stream of its own, so the V-cycles
-        // of several requests (small problems leave most of the GPU idle)
-        // run side by side; block-Jacobi requests share the server's stream
-        // with the batched CG kernels
-        const bool mg = r->opt.precond == 2;
-        int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
-                                   r->u0.data(), r->mask.data(), &r->mat, mg ? nullptr : S->stream, &P);
-        if (s == CP_OK && mg && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
-            s = cp::fail(CP_ERR_DEVICE, "cp_server: event creation failed");
-        if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
-        if (s == CP_OK && r->opt.precond == 2)
-            s = cp_problem_set_grid(P, r->n1, r->n2, r->node_of.empty() ? nullptr : r->node_of.data());
-        if (s != CP_OK) {
-            if (P) cp_problem_destroy(P);
-            fail_request(S, r, s);
-            continue;
-        }
-        r->P = P;
         r->state = RQ_ACTIVE;
-        r->t_admit = S->now();
-        S->used += r->bytes;
-        // inputs now live on the device
-        std::vector<double>().swap(r->x);
-        std::vector<double>().swap(r->y);
-        std::vector<int>().swap(r->conn);
         r->level = 1;
-        if (int q = start_level(r)) {
+        int q = start_level(r);
+        if (q == CP_OK && r->ev) q = enqueue_mg_round(S, r);
+        if (q != CP_OK) {
             fail_request(S, r, q);
             continue;
         }
         S->active.push_back(r);
+    }
+    // FIFO: hand the next requests to the worker while a slot and the
+    // budget allow
+    while (!S->queue.empty() && (int)S->active.size() + S->preparing < S->cfg.max_slots) {
+        Request* r = S->queue.front();
+        if (r->bytes > S->cfg.hbm_budget) {
+            S->queue.pop_front();
+            fail_request(S, r, cp::fail(CP_ERR_BUDGET, "cp_server: the request needs more than the HBM budget"));
+            continue;
+        }
+        if (S->used + r->bytes > S->cfg.hbm_budget) {
+            if (S->grave.empty()) break;
+            flush_grave(S);
+            continue;
+        }
+        S->queue.pop_front();
+        S->used += r->bytes;
+        if (r->opt.precond == 2) {
+            r->slot = S->free_slots.back();
+            S->free_slots.pop_back();
+        }
+        r->t_admit = S->now();
+        ++S->preparing;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->to_prep.push_back(r);
+        }
+        S->cv.notify_one();
     }
     return CP_OK;
 }
@@ -382,6 +564,14 @@ int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out)
     S->t0 = clock_now();
     CP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
     if (int s = ensure_buffers(S.get(), 1 << 20)) return s;
+    CP_HIP(hipMalloc(&S->mir_dev, kMirStride * cfg->max_slots));
+    CP_HIP(hipHostMalloc((void**)&S->mir_host, kMirStride * cfg->max_slots, hipHostMallocDefault));
+    for (int i = cfg->max_slots - 1; i >= 0; --i) S->free_slots.push_back(i);
+    try {
+        S->worker = std::thread(worker_main, S.get());
+    } catch (...) {
+        return cp::fail(CP_ERR_DEVICE, "cp_server_create: cannot start the preparation thread");
+    }
     *out = S.release();
     return CP_OK;
 }
@@ -389,13 +579,22 @@ int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out)
 int cp_server_destroy(cp_server* S)
 {
     if (!S) return CP_OK;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
     (void)hipStreamSynchronize(S->stream);
     for (auto& kv : S->reqs) {
-        if (kv.second->P) cp_problem_destroy(kv.second->P);
+        if (kv.second->P) cp_problem_destroy(kv.second->P);   // waits for its stream
         if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
     }
+    flush_grave(S);
     if (S->dev) (void)hipFree(S->dev);
     if (S->host) (void)hipHostFree(S->host);
+    if (S->mir_dev) (void)hipFree(S->mir_dev);
+    if (S->mir_host) (void)hipHostFree(S->mir_host);
     (void)hipStreamDestroy(S->stream);
     delete S;
     return CP_OK;
@@ -464,17 +663,36 @@ int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const d
 int cp_server_step(cp_server* S, int* in_flight)
 {
     if (!S) return cp::fail(CP_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> step_lock(S->step_mu);
     if (int s = admit(S)) return s;
     hipStream_t st = S->stream;
-    std::vector<Request*>& act = S->active;
+    // multigrid requests: advance every one whose last readback has landed
+    for (Request* r : S->active) {
+        if (!r->pending) continue;
+        const hipError_t q = hipEventQuery(r->ev);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) {
+            fail_request(S, r, cp::failf(CP_ERR_DEVICE, "cp_server: event query: %s", hipGetErrorString(q)));
+            continue;
+        }
+        r->pending = false;
+        const Mirror m = *reinterpret_cast<const Mirror*>(S->mir_host + (size_t)r->slot * kMirStride);
+        if (r->phase == PH_CG) update_chunk(r, m.cg, 0);
+        if (int s = advance(S, r, m)) fail_request(S, r, s);
+        if (r->state != RQ_ACTIVE) continue;
+        if (r->phase == PH_DONE) finish(S, r);
+        else if (int s = enqueue_mg_round(S, r)) fail_request(S, r, s);
+    }
+    // block-Jacobi requests: one batched round on the server's stream
+    std::vector<Request*> act;
+    for (Request* r : S->active)
+        if (r->state == RQ_ACTIVE && !r->ev) act.push_back(r);
     if (!act.empty()) {
-        // requests in their CG phase, grouped by (precision, preconditioner);
-        // multigrid requests replay their own iteration graphs instead
-        std::vector<Request*> g64[2], g32[2], gmg;
+        // requests in their CG phase, grouped by (precision, preconditioner)
+        std::vector<Request*> g64[2], g32[2];
         for (Request* r : act) {
             if (r->phase != PH_CG) continue;
-            if (r->opt.precond == 2) gmg.push_back(r);
-            else (r->opt.precision == CP_FP64 ? g64 : g32)[r->opt.precond ? 1 : 0].push_back(r);
+            (r->opt.precision == CP_FP64 ? g64 : g32)[r->opt.precond ? 1 : 0].push_back(r);
         }
         // device tables: [CgArgs groups][tiles][pack items] then the pack output
         size_t off = 0;
@@ -533,7 +751,7 @@ int cp_server_step(cp_server* S, int* in_flight)
         // many as the member closest to convergence is predicted to need
         int chunk = 0;
         for (Request* r : act)
-            if (r->phase == PH_CG && r->opt.precond != 2) chunk = chunk == 0 ? r->next_chunk : std::min(chunk, r->next_chunk);
+            if (r->phase == PH_CG) chunk = chunk == 0 ? r->next_chunk : std::min(chunk, r->next_chunk);
         chunk = std::max(4, chunk == 0 ? 16 : chunk);
         for (const Launch& L : launches) {
             if (S->cfg.batch_cg) {
@@ -552,14 +770,6 @@ int cp_server_step(cp_server* S, int* in_flight)
                 }
             }
         }
-        for (Request* r : gmg)
-            if (int s = cp::mg_cg_enqueue_iterations(r->P, r->opt.precision, r->next_chunk)) return s;
-        // the scalar pack waits for every request working on its own stream
-        for (Request* r : act)
-            if (r->ev) {
-                CP_HIP(hipEventRecord(r->ev, r->P->stream));
-                CP_HIP(hipStreamWaitEvent(st, r->ev, 0));
-            }
         CP_HIP(hipGetLastError());
         hipLaunchKernelGGL(k_pack, dim3(ni), dim3(64), 0, st, reinterpret_cast<const PackItem*>(S->dev + pack_off), ni,
                            S->dev + mir_off);
@@ -569,46 +779,21 @@ int cp_server_step(cp_server* S, int* in_flight)
         const Mirror* mir = reinterpret_cast<const Mirror*>(S->host + mir_off);
 
         // advance every request with this round's scalars
-        std::vector<Request*> keep;
         std::vector<Mirror> mcopy(mir, mir + act.size());
         for (size_t i = 0; i < act.size(); ++i) {
             Request* r = act[i];
-            if (r->phase == PH_CG) {
-                // remaining-iteration estimate from the observed decay of (r, r)
-                const cpk::CgScal& g = mcopy[i].cg;
-                const bool mg = r->opt.precond == 2;
-                int next = mg ? std::min(64, 2 * r->next_chunk) : std::min(1024, 2 * chunk);
-                const double r0 = r->cg_rho_prev > 0. ? r->cg_rho_prev : (mg ? g.rho0 : -1.);
-                if (r0 > 0. && g.rho > 0. && g.rho < r0 && g.iters > r->cg_iters_seen) {
-                    const double target = g.tol2 * g.bnorm2;
-                    const double rate = std::log(r0 / g.rho) / double(g.iters - r->cg_iters_seen);
-                    const double rem = std::log(g.rho / target) / rate;
-                    if (rem > 0 && rem < 1e7)
-                        next = mg ? (int)std::min(64., std::max(1., std::ceil(rem)))
-                                  : (int)std::min(1024., std::max(4., rem * 1.05 + 2.));
-                }
-                r->cg_rho_prev = g.rho;
-                r->cg_iters_seen = g.iters;
-                r->next_chunk = next;
-            }
+            if (r->phase == PH_CG) update_chunk(r, mcopy[i].cg, chunk);
             if (int s = advance(S, r, mcopy[i])) fail_request(S, r, s);
-            if (r->state == RQ_ACTIVE && r->phase == PH_DONE) {
-                CP_HIP(hipStreamSynchronize(st));
-                r->u_out.resize(2 * (size_t)r->nn);
-                r->sig_out.resize(r->ne);
-                if (cp_download(r->P, CP_F_U, r->u_out.data()) || cp_download(r->P, CP_F_SIGI, r->sig_out.data())) {
-                    fail_request(S, r, CP_ERR_DEVICE);
-                } else {
-                    r->state = RQ_DONE;
-                    r->t_done = S->now();
-                    release(S, r);
-                }
-            }
-            if (r->state == RQ_ACTIVE) keep.push_back(r);
+            if (r->state == RQ_ACTIVE && r->phase == PH_DONE) finish(S, r);
         }
-        act.swap(keep);
     }
-    if (in_flight) *in_flight = (int)(S->active.size() + S->queue.size());
+    std::vector<Request*> keep;
+    for (Request* r : S->active)
+        if (r->state == RQ_ACTIVE) keep.push_back(r);
+    S->active.swap(keep);
+    // idle: return the finished problems' memory now
+    if (S->active.empty() && S->queue.empty() && S->preparing == 0 && !S->grave.empty()) flush_grave(S);
+    if (in_flight) *in_flight = (int)(S->active.size() + S->queue.size()) + S->preparing;
     return CP_OK;
 }
 
@@ -628,7 +813,7 @@ int cp_server_result(cp_server* S, long long id, cp_request_info* info, double* 
         info->t_admit = r->t_admit;
         info->t_done = r->t_done;
     }
-    if (r->state == RQ_FAILED) return cp::failf(r->status, "request %lld failed", id);
+    if (r->state == RQ_FAILED) return cp::failf(r->status, "request %lld failed: %s", id, r->err.c_str());
     if (r->state != RQ_DONE) return cp::fail(CP_ERR_STATE, "request not finished");
     if (u) std::memcpy(u, r->u_out.data(), sizeof(double) * r->u_out.size());
     if (sig_i) std::memcpy(sig_i, r->sig_out.data(), sizeof(double) * r->sig_out.size());

@@ -84,7 +84,9 @@ def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=Tru
     recs = []
     for k, rid in enumerate(ids):
         code, info, u, sig = srv.result(rid)
-        recs.append({"code": code, "latency": info.t_done - info.t_submit, "queue": info.t_admit - info.t_submit,
+        if code != A.CP_OK and log:
+            log(f"  request {k} failed: {srv.last_message}")
+        recs.append({"code": code, "message": srv.last_message, "latency": info.t_done - info.t_submit, "queue": info.t_admit - info.t_submit,
                      "t_done": info.t_done - t_start, "cg_iters": info.cg_iters, "outer": info.outer_iters,
                      "u": u, "sig": sig})
     srv.close()

@@ -112,6 +112,36 @@ def test_served_multigrid_equal_alone(ctx):
     srv.close()
 
 
+def test_served_stream_overlapping_preparation(ctx):
+    """A stream of 24 multigrid requests (plates and graded rings of
+    2k-40k quads, both hardening laws, FP64 and mixed) submitted at once to
+    6 slots: a worker thread prepares the next requests (pattern, upload,
+    lattice, graph capture) while the others run, and each request advances
+    as soon as its own readback lands.  Every result equals the problem
+    solved alone, bit for bit, and failures carry their message."""
+    from cudaplasticity_amd import serving
+    stream = serving.make_stream(24, seed=99, qmin=2e3, qmax=4e4)
+    alone = []
+    for k, (mesh, loads, mat, desc) in enumerate(stream):
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(*serving.lattice(desc))
+        P.solve_steps(3, 1.0, mg_opts(A.FP64 if k % 3 == 0 else A.MIXED))
+        alone.append((P.field("u"), P.field("sigI")))
+        P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 34, max_slots=6)
+    ids = [srv.submit(m, l, mat, 3, 1.0, mg_opts(A.FP64 if k % 3 == 0 else A.MIXED), grid=serving.lattice(d))
+           for k, (m, l, mat, d) in enumerate(stream)]
+    srv.run_until_done()
+    for i, rid in enumerate(ids):
+        code, info, u, sig = srv.result(rid)
+        assert code == A.CP_OK, (i, srv.last_message)
+        assert np.array_equal(u, alone[i][0]) and np.array_equal(sig, alone[i][1]), i
+        assert info.t_submit <= info.t_admit <= info.t_done
+    print("served", len(ids), "requests; last done at %.3f s" % max(srv.result(r, False)[1].t_done for r in ids),
+          flush=True)
+    srv.close()
+
+
 def test_budget_and_slots(ctx):
     (m, l, mat), _, _ = CASES[0]
     import ctypes as C
