@@ -351,6 +351,12 @@ int cp_problem_layout(cp_problem* prob, long long* nblk_stored, long long* nblk,
 /* ------------------------------------------------------------------ */
 /* serving: continuous batching of independent problems (new)          */
 /* ------------------------------------------------------------------ */
+/* Call the cp_server_* functions of one server from one thread.  The server
+ * owns a worker thread of its own that prepares admitted requests (pattern,
+ * upload, multigrid lattice and graphs); cp_server_destroy stops it.
+ * Multigrid requests advance asynchronously, each on a stream of its own;
+ * block-Jacobi requests advance in batched rounds on the server's stream.
+ * A request's results do not depend on what else is served. */
 typedef struct {
     size_t hbm_budget;    /* bytes of device memory the server may hold for problems */
     int    max_slots;     /* problems resident at once */
@@ -368,6 +374,7 @@ typedef struct {
 } cp_request_info;
 
 int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out);
+/* Waits for the device work of every request, then frees everything. */
 int cp_server_destroy(cp_server* srv);
 /* Submit one problem: mesh + loads + material + nsteps load levels to
  * final_factor with options (precond 0 or 1); inputs are copied. */
@@ -382,10 +389,17 @@ int cp_server_submit_grid(cp_server* srv, int nn, int ne, const double* x, const
                           const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
                           int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
                           long long* id);
-/* Advance the server: admit queued requests that fit, run one scheduling
- * round of device work.  Returns the number of requests still in flight. */
+/* Advance the server without waiting for multigrid work: start the
+ * requests the worker has prepared, admit queued requests that fit (FIFO,
+ * HBM budget, slots), advance every multigrid request whose last readback
+ * has landed, and run one batched round of the block-Jacobi requests
+ * (which waits for that round).  *in_flight = queued + preparing + active
+ * requests.  Memory of finished problems returns to the budget in batches
+ * (when admission needs it, every max_slots problems, or when idle). */
 int cp_server_step(cp_server* srv, int* in_flight);
-/* Completed request: info and (optional) results; returns CP_ERR_STATE if not done. */
+/* Completed request: info and (optional) results; returns CP_ERR_STATE if
+ * not done, or the request's failure code, with cp_last_error() giving the
+ * message of the failure that ended it. */
 int cp_server_result(cp_server* srv, long long id, cp_request_info* info, double* u, double* sig_i);
 int cp_server_clock(cp_server* srv, double* now);
 
