@@ -128,6 +128,16 @@ def smooth_kernel(P):
     return "cpk::k_mg_smooth<9,1,float>", "mg_smooth"
 
 
+def hbm_copy(A, ctx, nbytes=1 << 30, reps=20):
+    """Achievable HBM bandwidth: a plain device copy of 1 GiB (four times the
+    Infinity Cache; read once and written once per launch) and a read-only
+    pass over 1 GiB."""
+    ms, msr = A.bench_copy(ctx, nbytes, reps)
+    gbps, rgbps = 2 * nbytes / ms / 1e6, nbytes / msr / 1e6
+    return {"bytes": nbytes, "copy_ms": ms, "copy_gbps": gbps, "copy_frac_of_peak": gbps / HBM_PEAK_GBS,
+            "read_ms": msr, "read_gbps": rgbps, "read_frac_of_peak": rgbps / HBM_PEAK_GBS}
+
+
 def bench_mg(A, P, precision, reps):
     import ctypes as C
     v, sm, b = C.c_double(), C.c_double(), C.c_double()
@@ -362,6 +372,7 @@ def other_workload(A, args):
                          "kernel": smooth_kernel(Pm)[0] + " (finest-level Chebyshev smoothing step)",
                          "bytes_per_launch": byt, "avg_launch_ms": ms_s, "format_bytes_per_launch": fmt,
                          "achieved_format": fmt / ms_s / 1e6},
+            "hbm_copy": hbm_copy(A, ctx),
             "cpu_baseline": None}
     print(json.dumps(line), flush=True)
 
@@ -695,6 +706,7 @@ def main():
         "multigrid": mg,
         "mixed_block_jacobi": bj,
         "kernels_2000x1000": large,
+        "hbm_copy": hbm_copy(A, ctx),
         "served": serve,
         "served_block_jacobi": serve_bj,
         "cpu_baseline": None,

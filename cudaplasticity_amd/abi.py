@@ -452,6 +452,14 @@ class Problem:
         return ms.value, bs.value, ba.value
 
 
+def bench_copy(ctx, nbytes, reps=20):
+    """Average ms of a plain device copy of nbytes (read + write once) and of
+    a read-only pass over nbytes."""
+    ms, msr = C.c_double(), C.c_double()
+    check(lib().cp_bench_copy(ctx.ptr, C.c_size_t(nbytes), reps, C.byref(ms), C.byref(msr)))
+    return ms.value, msr.value
+
+
 def declared_functions(header=HEADER):
     """Names of every function declared in include/cp_abi.h."""
     import re

@@ -1369,6 +1369,86 @@ int cp_bench_spmv(cp_problem* P, int precision, int reps, double* ms, double* by
     return CP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Plain device copy, four float4 per thread, for the achievable HBM
+// bandwidth that the bandwidth-bound kernels are compared with.
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    float4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + 256 * k < n) v[k] = src[i + 256 * k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + 256 * k < n) dst[i + 256 * k] = v[k];
+}
+
+// Read-only pass over the same buffer: every element is loaded and summed;
+// the store happens only for a sum the zeroed input never gives, so the
+// loads stay and nothing is written.
+__global__ __launch_bounds__(256) void k_read(const float4* __restrict__ src, float4* __restrict__ dst, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * 1024 + threadIdx.x;
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (i + 256 * k < n) {
+            const float4 v = src[i + 256 * k];
+            acc += (v.x + v.y) + (v.z + v.w);
+        }
+    if (acc == 1.0f) dst[i] = make_float4(acc, acc, acc, acc);
+}
+}  // namespace
+
+extern "C" {
+
+int cp_bench_copy(cp_ctx* ctx, size_t bytes, int reps, double* ms, double* ms_read)
+{
+    if (!ctx || reps < 1 || bytes < 16 || bytes % 16 != 0 || !ms)
+        return cp::fail(CP_ERR_INVALID, "cp_bench_copy: bad arguments (bytes a positive multiple of 16)");
+    CP_HIP(hipSetDevice(ctx->device));
+    const size_t n = bytes / 16;
+    if ((n + 1023) / 1024 > 0x7fffffff) return cp::fail(CP_ERR_INVALID, "cp_bench_copy: too large");
+    float4 *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+    int status = CP_OK;
+    float t = 0.f;
+    if (hipMalloc(&src, bytes) != hipSuccess || hipMalloc(&dst, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess || hipEventCreate(&e2) != hipSuccess ||
+        hipMemsetAsync(src, 0, bytes, st) != hipSuccess) {
+        status = cp::fail(CP_ERR_NOMEM, "cp_bench_copy: allocation failed");
+    } else {
+        const dim3 g((unsigned)((n + 1023) / 1024));
+        hipLaunchKernelGGL(k_copy, g, dim3(256), 0, st, src, dst, n);   // warm-up
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, g, dim3(256), 0, st, src, dst, n);
+        (void)hipEventRecord(e1, st);
+        for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_read, g, dim3(256), 0, st, src, dst, n);
+        (void)hipEventRecord(e2, st);
+        float t2 = 0.f;
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e2) != hipSuccess ||
+            hipEventElapsedTime(&t, e0, e1) != hipSuccess || hipEventElapsedTime(&t2, e1, e2) != hipSuccess) {
+            status = cp::fail(CP_ERR_DEVICE, "cp_bench_copy: launch failed");
+        } else {
+            *ms = t / reps;
+            if (ms_read) *ms_read = t2 / reps;
+        }
+    }
+    if (st) (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (st) (void)hipStreamDestroy(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return status;
+}
+
 int cp_bench_assemble(cp_problem* P, int precision, int reps, double* ms)
 {
     if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED))

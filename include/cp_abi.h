@@ -345,6 +345,12 @@ int cp_bench_assemble(cp_problem* prob, int precision, int reps, double* ms);
  * library's storage format. */
 int cp_bench_mg(cp_problem* prob, int precision, int reps, double* ms_vcycle, double* ms_smooth,
                 double* bytes_smooth);
+/* Achievable HBM bandwidth: average ms of `reps` launches of a plain device
+ * copy of `bytes` (a multiple of 16; read once and written once per launch)
+ * between two buffers this call allocates and frees, and (ms_read, may be
+ * NULL) of a read-only pass over the same `bytes`.  New: a yardstick for the
+ * bandwidth-bound kernels, no counterpart in the reference. */
+int cp_bench_copy(cp_ctx* ctx, size_t bytes, int reps, double* ms, double* ms_read);
 /* Matrix storage info: nodes, stored blocks (with padding), true blocks. */
 int cp_problem_layout(cp_problem* prob, long long* nblk_stored, long long* nblk, int* max_width);
 
