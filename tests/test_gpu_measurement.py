@@ -1,0 +1,52 @@
+"""The measurement entry points on the GPU: the bandwidth yardstick
+(cp_bench_copy) reports physically possible figures and refuses bad sizes,
+and the roofline kernel's timing entry point (cp_bench_mg) returns a byte
+count within the bounds of the level-0 storage."""
+import ctypes as C
+
+import pytest
+
+from cudaplasticity_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+PEAK_GBS = 8000.0
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = A.Context(0)
+    yield c
+    c.close()
+
+
+def test_copy_yardstick_is_below_peak(ctx):
+    """1 GiB buffers, four times the Infinity Cache: a device copy and a
+    read-only pass both run below the 8 TB/s HBM peak and above 2 TB/s."""
+    n = 1 << 30
+    ms, msr = A.bench_copy(ctx, n, 5)
+    copy, read = 2 * n / ms / 1e6, n / msr / 1e6
+    print("copy %.0f GB/s, read %.0f GB/s" % (copy, read), flush=True)
+    assert 2000.0 < copy < PEAK_GBS and 2000.0 < read < PEAK_GBS
+    with pytest.raises(A.CPError) as e:
+        A.bench_copy(ctx, 1000, 1)                   # not a multiple of 16
+    assert e.value.code == A.CP_ERR_INVALID
+
+
+def test_smoothing_step_bytes_match_the_storage_formula(ctx):
+    """cp_bench_mg's byte count of the finest smoothing step lies between
+    the stored level-0 matrix and that matrix plus eight vectors, and the
+    V-cycle takes longer than one smoothing step."""
+    mesh = A.gen_quarter_plate(1.0, 0.1, 64, 32)
+    loads = A.Loads(mesh).force(1, 100.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    P = A.Problem(ctx, mesh, loads, A.material(2.1e5, 0.3))
+    P.set_grid(65, 33)
+    P.solve(A.options(cg_tol=1e-8, precision=A.MIXED, precond=2), elastic=True)
+    v, sm, b = C.c_double(), C.c_double(), C.c_double()
+    A.check(A.lib().cp_bench_mg(P.ptr, A.MIXED, 10, C.byref(v), C.byref(sm), C.byref(b)))
+    lay = P.layout()
+    print("V-cycle %.3f ms, smoothing step %.4f ms, %d bytes" % (v.value, sm.value, b.value), flush=True)
+    assert v.value > sm.value > 0.0
+    # every stored FP32 block (16 B of values and a 4 B column) is read once,
+    # and no more than the stored blocks and eight vectors of the level
+    assert 20.0 * lay["nblk_stored"] <= b.value <= 20.0 * lay["nblk_stored"] + 8 * 8.0 * mesh.nn + 64 * 1024
+    P.close()
