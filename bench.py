@@ -60,12 +60,12 @@ def build_case(A, Nt, Nr, S=300.0):
     return mesh, loads, mat
 
 
-def solver_options(A, args, precision, reference=False, precond=None):
+def solver_options(A, args, precision, reference=False, precond=None, tol=1e-6):
     if reference:
         # the reference's algorithm: unpreconditioned CG to cg_tol at every
         # secant iteration, previous level's state as the initial guess
-        return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision)
-    return A.options(cg_tol=args.cg_tol, tol=1e-6, aitken=True, max_outer=500, precision=precision,
+        return A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=precision)
+    return A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=precision,
                      precond=args.precond if precond is None else precond, predictor=1, cg_forcing=args.forcing,
                      inner_tol=args.inner_tol)
 
@@ -199,12 +199,14 @@ def digest(*arrays):
     return h.hexdigest()
 
 
-def solve_alone(A, ctx, stream, opt, nsteps):
+def solve_alone(A, ctx, stream, opt, nsteps, keep=(), times=None):
     """Every problem of the stream solved alone, back to back: total seconds
-    and the digest of each result (u, sigma_i)."""
+    and the digest of each result (u, sigma_i); the results of the problems
+    in `keep` and (list `times`) each problem's seconds are kept too."""
     from cudaplasticity_amd import serving
     t0 = time.perf_counter()
     sums = []
+    kept = {}
     for k, (mesh, loads, mat, desc) in enumerate(stream):
         P = A.Problem(ctx, mesh, loads, mat)
         if opt.precond == 2:
@@ -213,8 +215,14 @@ def solve_alone(A, ctx, stream, opt, nsteps):
         per = P.solve_steps(nsteps, 1.0, opt)
         log(f"  alone {k + 1}/{len(stream)}: {desc['kind']} {desc['quads']} quads, secant "
             f"{sum(s.outer_iters for s in per)}, cg {sum(s.cg_iters for s in per)}, {time.perf_counter() - ts:.3f} s")
+        if times is not None:
+            times.append(time.perf_counter() - ts)
         sums.append(digest(P.field("u"), P.field("sigI")))
+        if k in keep:
+            kept[k] = (P.field("u"), P.field("sigI"))
         P.close()
+    if keep:
+        return time.perf_counter() - t0, sums, kept
     return time.perf_counter() - t0, sums
 
 
@@ -245,7 +253,7 @@ def served_stream(A, ctx, args, precond=2, n=None, qmax=None):
     from cudaplasticity_amd import serving
     n = args.serve_n if n is None else n
     stream = serving.make_stream(n, qmin=2e3, qmax=args.serve_qmax if qmax is None else qmax)
-    opt = solver_options(A, args, A.MIXED, precond=precond)
+    opt = solver_options(A, args, A.MIXED, precond=precond, tol=args.serve_tol)
     t_seq, sums = solve_alone(A, ctx, stream, opt, args.serve_steps)
     rate = args.serve_load * len(stream) / t_seq
     log(f"served stream: {len(stream)} problems solved back to back in {t_seq:.2f} s; serving at {rate:.2f}/s")
@@ -272,10 +280,37 @@ def c5_workload(A, args):
     stream = serving.make_stream(n, qmin=2e3, qmax=args.serve_qmax)
     quads = np.array([s[3]["quads"] for s in stream])
     log(f"stream of {n} problems: {int(quads.sum())} quads in all, {quads.min()}-{quads.max()} each")
-    opt = solver_options(A, args, A.MIXED, precond=2)
-    t_seq, sums = solve_alone(A, ctx, stream, opt, args.serve_steps)
+    opt = solver_options(A, args, A.MIXED, precond=2, tol=args.serve_tol)
+    # the sample solved again with the plain FP64 GPU path
+    sample = [k for k, s in enumerate(stream) if s[3]["quads"] <= args.c5_ref_qmax][:args.c5_ref_n]
+    times = []
+    t_seq, sums, kept = solve_alone(A, ctx, stream, opt, args.serve_steps, keep=set(sample), times=times)
     base = n / t_seq
     log(f"back to back: {t_seq:.2f} s, {base:.2f} problems/s")
+    ref_opt = solver_options(A, args, A.FP64, reference=True, tol=args.serve_tol)
+    f64_opt = solver_options(A, args, A.FP64, precond=2, tol=args.serve_tol)
+    t_ref, eu, es, pu, ps = 0.0, [], [], [], []
+    for k in sample:
+        mesh, loads, mat, desc = stream[k]
+        P = A.Problem(ctx, mesh, loads, mat)
+        ts = time.perf_counter()
+        per = P.solve_steps(args.serve_steps, 1.0, ref_opt)
+        t_ref += time.perf_counter() - ts
+        eu.append(rel(kept[k][0], P.field("u")))
+        es.append(rel(kept[k][1], P.field("sigI")))
+        P.close()
+        # precision alone: FP64 with the served options
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(*serving.lattice(desc))
+        P.solve_steps(args.serve_steps, 1.0, f64_opt)
+        pu.append(rel(kept[k][0], P.field("u")))
+        ps.append(rel(kept[k][1], P.field("sigI")))
+        P.close()
+        log(f"  FP64 reference algorithm {k}: {desc['kind']} {desc['quads']} quads, secant "
+            f"{sum(s.outer_iters for s in per)}, cg {sum(s.cg_iters for s in per)}; mixed vs it: u {eu[-1]:.1e} "
+            f"sigma_i {es[-1]:.1e}; mixed vs FP64 with the same options: u {pu[-1]:.1e} sigma_i {ps[-1]:.1e}")
+    t_mix = sum(times[k] for k in sample)
+    ref_rate = len(sample) * args.serve_steps / t_ref
     sweep = []
     for load in args.serve_loads:
         log(f"serving at {load:g} x the back-to-back rate ({load * base:.2f}/s)")
@@ -291,11 +326,24 @@ def c5_workload(A, args):
             "config": {"workload": (f"served stream of {n} problems, plate with hole or graded ring uniformly, "
                                     f"2k-{args.serve_qmax:.0f} quads log-uniform, hole radius U(0.05,0.3), yield "
                                     f"U(200,400) MPa, linear or Swift hardening, {args.serve_steps} increments each, "
-                                    f"Poisson arrivals; mixed precision, multigrid CG"),
+                                    f"Poisson arrivals; mixed precision, multigrid CG, secant iterations to "
+                                    f"{args.serve_tol:g}"),
                        "problems": n, "quads_total": int(quads.sum()),
                        "quads_range": [int(quads.min()), int(quads.max())], "max_slots": args.serve_slots,
                        "hbm_budget_gb": args.serve_budget_gb},
             "sequential_alone_load_steps_per_s": n * args.serve_steps / t_seq,
+            "fp64_reference_algorithm_sample": {
+                "what": ("the plain FP64 GPU path (the reference's algorithm: unpreconditioned FP64 CG to cg_tol "
+                         "at every secant iteration, no predictor) on the first problems of the stream of at most "
+                         f"{args.c5_ref_qmax:.0f} quads"),
+                "problems": len(sample), "quads": [int(stream[k][3]["quads"]) for k in sample],
+                "secant_tol": args.serve_tol,
+                "load_steps_per_s": ref_rate,
+                "mixed_alone_load_steps_per_s_same_problems": len(sample) * args.serve_steps / t_mix,
+                "mixed_alone_speedup": t_ref / t_mix,
+                "served_capacity_vs_fp64_reference": top["throughput_load_steps_per_s"] / ref_rate,
+                "max_rel_err_mixed_vs_reference": {"u": max(eu), "sigma_i": max(es)},
+                "max_rel_err_mixed_vs_fp64_same_options": {"u": max(pu), "sigma_i": max(ps)}},
             "sweep": sweep, "all_results_equal_alone_bitwise": bool(ok),
             "roofline": None, "cpu_baseline": None}
     print(json.dumps(line), flush=True)
@@ -474,7 +522,14 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=5)
     ap.add_argument("--serve-slots", type=int, default=16)
     ap.add_argument("--serve-load", type=float, default=1.0)
+    ap.add_argument("--serve-tol", type=float, default=1e-7,
+                    help="secant exit tolerance of the served streams: problems loaded near or past their limit "
+                         "load need 1e-7 for u within 1e-6 of FP64")
     ap.add_argument("--c5-n", type=int, default=512, help="--workload c5: problems in the stream")
+    ap.add_argument("--c5-ref-n", type=int, default=8,
+                    help="--workload c5: problems solved again with the plain FP64 GPU path")
+    ap.add_argument("--c5-ref-qmax", type=float, default=1e5,
+                    help="--workload c5: largest problem (quads) of that sample")
     ap.add_argument("--serve-loads", type=float, nargs="+", default=[0.5, 1.0, 2.0, 4.0],
                     help="--workload c5: arrival rates swept, as multiples of the back-to-back rate")
     ap.add_argument("--serve-budget-gb", type=float, default=64.0)
