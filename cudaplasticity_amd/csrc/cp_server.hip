@@ -423,6 +423,7 @@ void prepare(cp_server* S, Request* r)
     // batched CG kernels
     int s = cp::problem_create(S->ctx, r->nn, r->ne, r->x.data(), r->y.data(), r->conn.data(), r->rp.data(),
                                r->u0.data(), r->mask.data(), &r->mat, mg ? nullptr : S->stream, &P);
+    r->P = P;   // from here on the caller destroys it if anything fails
     if (s == CP_OK && mg && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
         s = cp::fail(CP_ERR_DEVICE, "cp_server: event creation failed");
     if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
@@ -431,11 +432,10 @@ void prepare(cp_server* S, Request* r)
         std::lock_guard<std::mutex> lk(S->step_mu);
         s = cp::mg_prepare(P, r->opt.precision);
     }
-    if (s != CP_OK && P) {
-        cp_problem_destroy(P);
-        P = nullptr;
+    if (s != CP_OK && r->P) {
+        cp_problem_destroy(r->P);
+        r->P = nullptr;
     }
-    r->P = P;
     r->prep_status = s;
     if (s != CP_OK) r->err = cp_last_error();
     std::vector<double>().swap(r->x);
@@ -455,7 +455,16 @@ void worker_main(cp_server* S)
             r = S->to_prep.front();
             S->to_prep.pop_front();
         }
-        prepare(S, r);
+        try {
+            prepare(S, r);
+        } catch (...) {   // host memory for the pattern or the hierarchy
+            if (r->P) {
+                cp_problem_destroy(r->P);
+                r->P = nullptr;
+            }
+            r->prep_status = cp::fail(CP_ERR_NOMEM, "cp_server: preparation ran out of host memory");
+            r->err = cp_last_error();
+        }
         std::lock_guard<std::mutex> lk(S->mu);
         S->prepared.push_back(r);
     }
