@@ -351,6 +351,15 @@ int run_gpu()
         };
         std::vector<double> e = {ps.polarSigmaError(kirsch_r, kirsch_t, kirsch_rt, 0.3)};
         dump("elast_polar_err.f64", e);
+        // the two error measures with fields of every component, for the
+        // restatement of their definitions in the test
+        dump("elast_syy.f64", ps.field(CP_F_SYY));
+        dump("elast_tau.f64", ps.field(CP_F_TAU));
+        auto fx = [](vec2 p) { return 100. + 40. * p.x - 7. * p.y; };
+        auto fy = [](vec2 p) { return 30. + 12. * p.y * p.y; };
+        auto fxy = [](vec2 p) { return -5. + 9. * p.x * p.y; };
+        std::vector<double> e2 = {ps.sigmaError(fx, fy, fxy), ps.polarSigmaError(fx, fy, fxy, 0.6)};
+        dump("elast_sigma_errors.f64", e2);
     }
     {
         // additive API: load steps, mixed precision, block-Jacobi, predictor, forcing

@@ -87,6 +87,37 @@ def layouts():
         return json.load(f)
 
 
+def sigma_errors_restated(mesh, sx, sy, txy, max_r):
+    """PlasticitySolver::sigmaError and polarSigmaError of the reference for
+    the test functions of api_test.cpp: the largest error at the element
+    centres (mean of the 4 corners) of each component, times the mesh area
+    over that component's area-weighted integral (signed for the Cartesian
+    measure, of absolute values for the polar one, which keeps elements with
+    centre radius <= max_r and takes the functions of (r, phi))."""
+    conn = mesh.conn
+    X, Y = mesh.x[conn], mesh.y[conn]
+    cx, cy = X.mean(axis=1), Y.mean(axis=1)
+    # bilinear quad area by the shoelace formula (exact for straight edges)
+    area = 0.5 * np.abs((X[:, 0] * Y[:, 1] - X[:, 1] * Y[:, 0]) + (X[:, 1] * Y[:, 2] - X[:, 2] * Y[:, 1]) +
+                        (X[:, 2] * Y[:, 3] - X[:, 3] * Y[:, 2]) + (X[:, 3] * Y[:, 0] - X[:, 0] * Y[:, 3]))
+    fx = lambda a, b: 100. + 40. * a - 7. * b
+    fy = lambda a, b: 30. + 12. * b * b
+    fxy = lambda a, b: -5. + 9. * a * b
+    vol = area.sum()
+    cart = max(np.abs(f(cx, cy) - s).max() * vol / (s * area).sum() for f, s in ((fx, sx), (fy, sy), (fxy, txy)))
+    r = np.hypot(cx, cy)
+    phi = np.arctan(cy / cx) + np.where(cx < 0, np.pi, 0.0)
+    keep = r <= max_r
+    co, si = cx / r, cy / r
+    srr = co * co * sx + si * si * sy + 2 * co * si * txy
+    spp = si * si * sx + co * co * sy - 2 * co * si * txy
+    srp = co * si * (sy - sx) + txy * (co * co - si * si)
+    pv = area[keep].sum()
+    polar = max(np.abs(f(r, phi) - s)[keep].max() * pv / (np.abs(s) * area)[keep].sum()
+                for f, s in ((fx, srr), (fy, spp), (fxy, srp)))
+    return np.array([cart, polar])
+
+
 def test_mesh_analysis_and_vtk_writer(cpu_out):
     """Mesh::printAnalysis + Mesh::saveAsVtk (Mesh.cu:939-989, 1032-1080) on
     the points and quads of the reference's data/ring.vtk: the same section
@@ -338,6 +369,12 @@ def test_plasticity_test_through_cpp_api(tmp_path):
     assert rel(rd(tmp_path, "elast_u.f64"), uo) < 1e-8
     assert rel(rd(tmp_path, "elast_sxx.f64"), O.fields(so)["sxx"]) < 1e-8
     assert rd(tmp_path, "elast_polar_err.f64")[0] < 0.2   # Kirsch (infinite plate) near the hole
+    # sigmaError / polarSigmaError against their definitions
+    # (PlasticitySolver.cuh:464-494, 497-541) restated over the same fields
+    sx, sy, txy = rd(tmp_path, "elast_sxx.f64"), rd(tmp_path, "elast_syy.f64"), rd(tmp_path, "elast_tau.f64")
+    got = rd(tmp_path, "elast_sigma_errors.f64")
+    want = sigma_errors_restated(A.gen_quarter_plate(1.0, 0.1, 100, 49), sx, sy, txy, 0.6)
+    assert np.allclose(got, want, rtol=1e-12, atol=0), (got, want)
 
     # additive API == C-ABI path
     pm = A.gen_quarter_plate(1.0, 0.1, 40, 20)
