@@ -406,6 +406,78 @@ int run_gpu()
         dump("q8_energy.f64", e);
         ps.saveAsVtk(outdir + "/q8_plate.vtk");
     }
+    {
+        // fixBorderVert / fixBorderHor through solveElastCUDA: a rectangle in
+        // uniaxial tension with the symmetry borders has the exact uniform
+        // state u = (S x / E, -nu S y / E), which bilinear quads reproduce
+        Mesh mesh;
+        mesh.genRectangle(0., 2., 0., 1., 8, 4);
+        auto find_border = [&](auto on) {
+            for (int b = 0; b < mesh.bordersCount; ++b) {
+                bool all = true;
+                for (int i = 0; i < mesh.borderLength[b]; ++i) all = all && on(mesh.node[mesh.borders[b][i]]);
+                if (all) return b;
+            }
+            return -1;
+        };
+        const int right = find_border([](vec2 p) { return p.x == 2.; });
+        const int left = find_border([](vec2 p) { return p.x == 0.; });
+        const int bottom = find_border([](vec2 p) { return p.y == 0.; });
+        LoadConditions lc;
+        lc.setForce(right, vec2(50., 0.));
+        lc.fixBorderVert(left);     // u_x = 0 on x = 0
+        lc.fixBorderHor(bottom);    // u_y = 0 on y = 0
+        Material m;
+        m.E = 2.1e5;
+        m.nu = 0.3;
+        PlasticitySolver ps(m, mesh, lc);
+        ps.options.cgTol = 1e-12;
+        ps.iterOutput = false;
+        ps.solveElastCUDA();
+        std::vector<double> out = {(double)right, (double)left, (double)bottom};
+        const std::vector<double>& u = ps.displacements();
+        for (int i = 0; i < mesh.nodeCount; ++i) {
+            out.push_back(mesh.node[i].x);
+            out.push_back(mesh.node[i].y);
+            out.push_back(u[2 * (size_t)i]);
+            out.push_back(u[2 * (size_t)i + 1]);
+        }
+        dump("patch_u.f64", out);
+    }
+    {
+        // fixBorder + setVolumeForce of a constant field equals setGravity
+        // (the same body-force load), bit for bit, through solveCUDA; and
+        // solveCUDA_FD (FP32 CG with FP64 refinement) agrees with it
+        std::vector<double> res[3];
+        for (int variant = 0; variant < 3; ++variant) {
+            Mesh mesh;
+            mesh.genRectangle(0., 1., 0., 4., 3, 8);
+            int top = -1;
+            for (int b = 0; b < mesh.bordersCount && top < 0; ++b) {
+                bool all = true;
+                for (int i = 0; i < mesh.borderLength[b]; ++i) all = all && mesh.node[mesh.borders[b][i]].y == 4.;
+                if (all) top = b;
+            }
+            LoadConditions lc;
+            lc.fixBorder(top);
+            if (variant == 0) lc.setGravity(9.81);
+            else lc.setVolumeForce([](vec2) { return vec2(0., -9.81); });
+            Material m;
+            m.setLinearPlast(2.1e5, 10., 2.1e5 / 101);   // the column's own weight (up to 39) exceeds it
+            PlasticitySolver ps(m, mesh, lc);
+            ps.iterOutput = false;
+            ps.options.cgTol = 1e-10;
+            ps.options.tol = 1e-8;
+            if (variant < 2) ps.solveCUDA();
+            else ps.solveCUDA_FD();
+            res[variant] = ps.displacements();
+            const std::vector<double>& si = ps.field(CP_F_SIGI);
+            res[variant].insert(res[variant].end(), si.begin(), si.end());
+        }
+        dump("vol_gravity.f64", res[0]);
+        dump("vol_volume.f64", res[1]);
+        dump("vol_fd.f64", res[2]);
+    }
     std::cout << "gpu cases written to " << outdir << "\n";
     return 0;
 }

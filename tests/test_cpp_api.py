@@ -369,6 +369,18 @@ def test_plasticity_test_through_cpp_api(tmp_path):
     assert rel(rd(tmp_path, "elast_u.f64"), uo) < 1e-8
     assert rel(rd(tmp_path, "elast_sxx.f64"), O.fields(so)["sxx"]) < 1e-8
     assert rd(tmp_path, "elast_polar_err.f64")[0] < 0.2   # Kirsch (infinite plate) near the hole
+    # fixBorderVert / fixBorderHor: the exact uniform state of uniaxial tension
+    pu = rd(tmp_path, "patch_u.f64")
+    right, left, bottom = pu[:3].astype(int)
+    assert len({right, left, bottom}) == 3 and min(right, left, bottom) >= 0
+    X, Y, UX, UY = pu[3:].reshape(-1, 4).T
+    assert np.abs(UX - 50.0 * X / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
+    assert np.abs(UY + 0.3 * 50.0 * Y / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
+    # fixBorder + setVolumeForce == setGravity bit for bit; solveCUDA_FD within 1e-6
+    g, v, fd = rd(tmp_path, "vol_gravity.f64"), rd(tmp_path, "vol_volume.f64"), rd(tmp_path, "vol_fd.f64")
+    assert np.array_equal(g, v)
+    assert np.abs(g).max() > 0
+    assert rel(fd, g) < 1e-6
     # sigmaError / polarSigmaError against their definitions
     # (PlasticitySolver.cuh:464-494, 497-541) restated over the same fields
     sx, sy, txy = rd(tmp_path, "elast_sxx.f64"), rd(tmp_path, "elast_syy.f64"), rd(tmp_path, "elast_tau.f64")
