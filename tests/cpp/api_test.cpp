@@ -443,6 +443,13 @@ int run_gpu()
             out.push_back(u[2 * (size_t)i + 1]);
         }
         dump("patch_u.f64", out);
+        // the same in plane strain (setPlainCondition)
+        PlasticitySolver pe(m, mesh, lc);
+        pe.setPlainCondition(strain);
+        pe.options.cgTol = 1e-12;
+        pe.iterOutput = false;
+        pe.solveElastCUDA();
+        dump("patch_u_strain.f64", pe.displacements());
     }
     {
         // fixBorder + setVolumeForce of a constant field equals setGravity

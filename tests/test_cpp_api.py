@@ -376,6 +376,9 @@ def test_plasticity_test_through_cpp_api(tmp_path):
     X, Y, UX, UY = pu[3:].reshape(-1, 4).T
     assert np.abs(UX - 50.0 * X / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
     assert np.abs(UY + 0.3 * 50.0 * Y / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
+    ue = rd(tmp_path, "patch_u_strain.f64")                 # setPlainCondition(strain)
+    assert np.abs(ue[0::2] - 50.0 * (1 - 0.09) * X / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
+    assert np.abs(ue[1::2] + 50.0 * 0.3 * 1.3 * Y / 2.1e5).max() <= 1e-9 * 50.0 * 2.0 / 2.1e5
     # fixBorder + setVolumeForce == setGravity bit for bit; solveCUDA_FD within 1e-6
     g, v, fd = rd(tmp_path, "vol_gravity.f64"), rd(tmp_path, "vol_volume.f64"), rd(tmp_path, "vol_fd.f64")
     assert np.array_equal(g, v)

@@ -268,3 +268,53 @@ def test_inexact_secant_iterations_parity(ctx):
     assert sum(s.cg_iters for s in per) < sum(s.cg_iters for s in per2)
     P.close()
     P2.close()
+
+
+@pytest.mark.parametrize("plane", [A.PLANE_STRESS, A.PLANE_STRAIN])
+def test_uniaxial_patch_closed_form(ctx, plane):
+    """A rectangle in uniaxial tension S with symmetry lines x = 0, y = 0
+    through the C-ABI: the exact uniform state, eps_xx = S / E and
+    eps_yy = -nu S / E in plane stress, S (1 - nu^2) / E and
+    -S nu (1 + nu) / E in plane strain (formC_plainStrain)."""
+    E, nu, S = 2.1e5, 0.3, 50.0
+    m = A.gen_rect(0.0, 2.0, 0.0, 1.0, 8, 4)
+    right = next(i for i, b in enumerate(m.borders) if np.all(m.x[b] == 2.0))
+    loads = A.Loads(m).force(right, S, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    P = A.Problem(ctx, m, loads, A.material(E, nu, plane=plane))
+    P.solve(A.options(cg_tol=1e-13), elastic=True)
+    u = P.field("u")
+    exx, eyy = (S / E, -nu * S / E) if plane == A.PLANE_STRESS else (S * (1 - nu * nu) / E, -S * nu * (1 + nu) / E)
+    scale = S / E * 2.0
+    print("plane %d: max error %.2e %.2e" % (plane, np.abs(u[0::2] - exx * m.x).max() / scale,
+                                               np.abs(u[1::2] - eyy * m.y).max() / scale), flush=True)
+    assert np.abs(u[0::2] - exx * m.x).max() <= 1e-10 * scale
+    assert np.abs(u[1::2] - eyy * m.y).max() <= 1e-10 * scale
+    P.close()
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.MIXED])
+def test_plane_strain_plastic_parity(ctx, precision):
+    """Plane strain on the quarter plate with linear hardening (8 fixed
+    secant iterations, tight CG): u and the stress fields within 1e-8 of the
+    oracle in FP64, 1e-6 in mixed precision; plastic flags equal."""
+    E = 2.1e5
+    pm = A.gen_quarter_plate(1.0, 0.1, 24, 12)
+    pl = A.Loads(pm).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    om = O.gen_quarter_plate(1.0, 0.1, 24, 12)
+    ol = O.Problem(om).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    pmat = A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101, plane=A.PLANE_STRAIN)
+    omat = O.material(E, 0.3, 240.0, O.LAW_LINEAR, K_T=E / 101, plane=O.PLANE_STRAIN)
+    P = A.Problem(ctx, pm, pl, pmat)
+    P.solve(A.options(cg_tol=1e-12, fixed_outer=8, precision=precision, precond=True, inner_tol=1e-5))
+    u_o, s_o, _ = O.solve(ol, omat, O.options(cg_tol=1e-13, fixed_outer=8))
+    F_o = O.fields(s_o)
+    bar = 1e-8 if precision == A.FP64 else 1e-6
+    print("plane strain, precision %d: u %.2e sigma_i %.2e" %
+          (precision, relnorm(P.field("u"), u_o), relnorm(P.field("sigI"), F_o["sigI"])), flush=True)
+    assert relnorm(P.field("u"), u_o) < bar
+    for f in STRESS_FIELDS:
+        assert relnorm(P.field(f), F_o[f]) < bar, f
+    epsT = pmat.sigmaT / pmat.E
+    assert np.array_equal(P.field("epsI") >= epsT, F_o["epsI"] >= epsT)
+    assert (F_o["epsI"] >= epsT).any()
+    P.close()

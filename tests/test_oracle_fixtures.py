@@ -419,3 +419,27 @@ def test_deform_energy_closed_form():
         got = O.deform_energy(m, mat, st)
         tol = 1e-13 if mat.law == O.LAW_LINEAR else 1e-5
         assert abs(got - want) <= tol * want, (mat.law, got, want)
+
+
+def _right_border(mesh, x):
+    return next(i for i, b in enumerate(mesh.borders) if np.all(mesh.x[b] == x))
+
+
+@pytest.mark.parametrize("plane", [O.PLANE_STRESS, O.PLANE_STRAIN])
+def test_uniaxial_patch_closed_form(plane):
+    """A rectangle in uniaxial tension S with symmetry lines x = 0 and y = 0:
+    bilinear quads reproduce the uniform state exactly.  Plane stress
+    (formC): eps_xx = S / E, eps_yy = -nu S / E.  Plane strain
+    (formC_plainStrain, eps_zz = 0): eps_xx = S (1 - nu^2) / E,
+    eps_yy = -S nu (1 + nu) / E.  A transposed or mis-scaled entry of either
+    C fails."""
+    E, nu, S = 2.1e5, 0.3, 50.0
+    m = O.gen_rect(0.0, 2.0, 0.0, 1.0, 8, 4)
+    p = O.Problem(m).force(_right_border(m, 2.0), S, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    u, st, _ = O.solve(p, O.material(E, nu, plane=plane), O.options(cg_tol=1e-13, elastic=True))
+    exx, eyy = (S / E, -nu * S / E) if plane == O.PLANE_STRESS else (S * (1 - nu * nu) / E, -S * nu * (1 + nu) / E)
+    scale = S / E * 2.0
+    assert np.abs(u[0::2] - exx * m.x).max() <= 1e-10 * scale
+    assert np.abs(u[1::2] - eyy * m.y).max() <= 1e-10 * scale
+    f = O.fields(st)
+    assert np.allclose(f["sxx"], S, rtol=1e-10) and np.abs(f["syy"]).max() <= 1e-9 * S
