@@ -402,6 +402,9 @@ class Problem:
         check(lib().cp_return_map(self.ptr, C.byref(r)))
         return r.value
 
+    def plastic_strains(self):
+        check(lib().cp_plastic_strains(self.ptr))
+
     def solve(self, opt=None, elastic=False, allow=(CP_OK,)):
         st = Stats()
         fn = lib().cp_solve_elastic if elastic else lib().cp_solve

@@ -318,3 +318,38 @@ def test_plane_strain_plastic_parity(ctx, precision):
     assert np.array_equal(P.field("epsI") >= epsT, F_o["epsI"] >= epsT)
     assert (F_o["epsI"] >= epsT).any()
     P.close()
+
+
+def test_stepwise_entry_points(ctx):
+    """The secant loop driven by hand through the step-wise entry points
+    (cp_reset_state, cp_assemble, cp_cg, cp_return_map, cp_plastic_strains)
+    reaches the state of cp_solve with the same number of plain secant
+    iterations and of the oracle (1e-8), and the exit measure falls;
+    cp_upload_state round-trips a displacement field bit for bit."""
+    pm, pl, pmat, om, ol, omat = hole_fixture_case()
+    K = 6
+    P = A.Problem(ctx, pm, pl, pmat)
+    P.solve(A.options(cg_tol=1e-13, fixed_outer=K))
+    Q = A.Problem(ctx, pm, pl, pmat)
+    Q.reset()
+    rels = []
+    for _ in range(K):
+        Q.assemble(A.FP64)
+        Q.cg(tol=1e-13, precision=A.FP64)
+        rels.append(Q.return_map())
+    Q.plastic_strains()
+    u_o, s_o, _ = O.solve(ol, omat, O.options(cg_tol=1e-13, fixed_outer=K))
+    F_o = O.fields(s_o)
+    print("exit measures", ["%.2e" % r for r in rels], "u vs solve %.2e" % relnorm(Q.field("u"), P.field("u")),
+          flush=True)
+    assert rels[-1] < rels[1]
+    assert relnorm(Q.field("u"), P.field("u")) < 1e-10
+    assert relnorm(Q.field("u"), u_o) < 1e-8
+    for f in STRESS_FIELDS + ["psi", "exx_p", "epsI_p"]:
+        assert relnorm(Q.field(f), P.field(f)) < 1e-10, f
+        assert relnorm(Q.field(f), F_o[f]) < 1e-8, f
+    R = A.Problem(ctx, pm, pl, pmat)
+    R.upload_state(u=P.field("u"))
+    assert np.array_equal(R.field("u"), P.field("u"))
+    for X in (P, Q, R):
+        X.close()
