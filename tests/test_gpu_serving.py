@@ -142,6 +142,36 @@ def test_served_stream_overlapping_preparation(ctx):
     srv.close()
 
 
+def test_failed_requests_carry_their_message(ctx):
+    """A request whose level cannot converge (one secant iteration allowed,
+    a plastic problem) fails with CP_ERR_NOT_CONVERGED and the failure's
+    message, on the multigrid path and on the batched block-Jacobi path,
+    while a healthy request served beside them finishes with the bits it
+    has alone."""
+    (m, l, mat), grid, n, p = MG_CASES[0]
+    bad = A.options(cg_tol=1e-9, tol=1e-12, max_outer=1, precision=A.MIXED, precond=2, inner_tol=1e-5)
+    bad_bj = A.options(cg_tol=1e-9, tol=1e-12, max_outer=1, precision=A.MIXED, precond=True, inner_tol=1e-5)
+    (gm, gl, gmat), ggrid, gn, gp = MG_CASES[1]
+    P = A.Problem(ctx, gm, gl, gmat)
+    P.set_grid(*ggrid)
+    P.solve_steps(gn, 1.0, mg_opts(gp))
+    good_alone = P.field("u")
+    P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 32, max_slots=4)
+    i_mg = srv.submit(m, l, mat, n, 1.0, bad, grid=grid)
+    i_bj = srv.submit(m, l, mat, n, 1.0, bad_bj)
+    i_ok = srv.submit(gm, gl, gmat, gn, 1.0, mg_opts(gp), grid=ggrid)
+    srv.run_until_done()
+    for rid in (i_mg, i_bj):
+        code, info, _, _ = srv.result(rid)
+        print("request", rid, "code", code, srv.last_message, flush=True)
+        assert code == A.CP_ERR_NOT_CONVERGED and info.status == A.CP_ERR_NOT_CONVERGED
+        assert "did not converge" in srv.last_message
+    code, info, u, _ = srv.result(i_ok)
+    assert code == A.CP_OK and np.array_equal(u, good_alone)
+    srv.close()
+
+
 def test_budget_and_slots(ctx):
     (m, l, mat), _, _ = CASES[0]
     import ctypes as C
