@@ -656,11 +656,16 @@ static int run_cg_iterations(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal
         CP_HIP(hipStreamSynchronize(st));
         const cpk::CgScal h = *P->h_cg;
         if (h.done) break;
-        // predict the remaining iterations from the observed decay of gamma
+        // predict the remaining iterations from the observed decay of (r, r):
+        // since the last observation, or since the start at the first one
+        // (instead of doubling the chunk there); iterations launched after
+        // convergence return at once, so the chunk sizes never change results
         const double target = tol2 * (bnorm2 > 0. ? bnorm2 : h.bnorm2);
         int next = std::min(4096, chunk * 2);
-        if (g_prev > 0. && h.rho > 0. && h.rho < g_prev && target > 0.) {
-            const double rate = std::log(g_prev / h.rho) / double(h.iters - it_prev);
+        const double r0 = g_prev > 0. ? g_prev : h.rho0;
+        const long long i0 = g_prev > 0. ? it_prev : 0;
+        if (r0 > 0. && h.rho > 0. && h.rho < r0 && target > 0. && h.iters > i0) {
+            const double rate = std::log(r0 / h.rho) / double(h.iters - i0);
             const double rem = std::log(h.rho / target) / rate;
             if (rem > 0 && rem < 1e7) next = (int)std::min(4096., std::max(8., rem * 1.05 + 4.));
         }
