@@ -1,25 +1,28 @@
 """Benchmark of the elasto-plastic load-stepping loop on one MI355X.
 
-Workload (config 2): quarter plate [0,1]^2 with a hole of radius 0.1,
-N_theta x N_r = 632 x 316 bilinear quads (199 712 elements, 200 661 nodes,
-401 322 DOF), plane stress, E = 2.1e5 MPa, nu = 0.3, von Mises secant
-(deformation theory) update with linear hardening sigma_Y = 240 MPa,
-K_T = E/101 (H = E/100), 20 equal load increments of the far-edge traction
-up to 300 MPa, Aitken-accelerated secant iterations to a 1e-6 stress
-tolerance, linear solves to ||r||/||b|| = 1e-9.
+Default workload (config 3): quarter plate [0,1]^2 with a hole of radius
+0.1, N_theta x N_r = 2000 x 1000 bilinear quads (2,000,000 elements,
+2,003,001 nodes, 4,006,002 DOF), plane stress, E = 2.1e5 MPa, nu = 0.3, von
+Mises secant (deformation theory) update with Swift hardening
+sigma = sigma_Y (1 + eps_p/eps0)^0.2, sigma_Y = 240 MPa, eps0 = sigma_Y/E,
+40 equal load increments of the far-edge traction up to 300 MPa,
+Aitken-accelerated secant iterations to a 1e-6 stress tolerance, linear
+solves to ||r||/||b|| = 1e-9.
 
 A "step" is one load increment: warm-started from the previous one,
 assemble -> CG -> return map until the secant iteration converges.  The
-timed region runs the first K increments of the 20-increment ramp on a
-fresh device problem, all inputs already in HBM; W warm-up increments run
-first on a separate instance.  `value` is the mixed-precision path: FP32 CG
-preconditioned by a geometric multigrid V-cycle, FP64 iterative refinement,
-FP64 return map and Aitken.  The same line reports the FP64 path with the
-same solver settings, the mixed path with block-Jacobi CG instead of
-multigrid, the reference algorithm in FP64 (unpreconditioned CG to the full
-tolerance at every secant iteration, no predictor) over the first
-increments, the roofline of the dominant kernel, a served stream of
-independent problems (p50/p99 latency) and the CPU oracle.
+timed region runs the 40-increment ramp on a fresh device problem with all
+inputs already in HBM; with --steps K < 40 the first 40 - K increments run
+untimed first and the last K are timed.  W warm-up increments (--warmup)
+run before, on a separate problem.  `value` is the mixed-precision path:
+FP32 CG preconditioned by a geometric multigrid V-cycle, FP64 iterative
+refinement, FP64 return map and Aitken.  The same line reports the FP64
+path with the same solver settings, the plain FP64 GPU path (the
+reference's algorithm: unpreconditioned CG to the full tolerance at every
+secant iteration) over the first increments, the roofline of the dominant
+kernel, the 632x316 plate (config 2) and the 1M-quad graded ring (config 4)
+as sub-records, a served stream of independent problems at three arrival
+rates (p50/p99 latency) and the CPU oracle on a capped sample.
 """
 import argparse
 import hashlib
@@ -414,12 +417,11 @@ def other_workload(A, args):
             "plastic_elements": int((Pm.field("epsI") >= epsT).sum()),
             "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
                            "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
-            "roofline": {"bound": "hbm", "achieved": byt / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": byt / ms_s / 1e6 / HBM_PEAK_GBS,
+            "roofline": {"bound": "hbm", "achieved": fmt / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": fmt / ms_s / 1e6 / HBM_PEAK_GBS,
                          "traffic": MEASURED_TRAFFIC.get((nt, nr, smooth_kernel(Pm)[1])),
                          "kernel": smooth_kernel(Pm)[0] + " (finest-level Chebyshev smoothing step)",
-                         "bytes_per_launch": byt, "avg_launch_ms": ms_s, "format_bytes_per_launch": fmt,
-                         "achieved_format": fmt / ms_s / 1e6},
+                         "bytes_per_launch": fmt, "avg_launch_ms": ms_s, "reference_csr_bytes_per_launch": byt},
             "hbm_copy": hbm_copy(A, ctx),
             "cpu_baseline": None}
     print(json.dumps(line), flush=True)
@@ -484,25 +486,347 @@ def q8_workload(A, args):
             "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
                            "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
             "assembly": asm,
-            "roofline": {"bound": "hbm", "achieved": byt / ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": byt / ms / 1e6 / HBM_PEAK_GBS, "traffic": None,
+            "roofline": {"bound": "hbm", "achieved": fmt / ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": fmt / ms / 1e6 / HBM_PEAK_GBS, "traffic": None,
                          "kernel": "cpk::k_cg_iter_fused<0,true,float> (block-Jacobi CG iteration, 8-node matrix)",
-                         "bytes_per_launch": byt, "avg_launch_ms": ms, "format_bytes_per_launch": fmt,
-                         "achieved_format": fmt / ms / 1e6},
+                         "bytes_per_launch": fmt, "avg_launch_ms": ms, "reference_csr_bytes_per_launch": byt,
+                         "working_set_bytes": fmt, "cache_resident": fmt < 256e6},
             "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
+
+
+NSTEPS_C3 = 40
+
+
+def cpu_model():
+    """CPU model name of this host (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.lower().startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def plate_case(A, nt, nr, law):
+    E = 2.1e5
+    mesh = A.gen_quarter_plate(1.0, 0.1, nt, nr)
+    loads = A.Loads(mesh).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    if law == "swift":
+        mat = A.material(E, 0.3, 240.0, A.LAW_SWIFT, n=0.2)
+    else:
+        mat = A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101)
+    return mesh, loads, mat
+
+
+def ring_case(A, nt=16700, nr=60):
+    import math
+    E = 2.1e5
+    mesh = A.gen_quarter_ring(0.5, 1.0, nt, nr, 1.05)
+    p = 1.2 * 240.0 * math.log(2.0)
+    loads = A.Loads(mesh).force(0, -p, 0.0, norm_orient=True).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    return mesh, loads, A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101), p
+
+
+def ramp_timed(A, ctx, mesh, loads, mat, grid, nsteps, opt, tag, timed=None, upto=None):
+    """Increments 1..upto (default nsteps) of an nsteps ramp on a fresh
+    problem; the last `timed` of them (default all) are timed, the ones
+    before run untimed.  Every solve ends with its stream synchronised, so
+    the clock brackets only the timed increments' device work.
+    Returns (timed seconds, per-increment stats, problem)."""
+    upto = nsteps if upto is None else upto
+    timed = upto if timed is None else timed
+    P = A.Problem(ctx, mesh, loads, mat)
+    if opt.precond == 2:
+        P.set_grid(*grid)
+        if not GRAPHS:
+            P.set_graphs(False)
+    per = []
+    t0 = None
+    for n in range(1, upto + 1):
+        o = A.Options.from_buffer_copy(opt)
+        o.warm = 1 if n > 1 else 0
+        if n == upto - timed + 1:
+            t0 = time.perf_counter()
+        if n == 1:
+            P.set_load(n / nsteps)
+        else:
+            P.advance_load(n / nsteps, int(opt.predictor))
+        st = P.solve(o)
+        per.append(st)
+        log(f"  {tag} increment {n}/{nsteps}: secant {st.outer_iters} cg {st.cg_iters} refinements "
+            f"{st.refinements} rel_err {st.rel_err:.2e} {st.seconds:.3f} s")
+    return time.perf_counter() - t0, per, P
+
+
+def mg_options(A, args, precision, tol):
+    return A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=precision, precond=2,
+                     predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
+
+
+def pair_record(A, Pm, Pf, mat):
+    epsT = mat.sigmaT / mat.E
+    return {"u": rel(Pm.field("u"), Pf.field("u")), "sigma_i": rel(Pm.field("sigI"), Pf.field("sigI")),
+            "plastic_flags_equal": bool(np.array_equal(Pm.field("epsI") >= epsT, Pf.field("epsI") >= epsT))}
+
+
+def sub_config(A, ctx, args, which):
+    """Config 2 (632x316 plate, linear hardening, 20 increments) or config 4
+    (16700x60 graded ring, 30 increments): the whole ramp in mixed precision
+    and in FP64 with the same options, each timed."""
+    if which == "c2":
+        nt, nr, nsteps, tol = 632, 316, 20, 1e-6
+        mesh, loads, mat = plate_case(A, nt, nr, "linear")
+        name = (f"plate with hole, quarter model {nt}x{nr} quads ({mesh.ne}), linear hardening sigma_Y 240 MPa "
+                f"H E/100, {nsteps} increments to 300 MPa, Aitken to {tol:g}")
+    else:
+        nt, nr, nsteps, tol = 16700, 60, 30, 1e-8
+        mesh, loads, mat, p = ring_case(A, nt, nr)
+        name = (f"thick ring a 0.5 b 1.0, quarter model {nt}x{nr} quads ({mesh.ne}) graded 1.05 per element "
+                f"radially, linear hardening H E/100, {nsteps} increments to 1.2 sigma_Y ln(b/a) = {p:.1f} MPa, "
+                f"Aitken to {tol:g} (past the limit pressure a stress difference of the exit tolerance shows up "
+                f"about a hundredfold in u)")
+    log(name)
+    grid = (nt + 1, nr + 1)
+    tm, per_m, Pm = ramp_timed(A, ctx, mesh, loads, mat, grid, nsteps, mg_options(A, args, A.MIXED, tol),
+                               which + " mixed")
+    tf, per_f, Pf = ramp_timed(A, ctx, mesh, loads, mat, grid, nsteps, mg_options(A, args, A.FP64, tol),
+                               which + " fp64")
+    rec = {"workload": name, "increments": nsteps, "load_steps_per_s": nsteps / tm,
+           "fp64_load_steps_per_s": nsteps / tf, "speedup_vs_fp64": tf / tm,
+           "rel_err_vs_fp64": pair_record(A, Pm, Pf, mat),
+           "cg_iterations": {"mixed": sum(s.cg_iters for s in per_m), "fp64": sum(s.cg_iters for s in per_f)}}
+    mv, ms, _ = bench_mg(A, Pm, A.MIXED, 100)
+    rec["vcycle_ms_mixed"] = mv
+    rec["finest_smooth_step_ms_mixed"] = ms
+    rec["vcycle_over_finest_step"] = mv / ms
+    Pm.close()
+    Pf.close()
+    return rec
+
+
+def served_sweep(A, ctx, args):
+    """A config-5 stream of --serve-n problems (plate or graded ring, 2k-500k
+    quads log-uniform, random hole radius, yield and law; --serve-steps
+    increments each; mixed precision, multigrid CG) solved alone back to back
+    for each result's digest, then served with Poisson arrivals at
+    --serve-rates times the back-to-back rate, every served result checked
+    bit for bit against its digest.  A sample of the stream is solved again
+    with the plain FP64 GPU path (the reference's algorithm) for the error of
+    the served results against it."""
+    from cudaplasticity_amd import serving
+    stream = serving.make_stream(args.serve_n, qmin=2e3, qmax=args.serve_qmax)
+    opt = solver_options(A, args, A.MIXED, precond=2, tol=args.serve_tol)
+    sample = [k for k, s in enumerate(stream) if s[3]["quads"] <= args.serve_ref_qmax][:args.serve_ref_n]
+    t_seq, sums, kept = solve_alone(A, ctx, stream, opt, args.serve_steps, keep=set(sample))
+    base = len(stream) / t_seq
+    log(f"served stream: {len(stream)} problems back to back in {t_seq:.2f} s ({base:.2f} problems/s)")
+    sweep = []
+    for load in args.serve_rates:
+        rec = serve_once(A, ctx, args, stream, opt, load * base, sums)
+        rec["load"] = load
+        log("  served at %g x: %s" % (load, json.dumps(rec)))
+        sweep.append(rec)
+    ref_opt = solver_options(A, args, A.FP64, reference=True, tol=args.serve_ref_tol)
+    eu, es = [], []
+    for k in sample:
+        mesh, loads, mat, desc = stream[k]
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.solve_steps(args.serve_steps, 1.0, ref_opt)
+        eu.append(rel(kept[k][0], P.field("u")))
+        es.append(rel(kept[k][1], P.field("sigI")))
+        log(f"  plain FP64 path, problem {k} ({desc['kind']}, {desc['quads']} quads): served mixed vs it "
+            f"u {eu[-1]:.1e} sigma_i {es[-1]:.1e}")
+        P.close()
+    quads = [s[3]["quads"] for s in stream]
+    return {"problems": len(stream), "load_steps_each": args.serve_steps, "quads_range": [min(quads), max(quads)],
+            "secant_tol": args.serve_tol, "max_slots": args.serve_slots,
+            "sequential_alone_load_steps_per_s": len(stream) * args.serve_steps / t_seq,
+            "sweep": sweep,
+            "capacity_vs_back_to_back": max(r["throughput_load_steps_per_s"] for r in sweep) /
+            (len(stream) * args.serve_steps / t_seq),
+            "all_results_equal_alone_bitwise": all(r["results_equal_alone_bitwise"] for r in sweep),
+            "rel_err_vs_plain_fp64_sample": {
+                "problems": [int(quads[k]) for k in sample],
+                "plain_fp64": (f"unpreconditioned FP64 CG to {args.cg_tol:g} at every secant iteration, secant "
+                               f"loop to {args.serve_ref_tol:g}"),
+                "max_u": max(eu) if eu else None, "max_sigma_i": max(es) if es else None}}
+
+
+def cpu_baseline_c3(args, ref_cg_first, cores):
+    """The oracle (the CPU restatement of the reference solver, OpenMP) on
+    the 2000x1000 plate's first increment (7.5 MPa, elastic: one secant
+    iteration), the reference algorithm (unpreconditioned FP64 CG), under a
+    wall-time cap: a short run gives the time of the set-up and of one CG
+    iteration, then one run fills the cap.  The fraction of the increment
+    done is the CG iterations completed over the `ref_cg_first` iterations
+    the same algorithm needs on the GPU to finish it."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    om = O.gen_quarter_plate(1.0, 0.1, 2000, 1000)
+    ol = O.Problem(om).force(1, 300.0 / NSTEPS_C3, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    mat = O.material(2.1e5, 0.3, 240.0, O.LAW_SWIFT, n=0.2)
+    t0 = time.perf_counter()
+    O.solve(ol, mat, O.options(cg_tol=1e-30, cg_maxit=20, fixed_outer=1))
+    t_short = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.solve(ol, mat, O.options(cg_tol=1e-30, cg_maxit=120, fixed_outer=1))
+    per_it = max(1e-6, (time.perf_counter() - t0 - t_short) / 100)
+    n_it = int(max(200, min(ref_cg_first, (args.cpu_cap - t_short) / per_it)))
+    t0 = time.perf_counter()
+    _, _, st = O.solve(ol, mat, O.options(cg_tol=args.cg_tol, cg_maxit=n_it, fixed_outer=1))
+    secs = time.perf_counter() - t0
+    done = min(1.0, st.cg_iters / ref_cg_first)
+    nnz = O.lib.or_csr_nnz(om.nn, om.ne, O.ip(om.conn))
+    cg_bytes = cg_iteration_bytes(nnz, 2 * om.nn, 8)
+    return {"value": done / secs, "unit": "load-steps/s", "cores": cores, "kind": "port", "cpu_model": cpu_model(),
+            "seconds": secs, "increments_completed": int(st.status == 0 and st.cg_iters < n_it),
+            "cg_iterations_done": int(st.cg_iters), "cg_iterations_needed": int(ref_cg_first),
+            "cg_gbps": st.cg_iters * cg_bytes / max(1e-9, secs - t_short) / 1e9,
+            "sample": (f"oracle/cp_oracle.c (OpenMP, {cores} threads) on the 2000x1000 plate, first of the 40 "
+                       f"increments (7.5 MPa, elastic), the reference algorithm (unpreconditioned FP64 CG to "
+                       f"{args.cg_tol:g}) under a {args.cpu_cap:g} s cap: {st.cg_iters} of the {ref_cg_first} CG "
+                       f"iterations the increment needs (the GPU run of the same algorithm finishes it) in "
+                       f"{secs:.1f} s, i.e. {done:.4f} of one load step")}
+
+
+def c3_workload(A, args):
+    """The headline line: config 3 (2000x1000 plate, Swift law, 40
+    increments)."""
+    nt, nr, tol = 2000, 1000, 1e-6
+    mesh, loads, mat = plate_case(A, nt, nr, "swift")
+    grid = (nt + 1, nr + 1)
+    K = max(1, min(args.steps, NSTEPS_C3))
+    ctx = A.Context(0)
+    log("device", ctx.info())
+    log(f"mesh {nt}x{nr}: {mesh.nn} nodes, {mesh.ne} quads")
+    opt_mx, opt_64 = mg_options(A, args, A.MIXED, tol), mg_options(A, args, A.FP64, tol)
+    if args.warmup > 0:
+        _, _, P = ramp_timed(A, ctx, mesh, loads, mat, grid, NSTEPS_C3, opt_mx, "warmup",
+                             upto=min(args.warmup, NSTEPS_C3))
+        P.close()
+    t_mx, per_mx, Pmx = ramp_timed(A, ctx, mesh, loads, mat, grid, NSTEPS_C3, opt_mx, "mixed", timed=K)
+    log(f"mixed: last {K} of {NSTEPS_C3} increments in {t_mx:.3f} s")
+    t_64, per_64, P64 = ramp_timed(A, ctx, mesh, loads, mat, grid, NSTEPS_C3, opt_64, "fp64", timed=K)
+    log(f"fp64: last {K} of {NSTEPS_C3} increments in {t_64:.3f} s")
+    err = pair_record(A, Pmx, P64, mat)
+    P64.close()
+
+    ref = None
+    if args.ref_steps > 0:
+        R = min(args.ref_steps, NSTEPS_C3)
+        t_ref, per_ref, Pref = ramp_timed(A, ctx, mesh, loads, mat, grid, NSTEPS_C3,
+                                          solver_options(A, args, A.FP64, reference=True, tol=tol), "plain-fp64",
+                                          upto=R)
+        t_mx_R = sum(s.seconds for s in per_mx[:R])
+        ref = {"what": ("the plain FP64 GPU path: the reference's algorithm (unpreconditioned FP64 CG to cg_tol at "
+                        "every secant iteration, the previous increment's state as initial guess)"),
+               "increments": R, "seconds": t_ref, "load_steps_per_s": R / t_ref,
+               "mixed_same_increments_seconds": t_mx_R, "speedup_mixed_vs_plain_fp64": t_ref / t_mx_R,
+               "iterations": [[s.outer_iters, s.cg_iters] for s in per_ref]}
+        Pref.close()
+
+    # dominant kernel of the ramp: the finest level's Chebyshev smoothing step
+    reps = args.reps
+    mv32, ms_s32, fmt_s32 = bench_mg(A, Pmx, A.MIXED, reps)
+    nnz = scalar_nnz(A, Pmx)
+    ref_bytes = cheb_step_bytes(nnz, 2 * mesh.nn, 4)
+    kname, tkey = smooth_kernel(Pmx)
+    traffic = MEASURED_TRAFFIC.get((nt, nr, tkey))
+    roof = {"bound": "hbm", "achieved": fmt_s32 / ms_s32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": fmt_s32 / ms_s32 / 1e6 / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": kname + " (finest-level Chebyshev smoothing step, FP32)",
+            "bytes_per_launch": fmt_s32, "avg_launch_ms": ms_s32, "reps": reps,
+            "bytes_model": ("what one launch must move in the 2x2-block layout: every stored block (4 values + "
+                            "its column) once, slice descriptors, x, d, b read, x, d written, the pivot inverses "
+                            "and the mask once each"),
+            "working_set_bytes": fmt_s32,
+            "traffic_gbps": traffic / ms_s32 / 1e6 if traffic else None,
+            "reference_csr_bytes_per_launch": ref_bytes}
+    mv64, ms_s64, fmt_s64 = bench_mg(A, Pmx, A.FP64, max(20, reps // 4))
+    mg = {"levels": Pmx.mg_levels(),
+          "smoothers": ["segments" if k else "block-Jacobi" for k in Pmx.mg_smoothers()[:-1]] + ["exact"],
+          "vcycle_ms": {"mixed": mv32, "fp64": mv64}, "smooth_step_ms": {"mixed": ms_s32, "fp64": ms_s64},
+          "vcycle_over_finest_step": {"mixed": mv32 / ms_s32, "fp64": mv64 / ms_s64},
+          "smooth_step_gbps": {"mixed": fmt_s32 / ms_s32 / 1e6, "fp64": fmt_s64 / ms_s64 / 1e6},
+          "cg_iterations": {"mixed": sum(s.cg_iters for s in per_mx[NSTEPS_C3 - K:]),
+                            "fp64": sum(s.cg_iters for s in per_64[NSTEPS_C3 - K:])}}
+    log("multigrid:", mg)
+    times = {"cg_s": sum(s.cg_seconds for s in per_mx[NSTEPS_C3 - K:]),
+             "assembly_s": sum(s.asm_seconds for s in per_mx[NSTEPS_C3 - K:]),
+             "return_map_s": sum(s.rm_seconds for s in per_mx[NSTEPS_C3 - K:])}
+    epsT = mat.sigmaT / mat.E
+    plastic = int((Pmx.field("epsI") >= epsT).sum())
+    Pmx.close()
+
+    subs = {}
+    for which in ("c2", "c4"):
+        if which not in args.skip:
+            subs[which] = sub_config(A, ctx, args, which)
+            log(which, subs[which])
+    served = None if "serve" in args.skip or args.serve_n <= 0 else served_sweep(A, ctx, args)
+    if served:
+        log("served:", served)
+    copy = hbm_copy(A, ctx)
+    ctx.close()
+
+    prefix = "" if K == NSTEPS_C3 else (f"; timed: increments {NSTEPS_C3 - K + 1}..{NSTEPS_C3}, after "
+                                        f"{NSTEPS_C3 - K} untimed ones")
+    line = {
+        "metric": METRIC, "value": K / t_mx, "unit": "load-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": args.warmup, "ms_per_step": t_mx / K * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": (f"plate with hole, quarter model {nt}x{nr} quads ({mesh.ne}), Swift hardening "
+                                f"sigma_Y 240 MPa n 0.2 eps0 sigma_Y/E, {NSTEPS_C3} increments to 300 MPa, Aitken "
+                                f"to {tol:g}, cg_tol {args.cg_tol:g}{prefix}"),
+                   "nodes": mesh.nn, "elements": mesh.ne, "dof": 2 * mesh.nn,
+                   "path": ("mixed: FP32 CG preconditioned by a geometric multigrid V-cycle + FP64 refinement, "
+                            f"FP64 return map and Aitken, load predictor, inexact secant iterations (forcing "
+                            f"{args.forcing:g})")},
+        "fp64_load_steps_per_s": K / t_64, "speedup_vs_fp64": t_64 / t_mx, "rel_err_vs_fp64": err,
+        "plastic_elements": plastic,
+        "plain_fp64_path": ref,
+        "iterations": {"mixed": [[s.outer_iters, s.cg_iters, s.refinements] for s in per_mx],
+                       "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
+        "device_time_mixed": times,
+        "roofline": roof, "multigrid": mg, "hbm_copy": copy,
+        "c2": subs.get("c2"), "c4": subs.get("c4"), "served": served,
+        "cpu_baseline": None,
+    }
+    if "cpu" not in args.skip and ref:
+        cores = len(os.sched_getaffinity(0))
+        os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+        line["cpu_baseline"] = cpu_baseline_c3(args, ref["iterations"][0][1], int(os.environ["OMP_NUM_THREADS"]))
+        log("cpu baseline:", line["cpu_baseline"])
     print(json.dumps(line), flush=True)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=("c2", "c3", "c4", "c5", "q8"),
-                    help="c2: the bench line (632x316 plate); c3: 2000x1000 plate, Swift law, 40 increments; "
-                         "c4: 16700x60 graded quarter ring, 30 increments; c5: served stream of --c5-n problems "
-                         "at swept arrival rates; q8: 8-node quads on the reference's energyConvTest plate")
+    ap.add_argument("--workload", default="c3", choices=("c2", "c3", "c4", "c5", "q8"),
+                    help="c3 (default): the headline line, 2000x1000 plate, Swift law, 40 increments, with the "
+                         "632x316 plate and the graded ring as sub-records and a served sweep; c2: the 632x316 "
+                         "plate alone with its kernel micro-benchmarks; c4: 16700x60 graded quarter ring, 30 "
+                         "increments; c5: served stream of --c5-n problems at swept arrival rates; q8: 8-node "
+                         "quads on the reference's energyConvTest plate")
+    ap.add_argument("--skip", nargs="*", default=[], choices=("c2", "c4", "serve", "cpu"),
+                    help="default workload: parts of the line to leave out")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[1.0, 2.0, 4.0],
+                    help="default workload: served arrival rates, multiples of the back-to-back rate")
+    ap.add_argument("--serve-ref-n", type=int, default=4,
+                    help="default workload: served problems solved again with the plain FP64 GPU path")
+    ap.add_argument("--serve-ref-qmax", type=float, default=5e4,
+                    help="default workload: largest problem (quads) of that sample")
+    ap.add_argument("--serve-ref-tol", type=float, default=1e-9,
+                    help="default workload: secant tolerance of the plain FP64 runs of the sample")
+    ap.add_argument("--cpu-cap", type=float, default=20.0,
+                    help="default workload: wall-time cap (s) of the CPU oracle's sample")
     ap.add_argument("--q8-k", type=int, default=64,
                     help="q8 mesh: genRectWithHole(-3, 3, -2, 2, 0.5, 3k, 2k, order 2)")
-    ap.add_argument("--steps", type=int, default=NSTEPS)
+    ap.add_argument("--steps", type=int, default=NSTEPS_C3)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nt", type=int, default=632)
     ap.add_argument("--nr", type=int, default=316)
@@ -514,7 +838,8 @@ def main():
     ap.add_argument("--precond", type=int, default=2, choices=(1, 2),
                     help="1 block-Jacobi CG, 2 multigrid-preconditioned CG (headline)")
     ap.add_argument("--inner-tol", type=float, default=1e-5)
-    ap.add_argument("--ref-steps", type=int, default=8)
+    ap.add_argument("--ref-steps", type=int, default=2,
+                    help="increments of the plain FP64 GPU path (the reference's algorithm) to time")
     ap.add_argument("--large-nt", type=int, default=2000)
     ap.add_argument("--large-nr", type=int, default=1000)
     ap.add_argument("--serve-n", type=int, default=64)
@@ -551,6 +876,10 @@ def main():
         print("bench.py measures one GPU: each problem runs on a single MI355X here; how a problem or a "
               "stream would shard over several GPUs is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3" and args.copy_calibration <= 0 and not args.roofline_only:
+        from cudaplasticity_amd import abi as A
+        c3_workload(A, args)
+        return
     K = max(1, min(args.steps, NSTEPS))
     if args.workload == "q8":
         import cudaplasticity_amd.abi as A
@@ -560,10 +889,12 @@ def main():
         from cudaplasticity_amd import abi as A
         c5_workload(A, args)
         return
-    if args.workload != "c2":
+    if args.workload == "c4":
         from cudaplasticity_amd import abi as A
         other_workload(A, args)
         return
+    if args.workload == "c3":   # --roofline-only / --copy-calibration on the 2000x1000 plate
+        args.nt, args.nr = 2000, 1000
     if args.copy_calibration > 0:
         import torch
         n = int(args.copy_calibration * (1 << 20)) // 4
@@ -745,19 +1076,25 @@ def main():
                     if cg_s > 0 and args.precond != 2 else None},
         "iterations": {"mixed": [[s.outer_iters, s.cg_iters, s.refinements] for s in per_mx],
                        "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
-        "roofline": ({"bound": "hbm", "achieved": bs32 / ms_s32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": bs32 / ms_s32 / 1e6 / HBM_PEAK_GBS,
+        # bytes each launch must move in this layout; config 2's working set
+        # (about 48 MB) sits inside the 256 MB Infinity Cache, so these rates
+        # are cache-resident ones
+        "roofline": ({"bound": "hbm", "achieved": fmt_s32 / ms_s32 / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": fmt_s32 / ms_s32 / 1e6 / HBM_PEAK_GBS,
                       "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, smooth_kernel(Pmx)[1])),
                       "kernel": smooth_kernel(Pmx)[0] + " (finest-level Chebyshev smoothing step)",
-                      "bytes_per_launch": bs32, "avg_launch_ms": ms_s32, "reps": reps,
-                      "format_bytes_per_launch": fmt_s32, "achieved_format": fmt_s32 / ms_s32 / 1e6}
+                      "bytes_per_launch": fmt_s32, "avg_launch_ms": ms_s32, "reps": reps,
+                      "working_set_bytes": fmt_s32, "cache_resident": fmt_s32 < 256e6,
+                      "reference_csr_bytes_per_launch": bs32}
                      if mg else
-                     {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": ach32 / HBM_PEAK_GBS, "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, "cg")),
+                     {"bound": "hbm", "achieved": fmt32 / (ms32 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": fmt32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                      "traffic": MEASURED_TRAFFIC.get((args.nt, args.nr, "cg")),
                       "kernel": ("cpk::k_cg_iter_fused<9,true,float> (block-Jacobi Chronopoulos-Gear iteration, "
                                  "scalar step included)" if fused else "cpk::k_cg_iter<9,true,float>"),
-                      "bytes_per_launch": bytes32, "avg_launch_ms": ms32, "reps": reps,
-                      "format_bytes_per_launch": fmt32, "achieved_format": fmt32 / (ms32 * 1e-3) / 1e9}),
+                      "bytes_per_launch": fmt32, "avg_launch_ms": ms32, "reps": reps,
+                      "working_set_bytes": fmt32, "cache_resident": fmt32 < 256e6,
+                      "reference_csr_bytes_per_launch": bytes32}),
         "multigrid": mg,
         "mixed_block_jacobi": bj,
         "kernels_2000x1000": large,

@@ -239,6 +239,38 @@ def gen_quarter_ring(a, b, Nt, Nr, ratio):
     return _gen("cp_mesh_quarter_ring_sizes", (Nt, Nr), "cp_mesh_quarter_ring", (_f(a), _f(b), Nt, Nr, _f(ratio)))
 
 
+def renum_by_direction(mesh, dx=1.0, dy=1.0):
+    """Mesh::renumByDirection (Mesh.cu:537-630): a copy of `mesh` with its
+    nodes renumbered by increasing x dx + y dy."""
+    x, y, conn = mesh.x.copy(), mesh.y.copy(), mesh.conn.copy()
+    bord = np.concatenate(mesh.borders).astype(np.int32) if mesh.borders else np.zeros(0, dtype=np.int32)
+    check(lib().cp_mesh_renum_by_direction(mesh.nn, dp(x), dp(y), mesh.ne, mesh.npe, ip(conn), bord.size, ip(bord),
+                                           _f(dx), _f(dy)))
+    out, off = [], 0
+    for b in mesh.borders:
+        out.append(bord[off: off + b.size].copy())
+        off += b.size
+    sec = None
+    if mesh.sec is not None:
+        sec = np.zeros(mesh.nn, dtype=np.uint8)
+        sec[conn[:, 4:].ravel()] = 1
+    return Mesh(x, y, conn, out, npe=mesh.npe, sec=sec)
+
+
+def renumerate_ring(mesh, border_n):
+    """Mesh::renumerateRing (Mesh.cu:332-384) on a 4-node mesh: a renumbered copy."""
+    x, y, conn = mesh.x.copy(), mesh.y.copy(), mesh.conn.copy()
+    blen = np.array([b.size for b in mesh.borders], dtype=np.int32)
+    bord = np.concatenate(mesh.borders).astype(np.int32) if mesh.borders else np.zeros(0, dtype=np.int32)
+    check(lib().cp_mesh_renumerate_ring(mesh.nn, mesh.ne, dp(x), dp(y), ip(conn), blen.size, ip(bord), ip(blen),
+                                        int(border_n)))
+    out, off = [], 0
+    for n in blen:
+        out.append(bord[off: off + n].copy())
+        off += n
+    return Mesh(x, y, conn, out)
+
+
 def csr_pattern(mesh):
     nnz = C.c_longlong()
     check(lib().cp_csr_nnz_n(mesh.nn, mesh.ne, mesh.npe, ip(mesh.conn), C.byref(nnz)))

@@ -465,6 +465,66 @@ void Mesh::saveAsVtk(const std::string& fileName) const
     for (int i = 0; i < nodeCount; ++i) f << "\n" << i;
 }
 
+namespace {
+
+// Flat copies of the mesh arrays the C-ABI renumberings take.
+struct FlatMesh {
+    std::vector<double> x, y;
+    std::vector<int> bord;
+    void load(const Mesh& m)
+    {
+        x.resize(m.nodeCount);
+        y.resize(m.nodeCount);
+        for (int i = 0; i < m.nodeCount; ++i) {
+            x[i] = m.node[i].x;
+            y[i] = m.node[i].y;
+        }
+        bord.clear();
+        for (int b = 0; b < m.bordersCount; ++b) bord.insert(bord.end(), m.borders[b], m.borders[b] + m.borderLength[b]);
+    }
+    void store(Mesh& m) const
+    {
+        for (int i = 0; i < m.nodeCount; ++i) m.node[i] = vec2(x[i], y[i]);
+        size_t off = 0;
+        for (int b = 0; b < m.bordersCount; ++b) {
+            std::copy(bord.begin() + off, bord.begin() + off + m.borderLength[b], m.borders[b]);
+            off += m.borderLength[b];
+        }
+    }
+};
+
+}  // namespace
+
+void Mesh::renumerateRing(int borderN)
+{
+    if (count8 > 0 || count3 > 0) throw std::invalid_argument("renumerateRing: 4-node meshes only");
+    FlatMesh f;
+    f.load(*this);
+    check(cp_mesh_renumerate_ring(nodeCount, count4, f.x.data(), f.y.data(), elem4, bordersCount,
+                                  f.bord.data(), borderLength, borderN),
+          "renumerateRing");
+    f.store(*this);
+}
+
+void Mesh::renumByDirection(vec2 direction)
+{
+    if (count3 > 0) throw std::invalid_argument("renumByDirection: triangle meshes are not supported");
+    FlatMesh f;
+    f.load(*this);
+    const bool q8 = count8 > 0;
+    check(cp_mesh_renum_by_direction(nodeCount, f.x.data(), f.y.data(), q8 ? count8 : count4, q8 ? 8 : 4,
+                                     q8 ? elem8 : elem4, (int)f.bord.size(), f.bord.data(), direction.x,
+                                     direction.y),
+          "renumByDirection");
+    f.store(*this);
+    // mid-side nodes follow their elements (Mesh.cu:619-623)
+    if (secOrdNodes) {
+        std::fill(secOrdNodes, secOrdNodes + nodeCount, false);
+        for (int e = 0; e < count8; ++e)
+            for (int i = 4; i < 8; ++i) secOrdNodes[elem8[8 * e + i]] = true;
+    }
+}
+
 void Mesh::print() const
 {
     std::cout << "Nodes: " << nodeCount << "\n";

@@ -1009,7 +1009,10 @@ static __global__ __launch_bounds__(kBlock) void k_rm_decide(RmArgs a)
     reduce_partials<5>(a.partials, a.nblocks, v);
     if (threadIdx.x == 0) {
         RmScal* sc = a.sc;
-        const double rel = sqrt(v[0] / v[1]);
+        // unloaded state: sigma_i = sigma_tab = 0 everywhere, so 0/0; the
+        // reference's loop leaves there (NaN > tol is false), here the level
+        // is converged with rel_err = 0.  A NaN in either sum stays NaN.
+        const double rel = (v[0] == 0. && v[1] == 0.) ? 0. : sqrt(v[0] / v[1]);
         sc->rel_err = rel;
         const int nan = rel != rel;
         const int last = a.elastic || nan ||

@@ -918,6 +918,55 @@ int cp_mesh_ring2(double a, double b, int Nphi, int Nr, double* x, double* y, in
     return remap_into(nn, ne, X, Y, C, 2 * Nr, B, bord, x, y, conn, sec);
 }
 
+int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
+                            int borderN)
+{
+    if (nn < 1 || ne < 0 || !x || !y || (ne > 0 && !conn) || nb < 0 || (nb > 0 && (!bord || !blen)) || borderN < 1 ||
+        nn % borderN != 0)
+        return cp::fail(CP_ERR_INVALID, "cp_mesh_renumerate_ring: the node count must be a multiple of borderN");
+    for (int k = 0; k < 4 * ne; ++k)
+        if (conn[k] < 0 || conn[k] >= nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_renumerate_ring: node out of range");
+    renumber_ring(nn, ne, x, y, conn, nb, bord, blen, borderN);
+    return CP_OK;
+}
+
+// Mesh::renumByDirection (Mesh.cu:537-630): nodes renumbered in increasing
+// order of their projection x dx + y dy onto the direction.  The reference
+// sorts with a top-down merge sort that takes the element of the right run
+// on ties (Mesh.cu:505-535); that order is the order of (projection, original
+// number descending), which is what ties get here.
+int cp_mesh_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
+                               double dx, double dy)
+{
+    if (nn < 1 || !x || !y || ne < 0 || (npe != 4 && npe != 8) || (ne > 0 && !conn) || nbord < 0 ||
+        (nbord > 0 && !bord))
+        return cp::fail(CP_ERR_INVALID, "cp_mesh_renum_by_direction: bad arguments");
+    std::vector<double> key(nn);
+    for (int i = 0; i < nn; ++i) {
+        key[i] = x[i] * dx + y[i] * dy;
+        if (key[i] != key[i]) return cp::fail(CP_ERR_INVALID, "cp_mesh_renum_by_direction: NaN coordinate or direction");
+    }
+    for (long long k = 0; k < (long long)npe * ne; ++k)
+        if (conn[k] < 0 || conn[k] >= nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_renum_by_direction: node out of range");
+    for (int k = 0; k < nbord; ++k)
+        if (bord[k] < 0 || bord[k] >= nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_renum_by_direction: node out of range");
+    std::vector<int> old_at(nn);   // old node at each new position
+    for (int i = 0; i < nn; ++i) old_at[i] = i;
+    std::sort(old_at.begin(), old_at.end(), [&](int a, int b) { return key[a] < key[b] || (key[a] == key[b] && a > b); });
+    std::vector<int> perm(nn);
+    for (int i = 0; i < nn; ++i) perm[old_at[i]] = i;
+    for (long long k = 0; k < (long long)npe * ne; ++k) conn[k] = perm[conn[k]];
+    for (int k = 0; k < nbord; ++k) bord[k] = perm[bord[k]];
+    std::vector<double> nx(nn), ny(nn);
+    for (int i = 0; i < nn; ++i) {
+        nx[perm[i]] = x[i];
+        ny[perm[i]] = y[i];
+    }
+    std::memcpy(x, nx.data(), sizeof(double) * nn);
+    std::memcpy(y, ny.data(), sizeof(double) * nn);
+    return CP_OK;
+}
+
 }  // extern "C"
 
 namespace cp {

@@ -193,7 +193,11 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
         return fail(CP_ERR_INVALID, "cp_problem_create: null argument or empty mesh");
     if (npe != 4 && npe != 8) return fail(CP_ERR_INVALID, "cp_problem_create: elements must have 4 or 8 nodes");
     if (!(mat->E > 0) || !(mat->sigmaT > 0) || mat->law < 0 || mat->law > 2 || mat->plane < 0 || mat->plane > 1)
-        return fail(CP_ERR_INVALID, "cp_problem_create: invalid material");
+        return fail(CP_ERR_INVALID, "cp_problem_create: invalid material (E and sigmaT must be positive)");
+    // the elasticity matrix (formC, PlasticitySolver.cu:226-242) is positive
+    // definite only for -1 < nu < 1/2
+    if (!(mat->nu > -1.) || !(mat->nu < 0.5) || !(mat->h > 0))
+        return fail(CP_ERR_INVALID, "cp_problem_create: invalid material (need -1 < nu < 0.5 and h > 0)");
     if (nn > (1 << 30) / 2 || ne > (npe == 8 ? (1 << 24) : (1 << 29)))
         return fail(CP_ERR_INVALID, "cp_problem_create: mesh too large");
     Adjacency adj;
@@ -940,6 +944,8 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     const double inner = opt->inner_tol > 0 ? opt->inner_tol : 1e-3;
     int rc = CP_OK;
     double prev_rel = 1.0;
+    std::string lin_msg;
+    bool lin_last = false;
     for (int k = 0; k < max_outer; ++k) {
         CP_HIP(hipEventRecord(P->ev[0], stream));
         // multigrid: a fresh hierarchy at the first secant iteration of the
@@ -966,8 +972,14 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
             break;
         }
         if (s && s != CP_ERR_NOT_CONVERGED) return s;
+        // a linear solve that stopped at its cap (cg_maxit, or a stalled
+        // refinement) is not a tight one: the level cannot exit on it, and a
+        // level whose last iteration ends on one is not converged
+        const bool lin_ok = s == CP_OK;
+        if (!lin_ok) lin_msg = cp_last_error();
         CP_HIP(hipEventRecord(P->ev[2], stream));
-        if (int q = launch_return_map(P, k, opt->aitken, opt->fixed_outer, max_outer, elastic, opt->tol, tight))
+        if (int q = launch_return_map(P, k, opt->aitken, opt->fixed_outer, max_outer, elastic, opt->tol,
+                                      tight && lin_ok))
             return q;
         CP_HIP(hipMemcpyAsync(P->h_rm, P->rms, sizeof(cpk::RmScal), hipMemcpyDeviceToHost, stream));
         CP_HIP(hipEventRecord(P->ev[3], stream));
@@ -988,7 +1000,8 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
             break;
         }
         if (h.last) {
-            S.status = h.converged ? CP_OK : CP_ERR_NOT_CONVERGED;
+            S.status = h.converged && lin_ok ? CP_OK : CP_ERR_NOT_CONVERGED;
+            lin_last = !lin_ok;
             break;
         }
     }
@@ -1002,6 +1015,8 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
     P->solved = (S.status == CP_OK);
     if (st) *st = S;
     if (rc) return rc;
+    if (S.status == CP_ERR_NOT_CONVERGED && lin_last)
+        return failf(CP_ERR_NOT_CONVERGED, "the level's last linear solve did not converge (%s)", lin_msg.c_str());
     if (S.status == CP_ERR_NOT_CONVERGED) return fail(CP_ERR_NOT_CONVERGED, "secant loop hit its iteration cap");
     return CP_OK;
 }

@@ -93,7 +93,7 @@ typedef struct {
                             for (each asks for what the FP64 residual test still needs) */
     int    precond;      /* 0: unpreconditioned CG (the reference's); 1: 2x2 block-Jacobi PCG;
                             2: CG preconditioned by a geometric multigrid V-cycle (needs
-                            cp_problem_set_grid; not in the serving loop) */
+                            cp_problem_set_grid; served through cp_server_submit_grid) */
     int    predictor;    /* load stepping initial guess from the last two levels: 0 none (previous
                             level's state), 1 extrapolate u, 2 extrapolate u and log(psi) */
     double cg_forcing;   /* > 0: inexact secant iterations -- secant iteration k solves to
@@ -152,6 +152,18 @@ int cp_mesh_quarter_plate(double L, double r, int Nt, int Nr,
 int cp_mesh_quarter_ring_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen);
 int cp_mesh_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
                          double* x, double* y, int* conn, int* bord, int* blen);
+/* Mesh::renumerateRing (Mesh.cu:332-384) on a 4-node mesh, in place: the
+ * nn / borderN rows of borderN columns are numbered column by column,
+ * alternately from the first and the last column (the generators of plates
+ * with a hole and rings already apply it). */
+int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
+                            int borderN);
+/* Mesh::renumByDirection (Mesh.cu:537-630) in place: nodes renumbered by
+ * increasing x dx + y dy, ties in decreasing original number (the order the
+ * reference's merge sort, Mesh.cu:505-535, leaves them in); conn (npe = 4 or
+ * 8 nodes per element) and the nbord border entries follow. */
+int cp_mesh_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
+                               double dx, double dy);
 /* Mesh::findMaxIndexDiff (Mesh.cu:417-441) */
 int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
 /* Mesh::remapOrder (Mesh.cu:443-503): the order-1 mesh of doubled resolution

@@ -65,6 +65,15 @@ public:
     // ("Unknown mesh file format", "There is no such file").
     bool loadFromFile(const std::string& fileName);
 
+    // Mesh.cu:332-384: renumbers a 4-node mesh of nodeCount / borderN rows of
+    // borderN columns column by column, alternately from both ends
+    // (genRectWithHole and genRing already do).
+    void renumerateRing(int borderN);
+    // Mesh.cu:537-630: renumbers the nodes by increasing projection onto
+    // `direction` (ties in decreasing original number, as the reference's
+    // merge sort leaves them); elements, borders and secOrdNodes follow.
+    void renumByDirection(vec2 direction = {1., 1.});
+
     // Mesh.cu:417-441
     int findMaxIndexDiff() const;
 

@@ -21,6 +21,8 @@ struct vec2 {
     vec2& operator*=(double k) { x *= k; y *= k; return *this; }
     vec2& operator/=(double k) { x /= k; y /= k; return *this; }
 
+    // z component of the cross product (vec2.cuh:81-85)
+    double crossZ(const vec2& v) const { return x * v.y - y * v.x; }
     double norm() const { return std::sqrt(x * x + y * y); }
     vec2 normalize() const { return *this * (1. / norm()); }
 };

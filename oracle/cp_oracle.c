@@ -393,6 +393,49 @@ void or_gen_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
     for (int k = 0; k <= Nr; ++k) b1[k] = Nt * (Nr + 1) + Nr - k;         /* theta=pi/2, b -> a */
 }
 
+/* renumSort (Mesh.cu:505-535): top-down merge sort of the keys r with the
+ * node numbers n alongside; split at half, and on equal keys the merge takes
+ * the element of the right run. */
+static void renum_sort(double* r, int* n, int len, double* br, int* bn)
+{
+    if (len <= 1) return;
+    int half = len / 2;
+    renum_sort(r, n, half, br, bn);
+    renum_sort(r + half, n + half, len - half, br + half, bn + half);
+    int i = 0, j = half, k = 0;
+    while (i < half && j < len) {
+        if (r[i] < r[j]) { br[k] = r[i]; bn[k++] = n[i++]; }
+        else { br[k] = r[j]; bn[k++] = n[j++]; }
+    }
+    while (i < half) { br[k] = r[i]; bn[k++] = n[i++]; }
+    while (j < len) { br[k] = r[j]; bn[k++] = n[j++]; }
+    memcpy(r, br, sizeof(double) * len);
+    memcpy(n, bn, sizeof(int) * len);
+}
+
+/* renumByDirection (Mesh.cu:537-630): keys node . direction, sorted by
+ * renumSort; newNodes[oldNodes[i]] = i renumbers elements, borders, nodes. */
+void or_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
+                           double dx, double dy)
+{
+    double* r = (double*)malloc(sizeof(double) * nn);
+    double* br = (double*)malloc(sizeof(double) * nn);
+    int* oldn = (int*)malloc(sizeof(int) * nn);
+    int* bn = (int*)malloc(sizeof(int) * nn);
+    int* newn = (int*)malloc(sizeof(int) * nn);
+    double* nx = (double*)malloc(sizeof(double) * nn);
+    double* ny = (double*)malloc(sizeof(double) * nn);
+    for (int i = 0; i < nn; ++i) { oldn[i] = i; r[i] = x[i] * dx + y[i] * dy; }
+    renum_sort(r, oldn, nn, br, bn);
+    for (int i = 0; i < nn; ++i) newn[oldn[i]] = i;
+    for (long long k = 0; k < (long long)npe * ne; ++k) conn[k] = newn[conn[k]];
+    for (int k = 0; k < nbord; ++k) bord[k] = newn[bord[k]];
+    for (int i = 0; i < nn; ++i) { nx[newn[i]] = x[i]; ny[newn[i]] = y[i]; }
+    memcpy(x, nx, sizeof(double) * nn);
+    memcpy(y, ny, sizeof(double) * nn);
+    free(r); free(br); free(oldn); free(bn); free(newn); free(nx); free(ny);
+}
+
 /* findMaxIndexDiff -- Mesh.cu:417-441 (4-node part) */
 int or_max_index_diff(int ne, const int* conn)
 {
@@ -1062,6 +1105,47 @@ static void element_strain(int npe, const double* Bs, const int* c, const double
     *exx = ex * 0.25; *eyy = ey * 0.25; *gxy = gx * 0.25;
 }
 
+/* strain4 / strain8 + plastStress (:1517-1610) for every element, with the
+ * exitNorm2 sums (:1664-1691): sums = {sum (sigma_i - sigma_tab)^2,
+ * sum sigma_tab^2, and with rcur given the Aitken sums rr, dr, dd}. */
+static void secant_pass(int ne, int npe, const double* B, size_t bstride, const int* conn, const double* u,
+                        const or_material* mat, double* const* F, double* G, double* rcur, const double* rprev,
+                        double sums[5])
+{
+    double num = 0., den = 0., rr = 0., dr = 0., dd = 0.;
+#pragma omp parallel for reduction(+ : num, den, rr, dr, dd) schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        double exx, eyy, gxy;
+        element_strain(npe, B + bstride * e, conn + (size_t)npe * e, u, &exx, &eyy, &gxy);
+        double o[8];
+        or_secant_update(mat, F[OR_F_EC][e], F[OR_F_NUC][e], exx, eyy, gxy, o);
+        F[OR_F_EXX][e] = exx; F[OR_F_EYY][e] = eyy; F[OR_F_GXY][e] = gxy;
+        F[OR_F_SXX][e] = o[0]; F[OR_F_SYY][e] = o[1]; F[OR_F_TAU][e] = o[2];
+        F[OR_F_SIGI][e] = o[3]; F[OR_F_EPSI][e] = o[4]; F[OR_F_SIGTAB][e] = o[5];
+        G[e] = o[6];
+        double d = o[3] - o[5];
+        num += d * d;
+        den += o[5] * o[5];
+        if (rcur) {
+            double rc = log(o[6]) - log(F[OR_F_PSI][e]);
+            double q = rc - rprev[e];
+            rcur[e] = rc;
+            rr += rc * rc;
+            dr += rprev[e] * q;
+            dd += q * q;
+        }
+    }
+    sums[0] = num; sums[1] = den; sums[2] = rr; sums[3] = dr; sums[4] = dd;
+}
+
+/* exitCondition (:1722-1735): sqrt(num / den).  An unloaded body has
+ * sigma_i = sigma_tab = 0 everywhere; the reference's 0/0 = NaN leaves its
+ * loop (NaN > tol is false), here that state counts as converged (0). */
+static double exit_measure(double num, double den)
+{
+    return (num == 0. && den == 0.) ? 0. : sqrt(num / den);
+}
+
 int or_solve_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
                const double* rp, const unsigned char* mask, double* u,
                const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
@@ -1115,31 +1199,10 @@ int or_solve_n(int nn, int ne, int npe, const double* x, const double* y, const 
         stats->outer_iters = k + 1;
         if (cgs == OR_CG_NAN) { stats->status = 2; break; }
 
-        /* strain4 / strain8 + plastStress (:1590-1610) */
-        double num = 0., den = 0., rr = 0., dr = 0., dd = 0.;
-#pragma omp parallel for reduction(+ : num, den, rr, dr, dd) schedule(static)
-        for (int e = 0; e < ne; ++e) {
-            double exx, eyy, gxy;
-            element_strain(npe, B + bstride * e, conn + (size_t)npe * e, u, &exx, &eyy, &gxy);
-            double o[8];
-            or_secant_update(mat, F[OR_F_EC][e], F[OR_F_NUC][e], exx, eyy, gxy, o);
-            F[OR_F_EXX][e] = exx; F[OR_F_EYY][e] = eyy; F[OR_F_GXY][e] = gxy;
-            F[OR_F_SXX][e] = o[0]; F[OR_F_SYY][e] = o[1]; F[OR_F_TAU][e] = o[2];
-            F[OR_F_SIGI][e] = o[3]; F[OR_F_EPSI][e] = o[4]; F[OR_F_SIGTAB][e] = o[5];
-            G[e] = o[6];
-            double d = o[3] - o[5];
-            num += d * d;
-            den += o[5] * o[5];
-            if (opt->aitken) {
-                double rc = log(o[6]) - log(F[OR_F_PSI][e]);
-                double q = rc - rprev[e];
-                rcur[e] = rc;
-                rr += rc * rc;
-                dr += rprev[e] * q;
-                dd += q * q;
-            }
-        }
-        double relErr = sqrt(num / den);
+        double sums[5];
+        secant_pass(ne, npe, B, bstride, conn, u, mat, F, G, opt->aitken ? rcur : NULL, rprev, sums);
+        double num = sums[0], den = sums[1], rr = sums[2], dr = sums[3], dd = sums[4];
+        double relErr = exit_measure(num, den);
         stats->rel_err = relErr;
         if (relErr != relErr) { stats->status = 2; break; }
         int last = opt->elastic || (opt->fixed_outer > 0 ? (k + 1 == opt->fixed_outer)
@@ -1195,6 +1258,35 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
              const or_material* mat, const or_options* opt, double* st_, or_stats* stats)
 {
     return or_solve_n(nn, ne, 4, x, y, conn, rp, mask, u, mat, opt, st_, stats);
+}
+
+double or_return_map_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+                       const double* u, const or_material* mat, double* st_)
+{
+    (void)nn;
+    const int ngp = npe == 8 ? 9 : 4;
+    const size_t bstride = (size_t)ngp * 2 * npe;
+    double* B = (double*)malloc(sizeof(double) * bstride * (size_t)ne);
+    double* dJ = (double*)malloc(sizeof(double) * ngp * (size_t)ne);
+    double* G = (double*)malloc(sizeof(double) * ne);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; ++e) {
+        if (npe == 8) or_calc_b8(x, y, conn, e, (double (*)[2][8])(B + bstride * e), dJ + ngp * (size_t)e);
+        else or_calc_b4(x, y, conn, e, (double (*)[2][4])(B + bstride * e), dJ + ngp * (size_t)e);
+    }
+    double* F[OR_NFIELDS];
+    for (int f = 0; f < OR_NFIELDS; ++f) F[f] = st_ + (size_t)f * ne;
+    double sums[5];
+    secant_pass(ne, npe, B, bstride, conn, u, mat, F, G, NULL, NULL, sums);
+    /* the plain update of or_solve_n */
+    for (int e = 0; e < ne; ++e) {
+        double p = G[e];
+        F[OR_F_PSI][e] = p;
+        F[OR_F_EC][e] = mat->E / p;
+        F[OR_F_NUC][e] = 0.5 * (1. - (1. - 2. * mat->nu) / p);
+    }
+    free(B); free(dJ); free(G);
+    return exit_measure(sums[0], sums[1]);
 }
 
 /* deformEnergy -- PlasticitySolver.cuh:415-445.  4-node elements: the

@@ -18,6 +18,9 @@
  *   - CG and the plastic loop have iteration caps and a status code;
  *   - a zero intensity stress (sigma_tab == 0) keeps psi = 1 instead of
  *     producing 0/0 (PlasticitySolver.cu:1607);
+ *   - an unloaded body (sigma_i = sigma_tab = 0 everywhere) has exit
+ *     measure 0 instead of 0/0; the reference's loop leaves there as well,
+ *     since NaN > 1e-5 is false (PlasticitySolver.cu:503);
  *   - renumerateRing with an odd border count assigns the middle column
  *     correctly (the reference's Mesh.cu:347-349 writes the map reversed).
  *
@@ -82,6 +85,9 @@ void or_quarter_ring_sizes(int Nt, int Nr, int* nn, int* ne, int* nb, int* blen)
 void or_gen_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
                          double* x, double* y, int* conn, int* bord, int* blen);
 
+/* renumByDirection + renumSort (Mesh.cu:505-630), in place; npe = 4 or 8. */
+void or_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
+                           double dx, double dy);
 int    or_max_index_diff(int ne, const int* conn);             /* Mesh.cu:417-441 */
 double or_elem_space4(const double* x, const double* y, const int* conn, int e); /* Mesh.cuh:258-267 */
 double or_elem_space8(const double* x, const double* y, const int* conn, int e); /* Mesh.cuh:269-278 */
@@ -176,6 +182,15 @@ int or_solve(int nn, int ne, const double* x, const double* y, const int* conn,
 int or_solve_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
                const double* rp, const unsigned char* mask, double* u,
                const or_material* mat, const or_options* opt, double* state, or_stats* st);
+
+/* One secant iteration's update from a given u, as inside or_solve_n:
+ * strain4 / strain8 + plastStress (:1517-1610) of every element from the
+ * state's E_c, nu_c, then the plain update of psi, E_c, nu_c
+ * (updateParamDouble :1613-1625).  Returns the exit measure (exitCondition
+ * :1722-1735).  Lets a test pair the oracle's assembly and secant update
+ * with a direct sparse solve where its CG would take too long. */
+double or_return_map_n(int nn, int ne, int npe, const double* x, const double* y, const int* conn,
+                       const double* u, const or_material* mat, double* state);
 
 /* Secant update for one element given its strains (updateParameters :1320-1335). */
 void or_secant_update(const or_material* mat, double Ec, double nuc,

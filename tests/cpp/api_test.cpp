@@ -93,6 +93,23 @@ int run_cpu()
         dump_mesh("holecopy", copy);
     }
     {
+        // contactTest's renumbering (Main.cpp:1029: mesh.renumByDirection()
+        // with the default direction {1, 1}) on the fixture plate, and
+        // renumerateRing applied by hand to a rectangle
+        Mesh mesh;
+        mesh.genRectWithHole(-3., 3., -2., 2., 0.5, 12, 8, 1);
+        mesh.renumByDirection();
+        dump_mesh("renum", mesh);
+        Mesh rect;
+        rect.genRectangle(0., 1., 0., 1., 7, 5);
+        rect.renumerateRing(8);
+        dump_mesh("ringrenum", rect);
+        // vec2::crossZ (vec2.cuh:81-85)
+        vec2 a(3., -2.), b(0.5, 4.);
+        std::vector<double> cz = {a.crossZ(b), b.crossZ(a), a.crossZ(a)};
+        dump("crossz.f64", cz);
+    }
+    {
         // testRing (Main.cpp:75-106): position-dependent border force, h = 0.1
         Mesh mesh;
         mesh.genRing(0.1, 0.2, 40, 5);

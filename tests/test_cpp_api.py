@@ -158,6 +158,22 @@ def check_conditions(d, tag, prob):
     assert np.array_equal(rd(d, f"{tag}_u0.f64"), prob.u), tag
 
 
+def test_renumberings_and_crossz_through_cpp_api(cpu_out):
+    """Mesh::renumByDirection() with the default direction {1, 1} (as
+    Main.cpp:1029 calls it), Mesh::renumerateRing and vec2::crossZ through
+    the C++ classes, against the oracle's restatements."""
+    om = O.renum_by_direction(O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8), 1.0, 1.0)
+    assert np.array_equal(rd(cpu_out, "renum_xy.f64"), np.c_[om.x, om.y].ravel())
+    assert np.array_equal(rd(cpu_out, "renum_elem4.i32"), om.conn.ravel())
+    r = O.gen_rect(0, 1, 0, 1, 7, 5)
+    blen = np.array([b.size for b in r.borders], dtype=np.int32)
+    x, y, conn, bord = r.x.copy(), r.y.copy(), r.conn.copy(), np.concatenate(r.borders).astype(np.int32)
+    O.lib.or_renumerate_ring(r.nn, r.ne, O.dp(x), O.dp(y), O.ip(conn), blen.size, O.ip(bord), O.ip(blen), 8)
+    assert np.array_equal(rd(cpu_out, "ringrenum_xy.f64"), np.c_[x, y].ravel())
+    assert np.array_equal(rd(cpu_out, "ringrenum_elem4.i32"), conn.ravel())
+    assert rd(cpu_out, "crossz.f64").tolist() == [3.0 * 4.0 - (-2.0) * 0.5, 0.5 * -2.0 - 4.0 * 3.0, 0.0]
+
+
 def test_hole_plate_mesh_and_conditions(cpu_out):
     """plasticityTest's mesh (Main.cpp:256) and loads (:266-272)."""
     om = O.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 48, 32)

@@ -1,0 +1,109 @@
+"""The 632 x 316 quarter plate (199,712 quads, 401,322 DOF) at its full size,
+through the C-ABI.
+
+* Oracle parity: two plain secant iterations at the final 300 MPa level,
+  the FP64 GPU path (multigrid CG to 1e-13) against the oracle's assembly
+  and secant update with a direct sparse solve of each linear system
+  (oracle_lib.solve_direct; the oracle's unpreconditioned CG to the same
+  tolerance would need about 10^5 iterations per solve).  u and the stress
+  fields within 1e-8, plastic flags bit-exact.  The smallest distance of an
+  element's eps_i from the yield strain in this state is 3.7e-5 relative,
+  far above the 1e-8 agreement, so exact flags are a fair bar.
+* The 20-increment ramp to 300 MPa (linear hardening, K_T = E/101, Aitken to
+  1e-6) through size-independent checks: mixed within 1e-6 of FP64 with the
+  same options, every level converged, the same plastic zone, free-DOF
+  equilibrium and the reactions against the applied load.
+The reference's loop is solveCUDA (PlasticitySolver.cu:469-512)."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from cudaplasticity_amd import abi as A
+
+pytestmark = pytest.mark.gpu
+
+E = 2.1e5
+NT, NR = 632, 316
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = A.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - b) / np.linalg.norm(b))
+
+
+def case():
+    pm = A.gen_quarter_plate(1.0, 0.1, NT, NR)
+    pl = A.Loads(pm).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    return pm, pl, A.material(E, 0.3, 240.0, A.LAW_LINEAR, K_T=E / 101)
+
+
+def test_c2_two_secant_iterations_vs_oracle(ctx):
+    pm, pl, mat = case()
+    om = O.gen_quarter_plate(1.0, 0.1, NT, NR)
+    ol = O.Problem(om).force(1, 300.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    assert np.array_equal(pm.conn, om.conn) and np.array_equal(pl.rp, ol.rp) and np.array_equal(pl.mask, ol.mask)
+    print("case: plate 632x316, 300 MPa, fixed_outer=2, FP64 multigrid CG to 1e-13", flush=True)
+    P = A.Problem(ctx, pm, pl, mat)
+    P.set_grid(NT + 1, NR + 1)
+    st = P.solve(A.options(cg_tol=1e-13, fixed_outer=2, precond=2, cg_maxit=4000))
+    print("gpu: outer", st.outer_iters, "cg", st.cg_iters, "rel_err %.6e" % st.rel_err, flush=True)
+    rows, cols = P.csr()
+    orows, ocols = O.csr_pattern(om)
+    assert np.array_equal(rows[: orows.size], orows) and np.array_equal(cols, ocols)
+    u_o, s_o, rels = O.solve_direct(ol, O.material(E, 0.3, 240.0, O.LAW_LINEAR, K_T=E / 101), 2)
+    print("oracle (direct solves): exit measures", rels, flush=True)
+    eu = rel(P.field("u"), u_o)
+    f_o = O.fields(s_o)
+    errs = {k: rel(P.field(k), f_o[k]) for k in ("exx", "eyy", "gxy", "sxx", "syy", "tau", "sigI", "epsI", "psi")}
+    print("u %.3e" % eu, " ".join("%s %.2e" % kv for kv in errs.items()), flush=True)
+    assert eu < 1e-8
+    assert max(errs.values()) < 1e-8
+    assert abs(st.rel_err - rels[-1]) <= 1e-8 * rels[-1]
+    epsT = 240.0 / E
+    fg, fo = P.field("epsI") >= epsT, f_o["epsI"] >= epsT
+    print("plastic elements", int(fo.sum()), "of", pm.ne, flush=True)
+    assert np.array_equal(fg, fo)
+    P.close()
+
+
+def ramp(ctx, prec):
+    pm, pl, mat = case()
+    P = A.Problem(ctx, pm, pl, mat)
+    P.set_grid(NT + 1, NR + 1)
+    opt = A.options(cg_tol=1e-9, tol=1e-6, aitken=True, max_outer=500, precision=prec, precond=2, predictor=1,
+                    cg_forcing=0.3, inner_tol=1e-5)
+    per = P.solve_steps(20, 1.0, opt)
+    return pm, pl, mat, P, per
+
+
+def test_c2_full_ramp_mixed_vs_fp64(ctx):
+    print("case: plate 632x316, 20 increments to 300 MPa, mixed and FP64", flush=True)
+    pm, pl, mat, Pm, per_m = ramp(ctx, A.MIXED)
+    _, _, _, Pf, per_f = ramp(ctx, A.FP64)
+    print("mixed", [(s.outer_iters, s.cg_iters) for s in per_m], flush=True)
+    assert all(s.status == A.CP_OK and s.rel_err <= 1e-6 for s in per_m + per_f)
+    eu, es = rel(Pm.field("u"), Pf.field("u")), rel(Pm.field("sigI"), Pf.field("sigI"))
+    print("mixed vs fp64: u %.2e sigma_i %.2e" % (eu, es), flush=True)
+    assert eu < 1e-6 and es < 1e-6
+    epsT = mat.sigmaT / mat.E
+    fm, ff = Pm.field("epsI") >= epsT, Pf.field("epsI") >= epsT
+    print("plastic", int(ff.sum()), "of", pm.ne, "differing", int((fm != ff).sum()), flush=True)
+    assert (fm != ff).sum() <= 1e-4 * pm.ne
+    for P in (Pm, Pf):
+        f = P.internal_force()
+        free = pl.mask.astype(bool)
+        r = np.linalg.norm((f - pl.rp)[free]) / np.linalg.norm(pl.rp[free])
+        fixed_x = 2 * pm.borders[3]
+        react = f[fixed_x].sum()
+        applied = pl.rp[0::2].sum()
+        print("free-DOF residual %.2e, x reactions %.6e, applied %.6e" % (r, react, applied), flush=True)
+        assert r < 1e-6
+        assert abs(react + applied) <= 1e-6 * abs(applied)
+    Pm.close()
+    Pf.close()

@@ -158,8 +158,48 @@ def test_mixed_precision_error_vs_fp64(ctx):
     assert relnorm(P32.field("u"), P64.field("u")) < 1e-6
     for f in ["sxx", "sigI", "Ec"]:
         assert relnorm(P32.field(f), P64.field(f)) < 1e-6, f
+    # the same Swift-law ramp against the oracle's incremental loop; each
+    # loop stops once its exit measure is below 1e-9, so two correct runs
+    # agree to about that times the load path's amplification (bar 1e-7)
+    u_o, s_o, per_o = O.solve_steps(ol, omat, 4, O.options(cg_tol=1e-11, tol=1e-9, aitken=True, max_outer=2000))
+    f_o = O.fields(s_o)
+    print("oracle:", per_o, flush=True)
+    assert all(p[3] == 0 for p in per_o)
+    for P, bar in ((P64, 1e-7), (P32, 1e-6)):
+        assert relnorm(P.field("u"), u_o) < bar
+        for f in ["sxx", "syy", "tau", "sigI", "epsI"]:
+            assert relnorm(P.field(f), f_o[f]) < bar, f
     P64.close()
     P32.close()
+
+
+@pytest.mark.parametrize("precond", [0, 2])
+def test_swift_law_fixed_iterations_vs_oracle(ctx, precond):
+    """The Swift law sigma = sigma_Y (1 + eps_p/eps0)^n (the 2M-quad plate's
+    law) evaluated in the device return map, against the oracle: 8 plain
+    secant iterations at 300 MPa on a 60x30 quarter plate, FP64 within 1e-8
+    (plain and multigrid CG), mixed within 1e-6, plastic flags exact."""
+    pm, pl, pmat, om, ol, omat = quarter_plate_case(60, 30, 300.0, "swift")
+    print("case: swift plate 60x30, fixed_outer=8, precond", precond, flush=True)
+    u_o, s_o, st_o = O.solve(ol, omat, O.options(cg_tol=1e-13, fixed_outer=8))
+    f_o = O.fields(s_o)
+    epsT = 240.0 / 2.1e5
+    q = np.abs(f_o["epsI"] / epsT - 1.0)
+    print("oracle rel_err %.6e, plastic %d of %d, min flag margin %.2e" % (st_o.rel_err, (f_o["epsI"] >= epsT).sum(),
+                                                                         pm.ne, q.min()), flush=True)
+    for prec, bar in ((A.FP64, 1e-8), (A.MIXED, 1e-6)):
+        P = A.Problem(ctx, pm, pl, pmat)
+        if precond == 2:
+            P.set_grid(61, 31)
+        st = P.solve(A.options(cg_tol=1e-13 if prec == A.FP64 else 1e-11, fixed_outer=8, precision=prec,
+                               precond=precond, inner_tol=1e-5))
+        eu = relnorm(P.field("u"), u_o)
+        es = {f: relnorm(P.field(f), f_o[f]) for f in ["sxx", "syy", "tau", "sigI", "epsI", "sigTab", "psi"]}
+        print("precision", prec, "u %.2e" % eu, es, flush=True)
+        assert eu < bar and max(es.values()) < bar
+        assert abs(st.rel_err - st_o.rel_err) <= bar * st_o.rel_err
+        assert np.array_equal(P.field("epsI") >= epsT, f_o["epsI"] >= epsT)
+        P.close()
 
 
 def test_internal_force_equals_K_u(ctx):

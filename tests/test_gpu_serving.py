@@ -194,3 +194,43 @@ def test_budget_and_slots(ctx):
     code, info, _, _ = srv.result(big_id)
     assert code == A.CP_ERR_BUDGET and info.status == A.CP_ERR_BUDGET
     srv.close()
+
+
+def test_served_stream_full_range_two_rates(ctx):
+    """The config-5 size range at its ends: 14 requests drawn log-uniform over
+    2k-500k quads (plates and graded rings, both laws, mixed precision,
+    multigrid CG) plus a 500k-quad plate and a 500k-quad ring, served with
+    Poisson arrivals at two rates (all at once, and spread at about the
+    back-to-back service rate).  Each served result equals the problem
+    solved alone, bit for bit, at both rates."""
+    import time
+    from cudaplasticity_amd import serving
+    rng = np.random.Generator(np.random.PCG64(20261204))
+    stream = [serving.make_problem(rng, 2e3, 5e5) for _ in range(14)]
+    big = np.random.Generator(np.random.PCG64(5))
+    while len(stream) < 16:
+        mesh, loads, mat, desc = serving.make_problem(big, 4.9e5, 5e5)
+        if all(d["kind"] != desc["kind"] or d["quads"] < 4.5e5 for _, _, _, d in stream):
+            stream.append((mesh, loads, mat, desc))
+    quads = [d["quads"] for _, _, _, d in stream]
+    print("stream quads", sorted(quads), flush=True)
+    assert min(quads) < 1e4 and max(quads) > 4.8e5
+    nsteps = 3
+    opt = mg_opts(A.MIXED)
+    alone = []
+    t0 = time.time()
+    for mesh, loads, mat, desc in stream:
+        P = A.Problem(ctx, mesh, loads, mat)
+        P.set_grid(*serving.lattice(desc))
+        P.solve_steps(nsteps, 1.0, opt)
+        alone.append((P.field("u"), P.field("sigI")))
+        P.close()
+    span = time.time() - t0
+    print("alone back to back: %.2f s" % span, flush=True)
+    for rate in (0.0, len(stream) / span):
+        recs = serving.serve(ctx, stream, nsteps, opt, rate, hbm_budget=40 << 30, max_slots=8)
+        lat = sorted(r["latency"] for r in recs)
+        print("rate %.2f/s: p50 %.3f s, p99 %.3f s" % (rate, lat[len(lat) // 2], lat[-1]), flush=True)
+        for i, r in enumerate(recs):
+            assert r["code"] == A.CP_OK, (i, r["message"])
+            assert np.array_equal(r["u"], alone[i][0]) and np.array_equal(r["sig"], alone[i][1]), (rate, i)

@@ -264,3 +264,43 @@ def test_body_force_8node_matches_oracle_and_closed_form():
     w = O.elem_space(o, 0)
     np.testing.assert_allclose(l1.rp[2 * m.conn[0, :4] + 1], w / 12, rtol=1e-14)
     np.testing.assert_allclose(l1.rp[2 * m.conn[0, 4:] + 1], -w / 3, rtol=1e-14)
+
+
+@pytest.mark.parametrize("mesh_fn,d", [
+    (lambda M: M.gen_rect(0, 1, 0, 1, 12, 8), (1.0, 1.0)),          # ties on every anti-diagonal
+    (lambda M: M.gen_rect(0, 2, 0, 1, 9, 7), (0.0, 1.0)),           # ties along every row
+    (lambda M: M.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 12, 8), (1.0, 1.0)),
+    (lambda M: M.gen_ring(0.5, 1.0, 24, 6, order=2), (1.0, -0.5)),
+])
+def test_renum_by_direction_matches_oracle(mesh_fn, d):
+    """Mesh::renumByDirection (Mesh.cu:537-630) against the oracle's step by
+    step restatement of the reference's merge sort (renumSort,
+    Mesh.cu:505-535, right run first on ties): equal numbering, also where
+    many nodes have the same projection; the projections come out sorted
+    and the geometry (element areas) is unchanged."""
+    pm, om = mesh_fn(A), mesh_fn(O)
+    rp, ro = A.renum_by_direction(pm, *d), O.renum_by_direction(om, *d)
+    assert np.array_equal(rp.x, ro.x) and np.array_equal(rp.y, ro.y) and np.array_equal(rp.conn, ro.conn)
+    assert all(np.array_equal(a, b) for a, b in zip(rp.borders, ro.borders))
+    key = rp.x * d[0] + rp.y * d[1]
+    assert np.all(np.diff(key) >= 0)
+    n_ties = int((np.diff(np.sort(pm.x * d[0] + pm.y * d[1])) == 0).sum())
+    print("nodes", pm.nn, "tied neighbours", n_ties)
+    orig = [O.elem_space(O.Mesh(pm.x, pm.y, pm.conn, [], npe=pm.npe), e) for e in range(pm.ne)]
+    ren = [O.elem_space(O.Mesh(rp.x, rp.y, rp.conn, [], npe=pm.npe), e) for e in range(pm.ne)]
+    assert np.array_equal(orig, ren)
+    if pm.sec is not None:
+        assert rp.sec.sum() == pm.sec.sum() and np.all(rp.sec[rp.conn[:, 4:]] == 1)
+
+
+def test_renumerate_ring_matches_oracle_and_refuses_bad_counts():
+    m = A.gen_rect(0, 1, 0, 1, 7, 5)                                # 8 x 6 nodes
+    om = O.gen_rect(0, 1, 0, 1, 7, 5)
+    r = A.renumerate_ring(m, 8)
+    blen = np.array([b.size for b in om.borders], dtype=np.int32)
+    x, y, conn = om.x.copy(), om.y.copy(), om.conn.copy()
+    bord = np.concatenate(om.borders).astype(np.int32)
+    O.lib.or_renumerate_ring(om.nn, om.ne, O.dp(x), O.dp(y), O.ip(conn), blen.size, O.ip(bord), O.ip(blen), 8)
+    assert np.array_equal(r.x, x) and np.array_equal(r.conn, conn)
+    with pytest.raises(A.CPError):
+        A.renumerate_ring(m, 7)

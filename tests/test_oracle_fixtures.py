@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
+from cudaplasticity_amd import abi as A
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 E_304L, S_T_304L, NU_304L = 193e9, 170e6, 0.29
@@ -372,6 +373,30 @@ def test_lame_thick_cylinder():
     assert np.abs(ur - ur_exact).max() < 2e-3 * ur_exact.max()
     F = O.fields(st)
     assert p.rp.reshape(-1, 2)[m.borders[0]].sum(axis=0) @ np.array([1, 1]) > 0
+
+
+@pytest.mark.parametrize("ratio,Nt,Nr", [(1.05, 40, 60), (1.2, 12, 9), (0.9, 8, 7)])
+def test_graded_quarter_ring_closed_form(ratio, Nt, Nr):
+    """Radial grading of the quarter ring, pinned by its definition: radii
+    rho_0 = a, rho_Nr = b, successive element sizes growing by `ratio` from
+    the inner arc (a reversed growth direction fails the ratio test), every
+    node of a radial line at the same angle, the arcs at radii a and b, and
+    equal angular steps pi / (2 Nt)."""
+    a, b = 0.5, 1.0
+    m = O.gen_quarter_ring(a, b, Nt, Nr, ratio)
+    pm = A.gen_quarter_ring(a, b, Nt, Nr, ratio)
+    assert np.array_equal(m.x, pm.x) and np.array_equal(m.y, pm.y) and np.array_equal(m.conn, pm.conn)
+    rho = np.hypot(m.x, m.y).reshape(Nt + 1, Nr + 1)
+    th = np.arctan2(m.y, m.x).reshape(Nt + 1, Nr + 1)
+    assert np.abs(rho - rho[0]).max() < 1e-14                # every radial line has the same radii
+    assert rho[0, 0] == a and abs(rho[0, -1] - b) < 1e-15
+    h = np.diff(rho[0])
+    assert np.all(h > 0)
+    np.testing.assert_allclose(h[1:] / h[:-1], ratio, rtol=1e-10)
+    np.testing.assert_allclose(h[0], (b - a) * (ratio - 1) / (ratio ** Nr - 1), rtol=1e-12)
+    assert np.abs(th - th[:, :1]).max() < 1e-14
+    np.testing.assert_allclose(np.diff(th[:, 0]), np.pi / (2 * Nt), rtol=1e-12)
+    assert th[0, 0] == 0.0 and abs(th[-1, 0] - np.pi / 2) < 1e-15
 
 
 def test_quarter_plate_generator():
