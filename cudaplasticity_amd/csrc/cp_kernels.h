@@ -771,6 +771,10 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CgArgs<T> a, const typename V2<
 
 struct RefScal {
     double rnorm2;
+    // sum over free DOFs of (|K| |u|)^2, |K| |u| the internal force with
+    // every product taken in absolute value: the size of the rounding error
+    // of the FP64 residual is about eps64 times its norm (0: not computed)
+    double mag2;
     unsigned counter, pad;
 };
 
@@ -797,9 +801,9 @@ static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
 {
     const int lb = xcd_block(a.nblocks);
     const int n = lb * kBlock + threadIdx.x;
-    double v[1] = {0.};
+    double v[2] = {0., 0.};
     if (n < a.nn) {
-        double fx = 0., fy = 0.;
+        double fx = 0., fy = 0., mx = 0., my = 0.;
         for (int q = a.incPtr[n]; q < a.incPtr[n + 1]; ++q) {
             const int2 ic = a.inc[q];
             const int4 c = a.conn[ic.x];
@@ -816,7 +820,7 @@ static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
             }
             double c00, c01, c22;
             elastic_c(a.plane, en.x, en.y, c00, c01, c22);
-            double ex = 0., ey = 0.;
+            double ex = 0., ey = 0., ax = 0., ay = 0.;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 double B0[4], B1[4], dJ;
@@ -829,9 +833,23 @@ static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
                 const double bi0 = pick4(B0, i), bi1 = pick4(B1, i);
                 ex += dJ * (bi0 * sx + bi1 * tx);
                 ey += dJ * (bi1 * sy + bi0 * tx);
+                // the same sums with every product in absolute value
+                double qx = 0., qy = 0., qg = 0.;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    qx += fabs(B0[j] * ux[j]);
+                    qy += fabs(B1[j] * uy[j]);
+                    qg += fabs(B1[j] * ux[j]) + fabs(B0[j] * uy[j]);
+                }
+                const double px = fabs(c00) * qx + fabs(c01) * qy, py = fabs(c01) * qx + fabs(c00) * qy,
+                             pt = fabs(c22) * qg;
+                ax += fabs(dJ) * (fabs(bi0) * px + fabs(bi1) * pt);
+                ay += fabs(dJ) * (fabs(bi1) * py + fabs(bi0) * pt);
             }
             fx += ex * a.h;
             fy += ey * a.h;
+            mx += ax * fabs(a.h);
+            my += ay * fabs(a.h);
         }
         double rx = fx, ry = fy;
         if (a.b) {
@@ -840,18 +858,26 @@ static __global__ __launch_bounds__(kBlock) void k_residual64(ResArgs a)
             rx = mk.x ? bn.x - fx : 0.;
             ry = mk.y ? bn.y - fy : 0.;
             v[0] = rx * rx + ry * ry;
+            const double gx = mk.x ? mx + fabs(bn.x) : 0., gy = mk.y ? my + fabs(bn.y) : 0.;
+            v[1] = gx * gx + gy * gy;
         }
         a.r64[n] = make_double2(rx, ry);
         if (a.r32) a.r32[n] = make_float2((float)rx, (float)ry);
     }
-    if (a.sc) block_partial<1>(v, a.partials, lb);
+    if (a.sc) block_partial<2>(v, a.partials, lb);
 }
 
-static __global__ __launch_bounds__(kBlock) void k_residual_total(RefScal* sc, const double* partials, int nblocks)
+// Totals of the residual partials: (r, r) and, with two partials per block
+// (the 4-node residual), the squared magnitude of |K| |u| + |b|.
+template <int NV>
+__global__ __launch_bounds__(kBlock) void k_residual_total(RefScal* sc, const double* partials, int nblocks)
 {
-    double v[1];
-    reduce_partials<1>(partials, nblocks, v);
-    if (threadIdx.x == 0) sc->rnorm2 = v[0];
+    double v[NV];
+    reduce_partials<NV>(partials, nblocks, v);
+    if (threadIdx.x == 0) {
+        sc->rnorm2 = v[0];
+        sc->mag2 = NV > 1 ? v[NV - 1] : 0.;
+    }
 }
 
 // u += (double) d on every node (the refinement update).

@@ -1005,9 +1005,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_coarse_solve(const double* Dinv, 
 // ------------------------------------------------------------ coarse tail
 
 constexpr int kTailThreads = 1024;   // levels of at most this many nodes run in one workgroup
-// LDS of the tail: the six vectors of every tail level live there (the
-// host keeps the tail within it: at most 1536 nodes in FP64)
-constexpr int kTailLds = 144 * 1024;
+// LDS of the tail: the six vectors of every tail level (the host keeps
+// them within kTailVecLds: at most 1536 nodes in FP64) and, where they
+// fit in the rest, LDS copies of the levels' rows, pivot inverses and masks
+constexpr int kTailVecLds = 144 * 1024;
+constexpr int kTailMaxLevels = 16;
+// 160 KB per workgroup on gfx950, less the dense solve's right-hand side
+// and the levels' eigenvalue bounds
+constexpr int kTailLds = 160 * 1024 - (kMgDenseMax + kTailMaxLevels) * 8;
 
 template <typename T>
 struct TailLevel {
@@ -1015,9 +1020,20 @@ struct TailLevel {
     MgVec<T> v;        // vectors in global memory (v.b: right-hand side)
     MgGrid grid;       // to the next coarser level
     int lds;           // byte offset of the level's b, xa, xb, da, db, res in the tail's LDS
+    int mat_lds;       // byte offset of the LDS copy of the rows (slot-major, k nn + n), or -1
+    int minv_lds;      // byte offset of the LDS copy of the pivot inverses, or -1
+    int mask_lds;      // byte offset of the LDS copy of the mask, or -1
 };
 
-// Level l of the tail with its vectors pointed into the LDS arena.
+// LDS bytes of the copies of one tail level's rows, pivot inverses and mask.
+template <typename T>
+__host__ __device__ constexpr int tail_rows_bytes(int nn) { return (int)((9 * (size_t)nn * 4 * sizeof(T) + 15) / 16 * 16); }
+template <typename T>
+__host__ __device__ constexpr int tail_minv_bytes(int nn) { return (int)(((size_t)nn * 4 * sizeof(T) + 15) / 16 * 16); }
+__host__ __device__ constexpr int tail_mask_bytes(int nn) { return (int)((2 * (size_t)nn + 15) / 16 * 16); }
+
+// Level l of the tail with its vectors, and the copied pivot inverses and
+// mask, pointed into the LDS arena.
 template <typename T>
 __device__ __forceinline__ TailLevel<T> tail_level(const TailLevel<T>* lv, int l, char* lds)
 {
@@ -1031,49 +1047,201 @@ __device__ __forceinline__ TailLevel<T> tail_level(const TailLevel<T>* lv, int l
     L.v.da = base + 3 * nn;
     L.v.db = base + 4 * nn;
     L.v.res = base + 5 * nn;
+    if (L.minv_lds >= 0) L.a.minv = reinterpret_cast<const typename V4<T>::type*>(lds + L.minv_lds);
+    if (L.mask_lds >= 0) L.a.mask = reinterpret_cast<const uchar2*>(lds + L.mask_lds);
     return L;
 }
 
+// Block k of node n's row: the LDS copy (slot-major; FP64 as the two
+// planes one after the other) or the lattice-addressed global rows.
+template <typename T>
+__device__ __forceinline__ void tail_block(const TailLevel<T>& L, const char* lds, int n, int k, T (&b)[4])
+{
+    if (L.mat_lds >= 0) {
+        const size_t q = (size_t)k * L.a.nn + n;
+        if constexpr (std::is_same<T, float>::value) {
+            const float4 u = reinterpret_cast<const float4*>(lds + L.mat_lds)[q];
+            b[0] = u.x; b[1] = u.y; b[2] = u.z; b[3] = u.w;
+        } else {
+            const double2* pa = reinterpret_cast<const double2*>(lds + L.mat_lds);
+            const double2 u = pa[q], w = pa[q + (size_t)9 * L.a.nn];
+            b[0] = u.x; b[1] = u.y; b[2] = w.x; b[3] = w.y;
+        }
+    } else {
+        load_block(L.a, lat_base(n) + (size_t)k * kWave, b[0], b[1], b[2], b[3]);
+    }
+}
+
+// Copy the rows of level L into its slot-major LDS block.
+template <typename T>
+__device__ __forceinline__ void tail_copy_rows(const TailLevel<T>& L, char* lds, int tid)
+{
+    const int nn = L.a.nn;
+    for (int i = tid; i < 9 * nn; i += kTailThreads) {
+        const int k = i / nn, n = i - k * nn;
+        const size_t src = lat_base(n) + (size_t)k * kWave;
+        if constexpr (std::is_same<T, float>::value) {
+            reinterpret_cast<float4*>(lds + L.mat_lds)[i] = reinterpret_cast<const float4*>(L.a.va)[src];
+        } else {
+            double2* pa = reinterpret_cast<double2*>(lds + L.mat_lds);
+            pa[i] = reinterpret_cast<const double2*>(L.a.va)[src];
+            pa[i + 9 * nn] = reinterpret_cast<const double2*>(L.a.vb)[src];
+        }
+    }
+}
+
+// One smoothing step of node n of a tail level from its nine row blocks
+// (rowf(k, b) fills block k): the arithmetic of mg_smooth_node with lattice
+// addressing, the Chebyshev coefficients given.
+template <int MODE, typename T, typename RowF>
+__device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<T>& a, const MgVec<T>& v,
+                                                 const ChebCoef<T>& c, const typename V2<T>::type* xi,
+                                                 const typename V2<T>::type* di, typename V2<T>::type* xo,
+                                                 typename V2<T>::type* dout, int second, int last, int n)
+{
+    using V = typename V2<T>::type;
+    const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
+    const int i1 = n / a.lat_n2, i2 = n - i1 * a.lat_n2;
+    T yx = 0, yy = 0;
+    // FP64 blocks three at a time: nine in flight would not fit the 128
+    // registers of a 1024-thread workgroup
+    constexpr int kUnroll = std::is_same<T, double>::value ? 3 : 9;
+#pragma unroll kUnroll
+    for (int k = 0; k < 9; ++k) {
+        T m[4];
+        rowf(k, m);
+        const int nb = lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n);
+        T zx, zy;
+        if constexpr (MODE == MG_PRE1) {
+            const V bm = v.b[nb];
+            apply_minv<T>(a.minv[nb], bm.x, bm.y, zx, zy);
+            zx *= c.c0;
+            zy *= c.c0;
+        } else if constexpr (MODE == MG_POST1) {
+            const V xm = xi[nb];
+            zx = xm.x;
+            zy = xm.y;
+        } else {
+            const V xm = xi[nb], dm = di[nb];
+            zx = xm.x + dm.x;
+            zy = xm.y + dm.y;
+        }
+        yx += m[0] * zx + m[1] * zy;
+        yy += m[2] * zx + m[3] * zy;
+    }
+    const uchar2 mk = a.mask[n];
+    const V bn = v.b[n];
+    const T rx = mk.x ? bn.x - yx : (T)0, ry = mk.y ? bn.y - yy : (T)0;
+    const typename V4<T>::type mi = a.minv[n];
+    T gx, gy;
+    apply_minv<T>(mi, rx, ry, gx, gy);
+    if constexpr (MODE == MG_PRE1) {
+        T x1x, x1y;
+        apply_minv<T>(mi, bn.x, bn.y, x1x, x1y);
+        x1x *= c.c0;
+        x1y *= c.c0;
+        xo[n] = V{x1x, x1y};
+        dout[n] = V{c.a1 * x1x + c.b1 * gx, c.a1 * x1y + c.b1 * gy};
+    } else if constexpr (MODE == MG_POST1) {
+        dout[n] = V{c.c0 * gx, c.c0 * gy};
+    } else if constexpr (MODE == MG_CHEB) {
+        const V xn = xi[n], dn = di[n];
+        const V x = V{xn.x + dn.x, xn.y + dn.y};
+        const V d = V{ca * dn.x + cb * gx, ca * dn.y + cb * gy};
+        if (last) {
+            xo[n] = V{x.x + d.x, x.y + d.y};
+        } else {
+            xo[n] = x;
+            dout[n] = d;
+        }
+    } else {
+        const V xn = xi[n], dn = di[n];
+        xo[n] = V{xn.x + dn.x, xn.y + dn.y};
+        v.res[n] = V{rx, ry};
+    }
+}
+
 // Every level from the first one of at most kTailThreads nodes down to the
-// coarsest and back, in one workgroup: the same per-node work as the
-// level kernels, with workgroup barriers instead of launches between the
-// steps, and every vector of these levels in LDS (the matrices are read
-// from global memory).  lv[0] is the tail's finest level: its right-hand
-// side, restricted by the level above, is read from lv[0].v.b and its
-// result is written to lv[0].v.xb.
+// coarsest and back, in one workgroup: the same per-node work as the level
+// kernels, with workgroup barriers instead of launches between the steps.
+// Everything a step reads is on the chip: the vectors of every level in
+// LDS; each level's rows, pivot inverses and mask copied into LDS once per
+// V-cycle where the host found room (coarsest first); in FP32 the rows of
+// the tail's finest level in registers, one node per thread.  lv[0] is the
+// tail's finest level: its right-hand side, restricted by the level above,
+// is read from lv[0].v.b and its result is written to lv[0].v.xb.  The
+// coarsest level is solved by its explicit inverse, eight threads per row.
 template <typename T>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
                                                           const CgScal* sc)
 {
     using V = typename V2<T>::type;
-    // rows through lattice addressing in FP32; in FP64 (at most 128 VGPRs
-    // with 1024 threads) through col / slice, which needs fewer registers
-    constexpr int TW = std::is_same<T, double>::value ? 9 : kLatW;
+    constexpr bool kRegRows = std::is_same<T, float>::value;
     if (sc && sc->done) return;
     __shared__ __align__(16) char lds[kTailLds];
+    __shared__ double lam[kTailMaxLevels];
+    __shared__ double bs[kMgDenseMax];
     const int tid = threadIdx.x;
+    T r0[9][4];
     {
-        // right-hand side of the first tail level (restricted by the level above)
         const TailLevel<T> L0 = lv[0];
+        if constexpr (kRegRows) {
+            if (tid < L0.a.nn) {
+                const size_t base = lat_base(tid);
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    const float4 u = reinterpret_cast<const float4*>(L0.a.va)[base + (size_t)k * kWave];
+                    r0[k][0] = u.x; r0[k][1] = u.y; r0[k][2] = u.z; r0[k][3] = u.w;
+                }
+            }
+        }
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
         for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
+        for (int l = 0; l < nl; ++l) {
+            const TailLevel<T> L = lv[l];
+            if (tid == 0) lam[l] = *L.v.lam;
+            if (L.mat_lds >= 0) tail_copy_rows<T>(L, lds, tid);
+            if (L.minv_lds >= 0)
+                for (int n = tid; n < L.a.nn; n += kTailThreads)
+                    reinterpret_cast<typename V4<T>::type*>(lds + L.minv_lds)[n] = L.a.minv[n];
+            if (L.mask_lds >= 0)
+                for (int n = tid; n < L.a.nn; n += kTailThreads)
+                    reinterpret_cast<uchar2*>(lds + L.mask_lds)[n] = L.a.mask[n];
+        }
         __syncthreads();
     }
+    auto step = [&](const TailLevel<T>& L, int l, auto mode, const V* xi, const V* di, V* xo, V* dout, int second,
+                    int last) {
+        constexpr int MODE = decltype(mode)::value;
+        if (tid >= L.a.nn) return;
+        const ChebCoef<T> c = mg_cheb<T>(lam[l]);
+        if (kRegRows && l == 0) {
+            tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) m[j] = r0[k][j];
+            }, L.a, L.v, c, xi, di, xo, dout, second, last, tid);
+        } else {
+            tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi,
+                                      di, xo, dout, second, last, tid);
+        }
+    };
+    using Pre1 = std::integral_constant<int, MG_PRE1>;
+    using Cheb = std::integral_constant<int, MG_CHEB>;
+    using Resid = std::integral_constant<int, MG_RESID>;
+    using Post1 = std::integral_constant<int, MG_POST1>;
     for (int l = 0; l < nl - 1; ++l) {
         const TailLevel<T> L = tail_level<T>(lv, l, lds);
-        const int nn = L.a.nn;
-        if (tid < nn) mg_smooth_node<TW, MG_PRE1, T>(L.a, L.v, nullptr, nullptr, L.v.xa, L.v.da, 0, 0, tid);
+        step(L, l, Pre1(), nullptr, nullptr, L.v.xa, L.v.da, 0, 0);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.da, L.v.xb, L.v.db, 1, 0, tid);
+        step(L, l, Cheb(), L.v.xa, L.v.da, L.v.xb, L.v.db, 1, 0);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<TW, MG_RESID, T>(L.a, L.v, L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0, tid);
+        step(L, l, Resid(), L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0);
         __syncthreads();
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         if (tid < Cn.a.nn) mg_restrict_node<T>(L.grid, L.v.res, const_cast<V*>(Cn.v.b), Cn.a.mask, tid);
         __syncthreads();
     }
     {
-        __shared__ double bs[kMgDenseMax];
         const TailLevel<T> Cn = tail_level<T>(lv, nl - 1, lds);
         for (int i = tid; i < nd / 2; i += kTailThreads) {
             const V b = Cn.v.b[i];
@@ -1081,20 +1249,31 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
             bs[2 * i + 1] = b.y;
         }
         __syncthreads();
-        mg_coarse_rows<T>(Dinv, nd, bs, Cn.v.xb, tid, kTailThreads);
+        // row i = tid / 8 of the inverse, its columns j = tid % 8 + 8 t,
+        // summed over the eight lanes of the row in a fixed order
+        const int i = tid >> 3, j0 = tid & 7;
+        double s = 0.;
+        if (i < nd)
+            for (int j = j0; j < nd; j += 8) s += Dinv[(size_t)i * nd + j] * bs[j];
+        s += __shfl_xor(s, 1);
+        s += __shfl_xor(s, 2);
+        s += __shfl_xor(s, 4);
+        if (i < nd && j0 == 0) {
+            T* xc = reinterpret_cast<T*>(Cn.v.xb);
+            xc[i] = (T)s;
+        }
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
         const TailLevel<T> L = tail_level<T>(lv, l, lds);
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
-        const int nn = L.a.nn;
-        if (tid < nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
+        if (tid < L.a.nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<TW, MG_POST1, T>(L.a, L.v, L.v.xb, nullptr, nullptr, L.v.da, 0, 0, tid);
+        step(L, l, Post1(), L.v.xb, nullptr, nullptr, L.v.da, 0, 0);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xb, L.v.da, L.v.xa, L.v.db, 0, 0, tid);
+        step(L, l, Cheb(), L.v.xb, L.v.da, L.v.xa, L.v.db, 0, 0);
         __syncthreads();
-        if (tid < nn) mg_smooth_node<TW, MG_CHEB, T>(L.a, L.v, L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1, tid);
+        step(L, l, Cheb(), L.v.xa, L.v.db, L.v.xb, nullptr, 1, 1);
         __syncthreads();
     }
     {

@@ -153,9 +153,30 @@ static int mg_tail_table(cp_problem* P)
             t.grid.co1 = L.co1;
             t.grid.co2 = L.co2;
         }
+        t.mat_lds = t.minv_lds = t.mask_lds = -1;
         tab.push_back(t);
     }
-    if (lds > cpk::kTailLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
+    if (lds > cpk::kTailVecLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
+    // the rest of the LDS: copies of the pivot inverses and masks, then of
+    // the rows, coarsest level first (in FP32 the rows of the tail's finest
+    // level live in registers)
+    const int nt = (int)tab.size();
+    for (int i = nt - 1; i >= 0; --i) {
+        const int nn = tab[i].a.nn;
+        const int need = cpk::tail_minv_bytes<T>(nn) + cpk::tail_mask_bytes(nn);
+        if (lds + need > cpk::kTailLds) break;
+        tab[i].minv_lds = lds;
+        tab[i].mask_lds = lds + cpk::tail_minv_bytes<T>(nn);
+        lds += need;
+    }
+    for (int i = nt - 1; i >= 0; --i) {
+        if (i == 0 && std::is_same<T, float>::value) break;
+        const int need = cpk::tail_rows_bytes<T>(tab[i].a.nn);
+        if (lds + need > cpk::kTailLds) break;
+        tab[i].mat_lds = lds;
+        lds += need;
+    }
+    if (nt > cpk::kTailMaxLevels) return fail(CP_ERR_INVALID, "multigrid: too many coarse tail levels");
     cpk::TailLevel<T>* d = nullptr;
     if (int s = G.get(&d, tab.size())) return s;
     CP_HIP(hipMemcpy(d, tab.data(), sizeof(cpk::TailLevel<T>) * tab.size(), hipMemcpyHostToDevice));
@@ -847,7 +868,7 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     {
         int lds = tail_lds_bytes(G->L[G->lt].nn, sizeof(double2));
         while (G->lt > 1 && G->L[G->lt - 1].nn <= cpk::kTailThreads &&
-               lds + tail_lds_bytes(G->L[G->lt - 1].nn, sizeof(double2)) <= cpk::kTailLds) {
+               lds + tail_lds_bytes(G->L[G->lt - 1].nn, sizeof(double2)) <= cpk::kTailVecLds) {
             --G->lt;
             lds += tail_lds_bytes(G->L[G->lt].nn, sizeof(double2));
         }

@@ -124,6 +124,24 @@ cpk::ResArgs residual_args(cp_problem* P);
 int launch_residual(cp_problem* P, const cpk::ResArgs& ra, hipStream_t st);
 int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int maxit);
 int enqueue_residual(cp_problem* P);
+
+// One pass of the FP64 iterative refinement of the mixed-precision solve,
+// decided from the FP64 residual of the current u (the decision of cg_mixed
+// and of the serving loop's refinement phase, so that both take the same
+// passes).
+struct Refine {
+    double ref_norm = 0.;   // ||r_0||^2 of the solve (||b||^2, or the first residual when b = 0)
+    double prev = -1.;      // ||r||^2 of the previous pass
+    int stalled = 0;        // passes that reduced ||r||^2 by less than 4x
+};
+enum { REF_DONE = 0, REF_SOLVE = 1, REF_STALLED = 2, REF_NAN = 3 };
+// rn = ||r||^2, mag2 = ||(|K| |u| + |b|)||^2 over free DOFs (0 if unknown).
+// REF_DONE: the residual meets tol, or lies at the rounding floor of its own
+// FP64 evaluation (kRefFloor eps64 ||(|K| |u| + |b|)||), below which no pass
+// can be verified; REF_SOLVE: run one FP32 solve asking for *itol;
+// REF_STALLED: three passes without progress above that floor.
+constexpr double kRefFloor = 16.;
+int refine_decide(Refine& R, double rn, double mag2, double tol, double inner_tol, double* itol);
 int enqueue_correction(cp_problem* P);
 int enqueue_plastic(cp_problem* P);
 int enqueue_rm_reset(cp_problem* P);

@@ -63,8 +63,10 @@ struct Request {
     bool pending = false;            // multigrid requests: a readback is in flight
     int phase = PH_LEVEL;
     int level = 0, k = 0, tight = 1;
-    double prev_rel = 1., ctol = 0., ref_norm = 0., bnorm2 = 0., prev_rn = -1., itol = 0., rn = 0.;
-    int stalled = 0;
+    double prev_rel = 1., ctol = 0., bnorm2 = 0., itol = 0., rn = 0.;
+    cp::Refine refine;               // refinement passes of the current mixed-precision solve
+    bool lin_ok = true;              // the current secant iteration's linear solve converged
+    std::string lin_msg;             // why it did not
     long long cg_iters = 0;
     int outer_total = 0;
     int status = CP_OK;
@@ -224,6 +226,8 @@ int start_iteration(Request* r)
     r->ctol = ctol;
     r->tight = ctol <= r->opt.cg_tol;
     r->bnorm2 = cp::host_bnorm2(P);
+    r->lin_ok = true;
+    r->lin_msg.clear();
     const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : (P->precond == 2 ? 2000 : 1000000000);
     if (r->opt.precision == CP_FP64) {
         if (int s = cp::enqueue_cg_init(P, CP_FP64, r->bnorm2, ctol, maxit)) return s;
@@ -234,9 +238,8 @@ int start_iteration(Request* r)
         r->cg_iters_seen = 0;
         r->next_chunk = P->precond == 2 ? cp::mg_cg_first_chunk(P, ctol * ctol, false) : 16;
     } else {
-        r->ref_norm = r->bnorm2;
-        r->prev_rn = -1.;
-        r->stalled = 0;
+        r->refine = cp::Refine();
+        r->refine.ref_norm = r->bnorm2;
         if (int s = cp::enqueue_residual(P)) return s;
         r->phase = PH_WAIT_RES;
     }
@@ -278,8 +281,10 @@ int advance(cp_server* S, Request* r, const Mirror& m)
         P->last_cg_iters = c.iters;
         if (P->precond == 2) cp::mg_cg_observe(P, c);
         if (r->opt.precision == CP_FP64) {
+            // as cp_solve: a solve that stopped at cg_maxit is not a tight one
+            r->lin_ok = c.status == 0;
             if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
-                                              r->opt.tol, r->tight))
+                                              r->opt.tol, r->tight && r->lin_ok))
                 return s;
             r->phase = PH_WAIT_RM;
         } else {
@@ -301,20 +306,21 @@ int advance(cp_server* S, Request* r, const Mirror& m)
     }
     case PH_WAIT_RES: {
         const double rn = m.ref.rnorm2;
-        if (rn != rn) return fail_request(S, r, cp::fail(CP_ERR_NAN, "refinement: NaN in the residual"));
-        if (r->ref_norm == 0.) r->ref_norm = rn;
-        if (rn == 0. || rn <= r->ctol * r->ctol * r->ref_norm || r->stalled >= 3) {
+        const double inner = r->opt.inner_tol > 0 ? r->opt.inner_tol : 1e-3;
+        double itol = 0.;
+        const int act = cp::refine_decide(r->refine, rn, m.ref.mag2, r->ctol, inner, &itol);
+        if (act == cp::REF_NAN) return fail_request(S, r, cp::fail(CP_ERR_NAN, "refinement: NaN in the residual"));
+        if (act == cp::REF_DONE || act == cp::REF_STALLED) {
+            // as cp_solve: a stalled refinement is not a tight solve
+            r->lin_ok = act == cp::REF_DONE;
+            if (!r->lin_ok)
+                r->lin_msg = "refinement stalled at relative residual " + std::to_string(std::sqrt(rn / r->refine.ref_norm));
             if (int s = cp::launch_return_map(P, r->k, r->opt.aitken, r->opt.fixed_outer, max_outer_of(r), 0,
-                                              r->opt.tol, r->tight && r->stalled < 3))
+                                              r->opt.tol, r->tight && r->lin_ok))
                 return s;
             r->phase = PH_WAIT_RM;
             return CP_OK;
         }
-        if (r->prev_rn > 0. && rn > 0.25 * r->prev_rn) ++r->stalled;
-        r->prev_rn = rn;
-        const double inner = r->opt.inner_tol > 0 ? r->opt.inner_tol : 1e-3;
-        const double need = 0.5 * r->ctol * std::sqrt(r->ref_norm / rn);
-        const double itol = std::max(inner, std::min(need, 0.5));
         r->itol = itol;
         r->rn = rn;
         const int maxit = r->opt.cg_maxit > 0 ? r->opt.cg_maxit : (P->precond == 2 ? 2000 : 1000000000);
@@ -337,6 +343,9 @@ int advance(cp_server* S, Request* r, const Mirror& m)
             return start_iteration(r);
         }
         if (!h.converged) return fail_request(S, r, cp::fail(CP_ERR_NOT_CONVERGED, "secant iterations did not converge"));
+        if (!r->lin_ok)
+            return fail_request(S, r, cp::failf(CP_ERR_NOT_CONVERGED, "the level's last linear solve did not converge (%s)",
+                                                r->lin_msg.empty() ? "CG: cg_maxit iterations" : r->lin_msg.c_str()));
         // end of a level, as in cp_solve_steps: plastic strains, then the next level
         if (int s = cp::enqueue_plastic(P)) return s;
         P->solved = true;

@@ -753,10 +753,31 @@ int enqueue_cg_init(cp_problem* P, int precision, double bnorm2, double tol, int
 int enqueue_residual(cp_problem* P)
 {
     if (int s = launch_residual(P, residual_args(P), P->stream)) return s;
-    hipLaunchKernelGGL(cpk::k_residual_total, dim3(1), dim3(cpk::kBlock), 0, P->stream, P->refs,
-                       (const double*)P->partials, P->nbn);
+    // the 4-node residual also sums the magnitude of |K| |u| + |b|
+    if (P->npe == 4)
+        hipLaunchKernelGGL(cpk::k_residual_total<2>, dim3(1), dim3(cpk::kBlock), 0, P->stream, P->refs,
+                           (const double*)P->partials, P->nbn);
+    else
+        hipLaunchKernelGGL(cpk::k_residual_total<1>, dim3(1), dim3(cpk::kBlock), 0, P->stream, P->refs,
+                           (const double*)P->partials, P->nbn);
     CP_HIP(hipGetLastError());
     return CP_OK;
+}
+
+int refine_decide(Refine& R, double rn, double mag2, double tol, double inner_tol, double* itol)
+{
+    if (rn != rn) return REF_NAN;
+    if (R.ref_norm == 0.) R.ref_norm = rn;
+    const double eps = 2.220446049250313e-16;
+    const double floor2 = kRefFloor * kRefFloor * eps * eps * mag2;
+    if (rn == 0. || rn <= tol * tol * R.ref_norm || rn <= floor2) return REF_DONE;
+    if (R.prev > 0. && rn > 0.25 * R.prev && ++R.stalled >= 3) return REF_STALLED;
+    R.prev = rn;
+    // ask one FP32 solve for the reduction still needed (half of it, for
+    // margin), but never for less than inner_tol: FP32 cannot go further
+    const double need = 0.5 * tol * std::sqrt(R.ref_norm / rn);
+    *itol = std::max(inner_tol, std::min(need, 0.5));
+    return REF_SOLVE;
 }
 
 int enqueue_correction(cp_problem* P)
@@ -796,28 +817,24 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
     *iters = 0;
     *refs = 0;
     *ms = 0.;
-    double ref_norm = bn;
-    int stalled = 0;
-    double prev = -1.;
+    Refine R;
+    R.ref_norm = bn;
     for (int k = 0; k < 200; ++k) {
         if (int s = enqueue_residual(P)) return s;
         CP_HIP(hipMemcpyAsync(P->h_ref, P->refs, sizeof(cpk::RefScal), hipMemcpyDeviceToHost, st));
         CP_HIP(hipStreamSynchronize(st));
         const double rn = P->h_ref->rnorm2;
         ++*refs;
-        if (rn != rn) return fail(CP_ERR_NAN, "refinement: NaN residual");
-        if (ref_norm == 0.) ref_norm = rn;
-        if (relres) *relres = ref_norm > 0 ? std::sqrt(rn / ref_norm) : 0.;
-        if (rn == 0. || rn <= tol * tol * ref_norm) return CP_OK;
-        if (prev > 0. && rn > 0.25 * prev) {
-            if (++stalled >= 3) return failf(CP_ERR_NOT_CONVERGED, "refinement stalled at relative residual %.3e",
-                                             std::sqrt(rn / ref_norm));
-        }
-        prev = rn;
-        // ask one FP32 solve for the reduction still needed (half of it, for
-        // margin), but never for less than inner_tol: FP32 cannot go further
-        const double need = 0.5 * tol * std::sqrt(ref_norm / rn);
-        const double itol = std::max(inner_tol, std::min(need, 0.5));
+        double itol = 0.;
+        const int act = refine_decide(R, rn, P->h_ref->mag2, tol, inner_tol, &itol);
+        if (act == REF_NAN) return fail(CP_ERR_NAN, "refinement: NaN residual");
+        if (relres) *relres = R.ref_norm > 0 ? std::sqrt(rn / R.ref_norm) : 0.;
+        if (act == REF_DONE) return CP_OK;
+        if (act == REF_STALLED)
+            return failf(CP_ERR_NOT_CONVERGED,
+                         "refinement stalled at relative residual %.3e (rounding floor of the FP64 residual %.3e)",
+                         std::sqrt(rn / R.ref_norm),
+                         kRefFloor * 2.220446049250313e-16 * std::sqrt(P->h_ref->mag2 / R.ref_norm));
         if (int s = enqueue_cg_init(P, CP_MIXED, rn, itol, maxit)) return s;
         long long it = 0;
         int status = 0;
@@ -833,7 +850,7 @@ static int cg_mixed(cp_problem* P, double tol, int maxit, double inner_tol, long
         if (int s = enqueue_correction(P)) return s;
         const cpk::CgScal& h = *P->h_cg;
         if (loose && itol > inner_tol && status == 0 && h.done && h.rho0 > 0. && h.rho <= itol * itol * h.rho0) {
-            if (relres) *relres = std::sqrt(rn * (h.rho / h.rho0) / ref_norm);
+            if (relres) *relres = std::sqrt(rn * (h.rho / h.rho0) / R.ref_norm);
             return CP_OK;
         }
     }

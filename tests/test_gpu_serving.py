@@ -172,6 +172,61 @@ def test_failed_requests_carry_their_message(ctx):
     srv.close()
 
 
+@pytest.mark.parametrize("precond", [1])
+def test_stalled_refinement_same_alone_and_served(ctx, precond):
+    """A mixed-precision solve whose refinement stalls (block-Jacobi CG,
+    inner_tol 0.9: each FP32 solve reduces the residual by 10 % only) takes
+    the same passes alone and served: cp_solve and the serving loop share the
+    refinement decision (one multigrid iteration reduces the residual far
+    more than that, so the multigrid path does not stall this way).  The
+    level's last linear solve did not converge, so both report
+    CP_ERR_NOT_CONVERGED after the same secant and CG iterations."""
+    (m, l, mat), grid, n, p = MG_CASES[0]
+    stall = A.options(cg_tol=1e-9, tol=1e-6, fixed_outer=3, precision=A.MIXED, precond=precond, inner_tol=0.9)
+    P = A.Problem(ctx, m, l, mat)
+    if precond == 2:
+        P.set_grid(*grid)
+    P.set_load(1.0 / n)
+    st = A.Stats()
+    code = A.lib().cp_solve(P.ptr, A.C.byref(stall), A.C.byref(st))
+    msg = A.lib().cp_last_error().decode()
+    print("alone: code", code, "outer", st.outer_iters, "cg", st.cg_iters, msg, flush=True)
+    assert code == A.CP_ERR_NOT_CONVERGED and "stalled" in msg
+    P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 32, max_slots=2)
+    rid = srv.submit(m, l, mat, n, 1.0, stall, grid=grid if precond == 2 else None)
+    srv.run_until_done()
+    code, info, _, _ = srv.result(rid)
+    print("served: code", code, "outer", info.outer_iters, "cg", info.cg_iters, srv.last_message, flush=True)
+    assert code == A.CP_ERR_NOT_CONVERGED and "stalled" in srv.last_message
+    assert (info.outer_iters, info.cg_iters) == (st.outer_iters, st.cg_iters)
+    srv.close()
+
+
+def test_refinement_floor_same_alone_and_served(ctx):
+    """cg_tol 1e-15 lies below what an FP64 residual of this system can
+    show: the refinement ends at the rounding floor of its own FP64
+    evaluation (16 eps64 ||(|K| |u| + |b|)||) and the level converges, with
+    the same bits alone and served (multigrid CG)."""
+    (m, l, mat), grid, n, p = MG_CASES[0]
+    opt = A.options(cg_tol=1e-15, tol=1e-6, aitken=True, max_outer=100, precision=A.MIXED, precond=2,
+                    inner_tol=1e-5)
+    P = A.Problem(ctx, m, l, mat)
+    P.set_grid(*grid)
+    per = P.solve_steps(2, 1.0, opt)
+    print("alone:", [(s.outer_iters, s.cg_iters, s.refinements) for s in per], flush=True)
+    assert all(s.status == A.CP_OK for s in per)
+    u, sig = P.field("u"), P.field("sigI")
+    P.close()
+    srv = A.Server(ctx, hbm_budget=1 << 32, max_slots=2)
+    rid = srv.submit(m, l, mat, 2, 1.0, opt, grid=grid)
+    srv.run_until_done()
+    code, info, su, ssig = srv.result(rid)
+    assert code == A.CP_OK, srv.last_message
+    assert np.array_equal(su, u) and np.array_equal(ssig, sig)
+    srv.close()
+
+
 def test_budget_and_slots(ctx):
     (m, l, mat), _, _ = CASES[0]
     import ctypes as C
@@ -197,34 +252,43 @@ def test_budget_and_slots(ctx):
 
 
 def test_served_stream_full_range_two_rates(ctx):
-    """The config-5 size range at its ends: 14 requests drawn log-uniform over
+    """The config-5 size range at its ends: 8 requests drawn log-uniform over
     2k-500k quads (plates and graded rings, both laws, mixed precision,
-    multigrid CG) plus a 500k-quad plate and a 500k-quad ring, served with
-    Poisson arrivals at two rates (all at once, and spread at about the
-    back-to-back service rate).  Each served result equals the problem
-    solved alone, bit for bit, at both rates."""
+    multigrid CG, 5 increments each as in the bench's stream) plus a
+    500k-quad plate and a 500k-quad ring, served with Poisson arrivals at two
+    rates (all at once, and spread at about the back-to-back service rate).
+    Each served result equals the problem solved alone, bit for bit, at both
+    rates."""
     import time
     from cudaplasticity_amd import serving
     rng = np.random.Generator(np.random.PCG64(20261204))
-    stream = [serving.make_problem(rng, 2e3, 5e5) for _ in range(14)]
+    stream = [serving.make_problem(rng, 2e3, 5e5) for _ in range(8)]
+    # one plate and one ring of about 500k quads: the first of each kind
+    # drawn from a second generator (each draw is a plate with probability 1/2)
     big = np.random.Generator(np.random.PCG64(5))
-    while len(stream) < 16:
+    kinds = {}
+    for _ in range(64):
         mesh, loads, mat, desc = serving.make_problem(big, 4.9e5, 5e5)
-        if all(d["kind"] != desc["kind"] or d["quads"] < 4.5e5 for _, _, _, d in stream):
-            stream.append((mesh, loads, mat, desc))
+        kinds.setdefault(desc["kind"], (mesh, loads, mat, desc))
+        if len(kinds) == 2:
+            break
+    assert len(kinds) == 2
+    stream += [kinds["plate"], kinds["ring"]]
     quads = [d["quads"] for _, _, _, d in stream]
     print("stream quads", sorted(quads), flush=True)
     assert min(quads) < 1e4 and max(quads) > 4.8e5
-    nsteps = 3
+    nsteps = 5
     opt = mg_opts(A.MIXED)
     alone = []
     t0 = time.time()
     for mesh, loads, mat, desc in stream:
+        ts = time.time()
         P = A.Problem(ctx, mesh, loads, mat)
         P.set_grid(*serving.lattice(desc))
         P.solve_steps(nsteps, 1.0, opt)
         alone.append((P.field("u"), P.field("sigI")))
         P.close()
+        print("  alone: %s %d quads %.2f s" % (desc["kind"], desc["quads"], time.time() - ts), flush=True)
     span = time.time() - t0
     print("alone back to back: %.2f s" % span, flush=True)
     for rate in (0.0, len(stream) / span):
