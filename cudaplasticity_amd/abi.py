@@ -476,6 +476,13 @@ class Problem:
         check(lib().cp_internal_force(self.ptr, dp(f)))
         return f
 
+    def internal_force_noalloc(self):
+        """cp_internal_force_noalloc: the same force, staged in the library's
+        static device array (no device allocation)."""
+        f = np.zeros(2 * self.mesh.nn)
+        check(lib().cp_internal_force_noalloc(self.ptr, dp(f)))
+        return f
+
     def last_cg_ms_per_iter(self):
         ms = C.c_double()
         check(lib().cp_last_cg_kernel_ms(self.ptr, C.byref(ms)))

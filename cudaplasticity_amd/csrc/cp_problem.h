@@ -18,6 +18,9 @@
 struct cp_ctx {
     int device;
     hipDeviceProp_t prop;
+    // device address of this GPU's copy of the library's static staging
+    // array (cp_internal_force_noalloc), from hipGetSymbolAddress
+    void* scratch = nullptr;
 };
 
 namespace cp {

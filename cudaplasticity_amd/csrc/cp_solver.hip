@@ -21,6 +21,13 @@
 namespace {
 thread_local std::string g_err;
 
+// Device staging of cp_internal_force_noalloc: the internal force of a
+// problem of up to kScratchBytes / 16 nodes.  Each GPU holds its own copy
+// until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (cp_ctx::scratch).
+constexpr size_t kScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -1074,6 +1081,12 @@ int cp_ctx_create(int device, cp_ctx** out)
         delete c;
         return cp::fail(CP_ERR_DEVICE, "hipGetDeviceProperties failed");
     }
+    // this device's copy of the staging array (device `device` is current)
+    const hipError_t e = hipGetSymbolAddress(&c->scratch, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !c->scratch) {
+        delete c;
+        return cp::failf(CP_ERR_DEVICE, "hipGetSymbolAddress of the staging array failed: %s", hipGetErrorString(e));
+    }
     *out = c;
     return CP_OK;
 }
@@ -1155,6 +1168,34 @@ int cp_problem_get_csr(cp_problem* P, int* rows, int* cols)
     adj.ptr = P->adj_ptr;
     adj.node = P->adj_node;
     cp::export_csr(adj, P->nn, rows, cols);
+    return CP_OK;
+}
+
+int cp_internal_force_noalloc(cp_problem* P, double* f_host)
+{
+    if (!P || !f_host) return cp::fail(CP_ERR_INVALID, "null argument");
+    if ((size_t)P->nn * sizeof(double2) > kScratchBytes)
+        return cp::failf(CP_ERR_INVALID,
+                         "cp_internal_force_noalloc: %d nodes exceed the %zu-byte device staging (at most %zu nodes)",
+                         P->nn, kScratchBytes, kScratchBytes / sizeof(double2));
+    if (!P->ctx || !P->ctx->scratch) return cp::fail(CP_ERR_STATE, "cp_internal_force_noalloc: no staging address");
+    // every entry of the staged force is written by the kernel: nothing to clear
+    double2* f_dev = static_cast<double2*>(P->ctx->scratch);
+    cpk::ResArgs ra{};
+    ra.incPtr = P->inc_ptr;
+    ra.inc = P->inc;
+    ra.conn = P->conn;
+    ra.xy = P->xy;
+    ra.ecnu = P->ecnu;
+    ra.u = P->u;
+    ra.r64 = f_dev;
+    ra.nn = P->nn;
+    ra.nblocks = P->nbn;
+    ra.plane = P->M.plane;
+    ra.h = P->M.h;
+    if (int s = cp::launch_residual(P, ra, P->stream)) return s;
+    CP_HIP(hipMemcpyAsync(f_host, f_dev, sizeof(double2) * P->nn, hipMemcpyDeviceToHost, P->stream));
+    CP_HIP(hipStreamSynchronize(P->stream));
     return CP_OK;
 }
 

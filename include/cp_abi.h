@@ -318,6 +318,15 @@ int cp_return_map(cp_problem* prob, double* rel_err);
 int cp_plastic_strains(cp_problem* prob);
 /* f = K(E_c, nu_c) u computed matrix-free in FP64 (element internal force). */
 int cp_internal_force(cp_problem* prob, double* f_host);
+/* The same f with no device allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  cp_internal_force may
+ * allocate the problem's mixed-precision arena on first use to stage f;
+ * this call stages it in the library's static 8 MiB device array instead,
+ * so problems of more than 524,288 nodes are refused with CP_ERR_INVALID.
+ * The array is shared by every call on one GPU (its copy is held until the
+ * process ends): calls that overlap on one GPU, from several threads, race. */
+int cp_internal_force_noalloc(cp_problem* prob, double* f_host);
 
 /* PlasticitySolver::solve / solveElast for one load level (solveCUDA :469-512). */
 int cp_solve(cp_problem* prob, const cp_options* opt, cp_stats* stats);
