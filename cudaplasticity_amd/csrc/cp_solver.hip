@@ -19,8 +19,6 @@
 #include "cp_q8.h"
 
 namespace {
-thread_local std::string g_err;
-
 // Device staging of cp_internal_force_noalloc: the internal force of a
 // problem of up to kScratchBytes / 16 nodes.  Each GPU holds its own copy
 // until the process ends; host code reaches it only through the address
@@ -37,23 +35,6 @@ int blocks_for(long long n) { return (int)((n + cpk::kBlock - 1) / cpk::kBlock);
 }  // namespace
 
 namespace cp {
-
-int fail(int code, const char* msg)
-{
-    g_err = msg;
-    return code;
-}
-
-int failf(int code, const char* fmt, ...)
-{
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(buf, sizeof(buf), fmt, ap);
-    va_end(ap);
-    g_err = buf;
-    return code;
-}
 
 Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax, int npe)
 {
@@ -1051,7 +1032,6 @@ int solve_level(cp_problem* P, const cp_options* opt, int elastic, cp_stats* st)
 
 extern "C" {
 
-const char* cp_last_error(void) { return g_err.c_str(); }
 
 int cp_abi_version(void) { return CP_ABI_VERSION; }
 
