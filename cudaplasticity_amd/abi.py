@@ -239,13 +239,15 @@ def gen_quarter_ring(a, b, Nt, Nr, ratio):
     return _gen("cp_mesh_quarter_ring_sizes", (Nt, Nr), "cp_mesh_quarter_ring", (_f(a), _f(b), Nt, Nr, _f(ratio)))
 
 
-def renum_by_direction(mesh, dx=1.0, dy=1.0):
+def renum_by_direction(mesh, dx=1.0, dy=1.0, with_perm=False):
     """Mesh::renumByDirection (Mesh.cu:537-630): a copy of `mesh` with its
-    nodes renumbered by increasing x dx + y dy."""
+    nodes renumbered by increasing x dx + y dy (with_perm: and the new
+    number of every old node)."""
     x, y, conn = mesh.x.copy(), mesh.y.copy(), mesh.conn.copy()
     bord = np.concatenate(mesh.borders).astype(np.int32) if mesh.borders else np.zeros(0, dtype=np.int32)
+    perm = np.zeros(mesh.nn, dtype=np.int32)
     check(lib().cp_mesh_renum_by_direction(mesh.nn, dp(x), dp(y), mesh.ne, mesh.npe, ip(conn), bord.size, ip(bord),
-                                           _f(dx), _f(dy)))
+                                           _f(dx), _f(dy), ip(perm)))
     out, off = [], 0
     for b in mesh.borders:
         out.append(bord[off: off + b.size].copy())
@@ -254,7 +256,17 @@ def renum_by_direction(mesh, dx=1.0, dy=1.0):
     if mesh.sec is not None:
         sec = np.zeros(mesh.nn, dtype=np.uint8)
         sec[conn[:, 4:].ravel()] = 1
-    return Mesh(x, y, conn, out, npe=mesh.npe, sec=sec)
+    m = Mesh(x, y, conn, out, npe=mesh.npe, sec=sec)
+    return (m, perm) if with_perm else m
+
+
+def ring_lattice(mesh, border_n):
+    """cp_mesh_ring_lattice: (n1, n2, node_of) of a mesh numbered by
+    renumerateRing (periodic along direction 1)."""
+    n1, n2 = C.c_int(), C.c_int()
+    node_of = np.zeros(mesh.nn, dtype=np.int32)
+    check(lib().cp_mesh_ring_lattice(mesh.nn, int(border_n), C.byref(n1), C.byref(n2), ip(node_of)))
+    return n1.value, n2.value, node_of
 
 
 def renumerate_ring(mesh, border_n):
@@ -263,7 +275,7 @@ def renumerate_ring(mesh, border_n):
     blen = np.array([b.size for b in mesh.borders], dtype=np.int32)
     bord = np.concatenate(mesh.borders).astype(np.int32) if mesh.borders else np.zeros(0, dtype=np.int32)
     check(lib().cp_mesh_renumerate_ring(mesh.nn, mesh.ne, dp(x), dp(y), ip(conn), blen.size, ip(bord), ip(blen),
-                                        int(border_n)))
+                                        int(border_n), None))
     out, off = [], 0
     for n in blen:
         out.append(bord[off: off + n].copy())
@@ -388,10 +400,12 @@ class Problem:
         check(lib().cp_problem_layout(self.ptr, C.byref(a), C.byref(b), C.byref(w)))
         return {"nblk_stored": a.value, "nblk": b.value, "max_width": w.value}
 
-    def set_grid(self, n1, n2, node_of=None):
-        """Logical lattice of a structured mesh (for precond=2, multigrid)."""
+    def set_grid(self, n1, n2, node_of=None, periodic=False):
+        """Logical lattice of a structured mesh (for precond=2, multigrid);
+        periodic: direction 1 closes on itself (rings, plates with a hole)."""
         no = None if node_of is None else np.ascontiguousarray(node_of, dtype=np.int32)
-        check(lib().cp_problem_set_grid(self.ptr, int(n1), int(n2), ip(no) if no is not None else None))
+        fn = lib().cp_problem_set_grid_periodic if periodic else lib().cp_problem_set_grid
+        check(fn(self.ptr, int(n1), int(n2), ip(no) if no is not None else None))
 
     def set_graphs(self, enable):
         check(lib().cp_problem_set_graphs(self.ptr, int(enable)))
@@ -520,8 +534,9 @@ class Server:
         self.sizes = {}
         self.last_message = ""   # message of the last failed result()
 
-    def submit(self, mesh, loads, mat, nsteps, final_factor, opt, grid=None, node_of=None):
-        """grid = (n1, n2): the mesh's lattice, needed for multigrid CG (precond 2)."""
+    def submit(self, mesh, loads, mat, nsteps, final_factor, opt, grid=None, node_of=None, periodic=False):
+        """grid = (n1, n2): the mesh's lattice, needed for multigrid CG (precond 2);
+        periodic: direction 1 closes on itself (rings, plates with a hole)."""
         rid = C.c_longlong()
         if grid is None:
             check(lib().cp_server_submit(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
@@ -529,10 +544,10 @@ class Server:
                                          _f(final_factor), C.byref(opt), C.byref(rid)))
         else:
             no = None if node_of is None else np.ascontiguousarray(node_of, dtype=np.int32)
-            check(lib().cp_server_submit_grid(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn),
-                                              dp(loads.rp), dp(loads.u), u8p(loads.mask), C.byref(mat), int(nsteps),
-                                              _f(final_factor), C.byref(opt), int(grid[0]), int(grid[1]),
-                                              ip(no) if no is not None else None, C.byref(rid)))
+            fn = lib().cp_server_submit_grid_periodic if periodic else lib().cp_server_submit_grid
+            check(fn(self.ptr, mesh.nn, mesh.ne, dp(mesh.x), dp(mesh.y), ip(mesh.conn), dp(loads.rp), dp(loads.u),
+                     u8p(loads.mask), C.byref(mat), int(nsteps), _f(final_factor), C.byref(opt), int(grid[0]),
+                     int(grid[1]), ip(no) if no is not None else None, C.byref(rid)))
         self.sizes[rid.value] = (mesh.nn, mesh.ne)
         return rid.value
 

@@ -96,6 +96,10 @@ Mesh& Mesh::operator=(const Mesh& o)
             std::copy(o.borders[b], o.borders[b] + borderLength[b], borders[b]);
         }
     }
+    latticeN1 = o.latticeN1;
+    latticeN2 = o.latticeN2;
+    latticePeriodic = o.latticePeriodic;
+    latticeNodes = o.latticeNodes;
     return *this;
 }
 
@@ -125,6 +129,9 @@ void Mesh::clear()
     count3 = count4 = count8 = 0;
     elemPos[0] = elemPos[1] = elemPos[2] = 0;
     analysed = false;
+    latticeN1 = latticeN2 = 0;
+    latticePeriodic = false;
+    latticeNodes.clear();
 }
 
 // sizes(nn, ne, nb, blen) then gen(x, y, conn, bord, blen) as the C-ABI
@@ -208,6 +215,35 @@ void Mesh::setBorders(int nb, const int* blen, const std::vector<int>& bord)
     ramSaved = true;
 }
 
+// Rectangles and quarter meshes number their nodes lattice point by lattice
+// point: node i1 * n2 + i2.
+void Mesh::setLattice(int n1, int n2)
+{
+    if ((long long)n1 * n2 != nodeCount) throw std::logic_error("mesh lattice does not match the node count");
+    latticeN1 = n1;
+    latticeN2 = n2;
+    latticePeriodic = false;
+    latticeNodes.resize(nodeCount);
+    for (int n = 0; n < nodeCount; ++n) latticeNodes[n] = n;
+}
+
+// Plates with a hole and rings: borderN rays around the hole, numbered by
+// renumerateRing.
+void Mesh::setRingLattice(int borderN)
+{
+    int n1 = 0, n2 = 0;
+    latticeNodes.resize(nodeCount);
+    check(cp_mesh_ring_lattice(nodeCount, borderN, &n1, &n2, latticeNodes.data()), "mesh lattice");
+    latticeN1 = n1;
+    latticeN2 = n2;
+    latticePeriodic = true;
+}
+
+void Mesh::renumberLattice(const std::vector<int>& newOfOld)
+{
+    for (int& n : latticeNodes) n = newOfOld[n];
+}
+
 void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, size_t N2, int order)
 {
     check_order(order);
@@ -222,6 +258,7 @@ void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, s
              [&](double* x, double* y, int* c, int* b, int* bl) {
                  return cp_mesh_rect(x1, x2, y1, y2, n1, n2, x, y, c, b, bl);
              });
+    setLattice(n1 + 1, n2 + 1);
 }
 
 void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double holeRad, size_t N1, size_t N2,
@@ -244,6 +281,7 @@ void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double ho
         [&](double* x, double* y, int* c, int* b, int* bl) {
             return cp_mesh_hole(x1, x2, y1, y2, holeRad, n1, n2, x, y, c, b, bl);
         });
+    setRingLattice(2 * (n1 + n2));
 }
 
 void Mesh::genRing(double a, double b, size_t N_phi, size_t N_r, int order)
@@ -260,6 +298,7 @@ void Mesh::genRing(double a, double b, size_t N_phi, size_t N_r, int order)
              [&](double* x, double* y, int* c, int* bd, int* bl) {
                  return cp_mesh_ring(a, b, np, nr, x, y, c, bd, bl);
              });
+    setRingLattice(np);
 }
 
 void Mesh::genQuarterPlate(double L, double r, size_t N_theta, size_t N_r)
@@ -270,6 +309,7 @@ void Mesh::genQuarterPlate(double L, double r, size_t N_theta, size_t N_r)
         [&](double* x, double* y, int* c, int* b, int* bl) {
             return cp_mesh_quarter_plate(L, r, nt, nr, x, y, c, b, bl);
         });
+    setLattice(nt + 1, nr + 1);
 }
 
 void Mesh::genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double ratio)
@@ -280,6 +320,7 @@ void Mesh::genQuarterRing(double a, double b, size_t N_theta, size_t N_r, double
         [&](double* x, double* y, int* c, int* bd, int* bl) {
             return cp_mesh_quarter_ring(a, b, nt, nr, ratio, x, y, c, bd, bl);
         });
+    setLattice(nt + 1, nr + 1);
 }
 
 bool Mesh::loadFromFile(const std::string& fileName)
@@ -500,10 +541,12 @@ void Mesh::renumerateRing(int borderN)
     if (count8 > 0 || count3 > 0) throw std::invalid_argument("renumerateRing: 4-node meshes only");
     FlatMesh f;
     f.load(*this);
+    std::vector<int> perm(nodeCount);
     check(cp_mesh_renumerate_ring(nodeCount, count4, f.x.data(), f.y.data(), elem4, bordersCount,
-                                  f.bord.data(), borderLength, borderN),
+                                  f.bord.data(), borderLength, borderN, perm.data()),
           "renumerateRing");
     f.store(*this);
+    renumberLattice(perm);
 }
 
 void Mesh::renumByDirection(vec2 direction)
@@ -512,11 +555,13 @@ void Mesh::renumByDirection(vec2 direction)
     FlatMesh f;
     f.load(*this);
     const bool q8 = count8 > 0;
+    std::vector<int> perm(nodeCount);
     check(cp_mesh_renum_by_direction(nodeCount, f.x.data(), f.y.data(), q8 ? count8 : count4, q8 ? 8 : 4,
                                      q8 ? elem8 : elem4, (int)f.bord.size(), f.bord.data(), direction.x,
-                                     direction.y),
+                                     direction.y, perm.data()),
           "renumByDirection");
     f.store(*this);
+    renumberLattice(perm);
     // mid-side nodes follow their elements (Mesh.cu:619-623)
     if (secOrdNodes) {
         std::fill(secOrdNodes, secOrdNodes + nodeCount, false);

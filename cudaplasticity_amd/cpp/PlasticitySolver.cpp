@@ -56,6 +56,7 @@ void PlasticitySolver::setPlainCondition(plainCondition pc)
     if (pc != pc_ && prob_) {
         cp_problem_destroy(prob_);
         prob_ = nullptr;
+        gridSet_ = false;
     }
     pc_ = pc;
 }
@@ -171,6 +172,18 @@ void PlasticitySolver::buildProblem()
           "problem setup");
 }
 
+// The problem, and with options.multigrid its lattice (once per problem).
+void PlasticitySolver::prepare()
+{
+    if (options.multigrid && mesh_.latticeN1 == 0)
+        throw std::invalid_argument("PlasticitySolver: multigrid needs a generated 4-node mesh (no lattice)");
+    if (!prob_) buildProblem();
+    if (!options.multigrid || gridSet_) return;
+    const auto set = mesh_.latticePeriodic ? cp_problem_set_grid_periodic : cp_problem_set_grid;
+    check(set(prob_, mesh_.latticeN1, mesh_.latticeN2, mesh_.latticeNodes.data()), "multigrid lattice");
+    gridSet_ = true;
+}
+
 cp_options PlasticitySolver::abiOptions(bool mixed) const
 {
     cp_options o{};
@@ -183,7 +196,7 @@ cp_options PlasticitySolver::abiOptions(bool mixed) const
     o.precision = mixed ? CP_MIXED : CP_FP64;
     o.warm = 0;
     o.inner_tol = options.innerTol;
-    o.precond = options.blockJacobi ? 1 : 0;
+    o.precond = options.multigrid ? 2 : options.blockJacobi ? 1 : 0;
     o.predictor = options.predictor;
     o.cg_forcing = options.cgForcing;
     return o;
@@ -192,7 +205,7 @@ cp_options PlasticitySolver::abiOptions(bool mixed) const
 double PlasticitySolver::run(bool elastic, bool mixed)
 {
     const double t0 = now_s();
-    if (!prob_) buildProblem();
+    prepare();
     check(cp_set_load(prob_, 1.0), "set load");
     const cp_options o = abiOptions(mixed);
     const int s = elastic ? cp_solve_elastic(prob_, &o, &stats_) : cp_solve(prob_, &o, &stats_);
@@ -226,7 +239,7 @@ double PlasticitySolver::solveSteps(int nsteps, double finalFactor)
 {
     if (nsteps < 1) throw std::invalid_argument("solveSteps: nsteps < 1");
     const double t0 = now_s();
-    if (!prob_) buildProblem();
+    prepare();
     const cp_options o = abiOptions(floatBoost || options.mixedPrecision);
     stepStats_.assign(nsteps, cp_stats{});
     const int s = cp_solve_steps(prob_, nsteps, finalFactor, &o, stepStats_.data());

@@ -350,6 +350,7 @@ struct CgArgs {
     // coarse multigrid levels: the lattice of their 9-slot stencil rows,
     // whose addresses follow from the node (0: rows through col / slice)
     int lat_n1 = 0, lat_n2 = 0;
+    int lat_per1 = 0;        // 1: the lattice is periodic in direction 1 (rings, plates with a hole)
 };
 
 template <typename T>

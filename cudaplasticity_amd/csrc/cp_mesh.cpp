@@ -33,8 +33,8 @@ struct BorderWriter {
 // alternately from the front and the back so the band stays narrow.  An odd
 // column count numbers the middle column last (the reference writes that
 // map reversed, Mesh.cu:347-349).
-void renumber_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
-                   int columns)
+// The new number of old node i * columns + j (column j, row i).
+std::vector<int> ring_permutation(int nn, int columns)
 {
     const int rowsPerCol = nn / columns;
     std::vector<int> perm(nn);
@@ -46,6 +46,13 @@ void renumber_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int*
     }
     if (lo == hi)
         for (int i = 0; i < rowsPerCol; ++i) perm[i * columns + lo] = next++;
+    return perm;
+}
+
+void renumber_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
+                   int columns)
+{
+    const std::vector<int> perm = ring_permutation(nn, columns);
     for (int k = 0; k < 4 * ne; ++k) conn[k] = perm[conn[k]];
     int total = 0;
     for (int b = 0; b < nb; ++b) total += blen[b];
@@ -919,7 +926,7 @@ int cp_mesh_ring2(double a, double b, int Nphi, int Nr, double* x, double* y, in
 }
 
 int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
-                            int borderN)
+                            int borderN, int* new_of_old)
 {
     if (nn < 1 || ne < 0 || !x || !y || (ne > 0 && !conn) || nb < 0 || (nb > 0 && (!bord || !blen)) || borderN < 1 ||
         nn % borderN != 0)
@@ -927,6 +934,28 @@ int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int
     for (int k = 0; k < 4 * ne; ++k)
         if (conn[k] < 0 || conn[k] >= nn) return cp::fail(CP_ERR_INVALID, "cp_mesh_renumerate_ring: node out of range");
     renumber_ring(nn, ne, x, y, conn, nb, bord, blen, borderN);
+    if (new_of_old) {
+        const std::vector<int> perm = ring_permutation(nn, borderN);
+        std::copy(perm.begin(), perm.end(), new_of_old);
+    }
+    return CP_OK;
+}
+
+// Lattice of a mesh numbered by renumerateRing: columns (the rays of a ring
+// or of a plate with a hole) along direction 1, periodic, rows along
+// direction 2; node_of[j * (nn / borderN) + i] = number of ray j's row-i node.
+int cp_mesh_ring_lattice(int nn, int borderN, int* n1, int* n2, int* node_of)
+{
+    if (nn < 1 || borderN < 3 || nn % borderN != 0 || !n1 || !n2)
+        return cp::fail(CP_ERR_INVALID, "cp_mesh_ring_lattice: the node count must be a multiple of borderN >= 3");
+    const int rows = nn / borderN;
+    *n1 = borderN;
+    *n2 = rows;
+    if (node_of) {
+        const std::vector<int> perm = ring_permutation(nn, borderN);
+        for (int j = 0; j < borderN; ++j)
+            for (int i = 0; i < rows; ++i) node_of[j * rows + i] = perm[i * borderN + j];
+    }
     return CP_OK;
 }
 
@@ -936,7 +965,7 @@ int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int
 // on ties (Mesh.cu:505-535); that order is the order of (projection, original
 // number descending), which is what ties get here.
 int cp_mesh_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
-                               double dx, double dy)
+                               double dx, double dy, int* new_of_old)
 {
     if (nn < 1 || !x || !y || ne < 0 || (npe != 4 && npe != 8) || (ne > 0 && !conn) || nbord < 0 ||
         (nbord > 0 && !bord))
@@ -964,6 +993,7 @@ int cp_mesh_renum_by_direction(int nn, double* x, double* y, int ne, int npe, in
     }
     std::memcpy(x, nx.data(), sizeof(double) * nn);
     std::memcpy(y, ny.data(), sizeof(double) * nn);
+    if (new_of_old) std::copy(perm.begin(), perm.end(), new_of_old);
     return CP_OK;
 }
 

@@ -37,12 +37,28 @@ constexpr double kMgLowFrac = 1. / 30.;
 
 // 1-D coarsening: coarse point count of n fine points (co = 0: the
 // direction is kept, as when a level is semi-coarsened along the other one).
-__host__ __device__ inline int mg_coarse_n(int n, int co = 1) { return (n <= 2 || !co) ? n : n / 2 + 1; }
+// A periodic direction (per = 1: point n - 1 neighbours point 0) is halved
+// only while it stays even with at least 3 coarse points, so that its two
+// neighbours of a point are distinct points: coarse K sits on fine 2K.
+__host__ __device__ inline bool mg_per_halves(int n) { return n % 2 == 0 && n >= 6; }
+__host__ __device__ inline int mg_coarse_n(int n, int co = 1, int per = 0)
+{
+    if (per) return co ? n / 2 : n;
+    return (n <= 2 || !co) ? n : n / 2 + 1;
+}
 
 // Coarse parents of fine point f of an n-point direction: count (1 or 2),
 // coarse indices and bilinear weights.
-__host__ __device__ inline int mg_parents(int n, int co, int f, int (&K)[2], double (&w)[2])
+__host__ __device__ inline int mg_parents(int n, int co, int f, int (&K)[2], double (&w)[2], int per = 0)
 {
+    if (per && co) {
+        K[0] = f >> 1;
+        w[0] = 1.;
+        if ((f & 1) == 0) return 1;
+        K[1] = ((f >> 1) + 1) % (n / 2);
+        w[0] = w[1] = 0.5;
+        return 2;
+    }
     if (n <= 2 || !co || (f & 1) == 0) {
         K[0] = (n <= 2 || !co) ? f : (f >> 1);
         w[0] = 1.;
@@ -60,11 +76,11 @@ __host__ __device__ inline int mg_parents(int n, int co, int f, int (&K)[2], dou
 }
 
 // Weight of coarse point K in the interpolation to fine point f (0 if none).
-__device__ __forceinline__ double mg_weight(int n, int co, int f, int K)
+__device__ __forceinline__ double mg_weight(int n, int co, int f, int K, int per = 0)
 {
     int P[2];
     double w[2];
-    const int np = mg_parents(n, co, f, P, w);
+    const int np = mg_parents(n, co, f, P, w, per);
     double r = 0.;
     for (int p = 0; p < np; ++p)
         if (P[p] == K) r = w[p];
@@ -72,9 +88,30 @@ __device__ __forceinline__ double mg_weight(int n, int co, int f, int K)
 }
 
 // Fine position of coarse point K.
-__host__ __device__ inline int mg_pos(int n, int co, int K)
+__host__ __device__ inline int mg_pos(int n, int co, int K, int per = 0)
 {
+    if (per) return co ? 2 * K : K;
     return (n <= 2 || !co) ? K : (2 * K < n - 1 ? 2 * K : n - 1);
+}
+
+// Point f + d of an n-point direction, wrapped when it is periodic; -1
+// outside a direction that is not.
+__host__ __device__ inline int mg_step(int n, int per, int f, int d)
+{
+    const int q = f + d;
+    if (per) return q < 0 ? q + n : (q >= n ? q - n : q);
+    return (q < 0 || q >= n) ? -1 : q;
+}
+
+// Offset of point g from point f in [-n/2, n/2) along a periodic direction.
+__host__ __device__ inline int mg_offset(int n, int per, int f, int g)
+{
+    int e = g - f;
+    if (per) {
+        if (e > n / 2) e -= n;
+        else if (e < -(n - 1) / 2) e += n;
+    }
+    return e;
 }
 
 // Stencil slot of the coarse neighbour at offset (d1, d2), |d| <= 1.
@@ -130,6 +167,7 @@ struct GalArgs {
                                    // (nullptr: the coarse-level layout, position = slot)
     int n1f, n2f;
     int co1, co2;          // directions coarsened to the next level
+    int per1;              // direction 1 periodic
     int n1c, n2c, nnc, nbc;
     const int2* slice_c;
     double2* va_c;         // FP64 destination planes
@@ -152,14 +190,14 @@ __device__ __forceinline__ void load_block_d(const CgArgs<T>& a, size_t q, doubl
 
 // Children of coarse point K in an n-point direction: fine points and
 // their interpolation weights (at most 3).
-__device__ __forceinline__ int mg_children(int n, int co, int K, int (&f)[3], double (&w)[3])
+__device__ __forceinline__ int mg_children(int n, int co, int K, int (&f)[3], double (&w)[3], int per = 0)
 {
-    const int p = mg_pos(n, co, K);
+    const int p = mg_pos(n, co, K, per);
     int c = 0;
     for (int a = -1; a <= 1; ++a) {
-        const int q = p + a;
-        if (q < 0 || q >= n) continue;
-        const double wq = mg_weight(n, co, q, K);
+        const int q = mg_step(n, per, p, a);
+        if (q < 0) continue;
+        const double wq = mg_weight(n, co, q, K, per);
         if (wq == 0.) continue;
         f[c] = q;
         w[c] = wq;
@@ -184,13 +222,13 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
     // slot t -> coarse offset (d1, d2)
     const int idx = t == 0 ? 4 : (t <= 4 ? t - 1 : t);
     const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
-    const int D1 = K1 + d1, D2 = K2 + d2;
+    const int D1 = mg_step(g.n1c, g.per1, K1, d1), D2 = K2 + d2;
     double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
-    if (D1 >= 0 && D1 < g.n1c && D2 >= 0 && D2 < g.n2c) {
+    if (D1 >= 0 && D2 >= 0 && D2 < g.n2c) {
         int fa[3], fb[3], ga[3], gb[3];
         double wfa[3], wfb[3], wga[3], wgb[3];
-        const int nfa = mg_children(g.n1f, g.co1, K1, fa, wfa), nfb = mg_children(g.n2f, g.co2, K2, fb, wfb);
-        const int nga = mg_children(g.n1f, g.co1, D1, ga, wga), ngb = mg_children(g.n2f, g.co2, D2, gb, wgb);
+        const int nfa = mg_children(g.n1f, g.co1, K1, fa, wfa, g.per1), nfb = mg_children(g.n2f, g.co2, K2, fb, wfb);
+        const int nga = mg_children(g.n1f, g.co1, D1, ga, wga, g.per1), ngb = mg_children(g.n2f, g.co2, D2, gb, wgb);
         for (int i = 0; i < nfa; ++i)
             for (int j = 0; j < nfb; ++j) {
                 const int lf = fa[i] * g.n2f + fb[j];
@@ -198,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
                 const uchar2 mf = g.f.mask[fn];
                 const size_t base = (size_t)g.f.slice[fn >> 6].x + (fn & 63);
                 for (int k = 0; k < nga; ++k) {
-                    const int e1 = ga[k] - fa[i];
+                    const int e1 = mg_offset(g.n1f, g.per1, fa[i], ga[k]);
                     if (e1 < -1 || e1 > 1) continue;
                     for (int l = 0; l < ngb; ++l) {
                         const int e2 = gb[l] - fb[j];
@@ -395,12 +433,12 @@ enum { MG_PRE1 = 0, MG_CHEB = 1, MG_RESID = 2, MG_POST1 = 3 };
 // lanes each) with slot k holding the neighbour at a fixed lattice offset
 // (itself where the lattice ends, with zero values), so the row's
 // addresses and neighbours follow from (i1, i2) without loading col / slice.
-__device__ __forceinline__ int lat_neighbour(int k, int i1, int i2, int n1, int n2, int n)
+__device__ __forceinline__ int lat_neighbour(int k, int i1, int i2, int n1, int n2, int n, int per1 = 0)
 {
     const int idx = k == 0 ? 4 : (k <= 4 ? k - 1 : k);
     const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
-    const int j1 = i1 + d1, j2 = i2 + d2;
-    return (j1 >= 0 && j1 < n1 && j2 >= 0 && j2 < n2) ? n + d1 * n2 + d2 : n;
+    const int j1 = mg_step(n1, per1, i1, d1), j2 = i2 + d2;
+    return (j1 >= 0 && j2 >= 0 && j2 < n2) ? j1 * n2 + j2 : n;
 }
 
 __device__ __forceinline__ size_t lat_base(int n) { return (size_t)(n >> 6) * (9 * kWave) + (n & 63); }
@@ -418,7 +456,7 @@ __device__ __forceinline__ void row_product_lat(const CgArgs<T>& a, int n, int i
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         T zx, zy;
-        zfun(lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n), zx, zy);
+        zfun(lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n, a.lat_per1), zx, zy);
         yx += m[k][0] * zx + m[k][1] * zy;
         yy += m[k][2] * zx + m[k][3] * zy;
     }
@@ -521,6 +559,7 @@ struct MgGrid {
     const int* node_of;   // fine lattice index -> node (nullptr: identity)
     int n1f, n2f, n1c, n2c;
     int co1, co2;         // directions coarsened to the next level
+    int per1;             // direction 1 periodic
 };
 
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
@@ -531,12 +570,12 @@ __device__ __forceinline__ typename V2<T>::type mg_restrict_value(const MgGrid& 
 {
     using V = typename V2<T>::type;
     const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
-    const int p1 = mg_pos(gr.n1f, gr.co1, K1), p2 = mg_pos(gr.n2f, gr.co2, K2);
+    const int p1 = mg_pos(gr.n1f, gr.co1, K1, gr.per1), p2 = mg_pos(gr.n2f, gr.co2, K2);
     double sx = 0., sy = 0.;
     for (int a1 = -1; a1 <= 1; ++a1) {
-        const int f1 = p1 + a1;
-        if (f1 < 0 || f1 >= gr.n1f) continue;
-        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1);
+        const int f1 = mg_step(gr.n1f, gr.per1, p1, a1);
+        if (f1 < 0) continue;
+        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1, gr.per1);
         if (w1 == 0.) continue;
         for (int a2 = -1; a2 <= 1; ++a2) {
             const int f2 = p2 + a2;
@@ -581,7 +620,7 @@ __device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename
     const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
     int Q1[2], Q2[2];
     double W1[2], W2[2];
-    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
+    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1, gr.per1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
     double sx = 0., sy = 0.;
     for (int b1 = 0; b1 < nq1; ++b1)
         for (int b2 = 0; b2 < nq2; ++b2) {
@@ -631,6 +670,7 @@ constexpr double kSegLam = 4.;
 
 struct SegGrid {
     int n1, n2;              // lattice of the level
+    int per1;                // direction 1 periodic
     int nt2, ntiles;         // tiles along direction 2, tiles in all
     const int* node_of;      // lattice index -> node (level 0 only; nullptr: identity)
     const signed char* slot_pos;   // level 0: node x 9 lattice offsets -> row position
@@ -680,7 +720,10 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g, typename V4<T>::type* minv,
                                                          double* lam)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *lam = kSegLam;
+    // along a periodic direction an odd number of segments needs three
+    // colours (the last segment touches the first): lambda_max <= 6
+    const int nseg1 = (g.n1 + kSeg - 1) / kSeg;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *lam = (g.per1 && (nseg1 & 1)) ? 1.5 * kSegLam : kSegLam;
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     const int nseg = (g.n1 + kSeg - 1) / kSeg;
     if (idx >= nseg * g.n2) return;
@@ -827,7 +870,7 @@ __global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_mg_smooth_seg(CgArgs<T
 #pragma unroll
                 for (int k = 0; k < 9; ++k) {
                     T zx, zy;
-                    zfun(lat_neighbour(k, i1, i2, g.n1, g.n2, n), zx, zy);
+                    zfun(lat_neighbour(k, i1, i2, g.n1, g.n2, n, g.per1), zx, zy);
                     yx += m[k][0] * zx + m[k][1] * zy;
                     yy += m[k][2] * zx + m[k][3] * zy;
 #pragma unroll
@@ -1110,7 +1153,7 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
     for (int k = 0; k < 9; ++k) {
         T m[4];
         rowf(k, m);
-        const int nb = lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n);
+        const int nb = lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n, a.lat_per1);
         T zx, zy;
         if constexpr (MODE == MG_PRE1) {
             const V bm = v.b[nb];

@@ -28,6 +28,7 @@ struct MgLevel {
 
 struct MgState {
     int nlev = 0;
+    int per1 = 0;            // the lattice is periodic in direction 1
     std::vector<MgLevel> L;
     int* lg0 = nullptr;      // node -> lattice index of level 0 (nullptr: identity)
     int* node0 = nullptr;    // lattice index -> node of level 0 (nullptr: identity)
@@ -152,6 +153,7 @@ static int mg_tail_table(cp_problem* P)
             t.grid.n2c = G.L[l + 1].n2;
             t.grid.co1 = L.co1;
             t.grid.co2 = L.co2;
+            t.grid.per1 = G.per1;
         }
         t.mat_lds = t.minv_lds = t.mask_lds = -1;
         tab.push_back(t);
@@ -200,6 +202,7 @@ cpk::CgArgs<double> level_args<double>(cp_problem* P, int l)
     a.nblocks = L.nb;
     a.lat_n1 = L.n1;
     a.lat_n2 = L.n2;
+    a.lat_per1 = P->mg->per1;
     return a;
 }
 
@@ -219,6 +222,7 @@ cpk::CgArgs<float> level_args<float>(cp_problem* P, int l)
     a.nblocks = L.nb;
     a.lat_n1 = L.n1;
     a.lat_n2 = L.n2;
+    a.lat_per1 = P->mg->per1;
     return a;
 }
 
@@ -241,6 +245,7 @@ static cpk::SegGrid seg_grid(const MgState& G, int l)
     cpk::SegGrid g{};
     g.n1 = L.n1;
     g.n2 = L.n2;
+    g.per1 = G.per1;
     const int cols = l == 0 ? cpk::seg_cols<0>() : cpk::seg_cols<cpk::kLatW>();
     g.nt2 = (L.n2 + cols - 1) / cols;
     g.ntiles = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * g.nt2;
@@ -269,6 +274,7 @@ static int mg_setup_t(cp_problem* P, int nlev)
             g.n2f = F.n2;
             g.co1 = F.co1;
             g.co2 = F.co2;
+            g.per1 = G.per1;
             g.n1c = L.n1;
             g.n2c = L.n2;
             g.nnc = L.nn;
@@ -441,6 +447,7 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         g.n2c = G.L[l + 1].n2;
         g.co1 = G.L[l].co1;
         g.co2 = G.L[l].co2;
+        g.per1 = G.per1;
         return g;
     };
     const int last = G.lt;   // levels lt.. run in the one-workgroup tail
@@ -779,10 +786,14 @@ int cp_bench_mg(cp_problem* P, int precision, int reps, double* ms_vcycle, doubl
                                 : cp::mg_bench_t<float>(P, reps, ms_vcycle, ms_smooth, bytes_smooth);
 }
 
-int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
+namespace cp {
+
+// The lattice of a structured mesh and the level hierarchy on it; per1: the
+// lattice is periodic in direction 1 (the rays of a ring or of a plate with
+// a hole: ray n1 - 1 neighbours ray 0).
+static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
 {
-    using namespace cp;
-    if (!P || n1 < 2 || n2 < 2) return fail(CP_ERR_INVALID, "cp_problem_set_grid: bad arguments");
+    if (!P || n1 < 2 || n2 < 2 || (per1 && n1 < 3)) return fail(CP_ERR_INVALID, "cp_problem_set_grid: bad arguments");
     if (P->npe != 4) return fail(CP_ERR_INVALID, "cp_problem_set_grid: multigrid needs 4-node elements");
     if ((long long)n1 * n2 != P->nn)
         return failf(CP_ERR_INVALID, "cp_problem_set_grid: %d x %d lattice for %d nodes", n1, n2, P->nn);
@@ -803,7 +814,7 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
             const int m = P->adj_node[q];
             const int b1 = lg[m] / n2, b2 = lg[m] % n2;
-            if (std::abs(a1 - b1) > 1 || std::abs(a2 - b2) > 1)
+            if (std::abs(cpk::mg_offset(n1, per1, a1, b1)) > 1 || std::abs(a2 - b2) > 1)
                 return failf(CP_ERR_INVALID, "cp_problem_set_grid: nodes %d and %d share an element but are not lattice neighbours",
                              n, m);
         }
@@ -812,6 +823,7 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
     mg_destroy(P);
     auto* G = new MgState();
     P->mg = G;
+    G->per1 = per1;
     // mean lattice edge lengths along the two directions: a stretched mesh
     // couples strongly across its short edges, so only that direction is
     // halved until the two are within a factor 2 (semi-coarsening)
@@ -825,8 +837,8 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         for (int i1 = 0; i1 < n1; ++i1)
             for (int i2 = 0; i2 < n2; ++i2) {
                 const double2 p = pos[i1 * n2 + i2];
-                if (i1 + 1 < n1) {
-                    const double2 q = pos[(i1 + 1) * n2 + i2];
+                if (i1 + 1 < n1 || per1) {
+                    const double2 q = pos[((i1 + 1) % n1) * n2 + i2];
                     len1 += std::hypot(q.x - p.x, q.y - p.y);
                     ++c1n;
                 }
@@ -849,12 +861,14 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         L.nb = blocks_of(L.nn);
         L.nslices = (L.nn + cpk::kWave - 1) / cpk::kWave;
         const bool done = 2 * c1 * c2 <= cpk::kMgDenseMax;
-        L.co1 = !(len1 > 2. * len2) && c1 > 2;
+        // a periodic direction halves only to an even count of at least 3 points
+        const bool can1 = per1 ? cpk::mg_per_halves(c1) : c1 > 2;
+        L.co1 = !(len1 > 2. * len2) && can1;
         L.co2 = !(len2 > 2. * len1) && c2 > 2;
-        if (!L.co1 && !L.co2) L.co1 = c1 > 2, L.co2 = c2 > 2;
+        if (!L.co1 && !L.co2) L.co1 = can1, L.co2 = c2 > 2;
         G->L.push_back(L);
         if (done) break;
-        const int d1 = cpk::mg_coarse_n(c1, L.co1), d2 = cpk::mg_coarse_n(c2, L.co2);
+        const int d1 = cpk::mg_coarse_n(c1, L.co1, per1), d2 = cpk::mg_coarse_n(c2, L.co2);
         if (d1 == c1 && d2 == c2) break;
         if (d1 != c1) len1 *= 2.;
         if (d2 != c2) len2 *= 2.;
@@ -890,14 +904,16 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
                 ++cells;
                 stretched += h1 < 0.5 * h2;
             }
-        L.seg = L.co1 && L.co2 && cells > 0 && stretched >= kSegFrac * cells;
+        // (a periodic direction of one segment would couple that segment to
+        // itself across the seam, outside its tridiagonal solve)
+        L.seg = L.co1 && L.co2 && cells > 0 && stretched >= kSegFrac * cells && !(per1 && L.n1 <= cpk::kSeg);
         if (l + 1 < G->nlev) {
             const MgLevel& C = G->L[l + 1];
             std::vector<double2> pc((size_t)C.nn);
             for (int K1 = 0; K1 < C.n1; ++K1)
                 for (int K2 = 0; K2 < C.n2; ++K2)
                     pc[(size_t)K1 * C.n2 + K2] =
-                        pos[cpk::mg_pos(L.n1, L.co1, K1) * L.n2 + cpk::mg_pos(L.n2, L.co2, K2)];
+                        pos[cpk::mg_pos(L.n1, L.co1, K1, per1) * L.n2 + cpk::mg_pos(L.n2, L.co2, K2)];
             pos.swap(pc);
         }
     }
@@ -923,8 +939,8 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
                 const int K1 = C / L.n2, K2 = C % L.n2;
                 for (int d1 = -1; d1 <= 1; ++d1)
                     for (int d2 = -1; d2 <= 1; ++d2) {
-                        const int D1 = K1 + d1, D2 = K2 + d2;
-                        const bool in = D1 >= 0 && D1 < L.n1 && D2 >= 0 && D2 < L.n2;
+                        const int D1 = cpk::mg_step(L.n1, per1, K1, d1), D2 = K2 + d2;
+                        const bool in = D1 >= 0 && D2 >= 0 && D2 < L.n2;
                         hc[(size_t)hs[sl].x + (size_t)cpk::mg_slot(d1, d2) * cpk::kWave + lane] = in ? D1 * L.n2 + D2 : C;
                     }
             }
@@ -941,7 +957,7 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
             const int a1 = lg[n] / n2, a2 = lg[n] % n2;
             for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
                 const int m = P->adj_node[q];
-                const int e1 = lg[m] / n2 - a1, e2 = lg[m] % n2 - a2;
+                const int e1 = cpk::mg_offset(n1, per1, a1, lg[m] / n2), e2 = lg[m] % n2 - a2;
                 sp[(size_t)n * 9 + cpk::mg_slot(e1, e2)] = (signed char)(q - P->adj_ptr[n]);
             }
         }
@@ -968,6 +984,18 @@ int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
         return s;
     }
     return CP_OK;
+}
+
+}  // namespace cp
+
+int cp_problem_set_grid(cp_problem* P, int n1, int n2, const int* node_of)
+{
+    return cp::set_grid(P, n1, n2, node_of, 0);
+}
+
+int cp_problem_set_grid_periodic(cp_problem* P, int n1, int n2, const int* node_of)
+{
+    return cp::set_grid(P, n1, n2, node_of, 1);
 }
 
 int cp_problem_set_graphs(cp_problem* P, int enable)

@@ -50,6 +50,7 @@ struct Request {
     std::vector<int> conn;
     std::vector<unsigned char> mask;
     int n1 = 0, n2 = 0;              // multigrid lattice (precond 2)
+    int per1 = 0;                    // the lattice is periodic in direction 1
     std::vector<int> node_of;        // its node numbering (empty: identity)
     cp_material mat{};
     int nsteps = 1;
@@ -436,7 +437,9 @@ void prepare(cp_server* S, Request* r)
     if (s == CP_OK && mg && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
         s = cp::fail(CP_ERR_DEVICE, "cp_server: event creation failed");
     if (s == CP_OK) s = cp::problem_alloc_solver(P, r->opt.precision);
-    if (s == CP_OK && mg) s = cp_problem_set_grid(P, r->n1, r->n2, r->node_of.empty() ? nullptr : r->node_of.data());
+    if (s == CP_OK && mg)
+        s = (r->per1 ? cp_problem_set_grid_periodic : cp_problem_set_grid)(P, r->n1, r->n2,
+                                                                          r->node_of.empty() ? nullptr : r->node_of.data());
     if (s == CP_OK && mg) {
         std::lock_guard<std::mutex> lk(S->step_mu);
         s = cp::mg_prepare(P, r->opt.precision);
@@ -634,10 +637,11 @@ int cp_server_submit(cp_server* S, int nn, int ne, const double* x, const double
     return cp_server_submit_grid(S, nn, ne, x, y, conn, rp, u0, mask, mat, nsteps, final_factor, opt, 0, 0, nullptr, id);
 }
 
-int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn,
-                          const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
-                          int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
-                          long long* id)
+namespace {
+
+int submit_grid(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn, const double* rp,
+                const double* u0, const unsigned char* mask, const cp_material* mat, int nsteps, double final_factor,
+                const cp_options* opt, int n1, int n2, const int* node_of, int per1, long long* id)
 {
     if (!S || !x || !y || !conn || !rp || !mask || !mat || !opt || nsteps < 1 || nn < 1 || ne < 1)
         return cp::fail(CP_ERR_INVALID, "cp_server_submit: bad arguments");
@@ -649,6 +653,7 @@ int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const d
     if (opt->precond == 2) {
         r->n1 = n1;
         r->n2 = n2;
+        r->per1 = per1;
         if (node_of) r->node_of.assign(node_of, node_of + nn);
     }
     r->id = S->next_id++;
@@ -676,6 +681,24 @@ int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const d
     S->queue.push_back(raw);
     if (id) *id = raw->id;
     return CP_OK;
+}
+
+}  // namespace
+
+int cp_server_submit_grid(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn,
+                          const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
+                          int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
+                          long long* id)
+{
+    return submit_grid(S, nn, ne, x, y, conn, rp, u0, mask, mat, nsteps, final_factor, opt, n1, n2, node_of, 0, id);
+}
+
+int cp_server_submit_grid_periodic(cp_server* S, int nn, int ne, const double* x, const double* y, const int* conn,
+                                   const double* rp, const double* u0, const unsigned char* mask,
+                                   const cp_material* mat, int nsteps, double final_factor, const cp_options* opt,
+                                   int n1, int n2, const int* node_of, long long* id)
+{
+    return submit_grid(S, nn, ne, x, y, conn, rp, u0, mask, mat, nsteps, final_factor, opt, n1, n2, node_of, 1, id);
 }
 
 int cp_server_step(cp_server* S, int* in_flight)

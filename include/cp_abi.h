@@ -155,15 +155,23 @@ int cp_mesh_quarter_ring(double a, double b, int Nt, int Nr, double ratio,
 /* Mesh::renumerateRing (Mesh.cu:332-384) on a 4-node mesh, in place: the
  * nn / borderN rows of borderN columns are numbered column by column,
  * alternately from the first and the last column (the generators of plates
- * with a hole and rings already apply it). */
+ * with a hole and rings already apply it).  new_of_old (nn, may be NULL):
+ * the new number of every old node. */
 int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int nb, int* bord, const int* blen,
-                            int borderN);
+                            int borderN, int* new_of_old);
+/* The lattice of a mesh numbered by renumerateRing (genRectWithHole with
+ * borderN = 2 (N1 + N2), genRing with borderN = N_phi, order 1): n1 =
+ * borderN rays along direction 1, periodic, n2 = nn / borderN nodes per ray,
+ * node_of (nn entries, may be NULL) as cp_problem_set_grid_periodic takes
+ * it.  New: the reference has no multigrid. */
+int cp_mesh_ring_lattice(int nn, int borderN, int* n1, int* n2, int* node_of);
 /* Mesh::renumByDirection (Mesh.cu:537-630) in place: nodes renumbered by
  * increasing x dx + y dy, ties in decreasing original number (the order the
  * reference's merge sort, Mesh.cu:505-535, leaves them in); conn (npe = 4 or
- * 8 nodes per element) and the nbord border entries follow. */
+ * 8 nodes per element) and the nbord border entries follow; new_of_old
+ * (nn, may be NULL): the new number of every old node. */
 int cp_mesh_renum_by_direction(int nn, double* x, double* y, int ne, int npe, int* conn, int nbord, int* bord,
-                               double dx, double dy);
+                               double dx, double dy, int* new_of_old);
 /* Mesh::findMaxIndexDiff (Mesh.cu:417-441) */
 int cp_mesh_max_index_diff(int ne, const int* conn, int* out);
 /* Mesh::remapOrder (Mesh.cu:443-503): the order-1 mesh of doubled resolution
@@ -279,6 +287,10 @@ int cp_problem_get_values(cp_problem* prob, double* vals);
  * and n2 = N2 + 1 or N_r + 1).  Every pair of nodes sharing an element must
  * be lattice neighbours; the call builds the level hierarchy. */
 int cp_problem_set_grid(cp_problem* prob, int n1, int n2, const int* node_of);
+/* The same for a lattice periodic in direction 1: lattice point (n1 - 1, i2)
+ * neighbours (0, i2), as the rays of genRing and genRectWithHole close
+ * around the ring or the hole (Mesh.cu:112-245, 249-329).  New. */
+int cp_problem_set_grid_periodic(cp_problem* prob, int n1, int n2, const int* node_of);
 /* Replay of the multigrid CG iteration as a HIP graph (default 1); 0 launches
  * its kernels one by one (same kernels, same results). */
 int cp_problem_set_graphs(cp_problem* prob, int enable);
@@ -416,6 +428,12 @@ int cp_server_submit_grid(cp_server* srv, int nn, int ne, const double* x, const
                           const double* rp, const double* u0, const unsigned char* mask, const cp_material* mat,
                           int nsteps, double final_factor, const cp_options* opt, int n1, int n2, const int* node_of,
                           long long* id);
+/* The same with a lattice periodic in direction 1, as
+ * cp_problem_set_grid_periodic takes it (rings, plates with a hole).  New. */
+int cp_server_submit_grid_periodic(cp_server* srv, int nn, int ne, const double* x, const double* y, const int* conn,
+                                   const double* rp, const double* u0, const unsigned char* mask,
+                                   const cp_material* mat, int nsteps, double final_factor, const cp_options* opt,
+                                   int n1, int n2, const int* node_of, long long* id);
 /* Advance the server without waiting for multigrid work: start the
  * requests the worker has prepared, admit queued requests that fit (FIFO,
  * HBM budget, slots), advance every multigrid request whose last readback

@@ -41,6 +41,9 @@ struct SolverOptions {
     bool blockJacobi = false;
     int predictor = 0;          // solveSteps: 0 none, 1 extrapolate u, 2 u and log(psi)
     double cgForcing = 0.;      // > 0: inexact secant iterations
+    // New: geometric multigrid preconditioner on the mesh's lattice
+    // (Mesh::latticeN1; generated 4-node meshes), before blockJacobi.
+    bool multigrid = false;
 };
 
 class PlasticitySolver {
@@ -110,7 +113,10 @@ private:
     bool ramSaved_ = false;
     bool plastSolved_ = false;
 
+    bool gridSet_ = false;
+
     void buildProblem();
+    void prepare();
     cp_options abiOptions(bool mixed) const;
     double run(bool elastic, bool mixed);
     void dataToRAM();

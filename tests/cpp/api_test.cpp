@@ -52,6 +52,13 @@ void dump_mesh(const std::string& tag, const Mesh& m)
     dump(tag + "_meta.i32", meta);
 }
 
+void dump_lattice(const std::string& tag, const Mesh& m)
+{
+    std::vector<int> meta = {m.latticeN1, m.latticeN2, m.latticePeriodic ? 1 : 0};
+    dump(tag + "_lattice_meta.i32", meta);
+    dump(tag + "_lattice.i32", m.latticeNodes);
+}
+
 void dump_conditions(const std::string& tag, const PlasticitySolver& ps)
 {
     std::vector<double> rp, u0;
@@ -108,6 +115,29 @@ int run_cpu()
         vec2 a(3., -2.), b(0.5, 4.);
         std::vector<double> cz = {a.crossZ(b), b.crossZ(a), a.crossZ(a)};
         dump("crossz.f64", cz);
+        dump_lattice("renum", mesh);
+        dump_lattice("ringrenum", rect);
+    }
+    {
+        // the lattices the generators record (multigrid's input)
+        Mesh mesh;
+        mesh.genRectWithHole(-3., 3., -2., 2., 0.5, 48, 32, 1);
+        dump_mesh("lat_hole", mesh);
+        dump_lattice("lat_hole", Mesh(mesh));
+        mesh.genRing(0.1, 0.2, 40, 5);
+        dump_mesh("lat_ring", mesh);
+        dump_lattice("lat_ring", mesh);
+        mesh.genRectangle(0., 2., 0., 1., 9, 4);
+        dump_mesh("lat_rect", mesh);
+        dump_lattice("lat_rect", mesh);
+        mesh.genQuarterPlate(1.0, 0.1, 16, 7);
+        dump_mesh("lat_qplate", mesh);
+        dump_lattice("lat_qplate", mesh);
+        mesh.genQuarterRing(0.5, 1.0, 12, 6, 1.1);
+        dump_mesh("lat_qring", mesh);
+        dump_lattice("lat_qring", mesh);
+        mesh.genRectWithHole(-3., 3., -2., 2., 0.5, 6, 4, 2);
+        dump_lattice("lat_hole2", mesh);
     }
     {
         // testRing (Main.cpp:75-106): position-dependent border force, h = 0.1
@@ -225,6 +255,16 @@ int run_cpu()
         } catch (const std::invalid_argument&) {
             ++refused;
         }
+        {
+            // multigrid needs a lattice: none on an 8-node mesh
+            PlasticitySolver ps(Material(), mesh, LoadConditions());
+            ps.options.multigrid = true;
+            try {
+                ps.solve();
+            } catch (const std::invalid_argument&) {
+                ++refused;
+            }
+        }
         mesh.genRectangle(0, 1, 0, 1, 2, 2);
         PlasticitySolver ps(Material(), mesh, LoadConditions());
         try {
@@ -319,6 +359,31 @@ int run_gpu()
         ps.saveAsVtk(outdir + "/holePlatePlastPow.vtk");
         ps.polarCoord = true;
         ps.saveAsVtk(outdir + "/holePlatePlastPow_polar.vtk");
+    }
+    {
+        // the same with the multigrid preconditioner on the plate's periodic
+        // lattice, then on the plate renumbered by direction (contactTest's
+        // numbering), FP64 and mixed
+        for (int k = 0; k < 3; ++k) {
+            Mesh mesh;
+            LoadConditions lc;
+            Material m;
+            hole_plate(mesh, lc, m);
+            if (k > 0) mesh.renumByDirection();
+            PlasticitySolver ps(m, mesh, lc);
+            ps.options.multigrid = true;
+            ps.options.mixedPrecision = k == 2;
+            ps.iterOutput = false;
+            const double t = ps.solve();
+            const cp_stats& st = ps.lastStats();
+            const std::string tag = "plastmg" + std::to_string(k);
+            dump(tag + "_u.f64", ps.displacements());
+            std::vector<double> F;
+            for (int f = 0; f < CP_NFIELDS; ++f) F.insert(F.end(), ps.field(f).begin(), ps.field(f).end());
+            dump(tag + "_fields.f64", F);
+            std::vector<double> s = {(double)st.outer_iters, st.rel_err, (double)st.status, (double)st.cg_iters, t};
+            dump(tag + "_stats.f64", s);
+        }
     }
     {
         // solverTest (Main.cpp:14-38): elastic 4x4 block, end traction, mid fixings

@@ -109,7 +109,7 @@ void check_mesh(M& m, const char* what)
     // renumberings on a copy
     M r = m;
     expect(cp_mesh_renum_by_direction(r.nn, r.x.data(), r.y.data(), r.ne, r.npe, r.conn.data(), (int)r.bord.size(),
-                                      r.bord.data(), 1., 1.),
+                                      r.bord.data(), 1., 1., nullptr),
            CP_OK, what);
     add(r.conn);
     add(r.x);
@@ -175,11 +175,11 @@ int main()
     {
         M r = meshes[0];
         expect(cp_mesh_renumerate_ring(r.nn, r.ne, r.x.data(), r.y.data(), r.conn.data(), r.nb, r.bord.data(),
-                                       r.blen.data(), 8),
+                                       r.blen.data(), 8, nullptr),
                CP_OK, "renumerate ring");
         add(r.conn);
         expect(cp_mesh_renumerate_ring(r.nn, r.ne, r.x.data(), r.y.data(), r.conn.data(), r.nb, r.bord.data(),
-                                       r.blen.data(), 7),
+                                       r.blen.data(), 7, nullptr),
                CP_ERR_INVALID, "renumerate ring, bad count");
     }
     // .inp round trip and the reader's errors
@@ -212,7 +212,7 @@ int main()
     // refusals
     {
         M r = meshes[3];
-        expect(cp_mesh_renum_by_direction(r.nn, r.x.data(), r.y.data(), r.ne, 4, r.conn.data(), 0, nullptr, NAN, 1.),
+        expect(cp_mesh_renum_by_direction(r.nn, r.x.data(), r.y.data(), r.ne, 4, r.conn.data(), 0, nullptr, NAN, 1., nullptr),
                CP_ERR_INVALID, "renum NaN direction");
         r.conn[5] = r.nn + 3;
         long long nnz = 0;

@@ -259,9 +259,63 @@ def test_material_laws_on_host(cpu_out):
 
 
 def test_unsupported_paths_refuse(cpu_out):
-    """solveCPU and the 4-node mesh analysis of an 8-node mesh throw instead
-    of silently degrading."""
-    assert rd(cpu_out, "refused.i32")[0] == 2
+    """solveCPU, the 4-node mesh analysis of an 8-node mesh and multigrid on
+    a mesh with no lattice throw instead of silently degrading."""
+    assert rd(cpu_out, "refused.i32")[0] == 3
+
+
+def lattice_cells_ok(n1, n2, periodic, nodes, elem4):
+    """Every quad occupies one cell of the lattice: its corners sit at
+    (a, b), (a + 1, b), (a, b + 1), (a + 1, b + 1), direction 1 taken modulo
+    n1 when it is periodic.  Returns the number of quads that do not."""
+    nn = nodes.size
+    assert n1 * n2 == nn and np.array_equal(np.sort(nodes), np.arange(nn))
+    lg = np.empty(nn, dtype=np.int64)
+    lg[nodes] = np.arange(nn)
+    i1, i2 = lg[elem4] // n2, lg[elem4] % n2
+    bad = 0
+    for a, b in zip(i1, i2):
+        if np.ptp(b) != 1 or sorted(np.bincount(b - b.min()).tolist()) != [2, 2]:
+            bad += 1
+            continue
+        lo = a.min()
+        d = (a - lo) % n1 if periodic else a - lo
+        if periodic and d.max() > 1:   # the cell across the seam
+            d = (a - a.max()) % n1
+        if sorted(d.tolist()) != [0, 0, 1, 1]:
+            bad += 1
+    return bad
+
+
+def test_mesh_lattices_through_cpp_api(cpu_out):
+    """Mesh records the lattice of the structured meshes it generates
+    (multigrid's input): rectangles and quarter meshes node by node, plates
+    with a hole and rings as 2 (N1 + N2) or N_phi rays closing around the
+    hole; renumByDirection and renumerateRing carry it along.  Checked by
+    the defining property: every quad is one lattice cell."""
+    want = {"lat_hole": (160, 33, 1), "lat_ring": (40, 6, 1), "lat_rect": (10, 5, 0),
+            "lat_qplate": (17, 8, 0), "lat_qring": (13, 7, 0)}
+    for tag, dims in want.items():
+        n1, n2, per = rd(cpu_out, f"{tag}_lattice_meta.i32").tolist()
+        assert (n1, n2, per) == dims, tag
+        nodes = rd(cpu_out, f"{tag}_lattice.i32")
+        elem4 = rd(cpu_out, f"{tag}_elem4.i32").reshape(-1, 4)
+        assert lattice_cells_ok(n1, n2, per, nodes, elem4) == 0, tag
+        if not per:
+            assert np.array_equal(nodes, np.arange(nodes.size)), tag
+    for tag in ("renum", "ringrenum"):
+        n1, n2, per = rd(cpu_out, f"{tag}_lattice_meta.i32").tolist()
+        nodes = rd(cpu_out, f"{tag}_lattice.i32")
+        elem4 = rd(cpu_out, f"{tag}_elem4.i32").reshape(-1, 4)
+        assert n1 > 0 and lattice_cells_ok(n1, n2, per, nodes, elem4) == 0, tag
+        assert not np.array_equal(nodes, np.arange(nodes.size)), tag
+    # a hole plate's seam lies inside the lattice: the property is not vacuous
+    n1, n2, per = want["lat_hole"]
+    nodes = rd(cpu_out, "lat_hole_lattice.i32")
+    elem4 = rd(cpu_out, "lat_hole_elem4.i32").reshape(-1, 4)
+    assert lattice_cells_ok(n1, n2, 0, nodes, elem4) == n2 - 1
+    assert rd(cpu_out, "lat_hole2_lattice_meta.i32").tolist() == [0, 0, 0]
+    assert rd(cpu_out, "lat_hole2_lattice.i32").size == 0
 
 
 def test_load_from_file_through_cpp_api(cpu_out):
@@ -351,6 +405,30 @@ def test_plasticity_test_through_cpp_api(tmp_path):
                  "Sigma_intensity"):
         assert rel(VF[name], fx["f_" + name]) < 2e-7, name
     assert rel(VF["Displacement"].reshape(-1, 3)[:, :2].ravel(), fx["f_Displacement"][:, :2].ravel()) < 2e-7
+    # SolverOptions::multigrid: the plate's periodic lattice, then the plate
+    # renumbered by direction, FP64 and mixed: the fixture to the same bar.
+    key = om.x + om.y
+    old_of_new = np.lexsort((-np.arange(om.nn), key))
+    ro = O.renum_by_direction(om, 1.0, 1.0)
+    assert np.array_equal(om.x[old_of_new], ro.x) and np.array_equal(om.y[old_of_new], ro.y)
+    u_fx = fx["f_Displacement"][:, :2]
+    for k in range(3):
+        um = rd(tmp_path, f"plastmg{k}_u.f64").reshape(-1, 2)
+        Fm = rd(tmp_path, f"plastmg{k}_fields.f64").reshape(len(A.FIELDS), -1)
+        sm = rd(tmp_path, f"plastmg{k}_stats.f64")
+        ref_u = u_fx if k == 0 else u_fx[old_of_new]
+        errs = [rel(um.ravel(), ref_u.ravel())] + [rel(Fm[O.FIELDS.index(f)], fx["f_" + n]) for f, n in
+                                                   [("sxx", "Sigma_xx"), ("sigI", "Sigma_intensity"),
+                                                    ("psi", "Psi"), ("epsI_p", "Eps_p_intensity")]]
+        fl = Fm[O.FIELDS.index("epsI")] >= mat.sigmaT / mat.E
+        print("multigrid %d: outer %d rel_err %.3g cg %d, %.2f s; errors %s, plastic %d (%d differ)"
+              % (k, sm[0], sm[1], sm[3], sm[4], " ".join("%.2e" % e for e in errs), fl.sum(),
+                 (fl != flags).sum()), flush=True)
+        assert sm[2] == 0 and abs(sm[0] - 41) <= 1 and sm[1] <= 1e-5
+        assert max(errs) < 1e-6
+        assert fl.sum() == 3528 and np.array_equal(fl, flags)
+        assert sm[3] < 41 * 20
+
     play, PF, _, _ = read_vtk(os.path.join(tmp_path, "holePlatePlastPow_polar.vtk"))
     rename = {"Eps_xx": "Eps_rr", "Eps_yy": "Eps_phiphi", "Gamma_xy": "Gamma_rphi", "Eps_p_xx": "Eps_p_rr",
               "Eps_p_yy": "Eps_p_phiphi", "Gamma_p_xy": "Gamma_p_rphi", "Sigma_xx": "Sigma_rr",

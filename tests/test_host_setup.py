@@ -279,8 +279,12 @@ def test_renum_by_direction_matches_oracle(mesh_fn, d):
     many nodes have the same projection; the projections come out sorted
     and the geometry (element areas) is unchanged."""
     pm, om = mesh_fn(A), mesh_fn(O)
-    rp, ro = A.renum_by_direction(pm, *d), O.renum_by_direction(om, *d)
+    (rp, perm), ro = A.renum_by_direction(pm, *d, with_perm=True), O.renum_by_direction(om, *d)
     assert np.array_equal(rp.x, ro.x) and np.array_equal(rp.y, ro.y) and np.array_equal(rp.conn, ro.conn)
+    # the returned permutation: node `old` is node perm[old] afterwards
+    assert np.array_equal(np.sort(perm), np.arange(pm.nn))
+    assert np.array_equal(rp.x[perm], pm.x) and np.array_equal(rp.y[perm], pm.y)
+    assert np.array_equal(rp.conn, perm[pm.conn])
     assert all(np.array_equal(a, b) for a, b in zip(rp.borders, ro.borders))
     key = rp.x * d[0] + rp.y * d[1]
     assert np.all(np.diff(key) >= 0)
