@@ -52,7 +52,7 @@ class Stats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("batch_cg", C.c_int),
-                ("reserved", C.c_int * 5)]
+                ("batch_mg", C.c_int), ("reserved", C.c_int * 4)]
 
 
 class RequestInfo(C.Structure):
@@ -526,9 +526,10 @@ def declared_functions(header=HEADER):
 class Server:
     """Continuous-batching serving loop (cp_server_*)."""
 
-    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True):
+    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True, batch_mg=True):
         cfg = ServerConfig()
         cfg.hbm_budget, cfg.max_slots, cfg.batch_cg = int(hbm_budget), int(max_slots), int(batch_cg)
+        cfg.batch_mg = int(batch_mg)
         self.ptr = C.c_void_p()
         check(lib().cp_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)))
         self.sizes = {}
@@ -550,6 +551,13 @@ class Server:
                      int(grid[1]), ip(no) if no is not None else None, C.byref(rid)))
         self.sizes[rid.value] = (mesh.nn, mesh.ne)
         return rid.value
+
+    def batch_stats(self):
+        """(batches, members summed over batches, lockstep iterations) of the
+        lockstep multigrid batches run so far."""
+        a, b, c = C.c_longlong(), C.c_longlong(), C.c_longlong()
+        check(lib().cp_server_batch_stats(self.ptr, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def step(self):
         n = C.c_int()

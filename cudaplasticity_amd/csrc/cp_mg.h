@@ -802,19 +802,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 enum { SEG_X0 = 4, SEG_FIRST = 5, SEG_RX0 = 6 };
 
 template <int W, int MODE, typename T>
-__global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
-                                                          const typename V2<T>::type* di, typename V2<T>::type* xo,
-                                                          typename V2<T>::type* dout, int second, int last, SegGrid g)
+__device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgVec<T>& v,
+                                                   const typename V2<T>::type* xi, const typename V2<T>::type* di,
+                                                   typename V2<T>::type* xo, typename V2<T>::type* dout, int second,
+                                                   int last, const SegGrid& g, int tile)
 {
     using V = typename V2<T>::type;
-    if (v.sc && v.sc->done) return;
     // per node (row k of the tile, column c at k * kSegPitch + c: padded
     // against LDS bank conflicts in the column-wise scans)
     constexpr int kSegCols = seg_cols<W>(), kSegPitch = kSegCols + 1;
     __shared__ T sy[2][kSeg * kSegPitch];    // Dt^-1 rho, then z
     __shared__ T sf[4][kSeg * kSegPitch];    // Dt^-1 L
     __shared__ T sh[4][kSeg * kSegPitch];    // Dt^-1 U
-    const int tile = xcd_block(g.ntiles);
     const int T1 = tile / g.nt2, T2 = tile - T1 * g.nt2;
     const int t1 = threadIdx.x / kSegCols, t2 = threadIdx.x - t1 * kSegCols;
     const int i1 = T1 * kSeg + t1, i2 = T2 * kSegCols + t2;
@@ -1009,6 +1008,15 @@ __global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_mg_smooth_seg(CgArgs<T
             dout[n] = d;
         }
     }
+}
+
+template <int W, int MODE, typename T>
+__global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_mg_smooth_seg(CgArgs<T> a, MgVec<T> v, const typename V2<T>::type* xi,
+                                                          const typename V2<T>::type* di, typename V2<T>::type* xo,
+                                                          typename V2<T>::type* dout, int second, int last, SegGrid g)
+{
+    if (v.sc && v.sc->done) return;
+    mg_smooth_seg_tile<W, MODE, T>(a, v, xi, di, xo, dout, second, last, g, xcd_block(g.ntiles));
 }
 
 // Coarsest level: x = D^-1 b with the explicit inverse (bs: b in LDS).
@@ -1215,12 +1223,10 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
 // is read from lv[0].v.b and its result is written to lv[0].v.xb.  The
 // coarsest level is solved by its explicit inverse, eight threads per row.
 template <typename T>
-__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
-                                                          const CgScal* sc)
+__device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, const double* Dinv, int nd)
 {
     using V = typename V2<T>::type;
     constexpr bool kRegRows = std::is_same<T, float>::value;
-    if (sc && sc->done) return;
     __shared__ __align__(16) char lds[kTailLds];
     __shared__ double lam[kTailMaxLevels];
     __shared__ double bs[kMgDenseMax];
@@ -1327,15 +1333,21 @@ __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
+                                                          const CgScal* sc)
+{
+    if (sc && sc->done) return;
+    mg_tail_body<T>(lv, nl, Dinv, nd);
+}
+
 // ------------------------------------------------------------ CG with the V-cycle
 
 // w = mask (A u), partials gamma = (r, u), delta = (w, u), rho = (r, r).
 template <int W, typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_cg_dots(CgArgs<T> a, const typename V2<T>::type* u)
+__device__ __forceinline__ void mg_cg_dots_block(const CgArgs<T>& a, const typename V2<T>::type* u, int lb)
 {
     using V = typename V2<T>::type;
-    if (a.sc->done) return;
-    const int lb = xcd_block(a.nblocks);
     const int n = lb * kBlock + threadIdx.x;
     double v[3] = {0., 0., 0.};
     if (n < a.nn) {
@@ -1358,15 +1370,20 @@ __global__ __launch_bounds__(kBlock) void k_mg_cg_dots(CgArgs<T> a, const typena
     block_partial<3>(v, a.partials, lb);
 }
 
+template <int W, typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_cg_dots(CgArgs<T> a, const typename V2<T>::type* u)
+{
+    if (a.sc->done) return;
+    mg_cg_dots_block<W, T>(a, u, xcd_block(a.nblocks));
+}
+
 // p = u + beta p, s = w + beta s, x += alpha p, r -= alpha s (own node only).
 template <typename T>
 // The updated residual's (r, r) goes to per-block partials (partials_alt),
 // so that k_mg_cg_check can stop the solve before its V-cycle.
-__global__ __launch_bounds__(kBlock) void k_mg_cg_update(CgArgs<T> a, const typename V2<T>::type* u)
+__device__ __forceinline__ void mg_cg_update_block(const CgArgs<T>& a, const typename V2<T>::type* u, int lb)
 {
     using V = typename V2<T>::type;
-    if (a.sc->done) return;
-    const int lb = xcd_block(a.nblocks);
     const int n = lb * kBlock + threadIdx.x;
     double v[1] = {0.};
     if (n < a.nn) {
@@ -1384,15 +1401,21 @@ __global__ __launch_bounds__(kBlock) void k_mg_cg_update(CgArgs<T> a, const type
     block_partial<1>(v, a.partials_alt, lb);
 }
 
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_cg_update(CgArgs<T> a, const typename V2<T>::type* u)
+{
+    if (a.sc->done) return;
+    mg_cg_update_block<T>(a, u, xcd_block(a.nblocks));
+}
+
 // Stop test of the iteration on the updated residual, before the V-cycle:
 // a solve that has converged skips its last preconditioner application
 // (the V-cycle, dots and step kernels return at once once done is set).
 // The (r, r) summed here in block order equals the one the dots kernel
 // would sum, so the stop decision is the one the step would take.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mg_cg_check(CgArgs<T> a)
+__device__ __forceinline__ void mg_cg_check_body(const CgArgs<T>& a)
 {
-    if (a.sc->done) return;
     double v[1];
     reduce_partials<1>(a.partials_alt, a.nblocks, v);
     if (threadIdx.x == 0) {
@@ -1404,6 +1427,153 @@ __global__ __launch_bounds__(kBlock) void k_mg_cg_check(CgArgs<T> a)
             sc->done = 1;
         }
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_cg_check(CgArgs<T> a)
+{
+    if (a.sc->done) return;
+    mg_cg_check_body<T>(a);
+}
+
+// ------------------------------------------------------------ batched launches
+//
+// The serving loop runs the CG iterations of several multigrid problems in
+// lockstep: each launch below runs one step of the iteration (the CG vector
+// update or its stop test, one smoothing step of some level, a restriction,
+// a prolongation, the tail, the dots, the scalar step) for every problem of
+// the batch that has that step.  Tile t of a launch is {op, blk}: block blk
+// of what op's own launch of that step runs alone, with that launch's
+// arguments args[op], copied into registers first (as a kernel's own
+// arguments are).  Each problem therefore computes what it computes alone,
+// block for block, and its partial sums are reduced in the same order: its
+// results do not depend on the batch.  (The library is compiled with
+// -ffp-contract=on, so that a multiply-add is fused by the expression it
+// comes from and not by what the optimiser sees around it: the same
+// expression rounds the same way in every kernel that inlines it.)
+
+template <typename T>
+struct StepOp {   // smoothing step (point or segment smoother)
+    CgArgs<T> a;
+    MgVec<T> v;
+    const typename V2<T>::type* xi;
+    const typename V2<T>::type* di;
+    typename V2<T>::type* xo;
+    typename V2<T>::type* dout;
+    int second, last;
+    SegGrid g;
+};
+
+template <typename T>
+struct GridOp {   // restriction (src = finer residual, dst = coarse b) or prolongation
+    MgGrid gr;
+    const typename V2<T>::type* src;
+    const typename V2<T>::type* xin;
+    typename V2<T>::type* dst;
+    const uchar2* mask;
+    int nn;
+    const CgScal* sc;
+};
+
+template <typename T>
+struct TailOp {
+    const TailLevel<T>* lv;
+    int nl, nd;
+    const double* Dinv;
+    const CgScal* sc;
+};
+
+template <typename T>
+struct CgOp {
+    CgArgs<T> a;
+    const typename V2<T>::type* u;
+};
+
+template <int W, int MODE, typename T>
+__global__ __launch_bounds__(kBlock) void k_b_smooth(const StepOp<T>* ops, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const StepOp<T> o = ops[t.x];
+    if (o.v.sc && o.v.sc->done) return;
+    const int n = t.y * kBlock + threadIdx.x;
+    if (n >= o.a.nn) return;
+    mg_smooth_node<W, MODE, T>(o.a, o.v, o.xi, o.di, o.xo, o.dout, o.second, o.last, n);
+}
+
+template <int W, int MODE, typename T>
+__global__ __launch_bounds__(kSeg * seg_cols<W>()) void k_b_smooth_seg(const StepOp<T>* ops, const int2* tiles,
+                                                                       int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const StepOp<T> o = ops[t.x];
+    if (o.v.sc && o.v.sc->done) return;
+    mg_smooth_seg_tile<W, MODE, T>(o.a, o.v, o.xi, o.di, o.xo, o.dout, o.second, o.last, o.g, t.y);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_restrict(const GridOp<T>* ops, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const GridOp<T> o = ops[t.x];
+    if (o.sc && o.sc->done) return;
+    const int C = t.y * kBlock + threadIdx.x;
+    if (C >= o.nn) return;
+    mg_restrict_node<T>(o.gr, o.src, o.dst, o.mask, C);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_prolong(const GridOp<T>* ops, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const GridOp<T> o = ops[t.x];
+    if (o.sc && o.sc->done) return;
+    const int n = t.y * kBlock + threadIdx.x;
+    if (n >= o.nn) return;
+    mg_prolong_node<T>(o.gr, o.src, o.xin, o.dst, o.mask, n);
+}
+
+// One workgroup per problem.
+template <typename T>
+__global__ __launch_bounds__(kTailThreads) void k_b_tail(const TailOp<T>* ops)
+{
+    const TailOp<T> o = ops[blockIdx.x];
+    if (o.sc && o.sc->done) return;
+    mg_tail_body<T>(o.lv, o.nl, o.Dinv, o.nd);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_update(const CgOp<T>* ops, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const CgOp<T> o = ops[t.x];
+    if (o.a.sc->done) return;
+    mg_cg_update_block<T>(o.a, o.u, t.y);
+}
+
+template <int W, typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_dots(const CgOp<T>* ops, const int2* tiles, int ntiles)
+{
+    const int2 t = tiles[xcd_block(ntiles)];
+    const CgOp<T> o = ops[t.x];
+    if (o.a.sc->done) return;
+    mg_cg_dots_block<W, T>(o.a, o.u, t.y);
+}
+
+// One workgroup per problem.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_check(const CgOp<T>* ops)
+{
+    const CgOp<T> o = ops[blockIdx.x];
+    if (o.a.sc->done) return;
+    mg_cg_check_body<T>(o.a);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_b_cg_step(const CgOp<T>* ops)
+{
+    const CgOp<T> o = ops[blockIdx.x];
+    if (o.a.sc->done) return;
+    cg_step_scalars(o.a.sc, o.a.partials, o.a.nblocks);
 }
 
 // Zero p and s before the first update (beta = 0 multiplies them).

@@ -3,6 +3,8 @@
 // assembly, the V-cycle and CG preconditioned by it.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
+#include <map>
 #include <type_traits>
 #include <vector>
 
@@ -328,38 +330,126 @@ int mg_setup(cp_problem* P, int precision, int full)
     return precision == CP_FP64 ? mg_setup_t<double>(P, nl) : mg_setup_t<float>(P, nl);
 }
 
-template <int W, int MODE, typename T>
-static void smooth(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi, const void* di, void* xo,
-                   void* dout, int second, int last, hipStream_t st)
+// ------------------------------------------------------------ V-cycle steps
+//
+// The V-cycle and the CG iteration are written once, against an emitter that
+// takes each step with its key (its place in the iteration: section, level,
+// step) and its launch size: Direct launches it at once on the problem's
+// stream; BatchRec (serving) collects the steps of several problems and
+// launches each key's steps of all of them together.
+
+enum Sect : unsigned long long { SECT_UPDATE = 0, SECT_CHECK = 1, SECT_DOWN = 2, SECT_TAIL = 3, SECT_UP = 4,
+                                 SECT_DOTS = 5, SECT_STEP = 6 };
+constexpr unsigned long long step_key(Sect s, int level, int step)
+{
+    // the up-sweep runs from the coarse levels to the fine ones
+    return (static_cast<unsigned long long>(s) << 40) |
+           (static_cast<unsigned long long>(s == SECT_UP ? 65535 - level : level) << 8) | (unsigned)step;
+}
+constexpr int kStepRestrict = 16;   // restriction after a level's down steps
+
+struct Direct {
+    hipStream_t st;
+    template <int W, int MODE, typename T>
+    void smooth(unsigned long long, const cpk::StepOp<T>& o, int nblocks)
+    {
+        hipLaunchKernelGGL((cpk::k_mg_smooth<W, MODE, T>), dim3(nblocks), dim3(cpk::kBlock), 0, st, o.a, o.v, o.xi,
+                           o.di, o.xo, o.dout, o.second, o.last);
+    }
+    template <int W, int MODE, typename T>
+    void smooth_seg(unsigned long long, const cpk::StepOp<T>& o, int ntiles)
+    {
+        hipLaunchKernelGGL((cpk::k_mg_smooth_seg<W, MODE, T>), dim3(ntiles), dim3(cpk::kSeg * cpk::seg_cols<W>()), 0,
+                           st, o.a, o.v, o.xi, o.di, o.xo, o.dout, o.second, o.last, o.g);
+    }
+    template <typename T>
+    void restrict_(unsigned long long, const cpk::GridOp<T>& o, int nblocks)
+    {
+        hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(nblocks), dim3(cpk::kBlock), 0, st, o.gr, o.src, o.dst, o.mask,
+                           o.nn, nblocks, o.sc);
+    }
+    template <typename T>
+    void prolong(unsigned long long, const cpk::GridOp<T>& o, int nblocks)
+    {
+        hipLaunchKernelGGL(cpk::k_mg_prolong<T>, dim3(nblocks), dim3(cpk::kBlock), 0, st, o.gr, o.src, o.xin, o.dst,
+                           o.mask, o.nn, nblocks, o.sc);
+    }
+    template <typename T>
+    void tail(unsigned long long, const cpk::TailOp<T>& o)
+    {
+        hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st, o.lv, o.nl, o.Dinv, o.nd, o.sc);
+    }
+    template <typename T>
+    void update(unsigned long long, const cpk::CgOp<T>& o, int nblocks)
+    {
+        hipLaunchKernelGGL(cpk::k_mg_cg_update<T>, dim3(nblocks), dim3(cpk::kBlock), 0, st, o.a, o.u);
+    }
+    template <typename T>
+    void check(unsigned long long, const cpk::CgOp<T>& o)
+    {
+        hipLaunchKernelGGL(cpk::k_mg_cg_check<T>, dim3(1), dim3(cpk::kBlock), 0, st, o.a);
+    }
+    template <int W, typename T>
+    void dots(unsigned long long, const cpk::CgOp<T>& o, int nblocks)
+    {
+        hipLaunchKernelGGL((cpk::k_mg_cg_dots<W, T>), dim3(nblocks), dim3(cpk::kBlock), 0, st, o.a, o.u);
+    }
+    template <typename T>
+    void step(unsigned long long, const cpk::CgOp<T>& o)
+    {
+        hipLaunchKernelGGL(cpk::k_cg_step<T>, dim3(1), dim3(cpk::kBlock), 0, st, o.a);
+    }
+};
+
+template <int W, int MODE, typename T, typename E>
+static void smooth(E& e, unsigned long long key, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi,
+                   const void* di, void* xo, void* dout, int second, int last)
 {
     using V = typename cpk::V2<T>::type;
-    hipLaunchKernelGGL((cpk::k_mg_smooth<W, MODE, T>), dim3(a.nblocks), dim3(cpk::kBlock), 0, st, a, v, (const V*)xi,
-                       (const V*)di, (V*)xo, (V*)dout, second, last);
+    cpk::StepOp<T> o{};
+    o.a = a;
+    o.v = v;
+    o.xi = (const V*)xi;
+    o.di = (const V*)di;
+    o.xo = (V*)xo;
+    o.dout = (V*)dout;
+    o.second = second;
+    o.last = last;
+    e.template smooth<W, MODE, T>(key, o, a.nblocks);
 }
 
-template <int W, typename T>
-static void smooth_level_down(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, hipStream_t st)
+template <int W, typename T, typename E>
+static void smooth_level_down(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v)
 {
-    smooth<W, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
-    smooth<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
-    smooth<W, cpk::MG_RESID, T>(a, v, v.xb, v.db, v.xa, nullptr, 0, 0, st);
+    smooth<W, cpk::MG_PRE1, T>(e, step_key(SECT_DOWN, l, 0), a, v, nullptr, nullptr, v.xa, v.da, 0, 0);
+    smooth<W, cpk::MG_CHEB, T>(e, step_key(SECT_DOWN, l, 1), a, v, v.xa, v.da, v.xb, v.db, 1, 0);
+    smooth<W, cpk::MG_RESID, T>(e, step_key(SECT_DOWN, l, 2), a, v, v.xb, v.db, v.xa, nullptr, 0, 0);
 }
 
-template <int W, typename T>
-static void smooth_level_up(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, hipStream_t st)
+template <int W, typename T, typename E>
+static void smooth_level_up(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v)
 {
-    smooth<W, cpk::MG_POST1, T>(a, v, v.xb, nullptr, nullptr, v.da, 0, 0, st);
-    smooth<W, cpk::MG_CHEB, T>(a, v, v.xb, v.da, v.xa, v.db, 0, 0, st);
-    smooth<W, cpk::MG_CHEB, T>(a, v, v.xa, v.db, v.xb, nullptr, 1, 1, st);
+    smooth<W, cpk::MG_POST1, T>(e, step_key(SECT_UP, l, 1), a, v, v.xb, nullptr, nullptr, v.da, 0, 0);
+    smooth<W, cpk::MG_CHEB, T>(e, step_key(SECT_UP, l, 2), a, v, v.xb, v.da, v.xa, v.db, 0, 0);
+    smooth<W, cpk::MG_CHEB, T>(e, step_key(SECT_UP, l, 3), a, v, v.xa, v.db, v.xb, nullptr, 1, 1);
 }
 
-template <int W, int MODE, typename T>
-static void smooth_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi, const void* di, void* xo,
-                       void* dout, int second, int last, const cpk::SegGrid& g, hipStream_t st)
+template <int W, int MODE, typename T, typename E>
+static void smooth_seg(E& e, unsigned long long key, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const void* xi,
+                       const void* di, void* xo, void* dout, int second, int last, const cpk::SegGrid& g)
 {
     using V = typename cpk::V2<T>::type;
-    hipLaunchKernelGGL((cpk::k_mg_smooth_seg<W, MODE, T>), dim3(g.ntiles), dim3(cpk::kSeg * cpk::seg_cols<W>()), 0, st, a, v,
-                       (const V*)xi, (const V*)di, (V*)xo, (V*)dout, second, last, g);
+    cpk::StepOp<T> o{};
+    o.a = a;
+    o.v = v;
+    o.xi = (const V*)xi;
+    o.di = (const V*)di;
+    o.xo = (V*)xo;
+    o.dout = (V*)dout;
+    o.second = second;
+    o.last = last;
+    o.g = g;
+    e.template smooth_seg<W, MODE, T>(key, o, g.ntiles);
 }
 
 // The same steps with the segment smoother: x1 = c0 D^-1 b takes a launch
@@ -369,33 +459,35 @@ static void smooth_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const vo
 // Degree 2 stops the same recurrence one step earlier (the step-k
 // coefficients do not depend on the degree): the pre-smoothed x then lands
 // in xb, the prolongation writes xa, and the post-smoothing ends in xb.
-template <int W, typename T>
-static void smooth_level_down_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g, int deg,
-                                  hipStream_t st)
+template <int W, typename T, typename E>
+static void smooth_level_down_seg(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v,
+                                  const cpk::SegGrid& g, int deg)
 {
-    if (g.rres) smooth_seg<W, cpk::SEG_RX0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
-    else smooth_seg<W, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g, st);
-    smooth_seg<W, cpk::SEG_FIRST, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g, st);
+    const auto k = [&](int s) { return step_key(SECT_DOWN, l, s); };
+    if (g.rres) smooth_seg<W, cpk::SEG_RX0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
+    else smooth_seg<W, cpk::SEG_X0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
+    smooth_seg<W, cpk::SEG_FIRST, T>(e, k(1), a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g);
     if (deg == 2) {
-        smooth<W, cpk::MG_RESID, T>(a, v, v.xa, v.da, v.xb, nullptr, 0, 0, st);
+        smooth<W, cpk::MG_RESID, T>(e, k(2), a, v, v.xa, v.da, v.xb, nullptr, 0, 0);
         return;
     }
-    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, g, st);
-    smooth<W, cpk::MG_RESID, T>(a, v, v.xb, v.db, v.xa, nullptr, 0, 0, st);
+    smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xa, v.da, v.xb, v.db, 1, 0, g);
+    smooth<W, cpk::MG_RESID, T>(e, k(3), a, v, v.xb, v.db, v.xa, nullptr, 0, 0);
 }
 
-template <int W, typename T>
-static void smooth_level_up_seg(const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g, int deg,
-                                hipStream_t st)
+template <int W, typename T, typename E>
+static void smooth_level_up_seg(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v, const cpk::SegGrid& g,
+                                int deg)
 {
+    const auto k = [&](int s) { return step_key(SECT_UP, l, s); };
     if (deg == 2) {
-        smooth_seg<W, cpk::MG_POST1, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g, st);
-        smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, nullptr, 0, 1, g, st);
+        smooth_seg<W, cpk::MG_POST1, T>(e, k(1), a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g);
+        smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xa, v.da, v.xb, nullptr, 0, 1, g);
         return;
     }
-    smooth_seg<W, cpk::MG_POST1, T>(a, v, v.xb, nullptr, nullptr, v.da, 0, 0, g, st);
-    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xb, v.da, v.xa, v.db, 0, 0, g, st);
-    smooth_seg<W, cpk::MG_CHEB, T>(a, v, v.xa, v.db, v.xb, nullptr, 1, 1, g, st);
+    smooth_seg<W, cpk::MG_POST1, T>(e, k(1), a, v, v.xb, nullptr, nullptr, v.da, 0, 0, g);
+    smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xb, v.da, v.xa, v.db, 0, 0, g);
+    smooth_seg<W, cpk::MG_CHEB, T>(e, k(3), a, v, v.xa, v.db, v.xb, nullptr, 1, 1, g);
 }
 
 template <typename T, typename F>
@@ -417,13 +509,12 @@ static void by_level(const cp_problem* P, int l, const F& f)
 }
 
 // u = M r: one V-cycle; the result lands in level 0's xb.
-template <typename T>
-static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const cpk::CgScal* sc)
+template <typename T, typename E>
+static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs, const cpk::CgScal* sc)
 {
     using V = typename cpk::V2<T>::type;
     MgState& G = *P->mg;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
-    hipStream_t st = P->stream;
     auto vec = [&](int l) {
         MgLevel& L = G.L[l];
         cpk::MgVec<T> v{};
@@ -471,17 +562,30 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
                 sg.rg = grid(l - 1);
                 sg.rres = G.L[l - 1].vec[pi][5];
             }
-            by_level<T>(P, l, [&](auto w) { smooth_level_down_seg<decltype(w)::value, T>(a, v, sg, degree(l), st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_down_seg<decltype(w)::value, T>(e, l, a, v, sg, degree(l)); });
         } else {
-            by_level<T>(P, l, [&](auto w) { smooth_level_down<decltype(w)::value, T>(a, v, st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_down<decltype(w)::value, T>(e, l, a, v); });
         }
         if (fused_restrict(l + 1)) continue;
         const MgLevel& C = G.L[l + 1];
-        hipLaunchKernelGGL(cpk::k_mg_restrict<T>, dim3(C.nb), dim3(cpk::kBlock), 0, st, grid(l), (const V*)v.res,
-                           (V*)C.vec[pi][0], (const uchar2*)C.mask, C.nn, C.nb, sc);
+        cpk::GridOp<T> o{};
+        o.gr = grid(l);
+        o.src = (const V*)v.res;
+        o.dst = (V*)C.vec[pi][0];
+        o.mask = (const uchar2*)C.mask;
+        o.nn = C.nn;
+        o.sc = sc;
+        e.template restrict_<T>(step_key(SECT_DOWN, l, kStepRestrict), o, C.nb);
     }
-    hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st,
-                       (const cpk::TailLevel<T>*)G.tail[pi], G.nlev - G.lt, (const double*)G.dense, G.nd, sc);
+    {
+        cpk::TailOp<T> o{};
+        o.lv = (const cpk::TailLevel<T>*)G.tail[pi];
+        o.nl = G.nlev - G.lt;
+        o.Dinv = (const double*)G.dense;
+        o.nd = G.nd;
+        o.sc = sc;
+        e.template tail<T>(step_key(SECT_TAIL, 0, 0), o);
+    }
     for (int l = last - 1; l >= 0; --l) {
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
@@ -490,14 +594,20 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
         // the pre-smoothed x is in xa (degree 3) or xb (degree 2); the
         // prolongation writes the other buffer
         const bool d2 = degree(l) == 2;
-        hipLaunchKernelGGL(cpk::k_mg_prolong<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, grid(l),
-                           (const V*)C.vec[pi][2], (const V*)(d2 ? v.xb : v.xa), d2 ? v.xa : v.xb,
-                           (const uchar2*)a.mask, L.nn, L.nb, sc);
+        cpk::GridOp<T> o{};
+        o.gr = grid(l);
+        o.src = (const V*)C.vec[pi][2];
+        o.xin = (const V*)(d2 ? v.xb : v.xa);
+        o.dst = d2 ? v.xa : v.xb;
+        o.mask = (const uchar2*)a.mask;
+        o.nn = L.nn;
+        o.sc = sc;
+        e.template prolong<T>(step_key(SECT_UP, l, 0), o, L.nb);
         if (L.seg) {
             const cpk::SegGrid sg = seg_grid(G, l);
-            by_level<T>(P, l, [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(a, v, sg, degree(l), st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(e, l, a, v, sg, degree(l)); });
         } else {
-            by_level<T>(P, l, [&](auto w) { smooth_level_up<decltype(w)::value, T>(a, v, st); });
+            by_level<T>(P, l, [&](auto w) { smooth_level_up<decltype(w)::value, T>(e, l, a, v); });
         }
     }
 }
@@ -505,19 +615,18 @@ static void mg_vcycle(cp_problem* P, const typename cpk::V2<T>::type* rhs, const
 // One iteration of CG preconditioned by the V-cycle: p, s, x, r updates,
 // the stop test on the new r, u = M r, w = A u with the three dot products,
 // scalars of the next step.
-template <typename T>
-static void mg_cg_iteration(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc,
+template <typename T, typename E>
+static void mg_cg_iteration(E& e, cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc,
                             const typename cpk::V2<T>::type* u)
 {
-    hipStream_t st = P->stream;
-    const dim3 g(P->nbn), b(cpk::kBlock);
-    hipLaunchKernelGGL(cpk::k_mg_cg_update<T>, g, b, 0, st, a, u);
-    hipLaunchKernelGGL(cpk::k_mg_cg_check<T>, dim3(1), b, 0, st, a);
-    mg_vcycle<T>(P, a.r[0], dsc);
-    by_width<T>(P->max_w, [&](auto w) {
-        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
-    });
-    hipLaunchKernelGGL(cpk::k_cg_step<T>, dim3(1), b, 0, st, a);
+    cpk::CgOp<T> o{};
+    o.a = a;
+    o.u = u;
+    e.template update<T>(step_key(SECT_UPDATE, 0, 0), o, P->nbn);
+    e.template check<T>(step_key(SECT_CHECK, 0, 0), o);
+    mg_vcycle<T>(e, P, a.r[0], dsc);
+    by_width<T>(P->max_w, [&](auto w) { e.template dots<decltype(w)::value, T>(step_key(SECT_DOTS, 0, 0), o, P->nbn); });
+    e.template step<T>(step_key(SECT_STEP, 0, 0), o);
 }
 
 // The first V-cycle of a solve: u_0 = M r_0, w_0 = A u_0 with the dots,
@@ -528,12 +637,243 @@ static void mg_cg_start(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc
     hipStream_t st = P->stream;
     const dim3 g(P->nbn), b(cpk::kBlock);
     hipLaunchKernelGGL(cpk::k_mg_cg_clear<T>, g, b, 0, st, a);
-    mg_vcycle<T>(P, a.r[0], dsc);
+    Direct e{st};
+    mg_vcycle<T>(e, P, a.r[0], dsc);
     by_width<T>(P->max_w, [&](auto w) {
         hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
     });
     hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
 }
+
+// ------------------------------------------------------------ lockstep batches
+//
+// BatchRec collects the steps of one CG iteration of every problem of a
+// batch, grouped by (key, kernel): each group becomes one launch of the
+// batched kernel (cp_mg.h) over the tiles {op, blk} of all its problems,
+// in key order, so every problem's steps keep their order and each launch
+// advances every problem that has that step.
+
+using BatchLaunchFn = void (*)(const void* ops, const int2* tiles, int n, hipStream_t st);
+
+template <int W, int MODE, typename T>
+static void lb_smooth(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL((cpk::k_b_smooth<W, MODE, T>), dim3(n), dim3(cpk::kBlock), 0, st,
+                       (const cpk::StepOp<T>*)o, t, n);
+}
+template <int W, int MODE, typename T>
+static void lb_smooth_seg(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL((cpk::k_b_smooth_seg<W, MODE, T>), dim3(n), dim3(cpk::kSeg * cpk::seg_cols<W>()), 0, st,
+                       (const cpk::StepOp<T>*)o, t, n);
+}
+template <typename T>
+static void lb_restrict(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_restrict<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::GridOp<T>*)o, t, n);
+}
+template <typename T>
+static void lb_prolong(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_prolong<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::GridOp<T>*)o, t, n);
+}
+template <typename T>
+static void lb_tail(const void* o, const int2*, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_tail<T>, dim3(n), dim3(cpk::kTailThreads), 0, st, (const cpk::TailOp<T>*)o);
+}
+template <typename T>
+static void lb_update(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_cg_update<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o, t, n);
+}
+template <typename T>
+static void lb_check(const void* o, const int2*, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_cg_check<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o);
+}
+template <int W, typename T>
+static void lb_dots(const void* o, const int2* t, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL((cpk::k_b_cg_dots<W, T>), dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o, t, n);
+}
+template <typename T>
+static void lb_step(const void* o, const int2*, int n, hipStream_t st)
+{
+    hipLaunchKernelGGL(cpk::k_b_cg_step<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o);
+}
+
+struct BatchRec {
+    struct Group {
+        BatchLaunchFn fn = nullptr;
+        bool tiled = true;         // false: one workgroup per op
+        size_t op_size = 0;
+        std::vector<char> ops;
+        std::vector<int> nblk;
+    };
+    // ordered by step key; within a key, the groups of different kernels
+    // hold different problems and may run in any order
+    std::map<std::pair<unsigned long long, BatchLaunchFn>, Group> groups;
+
+    template <typename O>
+    void add(unsigned long long key, BatchLaunchFn fn, bool tiled, const O& o, int nblk)
+    {
+        Group& g = groups[{key, fn}];
+        g.fn = fn;
+        g.tiled = tiled;
+        g.op_size = sizeof(O);
+        const char* p = reinterpret_cast<const char*>(&o);
+        g.ops.insert(g.ops.end(), p, p + sizeof(O));
+        g.nblk.push_back(nblk);
+    }
+    template <int W, int MODE, typename T>
+    void smooth(unsigned long long k, const cpk::StepOp<T>& o, int nb) { add(k, &lb_smooth<W, MODE, T>, true, o, nb); }
+    template <int W, int MODE, typename T>
+    void smooth_seg(unsigned long long k, const cpk::StepOp<T>& o, int nt)
+    {
+        add(k, &lb_smooth_seg<W, MODE, T>, true, o, nt);
+    }
+    template <typename T>
+    void restrict_(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, &lb_restrict<T>, true, o, nb); }
+    template <typename T>
+    void prolong(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, &lb_prolong<T>, true, o, nb); }
+    template <typename T>
+    void tail(unsigned long long k, const cpk::TailOp<T>& o) { add(k, &lb_tail<T>, false, o, 1); }
+    template <typename T>
+    void update(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, &lb_update<T>, true, o, nb); }
+    template <typename T>
+    void check(unsigned long long k, const cpk::CgOp<T>& o) { add(k, &lb_check<T>, false, o, 1); }
+    template <int W, typename T>
+    void dots(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, &lb_dots<W, T>, true, o, nb); }
+    template <typename T>
+    void step(unsigned long long k, const cpk::CgOp<T>& o) { add(k, &lb_step<T>, false, o, 1); }
+};
+
+// A batch's launches, their tables in one device buffer, and the problems
+// they were recorded for (rebuilt when that set changes).
+struct cp_mg_batch {
+    struct Launch {
+        cp::BatchLaunchFn fn;
+        size_t ops_off, tiles_off;
+        int n;
+    };
+    std::vector<long long> ids;   // the callers' names of the problems recorded
+    int precision = -1;
+    std::vector<Launch> launches;
+    char* dev = nullptr;
+    size_t dev_size = 0;
+    hipEvent_t done = nullptr;    // the last run's launches have finished
+    bool ran = false;
+    ~cp_mg_batch()
+    {
+        if (ran) (void)hipEventSynchronize(done);
+        if (done) (void)hipEventDestroy(done);
+        if (dev) (void)hipFree(dev);
+    }
+};
+
+void mg_batch_destroy(cp_mg_batch* b) { delete b; }
+
+template <typename T>
+static int mg_batch_build(cp_mg_batch* B, cp_problem* const* probs, const long long* ids, int n)
+{
+    BatchRec rec;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    for (int i = 0; i < n; ++i) {
+        cp_problem* P = probs[i];
+        if (!P->mg || !P->mg->alloc[pi] || !P->mg->tail[pi])
+            return fail(CP_ERR_STATE, "multigrid batch: a problem without its prepared hierarchy");
+        cpk::CgArgs<T> a;
+        if constexpr (std::is_same<T, double>::value) a = P->cg64_args();
+        else a = P->cg32_args();
+        if (a.nblocks != P->nbn) return fail(CP_ERR_STATE, "multigrid batch: inconsistent block count");
+        using V = typename cpk::V2<T>::type;
+        mg_cg_iteration<T>(rec, P, a, a.sc, (const V*)P->mg->L[0].vec[pi][2]);
+    }
+    // layout: per group its ops, then its tiles (tiled groups), 256-byte aligned
+    auto align = [](size_t v) { return (v + 255) & ~size_t(255); };
+    size_t need = 0;
+    for (const auto& kv : rec.groups) {
+        const BatchRec::Group& g = kv.second;
+        need += align(g.ops.size());
+        if (g.tiled) {
+            long long nt = 0;
+            for (int b : g.nblk) nt += b;
+            if (nt <= 0 || nt > (1LL << 30)) return fail(CP_ERR_INVALID, "multigrid batch: bad tile count");
+            need += align((size_t)nt * sizeof(int2));
+        }
+    }
+    std::vector<char> host(need, 0);
+    std::vector<cp_mg_batch::Launch> launches;
+    size_t off = 0;
+    for (const auto& kv : rec.groups) {
+        const BatchRec::Group& g = kv.second;
+        const int nops = (int)g.nblk.size();
+        cp_mg_batch::Launch L{g.fn, off, 0, 0};
+        std::memcpy(host.data() + off, g.ops.data(), g.ops.size());
+        off += align(g.ops.size());
+        if (g.tiled) {
+            L.tiles_off = off;
+            int2* t = reinterpret_cast<int2*>(host.data() + off);
+            int nt = 0;
+            for (int i = 0; i < nops; ++i)
+                for (int b = 0; b < g.nblk[i]; ++b) t[nt++] = make_int2(i, b);
+            L.n = nt;
+            off += align((size_t)nt * sizeof(int2));
+        } else {
+            L.n = nops;
+        }
+        launches.push_back(L);
+    }
+    // the previous run may still read the old tables
+    if (B->ran) CP_HIP(hipEventSynchronize(B->done));
+    if (need > B->dev_size) {
+        if (B->dev) CP_HIP(hipFree(B->dev));
+        B->dev = nullptr;
+        B->dev_size = 0;
+        if (hipMalloc(&B->dev, need) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(CP_ERR_NOMEM, "multigrid batch: device allocation failed");
+        }
+        B->dev_size = need;
+    }
+    CP_HIP(hipMemcpy(B->dev, host.data(), need, hipMemcpyHostToDevice));
+    B->launches.swap(launches);
+    B->ids.assign(ids, ids + n);
+    B->precision = pi == 0 ? CP_FP64 : CP_MIXED;
+    return CP_OK;
+}
+
+int mg_batch_run(cp_mg_batch** pb, cp_problem* const* probs, const long long* ids, int n, int precision, int iters,
+                 hipStream_t st)
+{
+    if (n < 1 || iters < 1) return fail(CP_ERR_INVALID, "multigrid batch: empty");
+    if (!*pb) {
+        auto* b = new cp_mg_batch();
+        if (hipEventCreateWithFlags(&b->done, hipEventDisableTiming) != hipSuccess) {
+            delete b;
+            return fail(CP_ERR_DEVICE, "multigrid batch: event creation failed");
+        }
+        *pb = b;
+    }
+    cp_mg_batch* B = *pb;
+    const bool same = B->precision == precision && (int)B->ids.size() == n && std::equal(B->ids.begin(), B->ids.end(), ids);
+    if (!same) {
+        B->ids.clear();
+        if (int s = precision == CP_FP64 ? mg_batch_build<double>(B, probs, ids, n)
+                                         : mg_batch_build<float>(B, probs, ids, n))
+            return s;
+    }
+    for (int j = 0; j < iters; ++j)
+        for (const cp_mg_batch::Launch& L : B->launches)
+            L.fn(B->dev + L.ops_off, reinterpret_cast<const int2*>(B->dev + L.tiles_off), L.n, st);
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipEventRecord(B->done, st));
+    B->ran = true;
+    return CP_OK;
+}
+
+int mg_batch_launches(const cp_mg_batch* b) { return b ? (int)b->launches.size() : 0; }
 
 template <typename F>
 static int capture(cp_problem* P, hipGraphExec_t* exec, const F& record)
@@ -563,7 +903,11 @@ static int mg_graph(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc, co
     MgState& G = *P->mg;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     if (!G.iter[pi])
-        if (int s = capture(P, &G.iter[pi], [&] { mg_cg_iteration<T>(P, a, dsc, u); })) return s;
+        if (int s = capture(P, &G.iter[pi], [&] {
+                Direct e{P->stream};
+                mg_cg_iteration<T>(e, P, a, dsc, u);
+            }))
+            return s;
     if (!G.start[pi])
         if (int s = capture(P, &G.start[pi], [&] { mg_cg_start<T>(P, a, dsc, u); })) return s;
     *iter = G.iter[pi];
@@ -606,9 +950,10 @@ static int mg_iterations_t(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* 
     using V = typename cpk::V2<T>::type;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     const V* u = (const V*)P->mg->L[0].vec[pi][2];
+    Direct e{P->stream};
     for (int j = 0; j < n; ++j) {
         if (exec) CP_HIP(hipGraphLaunch(exec, P->stream));
-        else mg_cg_iteration<T>(P, a, dsc, u);
+        else mg_cg_iteration<T>(e, P, a, dsc, u);
     }
     CP_HIP(hipGetLastError());
     return CP_OK;
@@ -740,22 +1085,24 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     v.lam = L.lam + pi;
     const bool seg = L.seg != 0;
     const cpk::SegGrid sg = seg_grid(G, 0);
+    Direct e{st};
+    const unsigned long long k0 = 0;
     by_width<T>(P->max_w, [&](auto w) {
         if (seg) {
-            smooth_seg<decltype(w)::value, cpk::SEG_X0, T>(a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, sg, st);
-            smooth_seg<decltype(w)::value, cpk::SEG_FIRST, T>(a, v, v.xa, nullptr, nullptr, v.da, 0, 0, sg, st);
+            smooth_seg<decltype(w)::value, cpk::SEG_X0, T>(e, k0, a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, sg);
+            smooth_seg<decltype(w)::value, cpk::SEG_FIRST, T>(e, k0, a, v, v.xa, nullptr, nullptr, v.da, 0, 0, sg);
         } else {
-            smooth<decltype(w)::value, cpk::MG_PRE1, T>(a, v, nullptr, nullptr, v.xa, v.da, 0, 0, st);
+            smooth<decltype(w)::value, cpk::MG_PRE1, T>(e, k0, a, v, nullptr, nullptr, v.xa, v.da, 0, 0);
         }
     });
     CP_HIP(hipEventRecord(P->ev[4], st));
     for (int i = 0; i < reps; ++i)
         by_width<T>(P->max_w, [&](auto w) {
-            if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg, st);
-            else smooth<decltype(w)::value, cpk::MG_CHEB, T>(a, v, v.xa, v.da, v.xb, v.db, 1, 0, st);
+            if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg);
+            else smooth<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0);
         });
     CP_HIP(hipEventRecord(P->ev[5], st));
-    for (int i = 0; i < reps; ++i) mg_vcycle<T>(P, a.r[0], nullptr);
+    for (int i = 0; i < reps; ++i) mg_vcycle<T>(e, P, a.r[0], nullptr);
     CP_HIP(hipEventRecord(P->ev[6], st));
     CP_HIP(hipGetLastError());
     CP_HIP(hipEventSynchronize(P->ev[6]));

@@ -48,6 +48,7 @@ struct Layout {
 Layout problem_layout(int nn, int ne, long long nblk_stored, long long ninc, int nslices, int nbmax, int npe = 4);
 
 struct MgState;   // multigrid hierarchy (cp_mg.hip)
+struct cp_mg_batch;   // lockstep launches of several problems (cp_mg.hip)
 
 }  // namespace cp
 
@@ -178,4 +179,15 @@ int mg_cg_first_chunk(const cp_problem* P, double tol2, bool zero_start);
 // nodes together).
 size_t mg_bytes_estimate(int nn, int precision);
 void mg_cg_observe(cp_problem* P, const cpk::CgScal& h);
+// Lockstep CG iterations of several prepared multigrid problems of one
+// precision (the serving loop): `iters` iterations of every problem, each
+// step of the iteration one launch over the tiles of all of them, enqueued
+// on st.  *batch caches the recorded launches while the problems stay the
+// same, as named by ids (names never reused for another problem: a freed
+// problem's address may come back); created on first use, freed by
+// mg_batch_destroy.
+int mg_batch_run(cp_mg_batch** batch, cp_problem* const* probs, const long long* ids, int n, int precision, int iters,
+                 hipStream_t st);
+void mg_batch_destroy(cp_mg_batch* batch);
+int mg_batch_launches(const cp_mg_batch* batch);
 }  // namespace cp

@@ -14,8 +14,13 @@
 // else is in the batch), then one host synchronisation.  Multigrid requests
 // work on streams of their own and advance independently: each one's device
 // work ends with a readback of its scalars into a pinned host slot and an
-// event, and every step advances the requests whose event has completed, so
-// a small problem never waits for a large one's V-cycles.  Requests that
+// event, and every step advances the requests whose event has completed.
+// Their CG iterations (the V-cycles) run in lockstep batches: the requests
+// that entered their CG phase in a step advance together on the batch
+// stream, every step of the iteration (each level's smoothing steps, the
+// restrictions and prolongations, the one-workgroup tails, the vector
+// updates and dots) one launch over the tiles of all of them, so N small
+// problems cost the launches of one.  Requests that
 // arrive while others run are admitted (FIFO) at the next step as slots and
 // memory free up; a worker thread builds an admitted request's device problem
 // (CSR pattern, upload, multigrid lattice) while the steps go on advancing
@@ -59,7 +64,8 @@ struct Request {
     size_t bytes = 0;
     State state = RQ_QUEUED;
     cp_problem* P = nullptr;
-    hipEvent_t ev = nullptr;         // multigrid requests: their scalars' readback has landed
+    hipEvent_t ev = nullptr;         // multigrid requests: their scalars' readback has landed (also
+                                     // marks the end of their own stream's work for a batch to wait on)
     int slot = -1;                   // multigrid requests: mirror slot of the readback
     bool pending = false;            // multigrid requests: a readback is in flight
     int phase = PH_LEVEL;
@@ -151,6 +157,11 @@ struct cp_server {
     char* mir_dev = nullptr;
     char* mir_host = nullptr;
     std::vector<int> free_slots;
+    // lockstep multigrid CG batches, per precision, on their own stream
+    hipStream_t mg_stream = nullptr;
+    cp::cp_mg_batch* mgb[2] = {nullptr, nullptr};
+    std::vector<Request*> mg_ready;   // multigrid requests whose CG iterations go into this step's batches
+    long long mg_rounds = 0, mg_members = 0, mg_iters = 0;   // batches run, their members, lockstep iterations
     // device tables for the batched CG and the scalar pack
     char* dev = nullptr;
     size_t dev_size = 0;
@@ -383,26 +394,86 @@ void update_chunk(Request* r, const cpk::CgScal& g, int chunk)
     r->next_chunk = next;
 }
 
-// Multigrid request: its CG iterations if it is in the CG phase, then the
-// readback of its scalars into its mirror slot and the event that says so.
-int enqueue_mg_round(cp_server* S, Request* r)
+// Readback of a multigrid request's scalars into its mirror slot on stream
+// st, and the event that says it has landed.
+int enqueue_readback(cp_server* S, Request* r, hipStream_t st)
 {
     cp_problem* P = r->P;
-    if (r->phase == PH_CG)
-        if (int s = cp::mg_cg_enqueue_iterations(P, r->opt.precision, r->next_chunk)) return s;
     Pack3 pk;
     pk.it[0] = PackItem{(const char*)(r->opt.precision == CP_FP64 ? (const void*)P->sc64 : (const void*)P->sc32),
                         (int)offsetof(Mirror, cg), (int)sizeof(cpk::CgScal)};
     pk.it[1] = PackItem{(const char*)P->rms, (int)offsetof(Mirror, rm), (int)sizeof(cpk::RmScal)};
     pk.it[2] = PackItem{(const char*)P->refs, (int)offsetof(Mirror, ref), (int)sizeof(cpk::RefScal)};
     char* d = S->mir_dev + (size_t)r->slot * kMirStride;
-    hipLaunchKernelGGL(k_pack3, dim3(3), dim3(64), 0, P->stream, pk, d);
+    hipLaunchKernelGGL(k_pack3, dim3(3), dim3(64), 0, st, pk, d);
     CP_HIP(hipGetLastError());
-    CP_HIP(hipMemcpyAsync(S->mir_host + (size_t)r->slot * kMirStride, d, sizeof(Mirror), hipMemcpyDeviceToHost,
-                          P->stream));
-    CP_HIP(hipEventRecord(r->ev, P->stream));
+    CP_HIP(hipMemcpyAsync(S->mir_host + (size_t)r->slot * kMirStride, d, sizeof(Mirror), hipMemcpyDeviceToHost, st));
+    CP_HIP(hipEventRecord(r->ev, st));
     r->pending = true;
     return CP_OK;
+}
+
+// Multigrid request: its CG iterations if it is in the CG phase, then the
+// readback of its scalars.  With lockstep batches a request in its CG phase
+// waits for this step's batches instead.
+int enqueue_mg_round(cp_server* S, Request* r)
+{
+    cp_problem* P = r->P;
+    if (r->phase == PH_CG && S->cfg.batch_mg) {
+        S->mg_ready.push_back(r);
+        return CP_OK;
+    }
+    if (r->phase == PH_CG)
+        if (int s = cp::mg_cg_enqueue_iterations(P, r->opt.precision, r->next_chunk)) return s;
+    return enqueue_readback(S, r, P->stream);
+}
+
+// This step's lockstep batches: per precision, every multigrid request in
+// its CG phase runs the chunk of iterations the slowest of them is
+// predicted to need (the others' steps return at once once they are done),
+// after its own stream's work, then its readback.  A request alone replays
+// its own graphs.
+void run_mg_batches(cp_server* S)
+{
+    std::vector<Request*> ready;
+    ready.swap(S->mg_ready);
+    for (int pi = 0; pi < 2; ++pi) {
+        const int prec = pi == 0 ? CP_FP64 : CP_MIXED;
+        std::vector<Request*> g;
+        for (Request* r : ready)
+            if (r->state == RQ_ACTIVE && r->phase == PH_CG && r->opt.precision == prec) g.push_back(r);
+        if (g.empty()) continue;
+        if (g.size() == 1) {
+            Request* r = g[0];
+            int s = cp::mg_cg_enqueue_iterations(r->P, prec, r->next_chunk);
+            if (s == CP_OK) s = enqueue_readback(S, r, r->P->stream);
+            if (s != CP_OK) fail_request(S, r, s);
+            continue;
+        }
+        int chunk = 1;
+        std::vector<cp_problem*> probs;
+        std::vector<long long> ids;
+        int s = CP_OK;
+        for (Request* r : g) {
+            chunk = std::max(chunk, r->next_chunk);
+            probs.push_back(r->P);
+            ids.push_back(r->id);
+            if (s == CP_OK && (hipEventRecord(r->ev, r->P->stream) != hipSuccess ||
+                               hipStreamWaitEvent(S->mg_stream, r->ev, 0) != hipSuccess))
+                s = cp::fail(CP_ERR_DEVICE, "cp_server: batch dependency failed");
+        }
+        if (s == CP_OK) s = cp::mg_batch_run(&S->mgb[pi], probs.data(), ids.data(), (int)g.size(), prec, chunk, S->mg_stream);
+        if (s == CP_OK) {
+            ++S->mg_rounds;
+            S->mg_members += (long long)g.size();
+            S->mg_iters += chunk;
+        }
+        for (Request* r : g) {
+            int q = s;
+            if (q == CP_OK) q = enqueue_readback(S, r, S->mg_stream);
+            if (q != CP_OK) fail_request(S, r, q);
+        }
+    }
 }
 
 // A request whose work has finished: its results to the host, its memory to
@@ -584,6 +655,7 @@ int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out)
     S->cfg = *cfg;
     S->t0 = clock_now();
     CP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+    CP_HIP(hipStreamCreateWithFlags(&S->mg_stream, hipStreamNonBlocking));
     if (int s = ensure_buffers(S.get(), 1 << 20)) return s;
     CP_HIP(hipMalloc(&S->mir_dev, kMirStride * cfg->max_slots));
     CP_HIP(hipHostMalloc((void**)&S->mir_host, kMirStride * cfg->max_slots, hipHostMallocDefault));
@@ -607,6 +679,11 @@ int cp_server_destroy(cp_server* S)
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
     (void)hipStreamSynchronize(S->stream);
+    (void)hipStreamSynchronize(S->mg_stream);
+    for (cp::cp_mg_batch*& b : S->mgb) {
+        cp::mg_batch_destroy(b);
+        b = nullptr;
+    }
     for (auto& kv : S->reqs) {
         if (kv.second->P) cp_problem_destroy(kv.second->P);   // waits for its stream
         if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
@@ -617,7 +694,18 @@ int cp_server_destroy(cp_server* S)
     if (S->mir_dev) (void)hipFree(S->mir_dev);
     if (S->mir_host) (void)hipHostFree(S->mir_host);
     (void)hipStreamDestroy(S->stream);
+    (void)hipStreamDestroy(S->mg_stream);
     delete S;
+    return CP_OK;
+}
+
+int cp_server_batch_stats(cp_server* S, long long* rounds, long long* members, long long* iters)
+{
+    if (!S) return cp::fail(CP_ERR_INVALID, "null server");
+    std::lock_guard<std::mutex> step_lock(S->step_mu);
+    if (rounds) *rounds = S->mg_rounds;
+    if (members) *members = S->mg_members;
+    if (iters) *iters = S->mg_iters;
     return CP_OK;
 }
 
@@ -724,6 +812,7 @@ int cp_server_step(cp_server* S, int* in_flight)
         if (r->phase == PH_DONE) finish(S, r);
         else if (int s = enqueue_mg_round(S, r)) fail_request(S, r, s);
     }
+    run_mg_batches(S);
     // block-Jacobi requests: one batched round on the server's stream
     std::vector<Request*> act;
     for (Request* r : S->active)

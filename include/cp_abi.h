@@ -400,7 +400,9 @@ typedef struct {
     size_t hbm_budget;    /* bytes of device memory the server may hold for problems */
     int    max_slots;     /* problems resident at once */
     int    batch_cg;      /* 1: batched multi-problem CG kernels; 0: one launch per problem */
-    int    reserved[5];
+    int    batch_mg;      /* 1: the multigrid requests in their CG phase advance in lockstep, each
+                             step of the iteration one launch over all of them; 0: each on its own */
+    int    reserved[4];
 } cp_server_config;
 
 typedef struct {
@@ -447,6 +449,9 @@ int cp_server_step(cp_server* srv, int* in_flight);
  * message of the failure that ended it. */
 int cp_server_result(cp_server* srv, long long id, cp_request_info* info, double* u, double* sig_i);
 int cp_server_clock(cp_server* srv, double* now);
+/* Lockstep multigrid batches run so far (batch_mg): batches, their members
+ * summed over batches, and the CG iterations they ran in lockstep.  New. */
+int cp_server_batch_stats(cp_server* srv, long long* rounds, long long* members, long long* iters);
 
 #ifdef __cplusplus
 }

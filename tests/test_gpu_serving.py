@@ -81,10 +81,13 @@ def mg_opts(prec):
                      predictor=1, cg_forcing=0.3, inner_tol=1e-5)
 
 
-def test_served_multigrid_equal_alone(ctx):
-    """Multigrid requests (each replays its own V-cycle graphs inside the
-    serving rounds, next to block-Jacobi requests in the batched kernels)
-    give bitwise the results of the same problems solved alone."""
+@pytest.mark.parametrize("batch_mg", [True, False])
+def test_served_multigrid_equal_alone(ctx, batch_mg):
+    """Multigrid requests, served next to block-Jacobi requests in the
+    batched kernels, give bitwise the results of the same problems solved
+    alone: with lockstep batches (batch_mg: the CG iterations of the
+    requests in their CG phase advance together, each step one launch over
+    all of them) and with each request replaying its own V-cycle graphs."""
     alone = []
     for (mesh, loads, mat), grid, nsteps, prec in MG_CASES:
         P = A.Problem(ctx, mesh, loads, mat)
@@ -92,11 +95,17 @@ def test_served_multigrid_equal_alone(ctx):
         P.solve_steps(nsteps, 1.0, mg_opts(prec))
         alone.append((P.field("u"), P.field("sigI")))
         P.close()
-    srv = A.Server(ctx, hbm_budget=1 << 30, max_slots=3, batch_cg=True)
+    srv = A.Server(ctx, hbm_budget=1 << 30, max_slots=3, batch_cg=True, batch_mg=batch_mg)
     ids = [srv.submit(m, l, mat, n, 1.0, mg_opts(p), grid=g) for (m, l, mat), g, n, p in MG_CASES]
     (bm, bl, bmat), bn, bp = CASES[0]
     bj = srv.submit(bm, bl, bmat, bn, 1.0, opts(bp))          # a block-Jacobi request in the same rounds
     srv.run_until_done()
+    rounds, members, iters = srv.batch_stats()
+    print("lockstep batches", rounds, "members", members, "iterations", iters, flush=True)
+    if batch_mg:
+        assert rounds > 0 and members >= 2 * rounds
+    else:
+        assert rounds == 0
     for i, rid in enumerate(ids):
         code, info, u, sig = srv.result(rid)
         print("request", rid, "status", info.status, "outer", info.outer_iters, "cg", info.cg_iters,
@@ -137,8 +146,10 @@ def test_served_stream_overlapping_preparation(ctx):
         assert code == A.CP_OK, (i, srv.last_message)
         assert np.array_equal(u, alone[i][0]) and np.array_equal(sig, alone[i][1]), i
         assert info.t_submit <= info.t_admit <= info.t_done
+    rounds, members, iters = srv.batch_stats()
     print("served", len(ids), "requests; last done at %.3f s" % max(srv.result(r, False)[1].t_done for r in ids),
-          flush=True)
+          "; lockstep batches", rounds, "members", members, "iterations", iters, flush=True)
+    assert rounds > 0 and members >= 2 * rounds
     srv.close()
 
 
