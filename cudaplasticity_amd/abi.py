@@ -52,7 +52,7 @@ class Stats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_slots", C.c_int), ("batch_cg", C.c_int),
-                ("batch_mg", C.c_int), ("reserved", C.c_int * 4)]
+                ("batch_mg", C.c_int), ("batch_mg_max_nodes", C.c_int), ("reserved", C.c_int * 3)]
 
 
 class RequestInfo(C.Structure):
@@ -526,10 +526,10 @@ def declared_functions(header=HEADER):
 class Server:
     """Continuous-batching serving loop (cp_server_*)."""
 
-    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True, batch_mg=True):
+    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True, batch_mg=True, batch_mg_max_nodes=0):
         cfg = ServerConfig()
         cfg.hbm_budget, cfg.max_slots, cfg.batch_cg = int(hbm_budget), int(max_slots), int(batch_cg)
-        cfg.batch_mg = int(batch_mg)
+        cfg.batch_mg, cfg.batch_mg_max_nodes = int(batch_mg), int(batch_mg_max_nodes)
         self.ptr = C.c_void_p()
         check(lib().cp_server_create(ctx.ptr, C.byref(cfg), C.byref(self.ptr)))
         self.sizes = {}

@@ -653,120 +653,95 @@ static void mg_cg_start(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc
 // in key order, so every problem's steps keep their order and each launch
 // advances every problem that has that step.
 
-using BatchLaunchFn = void (*)(const void* ops, const int2* tiles, int n, hipStream_t st);
+// A batched kernel of cp_mg.h: its address, its workgroup size, and whether
+// it takes tiles (ops, tiles, ntiles) or runs one workgroup per op (ops).
+struct BatchKernel {
+    const void* func;
+    int threads;
+    bool tiled;
+};
 
 template <int W, int MODE, typename T>
-static void lb_smooth(const void* o, const int2* t, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL((cpk::k_b_smooth<W, MODE, T>), dim3(n), dim3(cpk::kBlock), 0, st,
-                       (const cpk::StepOp<T>*)o, t, n);
-}
+static BatchKernel bk_smooth() { return {reinterpret_cast<const void*>(&cpk::k_b_smooth<W, MODE, T>), cpk::kBlock, true}; }
 template <int W, int MODE, typename T>
-static void lb_smooth_seg(const void* o, const int2* t, int n, hipStream_t st)
+static BatchKernel bk_smooth_seg()
 {
-    hipLaunchKernelGGL((cpk::k_b_smooth_seg<W, MODE, T>), dim3(n), dim3(cpk::kSeg * cpk::seg_cols<W>()), 0, st,
-                       (const cpk::StepOp<T>*)o, t, n);
+    return {reinterpret_cast<const void*>(&cpk::k_b_smooth_seg<W, MODE, T>), cpk::kSeg * cpk::seg_cols<W>(), true};
 }
 template <typename T>
-static void lb_restrict(const void* o, const int2* t, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_restrict<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::GridOp<T>*)o, t, n);
-}
+static BatchKernel bk_restrict() { return {reinterpret_cast<const void*>(&cpk::k_b_restrict<T>), cpk::kBlock, true}; }
 template <typename T>
-static void lb_prolong(const void* o, const int2* t, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_prolong<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::GridOp<T>*)o, t, n);
-}
+static BatchKernel bk_prolong() { return {reinterpret_cast<const void*>(&cpk::k_b_prolong<T>), cpk::kBlock, true}; }
 template <typename T>
-static void lb_tail(const void* o, const int2*, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_tail<T>, dim3(n), dim3(cpk::kTailThreads), 0, st, (const cpk::TailOp<T>*)o);
-}
+static BatchKernel bk_tail() { return {reinterpret_cast<const void*>(&cpk::k_b_tail<T>), cpk::kTailThreads, false}; }
 template <typename T>
-static void lb_update(const void* o, const int2* t, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_cg_update<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o, t, n);
-}
+static BatchKernel bk_update() { return {reinterpret_cast<const void*>(&cpk::k_b_cg_update<T>), cpk::kBlock, true}; }
 template <typename T>
-static void lb_check(const void* o, const int2*, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_cg_check<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o);
-}
+static BatchKernel bk_check() { return {reinterpret_cast<const void*>(&cpk::k_b_cg_check<T>), cpk::kBlock, false}; }
 template <int W, typename T>
-static void lb_dots(const void* o, const int2* t, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL((cpk::k_b_cg_dots<W, T>), dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o, t, n);
-}
+static BatchKernel bk_dots() { return {reinterpret_cast<const void*>(&cpk::k_b_cg_dots<W, T>), cpk::kBlock, true}; }
 template <typename T>
-static void lb_step(const void* o, const int2*, int n, hipStream_t st)
-{
-    hipLaunchKernelGGL(cpk::k_b_cg_step<T>, dim3(n), dim3(cpk::kBlock), 0, st, (const cpk::CgOp<T>*)o);
-}
+static BatchKernel bk_step() { return {reinterpret_cast<const void*>(&cpk::k_b_cg_step<T>), cpk::kBlock, false}; }
 
 struct BatchRec {
     struct Group {
-        BatchLaunchFn fn = nullptr;
-        bool tiled = true;         // false: one workgroup per op
+        BatchKernel k{};
         size_t op_size = 0;
         std::vector<char> ops;
         std::vector<int> nblk;
     };
     // ordered by step key; within a key, the groups of different kernels
     // hold different problems and may run in any order
-    std::map<std::pair<unsigned long long, BatchLaunchFn>, Group> groups;
+    std::map<std::pair<unsigned long long, const void*>, Group> groups;
 
     template <typename O>
-    void add(unsigned long long key, BatchLaunchFn fn, bool tiled, const O& o, int nblk)
+    void add(unsigned long long key, const BatchKernel& k, const O& o, int nblk)
     {
-        Group& g = groups[{key, fn}];
-        g.fn = fn;
-        g.tiled = tiled;
+        Group& g = groups[{key, k.func}];
+        g.k = k;
         g.op_size = sizeof(O);
         const char* p = reinterpret_cast<const char*>(&o);
         g.ops.insert(g.ops.end(), p, p + sizeof(O));
-        g.nblk.push_back(nblk);
+        g.nblk.push_back(k.tiled ? nblk : 1);
     }
     template <int W, int MODE, typename T>
-    void smooth(unsigned long long k, const cpk::StepOp<T>& o, int nb) { add(k, &lb_smooth<W, MODE, T>, true, o, nb); }
+    void smooth(unsigned long long k, const cpk::StepOp<T>& o, int nb) { add(k, bk_smooth<W, MODE, T>(), o, nb); }
     template <int W, int MODE, typename T>
-    void smooth_seg(unsigned long long k, const cpk::StepOp<T>& o, int nt)
-    {
-        add(k, &lb_smooth_seg<W, MODE, T>, true, o, nt);
-    }
+    void smooth_seg(unsigned long long k, const cpk::StepOp<T>& o, int nt) { add(k, bk_smooth_seg<W, MODE, T>(), o, nt); }
     template <typename T>
-    void restrict_(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, &lb_restrict<T>, true, o, nb); }
+    void restrict_(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, bk_restrict<T>(), o, nb); }
     template <typename T>
-    void prolong(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, &lb_prolong<T>, true, o, nb); }
+    void prolong(unsigned long long k, const cpk::GridOp<T>& o, int nb) { add(k, bk_prolong<T>(), o, nb); }
     template <typename T>
-    void tail(unsigned long long k, const cpk::TailOp<T>& o) { add(k, &lb_tail<T>, false, o, 1); }
+    void tail(unsigned long long k, const cpk::TailOp<T>& o) { add(k, bk_tail<T>(), o, 1); }
     template <typename T>
-    void update(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, &lb_update<T>, true, o, nb); }
+    void update(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, bk_update<T>(), o, nb); }
     template <typename T>
-    void check(unsigned long long k, const cpk::CgOp<T>& o) { add(k, &lb_check<T>, false, o, 1); }
+    void check(unsigned long long k, const cpk::CgOp<T>& o) { add(k, bk_check<T>(), o, 1); }
     template <int W, typename T>
-    void dots(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, &lb_dots<W, T>, true, o, nb); }
+    void dots(unsigned long long k, const cpk::CgOp<T>& o, int nb) { add(k, bk_dots<W, T>(), o, nb); }
     template <typename T>
-    void step(unsigned long long k, const cpk::CgOp<T>& o) { add(k, &lb_step<T>, false, o, 1); }
+    void step(unsigned long long k, const cpk::CgOp<T>& o) { add(k, bk_step<T>(), o, 1); }
 };
 
-// A batch's launches, their tables in one device buffer, and the problems
-// they were recorded for (rebuilt when that set changes).
+// A batch's launches (their tables in one device buffer) as a graph of one
+// iteration, and the problems they were recorded for (rebuilt when that
+// set changes).  The graph is built node by node, not captured: the serving
+// loop's preparation thread makes device calls meanwhile, which would
+// invalidate a capture.
 struct cp_mg_batch {
-    struct Launch {
-        cp::BatchLaunchFn fn;
-        size_t ops_off, tiles_off;
-        int n;
-    };
     std::vector<long long> ids;   // the callers' names of the problems recorded
     int precision = -1;
-    std::vector<Launch> launches;
+    int nlaunch = 0;
     char* dev = nullptr;
     size_t dev_size = 0;
     hipEvent_t done = nullptr;    // the last run's launches have finished
     bool ran = false;
+    hipGraphExec_t exec = nullptr;
     ~cp_mg_batch()
     {
         if (ran) (void)hipEventSynchronize(done);
+        if (exec) (void)hipGraphExecDestroy(exec);
         if (done) (void)hipEventDestroy(done);
         if (dev) (void)hipFree(dev);
     }
@@ -796,7 +771,7 @@ static int mg_batch_build(cp_mg_batch* B, cp_problem* const* probs, const long l
     for (const auto& kv : rec.groups) {
         const BatchRec::Group& g = kv.second;
         need += align(g.ops.size());
-        if (g.tiled) {
+        if (g.k.tiled) {
             long long nt = 0;
             for (int b : g.nblk) nt += b;
             if (nt <= 0 || nt > (1LL << 30)) return fail(CP_ERR_INVALID, "multigrid batch: bad tile count");
@@ -804,15 +779,20 @@ static int mg_batch_build(cp_mg_batch* B, cp_problem* const* probs, const long l
         }
     }
     std::vector<char> host(need, 0);
-    std::vector<cp_mg_batch::Launch> launches;
+    struct Launch {
+        BatchKernel k;
+        size_t ops_off, tiles_off;
+        int n;
+    };
+    std::vector<Launch> launches;
     size_t off = 0;
     for (const auto& kv : rec.groups) {
         const BatchRec::Group& g = kv.second;
         const int nops = (int)g.nblk.size();
-        cp_mg_batch::Launch L{g.fn, off, 0, 0};
+        Launch L{g.k, off, 0, 0};
         std::memcpy(host.data() + off, g.ops.data(), g.ops.size());
         off += align(g.ops.size());
-        if (g.tiled) {
+        if (g.k.tiled) {
             L.tiles_off = off;
             int2* t = reinterpret_cast<int2*>(host.data() + off);
             int nt = 0;
@@ -827,6 +807,10 @@ static int mg_batch_build(cp_mg_batch* B, cp_problem* const* probs, const long l
     }
     // the previous run may still read the old tables
     if (B->ran) CP_HIP(hipEventSynchronize(B->done));
+    if (B->exec) {
+        CP_HIP(hipGraphExecDestroy(B->exec));
+        B->exec = nullptr;
+    }
     if (need > B->dev_size) {
         if (B->dev) CP_HIP(hipFree(B->dev));
         B->dev = nullptr;
@@ -838,7 +822,38 @@ static int mg_batch_build(cp_mg_batch* B, cp_problem* const* probs, const long l
         B->dev_size = need;
     }
     CP_HIP(hipMemcpy(B->dev, host.data(), need, hipMemcpyHostToDevice));
-    B->launches.swap(launches);
+    // one iteration as a chain of kernel nodes
+    hipGraph_t graph = nullptr;
+    CP_HIP(hipGraphCreate(&graph, 0));
+    hipGraphNode_t prev = nullptr;
+    int s = CP_OK;
+    for (const Launch& L : launches) {
+        const void* ops = B->dev + L.ops_off;
+        const void* tiles = B->dev + L.tiles_off;
+        int n = L.n;
+        void* targs[3] = {(void*)&ops, (void*)&tiles, (void*)&n};
+        hipKernelNodeParams kp{};
+        kp.func = const_cast<void*>(L.k.func);
+        kp.gridDim = dim3(L.n);
+        kp.blockDim = dim3(L.k.threads);
+        kp.sharedMemBytes = 0;
+        kp.kernelParams = targs;
+        kp.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        if (hipGraphAddKernelNode(&node, graph, prev ? &prev : nullptr, prev ? 1 : 0, &kp) != hipSuccess) {
+            s = fail(CP_ERR_DEVICE, "multigrid batch: graph node creation failed");
+            break;
+        }
+        prev = node;
+    }
+    if (s == CP_OK && hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        B->exec = nullptr;
+        s = fail(CP_ERR_DEVICE, "multigrid batch: graph instantiation failed");
+    }
+    (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    if (s != CP_OK) return s;
+    B->nlaunch = (int)launches.size();
     B->ids.assign(ids, ids + n);
     B->precision = pi == 0 ? CP_FP64 : CP_MIXED;
     return CP_OK;
@@ -864,16 +879,14 @@ int mg_batch_run(cp_mg_batch** pb, cp_problem* const* probs, const long long* id
                                          : mg_batch_build<float>(B, probs, ids, n))
             return s;
     }
-    for (int j = 0; j < iters; ++j)
-        for (const cp_mg_batch::Launch& L : B->launches)
-            L.fn(B->dev + L.ops_off, reinterpret_cast<const int2*>(B->dev + L.tiles_off), L.n, st);
+    for (int j = 0; j < iters; ++j) CP_HIP(hipGraphLaunch(B->exec, st));
     CP_HIP(hipGetLastError());
     CP_HIP(hipEventRecord(B->done, st));
     B->ran = true;
     return CP_OK;
 }
 
-int mg_batch_launches(const cp_mg_batch* b) { return b ? (int)b->launches.size() : 0; }
+int mg_batch_launches(const cp_mg_batch* b) { return b ? b->nlaunch : 0; }
 
 template <typename F>
 static int capture(cp_problem* P, hipGraphExec_t* exec, const F& record)

@@ -16,8 +16,8 @@
 // work ends with a readback of its scalars into a pinned host slot and an
 // event, and every step advances the requests whose event has completed.
 // Their CG iterations (the V-cycles) run in lockstep batches: the requests
-// that entered their CG phase in a step advance together on the batch
-// stream, every step of the iteration (each level's smoothing steps, the
+// that entered their CG phase in a step advance together on one of the
+// batch streams (taken in turn, so that batches overlap), every step of the iteration (each level's smoothing steps, the
 // restrictions and prolongations, the one-workgroup tails, the vector
 // updates and dots) one launch over the tiles of all of them, so N small
 // problems cost the launches of one.  Requests that
@@ -157,9 +157,13 @@ struct cp_server {
     char* mir_dev = nullptr;
     char* mir_host = nullptr;
     std::vector<int> free_slots;
-    // lockstep multigrid CG batches, per precision, on their own stream
-    hipStream_t mg_stream = nullptr;
-    cp::cp_mg_batch* mgb[2] = {nullptr, nullptr};
+    // lockstep multigrid CG batches on streams of their own, taken in turn
+    // so that the batches of successive steps overlap; per stream and
+    // precision the recorded launches of its last batch
+    static constexpr int kBatchStreams = 4;
+    hipStream_t mg_stream[kBatchStreams] = {};
+    cp::cp_mg_batch* mgb[kBatchStreams][2] = {};
+    int mg_next = 0;
     std::vector<Request*> mg_ready;   // multigrid requests whose CG iterations go into this step's batches
     long long mg_rounds = 0, mg_members = 0, mg_iters = 0;   // batches run, their members, lockstep iterations
     // device tables for the batched CG and the scalar pack
@@ -419,7 +423,7 @@ int enqueue_readback(cp_server* S, Request* r, hipStream_t st)
 int enqueue_mg_round(cp_server* S, Request* r)
 {
     cp_problem* P = r->P;
-    if (r->phase == PH_CG && S->cfg.batch_mg) {
+    if (r->phase == PH_CG && S->cfg.batch_mg && (S->cfg.batch_mg_max_nodes <= 0 || r->nn <= S->cfg.batch_mg_max_nodes)) {
         S->mg_ready.push_back(r);
         return CP_OK;
     }
@@ -450,6 +454,9 @@ void run_mg_batches(cp_server* S)
             if (s != CP_OK) fail_request(S, r, s);
             continue;
         }
+        const int bs = S->mg_next;
+        S->mg_next = (S->mg_next + 1) % cp_server::kBatchStreams;
+        hipStream_t bst = S->mg_stream[bs];
         int chunk = 1;
         std::vector<cp_problem*> probs;
         std::vector<long long> ids;
@@ -459,10 +466,10 @@ void run_mg_batches(cp_server* S)
             probs.push_back(r->P);
             ids.push_back(r->id);
             if (s == CP_OK && (hipEventRecord(r->ev, r->P->stream) != hipSuccess ||
-                               hipStreamWaitEvent(S->mg_stream, r->ev, 0) != hipSuccess))
+                               hipStreamWaitEvent(bst, r->ev, 0) != hipSuccess))
                 s = cp::fail(CP_ERR_DEVICE, "cp_server: batch dependency failed");
         }
-        if (s == CP_OK) s = cp::mg_batch_run(&S->mgb[pi], probs.data(), ids.data(), (int)g.size(), prec, chunk, S->mg_stream);
+        if (s == CP_OK) s = cp::mg_batch_run(&S->mgb[bs][pi], probs.data(), ids.data(), (int)g.size(), prec, chunk, bst);
         if (s == CP_OK) {
             ++S->mg_rounds;
             S->mg_members += (long long)g.size();
@@ -470,7 +477,7 @@ void run_mg_batches(cp_server* S)
         }
         for (Request* r : g) {
             int q = s;
-            if (q == CP_OK) q = enqueue_readback(S, r, S->mg_stream);
+            if (q == CP_OK) q = enqueue_readback(S, r, bst);
             if (q != CP_OK) fail_request(S, r, q);
         }
     }
@@ -655,7 +662,7 @@ int cp_server_create(cp_ctx* ctx, const cp_server_config* cfg, cp_server** out)
     S->cfg = *cfg;
     S->t0 = clock_now();
     CP_HIP(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-    CP_HIP(hipStreamCreateWithFlags(&S->mg_stream, hipStreamNonBlocking));
+    for (hipStream_t& s : S->mg_stream) CP_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     if (int s = ensure_buffers(S.get(), 1 << 20)) return s;
     CP_HIP(hipMalloc(&S->mir_dev, kMirStride * cfg->max_slots));
     CP_HIP(hipHostMalloc((void**)&S->mir_host, kMirStride * cfg->max_slots, hipHostMallocDefault));
@@ -679,10 +686,12 @@ int cp_server_destroy(cp_server* S)
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
     (void)hipStreamSynchronize(S->stream);
-    (void)hipStreamSynchronize(S->mg_stream);
-    for (cp::cp_mg_batch*& b : S->mgb) {
-        cp::mg_batch_destroy(b);
-        b = nullptr;
+    for (int i = 0; i < cp_server::kBatchStreams; ++i) {
+        (void)hipStreamSynchronize(S->mg_stream[i]);
+        for (cp::cp_mg_batch*& b : S->mgb[i]) {
+            cp::mg_batch_destroy(b);
+            b = nullptr;
+        }
     }
     for (auto& kv : S->reqs) {
         if (kv.second->P) cp_problem_destroy(kv.second->P);   // waits for its stream
@@ -694,7 +703,7 @@ int cp_server_destroy(cp_server* S)
     if (S->mir_dev) (void)hipFree(S->mir_dev);
     if (S->mir_host) (void)hipHostFree(S->mir_host);
     (void)hipStreamDestroy(S->stream);
-    (void)hipStreamDestroy(S->mg_stream);
+    for (hipStream_t s : S->mg_stream) (void)hipStreamDestroy(s);
     delete S;
     return CP_OK;
 }

@@ -55,13 +55,17 @@ def make_stream(n, seed=20261203, qmin=2e3, qmax=5e5):
     return [make_problem(rng, qmin, qmax) for _ in range(n)]
 
 
-def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=True, seed=7, log=None):
+def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=True, seed=7, log=None, batch_mg=True,
+          stats=None, batch_mg_max_nodes=0):
     """Submit `stream` with Poisson(rate) arrivals (rate in problems/s on the
-    server clock), run the loop until drained.  Returns per-request records."""
+    server clock; 0: all at once), run the loop until drained.  Returns
+    per-request records; `stats` (a dict), if given, receives the server's
+    lockstep batch counts."""
     rng = np.random.Generator(np.random.PCG64(seed))
     gaps = rng.exponential(1.0 / rate, size=len(stream)) if rate > 0 else np.zeros(len(stream))
     arrivals = np.cumsum(gaps) - gaps[0]
-    srv = A.Server(ctx, hbm_budget, max_slots=max_slots, batch_cg=batch_cg)
+    srv = A.Server(ctx, hbm_budget, max_slots=max_slots, batch_cg=batch_cg, batch_mg=batch_mg,
+                   batch_mg_max_nodes=batch_mg_max_nodes)
     ids = [None] * len(stream)
     t_start = srv.clock()
     i = 0
@@ -81,6 +85,8 @@ def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=Tru
         if log and time.time() - last_log > 20:
             log(f"  served {i} submitted, {in_flight} in flight, t={srv.clock() - t_start:.1f}s")
             last_log = time.time()
+    if stats is not None:
+        stats["batches"], stats["members"], stats["lockstep_iterations"] = srv.batch_stats()
     recs = []
     for k, rid in enumerate(ids):
         code, info, u, sig = srv.result(rid)

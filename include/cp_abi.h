@@ -402,7 +402,9 @@ typedef struct {
     int    batch_cg;      /* 1: batched multi-problem CG kernels; 0: one launch per problem */
     int    batch_mg;      /* 1: the multigrid requests in their CG phase advance in lockstep, each
                              step of the iteration one launch over all of them; 0: each on its own */
-    int    reserved[4];
+    int    batch_mg_max_nodes;   /* > 0: only problems of at most this many nodes join the lockstep
+                                    batches (larger ones replay their own graphs); 0: no limit */
+    int    reserved[3];
 } cp_server_config;
 
 typedef struct {
