@@ -351,6 +351,10 @@ struct CgArgs {
     // whose addresses follow from the node (0: rows through col / slice)
     int lat_n1 = 0, lat_n2 = 0;
     int lat_per1 = 0;        // 1: the lattice is periodic in direction 1 (rings, plates with a hole)
+    // multigrid level 0 of a lattice numbered node by node (FP32): the
+    // operator in the symmetric lattice layout (five blocks per node; see
+    // row_product_sym), nullptr otherwise
+    const void* vsym = nullptr;
 };
 
 template <typename T>

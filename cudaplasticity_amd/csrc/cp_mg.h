@@ -465,6 +465,112 @@ __device__ __forceinline__ void row_product_lat(const CgArgs<T>& a, int n, int i
 // Row width template value of the coarse levels: lattice addressing.
 constexpr int kLatW = -1;
 
+// Row width template value of level 0 in the symmetric lattice layout.
+// Where the nodes are numbered lattice point by lattice point (rectangles,
+// quarter plates and rings) the finest operator is also stored as five
+// blocks per node: its diagonal block (made symmetric) and its couplings to
+// the four "forward" neighbours at lattice offsets (0, 1), (1, -1), (1, 0),
+// (1, 1), in slots 0..4 of 64-node slices (slot k of node n at
+// (n >> 6) * 5 * 64 + k * 64 + (n & 63)); a neighbour outside the lattice
+// has a zero block.  The coupling to a backward neighbour m = n - offset is
+// the transpose of m's forward block: A is symmetric, so a row reads 80
+// bytes of its own and the rest from rows the neighbouring lanes and tiles
+// read anyway, instead of 9 blocks and 9 column indices (180 bytes).
+constexpr int kSymW = -2;
+constexpr int kSymSlots = 5;
+
+__host__ __device__ constexpr int sym_d1(int k) { return k == 0 || k == 1 ? 0 : 1; }
+__host__ __device__ constexpr int sym_d2(int k) { return k == 0 ? 0 : (k == 1 ? 1 : k - 3); }
+__device__ __forceinline__ size_t sym_base(int n) { return (size_t)(n >> 6) * (kSymSlots * kWave) + (n & 63); }
+
+__device__ __forceinline__ void load_sym(const CgArgs<float>& a, size_t q, float (&m)[4])
+{
+    const nf4 u = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(a.vsym) + q);
+    m[0] = u.x; m[1] = u.y; m[2] = u.z; m[3] = u.w;
+}
+// (the layout is built for FP32 only; the FP64 form exists for the templates)
+__device__ __forceinline__ void load_sym(const CgArgs<double>&, size_t, double (&m)[4]) { m[0] = m[1] = m[2] = m[3] = 0.; }
+
+// y = A z at node n of level 0 in the symmetric lattice layout.  Own blocks
+// whose neighbour lies outside the lattice are zero, and so are the
+// forward blocks a backward neighbour index reaches across a lattice column
+// (its own forward neighbour would lie outside too), so only indices
+// outside [0, nn) need a guard.
+template <typename T, typename ZF>
+__device__ __forceinline__ void row_product_sym(const CgArgs<T>& a, int n, const ZF& zfun, T& yx, T& yy)
+{
+    const int n2 = a.lat_n2;
+    T m[kSymSlots][4], t[kSymSlots][4];
+    int fw[kSymSlots], bw[kSymSlots];
+    const size_t base = sym_base(n);
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) {
+        load_sym(a, base + (size_t)k * kWave, m[k]);
+        const int off = sym_d1(k) * n2 + sym_d2(k);
+        fw[k] = n + off < a.nn ? n + off : n;
+        bw[k] = n - off;
+        if (k > 0 && bw[k] >= 0) {
+            load_sym(a, sym_base(bw[k]) + (size_t)k * kWave, t[k]);
+        } else {
+            t[k][0] = t[k][1] = t[k][2] = t[k][3] = 0;
+            bw[k] = n;
+        }
+    }
+    yx = 0;
+    yy = 0;
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) {
+        T zx, zy;
+        zfun(fw[k], zx, zy);
+        yx += m[k][0] * zx + m[k][1] * zy;
+        yy += m[k][2] * zx + m[k][3] * zy;
+        if (k > 0) {
+            zfun(bw[k], zx, zy);
+            yx += t[k][0] * zx + t[k][2] * zy;
+            yy += t[k][1] * zx + t[k][3] * zy;
+        }
+    }
+}
+
+// Level 0's operator in the symmetric lattice layout from its sliced-ELL
+// rows (slot_pos: node x 9 lattice offsets -> row position, -1 if none).
+__global__ __launch_bounds__(kBlock) void k_mg_sym_build(CgArgs<float> a, const signed char* slot_pos, float4* vs)
+{
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    const int2 sl = a.slice[n >> 6];
+    const size_t row = (size_t)sl.x + (n & 63), base = sym_base(n);
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) {
+        const int pos = slot_pos[(size_t)n * 9 + mg_slot(sym_d1(k), sym_d2(k))];
+        float4 b = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (pos >= 0) {
+            float m00, m01, m10, m11;
+            load_block(a, row + (size_t)pos * kWave, m00, m01, m10, m11);
+            if (k == 0) m01 = m10 = 0.5f * (m01 + m10);
+            b = make_float4(m00, m01, m10, m11);
+        }
+        vs[base + (size_t)k * kWave] = b;
+    }
+}
+
+// The couplings of level-0 node n to its previous (lo) and next (up)
+// neighbour along direction 1 in the symmetric lattice layout: the next
+// one is n's slot 3, the previous one the transpose of slot 3 of n - n2.
+template <typename T>
+__device__ __forceinline__ void sym_seg_couplings(const CgArgs<T>& a, int n, bool lo, bool up, T (&bl)[4], T (&bu)[4])
+{
+    if (lo) {
+        T t[4];
+        load_sym(a, sym_base(n - a.lat_n2) + 3 * (size_t)kWave, t);
+        bl[0] = t[0];
+        bl[1] = t[2];
+        bl[2] = t[1];
+        bl[3] = t[3];
+    }
+    if (up) load_sym(a, sym_base(n) + 3 * (size_t)kWave, bu);
+}
+
 // Row product of a multigrid level: lattice addressing on coarse levels
 // (W == kLatW), else the sliced-ELL row of width W.
 template <int W, typename T, typename ZF>
@@ -473,6 +579,8 @@ __device__ __forceinline__ void row_apply(const CgArgs<T>& a, int n, const ZF& z
     if constexpr (W == kLatW) {
         const int i1 = n / a.lat_n2;
         row_product_lat<T>(a, n, i1, n - i1 * a.lat_n2, zfun, yx, yy);
+    } else if constexpr (W == kSymW) {
+        row_product_sym<T>(a, n, zfun, yx, yy);
     } else {
         row_product<W, T>(a, n, zfun, yx, yy);
     }
@@ -744,8 +852,17 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
             if (nodes[c] < 0) continue;
             mk[c] = a.mask[nodes[c]];
             seg_block(a, g, nodes[c], 0, d[c]);
-            if (k0 + c > 0) seg_block(a, g, nodes[c], -1, l[c]);
-            else l[c][0] = l[c][1] = l[c][2] = l[c][3] = 0.;
+            l[c][0] = l[c][1] = l[c][2] = l[c][3] = 0.;
+            if (k0 + c > 0) {
+                if (a.vsym) {
+                    // the coupling the smoothing steps use: the symmetric layout's
+                    T bl[4], bu[4];
+                    sym_seg_couplings<T>(a, nodes[c], true, false, bl, bu);
+                    for (int j = 0; j < 4; ++j) l[c][j] = bl[j];
+                } else {
+                    seg_block(a, g, nodes[c], -1, l[c]);
+                }
+            }
         }
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
@@ -836,15 +953,19 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
         const typename V4<T>::type mi = a.minv[n];
         const unsigned pp = seg_unpack(mi.w);
         const int pl = pp & 0xff, pu = (pp >> 8) & 0xff;
-        constexpr bool lat = W == kLatW;
-        const int2 sl = lat ? make_int2(0, 9) : a.slice[n >> 6];
+        constexpr bool lat = W == kLatW, sym = W == kSymW;
+        const int2 sl = (lat || sym) ? make_int2(0, 9) : a.slice[n >> 6];
         const size_t base = lat ? lat_base(n) : (size_t)sl.x + (n & 63);
         T bl[4] = {0, 0, 0, 0}, bu[4] = {0, 0, 0, 0};   // couplings to the previous / next segment node
         if constexpr (MODE == SEG_X0 || MODE == SEG_RX0) {
             rx = mk.x ? bn.x : (T)0;
             ry = mk.y ? bn.y : (T)0;
-            if (pl != (int)kSegNone) load_block(a, base + (size_t)pl * kWave, bl[0], bl[1], bl[2], bl[3]);
-            if (pu != (int)kSegNone) load_block(a, base + (size_t)pu * kWave, bu[0], bu[1], bu[2], bu[3]);
+            if constexpr (sym) {
+                sym_seg_couplings<T>(a, n, pl != (int)kSegNone, pu != (int)kSegNone, bl, bu);
+            } else {
+                if (pl != (int)kSegNone) load_block(a, base + (size_t)pl * kWave, bl[0], bl[1], bl[2], bl[3]);
+                if (pu != (int)kSegNone) load_block(a, base + (size_t)pu * kWave, bu[0], bu[1], bu[2], bu[3]);
+            }
         } else {
             // the row product of mg_smooth_node, keeping the two blocks of
             // the segment couplings as they pass
@@ -860,7 +981,10 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
                 }
             };
             T yx = 0, yy = 0;
-            if constexpr (lat) {
+            if constexpr (sym) {
+                row_product_sym<T>(a, n, zfun, yx, yy);
+                sym_seg_couplings<T>(a, n, pl != (int)kSegNone, pu != (int)kSegNone, bl, bu);
+            } else if constexpr (lat) {
                 // coarse level: lattice addressing (the coupling slots are 2 and 7)
                 T m[9][4];
 #pragma unroll
@@ -1352,13 +1476,13 @@ __device__ __forceinline__ void mg_cg_dots_block(const CgArgs<T>& a, const typen
     double v[3] = {0., 0., 0.};
     if (n < a.nn) {
         T yx, yy;
-        row_product<W, T>(a, n,
-                          [&](int m, T& zx, T& zy) {
-                              const V um = u[m];
-                              zx = um.x;
-                              zy = um.y;
-                          },
-                          yx, yy);
+        row_apply<W, T>(a, n,
+                        [&](int m, T& zx, T& zy) {
+                            const V um = u[m];
+                            zx = um.x;
+                            zy = um.y;
+                        },
+                        yx, yy);
         const uchar2 mk = a.mask[n];
         const V wn = V{mk.x ? yx : (T)0, mk.y ? yy : (T)0};
         a.w[0][n] = wn;

@@ -35,6 +35,10 @@ struct MgState {
     int* lg0 = nullptr;      // node -> lattice index of level 0 (nullptr: identity)
     int* node0 = nullptr;    // lattice index -> node of level 0 (nullptr: identity)
     signed char* slot0 = nullptr;   // level 0: node x 9 lattice offsets -> position in its row
+    // level 0 in FP32 also in the symmetric lattice layout (nodes numbered
+    // lattice point by lattice point, open lattice, 4-node quads)
+    bool sym_ok = false;
+    float4* vsym32 = nullptr;
     double* dense = nullptr; // coarsest operator, inverted in place
     int nd = 0;
     bool alloc[2] = {false, false};
@@ -107,6 +111,8 @@ static int mg_alloc(cp_problem* P, int prec)
                 if (int s = G.get(&L.minv32, L.nn)) return s;
             }
         }
+        if (l == 0 && prec != CP_FP64 && G.sym_ok)
+            if (int s = G.get(&G.vsym32, (size_t)L.nslices * cpk::kWave * cpk::kSymSlots)) return s;
         const size_t vb = prec == CP_FP64 ? sizeof(double2) : sizeof(float2);
         for (int k = 0; k < 6; ++k) {
             if (l == 0 && k == 0) continue;   // level 0's right-hand side is the CG residual
@@ -211,7 +217,16 @@ cpk::CgArgs<double> level_args<double>(cp_problem* P, int l)
 template <>
 cpk::CgArgs<float> level_args<float>(cp_problem* P, int l)
 {
-    if (l == 0) return P->cg32_args();
+    if (l == 0) {
+        cpk::CgArgs<float> a = P->cg32_args();
+        const MgState& G = *P->mg;
+        if (G.vsym32) {
+            a.vsym = G.vsym32;
+            a.lat_n1 = G.L[0].n1;
+            a.lat_n2 = G.L[0].n2;
+        }
+        return a;
+    }
     const MgLevel& L = P->mg->L[l];
     cpk::CgArgs<float> a{};
     a.col = L.col;
@@ -262,6 +277,10 @@ static int mg_setup_t(cp_problem* P, int nlev)
     MgState& G = *P->mg;
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     hipStream_t st = P->stream;
+    if constexpr (std::is_same<T, float>::value)
+        if (G.vsym32)
+            hipLaunchKernelGGL(cpk::k_mg_sym_build, dim3(P->nbn), dim3(cpk::kBlock), 0, st, P->cg32_args(),
+                               (const signed char*)G.slot0, G.vsym32);
     for (int l = 0; l < nlev; ++l) {
         const MgLevel& L = G.L[l];
         cpk::CgArgs<T> a = level_args<T>(P, l);
@@ -499,12 +518,21 @@ static void by_width(int w, const F& f)
     else f(std::integral_constant<int, 0>());
 }
 
-// Row instantiation of level l: the problem's row width on level 0, lattice
-// addressing on the coarse levels.
+// Row instantiation of level 0: the symmetric lattice layout where it is
+// built (FP32), else the problem's row width.
+template <typename T, typename F>
+static void by_level0(const cp_problem* P, const F& f)
+{
+    if (std::is_same<T, float>::value && P->mg->vsym32) f(std::integral_constant<int, cpk::kSymW>());
+    else by_width<T>(P->max_w, f);
+}
+
+// Row instantiation of level l: level 0's, lattice addressing on the coarse
+// levels.
 template <typename T, typename F>
 static void by_level(const cp_problem* P, int l, const F& f)
 {
-    if (l == 0) by_width<T>(P->max_w, f);
+    if (l == 0) by_level0<T>(P, f);
     else f(std::integral_constant<int, cpk::kLatW>());
 }
 
@@ -615,17 +643,20 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
 // One iteration of CG preconditioned by the V-cycle: p, s, x, r updates,
 // the stop test on the new r, u = M r, w = A u with the three dot products,
 // scalars of the next step.
+// (`a` holds the solve's vectors; level 0's row layouts come with it from
+// level_args.)
 template <typename T, typename E>
 static void mg_cg_iteration(E& e, cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc,
                             const typename cpk::V2<T>::type* u)
 {
     cpk::CgOp<T> o{};
-    o.a = a;
+    o.a = level_args<T>(P, 0);
+    o.a.sc = a.sc;
     o.u = u;
     e.template update<T>(step_key(SECT_UPDATE, 0, 0), o, P->nbn);
     e.template check<T>(step_key(SECT_CHECK, 0, 0), o);
     mg_vcycle<T>(e, P, a.r[0], dsc);
-    by_width<T>(P->max_w, [&](auto w) { e.template dots<decltype(w)::value, T>(step_key(SECT_DOTS, 0, 0), o, P->nbn); });
+    by_level0<T>(P, [&](auto w) { e.template dots<decltype(w)::value, T>(step_key(SECT_DOTS, 0, 0), o, P->nbn); });
     e.template step<T>(step_key(SECT_STEP, 0, 0), o);
 }
 
@@ -639,8 +670,10 @@ static void mg_cg_start(cp_problem* P, const cpk::CgArgs<T>& a, cpk::CgScal* dsc
     hipLaunchKernelGGL(cpk::k_mg_cg_clear<T>, g, b, 0, st, a);
     Direct e{st};
     mg_vcycle<T>(e, P, a.r[0], dsc);
-    by_width<T>(P->max_w, [&](auto w) {
-        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a, u);
+    cpk::CgArgs<T> a0 = level_args<T>(P, 0);
+    a0.sc = a.sc;
+    by_level0<T>(P, [&](auto w) {
+        hipLaunchKernelGGL((cpk::k_mg_cg_dots<decltype(w)::value, T>), g, b, 0, st, a0, u);
     });
     hipLaunchKernelGGL(cpk::k_cg_start<T>, dim3(1), b, 0, st, a);
 }
@@ -1100,7 +1133,7 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     const cpk::SegGrid sg = seg_grid(G, 0);
     Direct e{st};
     const unsigned long long k0 = 0;
-    by_width<T>(P->max_w, [&](auto w) {
+    by_level0<T>(P, [&](auto w) {
         if (seg) {
             smooth_seg<decltype(w)::value, cpk::SEG_X0, T>(e, k0, a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, sg);
             smooth_seg<decltype(w)::value, cpk::SEG_FIRST, T>(e, k0, a, v, v.xa, nullptr, nullptr, v.da, 0, 0, sg);
@@ -1110,7 +1143,7 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     });
     CP_HIP(hipEventRecord(P->ev[4], st));
     for (int i = 0; i < reps; ++i)
-        by_width<T>(P->max_w, [&](auto w) {
+        by_level0<T>(P, [&](auto w) {
             if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg);
             else smooth<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0);
         });
@@ -1125,10 +1158,12 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     if (ms_smooth) *ms_smooth = t1 / reps;
     if (ms_vcycle) *ms_vcycle = t2 / reps;
     const double s = std::is_same<T, double>::value ? 8. : 4.;
-    // matrix + columns, slices; x, d, b read, x, d written (2 values per
-    // node each), the block-diagonal inverse (4 values per node), the mask
-    if (bytes_smooth)
-        *bytes_smooth = P->nblk_stored * (4. * s + 4.) + 8. * P->nslices + (5. * 2. * s + 4. * s) * P->nn + 2. * P->nn;
+    // matrix + columns, slices (the symmetric lattice layout: its five
+    // blocks per node); x, d, b read, x, d written (2 values per node
+    // each), the block-diagonal inverse (4 values per node), the mask
+    const double mat = G.vsym32 && pi == 1 ? (double)G.L[0].nslices * cpk::kWave * cpk::kSymSlots * 4. * s
+                                           : P->nblk_stored * (4. * s + 4.) + 8. * P->nslices;
+    if (bytes_smooth) *bytes_smooth = mat + (5. * 2. * s + 4. * s) * P->nn + 2. * P->nn;
     return CP_OK;
 }
 
@@ -1327,6 +1362,7 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
     }
     bool identity = true;
     for (int n = 0; n < nn; ++n) identity = identity && lg[n] == n;
+    G->sym_ok = identity && !per1 && P->npe == 4;
     if (!s && !identity) {
         std::vector<int> no(nn);
         for (int n = 0; n < nn; ++n) no[lg[n]] = n;
