@@ -125,10 +125,12 @@ def cheb_step_bytes(nnz, N, s):
 
 
 def smooth_kernel(P):
-    """Name and traffic key of the finest level's Chebyshev step kernel."""
+    """Name and traffic key of the finest level's FP32 Chebyshev step kernel
+    (row template -2: the symmetric lattice layout, 9: sliced-ELL rows)."""
+    w, tag = (-2, "_sym") if P.mg_level0_sym() else (9, "")
     if P.mg_smoothers()[0] == 1:
-        return "cpk::k_mg_smooth_seg<9,1,float>", "mg_smooth_seg"
-    return "cpk::k_mg_smooth<9,1,float>", "mg_smooth"
+        return "cpk::k_mg_smooth_seg<%d,1,float>" % w, "mg_smooth_seg" + tag
+    return "cpk::k_mg_smooth<%d,1,float>" % w, "mg_smooth" + tag
 
 
 def hbm_copy(A, ctx, nbytes=1 << 30, reps=20):
@@ -739,9 +741,13 @@ def c3_workload(A, args):
             "frac": fmt_s32 / ms_s32 / 1e6 / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": kname + " (finest-level Chebyshev smoothing step, FP32)",
             "bytes_per_launch": fmt_s32, "avg_launch_ms": ms_s32, "reps": reps,
-            "bytes_model": ("what one launch must move in the 2x2-block layout: every stored block (4 values + "
-                            "its column) once, slice descriptors, x, d, b read, x, d written, the pivot inverses "
-                            "and the mask once each"),
+            "bytes_model": (("what one launch must move in the symmetric lattice layout: the five stored 2x2 blocks "
+                             "of every node once (the backward couplings are the neighbours' forward blocks), x, d, "
+                             "b read, x, d written, the pivot inverses and the mask once each")
+                            if tkey.endswith("_sym") else
+                            ("what one launch must move in the 2x2-block layout: every stored block (4 values + "
+                             "its column) once, slice descriptors, x, d, b read, x, d written, the pivot inverses "
+                             "and the mask once each")),
             "working_set_bytes": fmt_s32,
             "traffic_gbps": traffic / ms_s32 / 1e6 if traffic else None,
             "reference_csr_bytes_per_launch": ref_bytes}

@@ -419,6 +419,13 @@ class Problem:
     def set_mg_rebuild(self, every):
         check(lib().cp_problem_set_mg_rebuild(self.ptr, int(every)))
 
+    def mg_level0_sym(self):
+        """True where the finest level's FP32 operator uses the symmetric
+        lattice layout (five blocks per node)."""
+        v = C.c_int()
+        check(lib().cp_problem_mg_level0_layout(self.ptr, C.byref(v)))
+        return bool(v.value)
+
     def mg_smoothers(self):
         """Per level: 0 = 2x2 block Jacobi, 1 = segments along direction 1."""
         n = len(self.mg_levels())
@@ -526,7 +533,7 @@ def declared_functions(header=HEADER):
 class Server:
     """Continuous-batching serving loop (cp_server_*)."""
 
-    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True, batch_mg=True, batch_mg_max_nodes=0):
+    def __init__(self, ctx, hbm_budget, max_slots=16, batch_cg=True, batch_mg=False, batch_mg_max_nodes=0):
         cfg = ServerConfig()
         cfg.hbm_budget, cfg.max_slots, cfg.batch_cg = int(hbm_budget), int(max_slots), int(batch_cg)
         cfg.batch_mg, cfg.batch_mg_max_nodes = int(batch_mg), int(batch_mg_max_nodes)

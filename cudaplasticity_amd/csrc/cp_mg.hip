@@ -1426,6 +1426,14 @@ int cp_problem_set_mg_rebuild(cp_problem* P, int every)
     return CP_OK;
 }
 
+int cp_problem_mg_level0_layout(cp_problem* P, int* sym)
+{
+    if (!P || !sym) return cp::fail(CP_ERR_INVALID, "null argument");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_problem_mg_level0_layout: no multigrid lattice set");
+    *sym = P->mg->sym_ok ? 1 : 0;
+    return CP_OK;
+}
+
 int cp_problem_mg_smoothers(cp_problem* P, int* kinds)
 {
     if (!P || !kinds) return cp::fail(CP_ERR_INVALID, "cp_problem_mg_smoothers: bad arguments");

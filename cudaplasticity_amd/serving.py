@@ -55,7 +55,7 @@ def make_stream(n, seed=20261203, qmin=2e3, qmax=5e5):
     return [make_problem(rng, qmin, qmax) for _ in range(n)]
 
 
-def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=True, seed=7, log=None, batch_mg=True,
+def serve(ctx, stream, nsteps, opt, rate, hbm_budget, max_slots=16, batch_cg=True, seed=7, log=None, batch_mg=False,
           stats=None, batch_mg_max_nodes=0):
     """Submit `stream` with Poisson(rate) arrivals (rate in problems/s on the
     server clock; 0: all at once), run the loop until drained.  Returns

@@ -297,6 +297,11 @@ int cp_problem_set_graphs(cp_problem* prob, int enable);
 /* Number of multigrid levels (0 without a lattice) and their lattice sizes
  * (dims: 2 per level, may be NULL). */
 int cp_problem_mg_levels(cp_problem* prob, int* levels, int* dims);
+/* Layout of the finest level's FP32 operator in the multigrid iterations:
+ * 1 = symmetric lattice layout (five 2x2 blocks per node, no column indices;
+ * lattices numbered node by node, not periodic), 0 = the sliced-ELL rows.
+ * New. */
+int cp_problem_mg_level0_layout(cp_problem* prob, int* sym);
 /* Smoother of each multigrid level except the coarsest (kinds: one per
  * level, 0 = 2x2 block Jacobi, 1 = block-tridiagonal segments along lattice
  * direction 1; the coarsest level, solved exactly, reports 0). */

@@ -234,7 +234,7 @@ def test_served_periodic_equals_alone(ctx):
         P.solve_steps(3, 1.0, opt)
         alone.append((P.field("u"), P.field("sigI")))
         P.close()
-    srv = A.Server(ctx, hbm_budget=1 << 32, max_slots=2)
+    srv = A.Server(ctx, hbm_budget=1 << 32, max_slots=2, batch_mg=True)
     ids = []
     for m, l, mat, bn in cases:
         n1, n2, node_of = A.ring_lattice(m, bn)

@@ -125,9 +125,11 @@ def test_served_stream_overlapping_preparation(ctx):
     """A stream of 24 multigrid requests (plates and graded rings of
     2k-40k quads, both hardening laws, FP64 and mixed) submitted at once to
     6 slots: a worker thread prepares the next requests (pattern, upload,
-    lattice, graph capture) while the others run, and each request advances
-    as soon as its own readback lands.  Every result equals the problem
-    solved alone, bit for bit, and failures carry their message."""
+    lattice, graph capture) while the others run, each request advances as
+    soon as its own readback lands, and the CG iterations of the requests
+    that reach their CG phase together run in lockstep batches.  Every
+    result equals the problem solved alone, bit for bit, and failures carry
+    their message."""
     from cudaplasticity_amd import serving
     stream = serving.make_stream(24, seed=99, qmin=2e3, qmax=4e4)
     alone = []
@@ -137,7 +139,7 @@ def test_served_stream_overlapping_preparation(ctx):
         P.solve_steps(3, 1.0, mg_opts(A.FP64 if k % 3 == 0 else A.MIXED))
         alone.append((P.field("u"), P.field("sigI")))
         P.close()
-    srv = A.Server(ctx, hbm_budget=1 << 34, max_slots=6)
+    srv = A.Server(ctx, hbm_budget=1 << 34, max_slots=6, batch_mg=True)
     ids = [srv.submit(m, l, mat, 3, 1.0, mg_opts(A.FP64 if k % 3 == 0 else A.MIXED), grid=serving.lattice(d))
            for k, (m, l, mat, d) in enumerate(stream)]
     srv.run_until_done()
