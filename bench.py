@@ -48,7 +48,7 @@ NSTEPS = 20
 # default) 44.93 MB read + 3.21 MB written; on the 2000x1000 plate
 # (HBM-resident) the same step 475.8 MB read + 33.0 MB written.
 MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6,
-                    (2000, 1000, "mg_smooth_seg"): 508.8e6}
+                    (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 316.4e6}
 
 
 def log(*a):
