@@ -477,6 +477,10 @@ constexpr int kLatW = -1;
 // bytes of its own and the rest from rows the neighbouring lanes and tiles
 // read anyway, instead of 9 blocks and 9 column indices (180 bytes).
 constexpr int kSymW = -2;
+// The same layout on a launched coarse level (the 16-column tiles of the
+// coarse levels' segment smoother).
+constexpr int kSymW16 = -3;
+template <int W> constexpr bool sym_rows() { return W == kSymW || W == kSymW16; }
 constexpr int kSymSlots = 5;
 
 __host__ __device__ constexpr int sym_d1(int k) { return k == 0 || k == 1 ? 0 : 1; }
@@ -554,6 +558,21 @@ __global__ __launch_bounds__(kBlock) void k_mg_sym_build(CgArgs<float> a, const 
     }
 }
 
+// The same layout for a coarse level from its lattice rows (FP32 slots of
+// fixed offsets), levels of an open lattice outside the tail.
+__global__ __launch_bounds__(kBlock) void k_mg_sym_from_lat(const float4* lat, int nn, float4* vs)
+{
+    const int n = blockIdx.x * kBlock + threadIdx.x;
+    if (n >= nn) return;
+    const size_t base = sym_base(n);
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) {
+        float4 b = lat[lat_base(n) + (size_t)mg_slot(sym_d1(k), sym_d2(k)) * kWave];
+        if (k == 0) b.y = b.z = 0.5f * (b.y + b.z);
+        vs[base + (size_t)k * kWave] = b;
+    }
+}
+
 // The couplings of level-0 node n to its previous (lo) and next (up)
 // neighbour along direction 1 in the symmetric lattice layout: the next
 // one is n's slot 3, the previous one the transpose of slot 3 of n - n2.
@@ -579,7 +598,7 @@ __device__ __forceinline__ void row_apply(const CgArgs<T>& a, int n, const ZF& z
     if constexpr (W == kLatW) {
         const int i1 = n / a.lat_n2;
         row_product_lat<T>(a, n, i1, n - i1 * a.lat_n2, zfun, yx, yy);
-    } else if constexpr (W == kSymW) {
+    } else if constexpr (sym_rows<W>()) {
         row_product_sym<T>(a, n, zfun, yx, yy);
     } else {
         row_product<W, T>(a, n, zfun, yx, yy);
@@ -773,7 +792,7 @@ constexpr int kSeg = 16;           // segment length (direction 1) and tile heig
 // the finest level, where the wider rows of consecutive nodes stream the
 // matrix better; 16 (256 threads) on the coarse levels, which need the
 // workgroups for parallelism.
-template <int W> constexpr int seg_cols() { return W == kLatW ? 16 : 32; }
+template <int W> constexpr int seg_cols() { return (W == kLatW || W == kSymW16) ? 16 : 32; }
 constexpr double kSegLam = 4.;
 
 struct SegGrid {
@@ -953,7 +972,7 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
         const typename V4<T>::type mi = a.minv[n];
         const unsigned pp = seg_unpack(mi.w);
         const int pl = pp & 0xff, pu = (pp >> 8) & 0xff;
-        constexpr bool lat = W == kLatW, sym = W == kSymW;
+        constexpr bool lat = W == kLatW, sym = sym_rows<W>();
         const int2 sl = (lat || sym) ? make_int2(0, 9) : a.slice[n >> 6];
         const size_t base = lat ? lat_base(n) : (size_t)sl.x + (n & 63);
         T bl[4] = {0, 0, 0, 0}, bu[4] = {0, 0, 0, 0};   // couplings to the previous / next segment node

@@ -24,6 +24,7 @@ struct MgLevel {
     float4* vf = nullptr;
     double4* minv64 = nullptr;
     float4* minv32 = nullptr;
+    float4* vsym32 = nullptr;   // launched coarse levels of an open lattice: FP32 symmetric lattice layout
     double* lam = nullptr;   // [0] FP64 operator, [1] FP32 operator
     void* vec[2][6] = {};    // per precision: b, xa, xb, da, db, res
 };
@@ -109,6 +110,8 @@ static int mg_alloc(cp_problem* P, int prec)
             } else {
                 if (int s = G.get(&L.vf, stored)) return s;
                 if (int s = G.get(&L.minv32, L.nn)) return s;
+                if (!G.per1 && l < G.lt)
+                    if (int s = G.get(&L.vsym32, (size_t)L.nslices * cpk::kWave * cpk::kSymSlots)) return s;
             }
         }
         if (l == 0 && prec != CP_FP64 && G.sym_ok)
@@ -240,6 +243,7 @@ cpk::CgArgs<float> level_args<float>(cp_problem* P, int l)
     a.lat_n1 = L.n1;
     a.lat_n2 = L.n2;
     a.lat_per1 = P->mg->per1;
+    a.vsym = L.vsym32;
     return a;
 }
 
@@ -309,6 +313,11 @@ static int mg_setup_t(cp_problem* P, int nlev)
             }
             g.mask_c = L.mask;
             hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
+            if constexpr (std::is_same<T, float>::value)
+                if (L.vsym32)
+                    hipLaunchKernelGGL(cpk::k_mg_sym_from_lat, dim3(L.nb), dim3(cpk::kBlock), 0, st,
+                                       (const float4*)L.vf, L.nn, L.vsym32);
+            a = level_args<T>(P, l);
         }
         if (l < G.nlev - 1 && L.seg) {
             const int cols = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * L.n2;
@@ -527,12 +536,14 @@ static void by_level0(const cp_problem* P, const F& f)
     else by_width<T>(P->max_w, f);
 }
 
-// Row instantiation of level l: level 0's, lattice addressing on the coarse
-// levels.
+// Row instantiation of level l: level 0's; on the coarse levels the
+// symmetric lattice layout where it is built (FP32, open lattice), else the
+// lattice rows.
 template <typename T, typename F>
 static void by_level(const cp_problem* P, int l, const F& f)
 {
     if (l == 0) by_level0<T>(P, f);
+    else if (std::is_same<T, float>::value && P->mg->L[l].vsym32) f(std::integral_constant<int, cpk::kSymW16>());
     else f(std::integral_constant<int, cpk::kLatW>());
 }
 
