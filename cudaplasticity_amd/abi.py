@@ -269,6 +269,19 @@ def ring_lattice(mesh, border_n):
     return n1.value, n2.value, node_of
 
 
+def corner_lattice2(kind, *args):
+    """cp_mesh_{rect2,hole2,ring2}_lattice: (m1, m2, corner_of) of an 8-node
+    generator's corner nodes; args as the generator takes them (rect2:
+    N1, N2; hole2: x1, x2, y1, y2, r, N1, N2; ring2: Nphi, Nr)."""
+    fn = getattr(lib(), f"cp_mesh_{kind}2_lattice")
+    conv = [_f(a) if isinstance(a, float) else int(a) for a in args]
+    m1, m2 = C.c_int(), C.c_int()
+    check(fn(*conv, C.byref(m1), C.byref(m2), None))
+    corner_of = np.zeros(m1.value * m2.value, dtype=np.int32)
+    check(fn(*conv, C.byref(m1), C.byref(m2), ip(corner_of)))
+    return m1.value, m2.value, corner_of
+
+
 def renumerate_ring(mesh, border_n):
     """Mesh::renumerateRing (Mesh.cu:332-384) on a 4-node mesh: a renumbered copy."""
     x, y, conn = mesh.x.copy(), mesh.y.copy(), mesh.conn.copy()

@@ -239,6 +239,20 @@ void Mesh::setRingLattice(int borderN)
     latticePeriodic = true;
 }
 
+// 8-node meshes: the lattice of their corner nodes, from the generator's
+// lattice call lat(m1, m2, corner_of).
+template <typename Lat>
+void Mesh::setCornerLattice(Lat lat, bool periodic)
+{
+    int m1 = 0, m2 = 0;
+    check(lat(&m1, &m2, nullptr), "mesh lattice");
+    latticeNodes.resize((size_t)m1 * m2);
+    check(lat(&m1, &m2, latticeNodes.data()), "mesh lattice");
+    latticeN1 = m1;
+    latticeN2 = m2;
+    latticePeriodic = periodic;
+}
+
 void Mesh::renumberLattice(const std::vector<int>& newOfOld)
 {
     for (int& n : latticeNodes) n = newOfOld[n];
@@ -248,12 +262,14 @@ void Mesh::genRectangle(double x1, double x2, double y1, double y2, size_t N1, s
 {
     check_order(order);
     const int n1 = as_int(N1, "genRectangle"), n2 = as_int(N2, "genRectangle");
-    if (order == 2)
-        return generate2(
-            [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect2_sizes(n1, n2, nn, ne, nb, bl); },
-            [&](double* x, double* y, int* c, unsigned char* sc, int* b, int* bl) {
-                return cp_mesh_rect2(x1, x2, y1, y2, n1, n2, x, y, c, sc, b, bl);
-            });
+    if (order == 2) {
+        generate2([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect2_sizes(n1, n2, nn, ne, nb, bl); },
+                  [&](double* x, double* y, int* c, unsigned char* sc, int* b, int* bl) {
+                      return cp_mesh_rect2(x1, x2, y1, y2, n1, n2, x, y, c, sc, b, bl);
+                  });
+        return setCornerLattice(
+            [&](int* m1, int* m2, int* co) { return cp_mesh_rect2_lattice(n1, n2, m1, m2, co); }, false);
+    }
     generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_rect_sizes(n1, n2, nn, ne, nb, bl); },
              [&](double* x, double* y, int* c, int* b, int* bl) {
                  return cp_mesh_rect(x1, x2, y1, y2, n1, n2, x, y, c, b, bl);
@@ -266,14 +282,20 @@ void Mesh::genRectWithHole(double x1, double x2, double y1, double y2, double ho
 {
     check_order(order);
     const int n1 = as_int(N1, "genRectWithHole"), n2 = as_int(N2, "genRectWithHole");
-    if (order == 2)
-        return generate2(
+    if (order == 2) {
+        generate2(
             [&](int* nn, int* ne, int* nb, int* bl) {
                 return cp_mesh_hole2_sizes(x1, x2, y1, y2, holeRad, n1, n2, nn, ne, nb, bl);
             },
             [&](double* x, double* y, int* c, unsigned char* sc, int* b, int* bl) {
                 return cp_mesh_hole2(x1, x2, y1, y2, holeRad, n1, n2, x, y, c, sc, b, bl);
             });
+        return setCornerLattice(
+            [&](int* m1, int* m2, int* co) {
+                return cp_mesh_hole2_lattice(x1, x2, y1, y2, holeRad, n1, n2, m1, m2, co);
+            },
+            true);
+    }
     generate(
         [&](int* nn, int* ne, int* nb, int* bl) {
             return cp_mesh_hole_sizes(x1, x2, y1, y2, holeRad, n1, n2, nn, ne, nb, bl);
@@ -288,12 +310,14 @@ void Mesh::genRing(double a, double b, size_t N_phi, size_t N_r, int order)
 {
     check_order(order);
     const int np = as_int(N_phi, "genRing"), nr = as_int(N_r, "genRing");
-    if (order == 2)
-        return generate2(
-            [&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring2_sizes(np, nr, nn, ne, nb, bl); },
-            [&](double* x, double* y, int* c, unsigned char* sc, int* bd, int* bl) {
-                return cp_mesh_ring2(a, b, np, nr, x, y, c, sc, bd, bl);
-            });
+    if (order == 2) {
+        generate2([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring2_sizes(np, nr, nn, ne, nb, bl); },
+                  [&](double* x, double* y, int* c, unsigned char* sc, int* bd, int* bl) {
+                      return cp_mesh_ring2(a, b, np, nr, x, y, c, sc, bd, bl);
+                  });
+        return setCornerLattice([&](int* m1, int* m2, int* co) { return cp_mesh_ring2_lattice(np, nr, m1, m2, co); },
+                                true);
+    }
     generate([&](int* nn, int* ne, int* nb, int* bl) { return cp_mesh_ring_sizes(np, nr, nn, ne, nb, bl); },
              [&](double* x, double* y, int* c, int* bd, int* bl) {
                  return cp_mesh_ring(a, b, np, nr, x, y, c, bd, bl);

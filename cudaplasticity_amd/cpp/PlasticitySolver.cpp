@@ -176,7 +176,7 @@ void PlasticitySolver::buildProblem()
 void PlasticitySolver::prepare()
 {
     if (options.multigrid && mesh_.latticeN1 == 0)
-        throw std::invalid_argument("PlasticitySolver: multigrid needs a generated 4-node mesh (no lattice)");
+        throw std::invalid_argument("PlasticitySolver: multigrid needs a generated mesh (no lattice)");
     if (!prob_) buildProblem();
     if (!options.multigrid || gridSet_) return;
     const auto set = mesh_.latticePeriodic ? cp_problem_set_grid_periodic : cp_problem_set_grid;

@@ -905,6 +905,78 @@ int cp_mesh_hole2(double x1, double x2, double y1, double y2, double r, int N1, 
     return remap_into(nn, ne, X, Y, C, radN, B, bord, x, y, conn, sec);
 }
 
+// The corner nodes of the 8-node mesh remapOrder builds from an order-1
+// mesh of lattice n1 x n2 (node_of1: lattice point -> order-1 node; quads in
+// strips of `width` along direction 2, one strip per direction-1 step): the
+// lattice points with both coordinates even, as order-2 node numbers (the
+// patch centres dropped and the rest renumbered in their order, as
+// cp_mesh_remap_order does).
+static int corner_lattice(int nn1, int ne1, const int* conn1, int width, int n1, int n2, const int* node_of1,
+                          bool per1, int* m1, int* m2, int* corner_of)
+{
+    if (width < 2 || (width & 1) || ne1 % width || (per1 ? (n1 & 1) : !(n1 & 1)) || !(n2 & 1))
+        return cp::fail(CP_ERR_INVALID, "corner lattice: the order-1 lattice must have even cell counts");
+    std::vector<unsigned char> keep(nn1, 1);
+    const int strips = ne1 / width;
+    for (int i = 0; i < strips; i += 2)
+        for (int j = 0; j < width; j += 2) keep[conn1[4 * ((size_t)i * width + j) + 0]] = 0;
+    std::vector<int> id(nn1, -1);
+    for (int n = 0, m = 0; n < nn1; ++n)
+        if (keep[n]) id[n] = m++;
+    *m1 = per1 ? n1 / 2 : n1 / 2 + 1;
+    *m2 = n2 / 2 + 1;
+    if (corner_of)
+        for (int I1 = 0; I1 < *m1; ++I1)
+            for (int I2 = 0; I2 < *m2; ++I2) {
+                const int c = id[node_of1[(size_t)(2 * I1) * n2 + 2 * I2]];
+                if (c < 0) return cp::fail(CP_ERR_INVALID, "corner lattice: a corner is a patch centre");
+                corner_of[(size_t)I1 * *m2 + I2] = c;
+            }
+    return CP_OK;
+}
+
+int cp_mesh_rect2_lattice(int N1, int N2, int* m1, int* m2, int* corner_of)
+{
+    if (!m1 || !m2) return cp::fail(CP_ERR_INVALID, "cp_mesh_rect2_lattice: null argument");
+    int nn, ne, nb, blen[8];
+    if (int s = cp_mesh_rect_sizes(2 * N1, 2 * N2, &nn, &ne, &nb, blen)) return s;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(2 * (blen[0] + blen[1])), node_of(nn);
+    if (int s = cp_mesh_rect(0., 1., 0., 1., 2 * N1, 2 * N2, X.data(), Y.data(), C.data(), B.data(), blen)) return s;
+    for (int n = 0; n < nn; ++n) node_of[n] = n;
+    return corner_lattice(nn, ne, C.data(), 2 * N2, 2 * N1 + 1, 2 * N2 + 1, node_of.data(), false, m1, m2, corner_of);
+}
+
+int cp_mesh_hole2_lattice(double x1, double x2, double y1, double y2, double r, int N1, int N2, int* m1, int* m2,
+                          int* corner_of)
+{
+    if (!m1 || !m2) return cp::fail(CP_ERR_INVALID, "cp_mesh_hole2_lattice: null argument");
+    int nn8, ne8, nb, blen[8];
+    if (int s = cp_mesh_hole2_sizes(x1, x2, y1, y2, r, N1, N2, &nn8, &ne8, &nb, blen)) return s;
+    const int radN = 2 * radial_segments(x1, x2, y1, y2, r, N1, N2), borderN = 4 * (N1 + N2);
+    const int nn = borderN * (radN + 1), ne = borderN * radN;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(2 * (blen[0] + blen[1]) + blen[4]), node_of(nn);
+    if (int s = hole_mesh(x1, x2, y1, y2, r, 2 * N1, 2 * N2, radN, X.data(), Y.data(), C.data(), B.data(), blen))
+        return s;
+    int l1 = 0, l2 = 0;
+    if (int s = cp_mesh_ring_lattice(nn, borderN, &l1, &l2, node_of.data())) return s;
+    return corner_lattice(nn, ne, C.data(), radN, l1, l2, node_of.data(), true, m1, m2, corner_of);
+}
+
+int cp_mesh_ring2_lattice(int Nphi, int Nr, int* m1, int* m2, int* corner_of)
+{
+    if (!m1 || !m2) return cp::fail(CP_ERR_INVALID, "cp_mesh_ring2_lattice: null argument");
+    int nn, ne, nb, blen[8];
+    if (int s = cp_mesh_ring_sizes(2 * Nphi, 2 * Nr, &nn, &ne, &nb, blen)) return s;
+    std::vector<double> X(nn), Y(nn);
+    std::vector<int> C(4 * (size_t)ne), B(blen[0] + blen[1]), node_of(nn);
+    if (int s = cp_mesh_ring(0.5, 1.0, 2 * Nphi, 2 * Nr, X.data(), Y.data(), C.data(), B.data(), blen)) return s;
+    int l1 = 0, l2 = 0;
+    if (int s = cp_mesh_ring_lattice(nn, 2 * Nphi, &l1, &l2, node_of.data())) return s;
+    return corner_lattice(nn, ne, C.data(), 2 * Nr, l1, l2, node_of.data(), true, m1, m2, corner_of);
+}
+
 int cp_mesh_ring2_sizes(int Nphi, int Nr, int* nn, int* ne, int* nb, int* blen)
 {
     int n1, e1;

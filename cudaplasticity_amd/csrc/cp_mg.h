@@ -33,6 +33,7 @@
 namespace cpk {
 
 constexpr int kMgDenseMax = 128;   // DOFs of the coarsest level (its inverse stays in registers)
+constexpr int kQ8Children = 5;     // fine children of a corner-lattice point: itself and its edges' mid-side nodes
 constexpr double kMgLowFrac = 1. / 30.;
 
 // 1-D coarsening: coarse point count of n fine points (co = 0: the
@@ -174,6 +175,9 @@ struct GalArgs {
     double2* vb_c;
     float4* vf_c;          // FP32 destination
     uchar2* mask_c;
+    // 8-node fine level (k_mg_galerkin_q8): the corner lattice's transfer
+    const int2* q8_par;    // fine node -> its coarse parents (one for a corner, two for a mid-side node)
+    const int* q8_ch;      // coarse point -> its fine children (kQ8Children, -1 padded)
 };
 
 template <typename T>
@@ -277,6 +281,67 @@ __global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
 
 // Block-Jacobi inverse (as k_block_jacobi) and the per-block maximum of the
 // Gershgorin row sums of |D^-1 A|.
+// Galerkin operator of the corner lattice under an 8-node fine level:
+// A_c(K, D) = sum over K's children f and D's children g of
+// w_f A_fg w_g, the weights of the transfer the V-cycle uses (a corner is
+// its own child with weight 1, each mid-side node of an edge a child of the
+// edge's two corners with weight 1/2).  One thread per coarse point and
+// stencil slot, as k_mg_galerkin; the fine row of each child is walked
+// through its column indices and every column's parents are looked up.
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void k_mg_galerkin_q8(GalArgs<TS, TD> g)
+{
+    const int id = xcd_block(g.nbc) * kBlock + threadIdx.x;
+    if (id >= 9 * g.nnc) return;
+    const int C = id / 9, t = id - 9 * C;
+    const int K1 = C / g.n2c, K2 = C - K1 * g.n2c;
+    const int idx = t == 0 ? 4 : (t <= 4 ? t - 1 : t);
+    const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
+    const int D1 = mg_step(g.n1c, g.per1, K1, d1), D2 = K2 + d2;
+    double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
+    if (D1 >= 0 && D2 >= 0 && D2 < g.n2c) {
+        const int D = D1 * g.n2c + D2;
+        for (int j = 0; j < kQ8Children; ++j) {
+            const int f = g.q8_ch[(size_t)C * kQ8Children + j];
+            if (f < 0) continue;
+            const double wf = j == 0 ? 1. : 0.5;
+            const uchar2 mf = g.f.mask[f];
+            const int2 sl = g.f.slice[f >> 6];
+            const size_t base = (size_t)sl.x + (f & 63);
+            for (int k = 0; k < sl.y; ++k) {
+                const size_t q = base + (size_t)k * kWave;
+                const int gn = g.f.col[q];
+                const int2 pp = g.q8_par[gn];
+                const double wg = pp.y < 0 ? (pp.x == D ? 1. : 0.) : ((pp.x == D || pp.y == D) ? 0.5 : 0.);
+                if (wg == 0.) continue;   // (padding blocks of the row are zero)
+                const uchar2 mg = g.f.mask[gn];
+                double m00, m01, m10, m11;
+                load_block_d(g.f, q, m00, m01, m10, m11);
+                const double ww = wf * wg;
+                acc0 += (mf.x && mg.x) ? ww * m00 : 0.;
+                acc1 += (mf.x && mg.y) ? ww * m01 : 0.;
+                acc2 += (mf.y && mg.x) ? ww * m10 : 0.;
+                acc3 += (mf.y && mg.y) ? ww * m11 : 0.;
+            }
+        }
+    }
+    if (t == 0) {
+        uchar2 mc;
+        mc.x = acc0 > 0. ? 1 : 0;
+        mc.y = acc3 > 0. ? 1 : 0;
+        if (!mc.x) acc0 = 1.;
+        if (!mc.y) acc3 = 1.;
+        g.mask_c[C] = mc;
+    }
+    const size_t q = (size_t)g.slice_c[C >> 6].x + (C & 63) + (size_t)t * kWave;
+    if (g.va_c) {
+        g.va_c[q] = make_double2(acc0, acc1);
+        g.vb_c[q] = make_double2(acc2, acc3);
+    } else {
+        g.vf_c[q] = make_float4((float)acc0, (float)acc1, (float)acc2, (float)acc3);
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_smooth_setup(CgArgs<T> a, typename V4<T>::type* minv,
                                                             double* partials)
@@ -687,6 +752,10 @@ struct MgGrid {
     int n1f, n2f, n1c, n2c;
     int co1, co2;         // directions coarsened to the next level
     int per1;             // direction 1 periodic
+    // an 8-node fine level above the corner lattice: its transfer tables
+    // (nullptr: a lattice level); see k_mg_galerkin_q8
+    const int2* q8_par;
+    const int* q8_ch;
 };
 
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
@@ -696,9 +765,22 @@ __device__ __forceinline__ typename V2<T>::type mg_restrict_value(const MgGrid& 
                                                                   uchar2 mc, int C)
 {
     using V = typename V2<T>::type;
+    double sx = 0., sy = 0.;
+    if (gr.q8_ch) {
+        // the corner itself with weight 1, the mid-side nodes of its edges 1/2
+#pragma unroll
+        for (int j = 0; j < kQ8Children; ++j) {
+            const int f = gr.q8_ch[(size_t)C * kQ8Children + j];
+            if (f < 0) continue;
+            const V r = res[f];
+            const double w = j == 0 ? 1. : 0.5;
+            sx += w * (double)r.x;
+            sy += w * (double)r.y;
+        }
+        return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
+    }
     const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
     const int p1 = mg_pos(gr.n1f, gr.co1, K1, gr.per1), p2 = mg_pos(gr.n2f, gr.co2, K2);
-    double sx = 0., sy = 0.;
     for (int a1 = -1; a1 <= 1; ++a1) {
         const int f1 = mg_step(gr.n1f, gr.per1, p1, a1);
         if (f1 < 0) continue;
@@ -743,6 +825,20 @@ __device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename
                                                 const uchar2* mask_f, int n)
 {
     using V = typename V2<T>::type;
+    if (gr.q8_par) {
+        const int2 pp = gr.q8_par[n];
+        const V c1 = xc[pp.x];
+        double sx = c1.x, sy = c1.y;
+        if (pp.y >= 0) {
+            const V c2 = xc[pp.y];
+            sx = 0.5 * (double)c1.x + 0.5 * (double)c2.x;
+            sy = 0.5 * (double)c1.y + 0.5 * (double)c2.y;
+        }
+        const uchar2 mk = mask_f[n];
+        const V x = xin[n];
+        xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
+        return;
+    }
     const int lf = gr.lg ? gr.lg[n] : n;
     const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
     int Q1[2], Q2[2];

@@ -40,6 +40,11 @@ struct MgState {
     // lattice point by lattice point, open lattice, 4-node quads)
     bool sym_ok = false;
     float4* vsym32 = nullptr;
+    // an 8-node mesh: level 0 is the mesh itself, level 1 its corner
+    // lattice; the transfer between them (cp_mg.h, MgGrid)
+    bool q8 = false;
+    int2* q8_par = nullptr;
+    int* q8_ch = nullptr;
     double* dense = nullptr; // coarsest operator, inverted in place
     int nd = 0;
     bool alloc[2] = {false, false};
@@ -312,7 +317,13 @@ static int mg_setup_t(cp_problem* P, int nlev)
                 g.vf_c = L.vf;
             }
             g.mask_c = L.mask;
-            hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
+            if (l == 1 && G.q8) {
+                g.q8_par = G.q8_par;
+                g.q8_ch = G.q8_ch;
+                hipLaunchKernelGGL((cpk::k_mg_galerkin_q8<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
+            } else {
+                hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
+            }
             if constexpr (std::is_same<T, float>::value)
                 if (L.vsym32)
                     hipLaunchKernelGGL(cpk::k_mg_sym_from_lat, dim3(L.nb), dim3(cpk::kBlock), 0, st,
@@ -578,6 +589,10 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         g.co1 = G.L[l].co1;
         g.co2 = G.L[l].co2;
         g.per1 = G.per1;
+        if (l == 0 && G.q8) {
+            g.q8_par = G.q8_par;
+            g.q8_ch = G.q8_ch;
+        }
         return g;
     };
     const int last = G.lt;   // levels lt.. run in the one-workgroup tail
@@ -1197,16 +1212,66 @@ namespace cp {
 // The lattice of a structured mesh and the level hierarchy on it; per1: the
 // lattice is periodic in direction 1 (the rays of a ring or of a plate with
 // a hole: ray n1 - 1 neighbours ray 0).
+// An 8-node mesh over its corner lattice (lg: node -> lattice point, -1 for
+// the mid-side nodes): every element's corners are lattice neighbours, each
+// mid-side node lies on the edge of two of them; the transfer tables of
+// MgGrid (parents per fine node, children per lattice point).
+static int q8_transfer(const cp_problem* P, int n1, int per1, int n2, const std::vector<int>& lg,
+                       std::vector<int2>& par, std::vector<int>& ch)
+{
+    const int nn = P->nn, ne = P->ne, nl = n1 * n2;
+    if (P->conn_host.size() != 8 * (size_t)ne) return fail(CP_ERR_STATE, "cp_problem_set_grid: no 8-node connectivity");
+    par.assign(nn, make_int2(-1, -1));
+    for (int n = 0; n < nn; ++n)
+        if (lg[n] >= 0) par[n] = make_int2(lg[n], -1);
+    for (int e = 0; e < ne; ++e) {
+        const int* c = P->conn_host.data() + 8 * (size_t)e;
+        for (int k = 0; k < 4; ++k) {
+            const int a = lg[c[k]], b = lg[c[(k + 1) % 4]], m = c[4 + k];
+            if (a < 0 || b < 0) return failf(CP_ERR_INVALID, "cp_problem_set_grid: corner of element %d is not on the lattice", e);
+            if (lg[m] >= 0) return failf(CP_ERR_INVALID, "cp_problem_set_grid: mid-side node %d is on the corner lattice", m);
+            const int o1 = cpk::mg_offset(n1, per1, a / n2, b / n2), o2 = b % n2 - a % n2;
+            if (std::abs(o1) + std::abs(o2) != 1)
+                return failf(CP_ERR_INVALID, "cp_problem_set_grid: the edge of element %d is not a lattice step", e);
+            const int2 pq = make_int2(std::min(a, b), std::max(a, b));
+            if (par[m].x < 0) par[m] = pq;
+            else if (par[m].x != pq.x || par[m].y != pq.y)
+                return failf(CP_ERR_INVALID, "cp_problem_set_grid: mid-side node %d on two edges", m);
+        }
+    }
+    for (int n = 0; n < nn; ++n)
+        if (par[n].x < 0) return failf(CP_ERR_INVALID, "cp_problem_set_grid: node %d is in no element", n);
+    ch.assign((size_t)nl * cpk::kQ8Children, -1);
+    for (int n = 0; n < nn; ++n) {
+        if (par[n].y < 0) {
+            ch[(size_t)par[n].x * cpk::kQ8Children] = n;
+            continue;
+        }
+        for (int p : {par[n].x, par[n].y}) {
+            int j = 1;
+            while (j < cpk::kQ8Children && ch[(size_t)p * cpk::kQ8Children + j] >= 0) ++j;
+            if (j == cpk::kQ8Children)
+                return failf(CP_ERR_INVALID, "cp_problem_set_grid: lattice point %d has more than 4 edges", p);
+            ch[(size_t)p * cpk::kQ8Children + j] = n;
+        }
+    }
+    return CP_OK;
+}
+
 static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
 {
     if (!P || n1 < 2 || n2 < 2 || (per1 && n1 < 3)) return fail(CP_ERR_INVALID, "cp_problem_set_grid: bad arguments");
-    if (P->npe != 4) return fail(CP_ERR_INVALID, "cp_problem_set_grid: multigrid needs 4-node elements");
-    if ((long long)n1 * n2 != P->nn)
-        return failf(CP_ERR_INVALID, "cp_problem_set_grid: %d x %d lattice for %d nodes", n1, n2, P->nn);
+    const bool q8 = P->npe == 8;
     const int nn = P->nn;
+    // 4-node elements: the nodes are the lattice; 8-node elements: the
+    // corners are, and level 0 is the mesh itself above it
+    const long long nl = (long long)n1 * n2;
+    if (q8 ? (nl > nn || !node_of) : nl != nn)
+        return failf(CP_ERR_INVALID, "cp_problem_set_grid: %d x %d lattice for %d nodes%s", n1, n2, nn,
+                     q8 ? " (8-node elements: the corner lattice and its node_of)" : "");
     std::vector<int> lg(nn, -1);
     if (node_of) {
-        for (int l = 0; l < nn; ++l) {
+        for (int l = 0; l < nl; ++l) {
             const int n = node_of[l];
             if (n < 0 || n >= nn || lg[n] >= 0) return fail(CP_ERR_INVALID, "cp_problem_set_grid: node_of is not a permutation");
             lg[n] = l;
@@ -1214,15 +1279,21 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
     } else {
         for (int n = 0; n < nn; ++n) lg[n] = n;
     }
-    // every coupling of the matrix must stay within one lattice step
-    for (int n = 0; n < nn; ++n) {
-        const int a1 = lg[n] / n2, a2 = lg[n] % n2;
-        for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
-            const int m = P->adj_node[q];
-            const int b1 = lg[m] / n2, b2 = lg[m] % n2;
-            if (std::abs(cpk::mg_offset(n1, per1, a1, b1)) > 1 || std::abs(a2 - b2) > 1)
-                return failf(CP_ERR_INVALID, "cp_problem_set_grid: nodes %d and %d share an element but are not lattice neighbours",
-                             n, m);
+    std::vector<int2> q8par;
+    std::vector<int> q8ch;
+    if (q8) {
+        if (int s = q8_transfer(P, n1, per1, n2, lg, q8par, q8ch)) return s;
+    } else {
+        // every coupling of the matrix must stay within one lattice step
+        for (int n = 0; n < nn; ++n) {
+            const int a1 = lg[n] / n2, a2 = lg[n] % n2;
+            for (int q = P->adj_ptr[n]; q < P->adj_ptr[n + 1]; ++q) {
+                const int m = P->adj_node[q];
+                const int b1 = lg[m] / n2, b2 = lg[m] % n2;
+                if (std::abs(cpk::mg_offset(n1, per1, a1, b1)) > 1 || std::abs(a2 - b2) > 1)
+                    return failf(CP_ERR_INVALID, "cp_problem_set_grid: nodes %d and %d share an element but are not lattice neighbours",
+                                 n, m);
+            }
         }
     }
     CP_HIP(hipStreamSynchronize(P->stream));
@@ -1230,15 +1301,17 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
     auto* G = new MgState();
     P->mg = G;
     G->per1 = per1;
+    G->q8 = q8;
     // mean lattice edge lengths along the two directions: a stretched mesh
     // couples strongly across its short edges, so only that direction is
     // halved until the two are within a factor 2 (semi-coarsening)
     double len1 = 0., len2 = 0.;
-    std::vector<double2> pos(nn);   // node coordinates in lattice order
+    std::vector<double2> pos(nl);   // lattice point coordinates in lattice order
     {
         std::vector<double2> xy(nn);
         CP_HIP(hipMemcpy(xy.data(), P->xy, sizeof(double2) * nn, hipMemcpyDeviceToHost));
-        for (int n = 0; n < nn; ++n) pos[lg[n]] = xy[n];
+        for (int n = 0; n < nn; ++n)
+            if (lg[n] >= 0) pos[lg[n]] = xy[n];
         long long c1n = 0, c2n = 0;
         for (int i1 = 0; i1 < n1; ++i1)
             for (int i2 = 0; i2 < n2; ++i2) {
@@ -1257,6 +1330,17 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
         len1 /= std::max<long long>(c1n, 1);
         len2 /= std::max<long long>(c2n, 1);
     }
+    // an 8-node mesh's own level above its corner lattice (no lattice of
+    // its own: 0 x 0, the point smoother on its rows)
+    if (q8) {
+        MgLevel L;
+        L.nn = nn;
+        L.nb = blocks_of(nn);
+        L.nslices = (nn + cpk::kWave - 1) / cpk::kWave;
+        L.co1 = L.co2 = 0;
+        G->L.push_back(L);
+    }
+    const int first = q8 ? 1 : 0;   // the first lattice level
     // levels until the coarsest has at most kMgDenseMax DOFs
     int c1 = n1, c2 = n2;
     for (;;) {
@@ -1298,7 +1382,7 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
     // along direction 1 (mean direction-1 edge below half the direction-2
     // edge) use the segment smoother; coarse nodes sit on their fine
     // positions, so the cells of every level are measured on the mesh
-    for (int l = 0; l < G->lt; ++l) {
+    for (int l = first; l < G->lt; ++l) {
         MgLevel& L = G->L[l];
         long long cells = 0, stretched = 0;
         for (int i1 = 0; i1 + 1 < L.n1; ++i1)
@@ -1355,9 +1439,15 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
             hipMemcpy(L.col, hc.data(), sizeof(int) * hc.size(), hipMemcpyHostToDevice) != hipSuccess)
             s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
     }
+    if (!s && q8) {
+        if (!(s = G->get(&G->q8_par, nn)) && !(s = G->get(&G->q8_ch, q8ch.size())))
+            if (hipMemcpy(G->q8_par, q8par.data(), sizeof(int2) * nn, hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(G->q8_ch, q8ch.data(), sizeof(int) * q8ch.size(), hipMemcpyHostToDevice) != hipSuccess)
+                s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
+    }
     // level 0 rows: position of each lattice offset in the node's row (the
     // row lists the node itself, then its neighbours, in adjacency order)
-    if (!s) {
+    if (!s && !q8) {
         std::vector<signed char> sp((size_t)nn * 9, -1);
         for (int n = 0; n < nn; ++n) {
             const int a1 = lg[n] / n2, a2 = lg[n] % n2;
@@ -1371,10 +1461,10 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
             if (hipMemcpy(G->slot0, sp.data(), sp.size(), hipMemcpyHostToDevice) != hipSuccess)
                 s = fail(CP_ERR_DEVICE, "cp_problem_set_grid: upload failed");
     }
-    bool identity = true;
-    for (int n = 0; n < nn; ++n) identity = identity && lg[n] == n;
+    bool identity = !q8;
+    for (int n = 0; n < nn && identity; ++n) identity = lg[n] == n;
     G->sym_ok = identity && !per1 && P->npe == 4;
-    if (!s && !identity) {
+    if (!s && !identity && !q8) {
         std::vector<int> no(nn);
         for (int n = 0; n < nn; ++n) no[lg[n]] = n;
         if (!(s = G->get(&G->lg0, nn)) && !(s = G->get(&G->node0, nn)))

@@ -64,6 +64,7 @@ struct cp_problem {
     double load = 1.0;
     std::vector<double> rp0;   // host copy for ||b||^2
     std::vector<int> adj_ptr, adj_node;
+    std::vector<int> conn_host;   // 8-node elements: the connectivity (multigrid's corner-lattice transfer)
     long long last_cg_iters = 0;
     double mg_lrate = 0.;   // multigrid CG: mean log reduction of (r, r) per iteration (0: none yet)
     double last_cg_ms_per_iter = 0.;

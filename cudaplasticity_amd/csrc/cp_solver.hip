@@ -200,6 +200,7 @@ int problem_create(cp_ctx* ctx, int nn, int ne, const double* x, const double* y
     P->nslices = (nn + cpk::kWave - 1) / cpk::kWave;
     P->nbn = blocks_for(nn);
     P->nbe = blocks_for(ne);
+    if (npe == 8) P->conn_host.assign(conn, conn + 8 * (size_t)ne);
 
     // sliced ELL layout
     std::vector<int2> slices(P->nslices);
@@ -473,7 +474,7 @@ int launch_assemble(cp_problem* P, int precision, int mg_full)
             else hipLaunchKernelGGL(cpk::k_block_jacobi<float>, g, b, 0, P->stream, P->cg32_args(), P->minv32);
             CP_HIP(hipGetLastError());
         } else if (P->precond == 2) {
-            return fail(CP_ERR_INVALID, "multigrid preconditioning needs a 4-node lattice mesh");
+            if (int s = mg_setup(P, precision, mg_full)) return s;
         }
         if (precision == CP_FP64) P->have_fp64_values = true;
         return CP_OK;

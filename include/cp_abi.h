@@ -165,6 +165,16 @@ int cp_mesh_renumerate_ring(int nn, int ne, double* x, double* y, int* conn, int
  * node_of (nn entries, may be NULL) as cp_problem_set_grid_periodic takes
  * it.  New: the reference has no multigrid. */
 int cp_mesh_ring_lattice(int nn, int borderN, int* n1, int* n2, int* node_of);
+/* The corner-node lattice of the 8-node generators (order 2: the order-1
+ * mesh of doubled resolution regrouped by remapOrder, Mesh.cu:443-503): its
+ * lattice points with both coordinates even, m1 x m2, corner_of[i1 * m2 + i2]
+ * the 8-node mesh's node there (may be NULL); the input of multigrid on an
+ * 8-node problem (cp_problem_set_grid / _periodic: direction 1 periodic for
+ * plates with a hole and rings).  New: the reference has no multigrid. */
+int cp_mesh_rect2_lattice(int N1, int N2, int* m1, int* m2, int* corner_of);
+int cp_mesh_hole2_lattice(double x1, double x2, double y1, double y2, double r, int N1, int N2, int* m1, int* m2,
+                          int* corner_of);
+int cp_mesh_ring2_lattice(int Nphi, int Nr, int* m1, int* m2, int* corner_of);
 /* Mesh::renumByDirection (Mesh.cu:537-630) in place: nodes renumbered by
  * increasing x dx + y dy, ties in decreasing original number (the order the
  * reference's merge sort, Mesh.cu:505-535, leaves them in); conn (npe = 4 or

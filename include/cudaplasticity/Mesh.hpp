@@ -39,12 +39,13 @@ public:
     int* borderLength = nullptr;
     int** borders = nullptr;
 
-    // New: the logical lattice of a generated 4-node mesh, the input of the
+    // New: the logical lattice of a generated mesh, the input of the
     // multigrid preconditioner (SolverOptions::multigrid): latticeN1 x
     // latticeN2 points, latticeNodes[i1 * latticeN2 + i2] the node at
-    // (i1, i2); direction 1 closes on itself (latticePeriodic) around the
-    // hole of genRectWithHole and around genRing.  Renumberings carry it
-    // along; latticeN1 = 0 for meshes read from files and 8-node meshes.
+    // (i1, i2) (8-node meshes: the lattice of their corner nodes); direction
+    // 1 closes on itself (latticePeriodic) around the hole of
+    // genRectWithHole and around genRing.  Renumberings carry it along;
+    // latticeN1 = 0 for meshes read from files.
     int latticeN1 = 0;
     int latticeN2 = 0;
     bool latticePeriodic = false;
@@ -122,6 +123,8 @@ private:
     void generate2(Sizes sizes, Gen gen);
     void setBorders(int nb, const int* blen, const std::vector<int>& bord);
     void setLattice(int n1, int n2);
+    template <typename Lat>
+    void setCornerLattice(Lat lat, bool periodic);
     void setRingLattice(int borderN);
     void renumberLattice(const std::vector<int>& newOfOld);
 };

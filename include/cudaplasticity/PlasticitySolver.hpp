@@ -42,7 +42,8 @@ struct SolverOptions {
     int predictor = 0;          // solveSteps: 0 none, 1 extrapolate u, 2 u and log(psi)
     double cgForcing = 0.;      // > 0: inexact secant iterations
     // New: geometric multigrid preconditioner on the mesh's lattice
-    // (Mesh::latticeN1; generated 4-node meshes), before blockJacobi.
+    // (Mesh::latticeN1; generated meshes, 8-node ones through their corner
+    // lattice), before blockJacobi.
     bool multigrid = false;
 };
 

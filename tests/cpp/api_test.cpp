@@ -256,8 +256,11 @@ int run_cpu()
             ++refused;
         }
         {
-            // multigrid needs a lattice: none on an 8-node mesh
-            PlasticitySolver ps(Material(), mesh, LoadConditions());
+            // multigrid needs a lattice: none on a mesh that was not
+            // generated (one read from a file has latticeN1 = 0)
+            Mesh plain(mesh);
+            plain.latticeN1 = 0;
+            PlasticitySolver ps(Material(), plain, LoadConditions());
             ps.options.multigrid = true;
             try {
                 ps.solve();

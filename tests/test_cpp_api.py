@@ -314,8 +314,11 @@ def test_mesh_lattices_through_cpp_api(cpu_out):
     nodes = rd(cpu_out, "lat_hole_lattice.i32")
     elem4 = rd(cpu_out, "lat_hole_elem4.i32").reshape(-1, 4)
     assert lattice_cells_ok(n1, n2, 0, nodes, elem4) == n2 - 1
-    assert rd(cpu_out, "lat_hole2_lattice_meta.i32").tolist() == [0, 0, 0]
-    assert rd(cpu_out, "lat_hole2_lattice.i32").size == 0
+    # an 8-node plate (order 2): the lattice of its corner nodes, periodic
+    assert rd(cpu_out, "lat_hole2_lattice_meta.i32").tolist() == [20, 4, 1]
+    corners = rd(cpu_out, "lat_hole2_lattice.i32")
+    m = A.gen_rect_with_hole(-3, 3, -2, 2, 0.5, 6, 4, order=2)
+    assert np.array_equal(np.sort(corners), np.unique(m.conn.reshape(-1, 8)[:, :4]))
 
 
 def test_load_from_file_through_cpp_api(cpu_out):

@@ -119,15 +119,93 @@ def test_q8_energy_table_at_reference_tolerances(ctx):
     P.close()
 
 
-def test_q8_multigrid_is_refused(ctx):
+def q8_grid(P, k):
+    """The corner lattice of the k plate (periodic around the hole)."""
+    m1, m2, corners = A.corner_lattice2("hole", -3., 3., -2., 2., 0.5, 3 * k, 2 * k)
+    P.set_grid(m1, m2, corners, periodic=True)
+    return m1, m2
+
+
+@pytest.mark.parametrize("precision", [A.FP64, A.MIXED])
+def test_q8_multigrid_matches_oracle(ctx, precision):
+    """Multigrid on an 8-node mesh: level 0 is the mesh itself (2x2
+    block-Jacobi Chebyshev smoothing on its rows), level 1 its corner
+    lattice (the Galerkin product of the transfer that gives a corner its
+    own value and a mid-side node the mean of its edge's corners), then the
+    lattice hierarchy.  The plastic plate of test_q8_plastic_matches_oracle
+    to the same bars, in a fraction of block-Jacobi's CG iterations."""
+    pm, pl, pmat, om, ol, omat = case(4, "power", 170e6)
+    opt = dict(cg_tol=1e-10, tol=1e-8, aitken=True)
+    P = A.Problem(ctx, pm, pl, pmat)
+    m1, m2 = q8_grid(P, 4)
+    levels = P.mg_levels()
+    print("levels", levels, "smoothers", P.mg_smoothers(), flush=True)
+    assert levels[0] == (0, 0) and levels[1] == (m1, m2) and len(levels) >= 3
+    st = P.solve(A.options(precision=precision, precond=2, inner_tol=1e-5, **opt))
+    u_o, st_o, so = O.solve(ol, omat, O.options(max_outer=1000, **opt))
+    Q = A.Problem(ctx, pm, pl, pmat)
+    sq = Q.solve(A.options(precision=precision, precond=1, inner_tol=1e-5, **opt))
+    e_u = relnorm(P.field("u"), u_o)
+    e_s = relnorm(P.field("sigI"), st_o[O.FIELDS.index("sigI")])
+    print("multigrid: outer %d cg %d; block-Jacobi: outer %d cg %d; u %.2e sigma_i %.2e" %
+          (st.outer_iters, st.cg_iters, sq.outer_iters, sq.cg_iters, e_u, e_s), flush=True)
+    assert st.status == A.CP_OK and so.status == 0
+    assert e_u < 1e-6 and e_s < 1e-6
+    assert st.cg_iters * 3 < sq.cg_iters
+    P.close()
+    Q.close()
+
+
+def test_q8_multigrid_fixed_iterations_vs_oracle(ctx):
+    """Six plain secant iterations with CG to 1e-12 on both sides: the
+    multigrid path within 1e-8 of the oracle (a preconditioner changes the
+    CG path, not the solution it converges to)."""
+    pm, pl, pmat, om, ol, omat = case(4, "power", 170e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    q8_grid(P, 4)
+    st = P.solve(A.options(cg_tol=1e-12, fixed_outer=6, precond=2))
+    u_o, st_o, so = O.solve(ol, omat, O.options(cg_tol=1e-12, fixed_outer=6))
+    print("cg %d, u %.2e" % (st.cg_iters, relnorm(P.field("u"), u_o)), flush=True)
+    assert relnorm(P.field("u"), u_o) < 1e-8
+    for f in ("sxx", "sigI", "Ec", "psi"):
+        assert relnorm(P.field(f), st_o[O.FIELDS.index(f)]) < 1e-8, f
+    P.close()
+
+
+def test_q8_multigrid_energy_table(ctx):
+    """The 660-element plate at the reference's tolerances (CG to 1e-7,
+    secant exit 1e-5) with multigrid: aitken__plast2.txt's 2.45491e+06."""
+    pm, pl, pmat, om, ol, omat = case(6, "power", 170e6)
+    P = A.Problem(ctx, pm, pl, pmat)
+    q8_grid(P, 6)
+    st = P.solve(A.options(precond=2))
+    en = energy(om, omat, P)
+    print("outer %d cg %d energy %.1f (table 2.45491e+06)" % (st.outer_iters, st.cg_iters, en), flush=True)
+    assert st.status == A.CP_OK
+    assert within_printed(en, 2.45491e6)
+    P.close()
+
+
+def test_q8_multigrid_refusals(ctx):
+    """An 8-node mesh's lattice is its corner lattice, given with node_of;
+    a lattice that misses that shape is refused."""
     pm, pl, pmat, om, ol, omat = case(2, "elastic", 1e6)
     P = A.Problem(ctx, pm, pl, pmat)
-    n1 = next(d for d in range(2, pm.nn) if pm.nn % d == 0)
+    m1, m2, corners = A.corner_lattice2("hole", -3., 3., -2., 2., 0.5, 6, 4)
     with pytest.raises(A.CPError) as e:
-        P.set_grid(n1, pm.nn // n1)
-    assert e.value.code == A.CP_ERR_INVALID and "4-node" in str(e.value)
+        P.set_grid(m1, m2, None, periodic=True)                     # node_of required
+    assert e.value.code == A.CP_ERR_INVALID
+    with pytest.raises(A.CPError) as e:
+        P.set_grid(m1, m2, corners, periodic=False)                 # the seam's edges are not lattice steps
+    assert e.value.code == A.CP_ERR_INVALID and "lattice step" in str(e.value)
+    mid = int(pm.conn.reshape(-1, 8)[0, 4])
+    bad = corners.copy()
+    bad[0] = mid
+    with pytest.raises(A.CPError) as e:
+        P.set_grid(m1, m2, bad, periodic=True)                      # a mid-side node as a lattice point
+    assert e.value.code == A.CP_ERR_INVALID
     with pytest.raises(A.CPError):
-        P.solve(A.options(precond=2))
+        P.solve(A.options(precond=2))                               # no lattice set
     P.close()
 
 

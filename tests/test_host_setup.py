@@ -308,3 +308,37 @@ def test_renumerate_ring_matches_oracle_and_refuses_bad_counts():
     assert np.array_equal(r.x, x) and np.array_equal(r.conn, conn)
     with pytest.raises(A.CPError):
         A.renumerate_ring(m, 7)
+
+
+@pytest.mark.parametrize("kind,args,mesh_args,periodic", [
+    ("rect", (7, 5), ("gen_rect", (0, 1, 0, 1, 7, 5)), False),
+    ("hole", (-3., 3., -2., 2., 0.5, 12, 8), ("gen_rect_with_hole", (-3, 3, -2, 2, 0.5, 12, 8)), True),
+    ("hole", (-3., 3., -2., 2., 0.5, 96, 64), ("gen_rect_with_hole", (-3, 3, -2, 2, 0.5, 96, 64)), True),
+    ("ring", (24, 5), ("gen_ring", (0.5, 1.0, 24, 5)), True)])
+def test_corner_lattice_of_8_node_meshes(kind, args, mesh_args, periodic):
+    """cp_mesh_*2_lattice: the corner nodes of the 8-node generators form an
+    m1 x m2 lattice (direction 1 periodic around a hole or a ring): every
+    element edge joins two corners one lattice step apart, and the lattice
+    holds exactly the corners."""
+    fn, margs = mesh_args
+    mesh = getattr(A, fn)(*margs, order=2)
+    m1, m2, corner_of = A.corner_lattice2(kind, *args)
+    conn = mesh.conn.reshape(-1, 8)
+    assert np.array_equal(np.sort(corner_of), np.unique(conn[:, :4]))
+    lg = -np.ones(mesh.nn, dtype=np.int64)
+    lg[corner_of] = np.arange(m1 * m2)
+    c = lg[conn[:, :4]]
+    i1, i2 = c // m2, c % m2
+    d1 = np.roll(i1, -1, axis=1) - i1
+    if periodic:
+        d1 = (d1 + m1 // 2) % m1 - m1 // 2
+    d2 = np.roll(i2, -1, axis=1) - i2
+    assert np.all(np.abs(d1) + np.abs(d2) == 1)
+    # each mid-side node belongs to one edge: every element through it has
+    # the same two corners at its ends
+    edges = {}
+    for e in conn:
+        for k in range(4):
+            key = tuple(sorted((int(e[k]), int(e[(k + 1) % 4]))))
+            assert edges.setdefault(int(e[4 + k]), key) == key
+    assert len(edges) == mesh.nn - m1 * m2
