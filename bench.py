@@ -432,67 +432,75 @@ def other_workload(A, args):
 def q8_workload(A, args):
     """8-node quads: the reference's energyConvTest plate (Main.cpp:105-178),
     genRectWithHole(-3, 3, -2, 2, 0.5, 3k, 2k, order 2), 304L power law,
-    traction 170 MPa on x = +-3 in 20 increments, Aitken to 1e-6;
-    block-Jacobi CG (the multigrid preconditioner takes 4-node lattices),
-    mixed and FP64; the CG iteration kernel on the 8-node matrix and the
-    element-block assembly.  One JSON line."""
+    traction 170 MPa on x = +-3 in 20 increments, Aitken to 1e-6; multigrid
+    CG on the mesh's corner lattice in mixed precision (the line's value) and
+    in FP64, and block-Jacobi CG in mixed precision (the path without a
+    lattice) beside them; the finest level's smoothing step on the 8-node
+    rows, the block-Jacobi CG iteration and the element-block assembly.  One
+    JSON line."""
     k, nsteps, tol, S = args.q8_k, NSTEPS, 1e-6, 170.0
     E = 193e3
     mesh = A.gen_rect_with_hole(-3., 3., -2., 2., 0.5, 3 * k, 2 * k, order=2)
     loads = A.Loads(mesh).force(1, S, 0.0).force(3, -S, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
     mat = A.material(E, 0.29, 170.0, A.LAW_POWER, A=0.01 * E, m=0.015)
+    lat = A.corner_lattice2("hole", -3., 3., -2., 2., 0.5, 3 * k, 2 * k)
     name = (f"plate with hole 6x4, r 0.5, 8-node quads ({mesh.ne}, genRectWithHole {3 * k}x{2 * k} order 2), "
             f"304L power law, {nsteps} increments to {S:g} MPa, Aitken to {tol:g}")
     log(name)
     ctx = A.Context(0)
     out = {}
-    for tag, prec in (("mixed", A.MIXED), ("fp64", A.FP64)):
+    for tag, prec, pre in (("mixed", A.MIXED, 2), ("fp64", A.FP64, 2), ("mixed_block_jacobi", A.MIXED, 1)):
         P = A.Problem(ctx, mesh, loads, mat)
-        opt = A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=prec, precond=1,
+        if pre == 2:
+            P.set_grid(lat[0], lat[1], lat[2], periodic=True)
+        opt = A.options(cg_tol=args.cg_tol, tol=tol, aitken=True, max_outer=500, precision=prec, precond=pre,
                         predictor=1, cg_forcing=args.forcing, inner_tol=args.inner_tol)
         t0 = time.perf_counter()
-        per = []
-        for n in range(1, nsteps + 1):
-            o = A.Options.from_buffer_copy(opt)
-            o.warm = 1 if n > 1 else 0
-            if n == 1:
-                P.set_load(n / nsteps)
-            else:
-                P.advance_load(n / nsteps, 1)
-            per.append(P.solve(o))
-            log(f"  {tag} increment {n}/{nsteps}: secant {per[-1].outer_iters} cg {per[-1].cg_iters} "
-                f"{per[-1].seconds:.3f} s")
+        per = P.solve_steps(nsteps, 1.0, opt)
         out[tag] = (time.perf_counter() - t0, per, P)
-    (tm, per_m, Pm), (tf, per_f, Pf) = out["mixed"], out["fp64"]
+        log(f"  {tag}: {nsteps} increments in {out[tag][0]:.3f} s, secant {sum(s.outer_iters for s in per)} "
+            f"cg {sum(s.cg_iters for s in per)}")
+    (tm, per_m, Pm), (tf, per_f, Pf), (tb, per_b, Pb) = out["mixed"], out["fp64"], out["mixed_block_jacobi"]
     nnz = scalar_nnz(A, Pm)
     N = 2 * mesh.nn
-    ms, fmt = bench_cg(A, Pm, A.MIXED, args.reps, precond=True)
-    byt = cg_iteration_bytes(nnz, N, 4)
+    mv, ms_s, fmt_s = bench_mg(A, Pm, A.MIXED, args.reps)
+    ms_cg, fmt_cg = bench_cg(A, Pb, A.MIXED, args.reps, precond=True)
     import ctypes as C
     asm = {}
     for tag, prec in (("fp64", A.FP64), ("fp32", A.MIXED)):
         t = C.c_double()
-        A.check(A.lib().cp_bench_assemble(Pm.ptr, prec, 20, C.byref(t)))
+        A.check(A.lib().cp_bench_assemble(Pb.ptr, prec, 20, C.byref(t)))
         asm[tag] = {"ms": t.value, "elements_per_s": mesh.ne / (t.value * 1e-3)}
     epsT = mat.sigmaT / mat.E
     line = {"metric": METRIC, "value": nsteps / tm, "unit": "load-steps/s", "n_gpus": 1, "steps": nsteps, "warmup": 0,
             "ms_per_step": tm / nsteps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32+f64", "data": "synthetic",
             "config": {"workload": name, "nodes": mesh.nn, "elements": mesh.ne, "dof": N, "nnz": nnz,
-                       "nodes_per_element": 8},
+                       "nodes_per_element": 8,
+                       "path": ("mixed: FP32 CG preconditioned by a multigrid V-cycle (level 0 the 8-node mesh, "
+                                "level 1 its corner lattice) + FP64 refinement, FP64 return map and Aitken")},
             "fp64_load_steps_per_s": nsteps / tf, "speedup_vs_fp64": tf / tm,
+            "block_jacobi_mixed_load_steps_per_s": nsteps / tb, "speedup_vs_block_jacobi": tb / tm,
             "rel_err_vs_fp64": {"u": rel(Pm.field("u"), Pf.field("u")), "sigma_i": rel(Pm.field("sigI"), Pf.field("sigI")),
                                 "plastic_flags_equal": bool(np.array_equal(Pm.field("epsI") >= epsT,
                                                                            Pf.field("epsI") >= epsT))},
+            "rel_err_block_jacobi_vs_multigrid": {"u": rel(Pb.field("u"), Pm.field("u")),
+                                                  "sigma_i": rel(Pb.field("sigI"), Pm.field("sigI"))},
             "plastic_elements": int((Pm.field("epsI") >= epsT).sum()),
+            "multigrid": {"levels": Pm.mg_levels(), "vcycle_ms": mv, "smooth_step_ms": ms_s,
+                          "vcycle_over_finest_step": mv / ms_s},
             "iterations": {"mixed": [[s.outer_iters, s.cg_iters] for s in per_m],
-                           "fp64": [[s.outer_iters, s.cg_iters] for s in per_f]},
+                           "fp64": [[s.outer_iters, s.cg_iters] for s in per_f],
+                           "mixed_block_jacobi": [[s.outer_iters, s.cg_iters] for s in per_b]},
             "assembly": asm,
-            "roofline": {"bound": "hbm", "achieved": fmt / ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": fmt / ms / 1e6 / HBM_PEAK_GBS, "traffic": None,
-                         "kernel": "cpk::k_cg_iter_fused<0,true,float> (block-Jacobi CG iteration, 8-node matrix)",
-                         "bytes_per_launch": fmt, "avg_launch_ms": ms, "reference_csr_bytes_per_launch": byt,
-                         "working_set_bytes": fmt, "cache_resident": fmt < 256e6},
+            "roofline": {"bound": "hbm", "achieved": fmt_s / ms_s / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": fmt_s / ms_s / 1e6 / HBM_PEAK_GBS, "traffic": None,
+                         "kernel": "cpk::k_mg_smooth<0,1,float> (finest-level Chebyshev smoothing step, 8-node rows)",
+                         "bytes_per_launch": fmt_s, "avg_launch_ms": ms_s,
+                         "working_set_bytes": fmt_s, "cache_resident": fmt_s < 256e6},
+            "block_jacobi_cg_iteration": {"kernel": "cpk::k_cg_iter_fused<0,true,float>", "avg_launch_ms": ms_cg,
+                                          "bytes_per_launch": fmt_cg, "gbps": fmt_cg / ms_cg / 1e6,
+                                          "reference_csr_bytes_per_launch": cg_iteration_bytes(nnz, N, 4)},
             "cpu_baseline": None}
     print(json.dumps(line), flush=True)
 
