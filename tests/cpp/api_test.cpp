@@ -490,6 +490,14 @@ int run_gpu()
         std::vector<double> e = {ps.deformEnergy(), (double)mesh.elemCount(), (double)ps.lastStats().status};
         dump("q8_energy.f64", e);
         ps.saveAsVtk(outdir + "/q8_plate.vtk");
+        // the same with multigrid on the mesh's corner lattice
+        PlasticitySolver pm(m, mesh, lc);
+        pm.iterOutput = false;
+        pm.options.multigrid = true;
+        pm.solve();
+        std::vector<double> em = {pm.deformEnergy(), (double)pm.lastStats().status, (double)pm.lastStats().cg_iters,
+                                  (double)ps.lastStats().cg_iters};
+        dump("q8mg_energy.f64", em);
     }
     {
         // fixBorderVert / fixBorderHor through solveElastCUDA: a rectangle in

@@ -371,6 +371,12 @@ def test_order2_energy_through_cpp_api(tmp_path):
     assert abs(en - 2.45491e6) <= 5.0
     lay, VF, _, polys = read_vtk(os.path.join(tmp_path, "q8_plate.vtk"))
     assert polys.shape == (660, 8) and VF["Sigma_intensity"].size == 660
+    # SolverOptions::multigrid on the 8-node mesh (its corner lattice, which
+    # genRectWithHole records): the same table entry, far fewer CG iterations
+    enm, stm, cgm, cg0 = rd(tmp_path, "q8mg_energy.f64")
+    print("8-node energy with multigrid %.1f, CG %d (plain CG %d)" % (enm, cgm, cg0), flush=True)
+    assert stm == 0 and abs(enm - 2.45491e6) <= 5.0
+    assert cgm * 10 < cg0
 
 
 @pytest.mark.gpu
