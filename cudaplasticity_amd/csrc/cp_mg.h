@@ -1456,8 +1456,9 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
 // kernels, with workgroup barriers instead of launches between the steps.
 // Everything a step reads is on the chip: the vectors of every level in
 // LDS; each level's rows, pivot inverses and mask copied into LDS once per
-// V-cycle where the host found room (coarsest first); in FP32 the rows of
-// the tail's finest level in registers, one node per thread.  lv[0] is the
+// V-cycle where the host found room (coarsest first), the others read
+// from L2 (rows kept in registers instead spilled to scratch at the 128
+// registers a 1024-thread workgroup leaves each lane).  lv[0] is the
 // tail's finest level: its right-hand side, restricted by the level above,
 // is read from lv[0].v.b and its result is written to lv[0].v.xb.  The
 // coarsest level is solved by its explicit inverse, eight threads per row.
@@ -1465,24 +1466,12 @@ template <typename T>
 __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, const double* Dinv, int nd)
 {
     using V = typename V2<T>::type;
-    constexpr bool kRegRows = std::is_same<T, float>::value;
     __shared__ __align__(16) char lds[kTailLds];
     __shared__ double lam[kTailMaxLevels];
     __shared__ double bs[kMgDenseMax];
     const int tid = threadIdx.x;
-    T r0[9][4];
     {
         const TailLevel<T> L0 = lv[0];
-        if constexpr (kRegRows) {
-            if (tid < L0.a.nn) {
-                const size_t base = lat_base(tid);
-#pragma unroll
-                for (int k = 0; k < 9; ++k) {
-                    const float4 u = reinterpret_cast<const float4*>(L0.a.va)[base + (size_t)k * kWave];
-                    r0[k][0] = u.x; r0[k][1] = u.y; r0[k][2] = u.z; r0[k][3] = u.w;
-                }
-            }
-        }
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
         for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
         for (int l = 0; l < nl; ++l) {
@@ -1503,15 +1492,8 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         constexpr int MODE = decltype(mode)::value;
         if (tid >= L.a.nn) return;
         const ChebCoef<T> c = mg_cheb<T>(lam[l]);
-        if (kRegRows && l == 0) {
-            tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) {
-#pragma unroll
-                for (int j = 0; j < 4; ++j) m[j] = r0[k][j];
-            }, L.a, L.v, c, xi, di, xo, dout, second, last, tid);
-        } else {
-            tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi,
-                                      di, xo, dout, second, last, tid);
-        }
+        tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi, di,
+                                  xo, dout, second, last, tid);
     };
     using Pre1 = std::integral_constant<int, MG_PRE1>;
     using Cheb = std::integral_constant<int, MG_CHEB>;

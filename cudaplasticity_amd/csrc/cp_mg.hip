@@ -176,8 +176,7 @@ static int mg_tail_table(cp_problem* P)
     }
     if (lds > cpk::kTailVecLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
     // the rest of the LDS: copies of the pivot inverses and masks, then of
-    // the rows, coarsest level first (in FP32 the rows of the tail's finest
-    // level live in registers)
+    // the rows, coarsest level first
     const int nt = (int)tab.size();
     for (int i = nt - 1; i >= 0; --i) {
         const int nn = tab[i].a.nn;
@@ -188,7 +187,6 @@ static int mg_tail_table(cp_problem* P)
         lds += need;
     }
     for (int i = nt - 1; i >= 0; --i) {
-        if (i == 0 && std::is_same<T, float>::value) break;
         const int need = cpk::tail_rows_bytes<T>(tab[i].a.nn);
         if (lds + need > cpk::kTailLds) break;
         tab[i].mat_lds = lds;
