@@ -63,6 +63,9 @@ struct cp_problem {
     cpk::MatParams M{};
     double load = 1.0;
     std::vector<double> rp0;   // host copy for ||b||^2
+    // ||b||^2 at load factor bn_load (one O(nn) host pass per load level,
+    // not one per linear solve)
+    mutable double bn_load = -1., bn_val = 0.;
     std::vector<int> adj_ptr, adj_node;
     std::vector<int> conn_host;   // 8-node elements: the connectivity (multigrid's corner-lattice transfer)
     long long last_cg_iters = 0;
