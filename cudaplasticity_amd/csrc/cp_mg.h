@@ -192,75 +192,137 @@ __device__ __forceinline__ void load_block_d(const CgArgs<T>& a, size_t q, doubl
     m11 = b11;
 }
 
-// Children of coarse point K in an n-point direction: fine points and
-// their interpolation weights (at most 3).
-__device__ __forceinline__ int mg_children(int n, int co, int K, int (&f)[3], double (&w)[3], int per = 0)
+// Galerkin product in tiles of kGalT x kGalT coarse lattice points: the
+// workgroup first copies the rows of every fine point its coarse points
+// interpolate to (at most kGalR x kGalR points, 9 blocks each in lattice
+// offset order) into LDS, each block with the fine DOFs the mask fixes
+// zeroed (its row by the point's mask, its columns by the neighbour's).
+// One thread per (coarse point C, stencil slot t) then forms
+// (P^T A P)(C, D_t), D_t the coarse neighbour of slot t: the sum over the
+// fine children f of C and g of D_t that are lattice neighbours of
+// w_f w_g A_fg, in lattice order, from LDS.  A_fg is found through the fine
+// row's slot table (offset (g - f) -> storage position; level 0 carries
+// one, coarser levels store slots in offset order).  Each fine row is read
+// from memory once per tile instead of once per (coarse point, slot) that
+// needs it.
+constexpr int kGalT = 8;
+constexpr int kGalThreads = 9 * kGalT * kGalT;
+constexpr int kGalR = 2 * kGalT + 1;
+
+// Children of coarse point K in an n-point direction: the fine points at
+// offsets -1, 0, 1 from its position, their interpolation weights (0: not a
+// child) and their place in a tile region whose first fine point (before
+// wrapping) is b.  Fixed-size, fully unrolled, so the arrays stay in
+// registers.
+__device__ __forceinline__ void mg_children_local(int n, int co, int per, int K, int b, int (&f)[3], double (&w)[3],
+                                                  int (&loc)[3])
 {
     const int p = mg_pos(n, co, K, per);
-    int c = 0;
-    for (int a = -1; a <= 1; ++a) {
-        const int q = mg_step(n, per, p, a);
-        if (q < 0) continue;
-        const double wq = mg_weight(n, co, q, K, per);
-        if (wq == 0.) continue;
-        f[c] = q;
-        w[c] = wq;
-        ++c;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int q = mg_step(n, per, p, a - 1);
+        f[a] = q;
+        w[a] = q < 0 ? 0. : mg_weight(n, co, q, K, per);
+        loc[a] = p + a - 1 - b;
     }
-    return c;
 }
 
-// One (coarse node C, stencil slot t) per thread: (P^T A P)(C, D_t), D_t
-// the coarse neighbour of slot t: the sum over the fine children f of C and
-// g of D_t that are lattice neighbours of w_f w_g A_fg (in lattice order).
-// A_fg is found through the fine row's slot table (offset (g - f) -> storage
-// position; level 0 carries one, coarser levels store slots in offset order).
-// nbc counts blocks over 9 x nnc threads.
-template <typename TS, typename TD>
-__global__ __launch_bounds__(kBlock) void k_mg_galerkin(GalArgs<TS, TD> g)
+// Fine region of the coarse points klo..khi of one direction: its first
+// point b (may be negative along a periodic direction: wrapped on use) and
+// its point count.
+__device__ __forceinline__ void mg_gal_region(int n, int co, int per, int klo, int khi, int& b, int& r)
 {
-    const int id = xcd_block(g.nbc) * kBlock + threadIdx.x;
-    if (id >= 9 * g.nnc) return;
-    const int C = id / 9, t = id - 9 * C;
-    const int K1 = C / g.n2c, K2 = C - K1 * g.n2c;
+    b = mg_pos(n, co, klo, per) - 1;
+    int e = mg_pos(n, co, khi, per) + 1;
+    if (!per) {
+        b = b < 0 ? 0 : b;
+        e = e > n - 1 ? n - 1 : e;
+    }
+    r = e - b + 1;
+}
+
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kGalThreads) void k_mg_galerkin(GalArgs<TS, TD> g, int nt2, int ntiles)
+{
+    __shared__ TS blk[kGalR * kGalR * 9 * 4];
+    const int tile = xcd_block(ntiles);
+    const int T1 = tile / nt2, T2 = tile - T1 * nt2;
+    const int K1lo = T1 * kGalT, K2lo = T2 * kGalT;
+    const int K1hi = min(K1lo + kGalT, g.n1c) - 1, K2hi = min(K2lo + kGalT, g.n2c) - 1;
+    int b1, R1, b2, R2;
+    mg_gal_region(g.n1f, g.co1, g.per1, K1lo, K1hi, b1, R1);
+    mg_gal_region(g.n2f, g.co2, 0, K2lo, K2hi, b2, R2);
+    for (int i = threadIdx.x; i < R1 * R2 * 9; i += kGalThreads) {
+        const int pt = i / 9, so = i - 9 * pt;
+        const int l1 = pt / R2, l2 = pt - l1 * R2;
+        int q1 = b1 + l1;
+        if (g.per1) q1 = q1 < 0 ? q1 + g.n1f : (q1 >= g.n1f ? q1 - g.n1f : q1);
+        const int q2 = b2 + l2;
+        TS m00 = 0, m01 = 0, m10 = 0, m11 = 0;
+        if (q1 >= 0 && q1 < g.n1f && q2 >= 0 && q2 < g.n2f) {
+            const int lf = q1 * g.n2f + q2;
+            const int fn = g.node_of ? g.node_of[lf] : lf;
+            const int pos = g.slot_pos ? g.slot_pos[(size_t)fn * 9 + so] : so;
+            if (pos >= 0) {
+                const size_t q = (size_t)g.f.slice[fn >> 6].x + (fn & 63) + (size_t)pos * kWave;
+                const uchar2 mf = g.f.mask[fn], mg = g.f.mask[g.f.col[q]];
+                load_block(g.f, q, m00, m01, m10, m11);
+                if (!(mf.x && mg.x)) m00 = 0;
+                if (!(mf.x && mg.y)) m01 = 0;
+                if (!(mf.y && mg.x)) m10 = 0;
+                if (!(mf.y && mg.y)) m11 = 0;
+            }
+        }
+        TS* d = blk + (size_t)i * 4;
+        d[0] = m00;
+        d[1] = m01;
+        d[2] = m10;
+        d[3] = m11;
+    }
+    __syncthreads();
+    const int cl = threadIdx.x / 9, t = threadIdx.x - 9 * cl;
+    const int K1 = K1lo + cl / kGalT, K2 = K2lo + cl % kGalT;
+    if (K1 > K1hi || K2 > K2hi) return;
+    const int C = K1 * g.n2c + K2;
     // slot t -> coarse offset (d1, d2)
     const int idx = t == 0 ? 4 : (t <= 4 ? t - 1 : t);
     const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
     const int D1 = mg_step(g.n1c, g.per1, K1, d1), D2 = K2 + d2;
     double acc0 = 0., acc1 = 0., acc2 = 0., acc3 = 0.;
     if (D1 >= 0 && D2 >= 0 && D2 < g.n2c) {
-        int fa[3], fb[3], ga[3], gb[3];
+        int fa[3], fb[3], ga[3], gb[3], la[3], lb[3], lx[3];
         double wfa[3], wfb[3], wga[3], wgb[3];
-        const int nfa = mg_children(g.n1f, g.co1, K1, fa, wfa, g.per1), nfb = mg_children(g.n2f, g.co2, K2, fb, wfb);
-        const int nga = mg_children(g.n1f, g.co1, D1, ga, wga, g.per1), ngb = mg_children(g.n2f, g.co2, D2, gb, wgb);
-        for (int i = 0; i < nfa; ++i)
-            for (int j = 0; j < nfb; ++j) {
-                const int lf = fa[i] * g.n2f + fb[j];
-                const int fn = g.node_of ? g.node_of[lf] : lf;
-                const uchar2 mf = g.f.mask[fn];
-                const size_t base = (size_t)g.f.slice[fn >> 6].x + (fn & 63);
-                for (int k = 0; k < nga; ++k) {
+        mg_children_local(g.n1f, g.co1, g.per1, K1, b1, fa, wfa, la);
+        mg_children_local(g.n2f, g.co2, 0, K2, b2, fb, wfb, lb);
+        mg_children_local(g.n1f, g.co1, g.per1, D1, b1, ga, wga, lx);
+        mg_children_local(g.n2f, g.co2, 0, D2, b2, gb, wgb, lx);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (wfa[i] == 0.) continue;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                if (wfb[j] == 0.) continue;
+                const TS* row = blk + (size_t)(la[i] * R2 + lb[j]) * 36;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    if (wga[k] == 0.) continue;
                     const int e1 = mg_offset(g.n1f, g.per1, fa[i], ga[k]);
                     if (e1 < -1 || e1 > 1) continue;
-                    for (int l = 0; l < ngb; ++l) {
+#pragma unroll
+                    for (int l = 0; l < 3; ++l) {
+                        if (wgb[l] == 0.) continue;
                         const int e2 = gb[l] - fb[j];
                         if (e2 < -1 || e2 > 1) continue;
-                        const int so = mg_slot(e1, e2);
-                        const int pos = g.slot_pos ? g.slot_pos[(size_t)fn * 9 + so] : so;
-                        if (pos < 0) continue;   // not coupled in the fine matrix
-                        const size_t q = base + (size_t)pos * kWave;
-                        const int gn = g.f.col[q];
-                        const uchar2 mg = g.f.mask[gn];
-                        double m00, m01, m10, m11;
-                        load_block_d(g.f, q, m00, m01, m10, m11);
+                        const TS* m = row + 4 * mg_slot(e1, e2);
                         const double ww = wfa[i] * wfb[j] * wga[k] * wgb[l];
-                        acc0 += (mf.x && mg.x) ? ww * m00 : 0.;
-                        acc1 += (mf.x && mg.y) ? ww * m01 : 0.;
-                        acc2 += (mf.y && mg.x) ? ww * m10 : 0.;
-                        acc3 += (mf.y && mg.y) ? ww * m11 : 0.;
+                        acc0 += ww * (double)m[0];
+                        acc1 += ww * (double)m[1];
+                        acc2 += ww * (double)m[2];
+                        acc3 += ww * (double)m[3];
                     }
                 }
             }
+        }
     }
     if (t == 0) {
         uchar2 mc;

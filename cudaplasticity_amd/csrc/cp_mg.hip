@@ -320,7 +320,9 @@ static int mg_setup_t(cp_problem* P, int nlev)
                 g.q8_ch = G.q8_ch;
                 hipLaunchKernelGGL((cpk::k_mg_galerkin_q8<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
             } else {
-                hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(g.nbc), dim3(cpk::kBlock), 0, st, g);
+                const int nt1 = (L.n1 + cpk::kGalT - 1) / cpk::kGalT, nt2 = (L.n2 + cpk::kGalT - 1) / cpk::kGalT;
+                hipLaunchKernelGGL((cpk::k_mg_galerkin<T, T>), dim3(nt1 * nt2), dim3(cpk::kGalThreads), 0, st, g,
+                                   nt2, nt1 * nt2);
             }
             if constexpr (std::is_same<T, float>::value)
                 if (L.vsym32)
