@@ -946,6 +946,71 @@ __global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename
 // colour, so lambda_max(D^-1 A) <= 4: the Chebyshev interval is
 // [4/30, 4.4] with no eigenvalue estimate.
 constexpr int kSeg = 16;           // segment length (direction 1) and tile height
+
+// Lane (lane - D) / (lane + D) of the caller's row of 16 lanes (DPP
+// row_shr / row_shl): the segment scans keep each segment in one row.  A
+// lane with no such source lane keeps its own value (the scans never use it).
+template <int D>
+__device__ __forceinline__ int row_up_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x110 | D, 0xf, 0xf, false); }
+template <int D>
+__device__ __forceinline__ int row_down_i(int v) { return __builtin_amdgcn_update_dpp(v, v, 0x100 | D, 0xf, 0xf, false); }
+template <int D>
+__device__ __forceinline__ float row_up(float v) { return __int_as_float(row_up_i<D>(__float_as_int(v))); }
+template <int D>
+__device__ __forceinline__ float row_down(float v) { return __int_as_float(row_down_i<D>(__float_as_int(v))); }
+template <int D>
+__device__ __forceinline__ double row_up(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = row_up_i<D>((int)(b & 0xffffffffLL)), hi = row_up_i<D>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int D>
+__device__ __forceinline__ double row_down(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = row_down_i<D>((int)(b & 0xffffffffLL)), hi = row_down_i<D>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// One step of each segment scan, D = 1, 2, 4, 8 (k: the lane's place in its
+// segment):
+//   forward:  (v, M) <- (v + M v', M M') from lane k - D, where k >= D
+//   backward: (v, N) <- (v + N v'', N N'') from lane k + D, where k + D < 16
+template <int D, typename T>
+__device__ __forceinline__ void seg_scan_up(int k, T& v0, T& v1, T& m00, T& m01, T& m10, T& m11)
+{
+    const T pv0 = row_up<D>(v0), pv1 = row_up<D>(v1);
+    const T p00 = row_up<D>(m00), p01 = row_up<D>(m01), p10 = row_up<D>(m10), p11 = row_up<D>(m11);
+    if (k >= D) {
+        const T n0 = v0 + m00 * pv0 + m01 * pv1, n1 = v1 + m10 * pv0 + m11 * pv1;
+        const T a00 = m00 * p00 + m01 * p10, a01 = m00 * p01 + m01 * p11;
+        const T a10 = m10 * p00 + m11 * p10, a11 = m10 * p01 + m11 * p11;
+        v0 = n0;
+        v1 = n1;
+        m00 = a00;
+        m01 = a01;
+        m10 = a10;
+        m11 = a11;
+    }
+}
+template <int D, typename T>
+__device__ __forceinline__ void seg_scan_down(int k, T& v0, T& v1, T& n00, T& n01, T& n10, T& n11)
+{
+    const T pw0 = row_down<D>(v0), pw1 = row_down<D>(v1);
+    const T p00 = row_down<D>(n00), p01 = row_down<D>(n01), p10 = row_down<D>(n10), p11 = row_down<D>(n11);
+    if (k + D < 16) {
+        const T w0 = v0 + n00 * pw0 + n01 * pw1, w1 = v1 + n10 * pw0 + n11 * pw1;
+        const T a00 = n00 * p00 + n01 * p10, a01 = n00 * p01 + n01 * p11;
+        const T a10 = n10 * p00 + n11 * p10, a11 = n10 * p01 + n11 * p11;
+        v0 = w0;
+        v1 = w1;
+        n00 = a00;
+        n01 = a01;
+        n10 = a10;
+        n11 = a11;
+    }
+}
 // Tile width (direction 2) of a smoothing tile: 32 columns (512 threads) on
 // the finest level, where the wider rows of consecutive nodes stream the
 // matrix better; 16 (256 threads) on the coarse levels, which need the
@@ -1250,42 +1315,17 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
         const int q = k * kSegPitch + c;
         T v0 = sy[0][q], v1 = sy[1][q];
         T m00 = -sf[0][q], m01 = -sf[1][q], m10 = -sf[2][q], m11 = -sf[3][q];
-#pragma unroll
-        for (int d = 1; d < kSeg; d <<= 1) {
-            const T pv0 = __shfl_up(v0, d, kSeg), pv1 = __shfl_up(v1, d, kSeg);
-            const T p00 = __shfl_up(m00, d, kSeg), p01 = __shfl_up(m01, d, kSeg);
-            const T p10 = __shfl_up(m10, d, kSeg), p11 = __shfl_up(m11, d, kSeg);
-            if (k >= d) {
-                const T n0 = v0 + m00 * pv0 + m01 * pv1, n1 = v1 + m10 * pv0 + m11 * pv1;
-                const T a00 = m00 * p00 + m01 * p10, a01 = m00 * p01 + m01 * p11;
-                const T a10 = m10 * p00 + m11 * p10, a11 = m10 * p01 + m11 * p11;
-                v0 = n0;
-                v1 = n1;
-                m00 = a00;
-                m01 = a01;
-                m10 = a10;
-                m11 = a11;
-            }
-        }
+        static_assert(kSeg == 16, "a segment is one row of 16 lanes");
+        seg_scan_up<1, T>(k, v0, v1, m00, m01, m10, m11);
+        seg_scan_up<2, T>(k, v0, v1, m00, m01, m10, m11);
+        seg_scan_up<4, T>(k, v0, v1, m00, m01, m10, m11);
+        seg_scan_up<8, T>(k, v0, v1, m00, m01, m10, m11);
         // v = y_k
         T n00 = -sh[0][q], n01 = -sh[1][q], n10 = -sh[2][q], n11 = -sh[3][q];
-#pragma unroll
-        for (int d = 1; d < kSeg; d <<= 1) {
-            const T pw0 = __shfl_down(v0, d, kSeg), pw1 = __shfl_down(v1, d, kSeg);
-            const T p00 = __shfl_down(n00, d, kSeg), p01 = __shfl_down(n01, d, kSeg);
-            const T p10 = __shfl_down(n10, d, kSeg), p11 = __shfl_down(n11, d, kSeg);
-            if (k + d < kSeg) {
-                const T w0 = v0 + n00 * pw0 + n01 * pw1, w1 = v1 + n10 * pw0 + n11 * pw1;
-                const T a00 = n00 * p00 + n01 * p10, a01 = n00 * p01 + n01 * p11;
-                const T a10 = n10 * p00 + n11 * p10, a11 = n10 * p01 + n11 * p11;
-                v0 = w0;
-                v1 = w1;
-                n00 = a00;
-                n01 = a01;
-                n10 = a10;
-                n11 = a11;
-            }
-        }
+        seg_scan_down<1, T>(k, v0, v1, n00, n01, n10, n11);
+        seg_scan_down<2, T>(k, v0, v1, n00, n01, n10, n11);
+        seg_scan_down<4, T>(k, v0, v1, n00, n01, n10, n11);
+        seg_scan_down<8, T>(k, v0, v1, n00, n01, n10, n11);
         sy[0][q] = v0;
         sy[1][q] = v1;
     }
