@@ -822,6 +822,40 @@ struct MgGrid {
 
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
 // w_f * res_f (fine residuals are already zero on fixed DOFs).
+// Restriction to coarse lattice point (K1, K2): the fine residuals of its
+// children weighted by their interpolation weights (the weights along
+// direction 2 formed once).
+template <typename T>
+__device__ __forceinline__ typename V2<T>::type mg_restrict_lat(const MgGrid& gr, const typename V2<T>::type* res,
+                                                                uchar2 mc, int K1, int K2)
+{
+    using V = typename V2<T>::type;
+    const int p1 = mg_pos(gr.n1f, gr.co1, K1, gr.per1), p2 = mg_pos(gr.n2f, gr.co2, K2);
+    double w2[3];
+#pragma unroll
+    for (int a2 = 0; a2 < 3; ++a2) {
+        const int f2 = p2 + a2 - 1;
+        w2[a2] = (f2 < 0 || f2 >= gr.n2f) ? 0. : mg_weight(gr.n2f, gr.co2, f2, K2);
+    }
+    double sx = 0., sy = 0.;
+#pragma unroll
+    for (int a1 = -1; a1 <= 1; ++a1) {
+        const int f1 = mg_step(gr.n1f, gr.per1, p1, a1);
+        if (f1 < 0) continue;
+        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1, gr.per1);
+        if (w1 == 0.) continue;
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2) {
+            if (w2[a2] == 0.) continue;
+            const int lf = f1 * gr.n2f + p2 + a2 - 1;
+            const V r = res[gr.node_of ? gr.node_of[lf] : lf];
+            sx += w1 * w2[a2] * (double)r.x;
+            sy += w1 * w2[a2] * (double)r.y;
+        }
+    }
+    return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
+}
+
 template <typename T>
 __device__ __forceinline__ typename V2<T>::type mg_restrict_value(const MgGrid& gr, const typename V2<T>::type* res,
                                                                   uchar2 mc, int C)
@@ -841,25 +875,8 @@ __device__ __forceinline__ typename V2<T>::type mg_restrict_value(const MgGrid& 
         }
         return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
     }
-    const int K1 = C / gr.n2c, K2 = C - K1 * gr.n2c;
-    const int p1 = mg_pos(gr.n1f, gr.co1, K1, gr.per1), p2 = mg_pos(gr.n2f, gr.co2, K2);
-    for (int a1 = -1; a1 <= 1; ++a1) {
-        const int f1 = mg_step(gr.n1f, gr.per1, p1, a1);
-        if (f1 < 0) continue;
-        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1, gr.per1);
-        if (w1 == 0.) continue;
-        for (int a2 = -1; a2 <= 1; ++a2) {
-            const int f2 = p2 + a2;
-            if (f2 < 0 || f2 >= gr.n2f) continue;
-            const double w2 = mg_weight(gr.n2f, gr.co2, f2, K2);
-            if (w2 == 0.) continue;
-            const int lf = f1 * gr.n2f + f2;
-            const V r = res[gr.node_of ? gr.node_of[lf] : lf];
-            sx += w1 * w2 * (double)r.x;
-            sy += w1 * w2 * (double)r.y;
-        }
-    }
-    return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
+    const int K1 = C / gr.n2c;
+    return mg_restrict_lat<T>(gr, res, mc, K1, C - K1 * gr.n2c);
 }
 
 template <typename T>
@@ -878,6 +895,28 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(MgGrid gr, const typenam
     const int C = xcd_block(nbc) * kBlock + threadIdx.x;
     if (C >= nnc) return;
     mg_restrict_node<T>(gr, res, bc, mask_c, C);
+}
+
+// Prolongation to fine node n at lattice point (f1, f2) of an n1f x n2f lattice.
+template <typename T>
+__device__ __forceinline__ void mg_prolong_lat(const MgGrid& gr, const typename V2<T>::type* xc,
+                                               const typename V2<T>::type* xin, typename V2<T>::type* xout,
+                                               const uchar2* mask_f, int n, int f1, int f2)
+{
+    using V = typename V2<T>::type;
+    int Q1[2], Q2[2];
+    double W1[2], W2[2];
+    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1, gr.per1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
+    double sx = 0., sy = 0.;
+    for (int b1 = 0; b1 < nq1; ++b1)
+        for (int b2 = 0; b2 < nq2; ++b2) {
+            const V c = xc[Q1[b1] * gr.n2c + Q2[b2]];
+            sx += W1[b1] * W2[b2] * (double)c.x;
+            sy += W1[b1] * W2[b2] * (double)c.y;
+        }
+    const uchar2 mk = mask_f[n];
+    const V x = xin[n];
+    xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
 }
 
 // Coarse-grid correction: x_f = xin_f + mask_f * sum over f's parents of w * xc.
@@ -902,21 +941,10 @@ __device__ __forceinline__ void mg_prolong_node(const MgGrid& gr, const typename
         return;
     }
     const int lf = gr.lg ? gr.lg[n] : n;
-    const int f1 = lf / gr.n2f, f2 = lf - f1 * gr.n2f;
-    int Q1[2], Q2[2];
-    double W1[2], W2[2];
-    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1, gr.per1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
-    double sx = 0., sy = 0.;
-    for (int b1 = 0; b1 < nq1; ++b1)
-        for (int b2 = 0; b2 < nq2; ++b2) {
-            const V c = xc[Q1[b1] * gr.n2c + Q2[b2]];
-            sx += W1[b1] * W2[b2] * (double)c.x;
-            sy += W1[b1] * W2[b2] * (double)c.y;
-        }
-    const uchar2 mk = mask_f[n];
-    const V x = xin[n];
-    xout[n] = V{x.x + (mk.x ? (T)sx : (T)0), x.y + (mk.y ? (T)sy : (T)0)};
+    const int f1 = lf / gr.n2f;
+    mg_prolong_lat<T>(gr, xc, xin, xout, mask_f, n, f1, lf - f1 * gr.n2f);
 }
+
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_mg_prolong(MgGrid gr, const typename V2<T>::type* xc,
@@ -1403,8 +1431,12 @@ constexpr int kTailThreads = 1024;   // levels of at most this many nodes run in
 constexpr int kTailVecLds = 144 * 1024;
 constexpr int kTailMaxLevels = 16;
 // 160 KB per workgroup on gfx950, less the dense solve's right-hand side
-// and the levels' eigenvalue bounds
-constexpr int kTailLds = 160 * 1024 - (kMgDenseMax + kTailMaxLevels) * 8;
+// and the levels' Chebyshev coefficients
+template <typename T>
+__host__ __device__ constexpr int tail_lds_budget()
+{
+    return 160 * 1024 - kMgDenseMax * 8 - kTailMaxLevels * (int)sizeof(ChebCoef<T>);
+}
 
 template <typename T>
 struct TailLevel {
@@ -1416,6 +1448,26 @@ struct TailLevel {
     int minv_lds;      // byte offset of the LDS copy of the pivot inverses, or -1
     int mask_lds;      // byte offset of the LDS copy of the mask, or -1
 };
+
+// The LDS copies of the tail levels' rows, pivot inverses and masks lie in
+// one byte range [lo, lo + 16 n16) of the tail's LDS; `img` holds the same
+// bytes in global memory (k_mg_tail_image, after each rebuild of the coarse
+// levels), so the tail loads them with one coalesced copy.
+struct TailImage {
+    const float4* img;
+    int n16;
+    int lo;
+};
+
+// q = n / d for 0 <= n < 2^20 and d > 0 (the tail's lattice indices): a
+// float reciprocal estimate within one of q, then corrected.
+__device__ __forceinline__ int small_div(int n, int d)
+{
+    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    const int r = n - q * d;
+    q += r < 0 ? -1 : (r >= d ? 1 : 0);
+    return q;
+}
 
 // LDS bytes of the copies of one tail level's rows, pivot inverses and mask.
 template <typename T>
@@ -1464,24 +1516,6 @@ __device__ __forceinline__ void tail_block(const TailLevel<T>& L, const char* ld
     }
 }
 
-// Copy the rows of level L into its slot-major LDS block.
-template <typename T>
-__device__ __forceinline__ void tail_copy_rows(const TailLevel<T>& L, char* lds, int tid)
-{
-    const int nn = L.a.nn;
-    for (int i = tid; i < 9 * nn; i += kTailThreads) {
-        const int k = i / nn, n = i - k * nn;
-        const size_t src = lat_base(n) + (size_t)k * kWave;
-        if constexpr (std::is_same<T, float>::value) {
-            reinterpret_cast<float4*>(lds + L.mat_lds)[i] = reinterpret_cast<const float4*>(L.a.va)[src];
-        } else {
-            double2* pa = reinterpret_cast<double2*>(lds + L.mat_lds);
-            pa[i] = reinterpret_cast<const double2*>(L.a.va)[src];
-            pa[i + 9 * nn] = reinterpret_cast<const double2*>(L.a.vb)[src];
-        }
-    }
-}
-
 // One smoothing step of node n of a tail level from its nine row blocks
 // (rowf(k, b) fills block k): the arithmetic of mg_smooth_node with lattice
 // addressing, the Chebyshev coefficients given.
@@ -1493,7 +1527,7 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
 {
     using V = typename V2<T>::type;
     const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
-    const int i1 = n / a.lat_n2, i2 = n - i1 * a.lat_n2;
+    const int i1 = small_div(n, a.lat_n2), i2 = n - i1 * a.lat_n2;
     T yx = 0, yy = 0;
     // FP64 blocks three at a time: nine in flight would not fit the 128
     // registers of a 1024-thread workgroup
@@ -1557,51 +1591,60 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
 // coarsest and back, in one workgroup: the same per-node work as the level
 // kernels, with workgroup barriers instead of launches between the steps.
 // Everything a step reads is on the chip: the vectors of every level in
-// LDS; each level's rows, pivot inverses and mask copied into LDS once per
-// V-cycle where the host found room (coarsest first), the others read
-// from L2 (rows kept in registers instead spilled to scratch at the 128
-// registers a 1024-thread workgroup leaves each lane).  lv[0] is the
-// tail's finest level: its right-hand side, restricted by the level above,
-// is read from lv[0].v.b and its result is written to lv[0].v.xb.  The
-// coarsest level is solved by its explicit inverse, eight threads per row.
+// LDS; each level's rows, pivot inverses and mask in LDS where the host
+// found room (coarsest first; one coalesced copy of the tail image), the
+// others read from L2 (rows kept in registers instead spilled to scratch at
+// the 128 registers a 1024-thread workgroup leaves each lane); each level's
+// Chebyshev coefficients formed once per call.  lv[0] is the tail's finest
+// level: its right-hand side, restricted by the level above, is read from
+// lv[0].v.b and its result is written to lv[0].v.xb.  The coarsest level is
+// solved by its explicit inverse, eight threads per row (in FP32 tails the
+// threads load their part of the inverse at the start, so its latency
+// overlaps the smoothing).
 template <typename T>
-__device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, const double* Dinv, int nd)
+__device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
+                                             const TailImage im)
 {
     using V = typename V2<T>::type;
-    __shared__ __align__(16) char lds[kTailLds];
-    __shared__ double lam[kTailMaxLevels];
+    __shared__ __align__(16) char lds[tail_lds_budget<T>()];
+    __shared__ ChebCoef<T> cf[kTailMaxLevels];
     __shared__ double bs[kMgDenseMax];
     const int tid = threadIdx.x;
     {
+        constexpr int kBatch = 8;
+        float4* dst = reinterpret_cast<float4*>(lds + im.lo);
+        for (int i0 = tid; i0 < im.n16; i0 += kBatch * kTailThreads) {
+            // every lane loads (past the end: the last entry again), so the
+            // batch stays in registers and its loads are all in flight at once
+            float4 t[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) t[u] = im.img[min(i0 + u * kTailThreads, im.n16 - 1)];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int i = i0 + u * kTailThreads;
+                if (i < im.n16) dst[i] = t[u];
+            }
+        }
         const TailLevel<T> L0 = lv[0];
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
         for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
-        for (int l = 0; l < nl; ++l) {
-            const TailLevel<T> L = lv[l];
-            if (tid == 0) lam[l] = *L.v.lam;
-            if (L.mat_lds >= 0) tail_copy_rows<T>(L, lds, tid);
-            if (L.minv_lds >= 0)
-                for (int n = tid; n < L.a.nn; n += kTailThreads)
-                    reinterpret_cast<typename V4<T>::type*>(lds + L.minv_lds)[n] = L.a.minv[n];
-            if (L.mask_lds >= 0)
-                for (int n = tid; n < L.a.nn; n += kTailThreads)
-                    reinterpret_cast<uchar2*>(lds + L.mask_lds)[n] = L.a.mask[n];
-        }
+        if (tid < nl) cf[tid] = mg_cheb<T>(*lv[tid].v.lam);
         __syncthreads();
     }
-    auto step = [&](const TailLevel<T>& L, int l, auto mode, const V* xi, const V* di, V* xo, V* dout, int second,
+    auto step = [&](const TailLevel<T>& L, int l, auto mode, const V* xi, const V* di_, V* xo, V* dout, int second,
                     int last) {
         constexpr int MODE = decltype(mode)::value;
         if (tid >= L.a.nn) return;
-        const ChebCoef<T> c = mg_cheb<T>(lam[l]);
-        tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi, di,
+        const ChebCoef<T> c = cf[l];
+        tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi, di_,
                                   xo, dout, second, last, tid);
     };
     using Pre1 = std::integral_constant<int, MG_PRE1>;
     using Cheb = std::integral_constant<int, MG_CHEB>;
     using Resid = std::integral_constant<int, MG_RESID>;
     using Post1 = std::integral_constant<int, MG_POST1>;
-    for (int l = 0; l < nl - 1; ++l) {
+    // down: level l's pre-smoothing, then the restriction of its residual
+    auto smooth_down = [&](int l) {
         const TailLevel<T> L = tail_level<T>(lv, l, lds);
         step(L, l, Pre1(), nullptr, nullptr, L.v.xa, L.v.da, 0, 0);
         __syncthreads();
@@ -1609,10 +1652,35 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         __syncthreads();
         step(L, l, Resid(), L.v.xb, L.v.db, L.v.xa, nullptr, 0, 0);
         __syncthreads();
+    };
+    auto restrict_down = [&](int l) {
+        const TailLevel<T> L = tail_level<T>(lv, l, lds);
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
-        if (tid < Cn.a.nn) mg_restrict_node<T>(L.grid, L.v.res, const_cast<V*>(Cn.v.b), Cn.a.mask, tid);
+        if (tid < Cn.a.nn) {
+            const int K1 = small_div(tid, L.grid.n2c);
+            const_cast<V*>(Cn.v.b)[tid] = mg_restrict_lat<T>(L.grid, L.v.res, Cn.a.mask[tid], K1, tid - K1 * L.grid.n2c);
+        }
         __syncthreads();
+    };
+    for (int l = 0; l < nl - 2; ++l) {
+        smooth_down(l);
+        restrict_down(l);
     }
+    // row i = tid / 8 of the inverse, its columns j = tid % 8 + 8 m; FP32
+    // tails load their part of it before the last restriction, whose work
+    // then covers the loads' latency
+    const int di = tid >> 3, dj0 = tid & 7;
+    constexpr int kDenseCols = kMgDenseMax / 8;
+    double dv[kDenseCols];
+    if (nl >= 2) smooth_down(nl - 2);
+    if constexpr (std::is_same<T, float>::value) {
+#pragma unroll
+        for (int m = 0; m < kDenseCols; ++m) {
+            const int j = dj0 + 8 * m;
+            dv[m] = (di < nd && j < nd) ? Dinv[(size_t)di * nd + j] : 0.;
+        }
+    }
+    if (nl >= 2) restrict_down(nl - 2);
     {
         const TailLevel<T> Cn = tail_level<T>(lv, nl - 1, lds);
         for (int i = tid; i < nd / 2; i += kTailThreads) {
@@ -1621,25 +1689,35 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
             bs[2 * i + 1] = b.y;
         }
         __syncthreads();
-        // row i = tid / 8 of the inverse, its columns j = tid % 8 + 8 t,
-        // summed over the eight lanes of the row in a fixed order
-        const int i = tid >> 3, j0 = tid & 7;
+        // the eight lanes of a row sum its columns in a fixed order
         double s = 0.;
-        if (i < nd)
-            for (int j = j0; j < nd; j += 8) s += Dinv[(size_t)i * nd + j] * bs[j];
+        if (di < nd) {
+            if constexpr (std::is_same<T, float>::value) {
+#pragma unroll
+                for (int m = 0; m < kDenseCols; ++m) {
+                    const int j = dj0 + 8 * m;
+                    if (j < nd) s += dv[m] * bs[j];
+                }
+            } else {
+                for (int j = dj0; j < nd; j += 8) s += Dinv[(size_t)di * nd + j] * bs[j];
+            }
+        }
         s += __shfl_xor(s, 1);
         s += __shfl_xor(s, 2);
         s += __shfl_xor(s, 4);
-        if (i < nd && j0 == 0) {
+        if (di < nd && dj0 == 0) {
             T* xc = reinterpret_cast<T*>(Cn.v.xb);
-            xc[i] = (T)s;
+            xc[di] = (T)s;
         }
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
         const TailLevel<T> L = tail_level<T>(lv, l, lds);
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
-        if (tid < L.a.nn) mg_prolong_node<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid);
+        if (tid < L.a.nn) {
+            const int f1 = small_div(tid, L.grid.n2f);
+            mg_prolong_lat<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid, f1, tid - f1 * L.grid.n2f);
+        }
         __syncthreads();
         step(L, l, Post1(), L.v.xb, nullptr, nullptr, L.v.da, 0, 0);
         __syncthreads();
@@ -1658,10 +1736,38 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
 
 template <typename T>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
-                                                          const CgScal* sc)
+                                                          const CgScal* sc, TailImage im)
 {
     if (sc && sc->done) return;
-    mg_tail_body<T>(lv, nl, Dinv, nd);
+    mg_tail_body<T>(lv, nl, Dinv, nd, im);
+}
+
+// The tail image (TailImage): one workgroup per tail level copies the
+// level's rows (slot-major, from its lattice rows), pivot inverses and mask
+// to where they lie in the tail's LDS, less lo.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_tail_image(const TailLevel<T>* lv, char* img, int lo)
+{
+    const TailLevel<T> L = lv[blockIdx.x];
+    const int nn = L.a.nn;
+    if (L.mat_lds >= 0) {
+        for (int i = threadIdx.x; i < 9 * nn; i += kBlock) {
+            const int k = i / nn, n = i - k * nn;
+            const size_t src = lat_base(n) + (size_t)k * kWave;
+            if constexpr (std::is_same<T, float>::value) {
+                reinterpret_cast<float4*>(img + L.mat_lds - lo)[i] = reinterpret_cast<const float4*>(L.a.va)[src];
+            } else {
+                double2* pa = reinterpret_cast<double2*>(img + L.mat_lds - lo);
+                pa[i] = reinterpret_cast<const double2*>(L.a.va)[src];
+                pa[i + 9 * nn] = reinterpret_cast<const double2*>(L.a.vb)[src];
+            }
+        }
+    }
+    if (L.minv_lds >= 0)
+        for (int n = threadIdx.x; n < nn; n += kBlock)
+            reinterpret_cast<typename V4<T>::type*>(img + L.minv_lds - lo)[n] = L.a.minv[n];
+    if (L.mask_lds >= 0)
+        for (int n = threadIdx.x; n < nn; n += kBlock) reinterpret_cast<uchar2*>(img + L.mask_lds - lo)[n] = L.a.mask[n];
 }
 
 // ------------------------------------------------------------ CG with the V-cycle
@@ -1804,6 +1910,7 @@ struct TailOp {
     int nl, nd;
     const double* Dinv;
     const CgScal* sc;
+    TailImage im;
 };
 
 template <typename T>
@@ -1861,7 +1968,7 @@ __global__ __launch_bounds__(kTailThreads) void k_b_tail(const TailOp<T>* ops)
 {
     const TailOp<T> o = ops[blockIdx.x];
     if (o.sc && o.sc->done) return;
-    mg_tail_body<T>(o.lv, o.nl, o.Dinv, o.nd);
+    mg_tail_body<T>(o.lv, o.nl, o.Dinv, o.nd, o.im);
 }
 
 template <typename T>

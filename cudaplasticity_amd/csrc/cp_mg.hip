@@ -63,6 +63,8 @@ struct MgState {
     int since[2] = {0, 0};
     bool built[2] = {false, false};
     void* tail[2] = {nullptr, nullptr};
+    // the tail image (cp_mg.h, TailImage) per precision
+    cpk::TailImage tail_img[2] = {};
 
     template <typename X>
     int get(X** p, size_t count)
@@ -175,20 +177,26 @@ static int mg_tail_table(cp_problem* P)
         tab.push_back(t);
     }
     if (lds > cpk::kTailVecLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
-    // the rest of the LDS: copies of the pivot inverses and masks, then of
-    // the rows, coarsest level first
+    // the rest of the LDS (the tail image): copies of the masks, then of the
+    // pivot inverses, then of the rows, coarsest level first; the coarsest
+    // level is solved with its inverse and needs only its mask
     const int nt = (int)tab.size();
+    const int budget = cpk::tail_lds_budget<T>(), img_lo = lds;
     for (int i = nt - 1; i >= 0; --i) {
-        const int nn = tab[i].a.nn;
-        const int need = cpk::tail_minv_bytes<T>(nn) + cpk::tail_mask_bytes(nn);
-        if (lds + need > cpk::kTailLds) break;
-        tab[i].minv_lds = lds;
-        tab[i].mask_lds = lds + cpk::tail_minv_bytes<T>(nn);
+        const int need = cpk::tail_mask_bytes(tab[i].a.nn);
+        if (lds + need > budget) break;
+        tab[i].mask_lds = lds;
         lds += need;
     }
-    for (int i = nt - 1; i >= 0; --i) {
+    for (int i = nt - 2; i >= 0; --i) {
+        const int need = cpk::tail_minv_bytes<T>(tab[i].a.nn);
+        if (lds + need > budget) break;
+        tab[i].minv_lds = lds;
+        lds += need;
+    }
+    for (int i = nt - 2; i >= 0; --i) {
         const int need = cpk::tail_rows_bytes<T>(tab[i].a.nn);
-        if (lds + need > cpk::kTailLds) break;
+        if (lds + need > budget) break;
         tab[i].mat_lds = lds;
         lds += need;
     }
@@ -196,7 +204,10 @@ static int mg_tail_table(cp_problem* P)
     cpk::TailLevel<T>* d = nullptr;
     if (int s = G.get(&d, tab.size())) return s;
     CP_HIP(hipMemcpy(d, tab.data(), sizeof(cpk::TailLevel<T>) * tab.size(), hipMemcpyHostToDevice));
+    float4* img = nullptr;
+    if (int s = G.get(&img, (size_t)(lds - img_lo) / 16)) return s;
     G.tail[pi] = d;
+    G.tail_img[pi] = cpk::TailImage{img, (lds - img_lo) / 16, img_lo};
     return CP_OK;
 }
 
@@ -344,6 +355,10 @@ static int mg_setup_t(cp_problem* P, int nlev)
             hipLaunchKernelGGL(cpk::k_mg_invert, dim3(1), dim3(1024), 0, st, G.dense, G.nd);
         }
     }
+    // the tail levels changed: their LDS image
+    if (nlev == G.nlev && G.tail[pi] && G.tail_img[pi].n16 > 0)
+        hipLaunchKernelGGL(cpk::k_mg_tail_image<T>, dim3(G.nlev - G.lt), dim3(cpk::kBlock), 0, st,
+                           (const cpk::TailLevel<T>*)G.tail[pi], (char*)G.tail_img[pi].img, G.tail_img[pi].lo);
     CP_HIP(hipGetLastError());
     return CP_OK;
 }
@@ -416,7 +431,8 @@ struct Direct {
     template <typename T>
     void tail(unsigned long long, const cpk::TailOp<T>& o)
     {
-        hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st, o.lv, o.nl, o.Dinv, o.nd, o.sc);
+        hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st, o.lv, o.nl, o.Dinv, o.nd, o.sc,
+                           o.im);
     }
     template <typename T>
     void update(unsigned long long, const cpk::CgOp<T>& o, int nblocks)
@@ -638,6 +654,7 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         o.Dinv = (const double*)G.dense;
         o.nd = G.nd;
         o.sc = sc;
+        o.im = G.tail_img[pi];
         e.template tail<T>(step_key(SECT_TAIL, 0, 0), o);
     }
     for (int l = last - 1; l >= 0; --l) {
