@@ -97,8 +97,10 @@ void mg_destroy(cp_problem* P)
 static int blocks_of(int n) { return (n + cpk::kBlock - 1) / cpk::kBlock; }
 
 constexpr double kSegFrac = 0.05;   // share of stretched cells that selects the segment smoother
-// Largest coarse segment level smoothed with Chebyshev degree 2.
+// Largest coarse segment level smoothed with Chebyshev degree 2 in FP64.
 constexpr int kDeg2Nodes = 65536;
+// Smallest finest segment level smoothed with Chebyshev degree 2.
+constexpr int kDeg2Finest = 1 << 20;
 
 // Per-precision value and vector arrays, allocated on first use.
 static int mg_alloc(cp_problem* P, int prec)
@@ -612,14 +614,20 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         return g;
     };
     const int last = G.lt;   // levels lt.. run in the one-workgroup tail
-    // Chebyshev degree per launched level: 3, except 2 on the coarser
-    // segment levels (from level 1 in FP32, level 2 in FP64) of at most
-    // kDeg2Nodes nodes, where each step is a latency-bound launch.  On the
-    // 632x316 plate the V-cycle is 14 % shorter for 8 % more CG iterations;
-    // on larger levels the launches are bandwidth-bound and the extra
-    // iterations cost more than the steps saved
-    const int deg2_from = std::is_same<T, double>::value ? 2 : 1;
-    auto degree = [&](int l) { return G.L[l].seg && l >= deg2_from && G.L[l].nn <= kDeg2Nodes ? 2 : 3; };
+    // Chebyshev degree per launched segment level: 2 on the coarse levels
+    // in FP32 (in FP64 from level 2, on levels of at most kDeg2Nodes nodes)
+    // and on a finest level of at least kDeg2Finest nodes, 3 elsewhere.
+    // Degree 2 saves two smoothing launches per level and V-cycle for
+    // about a quarter more CG iterations: a gain where the launches are
+    // latency-bound (small levels) or stream HBM (the 2M-node plate: 4 %
+    // faster ramp), a loss on a cache-resident finest level (the 632x316
+    // plate: 7 % slower ramp)
+    auto degree = [&](int l) {
+        if (!G.L[l].seg) return 3;
+        if (l == 0) return G.L[0].nn >= kDeg2Finest ? 2 : 3;
+        if (std::is_same<T, float>::value) return 2;
+        return l >= 2 && G.L[l].nn <= kDeg2Nodes ? 2 : 3;
+    };
     // a launched segment level below level 0 restricts its right-hand side
     // in its first smoothing launch
     auto fused_restrict = [&](int l) { return l > 0 && l < last && G.L[l].seg; };
