@@ -81,15 +81,30 @@ __device__ __forceinline__ void block_partial(double (&v)[K], double* partials, 
 }
 
 // Fixed-order sum of nblocks partials by one block; thread 0 gets the totals.
+// Thread t sums partials t, t + kBlock, ... in that order, eight of them
+// loaded before they are added (every lane loads: past the end, the last
+// partial again), so their loads are in flight together.
 template <int K>
 __device__ __forceinline__ void reduce_partials(const double* partials, int nblocks, double (&v)[K])
 {
     __shared__ double sh[4 * K];
 #pragma unroll
     for (int k = 0; k < K; ++k) v[k] = 0.;
-    for (int j = threadIdx.x; j < nblocks; j += kBlock)
+    constexpr int kBatch = 8;
+    for (int j0 = threadIdx.x; j0 < nblocks; j0 += kBatch * kBlock) {
+        double t[kBatch][K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) v[k] += partials[(size_t)j * K + k];
+        for (int u = 0; u < kBatch; ++u) {
+            const int j = min(j0 + u * kBlock, nblocks - 1);
+#pragma unroll
+            for (int k = 0; k < K; ++k) t[u][k] = partials[(size_t)j * K + k];
+        }
+#pragma unroll
+        for (int u = 0; u < kBatch; ++u)
+            if (j0 + u * kBlock < nblocks)
+#pragma unroll
+                for (int k = 0; k < K; ++k) v[k] += t[u][k];
+    }
     block_sum<K>(v, sh);
 }
 
