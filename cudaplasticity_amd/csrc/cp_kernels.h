@@ -83,7 +83,8 @@ __device__ __forceinline__ void block_partial(double (&v)[K], double* partials, 
 // Fixed-order sum of nblocks partials by one block; thread 0 gets the totals.
 // Thread t sums partials t, t + kBlock, ... in that order, eight of them
 // loaded before they are added (every lane loads: past the end, the last
-// partial again), so their loads are in flight together.
+// partial again, and adds +0 for it, which leaves the sum's bits as they
+// are), so their loads are in flight together.
 template <int K>
 __device__ __forceinline__ void reduce_partials(const double* partials, int nblocks, double (&v)[K])
 {
@@ -100,10 +101,11 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int nblo
             for (int k = 0; k < K; ++k) t[u][k] = partials[(size_t)j * K + k];
         }
 #pragma unroll
-        for (int u = 0; u < kBatch; ++u)
-            if (j0 + u * kBlock < nblocks)
+        for (int u = 0; u < kBatch; ++u) {
+            const bool in = j0 + u * kBlock < nblocks;
 #pragma unroll
-                for (int k = 0; k < K; ++k) v[k] += t[u][k];
+            for (int k = 0; k < K; ++k) v[k] += in ? t[u][k] : 0.;
+        }
     }
     block_sum<K>(v, sh);
 }

@@ -124,6 +124,11 @@ __host__ __device__ inline int mg_slot(int d1, int d2)
 
 template <typename T> struct ChebCoef { T c0, a1, b1, a2, b2; };
 
+// A level's lambda and, after it, the Chebyshev coefficients it gives
+// (kLamSlots doubles per precision): written together at set-up, read by
+// every smoothing step instead of formed again in each thread.
+constexpr int kLamSlots = 8;
+
 // Coefficients of the first three steps of the Chebyshev three-term
 // recurrence on [lam/30, 1.1 lam]; step k's coefficients do not depend on
 // the degree, so degree 2 uses c0, a1, b1 only.
@@ -138,6 +143,26 @@ __device__ __forceinline__ ChebCoef<T> mg_cheb(double lam)
     const double r2 = 1. / (2. * sigma - r1);
     return ChebCoef<T>{(T)(1. / theta), (T)(r1 * r0), (T)(2. * r1 / delta), (T)(r2 * r1), (T)(2. * r2 / delta)};
 }
+
+// Sets a level's lambda and its coefficients (one thread).
+__device__ __forceinline__ void mg_set_lam(double* lam, double value)
+{
+    const ChebCoef<double> c = mg_cheb<double>(value);
+    lam[0] = value;
+    lam[1] = c.c0;
+    lam[2] = c.a1;
+    lam[3] = c.b1;
+    lam[4] = c.a2;
+    lam[5] = c.b2;
+}
+
+// The coefficients stored by mg_set_lam, in precision T.
+template <typename T>
+__device__ __forceinline__ ChebCoef<T> mg_cheb_of(const double* lam)
+{
+    return ChebCoef<T>{(T)lam[1], (T)lam[2], (T)lam[3], (T)lam[4], (T)lam[5]};
+}
+
 
 __device__ __forceinline__ double wave_max(double v)
 {
@@ -455,7 +480,7 @@ static __global__ __launch_bounds__(kBlock) void k_mg_lam(const double* partials
     double v = 0.;
     for (int j = threadIdx.x; j < nblocks; j += kBlock) v = fmax(v, partials[j]);
     v = block_max(v);
-    if (threadIdx.x == 0) *lam = v > 0. ? v : 1.;
+    if (threadIdx.x == 0) mg_set_lam(lam, v > 0. ? v : 1.);
 }
 
 // Dense copy of a level's operator (nd = 2 nn DOFs), one node's two rows per thread.
@@ -744,7 +769,7 @@ __device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T
                                                typename V2<T>::type* dout, int second, int last, int n)
 {
     using V = typename V2<T>::type;
-    const ChebCoef<T> c = mg_cheb<T>(*v.lam);
+    const ChebCoef<T> c = mg_cheb_of<T>(v.lam);
     const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
     T yx, yy;
     row_apply<W, T>(a, n,
@@ -1101,7 +1126,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
     // along a periodic direction an odd number of segments needs three
     // colours (the last segment touches the first): lambda_max <= 6
     const int nseg1 = (g.n1 + kSeg - 1) / kSeg;
-    if (blockIdx.x == 0 && threadIdx.x == 0) *lam = (g.per1 && (nseg1 & 1)) ? 1.5 * kSegLam : kSegLam;
+    if (blockIdx.x == 0 && threadIdx.x == 0) mg_set_lam(lam, (g.per1 && (nseg1 & 1)) ? 1.5 * kSegLam : kSegLam);
     const int idx = blockIdx.x * kBlock + threadIdx.x;
     const int nseg = (g.n1 + kSeg - 1) / kSeg;
     if (idx >= nseg * g.n2) return;
@@ -1206,7 +1231,7 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
     const int i1 = T1 * kSeg + t1, i2 = T2 * kSegCols + t2;
     const bool in = i1 < g.n1 && i2 < g.n2;
     const int n = in ? seg_node(g, i1, i2) : 0;
-    const ChebCoef<T> c = mg_cheb<T>(*v.lam);
+    const ChebCoef<T> c = mg_cheb_of<T>(v.lam);
     T rx = 0, ry = 0;
     V xn = V{0, 0}, dn = V{0, 0};
     T f[4] = {0, 0, 0, 0}, h[4] = {0, 0, 0, 0};
@@ -1614,21 +1639,19 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         constexpr int kBatch = 8;
         float4* dst = reinterpret_cast<float4*>(lds + im.lo);
         for (int i0 = tid; i0 < im.n16; i0 += kBatch * kTailThreads) {
-            // every lane loads (past the end: the last entry again), so the
-            // batch stays in registers and its loads are all in flight at once
+            // every lane loads and stores (past the end: the last entry, with
+            // the value it holds), so no branch separates a load from its
+            // store and the batch's loads are all in flight at once
             float4 t[kBatch];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) t[u] = im.img[min(i0 + u * kTailThreads, im.n16 - 1)];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int i = i0 + u * kTailThreads;
-                if (i < im.n16) dst[i] = t[u];
-            }
+            for (int u = 0; u < kBatch; ++u) dst[min(i0 + u * kTailThreads, im.n16 - 1)] = t[u];
         }
         const TailLevel<T> L0 = lv[0];
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
         for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
-        if (tid < nl) cf[tid] = mg_cheb<T>(*lv[tid].v.lam);
+        if (tid < nl) cf[tid] = mg_cheb_of<T>(lv[tid].v.lam);
         __syncthreads();
     }
     auto step = [&](const TailLevel<T>& L, int l, auto mode, const V* xi, const V* di_, V* xo, V* dout, int second,

@@ -165,7 +165,7 @@ static int mg_tail_table(cp_problem* P)
         t.v.da = (V*)L.vec[pi][3];
         t.v.db = (V*)L.vec[pi][4];
         t.v.res = (V*)L.vec[pi][5];
-        t.v.lam = L.lam + pi;
+        t.v.lam = L.lam + pi * cpk::kLamSlots;
         if (l + 1 < G.nlev) {
             t.grid.n1f = L.n1;
             t.grid.n2f = L.n2;
@@ -346,12 +346,12 @@ static int mg_setup_t(cp_problem* P, int nlev)
         if (l < G.nlev - 1 && L.seg) {
             const int cols = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * L.n2;
             hipLaunchKernelGGL(cpk::k_mg_seg_setup<T>, dim3(blocks_of(cols)), dim3(cpk::kBlock), 0, st, a,
-                               seg_grid(G, l), level_minv<T>(P, l), L.lam + pi);
+                               seg_grid(G, l), level_minv<T>(P, l), L.lam + pi * cpk::kLamSlots);
         } else if (l < G.nlev - 1) {
             hipLaunchKernelGGL(cpk::k_mg_smooth_setup<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a,
                                level_minv<T>(P, l), P->partials);
             hipLaunchKernelGGL(cpk::k_mg_lam, dim3(1), dim3(cpk::kBlock), 0, st, (const double*)P->partials, L.nb,
-                               L.lam + pi);
+                               L.lam + pi * cpk::kLamSlots);
         } else {
             hipLaunchKernelGGL(cpk::k_mg_dense<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a, G.dense, G.nd);
             hipLaunchKernelGGL(cpk::k_mg_invert, dim3(1), dim3(1024), 0, st, G.dense, G.nd);
@@ -592,7 +592,7 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         v.da = (V*)L.vec[pi][3];
         v.db = (V*)L.vec[pi][4];
         v.res = (V*)L.vec[pi][5];
-        v.lam = L.lam + pi;
+        v.lam = L.lam + pi * cpk::kLamSlots;
         v.sc = sc;
         return v;
     };
@@ -1179,7 +1179,7 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     v.da = (V*)L.vec[pi][3];
     v.db = (V*)L.vec[pi][4];
     v.res = (V*)L.vec[pi][5];
-    v.lam = L.lam + pi;
+    v.lam = L.lam + pi * cpk::kLamSlots;
     const bool seg = L.seg != 0;
     const cpk::SegGrid sg = seg_grid(G, 0);
     Direct e{st};
@@ -1439,7 +1439,7 @@ static int set_grid(cp_problem* P, int n1, int n2, const int* node_of, int per1)
     int s = CP_OK;
     for (int l = 0; l < G->nlev && !s; ++l) {
         MgLevel& L = G->L[l];
-        if ((s = G->get(&L.lam, 2))) break;
+        if ((s = G->get(&L.lam, 2 * cpk::kLamSlots))) break;
         if (l == 0) continue;
         if ((s = G->get(&L.mask, L.nn))) break;
         if ((s = G->get(&L.slice, L.nslices))) break;
