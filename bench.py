@@ -46,9 +46,11 @@ NSTEPS = 20
 # the fused block-Jacobi CG iteration (--precond 1) 48.19 MB read + 8.05 MB
 # written; the finest level's segment smoothing step (multigrid, the
 # default) 44.93 MB read + 3.21 MB written; on the 2000x1000 plate
-# (HBM-resident) the same step 475.8 MB read + 33.0 MB written.
+# (HBM-resident) the same step 475.8 MB read + 33.0 MB written in the
+# sliced-ELL rows, 290.3 MB read + 32.1 MB written in the symmetric lattice
+# layout.
 MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6,
-                    (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 316.4e6}
+                    (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 322.4e6}
 
 
 def log(*a):
