@@ -48,9 +48,11 @@ NSTEPS = 20
 # default) 44.93 MB read + 3.21 MB written; on the 2000x1000 plate
 # (HBM-resident) the same step 475.8 MB read + 33.0 MB written in the
 # sliced-ELL rows, 290.3 MB read + 32.1 MB written in the symmetric lattice
-# layout.
+# layout; the multigrid CG's operator product on that plate (FP32, symmetric
+# lattice layout) 241.3 MB read + 16.2 MB written.
 MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6,
-                    (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 322.4e6}
+                    (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 322.4e6,
+                    (2000, 1000, "mg_cg_dots_sym"): 257.5e6}
 
 
 def log(*a):
@@ -150,6 +152,40 @@ def bench_mg(A, P, precision, reps):
     v, sm, b = C.c_double(), C.c_double(), C.c_double()
     A.check(A.lib().cp_bench_mg(P.ptr, precision, reps, C.byref(v), C.byref(sm), C.byref(b)))
     return v.value, sm.value, b.value
+
+
+def bench_mg_spmv(A, P, precision, reps):
+    """The multigrid CG's operator product alone (w = mask (A u) and the
+    iteration's three dot products): average ms and the bytes one launch
+    must move in level 0's layout."""
+    import ctypes as C
+    ms, b = C.c_double(), C.c_double()
+    A.check(A.lib().cp_bench_mg_spmv(P.ptr, precision, reps, C.byref(ms), C.byref(b)))
+    return ms.value, b.value
+
+
+def spmv_kernel(P):
+    """Name and traffic key of the multigrid CG's FP32 operator-product kernel."""
+    w, tag = (-2, "_sym") if P.mg_level0_sym() else (9, "")
+    return "cpk::k_mg_cg_dots<%d,float>" % w, "mg_cg_dots" + tag
+
+
+def spmv_record(A, P, nt, nr, nnz, N, reps):
+    """The SpMV of the north star on the path that runs: the multigrid CG's
+    w = mask (A u) kernel (FP32), its layout bytes per launch over the mean
+    launch time against the HBM peak, and the rocprofv3 traffic measured for
+    it where there is one."""
+    ms, fmt = bench_mg_spmv(A, P, A.MIXED, reps)
+    kname, tkey = spmv_kernel(P)
+    traffic = MEASURED_TRAFFIC.get((nt, nr, tkey))
+    return {"kernel": kname + " (multigrid CG: w = mask (A u) with (r, u), (w, u), (r, r); FP32)",
+            "avg_launch_ms": ms, "reps": reps, "bytes_per_launch": fmt,
+            "bytes_model": ("what one launch must move in level 0's layout: the stored operator once (the "
+                            "symmetric lattice layout: five 2x2 blocks per node), u and r read, w written, the "
+                            "mask and the per-block partials"),
+            "achieved": fmt / ms / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": fmt / ms / 1e6 / HBM_PEAK_GBS,
+            "traffic": traffic, "traffic_gbps": traffic / ms / 1e6 if traffic else None,
+            "reference_csr_bytes_per_launch": nnz * 8 + 4 * (N + 1) + 2 * N * 4}
 
 
 def rel(a, b):
@@ -761,6 +797,8 @@ def c3_workload(A, args):
             "working_set_bytes": fmt_s32,
             "traffic_gbps": traffic / ms_s32 / 1e6 if traffic else None,
             "reference_csr_bytes_per_launch": ref_bytes}
+    spmv = spmv_record(A, Pmx, nt, nr, nnz, 2 * mesh.nn, reps)
+    log("spmv:", spmv)
     mv64, ms_s64, fmt_s64 = bench_mg(A, Pmx, A.FP64, max(20, reps // 4))
     mg = {"levels": Pmx.mg_levels(),
           "smoothers": ["segments" if k else "block-Jacobi" for k in Pmx.mg_smoothers()[:-1]] + ["exact"],
@@ -807,7 +845,7 @@ def c3_workload(A, args):
         "iterations": {"mixed": [[s.outer_iters, s.cg_iters, s.refinements] for s in per_mx],
                        "fp64": [[s.outer_iters, s.cg_iters] for s in per_64]},
         "device_time_mixed": times,
-        "roofline": roof, "multigrid": mg, "hbm_copy": copy,
+        "roofline": roof, "spmv": spmv, "multigrid": mg, "hbm_copy": copy,
         "c2": subs.get("c2"), "c4": subs.get("c4"), "served": served,
         "cpu_baseline": None,
     }
@@ -935,16 +973,19 @@ def main():
         P = A.Problem(ctx, mesh, loads, mat)
         P.set_load(1.0)
         N, nnz = 2 * mesh.nn, scalar_nnz(A, P)
+        spmv = None
         if args.precond == 2:
             P.set_grid(*grid)
             _, ms, fmt = bench_mg(A, P, A.MIXED, args.reps)
             name, byt = smooth_kernel(P)[0], cheb_step_bytes(nnz, N, 4)
+            # the CG's operator product after it, in the same run (counter runs cover both)
+            spmv = spmv_record(A, P, args.nt, args.nr, nnz, N, args.reps)
         else:
             ms, fmt = bench_cg(A, P, A.MIXED, args.reps)
             name, byt = "cpk::k_cg_iter_fused<9,true,float>", cg_iteration_bytes(nnz, N, 4)
         print(json.dumps({"kernel": name, "launches": args.reps, "avg_launch_ms": ms, "bytes_per_launch": byt,
                           "achieved_gbps": byt / ms / 1e6, "format_bytes_per_launch": fmt,
-                          "achieved_format_gbps": fmt / ms / 1e6}), flush=True)
+                          "achieved_format_gbps": fmt / ms / 1e6, "spmv": spmv}), flush=True)
         P.close()
         return
     opt_mx = solver_options(A, args, A.MIXED)

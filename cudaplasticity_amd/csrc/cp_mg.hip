@@ -1218,6 +1218,42 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     return CP_OK;
 }
 
+// The multigrid CG's operator product alone: `reps` launches of its
+// w = mask (A u) kernel with the three dot products (level 0's layout: the
+// symmetric lattice layout in FP32 where it is built), u a V-cycle result.
+template <typename T>
+static int mg_bench_spmv_t(cp_problem* P, int reps, double* ms, double* bytes)
+{
+    using V = typename cpk::V2<T>::type;
+    MgState& G = *P->mg;
+    const int pi = std::is_same<T, double>::value ? 0 : 1;
+    hipStream_t st = P->stream;
+    cpk::CgArgs<T> a0;
+    if constexpr (std::is_same<T, double>::value) a0 = P->cg64_args();
+    else a0 = P->cg32_args();
+    hipLaunchKernelGGL(cpk::k_cg_setup, dim3(1), dim3(1), 0, st, a0.sc, 1.0, 0.0, 1 << 30);
+    cpk::CgOp<T> o{};
+    o.a = level_args<T>(P, 0);
+    o.a.sc = a0.sc;
+    o.u = (const V*)G.L[0].vec[pi][2];
+    Direct e{st};
+    CP_HIP(hipEventRecord(P->ev[4], st));
+    for (int i = 0; i < reps; ++i)
+        by_level0<T>(P, [&](auto w) { e.template dots<decltype(w)::value, T>(step_key(SECT_DOTS, 0, 0), o, P->nbn); });
+    CP_HIP(hipEventRecord(P->ev[5], st));
+    CP_HIP(hipGetLastError());
+    CP_HIP(hipEventSynchronize(P->ev[5]));
+    float t = 0.f;
+    CP_HIP(hipEventElapsedTime(&t, P->ev[4], P->ev[5]));
+    if (ms) *ms = t / reps;
+    const double s = std::is_same<T, double>::value ? 8. : 4.;
+    const double mat = G.vsym32 && pi == 1 ? (double)G.L[0].nslices * cpk::kWave * cpk::kSymSlots * 4. * s
+                                           : P->nblk_stored * (4. * s + 4.) + 8. * P->nslices;
+    // u and r read, w written (2 values per node each), the mask, the partials
+    if (bytes) *bytes = mat + 3. * 2. * s * P->nn + 2. * P->nn + 24. * P->nbn;
+    return CP_OK;
+}
+
 }  // namespace cp
 
 extern "C" {
@@ -1230,6 +1266,16 @@ int cp_bench_mg(cp_problem* P, int precision, int reps, double* ms_vcycle, doubl
     if (int s = cp::launch_assemble(P, precision)) return s;
     return precision == CP_FP64 ? cp::mg_bench_t<double>(P, reps, ms_vcycle, ms_smooth, bytes_smooth)
                                 : cp::mg_bench_t<float>(P, reps, ms_vcycle, ms_smooth, bytes_smooth);
+}
+
+int cp_bench_mg_spmv(cp_problem* P, int precision, int reps, double* ms, double* bytes)
+{
+    if (!P || reps < 1 || (precision != CP_FP64 && precision != CP_MIXED)) return cp::fail(CP_ERR_INVALID, "bad arguments");
+    if (!P->mg) return cp::fail(CP_ERR_STATE, "cp_bench_mg_spmv: no multigrid lattice set");
+    P->precond = 2;
+    if (int s = cp::launch_assemble(P, precision)) return s;
+    return precision == CP_FP64 ? cp::mg_bench_spmv_t<double>(P, reps, ms, bytes)
+                                : cp::mg_bench_spmv_t<float>(P, reps, ms, bytes);
 }
 
 namespace cp {

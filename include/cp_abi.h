@@ -393,6 +393,13 @@ int cp_bench_assemble(cp_problem* prob, int precision, int reps, double* ms);
  * library's storage format. */
 int cp_bench_mg(cp_problem* prob, int precision, int reps, double* ms_vcycle, double* ms_smooth,
                 double* bytes_smooth);
+/* The same set-up, then `reps` launches of the multigrid CG's operator
+ * product alone (w = mask (A u) with the iteration's three dot products, on
+ * level 0's storage: the symmetric lattice layout in FP32 where it is
+ * built); returns their average ms and the bytes one launch must move in
+ * that layout.  Replaces the timing of cg1sparse (ConjGradCuda.cu:39-60),
+ * the reference's SpMV. */
+int cp_bench_mg_spmv(cp_problem* prob, int precision, int reps, double* ms, double* bytes);
 /* Achievable HBM bandwidth: average ms of `reps` launches of a plain device
  * copy of `bytes` (a multiple of 16; read once and written once per launch)
  * between two buffers this call allocates and frees, and (ms_read, may be

@@ -1,7 +1,8 @@
 """The measurement entry points on the GPU: the bandwidth yardstick
 (cp_bench_copy) reports physically possible figures and refuses bad sizes,
-and the roofline kernel's timing entry point (cp_bench_mg) returns a byte
-count within the bounds of the level-0 storage."""
+and the timing entry points of the roofline kernel (cp_bench_mg) and of the
+CG's operator product (cp_bench_mg_spmv) return byte counts of the level-0
+storage."""
 import ctypes as C
 
 import pytest
@@ -52,4 +53,28 @@ def test_smoothing_step_bytes_match_the_storage_formula(ctx):
     sym = 5 * 16.0 * 64 * ((mesh.nn + 63) // 64)
     assert sym <= b.value <= sym + 8 * 8.0 * mesh.nn + 64 * 1024
     assert b.value < 20.0 * lay["nblk_stored"]
+    P.close()
+
+
+def test_cg_operator_product_bytes_and_results(ctx):
+    """cp_bench_mg_spmv times the multigrid CG's w = mask (A u) kernel on the
+    symmetric lattice layout: its byte count is the five 16-byte blocks per
+    node (64-node slices) plus three float2 vectors, the two-byte mask and
+    the three partial sums of each 256-node block; zero launches are
+    refused."""
+    mesh = A.gen_quarter_plate(1.0, 0.1, 64, 32)
+    loads = A.Loads(mesh).force(1, 100.0, 0.0).fix_vert_axis(0.0).fix_hor_axis(0.0)
+    P = A.Problem(ctx, mesh, loads, A.material(2.1e5, 0.3))
+    P.set_grid(65, 33)
+    P.solve(A.options(cg_tol=1e-8, precision=A.MIXED, precond=2), elastic=True)
+    ms, b = C.c_double(), C.c_double()
+    A.check(A.lib().cp_bench_mg_spmv(P.ptr, A.MIXED, 10, C.byref(ms), C.byref(b)))
+    nb = (mesh.nn + 255) // 256
+    expect = 5 * 16.0 * 64 * ((mesh.nn + 63) // 64) + 3 * 8.0 * mesh.nn + 2.0 * mesh.nn + 24.0 * nb
+    print("operator product %.4f ms, %d bytes" % (ms.value, b.value), flush=True)
+    assert ms.value > 0.0
+    assert b.value == pytest.approx(expect, rel=1e-12)
+    with pytest.raises(A.CPError) as e:
+        A.check(A.lib().cp_bench_mg_spmv(P.ptr, A.MIXED, 0, C.byref(ms), C.byref(b)))
+    assert e.value.code == A.CP_ERR_INVALID
     P.close()
