@@ -48,44 +48,40 @@ __host__ __device__ inline int mg_coarse_n(int n, int co = 1, int per = 0)
     return (n <= 2 || !co) ? n : n / 2 + 1;
 }
 
-// Coarse parents of fine point f of an n-point direction: count (1 or 2),
-// coarse indices and bilinear weights.
-__host__ __device__ inline int mg_parents(int n, int co, int f, int (&K)[2], double (&w)[2], int per = 0)
+// Coarse parents of fine point f of an n-point direction and twice their
+// bilinear weights, with integer selects only (no branches, so the code
+// stays straight-line inside divergent regions).  A direction not coarsened,
+// or an open one of at most 2 points: f itself.  Otherwise an even
+// f sits on coarse point f/2; an odd f lies between f/2 and f/2 + 1 (the
+// latter wrapping to 0 where the direction is periodic), except the last
+// point of an open direction of even length, which is coarse point n/2.
+// The parents come as q0, q1 with twice-weights c0, c1: 2 and 0 where f has
+// one parent (q1 repeats it), 1 and 1 where it has two.
+__device__ __forceinline__ void mg_parents_sel(int n, int co, int f, int per, int& q0, int& q1, int& c0, int& c1)
 {
-    if (per && co) {
-        K[0] = f >> 1;
-        w[0] = 1.;
-        if ((f & 1) == 0) return 1;
-        K[1] = ((f >> 1) + 1) % (n / 2);
-        w[0] = w[1] = 0.5;
-        return 2;
-    }
-    if (n <= 2 || !co || (f & 1) == 0) {
-        K[0] = (n <= 2 || !co) ? f : (f >> 1);
-        w[0] = 1.;
-        return 1;
-    }
-    if (f == n - 1) {
-        K[0] = n / 2;
-        w[0] = 1.;
-        return 1;
-    }
-    K[0] = f >> 1;
-    K[1] = (f >> 1) + 1;
-    w[0] = w[1] = 0.5;
-    return 2;
+    const int h = f >> 1;
+    const bool odd = (f & 1) != 0, pc = per && co, same = !pc && (n <= 2 || !co);
+    const bool one = same || !odd || (!pc && f == n - 1);
+    const int k1 = same ? f : (!odd ? h : n / 2);
+    q0 = one ? k1 : h;
+    q1 = one ? k1 : ((pc && h + 1 == n / 2) ? 0 : h + 1);
+    c0 = one ? 2 : 1;
+    c1 = one ? 0 : 1;
+}
+
+// Twice the weight (0, 1 or 2) of coarse point K in the interpolation to
+// fine point f.
+__device__ __forceinline__ int mg_weight2(int n, int co, int f, int K, int per = 0)
+{
+    int q0, q1, c0, c1;
+    mg_parents_sel(n, co, f, per, q0, q1, c0, c1);
+    return (K == q0 ? c0 : 0) + (K == q1 ? c1 : 0);
 }
 
 // Weight of coarse point K in the interpolation to fine point f (0 if none).
 __device__ __forceinline__ double mg_weight(int n, int co, int f, int K, int per = 0)
 {
-    int P[2];
-    double w[2];
-    const int np = mg_parents(n, co, f, P, w, per);
-    double r = 0.;
-    for (int p = 0; p < np; ++p)
-        if (P[p] == K) r = w[p];
-    return r;
+    return 0.5 * (double)mg_weight2(n, co, f, K, per);
 }
 
 // Fine position of coarse point K.
@@ -848,36 +844,46 @@ struct MgGrid {
 // Coarse right-hand side: b_c(C) = mask_c * sum over C's fine children f of
 // w_f * res_f (fine residuals are already zero on fixed DOFs).
 // Restriction to coarse lattice point (K1, K2): the fine residuals of its
-// children weighted by their interpolation weights (the weights along
-// direction 2 formed once).
+// children weighted by their interpolation weights.  All nine positions
+// around the point are read (those off the lattice at a clamped position,
+// with weight 0), so the loads issue together instead of one behind each
+// branch; a zero-weight term adds an exact zero, so the sums are those of
+// the children alone (they start at +0, and a rounded sum is never -0).
 template <typename T>
 __device__ __forceinline__ typename V2<T>::type mg_restrict_lat(const MgGrid& gr, const typename V2<T>::type* res,
                                                                 uchar2 mc, int K1, int K2)
 {
     using V = typename V2<T>::type;
     const int p1 = mg_pos(gr.n1f, gr.co1, K1, gr.per1), p2 = mg_pos(gr.n2f, gr.co2, K2);
-    double w2[3];
+    // twice the weights along each direction (integers, so no branches)
+    int r1[3], c2[3], e1[3], e2[3];
 #pragma unroll
-    for (int a2 = 0; a2 < 3; ++a2) {
-        const int f2 = p2 + a2 - 1;
-        w2[a2] = (f2 < 0 || f2 >= gr.n2f) ? 0. : mg_weight(gr.n2f, gr.co2, f2, K2);
+    for (int a = 0; a < 3; ++a) {
+        const int f1 = mg_step(gr.n1f, gr.per1, p1, a - 1), f2 = p2 + a - 1;
+        r1[a] = f1 < 0 ? p1 : f1;
+        c2[a] = (f2 < 0 || f2 >= gr.n2f) ? p2 : f2;
+        const int g1 = mg_weight2(gr.n1f, gr.co1, r1[a], K1, gr.per1), g2 = mg_weight2(gr.n2f, gr.co2, c2[a], K2);
+        e1[a] = f1 < 0 ? 0 : g1;
+        e2[a] = c2[a] != f2 ? 0 : g2;
     }
-    double sx = 0., sy = 0.;
+    V r[3][3];
 #pragma unroll
-    for (int a1 = -1; a1 <= 1; ++a1) {
-        const int f1 = mg_step(gr.n1f, gr.per1, p1, a1);
-        if (f1 < 0) continue;
-        const double w1 = mg_weight(gr.n1f, gr.co1, f1, K1, gr.per1);
-        if (w1 == 0.) continue;
+    for (int a1 = 0; a1 < 3; ++a1)
 #pragma unroll
         for (int a2 = 0; a2 < 3; ++a2) {
-            if (w2[a2] == 0.) continue;
-            const int lf = f1 * gr.n2f + p2 + a2 - 1;
-            const V r = res[gr.node_of ? gr.node_of[lf] : lf];
-            sx += w1 * w2[a2] * (double)r.x;
-            sy += w1 * w2[a2] * (double)r.y;
+            const int lf = r1[a1] * gr.n2f + c2[a2];
+            r[a1][a2] = res[gr.node_of ? gr.node_of[lf] : lf];
         }
-    }
+    double sx = 0., sy = 0.;
+#pragma unroll
+    for (int a1 = 0; a1 < 3; ++a1)
+#pragma unroll
+        for (int a2 = 0; a2 < 3; ++a2) {
+            // the product of the two weights, exactly
+            const double w = 0.25 * (double)(e1[a1] * e2[a2]);
+            sx += w * (double)r[a1][a2].x;
+            sy += w * (double)r[a1][a2].y;
+        }
     return V{mc.x ? (T)sx : (T)0, mc.y ? (T)sy : (T)0};
 }
 
@@ -929,15 +935,25 @@ __device__ __forceinline__ void mg_prolong_lat(const MgGrid& gr, const typename 
                                                const uchar2* mask_f, int n, int f1, int f2)
 {
     using V = typename V2<T>::type;
-    int Q1[2], Q2[2];
-    double W1[2], W2[2];
-    const int nq1 = mg_parents(gr.n1f, gr.co1, f1, Q1, W1, gr.per1), nq2 = mg_parents(gr.n2f, gr.co2, f2, Q2, W2);
+    // a lone parent is read twice, the second time with weight 0 (an exact
+    // zero term), so the four loads issue together
+    int Q1[2], Q2[2], E1[2], E2[2];
+    mg_parents_sel(gr.n1f, gr.co1, f1, gr.per1, Q1[0], Q1[1], E1[0], E1[1]);
+    mg_parents_sel(gr.n2f, gr.co2, f2, 0, Q2[0], Q2[1], E2[0], E2[1]);
+    V c[2][2];
+#pragma unroll
+    for (int b1 = 0; b1 < 2; ++b1)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) c[b1][b2] = xc[Q1[b1] * gr.n2c + Q2[b2]];
     double sx = 0., sy = 0.;
-    for (int b1 = 0; b1 < nq1; ++b1)
-        for (int b2 = 0; b2 < nq2; ++b2) {
-            const V c = xc[Q1[b1] * gr.n2c + Q2[b2]];
-            sx += W1[b1] * W2[b2] * (double)c.x;
-            sy += W1[b1] * W2[b2] * (double)c.y;
+#pragma unroll
+    for (int b1 = 0; b1 < 2; ++b1)
+#pragma unroll
+        for (int b2 = 0; b2 < 2; ++b2) {
+            // the product of the two weights, exactly
+            const double w = 0.25 * (double)(E1[b1] * E2[b2]);
+            sx += w * (double)c[b1][b2].x;
+            sy += w * (double)c[b1][b2].y;
         }
     const uchar2 mk = mask_f[n];
     const V x = xin[n];
@@ -1471,8 +1487,32 @@ struct TailLevel {
     int lds;           // byte offset of the level's b, xa, xb, da, db, res in the tail's LDS
     int mat_lds;       // byte offset of the LDS copy of the rows (slot-major, k nn + n), or -1
     int minv_lds;      // byte offset of the LDS copy of the pivot inverses, or -1
-    int mask_lds;      // byte offset of the LDS copy of the mask, or -1
+    int mask_lds;      // byte offset of the LDS copy of the mask (every tail level has one)
+    int where;         // kTailGlobal, kTailMinvLds or kTailAllLds (from minv_lds, mat_lds)
 };
+
+// Where a tail level's rows and pivot inverses are read from (its mask and
+// vectors are always in LDS): both from global memory, the pivot inverses
+// from LDS and the rows from global memory, or both from LDS.  A step picks
+// the case once per level, so every load in it has one address space: LDS
+// reads through pointers into the tail's LDS arena and global reads through
+// address-space-1 pointers (a generic pointer would compile to flat loads,
+// which count against the LDS counter as well and turn every LDS wait into
+// a wait for all outstanding loads).
+constexpr int kTailGlobal = 0, kTailMinvLds = 1, kTailAllLds = 2;
+
+// (hipcc's host pass parses device code as well; the address space is
+// given in the device pass only)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CP_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define CP_GLOBAL_AS
+#endif
+template <typename P>
+__device__ __forceinline__ const CP_GLOBAL_AS P* as_global(const P* p)
+{
+    return (const CP_GLOBAL_AS P*)p;
+}
 
 // The LDS copies of the tail levels' rows, pivot inverses and masks lie in
 // one byte range [lo, lo + 16 n16) of the tail's LDS; `img` holds the same
@@ -1501,8 +1541,7 @@ template <typename T>
 __host__ __device__ constexpr int tail_minv_bytes(int nn) { return (int)(((size_t)nn * 4 * sizeof(T) + 15) / 16 * 16); }
 __host__ __device__ constexpr int tail_mask_bytes(int nn) { return (int)((2 * (size_t)nn + 15) / 16 * 16); }
 
-// Level l of the tail with its vectors, and the copied pivot inverses and
-// mask, pointed into the LDS arena.
+// Level l of the tail with its vectors pointed into the LDS arena.
 template <typename T>
 __device__ __forceinline__ TailLevel<T> tail_level(const TailLevel<T>* lv, int l, char* lds)
 {
@@ -1516,74 +1555,120 @@ __device__ __forceinline__ TailLevel<T> tail_level(const TailLevel<T>* lv, int l
     L.v.da = base + 3 * nn;
     L.v.db = base + 4 * nn;
     L.v.res = base + 5 * nn;
-    if (L.minv_lds >= 0) L.a.minv = reinterpret_cast<const typename V4<T>::type*>(lds + L.minv_lds);
-    if (L.mask_lds >= 0) L.a.mask = reinterpret_cast<const uchar2*>(lds + L.mask_lds);
     return L;
 }
 
-// Block k of node n's row: the LDS copy (slot-major; FP64 as the two
-// planes one after the other) or the lattice-addressed global rows.
-template <typename T>
-__device__ __forceinline__ void tail_block(const TailLevel<T>& L, const char* lds, int n, int k, T (&b)[4])
+// The grid of a tail level to the next one: tail levels are lattices
+// numbered point by point, so the grid's node tables are absent.
+__device__ __forceinline__ MgGrid tail_grid(const MgGrid& g)
 {
-    if (L.mat_lds >= 0) {
-        const size_t q = (size_t)k * L.a.nn + n;
-        if constexpr (std::is_same<T, float>::value) {
-            const float4 u = reinterpret_cast<const float4*>(lds + L.mat_lds)[q];
-            b[0] = u.x; b[1] = u.y; b[2] = u.z; b[3] = u.w;
-        } else {
-            const double2* pa = reinterpret_cast<const double2*>(lds + L.mat_lds);
-            const double2 u = pa[q], w = pa[q + (size_t)9 * L.a.nn];
-            b[0] = u.x; b[1] = u.y; b[2] = w.x; b[3] = w.y;
-        }
-    } else {
-        load_block(L.a, lat_base(n) + (size_t)k * kWave, b[0], b[1], b[2], b[3]);
-    }
+    MgGrid r = g;
+    r.lg = nullptr;
+    r.node_of = nullptr;
+    r.q8_par = nullptr;
+    r.q8_ch = nullptr;
+    return r;
 }
 
-// One smoothing step of node n of a tail level from its nine row blocks
-// (rowf(k, b) fills block k): the arithmetic of mg_smooth_node with lattice
-// addressing, the Chebyshev coefficients given.
-template <int MODE, typename T, typename RowF>
-__device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<T>& a, const MgVec<T>& v,
-                                                 const ChebCoef<T>& c, const typename V2<T>::type* xi,
-                                                 const typename V2<T>::type* di, typename V2<T>::type* xo,
-                                                 typename V2<T>::type* dout, int second, int last, int n)
+// Neighbour k of node n = i1 n2 + i2 in the slot order of lat_neighbour (n
+// itself where there is none), with selects only: the direction-1 step
+// wraps where the lattice is periodic in direction 1.
+__device__ __forceinline__ int tail_neighbour(int k, int i1, int i2, int n1, int n2, int n, int per1)
+{
+    const int idx = k == 0 ? 4 : (k <= 4 ? k - 1 : k);
+    const int d1 = idx / 3 - 1, d2 = idx % 3 - 1;
+    const int q1 = i1 + d1, j2 = i2 + d2;
+    const bool out1 = q1 < 0 || q1 >= n1;
+    const int j1 = q1 < 0 ? q1 + n1 : (q1 >= n1 ? q1 - n1 : q1);
+    return ((!out1 || per1) && j2 >= 0 && j2 < n2) ? j1 * n2 + j2 : n;
+}
+
+// One smoothing step of node n of a tail level whose rows and pivot
+// inverses lie where WHERE says: the arithmetic of mg_smooth_node with
+// lattice addressing, the Chebyshev coefficients given.
+template <int MODE, int WHERE, typename T>
+__device__ __forceinline__ void tail_smooth_node(const TailLevel<T>& L, const char* lds, const ChebCoef<T>& c,
+                                                 const typename V2<T>::type* xi, const typename V2<T>::type* di,
+                                                 typename V2<T>::type* xo, typename V2<T>::type* dout, int second,
+                                                 int last, int n)
 {
     using V = typename V2<T>::type;
+    using M = typename V4<T>::type;
+    const CgArgs<T>& a = L.a;
+    const MgVec<T>& v = L.v;
+    const uchar2* mask = reinterpret_cast<const uchar2*>(lds + L.mask_lds);
+    auto minv = [&](int m) -> M {
+        if constexpr (WHERE == kTailGlobal) return as_global(a.minv)[m];
+        else return reinterpret_cast<const M*>(lds + L.minv_lds)[m];
+    };
+    // block k of node n's row: the LDS copy (slot-major; FP64 as the two
+    // planes one after the other) or the lattice-addressed global rows
+    auto row = [&](int k, T(&b)[4]) {
+        if constexpr (WHERE == kTailAllLds) {
+            const size_t q = (size_t)k * a.nn + n;
+            if constexpr (std::is_same<T, float>::value) {
+                const float4 u = reinterpret_cast<const float4*>(lds + L.mat_lds)[q];
+                b[0] = u.x; b[1] = u.y; b[2] = u.z; b[3] = u.w;
+            } else {
+                const double2* pa = reinterpret_cast<const double2*>(lds + L.mat_lds);
+                const double2 u = pa[q], w = pa[q + (size_t)9 * a.nn];
+                b[0] = u.x; b[1] = u.y; b[2] = w.x; b[3] = w.y;
+            }
+        } else {
+            const size_t q = lat_base(n) + (size_t)k * kWave;
+            if constexpr (std::is_same<T, float>::value) {
+                const float4 u = as_global(reinterpret_cast<const float4*>(a.va))[q];
+                b[0] = u.x; b[1] = u.y; b[2] = u.z; b[3] = u.w;
+            } else {
+                const double2 u = as_global(reinterpret_cast<const double2*>(a.va))[q];
+                const double2 w = as_global(reinterpret_cast<const double2*>(a.vb))[q];
+                b[0] = u.x; b[1] = u.y; b[2] = w.x; b[3] = w.y;
+            }
+        }
+    };
     const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
     const int i1 = small_div(n, a.lat_n2), i2 = n - i1 * a.lat_n2;
     T yx = 0, yy = 0;
-    // FP64 blocks three at a time: nine in flight would not fit the 128
-    // registers of a 1024-thread workgroup
-    constexpr int kUnroll = std::is_same<T, double>::value ? 3 : 9;
+    // FP32: all nine blocks and neighbour values loaded before the first
+    // use; FP64: one block at a time, three unrolled (nine in flight would
+    // not fit the 128 registers of a 1024-thread workgroup); the sums run
+    // over k = 0..8 in order either way
+    constexpr bool kF64 = std::is_same<T, double>::value;
+    constexpr int G = kF64 ? 1 : 9, kUnroll = kF64 ? 3 : 1;
 #pragma unroll kUnroll
-    for (int k = 0; k < 9; ++k) {
-        T m[4];
-        rowf(k, m);
-        const int nb = lat_neighbour(k, i1, i2, a.lat_n1, a.lat_n2, n, a.lat_per1);
-        T zx, zy;
-        if constexpr (MODE == MG_PRE1) {
-            const V bm = v.b[nb];
-            apply_minv<T>(a.minv[nb], bm.x, bm.y, zx, zy);
-            zx *= c.c0;
-            zy *= c.c0;
-        } else if constexpr (MODE == MG_POST1) {
-            const V xm = xi[nb];
-            zx = xm.x;
-            zy = xm.y;
-        } else {
-            const V xm = xi[nb], dm = di[nb];
-            zx = xm.x + dm.x;
-            zy = xm.y + dm.y;
+    for (int k0 = 0; k0 < 9; k0 += G) {
+        T m[G][4];
+        T zx[G], zy[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) row(k0 + g, m[g]);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int nb = tail_neighbour(k0 + g, i1, i2, a.lat_n1, a.lat_n2, n, a.lat_per1);
+            if constexpr (MODE == MG_PRE1) {
+                const V bm = v.b[nb];
+                apply_minv<T>(minv(nb), bm.x, bm.y, zx[g], zy[g]);
+                zx[g] *= c.c0;
+                zy[g] *= c.c0;
+            } else if constexpr (MODE == MG_POST1) {
+                const V xm = xi[nb];
+                zx[g] = xm.x;
+                zy[g] = xm.y;
+            } else {
+                const V xm = xi[nb], dm = di[nb];
+                zx[g] = xm.x + dm.x;
+                zy[g] = xm.y + dm.y;
+            }
         }
-        yx += m[0] * zx + m[1] * zy;
-        yy += m[2] * zx + m[3] * zy;
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            yx += m[g][0] * zx[g] + m[g][1] * zy[g];
+            yy += m[g][2] * zx[g] + m[g][3] * zy[g];
+        }
     }
-    const uchar2 mk = a.mask[n];
+    const uchar2 mk = mask[n];
     const V bn = v.b[n];
     const T rx = mk.x ? bn.x - yx : (T)0, ry = mk.y ? bn.y - yy : (T)0;
-    const typename V4<T>::type mi = a.minv[n];
+    const M mi = minv(n);
     T gx, gy;
     apply_minv<T>(mi, rx, ry, gx, gy);
     if constexpr (MODE == MG_PRE1) {
@@ -1615,8 +1700,8 @@ __device__ __forceinline__ void tail_smooth_node(const RowF& rowf, const CgArgs<
 // Every level from the first one of at most kTailThreads nodes down to the
 // coarsest and back, in one workgroup: the same per-node work as the level
 // kernels, with workgroup barriers instead of launches between the steps.
-// Everything a step reads is on the chip: the vectors of every level in
-// LDS; each level's rows, pivot inverses and mask in LDS where the host
+// Everything a step reads is on the chip: the vectors and mask of every
+// level in LDS; each level's rows and pivot inverses in LDS where the host
 // found room (coarsest first; one coalesced copy of the tail image), the
 // others read from L2 (rows kept in registers instead spilled to scratch at
 // the 128 registers a 1024-thread workgroup leaves each lane); each level's
@@ -1644,13 +1729,13 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
             // store and the batch's loads are all in flight at once
             float4 t[kBatch];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) t[u] = im.img[min(i0 + u * kTailThreads, im.n16 - 1)];
+            for (int u = 0; u < kBatch; ++u) t[u] = as_global(im.img)[min(i0 + u * kTailThreads, im.n16 - 1)];
 #pragma unroll
             for (int u = 0; u < kBatch; ++u) dst[min(i0 + u * kTailThreads, im.n16 - 1)] = t[u];
         }
         const TailLevel<T> L0 = lv[0];
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
-        for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = L0.v.b[n];
+        for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = as_global(L0.v.b)[n];
         if (tid < nl) cf[tid] = mg_cheb_of<T>(lv[tid].v.lam);
         __syncthreads();
     }
@@ -1659,8 +1744,12 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         constexpr int MODE = decltype(mode)::value;
         if (tid >= L.a.nn) return;
         const ChebCoef<T> c = cf[l];
-        tail_smooth_node<MODE, T>([&](int k, T(&m)[4]) { tail_block<T>(L, lds, tid, k, m); }, L.a, L.v, c, xi, di_,
-                                  xo, dout, second, last, tid);
+        if (L.where == kTailAllLds)
+            tail_smooth_node<MODE, kTailAllLds, T>(L, lds, c, xi, di_, xo, dout, second, last, tid);
+        else if (L.where == kTailMinvLds)
+            tail_smooth_node<MODE, kTailMinvLds, T>(L, lds, c, xi, di_, xo, dout, second, last, tid);
+        else
+            tail_smooth_node<MODE, kTailGlobal, T>(L, lds, c, xi, di_, xo, dout, second, last, tid);
     };
     using Pre1 = std::integral_constant<int, MG_PRE1>;
     using Cheb = std::integral_constant<int, MG_CHEB>;
@@ -1681,7 +1770,8 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         if (tid < Cn.a.nn) {
             const int K1 = small_div(tid, L.grid.n2c);
-            const_cast<V*>(Cn.v.b)[tid] = mg_restrict_lat<T>(L.grid, L.v.res, Cn.a.mask[tid], K1, tid - K1 * L.grid.n2c);
+            const uchar2 mc = reinterpret_cast<const uchar2*>(lds + Cn.mask_lds)[tid];
+            const_cast<V*>(Cn.v.b)[tid] = mg_restrict_lat<T>(tail_grid(L.grid), L.v.res, mc, K1, tid - K1 * L.grid.n2c);
         }
         __syncthreads();
     };
@@ -1700,7 +1790,7 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
 #pragma unroll
         for (int m = 0; m < kDenseCols; ++m) {
             const int j = dj0 + 8 * m;
-            dv[m] = (di < nd && j < nd) ? Dinv[(size_t)di * nd + j] : 0.;
+            dv[m] = (di < nd && j < nd) ? as_global(Dinv)[(size_t)di * nd + j] : 0.;
         }
     }
     if (nl >= 2) restrict_down(nl - 2);
@@ -1722,7 +1812,7 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
                     if (j < nd) s += dv[m] * bs[j];
                 }
             } else {
-                for (int j = dj0; j < nd; j += 8) s += Dinv[(size_t)di * nd + j] * bs[j];
+                for (int j = dj0; j < nd; j += 8) s += as_global(Dinv)[(size_t)di * nd + j] * bs[j];
             }
         }
         s += __shfl_xor(s, 1);
@@ -1739,7 +1829,8 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         const TailLevel<T> Cn = tail_level<T>(lv, l + 1, lds);
         if (tid < L.a.nn) {
             const int f1 = small_div(tid, L.grid.n2f);
-            mg_prolong_lat<T>(L.grid, Cn.v.xb, L.v.xa, L.v.xb, L.a.mask, tid, f1, tid - f1 * L.grid.n2f);
+            mg_prolong_lat<T>(tail_grid(L.grid), Cn.v.xb, L.v.xa, L.v.xb,
+                              reinterpret_cast<const uchar2*>(lds + L.mask_lds), tid, f1, tid - f1 * L.grid.n2f);
         }
         __syncthreads();
         step(L, l, Post1(), L.v.xb, nullptr, nullptr, L.v.da, 0, 0);

@@ -179,17 +179,17 @@ static int mg_tail_table(cp_problem* P)
         tab.push_back(t);
     }
     if (lds > cpk::kTailVecLds) return fail(CP_ERR_INVALID, "multigrid: coarse tail exceeds its LDS");
-    // the rest of the LDS (the tail image): copies of the masks, then of the
-    // pivot inverses, then of the rows, coarsest level first; the coarsest
-    // level is solved with its inverse and needs only its mask
+    // the rest of the LDS (the tail image): copies of every level's mask
+    // (at most two bytes per node of the vectors' 96), then of the pivot
+    // inverses, then of the rows, coarsest level first, where they fit; the
+    // coarsest level is solved with its inverse and needs only its mask
     const int nt = (int)tab.size();
     const int budget = cpk::tail_lds_budget<T>(), img_lo = lds;
     for (int i = nt - 1; i >= 0; --i) {
-        const int need = cpk::tail_mask_bytes(tab[i].a.nn);
-        if (lds + need > budget) break;
         tab[i].mask_lds = lds;
-        lds += need;
+        lds += cpk::tail_mask_bytes(tab[i].a.nn);
     }
+    if (lds > budget) return fail(CP_ERR_INVALID, "multigrid: coarse tail masks exceed its LDS");
     for (int i = nt - 2; i >= 0; --i) {
         const int need = cpk::tail_minv_bytes<T>(tab[i].a.nn);
         if (lds + need > budget) break;
@@ -202,6 +202,8 @@ static int mg_tail_table(cp_problem* P)
         tab[i].mat_lds = lds;
         lds += need;
     }
+    for (auto& t : tab)
+        t.where = t.minv_lds < 0 ? cpk::kTailGlobal : (t.mat_lds < 0 ? cpk::kTailMinvLds : cpk::kTailAllLds);
     if (nt > cpk::kTailMaxLevels) return fail(CP_ERR_INVALID, "multigrid: too many coarse tail levels");
     cpk::TailLevel<T>* d = nullptr;
     if (int s = G.get(&d, tab.size())) return s;
