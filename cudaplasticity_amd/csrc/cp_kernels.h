@@ -255,11 +255,56 @@ struct AsmArgs {
     double h;
 };
 
-// Element stiffness (formKe4, PlasticitySolver.cu:156-186) gathered per
-// node: thread n computes, for every element e containing n (local index i),
-// the 2x2 blocks K_e(i, j), j = 0..3, and adds them to the slots of its
-// row; slot order and element order are fixed, so the sums are
-// deterministic.  W = compile-time bound on the row width.
+// Blocks K_e(i, j), j = 0..3, of element e's stiffness (formKe4,
+// PlasticitySolver.cu:156-186) for its local node i, before the thickness
+// factor: the 2 x 2 Gauss rule over B_i^T C B_j detJ.
+__device__ __forceinline__ void element_row_blocks(const AsmArgs& a, const int4& c, const double2& en, int i,
+                                                   double (&blk)[4][4])
+{
+    double x[4], y[4];
+    {
+        const double2 p0 = a.xy[c.x], p1 = a.xy[c.y], p2 = a.xy[c.z], p3 = a.xy[c.w];
+        x[0] = p0.x; y[0] = p0.y; x[1] = p1.x; y[1] = p1.y;
+        x[2] = p2.x; y[2] = p2.y; x[3] = p3.x; y[3] = p3.y;
+    }
+    double c00, c01, c22;
+    elastic_c(a.plane, en.x, en.y, c00, c01, c22);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) blk[j][0] = blk[j][1] = blk[j][2] = blk[j][3] = 0.;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        double B0[4], B1[4], dJ;
+        shape_grad(x, y, g, B0, B1, dJ);
+        const double bi0 = pick4(B0, i), bi1 = pick4(B1, i);
+        // BC = B_i^T C detJ (rows 0 and 1 of B^T C for node i)
+        const double t00 = bi0 * c00 * dJ, t01 = bi0 * c01 * dJ, t02 = bi1 * c22 * dJ;
+        const double t10 = bi1 * c01 * dJ, t11 = bi1 * c00 * dJ, t12 = bi0 * c22 * dJ;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            blk[j][0] += t00 * B0[j] + t02 * B1[j];
+            blk[j][1] += t01 * B1[j] + t02 * B0[j];
+            blk[j][2] += t10 * B0[j] + t12 * B1[j];
+            blk[j][3] += t11 * B1[j] + t12 * B0[j];
+        }
+    }
+}
+
+// Local index (0..3) of the row's own node among an incidence's packed row
+// slots: the one at row position 0 (the row lists the node itself first).
+__device__ __forceinline__ void incidence_slots(unsigned slots, int (&sl)[4], int& i)
+{
+    sl[0] = (int)(slots & 255u);
+    sl[1] = (int)((slots >> 8) & 255u);
+    sl[2] = (int)((slots >> 16) & 255u);
+    sl[3] = (int)(slots >> 24);
+    i = sl[0] == 0 ? 0 : (sl[1] == 0 ? 1 : (sl[2] == 0 ? 2 : 3));
+}
+
+// Element stiffness gathered per node: thread n computes, for every element
+// e containing n (local index i), the 2x2 blocks K_e(i, j), j = 0..3, and
+// adds them to the slots of its row; slot order and element order are
+// fixed, so the sums are deterministic.  W = compile-time bound on the row
+// width.
 template <int W, bool F64>
 __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 {
@@ -271,39 +316,10 @@ __global__ __launch_bounds__(kBlock) void k_assemble(AsmArgs a)
 
     for (int q = a.incPtr[n]; q < a.incPtr[n + 1]; ++q) {
         const int2 ic = a.inc[q];
-        const int4 c = a.conn[ic.x];
-        const double2 en = a.ecnu[ic.x];
-        const unsigned slots = (unsigned)ic.y;
-        const int sl[4] = {(int)(slots & 255u), (int)((slots >> 8) & 255u), (int)((slots >> 16) & 255u),
-                           (int)(slots >> 24)};
-        const int i = sl[0] == 0 ? 0 : (sl[1] == 0 ? 1 : (sl[2] == 0 ? 2 : 3));
-        double x[4], y[4];
-        {
-            const double2 p0 = a.xy[c.x], p1 = a.xy[c.y], p2 = a.xy[c.z], p3 = a.xy[c.w];
-            x[0] = p0.x; y[0] = p0.y; x[1] = p1.x; y[1] = p1.y;
-            x[2] = p2.x; y[2] = p2.y; x[3] = p3.x; y[3] = p3.y;
-        }
-        double c00, c01, c22;
-        elastic_c(a.plane, en.x, en.y, c00, c01, c22);
+        int sl[4], i;
+        incidence_slots((unsigned)ic.y, sl, i);
         double blk[4][4];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) blk[j][0] = blk[j][1] = blk[j][2] = blk[j][3] = 0.;
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            double B0[4], B1[4], dJ;
-            shape_grad(x, y, g, B0, B1, dJ);
-            const double bi0 = pick4(B0, i), bi1 = pick4(B1, i);
-            // BC = B_i^T C detJ (rows 0 and 1 of B^T C for node i)
-            const double t00 = bi0 * c00 * dJ, t01 = bi0 * c01 * dJ, t02 = bi1 * c22 * dJ;
-            const double t10 = bi1 * c01 * dJ, t11 = bi1 * c00 * dJ, t12 = bi0 * c22 * dJ;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                blk[j][0] += t00 * B0[j] + t02 * B1[j];
-                blk[j][1] += t01 * B1[j] + t02 * B0[j];
-                blk[j][2] += t10 * B0[j] + t12 * B1[j];
-                blk[j][3] += t11 * B1[j] + t12 * B0[j];
-            }
-        }
+        element_row_blocks(a, a.conn[ic.x], a.ecnu[ic.x], i, blk);
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
 #pragma unroll

@@ -721,6 +721,60 @@ __global__ __launch_bounds__(kBlock) void k_mg_sym_from_lat(const float4* lat, i
     }
 }
 
+// Level 0's operator assembled straight into the symmetric lattice layout,
+// for the set-ups that keep the coarse levels (no Galerkin product needs
+// the full rows): the bits k_assemble<W, false> followed by k_mg_sym_build
+// store, on a lattice numbered node by node with n2 >= 3.  Element node m
+// of an incidence of node n lies at offset m - n: 0 (slot 0), 1, n2 - 1, n2
+// or n2 + 1 (the forward slots 1..4), or below 0 (a backward coupling, which
+// that neighbour stores).  Each slot sums the same incidences in the same
+// order as k_assemble; the diagonal block also goes, unsymmetrised, to
+// position 0 of the node's FP32 row, where the segment set-up reads it.
+// Registers held to three waves per SIMD (168; an 8-byte spill), which
+// measured faster than the two waves the unconstrained 170 allow.
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_assemble_sym(AsmArgs a, int n2,
+                                                                                               float4* vs)
+{
+    const int n = xcd_block(a.nblocks) * kBlock + threadIdx.x;
+    if (n >= a.nn) return;
+    double acc[kSymSlots][4];
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) acc[k][0] = acc[k][1] = acc[k][2] = acc[k][3] = 0.;
+    for (int q = a.incPtr[n]; q < a.incPtr[n + 1]; ++q) {
+        const int2 ic = a.inc[q];
+        int sl[4], i;
+        incidence_slots((unsigned)ic.y, sl, i);
+        const int4 c = a.conn[ic.x];
+        double blk[4][4];
+        element_row_blocks(a, c, a.ecnu[ic.x], i, blk);
+        const int cn[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int d = cn[j] - n;
+            const int k = d == 0 ? 0 : (d == 1 ? 1 : (d == n2 - 1 ? 2 : (d == n2 ? 3 : (d == n2 + 1 ? 4 : -1))));
+#pragma unroll
+            for (int s = 0; s < kSymSlots; ++s) {
+                if (s == k) {
+                    acc[s][0] += blk[j][0] * a.h;
+                    acc[s][1] += blk[j][1] * a.h;
+                    acc[s][2] += blk[j][2] * a.h;
+                    acc[s][3] += blk[j][3] * a.h;
+                }
+            }
+        }
+    }
+    const size_t base = sym_base(n);
+#pragma unroll
+    for (int k = 0; k < kSymSlots; ++k) {
+        float4 b = make_float4((float)acc[k][0], (float)acc[k][1], (float)acc[k][2], (float)acc[k][3]);
+        if (k == 0) {
+            a.vf[(size_t)a.slice[n >> 6].x + (n & 63)] = b;
+            b.y = b.z = 0.5f * (b.y + b.z);
+        }
+        vs[base + (size_t)k * kWave] = b;
+    }
+}
+
 // The couplings of level-0 node n to its previous (lo) and next (up)
 // neighbour along direction 1 in the symmetric lattice layout: the next
 // one is n's slot 3, the previous one the transpose of slot 3 of n - n2.

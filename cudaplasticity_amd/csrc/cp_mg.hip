@@ -62,6 +62,9 @@ struct MgState {
     int rebuild = 8;
     int since[2] = {0, 0};
     bool built[2] = {false, false};
+    // level 0's symmetric layout written by the assembly itself (the next
+    // set-up skips k_mg_sym_build)
+    bool sym_assembled = false;
     void* tail[2] = {nullptr, nullptr};
     // the tail image (cp_mg.h, TailImage) per precision
     cpk::TailImage tail_img[2] = {};
@@ -300,9 +303,10 @@ static int mg_setup_t(cp_problem* P, int nlev)
     const int pi = std::is_same<T, double>::value ? 0 : 1;
     hipStream_t st = P->stream;
     if constexpr (std::is_same<T, float>::value)
-        if (G.vsym32)
+        if (G.vsym32 && !G.sym_assembled)
             hipLaunchKernelGGL(cpk::k_mg_sym_build, dim3(P->nbn), dim3(cpk::kBlock), 0, st, P->cg32_args(),
                                (const signed char*)G.slot0, G.vsym32);
+    G.sym_assembled = false;
     for (int l = 0; l < nlev; ++l) {
         const MgLevel& L = G.L[l];
         cpk::CgArgs<T> a = level_args<T>(P, l);
@@ -367,6 +371,25 @@ static int mg_setup_t(cp_problem* P, int nlev)
     return CP_OK;
 }
 
+// Whether the coming set-up rebuilds the coarse levels (mg_setup).
+static bool mg_rebuild_due(const MgState& G, int pi, int full)
+{
+    return full || !G.built[pi] || G.since[pi] + 1 >= G.rebuild;
+}
+
+int mg_assemble_direct(cp_problem* P, int precision, int full, const cpk::AsmArgs& a, int* done)
+{
+    *done = 0;
+    if (!P->mg || precision == CP_FP64 || P->precond != 2) return CP_OK;
+    MgState& G = *P->mg;
+    if (!G.sym_ok || !G.vsym32 || !G.L[0].seg || G.L[0].n2 < 3 || mg_rebuild_due(G, 1, full)) return CP_OK;
+    hipLaunchKernelGGL(cpk::k_assemble_sym, dim3(P->nbn), dim3(cpk::kBlock), 0, P->stream, a, G.L[0].n2, G.vsym32);
+    CP_HIP(hipGetLastError());
+    G.sym_assembled = true;
+    *done = 1;
+    return CP_OK;
+}
+
 int mg_setup(cp_problem* P, int precision, int full)
 {
     if (!P->mg) return fail(CP_ERR_STATE, "multigrid needs the mesh lattice: call cp_problem_set_grid first");
@@ -377,7 +400,7 @@ int mg_setup(cp_problem* P, int precision, int full)
     // a reused coarse hierarchy stays a symmetric positive definite
     // preconditioner (each level's smoother is set up from that level's own
     // operator); only its quality depends on how far the moduli have moved
-    const bool all = full || !G.built[pi] || G.since[pi] + 1 >= G.rebuild;
+    const bool all = mg_rebuild_due(G, pi, full);
     if (all) {
         G.built[pi] = true;
         G.since[pi] = 0;

@@ -162,6 +162,12 @@ void launch_cg_iters(int max_w, int pre, const cpk::CgArgs<T>& a, int count, dim
 // full is set, on the first set-up of a precision, or every
 // MgState::rebuild-th set-up, else the previous ones are kept.
 int mg_setup(cp_problem* P, int precision, int full);
+// The FP32 assembly of a set-up that keeps the coarse levels, straight into
+// level 0's symmetric lattice layout (k_assemble_sym) where that layout
+// exists and level 0 smooths by segments: *done = 1 when launched (the
+// set-up then skips its own conversion to that layout), else 0 and the
+// caller assembles the full rows.
+int mg_assemble_direct(cp_problem* P, int precision, int full, const cpk::AsmArgs& a, int* done);
 void mg_destroy(cp_problem* P);
 // zero_start: x_0 = 0, so (r_0, r_0) = bnorm2 (the refinement passes).
 template <typename T>

@@ -495,7 +495,10 @@ int launch_assemble(cp_problem* P, int precision, int mg_full)
     a.h = P->M.h;
     const dim3 g(P->nbn), b(cpk::kBlock);
     const bool f64 = precision == CP_FP64;
-    switch (max_width_instantiated(P->max_w)) {
+    int direct = 0;
+    if (P->precond == 2)
+        if (int s = mg_assemble_direct(P, precision, mg_full, a, &direct)) return s;
+    if (!direct) switch (max_width_instantiated(P->max_w)) {
 #define CP_ASM_CASE(W)                                                                                  \
     case W:                                                                                             \
         if (f64) hipLaunchKernelGGL((cpk::k_assemble<W, true>), g, b, 0, P->stream, a);                 \
