@@ -1175,10 +1175,12 @@ __device__ __forceinline__ int seg_pos(const SegGrid& g, int n, int d1, int d2)
     return g.slot_pos ? g.slot_pos[(size_t)n * 9 + so] : so;
 }
 
+// Block of node n's row coupling it to its neighbour at (d1, 0) along
+// direction 1; the node's own block (d1 = 0) is at row position 0.
 template <typename T>
 __device__ __forceinline__ void seg_block(const CgArgs<T>& a, const SegGrid& g, int n, int d1, double (&m)[4])
 {
-    const int pos = seg_pos(g, n, d1, 0);
+    const int pos = d1 == 0 ? 0 : seg_pos(g, n, d1, 0);
     if (pos < 0) {   // not coupled in the matrix
         m[0] = m[1] = m[2] = m[3] = 0.;
         return;
@@ -1204,8 +1206,11 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
     double p00 = 0., p01 = 0., p11 = 0.;   // Dt_{k-1}^-1 (zero on fixed DOFs)
     uchar2 mp = make_uchar2(0, 0);
     // rows in chunks of kChunk: the loads of a chunk are issued before its
-    // (dependent) eliminations and stores
+    // (dependent) eliminations and stores, all of them unconditionally (rows
+    // past the lattice's end read its last row, and are not used), so that
+    // no branch holds a load back
     constexpr int kChunk = 4;
+#pragma unroll
     for (int k0 = 0; k0 < kSeg; k0 += kChunk) {
         double d[kChunk][4], l[kChunk][4];
         uchar2 mk[kChunk];
@@ -1213,19 +1218,19 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 #pragma unroll
         for (int c = 0; c < kChunk; ++c) {
             const int i1 = s * kSeg + k0 + c;
-            nodes[c] = i1 < g.n1 ? seg_node(g, i1, i2) : -1;
-            if (nodes[c] < 0) continue;
-            mk[c] = a.mask[nodes[c]];
-            seg_block(a, g, nodes[c], 0, d[c]);
+            const int nc = seg_node(g, min(i1, g.n1 - 1), i2);
+            nodes[c] = i1 < g.n1 ? nc : -1;
+            mk[c] = a.mask[nc];
+            seg_block(a, g, nc, 0, d[c]);
             l[c][0] = l[c][1] = l[c][2] = l[c][3] = 0.;
             if (k0 + c > 0) {
                 if (a.vsym) {
                     // the coupling the smoothing steps use: the symmetric layout's
                     T bl[4], bu[4];
-                    sym_seg_couplings<T>(a, nodes[c], true, false, bl, bu);
+                    sym_seg_couplings<T>(a, nc, true, false, bl, bu);
                     for (int j = 0; j < 4; ++j) l[c][j] = bl[j];
                 } else {
-                    seg_block(a, g, nodes[c], -1, l[c]);
+                    seg_block(a, g, nc, -1, l[c]);
                 }
             }
         }
