@@ -48,11 +48,13 @@ NSTEPS = 20
 # default) 44.93 MB read + 3.21 MB written; on the 2000x1000 plate
 # (HBM-resident) the same step 475.8 MB read + 33.0 MB written in the
 # sliced-ELL rows, 290.3 MB read + 32.1 MB written in the symmetric lattice
-# layout; the multigrid CG's operator product on that plate (FP32, symmetric
-# lattice layout) 241.3 MB read + 16.2 MB written.
+# layout; the step that reads the combined iterate x + d (the one the
+# V-cycle runs) 300.8 MB read + 32.4 MB written in that layout; the
+# multigrid CG's operator product on that plate (FP32, symmetric lattice
+# layout) 241.3 MB read + 16.2 MB written.
 MEASURED_TRAFFIC = {(632, 316, "cg"): 56.24e6, (632, 316, "mg_smooth_seg"): 48.14e6,
                     (2000, 1000, "mg_smooth_seg"): 508.8e6, (2000, 1000, "mg_smooth_seg_sym"): 322.4e6,
-                    (2000, 1000, "mg_cg_dots_sym"): 257.5e6}
+                    (2000, 1000, "mg_cheb_x_seg_sym"): 333.2e6, (2000, 1000, "mg_cg_dots_sym"): 257.5e6}
 
 
 def log(*a):
@@ -130,10 +132,12 @@ def cheb_step_bytes(nnz, N, s):
 
 def smooth_kernel(P):
     """Name and traffic key of the finest level's FP32 Chebyshev step kernel
-    (row template -2: the symmetric lattice layout, 9: sliced-ELL rows)."""
+    (row template -2: the symmetric lattice layout, 9: sliced-ELL rows; step
+    mode 9 on a segment level: the step that reads the combined iterate
+    x + d, 1 on a point-smoother level)."""
     w, tag = (-2, "_sym") if P.mg_level0_sym() else (9, "")
     if P.mg_smoothers()[0] == 1:
-        return "cpk::k_mg_smooth_seg<%d,1,float>" % w, "mg_smooth_seg" + tag
+        return "cpk::k_mg_smooth_seg<%d,9,float>" % w, "mg_cheb_x_seg" + tag
     return "cpk::k_mg_smooth<%d,1,float>" % w, "mg_smooth" + tag
 
 

@@ -576,6 +576,12 @@ struct MgVec {
 };
 
 enum { MG_PRE1 = 0, MG_CHEB = 1, MG_RESID = 2, MG_POST1 = 3 };
+// Steps whose input iterate comes combined (the previous step wrote x + d
+// as well as d, so a neighbour's iterate is one load, not two): the
+// residual step MG_RESID_X, and the segment steps MG_POST1_X, MG_CHEB_X,
+// SEG_FIRST_X.  The additions are the same ones, made once by the writer
+// instead of by every reader, so the results are bitwise the same.
+enum { MG_RESID_X = 7, MG_POST1_X = 8, MG_CHEB_X = 9, SEG_FIRST_X = 10 };
 
 // Coarse levels store lattice node n's row in slice n >> 6 (9 slots of 64
 // lanes each) with slot k holding the neighbour at a fixed lattice offset
@@ -813,6 +819,7 @@ __device__ __forceinline__ void row_apply(const CgArgs<T>& a, int n, const ZF& z
 //   MG_CHEB  x = xi + di                       -> xo = x (or x + d if last), do = a di + c D^-1 (b - A x)
 //   MG_RESID x = xi + di                       -> xo = x, res = b - A x
 //   MG_POST1 x = xi                            -> do = c0 D^-1 (b - A x)
+//   MG_RESID_X x = xi (the previous step wrote x + d) -> res = b - A x
 template <int W, int MODE, typename T>
 __device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T>& v, const typename V2<T>::type* xi,
                                                const typename V2<T>::type* di, typename V2<T>::type* xo,
@@ -829,7 +836,7 @@ __device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T
                               apply_minv<T>(a.minv[m], bm.x, bm.y, zx, zy);
                               zx *= c.c0;
                               zy *= c.c0;
-                          } else if constexpr (MODE == MG_POST1) {
+                          } else if constexpr (MODE == MG_POST1 || MODE == MG_RESID_X) {
                               const V xm = xi[m];
                               zx = xm.x;
                               zy = xm.y;
@@ -843,6 +850,10 @@ __device__ __forceinline__ void mg_smooth_node(const CgArgs<T>& a, const MgVec<T
     const uchar2 mk = a.mask[n];
     const V bn = v.b[n];
     const T rx = mk.x ? bn.x - yx : (T)0, ry = mk.y ? bn.y - yy : (T)0;
+    if constexpr (MODE == MG_RESID_X) {
+        v.res[n] = V{rx, ry};
+        return;
+    }
     const typename V4<T>::type mi = a.minv[n];
     T gx, gy;
     apply_minv<T>(mi, rx, ry, gx, gy);
@@ -1282,7 +1293,9 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 //   SEG_FIRST x = xi                              -> do = a1 x + b1 D^-1 (b - A x)
 //   MG_CHEB   x = xi + di                         -> xo = x (or x + d if last), do = a di + c D^-1 (b - A x)
 //   MG_POST1  x = xi                              -> do = c0 D^-1 (b - A x)
-// (the residual step MG_RESID needs no D^-1 and runs in k_mg_smooth).
+//   SEG_FIRST_X, MG_POST1_X: as SEG_FIRST, MG_POST1, and also xo = x + do
+//   MG_CHEB_X x = xi (combined), d = di at the node -> do = a d + c D^-1 (b - A x), xo = x + do
+// (the residual steps MG_RESID, MG_RESID_X need no D^-1 and run in k_mg_smooth).
 // One workgroup per tile of kSeg x kSegCols lattice nodes: every thread
 // forms its node's residual and the per-node factors, then one thread per
 // tile column runs the two short recurrences of its segment in LDS.
@@ -1409,7 +1422,7 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
             rx = mk.x ? bn.x - yx : (T)0;
             ry = mk.y ? bn.y - yy : (T)0;
             xn = xi[n];
-            if constexpr (MODE == MG_CHEB) dn = di[n];
+            if constexpr (MODE == MG_CHEB || MODE == MG_CHEB_X) dn = di[n];
         }
         apply_minv<T>(mi, rx, ry, ax, ay);
         // Dt^-1 times the couplings to the segment neighbours.  Rows of
@@ -1462,10 +1475,19 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
     const T gx = sy[0][q1], gy = sy[1][q1];
     if constexpr (MODE == SEG_X0 || MODE == SEG_RX0) {
         xo[n] = V{c.c0 * gx, c.c0 * gy};
-    } else if constexpr (MODE == SEG_FIRST) {
-        dout[n] = V{c.a1 * xn.x + c.b1 * gx, c.a1 * xn.y + c.b1 * gy};
-    } else if constexpr (MODE == MG_POST1) {
-        dout[n] = V{c.c0 * gx, c.c0 * gy};
+    } else if constexpr (MODE == SEG_FIRST || MODE == SEG_FIRST_X) {
+        const V d = V{c.a1 * xn.x + c.b1 * gx, c.a1 * xn.y + c.b1 * gy};
+        dout[n] = d;
+        if constexpr (MODE == SEG_FIRST_X) xo[n] = V{xn.x + d.x, xn.y + d.y};
+    } else if constexpr (MODE == MG_POST1 || MODE == MG_POST1_X) {
+        const V d = V{c.c0 * gx, c.c0 * gy};
+        dout[n] = d;
+        if constexpr (MODE == MG_POST1_X) xo[n] = V{xn.x + d.x, xn.y + d.y};
+    } else if constexpr (MODE == MG_CHEB_X) {
+        const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
+        const V d = V{ca * dn.x + cb * gx, ca * dn.y + cb * gy};
+        xo[n] = V{xn.x + d.x, xn.y + d.y};
+        if (!last) dout[n] = d;
     } else {
         const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
         const V x = V{xn.x + dn.x, xn.y + dn.y};

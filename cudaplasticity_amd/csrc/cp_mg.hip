@@ -537,10 +537,14 @@ static void smooth_seg(E& e, unsigned long long key, const cpk::CgArgs<T>& a, co
 // The same steps with the segment smoother: x1 = c0 D^-1 b takes a launch
 // of its own (D^-1 b at a neighbour is not local any more); on coarse
 // levels that launch also restricts the finer level's residual into b
-// (g.rres set).
+// (g.rres set).  Each step after the first writes the next iterate x + d
+// beside d (the _X modes), so the step after it reads one vector at the
+// neighbours, and the residual step writes the residual alone.
 // Degree 2 stops the same recurrence one step earlier (the step-k
 // coefficients do not depend on the degree): the pre-smoothed x then lands
 // in xb, the prolongation writes xa, and the post-smoothing ends in xb.
+// Buffers, degree 3: down x1 -> xa, x2 -> xb, x3 -> xa (d in da, db);
+// up (from xb) x' -> xa, x'' -> res (consumed by then), x -> xb.
 template <int W, typename T, typename E>
 static void smooth_level_down_seg(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v,
                                   const cpk::SegGrid& g, int deg)
@@ -548,13 +552,13 @@ static void smooth_level_down_seg(E& e, int l, const cpk::CgArgs<T>& a, const cp
     const auto k = [&](int s) { return step_key(SECT_DOWN, l, s); };
     if (g.rres) smooth_seg<W, cpk::SEG_RX0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
     else smooth_seg<W, cpk::SEG_X0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
-    smooth_seg<W, cpk::SEG_FIRST, T>(e, k(1), a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g);
+    smooth_seg<W, cpk::SEG_FIRST_X, T>(e, k(1), a, v, v.xa, nullptr, v.xb, v.da, 0, 0, g);
     if (deg == 2) {
-        smooth<W, cpk::MG_RESID, T>(e, k(2), a, v, v.xa, v.da, v.xb, nullptr, 0, 0);
+        smooth<W, cpk::MG_RESID_X, T>(e, k(2), a, v, v.xb, nullptr, nullptr, nullptr, 0, 0);
         return;
     }
-    smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xa, v.da, v.xb, v.db, 1, 0, g);
-    smooth<W, cpk::MG_RESID, T>(e, k(3), a, v, v.xb, v.db, v.xa, nullptr, 0, 0);
+    smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(2), a, v, v.xb, v.da, v.xa, v.db, 1, 0, g);
+    smooth<W, cpk::MG_RESID_X, T>(e, k(3), a, v, v.xa, nullptr, nullptr, nullptr, 0, 0);
 }
 
 template <int W, typename T, typename E>
@@ -563,13 +567,13 @@ static void smooth_level_up_seg(E& e, int l, const cpk::CgArgs<T>& a, const cpk:
 {
     const auto k = [&](int s) { return step_key(SECT_UP, l, s); };
     if (deg == 2) {
-        smooth_seg<W, cpk::MG_POST1, T>(e, k(1), a, v, v.xa, nullptr, nullptr, v.da, 0, 0, g);
-        smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xa, v.da, v.xb, nullptr, 0, 1, g);
+        smooth_seg<W, cpk::MG_POST1_X, T>(e, k(1), a, v, v.xa, nullptr, v.db, v.da, 0, 0, g);
+        smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(2), a, v, v.db, v.da, v.xb, nullptr, 0, 1, g);
         return;
     }
-    smooth_seg<W, cpk::MG_POST1, T>(e, k(1), a, v, v.xb, nullptr, nullptr, v.da, 0, 0, g);
-    smooth_seg<W, cpk::MG_CHEB, T>(e, k(2), a, v, v.xb, v.da, v.xa, v.db, 0, 0, g);
-    smooth_seg<W, cpk::MG_CHEB, T>(e, k(3), a, v, v.xa, v.db, v.xb, nullptr, 1, 1, g);
+    smooth_seg<W, cpk::MG_POST1_X, T>(e, k(1), a, v, v.xb, nullptr, v.xa, v.da, 0, 0, g);
+    smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(2), a, v, v.xa, v.da, v.res, v.db, 0, 0, g);
+    smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(3), a, v, v.res, v.db, v.xb, nullptr, 1, 1, g);
 }
 
 template <typename T, typename F>
@@ -1218,9 +1222,11 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
         }
     });
     CP_HIP(hipEventRecord(P->ev[4], st));
+    // the Chebyshev step the V-cycle runs: on a segment level the one that
+    // reads the combined iterate (x + d, as its previous step writes it)
     for (int i = 0; i < reps; ++i)
         by_level0<T>(P, [&](auto w) {
-            if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg);
+            if (seg) smooth_seg<decltype(w)::value, cpk::MG_CHEB_X, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0, sg);
             else smooth<decltype(w)::value, cpk::MG_CHEB, T>(e, k0, a, v, v.xa, v.da, v.xb, v.db, 1, 0);
         });
     CP_HIP(hipEventRecord(P->ev[5], st));
