@@ -281,6 +281,17 @@ float4* level_minv<float>(cp_problem* P, int l)
     return l == 0 ? P->minv32 : P->mg->L[l].minv32;
 }
 
+// Coarse levels of at least kWideTileNodes nodes in the FP32 symmetric
+// lattice layout smooth in the finest level's 16 x 32 tiles (row template
+// kSymW), smaller ones in 16 x 16 tiles (kSymW16).
+constexpr int kWideTileNodes = 1 << 18;
+template <typename T>
+static bool wide_tiles(const MgState& G, int l)
+{
+    return std::is_same<T, float>::value && l > 0 && G.L[l].vsym32 && G.L[l].nn >= kWideTileNodes;
+}
+
+template <typename T>
 static cpk::SegGrid seg_grid(const MgState& G, int l)
 {
     const MgLevel& L = G.L[l];
@@ -288,7 +299,7 @@ static cpk::SegGrid seg_grid(const MgState& G, int l)
     g.n1 = L.n1;
     g.n2 = L.n2;
     g.per1 = G.per1;
-    const int cols = l == 0 ? cpk::seg_cols<0>() : cpk::seg_cols<cpk::kLatW>();
+    const int cols = l == 0 || wide_tiles<T>(G, l) ? cpk::seg_cols<0>() : cpk::seg_cols<cpk::kLatW>();
     g.nt2 = (L.n2 + cols - 1) / cols;
     g.ntiles = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * g.nt2;
     g.node_of = l == 0 ? G.node0 : nullptr;
@@ -352,7 +363,7 @@ static int mg_setup_t(cp_problem* P, int nlev)
         if (l < G.nlev - 1 && L.seg) {
             const int cols = (L.n1 + cpk::kSeg - 1) / cpk::kSeg * L.n2;
             hipLaunchKernelGGL(cpk::k_mg_seg_setup<T>, dim3(blocks_of(cols)), dim3(cpk::kBlock), 0, st, a,
-                               seg_grid(G, l), level_minv<T>(P, l), L.lam + pi * cpk::kLamSlots);
+                               seg_grid<T>(G, l), level_minv<T>(P, l), L.lam + pi * cpk::kLamSlots);
         } else if (l < G.nlev - 1) {
             hipLaunchKernelGGL(cpk::k_mg_smooth_setup<T>, dim3(L.nb), dim3(cpk::kBlock), 0, st, a,
                                level_minv<T>(P, l), P->partials);
@@ -601,6 +612,7 @@ template <typename T, typename F>
 static void by_level(const cp_problem* P, int l, const F& f)
 {
     if (l == 0) by_level0<T>(P, f);
+    else if (wide_tiles<T>(*P->mg, l)) f(std::integral_constant<int, cpk::kSymW>());
     else if (std::is_same<T, float>::value && P->mg->L[l].vsym32) f(std::integral_constant<int, cpk::kSymW16>());
     else f(std::integral_constant<int, cpk::kLatW>());
 }
@@ -664,7 +676,7 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         const cpk::CgArgs<T> a = level_args<T>(P, l);
         const cpk::MgVec<T> v = vec(l);
         if (G.L[l].seg) {
-            cpk::SegGrid sg = seg_grid(G, l);
+            cpk::SegGrid sg = seg_grid<T>(G, l);
             if (fused_restrict(l)) {
                 sg.rg = grid(l - 1);
                 sg.rres = G.L[l - 1].vec[pi][5];
@@ -712,7 +724,7 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         o.sc = sc;
         e.template prolong<T>(step_key(SECT_UP, l, 0), o, L.nb);
         if (L.seg) {
-            const cpk::SegGrid sg = seg_grid(G, l);
+            const cpk::SegGrid sg = seg_grid<T>(G, l);
             by_level<T>(P, l, [&](auto w) { smooth_level_up_seg<decltype(w)::value, T>(e, l, a, v, sg, degree(l)); });
         } else {
             by_level<T>(P, l, [&](auto w) { smooth_level_up<decltype(w)::value, T>(e, l, a, v); });
@@ -1210,7 +1222,7 @@ static int mg_bench_t(cp_problem* P, int reps, double* ms_vcycle, double* ms_smo
     v.res = (V*)L.vec[pi][5];
     v.lam = L.lam + pi * cpk::kLamSlots;
     const bool seg = L.seg != 0;
-    const cpk::SegGrid sg = seg_grid(G, 0);
+    const cpk::SegGrid sg = seg_grid<T>(G, 0);
     Direct e{st};
     const unsigned long long k0 = 0;
     by_level0<T>(P, [&](auto w) {
