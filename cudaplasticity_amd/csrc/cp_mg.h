@@ -1295,6 +1295,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_seg_setup(CgArgs<T> a, SegGrid g,
 //   MG_POST1  x = xi                              -> do = c0 D^-1 (b - A x)
 //   SEG_FIRST_X, MG_POST1_X: as SEG_FIRST, MG_POST1, and also xo = x + do
 //   MG_CHEB_X x = xi (combined), d = di at the node -> do = a d + c D^-1 (b - A x), xo = x + do
+// (SEG_FIRST_X and MG_CHEB_X skip do where dout is null: no later step reads it)
 // (the residual steps MG_RESID, MG_RESID_X need no D^-1 and run in k_mg_smooth).
 // One workgroup per tile of kSeg x kSegCols lattice nodes: every thread
 // forms its node's residual and the per-node factors, then one thread per
@@ -1477,8 +1478,12 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
         xo[n] = V{c.c0 * gx, c.c0 * gy};
     } else if constexpr (MODE == SEG_FIRST || MODE == SEG_FIRST_X) {
         const V d = V{c.a1 * xn.x + c.b1 * gx, c.a1 * xn.y + c.b1 * gy};
-        dout[n] = d;
-        if constexpr (MODE == SEG_FIRST_X) xo[n] = V{xn.x + d.x, xn.y + d.y};
+        if constexpr (MODE == SEG_FIRST_X) {
+            xo[n] = V{xn.x + d.x, xn.y + d.y};
+            if (dout) dout[n] = d;   // (no later step of degree 2 reads it)
+        } else {
+            dout[n] = d;
+        }
     } else if constexpr (MODE == MG_POST1 || MODE == MG_POST1_X) {
         const V d = V{c.c0 * gx, c.c0 * gy};
         dout[n] = d;
@@ -1487,7 +1492,7 @@ __device__ __forceinline__ void mg_smooth_seg_tile(const CgArgs<T>& a, const MgV
         const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
         const V d = V{ca * dn.x + cb * gx, ca * dn.y + cb * gy};
         xo[n] = V{xn.x + d.x, xn.y + d.y};
-        if (!last) dout[n] = d;
+        if (!last && dout) dout[n] = d;
     } else {
         const T ca = second ? c.a2 : c.a1, cb = second ? c.b2 : c.b1;
         const V x = V{xn.x + dn.x, xn.y + dn.y};

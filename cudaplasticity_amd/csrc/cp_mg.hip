@@ -554,7 +554,7 @@ static void smooth_seg(E& e, unsigned long long key, const cpk::CgArgs<T>& a, co
 // Degree 2 stops the same recurrence one step earlier (the step-k
 // coefficients do not depend on the degree): the pre-smoothed x then lands
 // in xb, the prolongation writes xa, and the post-smoothing ends in xb.
-// Buffers, degree 3: down x1 -> xa, x2 -> xb, x3 -> xa (d in da, db);
+// Buffers, degree 3: down x1 -> xa, x2 -> xb, x3 -> xa (d1 in da);
 // up (from xb) x' -> xa, x'' -> res (consumed by then), x -> xb.
 template <int W, typename T, typename E>
 static void smooth_level_down_seg(E& e, int l, const cpk::CgArgs<T>& a, const cpk::MgVec<T>& v,
@@ -563,12 +563,14 @@ static void smooth_level_down_seg(E& e, int l, const cpk::CgArgs<T>& a, const cp
     const auto k = [&](int s) { return step_key(SECT_DOWN, l, s); };
     if (g.rres) smooth_seg<W, cpk::SEG_RX0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
     else smooth_seg<W, cpk::SEG_X0, T>(e, k(0), a, v, nullptr, nullptr, v.xa, nullptr, 0, 0, g);
-    smooth_seg<W, cpk::SEG_FIRST_X, T>(e, k(1), a, v, v.xa, nullptr, v.xb, v.da, 0, 0, g);
+    // d of the last pre-smoothing step is not stored: the residual step reads
+    // only the combined iterate
+    smooth_seg<W, cpk::SEG_FIRST_X, T>(e, k(1), a, v, v.xa, nullptr, v.xb, deg == 2 ? nullptr : v.da, 0, 0, g);
     if (deg == 2) {
         smooth<W, cpk::MG_RESID_X, T>(e, k(2), a, v, v.xb, nullptr, nullptr, nullptr, 0, 0);
         return;
     }
-    smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(2), a, v, v.xb, v.da, v.xa, v.db, 1, 0, g);
+    smooth_seg<W, cpk::MG_CHEB_X, T>(e, k(2), a, v, v.xb, v.da, v.xa, nullptr, 1, 0, g);
     smooth<W, cpk::MG_RESID_X, T>(e, k(3), a, v, v.xa, nullptr, nullptr, nullptr, 0, 0);
 }
 
