@@ -397,23 +397,27 @@ __device__ __forceinline__ void apply_minv(const typename V4<T>::type& m, T rx, 
     uy = m.y * rx + m.z * ry;
 }
 
-// Matrix values are streamed once per iteration: non-temporal loads keep
-// them from evicting the gathered vectors from L2.
+// Matrix values are read with ordinary (temporal) loads.  Non-temporal
+// loads, which mark the lines for early eviction, cost more than they save
+// here: a block is read again by a neighbour (the symmetric lattice layout's
+// backward couplings), and the multigrid levels' rows, read five times per
+// V-cycle, can stay in the Infinity Cache (measured: the mixed ramps 8-9 %
+// faster without them, FP64 unchanged).
 typedef double nd2 __attribute__((ext_vector_type(2)));
 typedef float nf4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void load_block(const CgArgs<double>& a, size_t q, double& m00, double& m01, double& m10,
                                            double& m11)
 {
-    const nd2 u = __builtin_nontemporal_load(reinterpret_cast<const nd2*>(a.va) + q);
-    const nd2 v = __builtin_nontemporal_load(reinterpret_cast<const nd2*>(a.vb) + q);
+    const nd2 u = reinterpret_cast<const nd2*>(a.va)[q];
+    const nd2 v = reinterpret_cast<const nd2*>(a.vb)[q];
     m00 = u.x; m01 = u.y; m10 = v.x; m11 = v.y;
 }
 
 __device__ __forceinline__ void load_block(const CgArgs<float>& a, size_t q, float& m00, float& m01, float& m10,
                                            float& m11)
 {
-    const nf4 u = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(a.va) + q);
+    const nf4 u = reinterpret_cast<const nf4*>(a.va)[q];
     m00 = u.x; m01 = u.y; m10 = u.z; m11 = u.w;
 }
 

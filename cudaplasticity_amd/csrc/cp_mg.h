@@ -643,7 +643,7 @@ __device__ __forceinline__ size_t sym_base(int n) { return (size_t)(n >> 6) * (k
 
 __device__ __forceinline__ void load_sym(const CgArgs<float>& a, size_t q, float (&m)[4])
 {
-    const nf4 u = __builtin_nontemporal_load(reinterpret_cast<const nf4*>(a.vsym) + q);
+    const nf4 u = reinterpret_cast<const nf4*>(a.vsym)[q];
     m[0] = u.x; m[1] = u.y; m[2] = u.z; m[3] = u.w;
 }
 // (the layout is built for FP32 only; the FP64 form exists for the templates)
