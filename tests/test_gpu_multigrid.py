@@ -193,3 +193,29 @@ def test_coarse_rebuild_interval(ctx, prec):
     with pytest.raises(A.CPError) as e:
         P.set_mg_rebuild(0)
     assert e.value.code == A.CP_ERR_INVALID
+
+
+@pytest.mark.parametrize("prec", [A.FP64, A.MIXED])
+def test_tail_level_read_from_global_memory(ctx, prec):
+    """A coarse tail whose finest level keeps its rows and pivot inverses in
+    global memory.  The 79x49 lattice has level 1 at 40x25 = 1000 nodes, so
+    the one-workgroup tail holds 40x25, 21x13, 11x7 and the 6x4 inverse
+    (1374 nodes).  In FP64 their six vectors take 96 B a node (132 KB of the
+    144 KB the vectors may use) and the masks 2 B a node; of the LDS that
+    remains the pivot inverses (32 B a node) fit for the coarse levels but not
+    for the 1000-node one, so its steps read rows and pivots from global
+    memory (in mixed precision its pivots fit and only its rows stay out).
+    Either way the solve matches the oracle."""
+    mesh, loads, mat, P = plate(ctx, 78, 48, S=100.0, sY=1e300)
+    levels = P.mg_levels()
+    print("levels", levels, flush=True)
+    assert levels[1] == (40, 25) and levels[2] == (21, 13) and levels[-1] == (6, 4)
+    st = P.solve(A.options(cg_tol=1e-10, precond=2, precision=prec), elastic=True)
+    assert st.status == A.CP_OK
+    om = O.gen_quarter_plate(1.0, 0.1, 78, 48)
+    ol = O.Problem(om).force(1, 100.0, 0.0).fix_vert_axis(0).fix_hor_axis(0)
+    uo, so, _ = O.solve(ol, O.material(E, 0.3), O.options(cg_tol=1e-10, elastic=True))
+    print("cg iterations", st.cg_iters, flush=True)
+    assert rel(P.field("u"), uo) < 1e-8
+    assert rel(P.field("sxx"), O.fields(so)["sxx"]) < 1e-8
+    P.close()
