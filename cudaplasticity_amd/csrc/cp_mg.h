@@ -1799,7 +1799,7 @@ __device__ __forceinline__ void tail_smooth_node(const TailLevel<T>& L, const ch
 // overlaps the smoothing).
 template <typename T>
 __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
-                                             const TailImage im)
+                                             const TailImage im, const MgGrid& rg, const typename V2<T>::type* rres)
 {
     using V = typename V2<T>::type;
     __shared__ __align__(16) char lds[tail_lds_budget<T>()];
@@ -1821,7 +1821,14 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
         }
         const TailLevel<T> L0 = lv[0];
         V* b0 = reinterpret_cast<V*>(lds + L0.lds);
-        for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = as_global(L0.v.b)[n];
+        if (rres) {
+            // the restriction of the level above's residual (rres, grid rg),
+            // done here instead of in a launch of its own
+            for (int n = tid; n < L0.a.nn; n += kTailThreads)
+                b0[n] = mg_restrict_value<T>(rg, rres, as_global(L0.a.mask)[n], n);
+        } else {
+            for (int n = tid; n < L0.a.nn; n += kTailThreads) b0[n] = as_global(L0.v.b)[n];
+        }
         if (tid < nl) cf[tid] = mg_cheb_of<T>(lv[tid].v.lam);
         __syncthreads();
     }
@@ -1936,10 +1943,11 @@ __device__ __forceinline__ void mg_tail_body(const TailLevel<T>* lv, int nl, con
 
 template <typename T>
 __global__ __launch_bounds__(kTailThreads) void k_mg_tail(const TailLevel<T>* lv, int nl, const double* Dinv, int nd,
-                                                          const CgScal* sc, TailImage im)
+                                                          const CgScal* sc, TailImage im, MgGrid rg,
+                                                          const typename V2<T>::type* rres)
 {
     if (sc && sc->done) return;
-    mg_tail_body<T>(lv, nl, Dinv, nd, im);
+    mg_tail_body<T>(lv, nl, Dinv, nd, im, rg, rres);
 }
 
 // The tail image (TailImage): one workgroup per tail level copies the
@@ -2111,6 +2119,11 @@ struct TailOp {
     const double* Dinv;
     const CgScal* sc;
     TailImage im;
+    // the level above the tail: its grid and residual, restricted into the
+    // tail's right-hand side by the tail itself (rres null: lv[0].v.b holds
+    // it already)
+    MgGrid rg;
+    const typename V2<T>::type* rres;
 };
 
 template <typename T>
@@ -2168,7 +2181,7 @@ __global__ __launch_bounds__(kTailThreads) void k_b_tail(const TailOp<T>* ops)
 {
     const TailOp<T> o = ops[blockIdx.x];
     if (o.sc && o.sc->done) return;
-    mg_tail_body<T>(o.lv, o.nl, o.Dinv, o.nd, o.im);
+    mg_tail_body<T>(o.lv, o.nl, o.Dinv, o.nd, o.im, o.rg, o.rres);
 }
 
 template <typename T>

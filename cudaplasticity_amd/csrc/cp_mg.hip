@@ -470,7 +470,7 @@ struct Direct {
     void tail(unsigned long long, const cpk::TailOp<T>& o)
     {
         hipLaunchKernelGGL(cpk::k_mg_tail<T>, dim3(1), dim3(cpk::kTailThreads), 0, st, o.lv, o.nl, o.Dinv, o.nd, o.sc,
-                           o.im);
+                           o.im, o.rg, o.rres);
     }
     template <typename T>
     void update(unsigned long long, const cpk::CgOp<T>& o, int nblocks)
@@ -687,7 +687,8 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         } else {
             by_level<T>(P, l, [&](auto w) { smooth_level_down<decltype(w)::value, T>(e, l, a, v); });
         }
-        if (fused_restrict(l + 1)) continue;
+        // (the level above the tail is restricted by the tail itself)
+        if (fused_restrict(l + 1) || l + 1 == last) continue;
         const MgLevel& C = G.L[l + 1];
         cpk::GridOp<T> o{};
         o.gr = grid(l);
@@ -706,6 +707,8 @@ static void mg_vcycle(E& e, cp_problem* P, const typename cpk::V2<T>::type* rhs,
         o.nd = G.nd;
         o.sc = sc;
         o.im = G.tail_img[pi];
+        o.rg = grid(last - 1);
+        o.rres = (const V*)G.L[last - 1].vec[pi][5];
         e.template tail<T>(step_key(SECT_TAIL, 0, 0), o);
     }
     for (int l = last - 1; l >= 0; --l) {
